@@ -1,0 +1,120 @@
+/*
+ * mercury_hip.h -- C ABI of libmercury_hip, the MI355X (gfx950) backend of the
+ * MERCURY6 massless-body hot path.
+ *
+ * The first eighteen entry points replace, one for one and with the same
+ * signatures and semantics, the CUDA backend's ABI declared in the reference's
+ * mercury_cuda.h (cited per function below); only the prefix changes from
+ * mercury_cuda_ to mercury_hip_, because the Fortran binding over them
+ * (mercury_gpu.f90) is rewritten for this library.  Arrays at the boundary keep
+ * MERCURY's Fortran layout: x(3,n), v(3,n) with body 1 the central body,
+ * masses and jcen pre-multiplied by K2.  Every int return is 0 on success and
+ * nonzero on failure with a "MERCURY HIP: ..." line on stderr; no exception
+ * crosses the ABI.  One global context per process; not re-entrant.
+ *
+ * The entry points after them have no counterpart in the reference: fused
+ * multi-step advance, batched ensembles and the tangent-linear/adjoint step.
+ */
+#ifndef MERCURY_HIP_H
+#define MERCURY_HIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Same layout as the reference's struct MercuryEvent (mercury_cuda.h:7-11):
+ * 1-based body indices i, j; kind bits 1 close-encounter minimum, 2 near miss,
+ * 4 collision, 8 central-body impact; now = separation at step end within the
+ * collision distance; xi, xj = states interpolated to the minimum. */
+typedef struct MercuryEvent {
+    int i, j, kind, now;
+    double distance, time;
+    double xi[6], xj[6];
+} MercuryEvent;
+
+/* ---- the reference ABI (mercury_cuda.h:12-34) --------------------------- */
+
+/* mercury_cuda.h:13  1 if a HIP device is present, else 0. */
+int mercury_hip_available(void);
+/* mercury_cuda.h:14  select the algorithm: 1 MVS, 9 TEST (MVS, no corrector),
+ * 4 RADAU, 10 HYBRID.  BS (2) and BS2 (3) are not provided and return 1. */
+int mercury_hip_configure(int algorithm);
+/* mercury_cuda.h:15  flag != 2 discards the MVS acceleration / RADAU history. */
+void mercury_hip_reset(int flag);
+/* mercury_cuda.h:16  new close-encounter limits rce[n]; history is kept. */
+int mercury_hip_limits(const double *rce);
+/* mercury_cuda.h:17  copy out the state; physical = 1 applies the MVS
+ * symplectic corrector (mco_mvs2h) on scratch, or gives HYBRID's physical
+ * velocities. */
+int mercury_hip_export(double h, int physical, double *x, double *v);
+/* mercury_cuda.h:18-21  load a system: n bodies, nbig Big (central included),
+ * pn = 1PN flag, ngflag = non-gravitational flag, m[n], x[3n], v[3n],
+ * ngf[5n], jcen[3], rce[n], rphys[n]. */
+int mercury_hip_upload(int n, int nbig, int pn, int ngflag,
+                       const double *m, const double *x, const double *v,
+                       const double *ngf, const double *jcen, const double *rce,
+                       const double *rphys);
+/* mercury_cuda.h:22-23  one step from time; MVS: *hdid = *h; RADAU: *h is
+ * replaced by the next proposal.  forces/rejected receive counts. */
+int mercury_hip_step(double time, double *h, double *hdid, double tol,
+                     int64_t *forces, int64_t *rejected);
+/* mercury_cuda.h:24  state after the last step, or before it (previous = 1). */
+int mercury_hip_download(double *x, double *v, int previous);
+/* mercury_cuda.h:25-26  events of the last step, library-owned array valid
+ * until the next call, sorted by (min, max) body index. */
+int mercury_hip_events(double time, double h, double rcen,
+                       const MercuryEvent **events, int *count);
+/* mercury_cuda.h:27  heliocentric accelerations a[3n] of the uploaded state. */
+int mercury_hip_force(double *a);
+/* mercury_cuda.h:28  HYBRID: kick, solar drift, Kepler drift, changeover scan. */
+int mercury_hip_hybrid_begin(double h, const double *crit, int flag, int cap, int *ce,
+                             int *nce, int *pi, int *pj, double *x, double *v,
+                             int64_t *forces);
+/* mercury_cuda.h:29  HYBRID: scatter members, solar drift, force, kick. */
+int mercury_hip_hybrid_finish(double h, const double *m, const double *x,
+                              const double *v, int64_t *forces);
+/* mercury_cuda.h:30  HYBRID compact encounter context: enter. */
+int mercury_hip_encounter_enter(int n, int nbig, const double *m, const double *x,
+                                const double *v, const double *crit, const double *rce,
+                                const double *rphys, int np, const int *pi, const int *pj);
+/* mercury_cuda.h:31  HYBRID compact encounter context: replace m, x, v. */
+int mercury_hip_encounter_update(const double *m, const double *x, const double *v);
+/* mercury_cuda.h:32  HYBRID compact encounter context: leave. */
+void mercury_hip_encounter_exit(void);
+/* mercury_cuda.h:33  release every device allocation. */
+void mercury_hip_free(void);
+
+/* ---- extensions (no counterpart in the reference) ----------------------- */
+
+/* Event of a multi-step or batched launch: step within the call, system. */
+typedef struct MercuryHipEvent {
+    MercuryEvent e;
+    int step, system;
+} MercuryHipEvent;
+
+/* nsteps MVS steps of the uploaded system in fused launches (steps_per_launch
+ * steps each), with in-kernel encounter/central screening when screen != 0.
+ * Same per-step arithmetic as mercury_hip_step. */
+int mercury_hip_advance(double time, double h, int nsteps, double rcen, int screen,
+                        int64_t *forces);
+/* Events of the last mercury_hip_advance / ensemble advance, sorted by
+ * (system, step, min index, max index). */
+int mercury_hip_advance_events(const MercuryHipEvent **events, int *count);
+/* Launch shape: steps fused per particle launch (>= 1), particle workgroup
+ * size (64..1024, multiple of 64).  0 keeps the current value. */
+int mercury_hip_set_tuning(int steps_per_launch, int block_size);
+/* Kernel timing with HIP events on the launch stream: enable = 1 clears and
+ * starts recording; the read returns the summed milliseconds and launch
+ * counts of the fast particle kernel and of the planet kernel. */
+int mercury_hip_timing(int enable);
+int mercury_hip_timing_read(double *particle_ms, int *particle_launches,
+                            double *planet_ms, int *planet_launches,
+                            double *general_ms, int *general_launches);
+/* Particles parked into the general kernel by the last advance. */
+int64_t mercury_hip_parked(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

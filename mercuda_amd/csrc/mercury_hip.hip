@@ -1,0 +1,559 @@
+// libmercury_hip: host orchestration and the C ABI (include/mercury_hip.h).
+//
+// One global context per process, as the reference's CUDA backend
+// (mercury_cuda.cu:16-32).  Device layout: component-major SoA over all
+// bodies, body 0 the central body, Big bodies 1..nbig-1, particles after.
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/mercury_hip.h"
+#include "mvs_device.hpp"
+
+using namespace mhip;
+
+namespace {
+
+void check(hipError_t code, const char* what) {
+    if (code != hipSuccess)
+        throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(code));
+}
+#define HIPCHECK(call) check((call), #call)
+
+int fail(const std::exception& e) {
+    std::fprintf(stderr, "MERCURY HIP: %s\n", e.what());
+    return 1;
+}
+
+template <class T>
+void dalloc(T*& p, size_t count) {
+    if (p) HIPCHECK(hipFree(p));
+    p = nullptr;
+    if (count) HIPCHECK(hipMalloc(reinterpret_cast<void**>(&p), sizeof(T) * count));
+}
+template <class T>
+void dfree(T*& p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+struct Timing {
+    bool on = false;
+    std::vector<hipEvent_t> pool;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> marks;   // kind, (start, stop)
+    size_t used = 0;
+    hipEvent_t get() {
+        if (used == pool.size()) {
+            hipEvent_t e;
+            HIPCHECK(hipEventCreate(&e));
+            pool.push_back(e);
+        }
+        return pool[used++];
+    }
+    void clear() { marks.clear(); used = 0; }
+};
+
+struct Context {
+    int algorithm = 1;
+    int n = 0, nbig = 0, pn = 0, ngflag = 0;
+    double jcen[3] = {0, 0, 0};
+    long long cap = 0;
+    bool history = false;            // accelerations valid (dtflag == 2)
+    bool saved_valid = false;        // step-start copies hold the last step
+    bool have_state = false;
+    // device state
+    double *x = nullptr, *v = nullptr, *a = nullptr, *m = nullptr, *rce = nullptr, *rphys = nullptr;
+    double *ox = nullptr, *ov = nullptr, *v0k = nullptr, *v1 = nullptr;
+    int* fault = nullptr;
+    double* tab = nullptr;
+    long long tab_cap = 0;           // doubles
+    Sys* dsys = nullptr;
+    int2* dtiles = nullptr;
+    int ntiles = 0;
+    DevEvent* dev = nullptr;
+    int* devcount = nullptr;
+    int ev_cap = 0;
+    Park* dpark = nullptr;
+    int* dparkcount = nullptr;
+    int park_cap = 0;
+    // host
+    std::vector<double> stage;
+    std::vector<DevEvent> hev;
+    std::vector<MercuryEvent> events;
+    std::vector<MercuryHipEvent> adv_events;
+    Sys sys{};                       // host copy of the single-system descriptor
+    bool sys_valid = false;
+    double mu = 0;
+    double* tcur = nullptr;          // device per-system time
+    hipStream_t stream = nullptr;
+    int steps_per_launch = 32;
+    int block = 256;
+    int64_t parked = 0;
+    Timing timing;
+};
+
+Context ctx;
+
+void ensure_stream() {
+    if (!ctx.stream) HIPCHECK(hipStreamCreateWithFlags(&ctx.stream, hipStreamNonBlocking));
+}
+
+void release_all() {
+    Context& c = ctx;
+    for (double** p : {&c.x, &c.v, &c.a, &c.m, &c.rce, &c.rphys, &c.ox, &c.ov, &c.v0k, &c.v1, &c.tab})
+        dfree(*p);
+    dfree(c.fault); dfree(c.dsys); dfree(c.tcur); c.sys_valid = false; dfree(c.dtiles); dfree(c.dev); dfree(c.devcount);
+    dfree(c.dpark); dfree(c.dparkcount);
+    c.cap = 0; c.tab_cap = 0; c.ev_cap = 0; c.park_cap = 0; c.ntiles = 0;
+    c.n = 0; c.nbig = 0; c.history = false; c.saved_valid = false; c.have_state = false;
+    c.events.clear(); c.adv_events.clear();
+}
+
+State device_state() {
+    State s;
+    s.x = ctx.x; s.v = ctx.v; s.a = ctx.a; s.m = ctx.m; s.rce = ctx.rce; s.rphys = ctx.rphys;
+    s.ox = ctx.ox; s.ov = ctx.ov; s.v0k = ctx.v0k; s.v1 = ctx.v1; s.fault = ctx.fault; s.cap = ctx.cap;
+    return s;
+}
+
+void ensure_capacity(int n) {
+    if (n <= ctx.cap) return;
+    long long cap = ((long long)n + 255) / 256 * 256;
+    size_t free_b = 0, total_b = 0;
+    HIPCHECK(hipMemGetInfo(&free_b, &total_b));
+    if ((size_t)cap * 8 * 40 > free_b) throw std::runtime_error("insufficient device memory");
+    for (double** p : {&ctx.x, &ctx.v, &ctx.a, &ctx.ox, &ctx.ov, &ctx.v0k, &ctx.v1}) dalloc(*p, 3 * cap);
+    for (double** p : {&ctx.m, &ctx.rce, &ctx.rphys}) dalloc(*p, cap);
+    if (!ctx.fault) dalloc(ctx.fault, 1);
+    if (!ctx.dsys) dalloc(ctx.dsys, 1);
+    if (!ctx.tcur) dalloc(ctx.tcur, 1);
+    ctx.park_cap = (int)std::min<long long>(cap, 1LL << 30);
+    dalloc(ctx.dpark, (size_t)ctx.park_cap);
+    if (!ctx.dparkcount) dalloc(ctx.dparkcount, 1);
+    if (!ctx.devcount) dalloc(ctx.devcount, 1);
+    if (ctx.ev_cap < 4096) {
+        ctx.ev_cap = 1 << 16;
+        dalloc(ctx.dev, (size_t)ctx.ev_cap);
+    }
+    ctx.cap = cap;
+}
+
+void ensure_table(long long doubles) {
+    if (doubles <= ctx.tab_cap) return;
+    dalloc(ctx.tab, (size_t)doubles);
+    ctx.tab_cap = doubles;
+}
+
+void ensure_events(int count) {
+    if (count <= ctx.ev_cap) return;
+    int cap = ctx.ev_cap;
+    while (cap < count) cap *= 2;
+    dalloc(ctx.dev, (size_t)cap);
+    ctx.ev_cap = cap;
+}
+
+// Tiles of one workgroup over the particles of the single system.
+void build_tiles() {
+    int npart = ctx.n - ctx.nbig;
+    int b = ctx.block;
+    int nt = npart > 0 ? (npart + b - 1) / b : 0;
+    std::vector<int2> t(nt);
+    for (int k = 0; k < nt; k++) t[k] = make_int2(0, ctx.nbig + k * b);
+    dalloc(ctx.dtiles, (size_t)std::max(nt, 1));
+    if (nt) HIPCHECK(hipMemcpy(ctx.dtiles, t.data(), sizeof(int2) * nt, hipMemcpyHostToDevice));
+    ctx.ntiles = nt;
+}
+
+// Fortran x(3,n) -> device SoA [3][cap]
+void upload_vec3(const double* src, double* dst) {
+    int n = ctx.n;
+    ctx.stage.resize((size_t)3 * n);
+    for (int j = 0; j < n; j++)
+        for (int k = 0; k < 3; k++) ctx.stage[(size_t)k * n + j] = src[3 * j + k];
+    for (int k = 0; k < 3; k++)
+        HIPCHECK(hipMemcpy(dst + k * ctx.cap, ctx.stage.data() + (size_t)k * n, sizeof(double) * n,
+                           hipMemcpyHostToDevice));
+}
+
+void download_vec3(const double* src, double* dst) {
+    int n = ctx.n;
+    ctx.stage.resize((size_t)3 * n);
+    for (int k = 0; k < 3; k++)
+        HIPCHECK(hipMemcpy(ctx.stage.data() + (size_t)k * n, src + k * ctx.cap, sizeof(double) * n,
+                           hipMemcpyDeviceToHost));
+    for (int j = 0; j < n; j++)
+        for (int k = 0; k < 3; k++) dst[3 * j + k] = j == 0 ? 0.0 : ctx.stage[(size_t)k * n + j];
+}
+
+void timed_launch_begin(int kind, hipEvent_t& start) {
+    if (!ctx.timing.on) return;
+    start = ctx.timing.get();
+    HIPCHECK(hipEventRecord(start, ctx.stream));
+    (void)kind;
+}
+void timed_launch_end(int kind, hipEvent_t start) {
+    if (!ctx.timing.on) return;
+    hipEvent_t stop = ctx.timing.get();
+    HIPCHECK(hipEventRecord(stop, ctx.stream));
+    ctx.timing.marks.push_back({kind, {start, stop}});
+}
+
+enum { K_PARTICLE = 0, K_PLANET = 1, K_GENERAL = 2 };
+
+// Descriptor of the single system for step h and launches of at most smax steps.
+void build_sys(double h, int smax, double rcen) {
+    Context& c = ctx;
+    int np = std::max(c.nbig - 1, 0);
+    Sys d{};
+    d.base = 0; d.n = c.n; d.nbig = c.nbig; d.ts = table_stride(np);
+    d.tab = 0; d.start = (long long)d.ts * smax;
+    d.mu = c.mu; d.h = h; d.hby2 = .5 * h; d.rcen = rcen;
+    d.j2 = c.jcen[0]; d.j4 = c.jcen[1]; d.j6 = c.jcen[2];
+    d.obl = (c.jcen[0] != 0 || c.jcen[1] != 0 || c.jcen[2] != 0);
+    ensure_table(d.start + start_stride(np));
+    if (c.sys_valid && std::memcmp(&d, &c.sys, sizeof(Sys)) == 0) return;
+    c.sys = d;
+    HIPCHECK(hipMemcpy(c.dsys, &c.sys, sizeof(Sys), hipMemcpyHostToDevice));
+    c.sys_valid = true;
+}
+
+// One fused launch sequence of `steps` MVS steps of the single system:
+// planets -> fast particles -> parked particles.
+void mvs_launch(double time, double h, int steps, bool reset, bool screen, bool save, double rcen) {
+    Context& c = ctx;
+    build_sys(h, std::max(c.steps_per_launch, steps), rcen);
+    State st = device_state();
+    EventSink es{c.dev, c.devcount, c.ev_cap};
+    hipEvent_t t0 = nullptr;
+    // planets (a system with no Big bodies still needs the table header)
+    PlanetArgs pa{c.dsys, st, c.tab, c.tcur, es, time, 1, steps, reset ? 1 : 0, screen ? 1 : 0,
+                  save ? 1 : 0, c.dparkcount};
+    timed_launch_begin(K_PLANET, t0);
+    hipLaunchKernelGGL(mvs_planets, dim3(1), dim3(64), 0, c.stream, pa);
+    HIPCHECK(hipGetLastError());
+    timed_launch_end(K_PLANET, t0);
+    if (c.ntiles == 0) return;
+    ParticleArgs args{c.dsys, c.dtiles, st, c.tab, es, ParkList{c.dpark, c.dparkcount, c.park_cap},
+                      steps, reset ? 1 : 0, screen ? 1 : 0, save ? 1 : 0};
+    timed_launch_begin(K_PARTICLE, t0);
+    if (screen && save) hipLaunchKernelGGL((mvs_particles_fast<true, true>), dim3(c.ntiles), dim3(c.block), 0, c.stream, args);
+    else if (screen) hipLaunchKernelGGL((mvs_particles_fast<true, false>), dim3(c.ntiles), dim3(c.block), 0, c.stream, args);
+    else if (save) hipLaunchKernelGGL((mvs_particles_fast<false, true>), dim3(c.ntiles), dim3(c.block), 0, c.stream, args);
+    else hipLaunchKernelGGL((mvs_particles_fast<false, false>), dim3(c.ntiles), dim3(c.block), 0, c.stream, args);
+    HIPCHECK(hipGetLastError());
+    timed_launch_end(K_PARTICLE, t0);
+    int npart = c.n - c.nbig;
+    int gblocks = std::max(1, std::min((npart + 63) / 64, 4096));
+    timed_launch_begin(K_GENERAL, t0);
+    if (screen && save) hipLaunchKernelGGL((mvs_particles_general<true, true>), dim3(gblocks), dim3(64), 0, c.stream, args);
+    else if (screen) hipLaunchKernelGGL((mvs_particles_general<true, false>), dim3(gblocks), dim3(64), 0, c.stream, args);
+    else if (save) hipLaunchKernelGGL((mvs_particles_general<false, true>), dim3(gblocks), dim3(64), 0, c.stream, args);
+    else hipLaunchKernelGGL((mvs_particles_general<false, false>), dim3(gblocks), dim3(64), 0, c.stream, args);
+    HIPCHECK(hipGetLastError());
+    timed_launch_end(K_GENERAL, t0);
+}
+
+int read_fault() {
+    int bad = 0;
+    HIPCHECK(hipMemcpyAsync(&bad, ctx.fault, sizeof(int), hipMemcpyDeviceToHost, ctx.stream));
+    HIPCHECK(hipStreamSynchronize(ctx.stream));
+    return bad;
+}
+
+void require_state() {
+    if (!ctx.have_state) throw std::runtime_error("no state uploaded");
+}
+
+bool mvs_family() { return ctx.algorithm == 1 || ctx.algorithm == 9; }
+
+MercuryEvent to_public(const DevEvent& e) {
+    MercuryEvent r;
+    r.i = e.i; r.j = e.j; r.kind = e.kind; r.now = e.now;
+    r.distance = e.distance; r.time = e.time;
+    for (int k = 0; k < 6; k++) { r.xi[k] = e.xi[k]; r.xj[k] = e.xj[k]; }
+    return r;
+}
+
+bool pair_less(int ai, int aj, int bi, int bj) {
+    int a0 = std::min(ai, aj), a1 = std::max(ai, aj), b0 = std::min(bi, bj), b1 = std::max(bi, bj);
+    return a0 < b0 || (a0 == b0 && a1 < b1);
+}
+
+// Fetch `count` device events into hev.
+void fetch_events(int count) {
+    ctx.hev.resize(count);
+    if (count)
+        HIPCHECK(hipMemcpy(ctx.hev.data(), ctx.dev, sizeof(DevEvent) * (size_t)count, hipMemcpyDeviceToHost));
+}
+
+}  // namespace
+
+// ===========================================================================
+extern "C" {
+
+int mercury_hip_available(void) {
+    int n = 0;
+    return hipGetDeviceCount(&n) == hipSuccess && n > 0;
+}
+
+int mercury_hip_configure(int algorithm) {
+    if (algorithm != 1 && algorithm != 9) return 1;
+    if (algorithm != ctx.algorithm) release_all();
+    ctx.algorithm = algorithm;
+    ctx.history = false;
+    return 0;
+}
+
+void mercury_hip_reset(int flag) {
+    if (flag != 2) ctx.history = false;
+}
+
+int mercury_hip_limits(const double* rce) {
+    try {
+        require_state();
+        HIPCHECK(hipMemcpy(ctx.rce, rce, sizeof(double) * ctx.n, hipMemcpyHostToDevice));
+        return 0;
+    } catch (const std::exception& e) { return fail(e); }
+}
+
+int mercury_hip_upload(int n, int nbig, int pn, int ngflag, const double* m, const double* x, const double* v,
+                       const double* ngf, const double* jcen, const double* rce, const double* rphys) {
+    try {
+        (void)ngf;
+        if (n < 1 || nbig < 1 || nbig > n) throw std::runtime_error("invalid body counts");
+        if (nbig > kMaxBig) throw std::runtime_error("more Big bodies than the HIP path supports (64)");
+        if (!(m[0] > 0)) throw std::runtime_error("central mass must be positive");
+        if (ngflag != 0) throw std::runtime_error("non-gravitational forces are not provided by the HIP path");
+        if (pn != 0 && mvs_family()) throw std::runtime_error("MVS does not take post-Newtonian forces");
+        bool obl = jcen[0] != 0 || jcen[1] != 0 || jcen[2] != 0;
+        if (obl)
+            for (int j = nbig; j < n; j++)
+                if (m[j] != 0) throw std::runtime_error("oblateness with massive Small bodies is not provided");
+        for (int j = 0; j < 3 * n; j++)
+            if (!std::isfinite(x[j]) || !std::isfinite(v[j])) throw std::runtime_error("nonfinite state");
+        ensure_stream();
+        ensure_capacity(n);
+        ctx.n = n; ctx.nbig = nbig; ctx.pn = pn; ctx.ngflag = ngflag; ctx.mu = m[0];
+        ctx.sys_valid = false;
+        for (int k = 0; k < 3; k++) ctx.jcen[k] = jcen[k];
+        HIPCHECK(hipMemcpy(ctx.m, m, sizeof(double) * n, hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(ctx.rce, rce, sizeof(double) * n, hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(ctx.rphys, rphys, sizeof(double) * n, hipMemcpyHostToDevice));
+        upload_vec3(x, ctx.x);
+        upload_vec3(v, ctx.v);
+        HIPCHECK(hipMemset(ctx.a, 0, sizeof(double) * 3 * ctx.cap));
+        HIPCHECK(hipMemset(ctx.fault, 0, sizeof(int)));
+        build_tiles();
+        ctx.history = false;
+        ctx.saved_valid = false;
+        ctx.have_state = true;
+        HIPCHECK(hipDeviceSynchronize());
+        return 0;
+    } catch (const std::exception& e) { return fail(e); }
+}
+
+int mercury_hip_step(double time, double* h, double* hdid, double tol, int64_t* forces, int64_t* rejected) {
+    try {
+        (void)tol;
+        require_state();
+        *rejected = 0;
+        if (!mvs_family()) throw std::runtime_error("algorithm not provided by the HIP path");
+        HIPCHECK(hipMemsetAsync(ctx.fault, 0, sizeof(int), ctx.stream));
+        bool reset = !ctx.history;
+        mvs_launch(time, *h, 1, reset, false, true, 0.0);
+        if (read_fault()) throw std::runtime_error("MVS force failed");
+        ctx.history = true;
+        ctx.saved_valid = true;
+        *hdid = *h;
+        *forces = reset ? 2 : 1;
+        return 0;
+    } catch (const std::exception& e) { return fail(e); }
+}
+
+int mercury_hip_download(double* x, double* v, int previous) {
+    try {
+        require_state();
+        if (previous && !ctx.saved_valid) throw std::runtime_error("no previous state");
+        download_vec3(previous ? ctx.ox : ctx.x, x);
+        download_vec3(previous ? ctx.ov : ctx.v, v);
+        return 0;
+    } catch (const std::exception& e) { return fail(e); }
+}
+
+int mercury_hip_events(double time, double h, double rcen, const MercuryEvent** out, int* count) {
+    try {
+        require_state();
+        if (!ctx.saved_valid) throw std::runtime_error("no step to screen");
+        if (!mvs_family()) throw std::runtime_error("algorithm not provided by the HIP path");
+        int n = ctx.n;
+        for (;;) {
+            HIPCHECK(hipMemsetAsync(ctx.devcount, 0, sizeof(int), ctx.stream));
+            SavedScreenArgs sa{ctx.dsys, device_state(), ctx.tab, EventSink{ctx.dev, ctx.devcount, ctx.ev_cap},
+                               time, h, rcen, 1};
+            if (n > 1) hipLaunchKernelGGL(screen_saved, dim3((n - 1 + 255) / 256), dim3(256), 0, ctx.stream, sa);
+            HIPCHECK(hipGetLastError());
+            HIPCHECK(hipMemcpyAsync(count, ctx.devcount, sizeof(int), hipMemcpyDeviceToHost, ctx.stream));
+            HIPCHECK(hipStreamSynchronize(ctx.stream));
+            if (*count <= ctx.ev_cap) break;
+            ensure_events(*count);   // screen again after growth: nothing is dropped
+        }
+        fetch_events(*count);
+        std::sort(ctx.hev.begin(), ctx.hev.end(), [](const DevEvent& a, const DevEvent& b) {
+            return pair_less(a.i, a.j, b.i, b.j);
+        });
+        ctx.events.resize(*count);
+        for (int k = 0; k < *count; k++) ctx.events[k] = to_public(ctx.hev[k]);
+        *out = ctx.events.data();
+        return 0;
+    } catch (const std::exception& e) { return fail(e); }
+}
+
+int mercury_hip_force(double* a) {
+    try {
+        (void)a;
+        throw std::runtime_error("heliocentric force interface requires algorithm 4");
+    } catch (const std::exception& e) { return fail(e); }
+}
+
+int mercury_hip_export(double h, int physical, double* x, double* v) {
+    try {
+        require_state();
+        (void)h;
+        if (physical && ctx.algorithm == 1) throw std::runtime_error("symplectic corrector not available yet");
+        download_vec3(ctx.x, x);
+        download_vec3(ctx.v, v);
+        return 0;
+    } catch (const std::exception& e) { return fail(e); }
+}
+
+int mercury_hip_hybrid_begin(double, const double*, int, int, int*, int*, int*, int*, double*, double*, int64_t*) {
+    std::fprintf(stderr, "MERCURY HIP: HYBRID not available yet\n");
+    return 1;
+}
+int mercury_hip_hybrid_finish(double, const double*, const double*, const double*, int64_t*) {
+    std::fprintf(stderr, "MERCURY HIP: HYBRID not available yet\n");
+    return 1;
+}
+int mercury_hip_encounter_enter(int, int, const double*, const double*, const double*, const double*,
+                                const double*, const double*, int, const int*, const int*) {
+    std::fprintf(stderr, "MERCURY HIP: HYBRID not available yet\n");
+    return 1;
+}
+int mercury_hip_encounter_update(const double*, const double*, const double*) {
+    std::fprintf(stderr, "MERCURY HIP: HYBRID not available yet\n");
+    return 1;
+}
+void mercury_hip_encounter_exit(void) {}
+
+void mercury_hip_free(void) { release_all(); }
+
+// ---- extensions -----------------------------------------------------------
+
+int mercury_hip_advance(double time, double h, int nsteps, double rcen, int screen, int64_t* forces) {
+    try {
+        require_state();
+        if (!mvs_family()) throw std::runtime_error("advance needs MVS");
+        if (nsteps < 0) throw std::runtime_error("negative step count");
+        ensure_stream();
+        HIPCHECK(hipMemsetAsync(ctx.fault, 0, sizeof(int), ctx.stream));
+        HIPCHECK(hipMemsetAsync(ctx.devcount, 0, sizeof(int), ctx.stream));
+        int64_t nf = 0;
+        ctx.adv_events.clear();
+        ctx.parked = 0;
+        double t = time;
+        int done = 0;
+        while (done < nsteps) {
+            int s = std::min(ctx.steps_per_launch, nsteps - done);
+            bool reset = !ctx.history;
+            mvs_launch(t, h, s, reset, screen != 0, false, rcen);
+            nf += s + (reset ? 1 : 0);
+            ctx.history = true;
+            // the next launch starts from the time the table's last step reached
+            for (int k = 0; k < s; k++) t = t + h;
+            if (screen) {
+                int cnt = 0, pk = 0;
+                HIPCHECK(hipMemcpyAsync(&cnt, ctx.devcount, sizeof(int), hipMemcpyDeviceToHost, ctx.stream));
+                HIPCHECK(hipMemcpyAsync(&pk, ctx.dparkcount, sizeof(int), hipMemcpyDeviceToHost, ctx.stream));
+                HIPCHECK(hipStreamSynchronize(ctx.stream));
+                ctx.parked += pk;
+                if (cnt > ctx.ev_cap) throw std::runtime_error("event buffer overflow in fused advance");
+                if (cnt) {
+                    fetch_events(cnt);
+                    for (auto& e : ctx.hev) {
+                        MercuryHipEvent r;
+                        r.e = to_public(e);
+                        r.step = done + e.step;
+                        r.system = e.sys;
+                        ctx.adv_events.push_back(r);
+                    }
+                    HIPCHECK(hipMemsetAsync(ctx.devcount, 0, sizeof(int), ctx.stream));
+                }
+            }
+            done += s;
+        }
+        if (read_fault()) throw std::runtime_error("MVS force failed");
+        ctx.saved_valid = false;
+        std::sort(ctx.adv_events.begin(), ctx.adv_events.end(), [](const MercuryHipEvent& a, const MercuryHipEvent& b) {
+            if (a.system != b.system) return a.system < b.system;
+            if (a.step != b.step) return a.step < b.step;
+            return pair_less(a.e.i, a.e.j, b.e.i, b.e.j);
+        });
+        *forces = nf;
+        return 0;
+    } catch (const std::exception& e) { return fail(e); }
+}
+
+int mercury_hip_advance_events(const MercuryHipEvent** events, int* count) {
+    *events = ctx.adv_events.data();
+    *count = (int)ctx.adv_events.size();
+    return 0;
+}
+
+int mercury_hip_set_tuning(int steps_per_launch, int block_size) {
+    try {
+        if (steps_per_launch < 0 || block_size < 0) throw std::runtime_error("invalid tuning");
+        if (steps_per_launch) ctx.steps_per_launch = steps_per_launch;
+        if (block_size) {
+            if (block_size % 64 || block_size > 1024) throw std::runtime_error("block size must be 64..1024, multiple of 64");
+            ctx.block = block_size;
+            if (ctx.have_state) build_tiles();
+        }
+        return 0;
+    } catch (const std::exception& e) { return fail(e); }
+}
+
+int mercury_hip_timing(int enable) {
+    try {
+        ctx.timing.clear();
+        ctx.timing.on = enable != 0;
+        return 0;
+    } catch (const std::exception& e) { return fail(e); }
+}
+
+int mercury_hip_timing_read(double* particle_ms, int* particle_launches, double* planet_ms, int* planet_launches,
+                            double* general_ms, int* general_launches) {
+    try {
+        if (ctx.stream) HIPCHECK(hipStreamSynchronize(ctx.stream));
+        double ms[3] = {0, 0, 0};
+        int cnt[3] = {0, 0, 0};
+        for (auto& mk : ctx.timing.marks) {
+            float t = 0;
+            HIPCHECK(hipEventElapsedTime(&t, mk.second.first, mk.second.second));
+            ms[mk.first] += t;
+            cnt[mk.first]++;
+        }
+        *particle_ms = ms[K_PARTICLE]; *particle_launches = cnt[K_PARTICLE];
+        *planet_ms = ms[K_PLANET]; *planet_launches = cnt[K_PLANET];
+        *general_ms = ms[K_GENERAL]; *general_launches = cnt[K_GENERAL];
+        return 0;
+    } catch (const std::exception& e) { return fail(e); }
+}
+
+int64_t mercury_hip_parked(void) { return ctx.parked; }
+
+}  // extern "C"

@@ -1,0 +1,806 @@
+// MVS step for massless particles on gfx950: data layout and kernels.
+//
+// One MVS step (mdt_mvs, mercury6_2.for:3987-4097) splits into
+//   * a planet prologue: the Big bodies' kick, Jacobi drift and interaction
+//     force.  Massless particles never act on Big bodies (mfo_mvs,
+//     mercury6_2.for:5016-5054), so one wavefront per system advances the
+//     planets S steps ahead and tabulates, per step, what every particle
+//     needs: the indirect term a0tp, the oblateness reaction acen, the planet
+//     positions after the drift with their masses, and the planets' screening
+//     data (padded x-y boxes, start/end states).
+//   * a particle pass: one lane per particle, SoA state in HBM, the planet
+//     table read with wave-uniform (scalar) loads.  The fast kernel handles
+//     the kepmd branch of the drift; a lane whose drift would take any other
+//     branch parks (wave-ballot compaction, one atomic per wave) and the
+//     general kernel finishes it from that step with the full drift chain.
+// Both particle kernels run kick -> drift -> encounter screen (mce_stat) ->
+// force (mfo_mvs small-body terms) -> kick -> central check (mce_cent) per
+// step, with the Fortran's operation order, so results are bit-identical to
+// the CPU path apart from libm calls.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "kepler.hpp"
+
+namespace mhip {
+
+constexpr int kMaxBig = 64;            // Big bodies per system, central body included
+constexpr int kHdr = 8;                // table header: a0tp[3], acen[3], time_end, pad
+constexpr int kF = 4;                  // force block per planet: X1[3], m
+constexpr int kScr = 15;               // screen block per planet
+// screen block layout per planet
+enum { SX0 = 0, SX1, SY0, SY1, SP = 4, SV0 = 7, SV1 = 10, SRCE = 13, SRPH = 14 };
+
+__host__ __device__ inline int table_stride(int np) { return (kHdr + (kF + kScr) * np + 7) & ~7; }
+__host__ __device__ inline int start_stride(int np) { return (kHdr + kF * np + 7) & ~7; }
+
+// Per-system launch parameters (device array, read with scalar loads).
+// Static for a given upload, step size and launch length; the per-launch
+// table base pointer and start time are kernel arguments.
+struct Sys {
+    int base;          // index of this system's central body in the SoA arrays
+    int n;             // bodies in the system, central body included
+    int nbig;          // Big bodies, central body included
+    int ts;            // table stride (doubles) per step
+    long long tab;     // offset (doubles) of this system's step tables in a launch's table block
+    long long start;   // offset (doubles) of this system's start table (used when reset)
+    double mu;         // G * central mass
+    double h, hby2;    // step and half step
+    double rcen;       // central-body radius (impact check)
+    double j2, j4, j6; // central oblateness terms J_n R^n
+    int obl;           // jcen != 0
+    int pad0;
+};
+
+// Body-state arrays: component-major SoA over all bodies of all systems.
+struct State {
+    double* x;     // [3][cap]
+    double* v;     // [3][cap]
+    double* a;     // [3][cap]  accelerations carried between steps
+    const double* m;
+    const double* rce;
+    const double* rphys;
+    // step-start copies for the host-driven event call (single-step mode only)
+    double* ox;    // position before the step
+    double* ov;    // velocity before the step
+    double* v0k;   // velocity after the first kick
+    double* v1;    // velocity after the drift
+    int* fault;    // set on a non-finite acceleration
+    long long cap;
+};
+
+// Device event record: the reference's 128-byte MercuryEvent (mercury_cuda.h:7-11)
+// followed by the step within the launch and the system index.
+struct DevEvent {
+    int i, j, kind, now;
+    double distance, time;
+    double xi[6], xj[6];
+    int step, sys;
+};
+
+struct EventSink {
+    DevEvent* buf;
+    int* count;
+    int cap;
+};
+
+struct Park {
+    int j, step, sys, pad;
+};
+
+struct ParkList {
+    Park* buf;
+    int* count;
+    int cap;
+};
+
+__device__ __forceinline__ void push_event(const EventSink& es, const DevEvent& e) {
+    int slot = atomicAdd(es.count, 1);
+    if (slot < es.cap) es.buf[slot] = e;
+}
+
+// ---------------------------------------------------------------------------
+// mfo_obl for one body (mercury6_2.for:5191-5217): acceleration before the
+// central reaction is subtracted.
+__device__ __forceinline__ void obl_body(double mu, double j2, double j4, double j6, double x, double y,
+                                         double z, double& ax, double& ay, double& az) {
+    double r2 = x * x + y * y + z * z;
+    double r_1 = 1.0 / sqrt(r2);
+    double r_2 = r_1 * r_1;
+    double r_3 = r_2 * r_1;
+    double jr2 = j2 * r_2;
+    double jr4 = j4 * r_2 * r_2;
+    double jr6 = j6 * r_2 * r_2 * r_2;
+    double u2 = z * z * r_2;
+    double u4 = u2 * u2;
+    double u6 = u4 * u2;
+    double tmp1 = mu * r_3;
+    double tmp2 = jr2 * (7.5 * u2 - 1.5) + jr4 * (39.375 * u4 - 26.25 * u2 + 1.875)
+                + jr6 * (187.6875 * u6 - 216.5625 * u4 + 59.0625 * u2 - 2.1875);
+    double tmp3 = jr2 * 3.0 + jr4 * (17.5 * u2 - 7.5) + jr6 * (86.625 * u4 - 78.75 * u2 + 13.125);
+    ax = x * tmp1 * tmp2;
+    ay = y * tmp1 * tmp2;
+    az = z * tmp1 * (tmp2 - tmp3);
+}
+
+// Small-body acceleration of mfo_mvs (mercury6_2.for:5034-5070):
+// a = a0tp + sum_i -(m_i / d^3) (x - X_i) in planet order, plus oblateness.
+// T points at one step's table (or the start table; both share the header
+// and force-block layout).
+__device__ __forceinline__ void particle_accel(const Sys& d, const double* __restrict__ T, int np,
+                                               double x, double y, double z,
+                                               double& ax, double& ay, double& az) {
+    const double* F = T + kHdr;
+    double a3x = 0.0, a3y = 0.0, a3z = 0.0;
+    for (int p = 0; p < np; p++) {
+        double dx = x - F[kF * p + 0];
+        double dy = y - F[kF * p + 1];
+        double dz = z - F[kF * p + 2];
+        double s2 = dx * dx + dy * dy + dz * dz;
+        double s_1 = 1.0 / sqrt(s2);
+        double s_3 = s_1 * s_1 * s_1;
+        double faci = F[kF * p + 3] * s_3;
+        a3x = a3x - faci * dx;
+        a3y = a3y - faci * dy;
+        a3z = a3z - faci * dz;
+    }
+    ax = T[0] + a3x;
+    ay = T[1] + a3y;
+    az = T[2] + a3z;
+    if (d.obl) {
+        double ox, oy, oz;
+        obl_body(d.mu, d.j2, d.j4, d.j6, x, y, z, ox, oy, oz);
+        ax = ax + (ox - T[3]);
+        ay = ay + (oy - T[4]);
+        az = az + (oz - T[5]);
+    }
+}
+
+// mce_min (mercury6_2.for:1572-1624)
+__device__ inline void mce_min(double d0, double d1, double d0t, double d1t, double h,
+                               double& d2min, double& tmin) {
+    if (d0t * h > 0.0 || d1t * h < 0.0) {
+        if (d0 <= d1) { d2min = d0; tmin = -h; }
+        else { d2min = d1; tmin = 0.0; }
+    } else {
+        double temp = 6.0 * (d0 - d1);
+        double a = temp + 3.0 * h * (d0t + d1t);
+        double b = temp + 2.0 * h * (d0t + 2.0 * d1t);
+        double c = h * d1t;
+        temp = -.5 * (b + fsign(sqrt(fmax(b * b - 4.0 * a * c, 0.0)), b));
+        double tau = (temp == 0.0) ? 0.0 : c / temp;
+        tau = fmin(tau, 0.0);
+        tau = fmax(tau, -1.0);
+        tmin = tau * h;
+        temp = 1.0 + tau;
+        d2min = tau * tau * ((3.0 + 2.0 * tau) * d0 + temp * h * d0t)
+              + temp * temp * ((1.0 - 2.0 * tau) * d1 + tau * h * d1t);
+        d2min = fmax(d2min, 0.0);
+    }
+}
+
+// One axis of mce_box (mercury6_2.for:1013-1036)
+__device__ __forceinline__ void box_axis(double h, double x0, double v0, double x1, double v1,
+                                         double& lo, double& hi) {
+    lo = fmin(x0, x1);
+    hi = fmax(x0, x1);
+    if ((v0 < 0.0 && v1 > 0.0) || (v0 > 0.0 && v1 < 0.0)) {
+        double temp = (v0 * x1 - v1 * x0 - .5 * h * v0 * v1) / (v0 - v1);
+        lo = fmin(lo, temp);
+        hi = fmax(hi, temp);
+    }
+}
+
+struct Ends {   // start (after the first kick) and end (after the drift) of one body
+    double x0, y0, z0, u0, v0, w0;
+    double x1, y1, z1, u1, v1, w1;
+};
+
+// The pair part of mce_stat (mercury6_2.for:1820-1925) for Big body i (0-based
+// local index ib, data in S) against body j (0-based local index jb).
+__device__ inline void screen_pair(const double* __restrict__ X1, const double* __restrict__ Sc, int ib, int jb,
+                                   bool j_big, double mi, double mj, const Ends& e, double rce_j, double rphys_j,
+                                   double h, double time_end, int step, int sys, const EventSink& es) {
+    double dx0 = Sc[SP + 0] - e.x0, dy0 = Sc[SP + 1] - e.y0, dz0 = Sc[SP + 2] - e.z0;
+    double du0 = Sc[SV0 + 0] - e.u0, dv0 = Sc[SV0 + 1] - e.v0, dw0 = Sc[SV0 + 2] - e.w0;
+    double d0t = (dx0 * du0 + dy0 * dv0 + dz0 * dw0) * 2.0;
+    double dx1 = X1[0] - e.x1, dy1 = X1[1] - e.y1, dz1 = X1[2] - e.z1;
+    double du1 = Sc[SV1 + 0] - e.u1, dv1 = Sc[SV1 + 1] - e.v1, dw1 = Sc[SV1 + 2] - e.w1;
+    double d1t = (dx1 * du1 + dy1 * dv1 + dz1 * dw1) * 2.0;
+    double d0 = dx0 * dx0 + dy0 * dy0 + dz0 * dz0;
+    double d1 = dx1 * dx1 + dy1 * dy1 + dz1 * dz1;
+    double d2min, tmin;
+    mce_min(d0, d1, d0t, d1t, h, d2min, tmin);
+    double d2ce = fmax(Sc[SRCE], rce_j);
+    double d2hit = Sc[SRPH] + rphys_j;
+    d2ce = d2ce * d2ce;
+    d2hit = d2hit * d2hit;
+    double d2near = d2hit * 4.0;
+    bool clo = (d2min <= d2ce && d0t * h <= 0.0 && d1t * h >= 0.0) || (d2min <= d2hit);
+    bool near = d2min <= d2near;
+    if (!clo && !near) return;
+    DevEvent ev;
+    bool swap = j_big && mj > mi;
+    ev.i = (swap ? jb : ib) + 1;
+    ev.j = (swap ? ib : jb) + 1;
+    ev.kind = (clo ? 1 : 0) | (near ? 2 : 0) | (d2min <= d2hit ? 4 : 0);
+    ev.now = d1 <= d2hit;
+    ev.distance = sqrt(fmax(0.0, d2min));
+    ev.time = tmin + time_end;
+    double hm1 = 1.0 / h;
+    double tmp0 = -tmin * hm1;
+    double tmp1 = 1.0 + tmin * hm1;
+    double pi[6] = {tmp0 * Sc[SP] + tmp1 * X1[0], tmp0 * Sc[SP + 1] + tmp1 * X1[1],
+                    tmp0 * Sc[SP + 2] + tmp1 * X1[2], tmp0 * Sc[SV0] + tmp1 * Sc[SV1],
+                    tmp0 * Sc[SV0 + 1] + tmp1 * Sc[SV1 + 1], tmp0 * Sc[SV0 + 2] + tmp1 * Sc[SV1 + 2]};
+    double pj[6] = {tmp0 * e.x0 + tmp1 * e.x1, tmp0 * e.y0 + tmp1 * e.y1, tmp0 * e.z0 + tmp1 * e.z1,
+                    tmp0 * e.u0 + tmp1 * e.u1, tmp0 * e.v0 + tmp1 * e.v1, tmp0 * e.w0 + tmp1 * e.w1};
+    for (int k = 0; k < 6; k++) {
+        ev.xi[k] = swap ? pj[k] : pi[k];
+        ev.xj[k] = swap ? pi[k] : pj[k];
+    }
+    ev.step = step;
+    ev.sys = sys;
+    push_event(es, ev);
+}
+
+// Encounter screen of body jb against the Big bodies 1..min(jb,nbig)-1 of one
+// step's table (mce_stat loop order is restored by the host's sort).
+__device__ inline void screen_body(const double* __restrict__ T, int np, int jb, int nbig, double mj,
+                                   const Ends& e, double rce_j, double rphys_j, double h, double time_end,
+                                   int step, int sys, const EventSink& es) {
+    double xmin, xmax, ymin, ymax;
+    box_axis(h, e.x0, e.u0, e.x1, e.u1, xmin, xmax);
+    box_axis(h, e.y0, e.v0, e.y1, e.v1, ymin, ymax);
+    double temp = rce_j * 1.2;
+    xmin = xmin - temp; xmax = xmax + temp;
+    ymin = ymin - temp; ymax = ymax + temp;
+    const double* F = T + kHdr;
+    const double* Sc = F + kF * np;
+    int lim = jb < nbig ? jb - 1 : np;
+    for (int p = 0; p < lim; p++) {
+        const double* B = Sc + kScr * p;
+        if (B[SX1] >= xmin && xmax >= B[SX0] && B[SY1] >= ymin && ymax >= B[SY0]) {
+            screen_pair(F + kF * p, B, p + 1, jb, jb < nbig, F[kF * p + 3], mj, e, rce_j, rphys_j, h,
+                        time_end, step, sys, es);
+        }
+    }
+}
+
+// mce_cent (mercury6_2.for:1059-1185) for one body: (xs,vs) before the step,
+// (xe,ve) after it.
+__device__ inline void central_check(double mcen, double mj, double rcen, double h, double time_end,
+                                     double xs, double ys, double zs, double us, double vs, double ws,
+                                     double xe, double ye, double ze, double ue, double ve, double we,
+                                     int jb, int step, int sys, const EventSink& es) {
+    double rcen2 = rcen * rcen;
+    double rr0 = xs * xs + ys * ys + zs * zs;
+    double rr1 = xe * xe + ye * ye + ze * ze;
+    double rv0 = us * xs + vs * ys + ws * zs;
+    double rv1 = ue * xe + ve * ye + we * ze;
+    if (!((rv0 * h <= 0.0 && rv1 * h >= 0.0) || fmin(rr0, rr1) <= rcen2)) return;
+    double hx = ys * ws - zs * vs;
+    double hy = zs * us - xs * ws;
+    double hz = xs * vs - ys * us;
+    double v2 = us * us + vs * vs + ws * ws;
+    double h2 = hx * hx + hy * hy + hz * hz;
+    double p = h2 / (mcen + mj);
+    double r0 = sqrt(rr0);
+    double temp = 1.0 + p * (v2 / (mcen + mj) - 2.0 / r0);
+    double e = sqrt(fmax(temp, 0.0));
+    double q = p / (1.0 + e);
+    if (!(q <= rcen)) return;
+    temp = 0.0;
+    if (e > 0.0 && e < 1.0) {
+        double a = q / (1.0 - e);
+        double uhit = fsign(acos(fmax(-1.0, fmin(1.0, (1.0 - rcen / a) / e))), -h);
+        double u0 = fsign(acos(fmax(-1.0, fmin(1.0, (1.0 - r0 / a) / e))), rv0);
+        double mhit = uhit - e * sin(uhit);
+        double m0 = u0 - e * sin(u0);
+        temp = (mhit - m0) / sqrt((mcen + mj) / (a * a * a));
+    } else if (e > 1.0) {
+        double a = q / (e - 1.0);
+        double uhit = fsign(acosh(fmax(1.0, (1.0 + rcen / a) / e)), -h);
+        double u0 = fsign(acosh(fmax(1.0, (1.0 + r0 / a) / e)), rv0);
+        double mhit = e * sinh(uhit) - uhit;
+        double m0 = e * sinh(u0) - u0;
+        temp = (mhit - m0) / sqrt((mcen + mj) / (a * a * a));
+    } else if (e == 1.0 && q > 0.0) {
+        double uhit = fsign(sqrt(fmax(0.0, rcen / q - 1.0)), -h);
+        double u0 = fsign(sqrt(fmax(0.0, r0 / q - 1.0)), rv0);
+        temp = sqrt(2.0 * (q * q * q) / (mcen + mj)) * (uhit + (uhit * uhit * uhit) / 3.0 - u0
+                                                        - (u0 * u0 * u0) / 3.0);
+    }
+    if (q == 0.0 && rr1 != rr0) temp = h * fmax(0.0, fmin(1.0, (r0 - rcen) / (r0 - sqrt(rr1))));
+    DevEvent ev;
+    ev.i = 1;
+    ev.j = jb + 1;
+    ev.kind = 8;
+    ev.now = 0;
+    ev.distance = rcen;
+    ev.time = time_end - h + temp;
+    for (int k = 0; k < 6; k++) { ev.xi[k] = 0.0; ev.xj[k] = 0.0; }
+    ev.step = step;
+    ev.sys = sys;
+    push_event(es, ev);
+}
+
+// ---------------------------------------------------------------------------
+// Planet prologue: one 64-lane workgroup per system, lane p owns Big body p+1.
+struct PlanetArgs {
+    const Sys* sys;
+    State st;
+    double* tab;      // this launch's table block (all systems)
+    double* tcur;     // per-system time, carried between launches
+    EventSink es;
+    double time0;     // start time when set_time != 0
+    int set_time;
+    int steps;
+    int reset;        // compute accelerations from positions first (dtflag != 2)
+    int screen;       // screen planet pairs and central impacts in-kernel
+    int save;         // write step-start copies (single-step mode)
+    int* zero;        // counter cleared by the launch (park count), may be null
+};
+
+__device__ inline void h2j_serial(int np, const double* m, const double* jdx, const double* X, const double* V,
+                                  double* XJ, double* VJ) {
+    // mco_h2j (mercury6_2.for:2613-2687); jdx[p] = 1/(mtot+m0) before body p is added
+    if (np < 1) return;
+    for (int k = 0; k < 3; k++) { XJ[k] = X[k]; VJ[k] = V[k]; }
+    double mx = m[1] * X[0], my = m[1] * X[1], mz = m[1] * X[2];
+    double mu = m[1] * V[0], mv = m[1] * V[1], mw = m[1] * V[2];
+    for (int p = 1; p < np; p++) {
+        double temp = jdx[p];
+        XJ[3 * p + 0] = X[3 * p + 0] - temp * mx;
+        XJ[3 * p + 1] = X[3 * p + 1] - temp * my;
+        XJ[3 * p + 2] = X[3 * p + 2] - temp * mz;
+        VJ[3 * p + 0] = V[3 * p + 0] - temp * mu;
+        VJ[3 * p + 1] = V[3 * p + 1] - temp * mv;
+        VJ[3 * p + 2] = V[3 * p + 2] - temp * mw;
+        if (p < np - 1) {
+            double mb = m[p + 1];
+            mx = mx + mb * X[3 * p + 0];
+            my = my + mb * X[3 * p + 1];
+            mz = mz + mb * X[3 * p + 2];
+            mu = mu + mb * V[3 * p + 0];
+            mv = mv + mb * V[3 * p + 1];
+            mw = mw + mb * V[3 * p + 2];
+        }
+    }
+}
+
+__device__ inline void j2h_serial(int np, const double* jhx, const double* XJ, const double* VJ, double* X,
+                                  double* V) {
+    // mco_j2h (mercury6_2.for:2700-2770); jhx[p] = m_b/(mtot+m0) after body p is added
+    if (np < 1) return;
+    for (int k = 0; k < 3; k++) { X[k] = XJ[k]; V[k] = VJ[k]; }
+    double temp = jhx[0];
+    double mx = temp * XJ[0], my = temp * XJ[1], mz = temp * XJ[2];
+    double mu = temp * VJ[0], mv = temp * VJ[1], mw = temp * VJ[2];
+    for (int p = 1; p < np; p++) {
+        X[3 * p + 0] = XJ[3 * p + 0] + mx;
+        X[3 * p + 1] = XJ[3 * p + 1] + my;
+        X[3 * p + 2] = XJ[3 * p + 2] + mz;
+        V[3 * p + 0] = VJ[3 * p + 0] + mu;
+        V[3 * p + 1] = VJ[3 * p + 1] + mv;
+        V[3 * p + 2] = VJ[3 * p + 2] + mw;
+        if (p < np - 1) {
+            temp = jhx[p];
+            mx = mx + temp * XJ[3 * p + 0];
+            my = my + temp * XJ[3 * p + 1];
+            mz = mz + temp * XJ[3 * p + 2];
+            mu = mu + temp * VJ[3 * p + 0];
+            mv = mv + temp * VJ[3 * p + 1];
+            mw = mw + temp * VJ[3 * p + 2];
+        }
+    }
+}
+
+// Big-body part of mfo_mvs (mercury6_2.for:4945-5025, 5062-5070) for the
+// planets in LDS.  Lane p computes body p+1; serial recurrences run in lane 0.
+// Writes A (Big accelerations) and the table header terms a0tp and acen.
+__device__ inline void big_forces(const Sys& d, int np, const double* m, const double* minside,
+                                  const double* X, const double* XJ, double* A, double* t0, double* t2,
+                                  double* a1, double* aob, double* hdr) {
+    int lane = threadIdx.x;
+    int p = lane;
+    if (p < np) {
+        // a1, and the terms summed serially into a0 and a2, for Fortran k = p+2 >= 3
+        if (p >= 1) {
+            const double* xk = X + 3 * p;
+            const double* xjk = XJ + 3 * p;
+            double r2 = xk[0] * xk[0] + xk[1] * xk[1] + xk[2] * xk[2];
+            double rj2 = xjk[0] * xjk[0] + xjk[1] * xjk[1] + xjk[2] * xjk[2];
+            double r = 1.0 / sqrt(r2);
+            double rj = 1.0 / sqrt(rj2);
+            double r3 = r * r * r;
+            double rj3 = rj * rj * rj;
+            double mk = m[p + 1];
+            double fac0 = mk * r3;
+            double fac12 = m[0] * rj3;
+            double fac2 = mk * fac12 / (minside[p] + m[0]);
+            double q = (r2 - rj2) * .5 / rj2;
+            double q2 = q * q, q3 = q * q2, q4 = q2 * q2, q5 = q2 * q3, q6 = q3 * q3, q7 = q3 * q4;
+            double fac1 = 402.1875 * q7 - 187.6875 * q6 + 86.625 * q5 - 39.375 * q4 + 17.5 * q3
+                        - 7.5 * q2 + 3.0 * q - 1.0;
+            for (int c = 0; c < 3; c++) {
+                t0[3 * p + c] = fac0 * xk[c];
+                a1[3 * p + c] = fac12 * (xjk[c] + fac1 * xk[c]);
+                t2[3 * p + c] = fac2 * xjk[c];
+            }
+        } else {
+            for (int c = 0; c < 3; c++) { t0[c] = 0.0; a1[c] = 0.0; t2[c] = 0.0; }
+        }
+        // a3 over Big pairs, in the Fortran's pair order for this body
+        double a3[3] = {0.0, 0.0, 0.0};
+        const double* xb = X + 3 * p;
+        for (int i = 0; i < p; i++) {
+            double dx = xb[0] - X[3 * i], dy = xb[1] - X[3 * i + 1], dz = xb[2] - X[3 * i + 2];
+            double s2 = dx * dx + dy * dy + dz * dz;
+            double s_1 = 1.0 / sqrt(s2);
+            double s_3 = s_1 * s_1 * s_1;
+            double faci = m[i + 1] * s_3;
+            a3[0] = a3[0] - faci * dx; a3[1] = a3[1] - faci * dy; a3[2] = a3[2] - faci * dz;
+        }
+        for (int j = p + 1; j < np; j++) {
+            double dx = X[3 * j] - xb[0], dy = X[3 * j + 1] - xb[1], dz = X[3 * j + 2] - xb[2];
+            double s2 = dx * dx + dy * dy + dz * dz;
+            double s_1 = 1.0 / sqrt(s2);
+            double s_3 = s_1 * s_1 * s_1;
+            double facj = m[j + 1] * s_3;
+            a3[0] = a3[0] + facj * dx; a3[1] = a3[1] + facj * dy; a3[2] = a3[2] + facj * dz;
+        }
+        for (int c = 0; c < 3; c++) A[3 * p + c] = a3[c];
+        if (d.obl) obl_body(d.mu, d.j2, d.j4, d.j6, xb[0], xb[1], xb[2], aob[3 * p], aob[3 * p + 1],
+                            aob[3 * p + 2]);
+    }
+    __syncthreads();
+    if (lane == 0) {
+        double a0[3] = {0.0, 0.0, 0.0};
+        double a2[3] = {0.0, 0.0, 0.0};
+        for (int q = 1; q < np; q++)
+            for (int c = 0; c < 3; c++) a0[c] = a0[c] - t0[3 * q + c];
+        double a0tp[3] = {0.0, 0.0, 0.0};
+        if (np >= 1) {
+            double r2 = X[0] * X[0] + X[1] * X[1] + X[2] * X[2];
+            double r = 1.0 / sqrt(r2);
+            double r3 = r * r * r;
+            double fac0 = m[1] * r3;
+            for (int c = 0; c < 3; c++) a0tp[c] = a0[c] - fac0 * X[c];
+        }
+        double acen[3] = {0.0, 0.0, 0.0};
+        if (d.obl) {
+            for (int q = 0; q < np; q++) {
+                double tmp4 = m[q + 1] / m[0];
+                for (int c = 0; c < 3; c++) acen[c] = acen[c] - tmp4 * aob[3 * q + c];
+            }
+        }
+        for (int q = 0; q < np; q++) {
+            for (int c = 0; c < 3; c++) {
+                if (q >= 1) a2[c] = a2[c] + t2[3 * q + c];
+                double a = a0[c] + a1[3 * q + c] + (q >= 1 ? a2[c] : 0.0) + A[3 * q + c];
+                if (d.obl) a = a + (aob[3 * q + c] - acen[c]);
+                A[3 * q + c] = a;
+            }
+        }
+        for (int c = 0; c < 3; c++) { hdr[c] = a0tp[c]; hdr[3 + c] = acen[c]; }
+    }
+    __syncthreads();
+}
+
+__global__ void __launch_bounds__(64) mvs_planets(PlanetArgs args) {
+    const int sid = blockIdx.x;
+    const Sys d = args.sys[sid];
+    const int np = d.nbig - 1;
+    const int lane = threadIdx.x;
+    __shared__ double m[kMaxBig], minside[kMaxBig], jdx[kMaxBig], jhx[kMaxBig], gm[kMaxBig];
+    __shared__ double X[3 * kMaxBig], V[3 * kMaxBig], A[3 * kMaxBig], XJ[3 * kMaxBig], VJ[3 * kMaxBig];
+    __shared__ double X0[3 * kMaxBig], V0[3 * kMaxBig], t0[3 * kMaxBig], t2[3 * kMaxBig];
+    __shared__ double a1[3 * kMaxBig], aob[3 * kMaxBig], hdr[8];
+    const long long cap = args.st.cap;
+    const int b0 = d.base;
+    for (int q = lane; q < d.nbig; q += 64) m[q] = args.st.m[b0 + q];
+    for (int p = lane; p < np; p += 64)
+        for (int c = 0; c < 3; c++) {
+            X[3 * p + c] = args.st.x[c * cap + b0 + 1 + p];
+            V[3 * p + c] = args.st.v[c * cap + b0 + 1 + p];
+            A[3 * p + c] = args.st.a[c * cap + b0 + 1 + p];
+        }
+    __syncthreads();
+    if (lane == 0) {
+        // constants of the Jacobi transforms and drifts (mdt_mvs mercury6_2.for:4041-4046)
+        double ms = 0.0;
+        for (int p = 0; p < np; p++) { minside[p] = ms; ms = ms + m[p + 1]; }
+        double mtot = (np >= 1) ? m[1] : 0.0;
+        for (int p = 1; p < np; p++) { jdx[p] = 1.0 / (mtot + m[0]); mtot = mtot + m[p + 1]; }
+        mtot = (np >= 1) ? m[1] : 0.0;
+        if (np >= 1) jhx[0] = m[1] / (mtot + m[0]);
+        for (int p = 1; p < np; p++) { mtot = mtot + m[p + 1]; jhx[p] = m[p + 1] / (mtot + m[0]); }
+        double mins = m[0];
+        for (int p = 0; p < np; p++) {
+            double msofar = mins + m[p + 1];
+            gm[p] = m[0] * msofar / mins;
+            mins = msofar;
+        }
+    }
+    __syncthreads();
+    const int ts = d.ts;
+    if (args.reset) {
+        if (lane == 0) h2j_serial(np, m, jdx, X, V, XJ, VJ);
+        __syncthreads();
+        big_forces(d, np, m, minside, X, XJ, A, t0, t2, a1, aob, hdr);
+        double* St = args.tab + d.start;
+        if (lane < 8) St[lane] = (lane < 6) ? hdr[lane] : 0.0;
+        for (int p = lane; p < np; p += 64) {
+            for (int c = 0; c < 3; c++) St[kHdr + kF * p + c] = X[3 * p + c];
+            St[kHdr + kF * p + 3] = m[p + 1];
+        }
+    }
+    double time_end = args.set_time ? args.time0 : args.tcur[sid];
+    if (args.zero && sid == 0 && lane == 0) *args.zero = 0;
+    const double h = d.h, hby2 = d.hby2;
+    for (int s = 0; s < args.steps; s++) {
+        double* T = args.tab + d.tab + (long long)s * ts;
+        time_end = time_end + h;
+        double xs[3], vs[3];
+        int p = lane;
+        if (p < np) {
+            for (int c = 0; c < 3; c++) { xs[c] = X[3 * p + c]; vs[c] = V[3 * p + c]; }
+            if (args.save)
+                for (int c = 0; c < 3; c++) {
+                    args.st.ox[c * cap + b0 + 1 + p] = xs[c];
+                    args.st.ov[c * cap + b0 + 1 + p] = vs[c];
+                }
+            for (int c = 0; c < 3; c++) {
+                V[3 * p + c] = V[3 * p + c] + hby2 * A[3 * p + c];
+                X0[3 * p + c] = X[3 * p + c];
+                V0[3 * p + c] = V[3 * p + c];
+            }
+        }
+        __syncthreads();
+        if (lane == 0) h2j_serial(np, m, jdx, X, V, XJ, VJ);
+        __syncthreads();
+        if (p < np) {
+            Orbit o{XJ[3 * p], XJ[3 * p + 1], XJ[3 * p + 2], VJ[3 * p], VJ[3 * p + 1], VJ[3 * p + 2]};
+            drift_one(gm[p], o, h);
+            XJ[3 * p] = o.x; XJ[3 * p + 1] = o.y; XJ[3 * p + 2] = o.z;
+            VJ[3 * p] = o.vx; VJ[3 * p + 1] = o.vy; VJ[3 * p + 2] = o.vz;
+        }
+        __syncthreads();
+        if (lane == 0) j2h_serial(np, jhx, XJ, VJ, X, V);
+        __syncthreads();
+        // screening data of this step (velocities after the drift, before the kick)
+        if (p < np) {
+            double* B = T + kHdr + kF * np + kScr * p;
+            double lo, hi;
+            box_axis(h, X0[3 * p], V0[3 * p], X[3 * p], V[3 * p], lo, hi);
+            double rce = args.st.rce[b0 + 1 + p];
+            double pad = rce * 1.2;
+            B[SX0] = lo - pad; B[SX1] = hi + pad;
+            box_axis(h, X0[3 * p + 1], V0[3 * p + 1], X[3 * p + 1], V[3 * p + 1], lo, hi);
+            B[SY0] = lo - pad; B[SY1] = hi + pad;
+            for (int c = 0; c < 3; c++) {
+                B[SP + c] = X0[3 * p + c];
+                B[SV0 + c] = V0[3 * p + c];
+                B[SV1 + c] = V[3 * p + c];
+                T[kHdr + kF * p + c] = X[3 * p + c];
+            }
+            B[SRCE] = rce;
+            B[SRPH] = args.st.rphys[b0 + 1 + p];
+            T[kHdr + kF * p + 3] = m[p + 1];
+            if (args.save)
+                for (int c = 0; c < 3; c++) {
+                    args.st.v0k[c * cap + b0 + 1 + p] = V0[3 * p + c];
+                    args.st.v1[c * cap + b0 + 1 + p] = V[3 * p + c];
+                }
+        }
+        big_forces(d, np, m, minside, X, XJ, A, t0, t2, a1, aob, hdr);
+        if (lane < 8) T[lane] = (lane < 6) ? hdr[lane] : (lane == 6 ? time_end : 0.0);
+        if (p < np) {
+            for (int c = 0; c < 3; c++) V[3 * p + c] = V[3 * p + c] + hby2 * A[3 * p + c];
+        }
+        __syncthreads();
+        if (args.screen && p < np) {
+            // make this step's table visible to the screening lanes
+            __threadfence_block();
+            int jb = p + 1;
+            Ends e{X0[3 * p], X0[3 * p + 1], X0[3 * p + 2], V0[3 * p], V0[3 * p + 1], V0[3 * p + 2],
+                   X[3 * p], X[3 * p + 1], X[3 * p + 2], T[kHdr + kF * np + kScr * p + SV1],
+                   T[kHdr + kF * np + kScr * p + SV1 + 1], T[kHdr + kF * np + kScr * p + SV1 + 2]};
+            screen_body(T, np, jb, d.nbig, m[jb], e, args.st.rce[b0 + jb], args.st.rphys[b0 + jb], h,
+                        time_end, s, sid, args.es);
+            central_check(m[0], m[jb], d.rcen, h, time_end, xs[0], xs[1], xs[2], vs[0], vs[1], vs[2],
+                          X[3 * p], X[3 * p + 1], X[3 * p + 2], V[3 * p], V[3 * p + 1], V[3 * p + 2], jb,
+                          s, sid, args.es);
+        }
+        __syncthreads();
+    }
+    for (int p = lane; p < np; p += 64)
+        for (int c = 0; c < 3; c++) {
+            args.st.x[c * cap + b0 + 1 + p] = X[3 * p + c];
+            args.st.v[c * cap + b0 + 1 + p] = V[3 * p + c];
+            args.st.a[c * cap + b0 + 1 + p] = A[3 * p + c];
+        }
+    if (lane == 0) args.tcur[sid] = time_end;
+}
+
+// ---------------------------------------------------------------------------
+// Particle passes.
+struct ParticleArgs {
+    const Sys* sys;
+    const int2* tiles;   // (system, first particle index) per workgroup
+    State st;
+    const double* tab;
+    EventSink es;
+    ParkList park;
+    int steps;
+    int reset;
+    int screen;
+    int save;
+};
+
+// Step-start copies for the host-driven event call (single-step mode).
+__device__ __forceinline__ void save_step(const State& st, int g, const double xs[6], const Orbit& k0,
+                                          const Orbit& o) {
+    const long long cap = st.cap;
+    st.ox[g] = xs[0]; st.ox[cap + g] = xs[1]; st.ox[2 * cap + g] = xs[2];
+    st.ov[g] = xs[3]; st.ov[cap + g] = xs[4]; st.ov[2 * cap + g] = xs[5];
+    st.v0k[g] = k0.vx; st.v0k[cap + g] = k0.vy; st.v0k[2 * cap + g] = k0.vz;
+    st.v1[g] = o.vx; st.v1[cap + g] = o.vy; st.v1[2 * cap + g] = o.vz;
+}
+
+// Rest of one step after the drift: screen, force, second kick, central check.
+// A non-finite acceleration raises the fault flag (the step then fails, as
+// mvs_acceleration's check does in the reference, mercury_cuda_symplectic.cuh:78).
+template <bool SCREEN>
+__device__ __forceinline__ void finish_step(const Sys& d, const double* __restrict__ T, int np, int jb,
+                                            double mj, double rce_j, double rphys_j, int s, int sid,
+                                            const EventSink& es, int* fault, const double xs[6],
+                                            const Orbit& k0, Orbit& o, double& ax, double& ay, double& az) {
+    if (SCREEN) {
+        Ends e{k0.x, k0.y, k0.z, k0.vx, k0.vy, k0.vz, o.x, o.y, o.z, o.vx, o.vy, o.vz};
+        screen_body(T, np, jb, d.nbig, mj, e, rce_j, rphys_j, d.h, T[6], s, sid, es);
+    }
+    particle_accel(d, T, np, o.x, o.y, o.z, ax, ay, az);
+    if (!isfinite(ax) || !isfinite(ay) || !isfinite(az)) *fault = 1;
+    o.vx = o.vx + d.hby2 * ax;
+    o.vy = o.vy + d.hby2 * ay;
+    o.vz = o.vz + d.hby2 * az;
+    if (SCREEN)
+        central_check(d.mu, mj, d.rcen, d.h, T[6], xs[0], xs[1], xs[2], xs[3], xs[4], xs[5], o.x, o.y, o.z,
+                      o.vx, o.vy, o.vz, jb, s, sid, es);
+}
+
+template <bool SCREEN, bool SAVE>
+__global__ void __launch_bounds__(256) mvs_particles_fast(ParticleArgs args) {
+    const int2 tile = args.tiles[blockIdx.x];
+    const int sid = tile.x;
+    const Sys d = args.sys[sid];
+    const int g = tile.y + threadIdx.x;            // global body index
+    const int jb = g - d.base;                     // local body index
+    if (jb >= d.n) return;
+    const long long cap = args.st.cap;
+    const int np = d.nbig - 1;
+    Orbit o{args.st.x[g], args.st.x[cap + g], args.st.x[2 * cap + g],
+            args.st.v[g], args.st.v[cap + g], args.st.v[2 * cap + g]};
+    double ax, ay, az;
+    if (args.reset) {
+        particle_accel(d, args.tab + d.start, np, o.x, o.y, o.z, ax, ay, az);
+    } else {
+        ax = args.st.a[g]; ay = args.st.a[cap + g]; az = args.st.a[2 * cap + g];
+    }
+    double mj = 0.0, rce_j = 0.0, rphys_j = 0.0;
+    if (SCREEN || SAVE) { mj = args.st.m[g]; rce_j = args.st.rce[g]; rphys_j = args.st.rphys[g]; }
+    const double* tab = args.tab + d.tab;
+    int s = 0;
+    for (; s < args.steps; s++) {
+        const double* T = tab + (long long)s * d.ts;
+        double xs[6] = {o.x, o.y, o.z, o.vx, o.vy, o.vz};
+        Orbit w = o;
+        w.vx = w.vx + d.hby2 * ax;
+        w.vy = w.vy + d.hby2 * ay;
+        w.vz = w.vz + d.hby2 * az;
+        Orbit k0 = w;
+        bool ok = drift_fast(d.mu, w, d.h);
+        // wave-ballot compaction: lanes whose drift leaves the kepmd branch
+        // park here and finish in the general kernel
+        unsigned long long mask = __ballot(!ok);
+        if (!ok) {
+            int leader = __ffsll((long long)mask) - 1;
+            int lane = __lane_id();
+            int base = 0;
+            if (lane == leader) base = atomicAdd(args.park.count, __popcll(mask));
+            base = __shfl(base, leader);
+            int slot = base + __popcll(mask & ((1ull << lane) - 1ull));
+            if (slot < args.park.cap) args.park.buf[slot] = Park{g, s, sid, 0};
+            break;
+        }
+        o = w;
+        if (SAVE) save_step(args.st, g, xs, k0, o);
+        finish_step<SCREEN>(d, T, np, jb, mj, rce_j, rphys_j, s, sid, args.es, args.st.fault, xs, k0, o,
+                            ax, ay, az);
+    }
+    // parked lanes store their pre-step state and step-start acceleration
+    args.st.x[g] = o.x; args.st.x[cap + g] = o.y; args.st.x[2 * cap + g] = o.z;
+    args.st.v[g] = o.vx; args.st.v[cap + g] = o.vy; args.st.v[2 * cap + g] = o.vz;
+    args.st.a[g] = ax; args.st.a[cap + g] = ay; args.st.a[2 * cap + g] = az;
+}
+
+
+// Parked lanes: resume each from its recorded step with the whole drift chain.
+// Lanes of one wave may start at different steps and belong to different
+// systems, so system and table reads here are per lane.
+template <bool SCREEN, bool SAVE>
+__global__ void __launch_bounds__(64) mvs_particles_general(ParticleArgs args) {
+    const int count = min(*args.park.count, args.park.cap);
+    const long long cap = args.st.cap;
+    for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < count; idx += gridDim.x * blockDim.x) {
+        const Park pk = args.park.buf[idx];
+        const Sys d = args.sys[pk.sys];
+        const int g = pk.j;
+        const int jb = g - d.base;
+        const int np = d.nbig - 1;
+        Orbit o{args.st.x[g], args.st.x[cap + g], args.st.x[2 * cap + g],
+                args.st.v[g], args.st.v[cap + g], args.st.v[2 * cap + g]};
+        double ax = args.st.a[g], ay = args.st.a[cap + g], az = args.st.a[2 * cap + g];
+        double mj = 0.0, rce_j = 0.0, rphys_j = 0.0;
+        if (SCREEN || SAVE) { mj = args.st.m[g]; rce_j = args.st.rce[g]; rphys_j = args.st.rphys[g]; }
+        const double* tab = args.tab + d.tab;
+        for (int s = pk.step; s < args.steps; s++) {
+            const double* T = tab + (long long)s * d.ts;
+            double xs[6] = {o.x, o.y, o.z, o.vx, o.vy, o.vz};
+            o.vx = o.vx + d.hby2 * ax;
+            o.vy = o.vy + d.hby2 * ay;
+            o.vz = o.vz + d.hby2 * az;
+            Orbit k0 = o;
+            drift_one(d.mu, o, d.h);
+            if (SAVE) save_step(args.st, g, xs, k0, o);
+            finish_step<SCREEN>(d, T, np, jb, mj, rce_j, rphys_j, s, pk.sys, args.es, args.st.fault, xs,
+                                k0, o, ax, ay, az);
+        }
+        args.st.x[g] = o.x; args.st.x[cap + g] = o.y; args.st.x[2 * cap + g] = o.z;
+        args.st.v[g] = o.vx; args.st.v[cap + g] = o.vy; args.st.v[2 * cap + g] = o.vz;
+        args.st.a[g] = ax; args.st.a[cap + g] = ay; args.st.a[2 * cap + g] = az;
+    }
+}
+
+// Host-driven event screen over the step-start copies of one single-step
+// launch (the reference's events call, mercury_cuda.cu:470-493): pairs with
+// the Big bodies of step 0's table and central impacts, every body j >= 1.
+struct SavedScreenArgs {
+    const Sys* sys;
+    State st;
+    const double* tab;
+    EventSink es;
+    double time, h, rcen;
+    int pairs;        // 0: central impacts only
+};
+
+__global__ void __launch_bounds__(256) screen_saved(SavedScreenArgs args) {
+    const Sys d = args.sys[0];
+    const int jb = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    if (jb >= d.n) return;
+    const long long cap = args.st.cap;
+    const int g = d.base + jb;
+    const int np = d.nbig - 1;
+    const double* T = args.tab + d.tab;
+    const double mj = args.st.m[g];
+    double x1 = args.st.x[g], y1 = args.st.x[cap + g], z1 = args.st.x[2 * cap + g];
+    double xs = args.st.ox[g], ys = args.st.ox[cap + g], zs = args.st.ox[2 * cap + g];
+    if (args.pairs) {
+        Ends e{xs, ys, zs, args.st.v0k[g], args.st.v0k[cap + g], args.st.v0k[2 * cap + g],
+               x1, y1, z1, args.st.v1[g], args.st.v1[cap + g], args.st.v1[2 * cap + g]};
+        screen_body(T, np, jb, d.nbig, mj, e, args.st.rce[g], args.st.rphys[g], args.h, args.time, 0, 0,
+                    args.es);
+    }
+    central_check(d.mu, mj, args.rcen, args.h, args.time, xs, ys, zs, args.st.ov[g], args.st.ov[cap + g],
+                  args.st.ov[2 * cap + g], x1, y1, z1, args.st.v[g], args.st.v[cap + g],
+                  args.st.v[2 * cap + g], jb, 0, 0, args.es);
+}
+
+__global__ void zero_ints(int* p, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+}  // namespace mhip

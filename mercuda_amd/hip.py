@@ -1,0 +1,188 @@
+"""ctypes front end of libmercury_hip (include/mercury_hip.h).
+
+``MercuryHip`` mirrors the calls the reference's Fortran binding makes over
+the backend ABI (mercury_gpu.f90: gpu_select/push/advance/pull/find_events/
+central) with the same argument meaning and the same failure behaviour: any
+nonzero return raises ``MercuryHipError`` -- there is no CPU fallback.  Arrays
+use MERCURY's layout, x[n,3] (C order == Fortran x(3,n)), body 0 central.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libmercury_hip.so"
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_i64p = C.POINTER(C.c_int64)
+
+
+class MercuryHipError(RuntimeError):
+    pass
+
+
+class MercuryEvent(C.Structure):
+    _fields_ = [("i", C.c_int), ("j", C.c_int), ("kind", C.c_int), ("now", C.c_int),
+                ("distance", C.c_double), ("time", C.c_double),
+                ("xi", C.c_double * 6), ("xj", C.c_double * 6)]
+
+
+class MercuryHipEvent(C.Structure):
+    _fields_ = [("e", MercuryEvent), ("step", C.c_int), ("system", C.c_int)]
+
+
+# Every function the header declares, with its ctypes signature.
+SIGNATURES = {
+    "mercury_hip_available": (C.c_int, []),
+    "mercury_hip_configure": (C.c_int, [C.c_int]),
+    "mercury_hip_reset": (None, [C.c_int]),
+    "mercury_hip_limits": (C.c_int, [_dp]),
+    "mercury_hip_export": (C.c_int, [C.c_double, C.c_int, _dp, _dp]),
+    "mercury_hip_upload": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
+    "mercury_hip_step": (C.c_int, [C.c_double, _dp, _dp, C.c_double, _i64p, _i64p]),
+    "mercury_hip_download": (C.c_int, [_dp, _dp, C.c_int]),
+    "mercury_hip_events": (C.c_int, [C.c_double, C.c_double, C.c_double,
+                                     C.POINTER(C.POINTER(MercuryEvent)), _ip]),
+    "mercury_hip_force": (C.c_int, [_dp]),
+    "mercury_hip_hybrid_begin": (C.c_int, [C.c_double, _dp, C.c_int, C.c_int, _ip, _ip, _ip, _ip,
+                                           _dp, _dp, _i64p]),
+    "mercury_hip_hybrid_finish": (C.c_int, [C.c_double, _dp, _dp, _dp, _i64p]),
+    "mercury_hip_encounter_enter": (C.c_int, [C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp,
+                                              C.c_int, _ip, _ip]),
+    "mercury_hip_encounter_update": (C.c_int, [_dp, _dp, _dp]),
+    "mercury_hip_encounter_exit": (None, []),
+    "mercury_hip_free": (None, []),
+    "mercury_hip_advance": (C.c_int, [C.c_double, C.c_double, C.c_int, C.c_double, C.c_int, _i64p]),
+    "mercury_hip_advance_events": (C.c_int, [C.POINTER(C.POINTER(MercuryHipEvent)), _ip]),
+    "mercury_hip_set_tuning": (C.c_int, [C.c_int, C.c_int]),
+    "mercury_hip_timing": (C.c_int, [C.c_int]),
+    "mercury_hip_timing_read": (C.c_int, [_dp, _ip, _dp, _ip, _dp, _ip]),
+    "mercury_hip_parked": (C.c_int64, []),
+}
+
+
+def load_library(path: Path = LIB_PATH) -> C.CDLL:
+    """Load libmercury_hip.so and bind every declared symbol (raises if any is missing)."""
+    if not Path(path).exists():
+        raise MercuryHipError(f"{path} is not built; run python -m mercuda_amd.build")
+    lib = C.CDLL(str(path))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _f64(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if shape is not None:
+        a = a.reshape(shape)
+    return a
+
+
+def event_tuple(e: MercuryEvent):
+    return (e.i, e.j, e.kind, e.now, e.distance, e.time, tuple(e.xi), tuple(e.xj))
+
+
+class MercuryHip:
+    """The HIP backend for one process (the library holds one global context)."""
+
+    def __init__(self, path: Path = LIB_PATH):
+        self.L = load_library(path)
+        self.n = 0
+        self.nbig = 0
+
+    def _ok(self, rc, what):
+        if rc != 0:
+            raise MercuryHipError(f"{what} failed")
+
+    def available(self) -> bool:
+        return bool(self.L.mercury_hip_available())
+
+    def configure(self, algorithm: int):
+        self._ok(self.L.mercury_hip_configure(algorithm), f"configure({algorithm})")
+
+    def reset(self, flag: int = 0):
+        self.L.mercury_hip_reset(flag)
+
+    def upload(self, m, x, v, nbig, *, pn=0, ngflag=0, ngf=None, jcen=(0.0, 0.0, 0.0), rce=None, rphys=None):
+        m = _f64(m); n = len(m)
+        x = _f64(x, (n, 3)); v = _f64(v, (n, 3))
+        ngf = _f64(np.zeros((n, 5)) if ngf is None else ngf, (n, 5))
+        jc = _f64(jcen, (3,))
+        rce = _f64(np.zeros(n) if rce is None else rce, (n,))
+        rphys = _f64(np.zeros(n) if rphys is None else rphys, (n,))
+        self._ok(self.L.mercury_hip_upload(n, nbig, pn, ngflag, _d(m), _d(x), _d(v), _d(ngf), _d(jc),
+                                           _d(rce), _d(rphys)), "upload")
+        self.n, self.nbig = n, nbig
+
+    def upload_system(self, sysd, **kw):
+        self.upload(sysd["m"], sysd["x"], sysd["v"], sysd["nbig"], jcen=sysd.get("jcen", (0, 0, 0)),
+                    rce=sysd["rce"], rphys=sysd["rphys"], **kw)
+
+    def step(self, time, h, tol=1e-12):
+        hh = C.c_double(h); hdid = C.c_double(0.0)
+        nf = C.c_int64(0); nr = C.c_int64(0)
+        self._ok(self.L.mercury_hip_step(time, C.byref(hh), C.byref(hdid), tol, C.byref(nf), C.byref(nr)),
+                 "step")
+        return hh.value, hdid.value, nf.value, nr.value
+
+    def download(self, previous=False):
+        x = np.zeros((self.n, 3)); v = np.zeros((self.n, 3))
+        self._ok(self.L.mercury_hip_download(_d(x), _d(v), int(previous)), "download")
+        return x, v
+
+    def export(self, h, physical=True):
+        x = np.zeros((self.n, 3)); v = np.zeros((self.n, 3))
+        self._ok(self.L.mercury_hip_export(h, int(physical), _d(x), _d(v)), "export")
+        return x, v
+
+    def events(self, time, h, rcen):
+        ptr = C.POINTER(MercuryEvent)(); cnt = C.c_int(0)
+        self._ok(self.L.mercury_hip_events(time, h, rcen, C.byref(ptr), C.byref(cnt)), "events")
+        return [event_tuple(ptr[k]) for k in range(cnt.value)]
+
+    def limits(self, rce):
+        rce = _f64(rce, (self.n,))
+        self._ok(self.L.mercury_hip_limits(_d(rce)), "limits")
+
+    def force(self):
+        a = np.zeros((self.n, 3))
+        self._ok(self.L.mercury_hip_force(_d(a)), "force")
+        return a
+
+    def advance(self, time, h, nsteps, rcen=0.005, screen=True):
+        nf = C.c_int64(0)
+        self._ok(self.L.mercury_hip_advance(time, h, nsteps, rcen, int(screen), C.byref(nf)), "advance")
+        return nf.value
+
+    def advance_events(self):
+        ptr = C.POINTER(MercuryHipEvent)(); cnt = C.c_int(0)
+        self._ok(self.L.mercury_hip_advance_events(C.byref(ptr), C.byref(cnt)), "advance_events")
+        return [(ptr[k].system, ptr[k].step) + event_tuple(ptr[k].e) for k in range(cnt.value)]
+
+    def set_tuning(self, steps_per_launch=0, block_size=0):
+        self._ok(self.L.mercury_hip_set_tuning(steps_per_launch, block_size), "set_tuning")
+
+    def timing(self, enable=True):
+        self._ok(self.L.mercury_hip_timing(int(enable)), "timing")
+
+    def timing_read(self):
+        pm = C.c_double(); pl = C.c_int(); qm = C.c_double(); ql = C.c_int(); gm = C.c_double(); gl = C.c_int()
+        self._ok(self.L.mercury_hip_timing_read(C.byref(pm), C.byref(pl), C.byref(qm), C.byref(ql),
+                                                C.byref(gm), C.byref(gl)), "timing_read")
+        return dict(particle_ms=pm.value, particle_launches=pl.value, planet_ms=qm.value,
+                    planet_launches=ql.value, general_ms=gm.value, general_launches=gl.value)
+
+    def parked(self) -> int:
+        return int(self.L.mercury_hip_parked())
+
+    def free(self):
+        self.L.mercury_hip_free()

@@ -1,0 +1,298 @@
+"""ctypes bindings for the checker libraries -- TEST INFRASTRUCTURE ONLY.
+
+``Oracle`` wraps liborc.so, the C restatement of the reference hot path
+(mercury_oracle.c).  ``Reference`` wraps _ref/libmercury_ref.so, the reference's
+own Fortran routines compiled from mercury6_2.for by oracle/Makefile, called the
+way tests/force_library.py in the reference calls them (by reference, with the
+/mercury_sizes/ common block set first, mercury.inc:17-21).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use these.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+ORC_PATH = HERE / "liborc.so"
+REF_PATH = HERE / "_ref" / "libmercury_ref.so"
+REF_BIN = HERE / "_ref" / "mercury6"
+REFERENCE_SRC = Path("/root/reference")
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+def build(reference: bool = True) -> None:
+    """Compile liborc.so, and _ref/ when the reference sources are present."""
+    subprocess.run(["make", "-s", "-C", str(HERE), "all"], check=True)
+    if reference and REFERENCE_SRC.is_dir():
+        subprocess.run(["make", "-s", "-C", str(HERE), "-j4", "ref"], check=True)
+
+
+class _Events(C.Structure):
+    _fields_ = [("cap", C.c_int), ("nclo", C.c_int),
+                ("iclo", _ip), ("jclo", _ip), ("dclo", _dp), ("tclo", _dp),
+                ("ixvclo", _dp), ("jxvclo", _dp),
+                ("nhit", C.c_int), ("ihit", _ip), ("jhit", _ip), ("chit", _ip),
+                ("dhit", _dp), ("thit", _dp), ("thit1", C.c_double),
+                ("nowflag", C.c_int), ("overflow", C.c_int)]
+
+
+class EventLists:
+    """Owner of the arrays an _Events struct points into."""
+
+    def __init__(self, cap=4096):
+        self.cap = cap
+        self.iclo = np.zeros(cap, np.int32); self.jclo = np.zeros(cap, np.int32)
+        self.dclo = np.zeros(cap); self.tclo = np.zeros(cap)
+        self.ixv = np.zeros((cap, 6)); self.jxv = np.zeros((cap, 6))
+        self.ihit = np.zeros(cap, np.int32); self.jhit = np.zeros(cap, np.int32)
+        self.chit = np.zeros(cap, np.int32)
+        self.dhit = np.zeros(cap); self.thit = np.zeros(cap)
+        self.s = _Events(cap, 0, _i(self.iclo), _i(self.jclo), _d(self.dclo), _d(self.tclo),
+                         _d(self.ixv), _d(self.jxv), 0, _i(self.ihit), _i(self.jhit),
+                         _i(self.chit), _d(self.dhit), _d(self.thit), 0.0, 0, 0)
+
+    def clo(self):
+        n = self.s.nclo
+        return [(int(self.iclo[k]), int(self.jclo[k]), float(self.dclo[k]), float(self.tclo[k]))
+                for k in range(n)]
+
+    def hit(self):
+        n = self.s.nhit
+        return [(int(self.ihit[k]), int(self.jhit[k]), int(self.chit[k]), float(self.dhit[k]),
+                 float(self.thit[k])) for k in range(n)]
+
+
+class Oracle:
+    """The C restatement (liborc.so)."""
+
+    def __init__(self, path: Path = ORC_PATH):
+        if not Path(path).exists():
+            raise FileNotFoundError(f"{path} not built; run make -C oracle")
+        L = self.L = C.CDLL(str(path))
+        L.orc_drift_one.restype = C.c_int
+        L.orc_drift_one.argtypes = [C.c_double, _dp, C.c_double]
+        L.orc_drift_branch.restype = C.c_int
+        L.orc_drift_branch.argtypes = [C.c_double, _dp, C.c_double]
+        L.orc_mco_sine.argtypes = [C.c_double, _dp, _dp]
+        L.orc_drift_kepu_stumpff.argtypes = [C.c_double, _dp]
+        L.orc_mco_h2j.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp]
+        L.orc_mco_j2h.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp]
+        L.orc_mfo_mvs.argtypes = [_dp, C.c_int, C.c_int, _dp, _dp, _dp, _dp]
+        L.orc_mfo_obl.argtypes = [_dp, C.c_int, _dp, _dp, _dp, _dp]
+        L.orc_mce_min.argtypes = [C.c_double] * 5 + [_dp, _dp]
+        L.orc_mce_stat.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
+                                   _dp, _dp, _dp, _dp, _dp, _dp, _dp, C.POINTER(_Events)]
+        L.orc_mce_cent.restype = C.c_int
+        L.orc_mce_cent.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
+                                   _dp, _dp, _dp, _dp, _dp, C.c_int, _ip, _dp, _dp]
+        L.orc_mdt_mvs.argtypes = [C.c_double, C.c_double, C.c_double, _dp, C.c_int, C.c_int,
+                                  _dp, _dp, _dp, _dp, _ip, _dp, _dp, C.POINTER(_Events)]
+        L.orc_mco_mvs2h.argtypes = [C.c_double, _dp, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp]
+        L.orc_mvs_run.restype = C.c_int
+        L.orc_mvs_run.argtypes = [C.c_double, C.c_double, C.c_double, _dp, C.c_int, C.c_int,
+                                  _dp, _dp, _dp, _dp, _dp, C.c_int, C.c_int, C.c_int, _ip, _dp]
+
+    def drift_one(self, mu, xv, dt):
+        w = np.array(xv, dtype=np.float64).copy()
+        flag = self.L.orc_drift_one(mu, _d(w), dt)
+        return w, flag
+
+    def drift_branch(self, mu, xv, dt):
+        w = np.ascontiguousarray(xv, dtype=np.float64)
+        return self.L.orc_drift_branch(mu, _d(w), dt)
+
+    def mco_sine(self, x):
+        s = np.zeros(1); c = np.zeros(1)
+        self.L.orc_mco_sine(x, _d(s), _d(c))
+        return s[0], c[0]
+
+    def stumpff(self, x):
+        c = np.zeros(4)
+        self.L.orc_drift_kepu_stumpff(x, _d(c))
+        return c
+
+    def mfo_mvs(self, m, x, nbig, jcen=(0.0, 0.0, 0.0)):
+        m = np.ascontiguousarray(m, np.float64); x = np.ascontiguousarray(x, np.float64)
+        n = len(m)
+        xj = np.zeros_like(x); vj = np.zeros_like(x)
+        self.L.orc_mco_h2j(nbig, _d(m), _d(x), _d(np.zeros_like(x)), _d(xj), _d(vj))
+        for j in range(nbig, n):
+            xj[j] = x[j]
+        a = np.zeros_like(x)
+        jc = np.ascontiguousarray(jcen, np.float64)
+        self.L.orc_mfo_mvs(_d(jc), n, nbig, _d(m), _d(x), _d(xj), _d(a))
+        return a
+
+    def mce_min(self, d0, d1, d0t, d1t, h):
+        a = np.zeros(1); b = np.zeros(1)
+        self.L.orc_mce_min(d0, d1, d0t, d1t, h, _d(a), _d(b))
+        return a[0], b[0]
+
+    def mce_stat(self, time, h, m, x0, v0, x1, v1, rce, rphys, nbig, cap=4096):
+        ev = EventLists(cap)
+        arrs = [np.ascontiguousarray(a, np.float64) for a in (m, x0, v0, x1, v1, rce, rphys)]
+        self.L.orc_mce_stat(time, h, 0.0, len(arrs[0]), nbig, *[_d(a) for a in arrs], C.byref(ev.s))
+        return ev
+
+    def mce_cent(self, time, h, rcen, m, x0, v0, x1, v1, i0=2):
+        arrs = [np.ascontiguousarray(a, np.float64) for a in (m, x0, v0, x1, v1)]
+        n = len(arrs[0])
+        j = np.zeros(max(n, 1), np.int32); t = np.zeros(max(n, 1)); d = np.zeros(max(n, 1))
+        k = self.L.orc_mce_cent(time, h, rcen, i0, n, *[_d(a) for a in arrs], n, _i(j), _d(t), _d(d))
+        return [(int(j[q]), float(t[q]), float(d[q])) for q in range(k)]
+
+    def mvs_run(self, sysd, nsteps, time0=0.0, screen=True, cap=200000):
+        """nsteps MVS steps of a whole system dict (see mercuda_amd.systems)."""
+        m = np.ascontiguousarray(sysd["m"], np.float64)
+        x = np.ascontiguousarray(sysd["x"], np.float64).copy()
+        v = np.ascontiguousarray(sysd["v"], np.float64).copy()
+        rce = np.ascontiguousarray(sysd["rce"], np.float64)
+        rphys = np.ascontiguousarray(sysd["rphys"], np.float64)
+        jc = np.ascontiguousarray(sysd.get("jcen", (0.0, 0.0, 0.0)), np.float64)
+        evi = np.zeros((cap, 4), np.int32); evd = np.zeros((cap, 2))
+        k = self.L.orc_mvs_run(time0, sysd["h"], sysd["rcen"], _d(jc), len(m), sysd["nbig"],
+                               _d(m), _d(x), _d(v), _d(rce), _d(rphys), nsteps, int(screen),
+                               cap, _i(evi), _d(evd))
+        if k < 0:
+            raise RuntimeError("oracle event capacity exceeded")
+        return x, v, evi[:k].copy(), evd[:k].copy()
+
+    def mvs2h(self, sysd, x, v):
+        m = np.ascontiguousarray(sysd["m"], np.float64)
+        jc = np.ascontiguousarray(sysd.get("jcen", (0.0, 0.0, 0.0)), np.float64)
+        x = np.ascontiguousarray(x, np.float64); v = np.ascontiguousarray(v, np.float64)
+        xh = np.zeros_like(x); vh = np.zeros_like(v)
+        self.L.orc_mco_mvs2h(sysd["h"], _d(jc), len(m), sysd["nbig"], _d(m), _d(x), _d(v),
+                             _d(xh), _d(vh))
+        return xh, vh
+
+
+class Reference:
+    """The reference's own Fortran routines (oracle/_ref/libmercury_ref.so)."""
+
+    NMESS = 200
+
+    def __init__(self, path: Path = REF_PATH, nmax=20000, cmax=20000):
+        if not Path(path).exists():
+            raise FileNotFoundError(f"{path} not built; run make -C oracle ref")
+        L = self.L = C.CDLL(str(path))
+        sizes = (C.c_int * 2).in_dll(L, "mercury_sizes_")
+        sizes[0] = nmax
+        sizes[1] = cmax
+        self.nmax, self.cmax = nmax, cmax
+
+    @staticmethod
+    def _r(v):
+        return C.byref(C.c_double(v))
+
+    def drift_one(self, mu, xv, dt):
+        w = [C.c_double(float(q)) for q in xv]
+        fl = C.c_int(0)
+        self.L.drift_one_(self._r(mu), *[C.byref(q) for q in w], self._r(dt), C.byref(fl))
+        return np.array([q.value for q in w]), fl.value
+
+    def mco_sine(self, x):
+        xx = C.c_double(x); s = C.c_double(); c = C.c_double()
+        self.L.mco_sine_(C.byref(xx), C.byref(s), C.byref(c))
+        return s.value, c.value
+
+    def stumpff(self, x):
+        xx = C.c_double(x); c = [C.c_double() for _ in range(4)]
+        self.L.drift_kepu_stumpff_(C.byref(xx), *[C.byref(q) for q in c])
+        return np.array([q.value for q in c])
+
+    def mfo_mvs(self, m, x, nbig, jcen=(0.0, 0.0, 0.0)):
+        m = np.ascontiguousarray(m, np.float64); x = np.ascontiguousarray(x, np.float64)
+        n = len(m)
+        xj = np.zeros_like(x); vj = np.zeros_like(x); v = np.zeros_like(x)
+        ngf = np.zeros((n, 5)); opt = np.zeros(8, np.int32)
+        jc = np.ascontiguousarray(jcen, np.float64)
+        z = C.c_double(0.0); ng = C.c_int(0)
+        self.L.mco_h2j_(C.byref(z), _d(jc), C.byref(C.c_int(nbig)), C.byref(C.c_int(nbig)),
+                        C.byref(z), _d(m), _d(x), _d(v), _d(xj), _d(vj), _d(ngf), C.byref(ng), _i(opt))
+        for j in range(nbig, n):
+            xj[j] = x[j]
+        a = np.zeros_like(x); stat = np.zeros(n, np.int32)
+        self.L.mfo_mvs_(_d(jc), C.byref(C.c_int(n)), C.byref(C.c_int(nbig)), _d(m), _d(x),
+                        _d(xj), _d(a), _i(stat))
+        return a
+
+    def mce_min(self, d0, d1, d0t, d1t, h):
+        a = C.c_double(); b = C.c_double()
+        self.L.mce_min_(self._r(d0), self._r(d1), self._r(d0t), self._r(d1t), self._r(h),
+                        C.byref(a), C.byref(b))
+        return a.value, b.value
+
+    def mce_cent(self, time, h, rcen, m, x0, v0, x1, v1, i0=2):
+        arrs = [np.ascontiguousarray(a, np.float64) for a in (m, x0, v0, x1, v1)]
+        n = len(arrs[0])
+        jhit = np.zeros(self.cmax, np.int32); thit = np.zeros(self.cmax); dhit = np.zeros(self.cmax)
+        nhit = C.c_int(0); jc = np.zeros(3); ngf = np.zeros((max(n, 1), 5))
+        self.L.mce_cent_(self._r(time), self._r(h), self._r(rcen), _d(jc), C.byref(C.c_int(i0)),
+                         C.byref(C.c_int(n)), C.byref(C.c_int(n)), _d(arrs[0]), _d(arrs[1]),
+                         _d(arrs[2]), _d(arrs[3]), _d(arrs[4]), C.byref(nhit), _i(jhit),
+                         _d(thit), _d(dhit), C.byref(C.c_int(1)), _d(ngf), C.byref(C.c_int(0)))
+        return [(int(jhit[q]), float(thit[q]), float(dhit[q])) for q in range(nhit.value)]
+
+    def mdt_mvs_run(self, sysd, nsteps, time0=0.0, algor=1):
+        """nsteps calls of the reference mdt_mvs on a whole system, plus the
+        driver's mce_cent after each step (mal_hcon mercury6_2.for:811-841).
+        Returns final (x, v) and event rows in Oracle.mvs_run's layout; the
+        near-miss list of mce_stat stays inside mdt_mvs, so only close-encounter
+        (kind 1) and central-impact (kind 8) rows are returned."""
+        m = np.ascontiguousarray(sysd["m"], np.float64)
+        n = len(m)
+        nbig = sysd["nbig"]
+        x = np.ascontiguousarray(sysd["x"], np.float64).copy()
+        v = np.ascontiguousarray(sysd["v"], np.float64).copy()
+        rce = np.ascontiguousarray(sysd["rce"], np.float64)
+        rphys = np.ascontiguousarray(sysd["rphys"], np.float64)
+        jc = np.ascontiguousarray(sysd.get("jcen", (0.0, 0.0, 0.0)), np.float64)
+        h = sysd["h"]; rcen = sysd["rcen"]
+        cm = self.cmax
+        s = np.zeros((n, 3)); rcrit = np.zeros(n); stat = np.zeros(n, np.int32)
+        ngf = np.zeros((n, 5)); opt = np.array([0, 1, 1, 2, 0, 1, 0, 0], np.int32)
+        ids = C.create_string_buffer(b" " * (25 * n))
+        outfile = C.create_string_buffer(b"/dev/null".ljust(80) * 3)
+        mem = C.create_string_buffer(b" " * (80 * self.NMESS))
+        lmem = np.ones(self.NMESS, np.int32)
+        iclo = np.zeros(cm, np.int32); jclo = np.zeros(cm, np.int32)
+        dclo = np.zeros(cm); tclo = np.zeros(cm)
+        ixv = np.zeros((cm, 6)); jxv = np.zeros((cm, 6))
+        nclo = C.c_int(0); colflag = C.c_int(0); dtflag = C.c_int(0)
+        en = np.zeros(3); am = np.zeros(3)
+        rows_i = []; rows_d = []
+        time = time0
+        for step in range(nsteps):
+            xs = x.copy(); vs = v.copy()
+            self.L.mdt_mvs_(self._r(time), self._r(time0), self._r(h), self._r(1e-12),
+                            self._r(100.0), _d(en), _d(am), _d(jc), self._r(rcen),
+                            C.byref(C.c_int(n)), C.byref(C.c_int(nbig)), _d(m), _d(x), _d(v),
+                            _d(s), _d(rphys), _d(rcrit), _d(rce), _i(stat), ids,
+                            _d(ngf), C.byref(C.c_int(algor)), _i(opt), C.byref(dtflag),
+                            C.byref(C.c_int(0)), C.byref(C.c_int(-1)), C.byref(colflag),
+                            C.byref(nclo), _i(iclo), _i(jclo), _d(dclo), _d(tclo), _d(ixv), _d(jxv),
+                            outfile, mem, _i(lmem),
+                            C.c_size_t(25), C.c_size_t(80), C.c_size_t(80))
+            time = time + h
+            for k in range(nclo.value):
+                rows_i.append((step, 1, int(iclo[k]), int(jclo[k])))
+                rows_d.append((float(dclo[k]), float(tclo[k])))
+            for (j, t, d) in self.mce_cent(time, h, rcen, m, xs, vs, x, v):
+                rows_i.append((step, 8, 1, j)); rows_d.append((d, t))
+        return x, v, np.array(rows_i, np.int32).reshape(-1, 4), np.array(rows_d).reshape(-1, 2)

@@ -1,0 +1,923 @@
+/*
+ * mercury_oracle.c -- CPU restatement of MERCURY6's massless-body hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker, never the thing measured or shipped).
+ *
+ * Each function follows one routine of mercury6_2.for line by line, keeping the
+ * Fortran's left-to-right association and operation order, so that
+ *     gcc -O2 -ffp-contract=off
+ * reproduces the flang-built reference bit for bit except where both call
+ * libm.  File:line citations are to the reference repository.
+ */
+#include "mercury_oracle.h"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* swift.inc:22-31, mercury.inc:37-44 */
+#define DANBYB 1.0e-13
+#define NLAG2 400
+static const double SW_TWOPI = 2.0 * 3.141592653589793;
+static const double HUGEV = 9.9e29;
+
+static double fsign(double a, double b) { return copysign(fabs(a), b); }
+
+/* ------------------------------------------------------------------------ */
+/* mco_sine  mercury6_2.for:2940-2983.  x is modified in place, as in the
+ * Fortran (drift_kepu_guess relies on the reduced value). */
+static void mco_sine_inplace(double *x, double *sx, double *cx)
+{
+    const double pi = 3.141592653589793;
+    const double twopi = 2.0 * pi;
+    if (*x > 0.0)
+        *x = fmod(*x, twopi);
+    else
+        *x = fmod(*x, twopi) + twopi;
+    *cx = cos(*x);
+    if (*x > pi)
+        *sx = -sqrt(1.0 - *cx * *cx);
+    else
+        *sx = sqrt(1.0 - *cx * *cx);
+}
+
+void orc_mco_sine(double x, double *sx, double *cx) { mco_sine_inplace(&x, sx, cx); }
+
+/* drift_kepu_stumpff  mercury6_2.for:8213-8277 */
+static void stumpff(double *xp, double *c0, double *c1, double *c2, double *c3)
+{
+    double x = *xp;
+    int n = 0;
+    const double xm = 0.1;
+    while (fabs(x) >= xm) {
+        n = n + 1;
+        x = x * .25;
+    }
+    double x2 = x * x;
+    double x3 = x * x2;
+    double x4 = x2 * x2;
+    double x5 = x2 * x3;
+    double x6 = x3 * x3;
+    *c2 = 1.147074559772972e-11 * x6 - 2.087675698786810e-9 * x5
+        + 2.755731922398589e-7 * x4 - 2.480158730158730e-5 * x3
+        + 1.388888888888889e-3 * x2 - 4.166666666666667e-2 * x + .5;
+    *c3 = 7.647163731819816e-13 * x6 - 1.605904383682161e-10 * x5
+        + 2.505210838544172e-8 * x4 - 2.755731922398589e-6 * x3
+        + 1.984126984126984e-4 * x2 - 8.333333333333333e-3 * x
+        + 1.666666666666667e-1;
+    *c1 = 1.0 - x * *c3;
+    *c0 = 1.0 - x * *c2;
+    if (n != 0) {
+        for (int i = n; i >= 1; i--) {
+            *c3 = (*c2 + *c0 * *c3) * .25;
+            *c2 = *c1 * *c1 * .5;
+            *c1 = *c0 * *c1;
+            *c0 = 2.0 * *c0 * *c0 - 1.0;
+            x = x * 4.0;
+        }
+    }
+    *xp = x;
+}
+
+void orc_drift_kepu_stumpff(double x, double *c) { stumpff(&x, &c[0], &c[1], &c[2], &c[3]); }
+
+/* drift_kepu_fchk  mercury6_2.for:7876-7907 */
+static double kepu_fchk(double dt, double r0, double mu, double alpha, double u, double s)
+{
+    double c0, c1, c2, c3;
+    double x = s * s * alpha;
+    stumpff(&x, &c0, &c1, &c2, &c3);
+    c1 = c1 * s;
+    c2 = c2 * s * s;
+    c3 = c3 * s * s * s;
+    return r0 * c1 + u * c2 + mu * c3 - dt;
+}
+
+/* drift_kepu_p3solve  mercury6_2.for:8140-8203 */
+static int kepu_p3solve(double dt, double r0, double mu, double alpha, double u, double *s)
+{
+    double denom = (mu - alpha * r0) / 6.0;
+    double a2 = 0.5 * u / denom;
+    double a1 = r0 / denom;
+    double a0 = -dt / denom;
+    double q = (a1 - a2 * a2 / 3.0) / 3.0;
+    /* a2**3d0, q**3d0, r**2d0: the flang build of the reference evaluates a
+     * real power with an integral constant exponent as repeated products */
+    double r = (a1 * a2 - 3.0 * a0) / 6.0 - (a2 * a2 * a2) / 27.0;
+    double sq2 = (q * q * q) + (r * r);
+    if (sq2 >= 0.0) {
+        double sq = sqrt(sq2), p1, p2;
+        if ((r + sq) <= 0.0)
+            p1 = -pow(-(r + sq), 1.0 / 3.0);
+        else
+            p1 = pow(r + sq, 1.0 / 3.0);
+        if ((r - sq) <= 0.0)
+            p2 = -pow(-(r - sq), 1.0 / 3.0);
+        else
+            p2 = pow(r - sq, 1.0 / 3.0);
+        *s = p1 + p2 - a2 / 3.0;
+        return 0;
+    }
+    *s = 0.0;
+    return 1;
+}
+
+/* drift_kepu_guess  mercury6_2.for:7909-7982 */
+static void kepu_guess(double dt, double r0, double mu, double alpha, double u, double *s)
+{
+    if (alpha > 0.0) {
+        if (dt / r0 <= 0.4) {
+            *s = dt / r0 - (dt * dt * u) / (2.0 * r0 * r0 * r0);
+            return;
+        }
+        double a = mu / alpha;
+        double en = sqrt(mu / (a * a * a));
+        double ec = 1.0 - r0 / a;
+        double es = u / (en * a * a);
+        double e = sqrt(ec * ec + es * es);
+        double y = en * dt - es;
+        double sy, cy;
+        mco_sine_inplace(&y, &sy, &cy);
+        double sigma = fsign(1.0, es * cy + ec * sy);
+        double x = y + sigma * .85 * e;
+        *s = x / sqrt(alpha);
+    } else {
+        if (kepu_p3solve(dt, r0, mu, alpha, u, s) != 0)
+            *s = dt / r0;
+    }
+}
+
+/* drift_kepu_new  mercury6_2.for:8073-8138 */
+static int kepu_new(double *s, double dt, double r0, double mu, double alpha, double u,
+                    double *fp, double *c1, double *c2, double *c3)
+{
+    double c0;
+    for (int nc = 0; nc <= 6; nc++) {
+        double s2 = *s * *s;
+        double x = s2 * alpha;
+        stumpff(&x, &c0, c1, c2, c3);
+        *c1 = *c1 * *s;
+        *c2 = *c2 * s2;
+        *c3 = *c3 * *s * s2;
+        double f = r0 * *c1 + u * *c2 + mu * *c3 - dt;
+        *fp = r0 * c0 + u * *c1 + mu * *c2;
+        double fpp = (mu - r0 * alpha) * *c1 + u * c0;
+        double fppp = (mu - r0 * alpha) * c0 - u * alpha * *c1;
+        double ds = -f / *fp;
+        ds = -f / (*fp + .5 * ds * fpp);
+        ds = -f / (*fp + .5 * ds * fpp + ds * ds * fppp * .1666666666666667);
+        *s = *s + ds;
+        double fdt = f / dt;
+        if (fdt * fdt < DANBYB * DANBYB)
+            return 0;
+    }
+    return 1;
+}
+
+/* drift_kepu_lag  mercury6_2.for:7984-8071 */
+static int kepu_lag(double *s, double dt, double r0, double mu, double alpha, double u,
+                    double *fp, double *c1, double *c2, double *c3)
+{
+    const double ln = 5.0;
+    double c0;
+    for (int nc = 0; nc <= NLAG2; nc++) {
+        double x = *s * *s * alpha;
+        stumpff(&x, &c0, c1, c2, c3);
+        *c1 = *c1 * *s;
+        *c2 = *c2 * *s * *s;
+        *c3 = *c3 * *s * *s * *s;
+        double f = r0 * *c1 + u * *c2 + mu * *c3 - dt;
+        *fp = r0 * c0 + u * *c1 + mu * *c2;
+        double fpp = (-40.0 * alpha + mu) * *c1 + u * c0;
+        double ds = -ln * f / (*fp + fsign(1.0, *fp) * sqrt(fabs((ln - 1.0) * (ln - 1.0) * *fp * *fp
+                                                            - (ln - 1.0) * ln * f * fpp)));
+        *s = *s + ds;
+        double fdt = f / dt;
+        if (fdt * fdt < DANBYB * DANBYB)
+            return 0;
+    }
+    return 2;
+}
+
+/* drift_kepu  mercury6_2.for:7830-7874 */
+static int kepu(double dt, double r0, double mu, double alpha, double u,
+                double *fp, double *c1, double *c2, double *c3)
+{
+    double s;
+    kepu_guess(dt, r0, mu, alpha, u, &s);
+    double st = s;
+    int iflg = kepu_new(&s, dt, r0, mu, alpha, u, fp, c1, c2, c3);
+    if (iflg != 0) {
+        double fo = kepu_fchk(dt, r0, mu, alpha, u, st);
+        double fn = kepu_fchk(dt, r0, mu, alpha, u, s);
+        if (fabs(fo) < fabs(fn))
+            s = st;
+        iflg = kepu_lag(&s, dt, r0, mu, alpha, u, fp, c1, c2, c3);
+    }
+    return iflg;
+}
+
+/* drift_kepmd  mercury6_2.for:7754-7808 */
+static void kepmd(double dm, double es, double ec, double *x, double *s, double *c)
+{
+    const double A0 = 39916800.0, A1 = 6652800.0, A2 = 332640.0, A3 = 7920.0, A4 = 110.0;
+    double fac1 = 1.0 / (1.0 - ec);
+    double q = fac1 * dm;
+    double fac2 = es * es * fac1 - ec / 3.0;
+    *x = q * (1.0 - 0.5 * fac1 * q * (es - q * fac2));
+    double y = *x * *x;
+    *s = *x * (A0 - y * (A1 - y * (A2 - y * (A3 - y * (A4 - y))))) / A0;
+    *c = sqrt(1.0 - *s * *s);
+    double f = *x - ec * *s + es * (1.0 - *c) - dm;
+    double fp = 1. - ec * *c + es * *s;
+    double fpp = ec * *s + es * *c;
+    double fppp = ec * *c - es * *s;
+    double dx = -f / fp;
+    dx = -f / (fp + 0.5 * dx * fpp);
+    dx = -f / (fp + 0.5 * dx * fpp + 0.16666666666666666 * dx * dx * fppp);
+    *x = *x + dx;
+    y = *x * *x;
+    *s = *x * (A0 - y * (A1 - y * (A2 - y * (A3 - y * (A4 - y))))) / A0;
+    *c = sqrt(1.0 - *s * *s);
+}
+
+/* drift_dan  mercury6_2.for:7602-7752.  branch_out (optional) reports the
+ * path taken: 0 kepmd, 1 kepu. */
+static int drift_dan_impl(double mu, double *xv, double dt0, int *branch_out)
+{
+    double x0 = xv[0], y0 = xv[1], z0 = xv[2], vx0 = xv[3], vy0 = xv[4], vz0 = xv[5];
+    double dt = dt0;
+    double f, g, fdot, gdot, c1, c2, c3, fp;
+    double r0 = sqrt(x0 * x0 + y0 * y0 + z0 * z0);
+    double v0s = vx0 * vx0 + vy0 * vy0 + vz0 * vz0;
+    double u = x0 * vx0 + y0 * vy0 + z0 * vz0;
+    double alpha = 2.0 * mu / r0 - v0s;
+
+    if (alpha > 0.0) {
+        double a = mu / alpha;
+        double asq = a * a;
+        double en = sqrt(mu / (a * asq));
+        double ec = 1.0 - r0 / a;
+        double es = u / (en * asq);
+        double esq = ec * ec + es * es;
+        double dm = dt * en - (double)((int)(dt * en / SW_TWOPI)) * SW_TWOPI;
+        dt = dm / en;
+        if (!((dm * dm > 0.16) || (esq > 0.36)) && (esq * dm * dm < 0.0016)) {
+            double xkep, s, c;
+            if (branch_out) *branch_out = 0;
+            kepmd(dm, es, ec, &xkep, &s, &c);
+            double fchk = (xkep - ec * s + es * (1.0 - c) - dm);
+            if (fchk * fchk > DANBYB)
+                return 1;
+            fp = 1.0 - ec * c + es * s;
+            f = (a / r0) * (c - 1.0) + 1.0;
+            g = dt + (s - xkep) / en;
+            fdot = -(a / (r0 * fp)) * en * s;
+            gdot = (c - 1.) / fp + 1.;
+            xv[0] = x0 * f + vx0 * g;
+            xv[1] = y0 * f + vy0 * g;
+            xv[2] = z0 * f + vz0 * g;
+            xv[3] = x0 * fdot + vx0 * gdot;
+            xv[4] = y0 * fdot + vy0 * gdot;
+            xv[5] = z0 * fdot + vz0 * gdot;
+            return 0;
+        }
+    }
+    if (branch_out) *branch_out = 1;
+    int iflg = kepu(dt, r0, mu, alpha, u, &fp, &c1, &c2, &c3);
+    if (iflg == 0) {
+        f = 1.0 - (mu / r0) * c2;
+        g = dt - mu * c3;
+        fdot = -(mu / (fp * r0)) * c1;
+        gdot = 1.0 - (mu / fp) * c2;
+        xv[0] = x0 * f + vx0 * g;
+        xv[1] = y0 * f + vy0 * g;
+        xv[2] = z0 * f + vz0 * g;
+        xv[3] = x0 * fdot + vx0 * gdot;
+        xv[4] = y0 * fdot + vy0 * gdot;
+        xv[5] = z0 * fdot + vz0 * gdot;
+    }
+    return iflg;
+}
+
+int orc_drift_dan(double mu, double *xv, double dt) { return drift_dan_impl(mu, xv, dt, NULL); }
+
+/* drift_one  mercury6_2.for:8279-8352 */
+int orc_drift_one(double mu, double *xv, double dt)
+{
+    int iflg = drift_dan_impl(mu, xv, dt, NULL);
+    if (iflg != 0) {
+        for (int i = 1; i <= 10; i++) {
+            double dttmp = dt / 10.0;
+            iflg = drift_dan_impl(mu, xv, dttmp, NULL);
+            if (iflg != 0)
+                return iflg;
+        }
+    }
+    return iflg;
+}
+
+int orc_drift_branch(double mu, const double *xv, double dt)
+{
+    double w[6];
+    memcpy(w, xv, sizeof w);
+    int branch = 0;
+    int iflg = drift_dan_impl(mu, w, dt, &branch);
+    if (iflg != 0) return 3;
+    if (branch == 0) return 0;
+    /* distinguish Newton from Laguerre by re-running Newton alone */
+    double x0 = xv[0], y0 = xv[1], z0 = xv[2], vx0 = xv[3], vy0 = xv[4], vz0 = xv[5];
+    double r0 = sqrt(x0 * x0 + y0 * y0 + z0 * z0);
+    double v0s = vx0 * vx0 + vy0 * vy0 + vz0 * vz0;
+    double u = x0 * vx0 + y0 * vy0 + z0 * vz0;
+    double alpha = 2.0 * mu / r0 - v0s;
+    double dtl = dt;
+    if (alpha > 0.0) {
+        double a = mu / alpha, asq = a * a, en = sqrt(mu / (a * asq));
+        double dm = dtl * en - (double)((int)(dtl * en / SW_TWOPI)) * SW_TWOPI;
+        dtl = dm / en;
+    }
+    double s, fp, c1, c2, c3;
+    kepu_guess(dtl, r0, mu, alpha, u, &s);
+    return kepu_new(&s, dtl, r0, mu, alpha, u, &fp, &c1, &c2, &c3) == 0 ? 1 : 2;
+}
+
+/* ------------------------------------------------------------------------ */
+/* mco_h2j  mercury6_2.for:2613-2687 (called with nbod = nbig by mdt_mvs) */
+void orc_mco_h2j(int nbig, const double *m, const double *xh, const double *vh,
+                 double *x, double *v)
+{
+    if (nbig < 2) return;
+    /* Fortran body index b (1-based) -> array index b-1 */
+    double mtot = m[1];
+    for (int k = 0; k < 3; k++) { x[3 + k] = xh[3 + k]; v[3 + k] = vh[3 + k]; }
+    double mx = m[1] * xh[3], my = m[1] * xh[4], mz = m[1] * xh[5];
+    double mu = m[1] * vh[3], mv = m[1] * vh[4], mw = m[1] * vh[5];
+    for (int j = 3; j <= nbig - 1; j++) {
+        int b = j - 1;
+        double temp = 1.0 / (mtot + m[0]);
+        mtot = mtot + m[b];
+        x[3 * b + 0] = xh[3 * b + 0] - temp * mx;
+        x[3 * b + 1] = xh[3 * b + 1] - temp * my;
+        x[3 * b + 2] = xh[3 * b + 2] - temp * mz;
+        v[3 * b + 0] = vh[3 * b + 0] - temp * mu;
+        v[3 * b + 1] = vh[3 * b + 1] - temp * mv;
+        v[3 * b + 2] = vh[3 * b + 2] - temp * mw;
+        mx = mx + m[b] * xh[3 * b + 0];
+        my = my + m[b] * xh[3 * b + 1];
+        mz = mz + m[b] * xh[3 * b + 2];
+        mu = mu + m[b] * vh[3 * b + 0];
+        mv = mv + m[b] * vh[3 * b + 1];
+        mw = mw + m[b] * vh[3 * b + 2];
+    }
+    if (nbig > 2) {
+        int b = nbig - 1;
+        double temp = 1.0 / (mtot + m[0]);
+        x[3 * b + 0] = xh[3 * b + 0] - temp * mx;
+        x[3 * b + 1] = xh[3 * b + 1] - temp * my;
+        x[3 * b + 2] = xh[3 * b + 2] - temp * mz;
+        v[3 * b + 0] = vh[3 * b + 0] - temp * mu;
+        v[3 * b + 1] = vh[3 * b + 1] - temp * mv;
+        v[3 * b + 2] = vh[3 * b + 2] - temp * mw;
+    }
+}
+
+/* mco_j2h  mercury6_2.for:2700-2770 */
+void orc_mco_j2h(int nbig, const double *m, const double *x, const double *v,
+                 double *xh, double *vh)
+{
+    if (nbig < 2) return;
+    for (int k = 0; k < 3; k++) { xh[3 + k] = x[3 + k]; vh[3 + k] = v[3 + k]; }
+    double mtot = m[1];
+    double temp = m[1] / (mtot + m[0]);
+    double mx = temp * x[3], my = temp * x[4], mz = temp * x[5];
+    double mu = temp * v[3], mv = temp * v[4], mw = temp * v[5];
+    for (int j = 3; j <= nbig - 1; j++) {
+        int b = j - 1;
+        xh[3 * b + 0] = x[3 * b + 0] + mx;
+        xh[3 * b + 1] = x[3 * b + 1] + my;
+        xh[3 * b + 2] = x[3 * b + 2] + mz;
+        vh[3 * b + 0] = v[3 * b + 0] + mu;
+        vh[3 * b + 1] = v[3 * b + 1] + mv;
+        vh[3 * b + 2] = v[3 * b + 2] + mw;
+        mtot = mtot + m[b];
+        temp = m[b] / (mtot + m[0]);
+        mx = mx + temp * x[3 * b + 0];
+        my = my + temp * x[3 * b + 1];
+        mz = mz + temp * x[3 * b + 2];
+        mu = mu + temp * v[3 * b + 0];
+        mv = mv + temp * v[3 * b + 1];
+        mw = mw + temp * v[3 * b + 2];
+    }
+    if (nbig > 2) {
+        int b = nbig - 1;
+        xh[3 * b + 0] = x[3 * b + 0] + mx;
+        xh[3 * b + 1] = x[3 * b + 1] + my;
+        xh[3 * b + 2] = x[3 * b + 2] + mz;
+        vh[3 * b + 0] = v[3 * b + 0] + mu;
+        vh[3 * b + 1] = v[3 * b + 1] + mv;
+        vh[3 * b + 2] = v[3 * b + 2] + mw;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* mfo_obl  mercury6_2.for:5176-5225 */
+void orc_mfo_obl(const double *jcen, int nbod, const double *m, const double *x,
+                 double *a, double *acen)
+{
+    acen[0] = acen[1] = acen[2] = 0.0;
+    for (int i = 1; i < nbod; i++) {
+        const double *xi = x + 3 * i;
+        double r2 = xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2];
+        double r_1 = 1.0 / sqrt(r2);
+        double r_2 = r_1 * r_1;
+        double r_3 = r_2 * r_1;
+        double jr2 = jcen[0] * r_2;
+        double jr4 = jcen[1] * r_2 * r_2;
+        double jr6 = jcen[2] * r_2 * r_2 * r_2;
+        double u2 = xi[2] * xi[2] * r_2;
+        double u4 = u2 * u2;
+        double u6 = u4 * u2;
+        double tmp1 = m[0] * r_3;
+        double tmp2 = jr2 * (7.5 * u2 - 1.5)
+                    + jr4 * (39.375 * u4 - 26.25 * u2 + 1.875)
+                    + jr6 * (187.6875 * u6 - 216.5625 * u4 + 59.0625 * u2 - 2.1875);
+        double tmp3 = jr2 * 3.0 + jr4 * (17.5 * u2 - 7.5)
+                    + jr6 * (86.625 * u4 - 78.75 * u2 + 13.125);
+        a[3 * i + 0] = xi[0] * tmp1 * tmp2;
+        a[3 * i + 1] = xi[1] * tmp1 * tmp2;
+        a[3 * i + 2] = xi[2] * tmp1 * (tmp2 - tmp3);
+        double tmp4 = m[i] / m[0];
+        acen[0] = acen[0] - tmp4 * a[3 * i + 0];
+        acen[1] = acen[1] - tmp4 * a[3 * i + 1];
+        acen[2] = acen[2] - tmp4 * a[3 * i + 2];
+    }
+}
+
+/* mfo_mvs  mercury6_2.for:4929-5074 */
+void orc_mfo_mvs(const double *jcen, int nbod, int nbig, const double *m,
+                 const double *x, const double *xj, double *a)
+{
+    double a0[3] = {0.0, 0.0, 0.0}, a0tp[3] = {0.0, 0.0, 0.0};
+    if (nbod < 2) return;
+    double *a1 = calloc((size_t)3 * nbod, sizeof(double));
+    double *a2 = calloc((size_t)3 * nbod, sizeof(double));
+    double *a3 = calloc((size_t)3 * nbod, sizeof(double));
+    double minside = 0.0;
+    for (int kk = 3; kk <= nbig; kk++) {
+        int k = kk - 1, k1 = kk - 2;
+        minside = minside + m[k1];
+        double r2 = x[3 * k] * x[3 * k] + x[3 * k + 1] * x[3 * k + 1] + x[3 * k + 2] * x[3 * k + 2];
+        double rj2 = xj[3 * k] * xj[3 * k] + xj[3 * k + 1] * xj[3 * k + 1] + xj[3 * k + 2] * xj[3 * k + 2];
+        double r = 1.0 / sqrt(r2);
+        double rj = 1.0 / sqrt(rj2);
+        double r3 = r * r * r;
+        double rj3 = rj * rj * rj;
+        double fac0 = m[k] * r3;
+        double fac12 = m[0] * rj3;
+        double fac2 = m[k] * fac12 / (minside + m[0]);
+        double q = (r2 - rj2) * .5 / rj2;
+        double q2 = q * q, q3 = q * q2, q4 = q2 * q2, q5 = q2 * q3, q6 = q3 * q3, q7 = q3 * q4;
+        double fac1 = 402.1875 * q7 - 187.6875 * q6 + 86.625 * q5
+                    - 39.375 * q4 + 17.5 * q3 - 7.5 * q2 + 3.0 * q - 1.0;
+        for (int c = 0; c < 3; c++) {
+            a0[c] = a0[c] - fac0 * x[3 * k + c];
+            a1[3 * k + c] = fac12 * (xj[3 * k + c] + fac1 * x[3 * k + c]);
+            a2[3 * k + c] = a2[3 * k1 + c] + fac2 * xj[3 * k + c];
+        }
+    }
+    if (nbig >= 2) {
+        double r2 = x[3] * x[3] + x[4] * x[4] + x[5] * x[5];
+        double r = 1.0 / sqrt(r2);
+        double r3 = r * r * r;
+        double fac0 = m[1] * r3;
+        for (int c = 0; c < 3; c++) a0tp[c] = a0[c] - fac0 * x[3 + c];
+    }
+    for (int i = 1; i < nbig; i++) {
+        for (int j = i + 1; j < nbig; j++) {
+            double dx = x[3 * j] - x[3 * i];
+            double dy = x[3 * j + 1] - x[3 * i + 1];
+            double dz = x[3 * j + 2] - x[3 * i + 2];
+            double s2 = dx * dx + dy * dy + dz * dz;
+            double s_1 = 1.0 / sqrt(s2);
+            double s_3 = s_1 * s_1 * s_1;
+            double faci = m[i] * s_3;
+            double facj = m[j] * s_3;
+            a3[3 * j + 0] = a3[3 * j + 0] - faci * dx;
+            a3[3 * j + 1] = a3[3 * j + 1] - faci * dy;
+            a3[3 * j + 2] = a3[3 * j + 2] - faci * dz;
+            a3[3 * i + 0] = a3[3 * i + 0] + facj * dx;
+            a3[3 * i + 1] = a3[3 * i + 1] + facj * dy;
+            a3[3 * i + 2] = a3[3 * i + 2] + facj * dz;
+        }
+        for (int j = nbig; j < nbod; j++) {
+            double dx = x[3 * j] - x[3 * i];
+            double dy = x[3 * j + 1] - x[3 * i + 1];
+            double dz = x[3 * j + 2] - x[3 * i + 2];
+            double s2 = dx * dx + dy * dy + dz * dz;
+            double s_1 = 1.0 / sqrt(s2);
+            double s_3 = s_1 * s_1 * s_1;
+            double faci = m[i] * s_3;
+            a3[3 * j + 0] = a3[3 * j + 0] - faci * dx;
+            a3[3 * j + 1] = a3[3 * j + 1] - faci * dy;
+            a3[3 * j + 2] = a3[3 * j + 2] - faci * dz;
+        }
+    }
+    for (int k = 1; k < nbig; k++)
+        for (int c = 0; c < 3; c++)
+            a[3 * k + c] = a0[c] + a1[3 * k + c] + a2[3 * k + c] + a3[3 * k + c];
+    for (int k = nbig; k < nbod; k++)
+        for (int c = 0; c < 3; c++)
+            a[3 * k + c] = a0tp[c] + a3[3 * k + c];
+    if (jcen[0] != 0 || jcen[1] != 0 || jcen[2] != 0) {
+        double *aobl = calloc((size_t)3 * nbod, sizeof(double));
+        double acen[3];
+        orc_mfo_obl(jcen, nbod, m, x, aobl, acen);
+        for (int k = 1; k < nbod; k++)
+            for (int c = 0; c < 3; c++)
+                a[3 * k + c] = a[3 * k + c] + (aobl[3 * k + c] - acen[c]);
+        free(aobl);
+    }
+    free(a1);
+    free(a2);
+    free(a3);
+}
+
+/* ------------------------------------------------------------------------ */
+/* mce_min  mercury6_2.for:1572-1624 */
+void orc_mce_min(double d0, double d1, double d0t, double d1t, double h,
+                 double *d2min, double *tmin)
+{
+    if (d0t * h > 0.0 || d1t * h < 0.0) {
+        if (d0 <= d1) {
+            *d2min = d0;
+            *tmin = -h;
+        } else {
+            *d2min = d1;
+            *tmin = 0.0;
+        }
+    } else {
+        double temp = 6.0 * (d0 - d1);
+        double a = temp + 3.0 * h * (d0t + d1t);
+        double b = temp + 2.0 * h * (d0t + 2.0 * d1t);
+        double c = h * d1t;
+        temp = -.5 * (b + fsign(sqrt(fmax(b * b - 4.0 * a * c, 0.0)), b));
+        double tau;
+        if (temp == 0.0)
+            tau = 0.0;
+        else
+            tau = c / temp;
+        tau = fmin(tau, 0.0);
+        tau = fmax(tau, -1.0);
+        *tmin = tau * h;
+        temp = 1.0 + tau;
+        *d2min = tau * tau * ((3.0 + 2.0 * tau) * d0 + temp * h * d0t)
+               + temp * temp * ((1.0 - 2.0 * tau) * d1 + tau * h * d1t);
+        *d2min = fmax(*d2min, 0.0);
+    }
+}
+
+/* mce_box  mercury6_2.for:997-1058 */
+void orc_mce_box(int nbod, double h, const double *x0, const double *v0,
+                 const double *x1, const double *v1, double *xmin, double *xmax,
+                 double *ymin, double *ymax)
+{
+    for (int j = 1; j < nbod; j++) {
+        xmin[j] = fmin(x0[3 * j], x1[3 * j]);
+        xmax[j] = fmax(x0[3 * j], x1[3 * j]);
+        ymin[j] = fmin(x0[3 * j + 1], x1[3 * j + 1]);
+        ymax[j] = fmax(x0[3 * j + 1], x1[3 * j + 1]);
+        if ((v0[3 * j] < 0.0 && v1[3 * j] > 0.0) || (v0[3 * j] > 0.0 && v1[3 * j] < 0.0)) {
+            double temp = (v0[3 * j] * x1[3 * j] - v1[3 * j] * x0[3 * j]
+                           - .5 * h * v0[3 * j] * v1[3 * j]) / (v0[3 * j] - v1[3 * j]);
+            xmin[j] = fmin(xmin[j], temp);
+            xmax[j] = fmax(xmax[j], temp);
+        }
+        if ((v0[3 * j + 1] < 0.0 && v1[3 * j + 1] > 0.0) || (v0[3 * j + 1] > 0.0 && v1[3 * j + 1] < 0.0)) {
+            double temp = (v0[3 * j + 1] * x1[3 * j + 1] - v1[3 * j + 1] * x0[3 * j + 1]
+                           - .5 * h * v0[3 * j + 1] * v1[3 * j + 1]) / (v0[3 * j + 1] - v1[3 * j + 1]);
+            ymin[j] = fmin(ymin[j], temp);
+            ymax[j] = fmax(ymax[j], temp);
+        }
+    }
+}
+
+/* mce_stat  mercury6_2.for:1765-1931 (all bodies active: stat = 0) */
+void orc_mce_stat(double time, double h, double rcen, int nbod, int nbig,
+                  const double *m, const double *x0, const double *v0,
+                  const double *x1, const double *v1, const double *rce,
+                  const double *rphys, orc_events *ev)
+{
+    (void)rcen;
+    ev->nhit = 0;
+    ev->thit1 = fsign(HUGEV, h);
+    double hm1 = 1.0 / h;
+    double *xmin = malloc(sizeof(double) * nbod), *xmax = malloc(sizeof(double) * nbod);
+    double *ymin = malloc(sizeof(double) * nbod), *ymax = malloc(sizeof(double) * nbod);
+    orc_mce_box(nbod, h, x0, v0, x1, v1, xmin, xmax, ymin, ymax);
+    for (int j = 1; j < nbod; j++) {
+        double temp = rce[j] * 1.2;
+        xmin[j] = xmin[j] - temp;
+        xmax[j] = xmax[j] + temp;
+        ymin[j] = ymin[j] - temp;
+        ymax[j] = ymax[j] + temp;
+    }
+    for (int i = 1; i < nbig; i++) {
+        for (int j = i + 1; j < nbod; j++) {
+            if (!(xmax[i] >= xmin[j] && xmax[j] >= xmin[i] && ymax[i] >= ymin[j] && ymax[j] >= ymin[i]))
+                continue;
+            double dx0 = x0[3 * i] - x0[3 * j];
+            double dy0 = x0[3 * i + 1] - x0[3 * j + 1];
+            double dz0 = x0[3 * i + 2] - x0[3 * j + 2];
+            double du0 = v0[3 * i] - v0[3 * j];
+            double dv0 = v0[3 * i + 1] - v0[3 * j + 1];
+            double dw0 = v0[3 * i + 2] - v0[3 * j + 2];
+            double d0t = (dx0 * du0 + dy0 * dv0 + dz0 * dw0) * 2.0;
+            double dx1 = x1[3 * i] - x1[3 * j];
+            double dy1 = x1[3 * i + 1] - x1[3 * j + 1];
+            double dz1 = x1[3 * i + 2] - x1[3 * j + 2];
+            double du1 = v1[3 * i] - v1[3 * j];
+            double dv1 = v1[3 * i + 1] - v1[3 * j + 1];
+            double dw1 = v1[3 * i + 2] - v1[3 * j + 2];
+            double d1t = (dx1 * du1 + dy1 * dv1 + dz1 * dw1) * 2.0;
+            double d0 = dx0 * dx0 + dy0 * dy0 + dz0 * dz0;
+            double d1 = dx1 * dx1 + dy1 * dy1 + dz1 * dz1;
+            double d2min, tmin;
+            orc_mce_min(d0, d1, d0t, d1t, h, &d2min, &tmin);
+            double d2ce = fmax(rce[i], rce[j]);
+            double d2hit = rphys[i] + rphys[j];
+            d2ce = d2ce * d2ce;
+            d2hit = d2hit * d2hit;
+            double d2near = d2hit * 4.0;
+            if ((d2min <= d2ce && d0t * h <= 0.0 && d1t * h >= 0.0) || (d2min <= d2hit)) {
+                if (ev->nclo >= ev->cap) {
+                    ev->overflow = 1;
+                } else {
+                    int k = ev->nclo++;
+                    ev->tclo[k] = tmin + time;
+                    ev->dclo[k] = sqrt(fmax(0.0, d2min));
+                    int ic = i, jc = j;
+                    if (m[j] > m[i] && j < nbig) { ic = j; jc = i; }
+                    ev->iclo[k] = ic + 1;
+                    ev->jclo[k] = jc + 1;
+                    double tmp0 = -tmin * hm1;
+                    double tmp1 = 1.0 + tmin * hm1;
+                    for (int c = 0; c < 3; c++) {
+                        ev->ixvclo[6 * k + c] = tmp0 * x0[3 * ic + c] + tmp1 * x1[3 * ic + c];
+                        ev->ixvclo[6 * k + 3 + c] = tmp0 * v0[3 * ic + c] + tmp1 * v1[3 * ic + c];
+                        ev->jxvclo[6 * k + c] = tmp0 * x0[3 * jc + c] + tmp1 * x1[3 * jc + c];
+                        ev->jxvclo[6 * k + 3 + c] = tmp0 * v0[3 * jc + c] + tmp1 * v1[3 * jc + c];
+                    }
+                }
+            }
+            if (d2min <= d2near) {
+                if (ev->nhit >= ev->cap) {
+                    ev->overflow = 1;
+                } else {
+                    int k = ev->nhit++;
+                    int ih = i, jh = j;
+                    ev->thit[k] = tmin + time;
+                    ev->dhit[k] = sqrt(d2min);
+                    ev->chit[k] = -1;
+                    if (d2min <= d2hit) ev->chit[k] = 1;
+                    if (m[jh] > m[ih] && j < nbig) { ih = j; jh = i; }
+                    ev->ihit[k] = ih + 1;
+                    ev->jhit[k] = jh + 1;
+                    if ((tmin - ev->thit1) * h < 0.0) {
+                        ev->thit1 = tmin;
+                        ev->nowflag = 0;
+                        if (d1 <= d2hit) ev->nowflag = 1;
+                    }
+                }
+            }
+        }
+    }
+    free(xmin); free(xmax); free(ymin); free(ymax);
+}
+
+/* mce_cent  mercury6_2.for:1059-1185 (algor != 11 branch) */
+int orc_mce_cent(double time, double h, double rcen, int i0, int nbod,
+                 const double *m, const double *x0, const double *v0,
+                 const double *x1, const double *v1, int cap, int *jhit,
+                 double *thit, double *dhit)
+{
+    if (i0 <= 0) i0 = 2;
+    int nhit = 0;
+    double rcen2 = rcen * rcen;
+    double mcen = m[0];
+    for (int jj = i0; jj <= nbod; jj++) {
+        int j = jj - 1;
+        double rr0 = x0[3 * j] * x0[3 * j] + x0[3 * j + 1] * x0[3 * j + 1] + x0[3 * j + 2] * x0[3 * j + 2];
+        double rr1 = x1[3 * j] * x1[3 * j] + x1[3 * j + 1] * x1[3 * j + 1] + x1[3 * j + 2] * x1[3 * j + 2];
+        double rv0 = v0[3 * j] * x0[3 * j] + v0[3 * j + 1] * x0[3 * j + 1] + v0[3 * j + 2] * x0[3 * j + 2];
+        double rv1 = v1[3 * j] * x1[3 * j] + v1[3 * j + 1] * x1[3 * j + 1] + v1[3 * j + 2] * x1[3 * j + 2];
+        if ((rv0 * h <= 0.0 && rv1 * h >= 0.0) || fmin(rr0, rr1) <= rcen2) {
+            double hx = x0[3 * j + 1] * v0[3 * j + 2] - x0[3 * j + 2] * v0[3 * j + 1];
+            double hy = x0[3 * j + 2] * v0[3 * j] - x0[3 * j] * v0[3 * j + 2];
+            double hz = x0[3 * j] * v0[3 * j + 1] - x0[3 * j + 1] * v0[3 * j];
+            double v2 = v0[3 * j] * v0[3 * j] + v0[3 * j + 1] * v0[3 * j + 1] + v0[3 * j + 2] * v0[3 * j + 2];
+            double h2 = hx * hx + hy * hy + hz * hz;
+            double p = h2 / (mcen + m[j]);
+            double r0 = sqrt(rr0);
+            double temp = 1.0 + p * (v2 / (mcen + m[j]) - 2.0 / r0);
+            double e = sqrt(fmax(temp, 0.0));
+            double q = p / (1.0 + e);
+            if (q <= rcen) {
+                if (nhit >= cap) return -1;
+                int k = nhit++;
+                jhit[k] = jj;
+                dhit[k] = rcen;
+                temp = 0.0;
+                if (e > 0.0 && e < 1.0) {
+                    double a = q / (1.0 - e);
+                    double uhit = fsign(acos(fmax(-1.0, fmin(1.0, (1.0 - rcen / a) / e))), -h);
+                    double u0 = fsign(acos(fmax(-1.0, fmin(1.0, (1.0 - r0 / a) / e))), rv0);
+                    double mhit = uhit - e * sin(uhit);
+                    double m0 = u0 - e * sin(u0);
+                    temp = (mhit - m0) / sqrt((mcen + m[j]) / (a * a * a));
+                } else if (e > 1.0) {
+                    double a = q / (e - 1.0);
+                    double uhit = fsign(acosh(fmax(1.0, (1.0 + rcen / a) / e)), -h);
+                    double u0 = fsign(acosh(fmax(1.0, (1.0 + r0 / a) / e)), rv0);
+                    double mhit = e * sinh(uhit) - uhit;
+                    double m0 = e * sinh(u0) - u0;
+                    temp = (mhit - m0) / sqrt((mcen + m[j]) / (a * a * a));
+                } else if (e == 1.0 && q > 0.0) {
+                    double uhit = fsign(sqrt(fmax(0.0, rcen / q - 1.0)), -h);
+                    double u0 = fsign(sqrt(fmax(0.0, r0 / q - 1.0)), rv0);
+                    temp = sqrt(2.0 * (q * q * q) / (mcen + m[j]))
+                         * (uhit + (uhit * uhit * uhit) / 3.0 - u0 - (u0 * u0 * u0) / 3.0);
+                }
+                if (q == 0.0 && rr1 != rr0)
+                    temp = h * fmax(0.0, fmin(1.0, (r0 - rcen) / (r0 - sqrt(rr1))));
+                thit[k] = time - h + temp;
+            }
+        }
+    }
+    return nhit;
+}
+
+/* ------------------------------------------------------------------------ */
+/* mdt_mvs  mercury6_2.for:3987-4097 (no user or non-gravitational forces) */
+void orc_mdt_mvs(double time, double h0, double rcen, const double *jcen,
+                 int nbod, int nbig, const double *m, double *x, double *v,
+                 double *a, int *dtflag, const double *rce, const double *rphys,
+                 orc_events *ev)
+{
+    double hby2 = .5 * h0;
+    size_t n3 = (size_t)3 * nbod;
+    double *xj = calloc(n3, sizeof(double)), *vj = calloc(n3, sizeof(double));
+    double *x0 = malloc(n3 * sizeof(double)), *v0 = malloc(n3 * sizeof(double));
+    double *gm = calloc((size_t)nbod, sizeof(double));
+    ev->nclo = 0;
+    ev->overflow = 0;
+    if (*dtflag != 2) {
+        *dtflag = 2;
+        orc_mco_h2j(nbig, m, x, v, xj, vj);
+        orc_mfo_mvs(jcen, nbod, nbig, m, x, xj, a);
+    }
+    double minside = m[0];
+    for (int j = 1; j < nbig; j++) {
+        double msofar = minside + m[j];
+        gm[j] = m[0] * msofar / minside;
+        minside = msofar;
+    }
+    for (int j = 1; j < nbod; j++) {
+        v[3 * j + 0] = v[3 * j + 0] + hby2 * (0.0 + 0.0 + a[3 * j + 0]);
+        v[3 * j + 1] = v[3 * j + 1] + hby2 * (0.0 + 0.0 + a[3 * j + 1]);
+        v[3 * j + 2] = v[3 * j + 2] + hby2 * (0.0 + 0.0 + a[3 * j + 2]);
+    }
+    memcpy(x0, x, n3 * sizeof(double));
+    memcpy(v0, v, n3 * sizeof(double));
+    orc_mco_h2j(nbig, m, x, v, xj, vj);
+    for (int j = 1; j < nbig; j++) {
+        double xv[6] = {xj[3 * j], xj[3 * j + 1], xj[3 * j + 2], vj[3 * j], vj[3 * j + 1], vj[3 * j + 2]};
+        orc_drift_one(gm[j], xv, h0);
+        for (int c = 0; c < 3; c++) { xj[3 * j + c] = xv[c]; vj[3 * j + c] = xv[3 + c]; }
+    }
+    for (int j = nbig; j < nbod; j++) {
+        double xv[6] = {x[3 * j], x[3 * j + 1], x[3 * j + 2], v[3 * j], v[3 * j + 1], v[3 * j + 2]};
+        orc_drift_one(m[0], xv, h0);
+        for (int c = 0; c < 3; c++) { x[3 * j + c] = xv[c]; v[3 * j + c] = xv[3 + c]; }
+    }
+    orc_mco_j2h(nbig, m, xj, vj, x, v);
+    if (ev)
+        orc_mce_stat(time + h0, h0, rcen, nbod, nbig, m, x0, v0, x, v, rce, rphys, ev);
+    orc_mfo_mvs(jcen, nbod, nbig, m, x, xj, a);
+    for (int j = 1; j < nbod; j++) {
+        v[3 * j + 0] = v[3 * j + 0] + hby2 * (0.0 + 0.0 + a[3 * j + 0]);
+        v[3 * j + 1] = v[3 * j + 1] + hby2 * (0.0 + 0.0 + a[3 * j + 1]);
+        v[3 * j + 2] = v[3 * j + 2] + hby2 * (0.0 + 0.0 + a[3 * j + 2]);
+    }
+    free(xj); free(vj); free(x0); free(v0); free(gm);
+}
+
+/* mco_mvs2h  mercury6_2.for:2107-2219 */
+void orc_mco_mvs2h(double h, const double *jcen, int nbod, int nbig,
+                   const double *m, const double *x, const double *v,
+                   double *xh, double *vh)
+{
+    size_t n3 = (size_t)3 * nbod;
+    double *xj = calloc(n3, sizeof(double)), *vj = calloc(n3, sizeof(double));
+    double *a = calloc(n3, sizeof(double)), *gm = calloc((size_t)nbod, sizeof(double));
+    double rt10 = sqrt(10.0);
+    double ha[2] = {h * rt10 * 3.0 / 10.0, h * rt10 / 5.0};
+    double hb[2] = {-h * rt10 / 72.0, h * rt10 / 24.0};
+    memcpy(xh, x, n3 * sizeof(double));
+    memcpy(vh, v, n3 * sizeof(double));
+    double minside = m[0];
+    for (int j = 1; j < nbig; j++) {
+        double msofar = minside + m[j];
+        gm[j] = m[0] * msofar / minside;
+        minside = msofar;
+    }
+    for (int k = 0; k < 2; k++) {
+        double dts[3] = {ha[k], -2.0 * ha[k], ha[k]};
+        double kick[2] = {-hb[k], hb[k]};
+        for (int phase = 0; phase < 3; phase++) {
+            orc_mco_h2j(nbig, m, xh, vh, xj, vj);
+            for (int j = 1; j < nbig; j++) {
+                double xv[6] = {xj[3 * j], xj[3 * j + 1], xj[3 * j + 2], vj[3 * j], vj[3 * j + 1], vj[3 * j + 2]};
+                orc_drift_one(gm[j], xv, dts[phase]);
+                for (int c = 0; c < 3; c++) { xj[3 * j + c] = xv[c]; vj[3 * j + c] = xv[3 + c]; }
+            }
+            for (int j = nbig; j < nbod; j++) {
+                double xv[6] = {xh[3 * j], xh[3 * j + 1], xh[3 * j + 2], vh[3 * j], vh[3 * j + 1], vh[3 * j + 2]};
+                orc_drift_one(m[0], xv, dts[phase]);
+                for (int c = 0; c < 3; c++) { xh[3 * j + c] = xv[c]; vh[3 * j + c] = xv[3 + c]; }
+            }
+            orc_mco_j2h(nbig, m, xj, vj, xh, vh);
+            if (phase == 2) break;
+            orc_mfo_mvs(jcen, nbod, nbig, m, xh, xj, a);
+            /* vh - hb*(...) for the first kick, vh + hb*(...) for the second */
+            for (int j = 1; j < nbod; j++)
+                for (int c = 0; c < 3; c++) {
+                    if (phase == 0)
+                        vh[3 * j + c] = vh[3 * j + c] - hb[k] * (0.0 + 0.0 + a[3 * j + c]);
+                    else
+                        vh[3 * j + c] = vh[3 * j + c] + hb[k] * (0.0 + 0.0 + a[3 * j + c]);
+                }
+            (void)kick;
+        }
+    }
+    free(xj); free(vj); free(a); free(gm);
+}
+
+/* ------------------------------------------------------------------------ */
+int orc_mvs_run(double time0, double h0, double rcen, const double *jcen,
+                int nbod, int nbig, const double *m, double *x, double *v,
+                const double *rce, const double *rphys, int nsteps,
+                int screen, int cap, int *evi, double *evd)
+{
+    size_t n3 = (size_t)3 * nbod;
+    double *a = calloc(n3, sizeof(double));
+    double *xs = malloc(n3 * sizeof(double)), *vs = malloc(n3 * sizeof(double));
+    int lcap = 4096;
+    orc_events ev;
+    memset(&ev, 0, sizeof ev);
+    ev.cap = lcap;
+    ev.iclo = malloc(sizeof(int) * lcap); ev.jclo = malloc(sizeof(int) * lcap);
+    ev.dclo = malloc(sizeof(double) * lcap); ev.tclo = malloc(sizeof(double) * lcap);
+    ev.ixvclo = malloc(sizeof(double) * 6 * lcap); ev.jxvclo = malloc(sizeof(double) * 6 * lcap);
+    ev.ihit = malloc(sizeof(int) * lcap); ev.jhit = malloc(sizeof(int) * lcap);
+    ev.chit = malloc(sizeof(int) * lcap); ev.dhit = malloc(sizeof(double) * lcap);
+    ev.thit = malloc(sizeof(double) * lcap);
+    int *cj = malloc(sizeof(int) * nbod);
+    double *ct = malloc(sizeof(double) * nbod), *cd = malloc(sizeof(double) * nbod);
+    int dtflag = 0, nev = 0, bad = 0;
+    double time = time0;
+    for (int s = 0; s < nsteps; s++) {
+        memcpy(xs, x, n3 * sizeof(double));
+        memcpy(vs, v, n3 * sizeof(double));
+        orc_mdt_mvs(time, h0, rcen, jcen, nbod, nbig, m, x, v, a, &dtflag, rce, rphys,
+                    screen ? &ev : NULL);
+        time = time + h0;
+        if (!screen) continue;
+        if (ev.overflow) bad = 1;
+        for (int k = 0; k < ev.nclo; k++) {
+            if (nev >= cap) { bad = 1; break; }
+            evi[4 * nev] = s; evi[4 * nev + 1] = 1;
+            evi[4 * nev + 2] = ev.iclo[k]; evi[4 * nev + 3] = ev.jclo[k];
+            evd[2 * nev] = ev.dclo[k]; evd[2 * nev + 1] = ev.tclo[k];
+            nev++;
+        }
+        for (int k = 0; k < ev.nhit; k++) {
+            if (nev >= cap) { bad = 1; break; }
+            evi[4 * nev] = s; evi[4 * nev + 1] = ev.chit[k] > 0 ? 6 : 2;
+            evi[4 * nev + 2] = ev.ihit[k]; evi[4 * nev + 3] = ev.jhit[k];
+            evd[2 * nev] = ev.dhit[k]; evd[2 * nev + 1] = ev.thit[k];
+            nev++;
+        }
+        int nc = orc_mce_cent(time, h0, rcen, 2, nbod, m, xs, vs, x, v, nbod, cj, ct, cd);
+        for (int k = 0; k < nc; k++) {
+            if (nev >= cap) { bad = 1; break; }
+            evi[4 * nev] = s; evi[4 * nev + 1] = 8;
+            evi[4 * nev + 2] = 1; evi[4 * nev + 3] = cj[k];
+            evd[2 * nev] = cd[k]; evd[2 * nev + 1] = ct[k];
+            nev++;
+        }
+    }
+    free(a); free(xs); free(vs); free(cj); free(ct); free(cd);
+    free(ev.iclo); free(ev.jclo); free(ev.dclo); free(ev.tclo); free(ev.ixvclo); free(ev.jxvclo);
+    free(ev.ihit); free(ev.jhit); free(ev.chit); free(ev.dhit); free(ev.thit);
+    return bad ? -1 : nev;
+}

@@ -1,0 +1,102 @@
+/*
+ * mercury_oracle.h -- CPU restatement of MERCURY6's massless-body hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP path in
+ * mercuda_amd/ is compared against.  Only tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg may load it; the product never links or calls it.
+ *
+ * Every function restates one Fortran routine of mercury6_2.for (cited per
+ * function in mercury_oracle.c) in the same floating-point operation order, so
+ * that a build with -ffp-contract=off reproduces the reference bit for bit
+ * except where the reference calls libm (cos, pow, fmod).
+ *
+ * Array conventions follow the Fortran: x[3*j+k] is component k of body j,
+ * body 0 is the central body, bodies 1..nbig-1 are Big, nbig..nbod-1 Small.
+ * Masses are G*M (AU^3/day^2).  Indices returned in event lists are 1-based,
+ * as in the Fortran.
+ */
+#ifndef MERCURY_ORACLE_H
+#define MERCURY_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Kepler drift chain: drift_one -> drift_dan -> drift_kepmd / drift_kepu. */
+int orc_drift_one(double mu, double *xv, double dt);          /* xv[6] in/out, returns iflg */
+int orc_drift_dan(double mu, double *xv, double dt);
+void orc_drift_kepu_stumpff(double x, double *c);              /* c[4] = c0..c3 */
+void orc_mco_sine(double x, double *sx, double *cx);
+/* Which branch drift_dan takes for this state: 0 kepmd, 1 kepu (Newton), 2
+ * kepu (Laguerre), 3 failed (iflg != 0).  Used only to label test vectors. */
+int orc_drift_branch(double mu, const double *xv, double dt);
+
+/* Coordinate transforms (Big bodies only, as mdt_mvs calls them). */
+void orc_mco_h2j(int nbig, const double *m, const double *xh, const double *vh,
+                 double *xj, double *vj);
+void orc_mco_j2h(int nbig, const double *m, const double *xj, const double *vj,
+                 double *xh, double *vh);
+
+/* Forces. */
+void orc_mfo_obl(const double *jcen, int nbod, const double *m, const double *x,
+                 double *a, double *acen);
+void orc_mfo_mvs(const double *jcen, int nbod, int nbig, const double *m,
+                 const double *x, const double *xj, double *a);
+
+/* Encounter screening. */
+void orc_mce_min(double d0, double d1, double d0t, double d1t, double h,
+                 double *d2min, double *tmin);
+void orc_mce_box(int nbod, double h, const double *x0, const double *v0,
+                 const double *x1, const double *v1, double *xmin, double *xmax,
+                 double *ymin, double *ymax);
+
+typedef struct {
+    int cap;                       /* capacity of every list below */
+    int nclo;                      /* close-encounter minima */
+    int *iclo, *jclo;
+    double *dclo, *tclo, *ixvclo, *jxvclo;   /* ixvclo[6*k+c] */
+    int nhit;                      /* near misses / collisions */
+    int *ihit, *jhit, *chit;
+    double *dhit, *thit;
+    double thit1;
+    int nowflag;
+    int overflow;                  /* set when a list would exceed cap */
+} orc_events;
+
+void orc_mce_stat(double time, double h, double rcen, int nbod, int nbig,
+                  const double *m, const double *x0, const double *v0,
+                  const double *x1, const double *v1, const double *rce,
+                  const double *rphys, orc_events *ev);
+/* Central-body impacts over bodies i0..nbod (1-based i0, as the Fortran). */
+int orc_mce_cent(double time, double h, double rcen, int i0, int nbod,
+                 const double *m, const double *x0, const double *v0,
+                 const double *x1, const double *v1, int cap, int *jhit,
+                 double *thit, double *dhit);
+
+/* One MVS step of the whole system (mdt_mvs).  a[3*nbod] carries the
+ * accelerations between calls; *dtflag != 2 recomputes them first. */
+void orc_mdt_mvs(double time, double h0, double rcen, const double *jcen,
+                 int nbod, int nbig, const double *m, double *x, double *v,
+                 double *a, int *dtflag, const double *rce, const double *rphys,
+                 orc_events *ev);
+
+/* Symplectic corrector mco_mvs2h: symplectic (x,v) -> physical (xh,vh). */
+void orc_mco_mvs2h(double h, const double *jcen, int nbod, int nbig,
+                   const double *m, const double *x, const double *v,
+                   double *xh, double *vh);
+
+/* Convenience for tests and the CPU baseline: nsteps MVS steps of a whole
+ * system with the driver's central-impact check after each, event lists
+ * flattened into caller arrays.  Returns the number of events written
+ * (or -1 if cap was exceeded).  Each event row: step, kind, i, j, and
+ * (distance, time) in evd[2*k]. */
+int orc_mvs_run(double time0, double h0, double rcen, const double *jcen,
+                int nbod, int nbig, const double *m, double *x, double *v,
+                const double *rce, const double *rphys, int nsteps,
+                int screen, int cap, int *evi, double *evd);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,122 @@
+"""Write the golden vectors under tests/golden/ from the reference's own CPU
+routines (oracle/_ref/libmercury_ref.so, compiled from mercury6_2.for by
+oracle/Makefile).  Run here, where /root/reference exists:
+
+    make -C oracle ref && python tests/golden/make_golden.py
+
+Each fixture holds inputs and the reference's outputs only (data, no code).
+"""
+from __future__ import annotations
+
+import sys
+import threading
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+ROOT = HERE.parents[1]
+sys.path.insert(0, str(ROOT))
+
+from mercuda_amd import systems  # noqa: E402
+from oracle.bindings import Reference  # noqa: E402
+
+MU = systems.K2
+
+
+def drift_cases(rng, k):
+    """Random states spanning every drift_dan branch: kepmd, kepu Newton and
+    Laguerre, elliptic and hyperbolic guesses, p3solve, retry path."""
+    r = 10 ** rng.uniform(-1.5, 1.5, k)
+    vc = np.sqrt(MU / r)
+    f = rng.choice([0.1, 0.5, 0.9, 1.0, 1.2, 1.41, 1.4142135623730951, 1.5, 2.0, 3.0], k) \
+        * rng.uniform(0.95, 1.05, k)
+    xd = rng.normal(size=(k, 3)); xd /= np.linalg.norm(xd, axis=1)[:, None]
+    vd = rng.normal(size=(k, 3)); vd /= np.linalg.norm(vd, axis=1)[:, None]
+    xv = np.concatenate([xd * r[:, None], vd * (f * vc)[:, None]], axis=1)
+    dt = 10 ** rng.uniform(-1, 3, k) * rng.choice([-1.0, 1.0], k)
+    mu = np.full(k, MU) * rng.choice([1.0, 1.0 + 1e-3, 0.5], k)
+    return mu, xv, dt
+
+
+def main():
+    R = Reference()
+    rng = np.random.default_rng(20261203)
+
+    # drift_one
+    mu, xv, dt = drift_cases(rng, 1500)
+    out = np.zeros_like(xv); flag = np.zeros(len(dt), np.int32)
+    for q in range(len(dt)):
+        out[q], flag[q] = R.drift_one(mu[q], xv[q], dt[q])
+    np.savez_compressed(HERE / "drift_one.npz", mu=mu, xv=xv, dt=dt, out=out, flag=flag)
+
+    # stumpff and mco_sine
+    xs = rng.uniform(-80, 80, 400) * 10 ** rng.uniform(-4, 0, 400)
+    st = np.array([R.stumpff(x) for x in xs])
+    ang = rng.uniform(-40, 40, 400)
+    sc = np.array([R.mco_sine(a) for a in ang])
+    np.savez_compressed(HERE / "stumpff_sine.npz", x=xs, c=st, ang=ang, sc=sc)
+
+    # mfo_mvs on random systems (Big + Small)
+    cases = {}
+    for c, (npl, npart, jc) in enumerate([(8, 40, (0, 0, 0)), (3, 10, (0, 0, 0)), (1, 5, (0, 0, 0)),
+                                          (12, 30, (0, 0, 0)), (5, 20, (1e-6, -2e-9, 3e-12))]):
+        s = systems.make_system(100 + c, npart, nplanets=npl)
+        a = R.mfo_mvs(s["m"], s["x"], s["nbig"], jcen=jc)
+        cases[f"m{c}"] = s["m"]; cases[f"x{c}"] = s["x"]; cases[f"nbig{c}"] = s["nbig"]
+        cases[f"jcen{c}"] = np.array(jc, float); cases[f"a{c}"] = a
+    np.savez_compressed(HERE / "mfo_mvs.npz", **cases)
+
+    # mce_min
+    k = 600
+    d0 = 10 ** rng.uniform(-6, 1, k); d1 = 10 ** rng.uniform(-6, 1, k)
+    d0t = rng.normal(size=k) * 10 ** rng.uniform(-6, 0, k)
+    d1t = rng.normal(size=k) * 10 ** rng.uniform(-6, 0, k)
+    h = rng.choice([-8.0, 2.0, 4.0, 8.0], k)
+    mm = np.array([R.mce_min(d0[q], d1[q], d0t[q], d1t[q], h[q]) for q in range(k)])
+    np.savez_compressed(HERE / "mce_min.npz", d0=d0, d1=d1, d0t=d0t, d1t=d1t, h=h, out=mm)
+
+    # whole MVS runs: mdt_mvs + the driver's mce_cent, encounter-rich and plain
+    runs = {}
+    for c, (seed, npart, nsteps, near) in enumerate([(1, 200, 456, 0.0), (3, 300, 300, 0.5)]):
+        if near:
+            s = systems.make_system(seed, npart, a_range=(0.5, 40.0), emax=0.3, imax_deg=20.0, h=2.0,
+                                    near_fraction=near)
+            s["rce"][1:s["nbig"]] *= 3.0
+        else:
+            s = systems.config_small(seed, npart)
+        x, v, evi, evd = R.mdt_mvs_run(s, nsteps)
+        for key in ("m", "x", "v", "rce", "rphys"):
+            runs[f"{key}{c}"] = s[key]
+        runs[f"meta{c}"] = np.array([s["nbig"], nsteps, s["h"], s["rcen"]], float)
+        runs[f"xout{c}"] = x; runs[f"vout{c}"] = v; runs[f"evi{c}"] = evi; runs[f"evd{c}"] = evd
+    np.savez_compressed(HERE / "mvs_runs.npz", **runs)
+
+    # central impacts: plunging particles through rcen
+    s = systems.config_small(7, 0)
+    k = 200
+    r0 = rng.uniform(0.05, 0.5, k)
+    q = rng.uniform(0.0, 0.012, k)
+    x0 = np.zeros((k, 3)); v0 = np.zeros((k, 3))
+    x0[:, 0] = r0
+    # velocity for pericentre q from r0: vt = sqrt(2 mu q / (r0 (r0+q))) * sqrt(r0/q)... use energy/AM
+    vt = np.sqrt(2 * MU * q / (r0 * (r0 + q)))
+    vr = -np.sqrt(np.maximum(2 * MU / r0 - vt ** 2 - 2 * MU / (r0 + q) * 0, 0) * 0.9)
+    v0[:, 0] = vr; v0[:, 1] = vt
+    m = np.concatenate([s["m"], np.zeros(k)])
+    X0 = np.concatenate([s["x"], x0]); V0 = np.concatenate([s["v"], v0])
+    X1 = X0.copy(); V1 = V0.copy()
+    for j in range(s["nbig"], len(m)):
+        out1, _ = R.drift_one(MU, np.concatenate([X0[j], V0[j]]), 4.0)
+        X1[j], V1[j] = out1[:3], out1[3:]
+    hits = R.mce_cent(4.0, 4.0, 0.005, m, X0, V0, X1, V1)
+    np.savez_compressed(HERE / "mce_cent.npz", m=m, x0=X0, v0=V0, x1=X1, v1=V1,
+                        hits=np.array(hits, float).reshape(-1, 3))
+    print("wrote", sorted(p.name for p in HERE.glob("*.npz")))
+
+
+if __name__ == "__main__":
+    threading.stack_size(512 << 20)
+    t = threading.Thread(target=main)
+    t.start()
+    t.join()

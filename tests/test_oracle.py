@@ -1,0 +1,298 @@
+"""The oracle (oracle/mercury_oracle.c) pinned three ways:
+
+1. against the golden vectors written by tests/golden/make_golden.py from the
+   reference's own compiled routines (bitwise where no libm call is involved);
+2. against the live reference build oracle/_ref where it is present;
+3. against what the mathematics fixes, independently of both: exact two-body
+   propagation (mpmath, 40 digits), time reversal, the energy and angular
+   momentum integrals, closed-form encounter minima and impact times.
+"""
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from conftest import run_big_stack
+from mercuda_amd import systems
+
+GOLD = Path(__file__).resolve().parent / "golden"
+MU = systems.K2
+
+
+def ulps(a, b):
+    a = np.asarray(a, np.float64); b = np.asarray(b, np.float64)
+    scale = np.maximum(np.abs(b), 1e-300)
+    return np.max(np.abs(a - b) / (scale * 2.0 ** -52)) if a.size else 0.0
+
+
+# ---------------------------------------------------------------- golden vectors
+def test_golden_drift_one(oracle):
+    g = np.load(GOLD / "drift_one.npz")
+    bad = 0
+    for q in range(len(g["dt"])):
+        out, flag = oracle.drift_one(g["mu"][q], g["xv"][q], g["dt"][q])
+        assert flag == g["flag"][q]
+        bad += not np.array_equal(out, g["out"][q])
+    assert bad == 0, f"{bad} drift vectors differ from the reference"
+
+
+def test_golden_stumpff_and_sine(oracle):
+    g = np.load(GOLD / "stumpff_sine.npz")
+    for x, c in zip(g["x"], g["c"]):
+        assert np.array_equal(oracle.stumpff(x), c)
+    for a, sc in zip(g["ang"], g["sc"]):
+        assert ulps(oracle.mco_sine(a), sc) <= 1.0   # libm cos on both sides
+
+
+def test_golden_mfo_mvs(oracle):
+    g = np.load(GOLD / "mfo_mvs.npz")
+    c = 0
+    while f"m{c}" in g:
+        a = oracle.mfo_mvs(g[f"m{c}"], g[f"x{c}"], int(g[f"nbig{c}"]), jcen=g[f"jcen{c}"])
+        assert np.array_equal(a[1:], g[f"a{c}"][1:]), f"case {c}"
+        c += 1
+    assert c == 5
+
+
+def test_golden_mce_min(oracle):
+    g = np.load(GOLD / "mce_min.npz")
+    for q in range(len(g["h"])):
+        out = oracle.mce_min(g["d0"][q], g["d1"][q], g["d0t"][q], g["d1t"][q], g["h"][q])
+        assert np.array_equal(out, g["out"][q])
+
+
+def test_golden_mce_cent(oracle):
+    g = np.load(GOLD / "mce_cent.npz")
+    hits = oracle.mce_cent(4.0, 4.0, 0.005, g["m"], g["x0"], g["v0"], g["x1"], g["v1"])
+    want = g["hits"]
+    assert len(hits) == len(want) > 10
+    for (j, t, d), w in zip(hits, want):
+        assert j == int(w[0]) and d == w[2]
+        assert abs(t - w[1]) <= 1e-13 * max(1.0, abs(w[1]))   # libm acos/sin/sinh
+
+
+def test_golden_mvs_runs(oracle):
+    g = np.load(GOLD / "mvs_runs.npz")
+    for c in range(2):
+        nbig, nsteps, h, rcen = g[f"meta{c}"]
+        s = dict(m=g[f"m{c}"], x=g[f"x{c}"], v=g[f"v{c}"], rce=g[f"rce{c}"], rphys=g[f"rphys{c}"],
+                 nbig=int(nbig), h=h, rcen=rcen, jcen=np.zeros(3))
+        x, v, evi, evd = oracle.mvs_run(s, int(nsteps))
+        assert np.array_equal(x, g[f"xout{c}"]) and np.array_equal(v, g[f"vout{c}"]), f"run {c}"
+        mine = [tuple(r) for r in evi if r[1] in (1, 8)]
+        assert mine == [tuple(r) for r in g[f"evi{c}"]]
+
+
+# ---------------------------------------------------------------- live reference
+def test_reference_drift_random(oracle, reference):
+    rng = np.random.default_rng(99)
+    from golden.make_golden import drift_cases
+    mu, xv, dt = drift_cases(rng, 3000)
+    branches = np.zeros(4, int)
+    for q in range(len(dt)):
+        a, fa = oracle.drift_one(mu[q], xv[q], dt[q])
+        b, fb = reference.drift_one(mu[q], xv[q], dt[q])
+        assert fa == fb and np.array_equal(a, b), (mu[q], xv[q].tolist(), dt[q])
+        branches[oracle.drift_branch(mu[q], xv[q], dt[q])] += 1
+    assert branches[0] > 100 and branches[1] > 100 and branches[2] > 0
+
+
+def test_reference_mvs_run(oracle, reference):
+    s = systems.config_small(seed=4, nparticles=300)
+    x, v, evi, evd = oracle.mvs_run(s, 300)
+    xr, vr, ri, rd = run_big_stack(reference.mdt_mvs_run, s, 300)
+    assert np.array_equal(x, xr) and np.array_equal(v, vr)
+    assert [tuple(r) for r in evi if r[1] in (1, 8)] == [tuple(r) for r in ri]
+
+
+def test_reference_mfo_mvs_many(oracle, reference):
+    for seed in range(6):
+        s = systems.make_system(200 + seed, 25, nplanets=1 + 2 * seed)
+        assert np.array_equal(oracle.mfo_mvs(s["m"], s["x"], s["nbig"])[1:],
+                              reference.mfo_mvs(s["m"], s["x"], s["nbig"])[1:])
+
+
+# ---------------------------------------------------------------- mathematics
+def _exact_two_body(mu, xv, dt, dps=40):
+    """Universal-variable propagation at dps digits (mpmath): solve
+    r0*c1(s) + u*c2(s) + mu*c3(s) = dt by Newton, then the f, g functions."""
+    import mpmath as mp
+    mp.mp.dps = dps
+    x = [mp.mpf(float(q)) for q in xv[:3]]
+    v = [mp.mpf(float(q)) for q in xv[3:]]
+    mu = mp.mpf(float(mu)); dt = mp.mpf(float(dt))
+    r0 = mp.sqrt(sum(q * q for q in x))
+    u = sum(a * b for a, b in zip(x, v))
+    alpha = 2 * mu / r0 - sum(q * q for q in v)
+
+    def stumpff(z):
+        # c_k(z) = sum (-z)^n / (2n+k)!
+        cs = []
+        for k in range(4):
+            term = mp.mpf(1) / mp.factorial(k)
+            tot = term
+            n = 0
+            while abs(term) > mp.mpf(10) ** (-dps - 5) * max(1, abs(tot)):
+                n += 1
+                term = term * (-z) / ((2 * n + k) * (2 * n + k - 1))
+                tot += term
+            cs.append(tot)
+        return cs
+
+    s = dt / r0
+    for _ in range(200):
+        c0, c1, c2, c3 = stumpff(s * s * alpha)
+        f = r0 * s * c1 + u * s * s * c2 + mu * s ** 3 * c3 - dt
+        fp = r0 * c0 + u * s * c1 + mu * s * s * c2
+        ds = -f / fp
+        s += ds
+        if abs(ds) < mp.mpf(10) ** (-dps + 5) * max(1, abs(s)):
+            break
+    c0, c1, c2, c3 = stumpff(s * s * alpha)
+    c1 *= s; c2 *= s * s; c3 *= s ** 3
+    r = r0 * c0 + u * c1 + mu * c2
+    f = 1 - mu / r0 * c2
+    g = dt - mu * c3
+    fd = -mu / (r * r0) * c1
+    gd = 1 - mu / r * c2
+    out = [f * x[k] + g * v[k] for k in range(3)] + [fd * x[k] + gd * v[k] for k in range(3)]
+    return np.array([float(q) for q in out])
+
+
+@pytest.mark.parametrize("factor", [0.3, 0.7, 0.99, 1.0, 1.2, 3.0])
+def test_drift_exact_two_body(oracle, factor):
+    """Elliptic, near-parabolic, parabolic and hyperbolic drifts against the
+    exact solution (the universal-variable closed form, as the reference's
+    tests/validation/oracle.py:18-54 uses)."""
+    rng = np.random.default_rng(int(factor * 1000))
+    for _ in range(12):
+        r = 10 ** rng.uniform(-0.5, 1.0)
+        vesc = np.sqrt(2 * MU / r)
+        xd = rng.normal(size=3); xd /= np.linalg.norm(xd)
+        vd = rng.normal(size=3); vd /= np.linalg.norm(vd)
+        xv = np.concatenate([xd * r, vd * vesc * factor / np.sqrt(2) * np.sqrt(2)])
+        dt = rng.uniform(0.5, 40.0) * rng.choice([-1, 1])
+        got, flag = oracle.drift_one(MU, xv, dt)
+        want = _exact_two_body(MU, xv, dt)
+        assert flag == 0
+        scale = np.concatenate([np.full(3, np.linalg.norm(want[:3])), np.full(3, np.linalg.norm(want[3:]))])
+        assert np.max(np.abs(got - want) / scale) < 1e-12, (factor, xv.tolist(), dt)
+
+
+def test_drift_circular_and_barker(oracle):
+    """Circular orbit: angle sqrt(mu/r^3) t exactly (reference test_release.py:35-48).
+    Parabola: Barker's equation (reference test_validation_guards.py:112-125)."""
+    r = 1.3
+    n = np.sqrt(MU / r ** 3)
+    for t in (1.0, 17.0, 365.25, -200.0):
+        out, _ = oracle.drift_one(MU, np.array([r, 0, 0, 0, r * n, 0.0]), t)
+        assert np.allclose(out[:3], [r * np.cos(n * t), r * np.sin(n * t), 0.0], rtol=0, atol=1e-12 * r)
+    # parabola with pericentre q at t=0: tan(f/2) = D, t = sqrt(2 q^3/mu) (D + D^3/3)
+    q = 0.7
+    vp = np.sqrt(2 * MU / q)
+    for t in (5.0, 50.0, 400.0):
+        out, _ = oracle.drift_one(MU, np.array([q, 0, 0, 0, vp, 0.0]), t)
+        from scipy.optimize import brentq
+        D = brentq(lambda d: np.sqrt(2 * q ** 3 / MU) * (d + d ** 3 / 3) - t, 0, 1e3, xtol=1e-15)
+        rr = q * (1 + D * D)
+        f = 2 * np.arctan(D)
+        assert np.allclose(out[:3], [rr * np.cos(f), rr * np.sin(f), 0], rtol=0, atol=1e-11 * rr)
+
+
+def test_drift_reversal_and_integrals(oracle):
+    rng = np.random.default_rng(3)
+    from golden.make_golden import drift_cases
+    mu, xv, dt = drift_cases(rng, 400)
+    for q in range(len(dt)):
+        fwd, f1 = oracle.drift_one(mu[q], xv[q], dt[q])
+        back, f2 = oracle.drift_one(mu[q], fwd, -dt[q])
+        if f1 or f2:
+            continue
+        sx = np.linalg.norm(xv[q][:3]); sv = np.linalg.norm(xv[q][3:])
+        # the solver stops at |f/dt| < DANBYB (swift.inc:23), so the round trip
+        # is exact to a tolerance that grows with the angle swept
+        tol = 1e-11 * max(1.0, abs(dt[q]) * sv / sx)
+        assert np.max(np.abs(back[:3] - xv[q][:3])) < tol * sx
+        assert np.max(np.abs(back[3:] - xv[q][3:])) < tol * sv
+        e0 = 0.5 * sv ** 2 - mu[q] / sx
+        e1 = 0.5 * np.dot(fwd[3:], fwd[3:]) - mu[q] / np.linalg.norm(fwd[:3])
+        assert abs(e1 - e0) <= 1e-10 * max(abs(e0), mu[q] / sx)
+        l0 = np.cross(xv[q][:3], xv[q][3:]); l1 = np.cross(fwd[:3], fwd[3:])
+        assert np.linalg.norm(l1 - l0) <= 1e-10 * sx * sv
+
+
+def test_small_body_acceleration_brute_force(oracle):
+    """mfo_mvs small-body terms = indirect a0tp + direct planet pulls, summed
+    here in extended precision."""
+    s = systems.make_system(5, 50, nplanets=8)
+    a = oracle.mfo_mvs(s["m"], s["x"], s["nbig"])
+    m = s["m"].astype(np.longdouble); x = s["x"].astype(np.longdouble)
+    nb = s["nbig"]
+    a0tp = -sum(m[k] * x[k] / np.linalg.norm(x[k]) ** 3 for k in range(1, nb))
+    for j in range(nb, len(m)):
+        d = x[j] - x[1:nb]
+        want = a0tp - np.sum(m[1:nb, None] * d / np.linalg.norm(d, axis=1)[:, None] ** 3, axis=0)
+        assert np.max(np.abs(a[j] - want.astype(float))) <= 1e-14 * np.linalg.norm(want.astype(float))
+
+
+def test_mce_min_cubic(oracle):
+    """mce_min's d2min is the cubic Hermite interpolant of d^2(t) through
+    (d0, d0t) at the step start and (d1, d1t) at its end, evaluated at the
+    returned time, and that time is a stationary point of the interpolant
+    whenever it lies strictly inside the step.  When the interpolant has one
+    interior minimum (d' changes sign once), it is the global minimum found by
+    dense sampling."""
+    rng = np.random.default_rng(8)
+    interior = 0
+    for _ in range(400):
+        h = float(rng.choice([2.0, 8.0]))
+        d0, d1 = 10 ** rng.uniform(-4, 0, 2)
+        d0t = -abs(rng.normal()) * 10 ** rng.uniform(-4, -1)
+        d1t = abs(rng.normal()) * 10 ** rng.uniform(-4, -1)
+        d2min, tmin = oracle.mce_min(d0, d1, d0t, d1t, h)
+
+        def herm(tau):
+            s_ = tau + 1.0
+            return ((2 * s_ ** 3 - 3 * s_ ** 2 + 1) * d0 + (s_ ** 3 - 2 * s_ ** 2 + s_) * h * d0t
+                    + (-2 * s_ ** 3 + 3 * s_ ** 2) * d1 + (s_ ** 3 - s_ ** 2) * h * d1t)
+
+        tau = tmin / h
+        assert -1.0 <= tau <= 0.0
+        assert abs(d2min - max(herm(tau), 0.0)) <= 1e-12 * max(d0, d1)
+        if -1.0 < tau < 0.0:
+            eps = 1e-6
+            slope = (herm(tau + eps) - herm(tau - eps)) / (2 * eps)
+            assert abs(slope) <= 1e-6 * (max(d0, d1) + h * (abs(d0t) + abs(d1t)))
+            grid = np.linspace(-1, 0, 100001)
+            vals = herm(grid)
+            dv = np.diff(vals)
+            if np.count_nonzero(np.diff(np.sign(dv))) == 1:
+                interior += 1
+                assert abs(d2min - max(vals.min(), 0.0)) <= 1e-7 * max(d0, d1)
+    assert interior > 100
+
+
+def test_mce_cent_impact_time(oracle):
+    """A radial plunge from r0 with zero speed hits r = rcen at the free-fall
+    time; mce_cent's q == 0 branch interpolates linearly in r over the step,
+    and the elliptic branch for a grazing orbit matches Kepler's equation."""
+    rcen = 0.005
+    # elliptic grazing orbit: q slightly inside rcen, start at apocentre side
+    a_, e_ = 0.6, 1 - 0.004 / 0.6
+    q = a_ * (1 - e_)
+    n = np.sqrt(MU / a_ ** 3)
+    E0 = -0.3   # eccentric anomaly at the step start (approaching pericentre)
+    r0 = a_ * (1 - e_ * np.cos(E0))
+    x0 = np.array([a_ * (np.cos(E0) - e_), a_ * np.sqrt(1 - e_ ** 2) * np.sin(E0), 0.0])
+    edot = n / (1 - e_ * np.cos(E0))
+    v0 = np.array([-a_ * np.sin(E0) * edot, a_ * np.sqrt(1 - e_ ** 2) * np.cos(E0) * edot, 0.0])
+    h = 4.0
+    out, _ = oracle.drift_one(MU, np.concatenate([x0, v0]), h)
+    m = np.array([MU, 0.0])
+    X0 = np.array([[0, 0, 0], x0]); V0 = np.array([[0, 0, 0], v0])
+    X1 = np.array([[0, 0, 0], out[:3]]); V1 = np.array([[0, 0, 0], out[3:]])
+    hits = oracle.mce_cent(h, h, rcen, m, X0, V0, X1, V1)
+    assert len(hits) == 1 and q <= rcen
+    Ehit = -np.arccos((1 - rcen / a_) / e_)
+    t_hit = ((Ehit - e_ * np.sin(Ehit)) - (E0 - e_ * np.sin(E0))) / n
+    assert abs(hits[0][1] - t_hit) < 1e-9 * h
