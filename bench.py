@@ -1,0 +1,193 @@
+"""Throughput of the MVS massless-particle step on one MI355X.
+
+Workload: 8 planets + 1,000,000 massless test particles (seeded synthetic
+system: planet masses log-U[1e-7,1e-3] Msun, a log-U[0.4,30] AU, >= 10 mutual
+Hill radii apart; particles a U[0.5,40] AU, e U[0,0.3], i U[0,20 deg]),
+FP64, step 4 d, encounter and central-impact screening on every step.  A
+"step" is one MVS step of the whole system (planets and all particles):
+kick, Kepler drift, close-encounter screen, planet-to-particle force, kick,
+central-body check.  Timed region: `steps` steps through
+mercury_hip_advance, state resident in HBM, host wall clock around the call
+(which synchronises), after `warmup` untimed steps.
+
+Prints one JSON line.  `roofline` refers to the dominant kernel (the fast
+particle kernel) timed with HIP events on its launch stream; `cpu_baseline` is
+the reference's own CPU path (mdt_mvs + mce_cent from oracle/_ref, compiled
+from the reference Fortran) on a bounded sample, one process per core.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+# Algorithmic FP64 operations per fast-path particle-step (kick, kepmd drift,
+# particle box + 8 planet box tests, 8-planet force, kick, central gate),
+# frozen from `python -m oracle.opcount` over this workload's population
+# (each IEEE +,-,*,/,sqrt = 1; no FMA in this build).
+FLOP_PER_PSTEP = 351.6
+# Algorithmic HBM bytes per particle per launch of the fast kernel: read
+# x, v, a (72 B) and rce (8 B), write x, v, a (72 B).
+BYTES_PER_PARTICLE_LAUNCH = 152.0
+PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(sysd, nproc=16, shard=6250, nsteps=384):
+    """The reference CPU path (flang build of mdt_mvs + mce_cent) on `nproc`
+    disjoint particle shards in parallel processes.  Massless particles do not
+    interact, so the shards are exact.  Returns particle-steps/s."""
+    import multiprocessing as mp
+    import numpy as np
+    from oracle.bindings import REF_PATH
+    if not REF_PATH.exists():
+        return None
+    nb = sysd["nbig"]
+    rng = np.random.default_rng(1)
+    pick = rng.choice(np.arange(nb, len(sysd["m"])), nproc * shard, replace=False)
+    jobs = []
+    for p in range(nproc):
+        idx = np.concatenate([np.arange(nb), pick[p * shard:(p + 1) * shard]])
+        sub = {k: sysd[k][idx] for k in ("m", "x", "v", "rce", "rphys")}
+        sub.update(nbig=nb, h=sysd["h"], rcen=sysd["rcen"], jcen=np.zeros(3))
+        jobs.append((sub, nsteps))
+    ctx = mp.get_context("spawn")
+    with ctx.Pool(nproc) as pool:
+        times = pool.map(_cpu_worker, jobs)
+    return nproc * shard * nsteps / max(times), max(times)
+
+
+def _cpu_worker(job):
+    import threading
+    sub, nsteps = job
+    out = {}
+
+    def run():
+        from oracle.bindings import Reference
+        R = Reference(nmax=len(sub["m"]) + 16, cmax=len(sub["m"]) + 16)
+        R.mdt_mvs_run(sub, 2)           # warm (allocations, page-in)
+        t = time.perf_counter()
+        R.mdt_mvs_run(sub, nsteps)
+        out["t"] = time.perf_counter() - t
+
+    threading.stack_size(256 << 20)
+    th = threading.Thread(target=run)
+    th.start()
+    th.join()
+    return out["t"]
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=512)
+    ap.add_argument("--warmup", type=int, default=64)
+    ap.add_argument("--particles", type=int, default=1_000_000)
+    ap.add_argument("--spl", type=int, default=32, help="steps fused per particle launch")
+    ap.add_argument("--block", type=int, default=128)
+    ap.add_argument("--no-screen", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-procs", type=int, default=16)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the massless-particle path has no multi-GPU mode in this build "
+              "(particles would shard across GPUs with a replicated planet table; see DESIGN.md).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from mercuda_amd import systems
+    from mercuda_amd.hip import MercuryHip
+
+    hip = MercuryHip()
+    if not hip.available():
+        raise SystemExit("no HIP device")
+    log(f"generating 8 planets + {args.particles} particles")
+    sysd = systems.config_large(seed=2, nparticles=args.particles)
+    hip.configure(1)
+    hip.upload_system(sysd)
+    hip.set_tuning(steps_per_launch=args.spl, block_size=args.block)
+    screen = not args.no_screen
+    h = sysd["h"]
+    log(f"warmup {args.warmup} steps")
+    hip.advance(0.0, h, args.warmup, sysd["rcen"], screen=screen)
+    t0 = args.warmup * h
+    hip.timing(True)
+    log(f"timed {args.steps} steps")
+    tic = time.perf_counter()
+    hip.advance(t0, h, args.steps, sysd["rcen"], screen=screen)
+    sec = time.perf_counter() - tic
+    tm = hip.timing_read()
+    fast_steps, gen_steps = hip.work_read()
+    nevents = len(hip.advance_events())
+    hip.timing(False)
+    npart = args.particles
+    value = npart * args.steps / sec
+    log(f"{value:.4e} particle-steps/s; kernels {tm}; events {nevents}; parked {hip.parked()}")
+
+    # roofline of the dominant kernel (fast particle kernel)
+    launches = max(tm["particle_launches"], 1)
+    avg_s = tm["particle_ms"] / launches / 1e3
+    flop_per_launch = FLOP_PER_PSTEP * fast_steps / launches
+    achieved = flop_per_launch / avg_s / 1e12
+    roofline = {
+        "bound": "fp64-valu",
+        "achieved": round(achieved, 3),
+        "peak": PEAK_FP64_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": round(achieved / PEAK_FP64_TFLOPS, 4),
+        "traffic": None,
+        "kernel": "mvs_particles_fast",
+        "avg_launch_ms": round(avg_s * 1e3, 4),
+        "flop_per_particle_step": FLOP_PER_PSTEP,
+        "particle_steps_per_launch": round(fast_steps / launches, 1),
+        "hbm_gbs_algorithmic": round(BYTES_PER_PARTICLE_LAUNCH * npart / avg_s / 1e9, 1),
+        "time_share": {"fast_ms": round(tm["particle_ms"], 2), "general_ms": round(tm["general_ms"], 2),
+                       "planet_ms": round(tm["planet_ms"], 2), "wall_ms": round(sec * 1e3, 2)},
+    }
+    cpu = None
+    if not args.no_cpu:
+        log(f"cpu baseline: reference mdt_mvs on {args.cpu_procs} processes")
+        r = cpu_baseline(sysd, nproc=args.cpu_procs)
+        if r:
+            rate, tmax = r
+            cpu = {"value": round(rate, 1), "unit": "particle-steps/s", "cores": args.cpu_procs,
+                   "kind": "reference",
+                   "sample": f"{args.cpu_procs} processes x 6250 particles x 384 steps of this workload "
+                             f"(reference mdt_mvs + mce_cent, flang -O3 -ffp-contract=off), "
+                             f"slowest process {tmax:.2f} s",
+                   "gpu_over_cpu": round(value / rate, 1)}
+    line = {
+        "metric": "FP64 particle-steps/s, MVS 8pl+1M TP",
+        "value": round(value, 1),
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(sec * 1e3 / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"MVS, 8 planets + {npart} massless particles, dt 4 d, "
+                               f"encounter screening {'on' if screen else 'off'}",
+                   "particles": npart, "planets": 8, "dt_days": h,
+                   "steps_per_launch": args.spl, "block": args.block, "parallelism": "1 GPU"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -102,7 +102,7 @@ int mercury_hip_advance(double time, double h, int nsteps, double rcen, int scre
  * (system, step, min index, max index). */
 int mercury_hip_advance_events(const MercuryHipEvent **events, int *count);
 /* Launch shape: steps fused per particle launch (>= 1), particle workgroup
- * size (64..1024, multiple of 64).  0 keeps the current value. */
+ * size (64..256, multiple of 64).  0 keeps the current value. */
 int mercury_hip_set_tuning(int steps_per_launch, int block_size);
 /* Kernel timing with HIP events on the launch stream: enable = 1 clears and
  * starts recording; the read returns the summed milliseconds and launch
@@ -113,6 +113,9 @@ int mercury_hip_timing_read(double *particle_ms, int *particle_launches,
                             double *general_ms, int *general_launches);
 /* Particles parked into the general kernel by the last advance. */
 int64_t mercury_hip_parked(void);
+/* Particle-steps completed by the fast and by the general particle kernel
+ * since timing was last enabled. */
+int mercury_hip_work_read(int64_t *fast_steps, int64_t *general_steps);
 
 #ifdef __cplusplus
 }

@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -78,9 +79,16 @@ struct Context {
     DevEvent* dev = nullptr;
     int* devcount = nullptr;
     int ev_cap = 0;
-    Park* dpark = nullptr;
-    int* dparkcount = nullptr;
+    Park* dpark = nullptr;           // slow-set list of the current call
+    int* dcnt = nullptr;             // [0] slow-set entries, [1+k] entry count after launch k's fast
+                                     // kernel, [1+kRing+k] screening requests of ring slot k
+    unsigned char* dslow = nullptr;  // slow-set flag per body
     int park_cap = 0;
+    ScreenReq* dreq = nullptr;       // kRing slots of req_cap requests
+    int req_cap = 0;
+    long long slot_doubles = 0;      // table doubles per ring slot
+    hipStream_t sp = nullptr, sg = nullptr;   // planet and general-kernel streams
+    hipEvent_t ev_plan[8] = {}, ev_fast[8] = {}, ev_done[8] = {};
     // host
     std::vector<double> stage;
     std::vector<DevEvent> hev;
@@ -92,15 +100,33 @@ struct Context {
     double* tcur = nullptr;          // device per-system time
     hipStream_t stream = nullptr;
     int steps_per_launch = 32;
-    int block = 256;
+    int block = 128;
+    int gen_per_wave = 4;
     int64_t parked = 0;
+    unsigned long long* work = nullptr;   // device particle-step counters (timing mode)
     Timing timing;
 };
 
+constexpr int kRing = 4;   // launches in flight: table, counter and request slots
 Context ctx;
 
 void ensure_stream() {
-    if (!ctx.stream) HIPCHECK(hipStreamCreateWithFlags(&ctx.stream, hipStreamNonBlocking));
+    if (ctx.stream) return;
+    if (const char* e = std::getenv("MERCURY_HIP_GEN_PER_WAVE")) ctx.gen_per_wave = std::max(1, std::min(64, std::atoi(e)));
+    HIPCHECK(hipStreamCreateWithFlags(&ctx.stream, hipStreamNonBlocking));
+    HIPCHECK(hipStreamCreateWithFlags(&ctx.sp, hipStreamNonBlocking));
+    HIPCHECK(hipStreamCreateWithFlags(&ctx.sg, hipStreamNonBlocking));
+    for (int k = 0; k < kRing; k++) {
+        HIPCHECK(hipEventCreateWithFlags(&ctx.ev_plan[k], hipEventDisableTiming));
+        HIPCHECK(hipEventCreateWithFlags(&ctx.ev_fast[k], hipEventDisableTiming));
+        HIPCHECK(hipEventCreateWithFlags(&ctx.ev_done[k], hipEventDisableTiming));
+    }
+}
+
+void sync_all() {
+    HIPCHECK(hipStreamSynchronize(ctx.sp));
+    HIPCHECK(hipStreamSynchronize(ctx.stream));
+    HIPCHECK(hipStreamSynchronize(ctx.sg));
 }
 
 void release_all() {
@@ -108,7 +134,7 @@ void release_all() {
     for (double** p : {&c.x, &c.v, &c.a, &c.m, &c.rce, &c.rphys, &c.ox, &c.ov, &c.v0k, &c.v1, &c.tab})
         dfree(*p);
     dfree(c.fault); dfree(c.dsys); dfree(c.tcur); c.sys_valid = false; dfree(c.dtiles); dfree(c.dev); dfree(c.devcount);
-    dfree(c.dpark); dfree(c.dparkcount);
+    dfree(c.dpark); dfree(c.dcnt); dfree(c.dslow); dfree(c.work); dfree(c.dreq); c.req_cap = 0;
     c.cap = 0; c.tab_cap = 0; c.ev_cap = 0; c.park_cap = 0; c.ntiles = 0;
     c.n = 0; c.nbig = 0; c.history = false; c.saved_valid = false; c.have_state = false;
     c.events.clear(); c.adv_events.clear();
@@ -134,10 +160,13 @@ void ensure_capacity(int n) {
     if (!ctx.tcur) dalloc(ctx.tcur, 1);
     ctx.park_cap = (int)std::min<long long>(cap, 1LL << 30);
     dalloc(ctx.dpark, (size_t)ctx.park_cap);
-    if (!ctx.dparkcount) dalloc(ctx.dparkcount, 1);
+    if (!ctx.dcnt) dalloc(ctx.dcnt, 1 + 2 * kRing);
+    dalloc(ctx.dslow, (size_t)cap);
+    ctx.req_cap = (int)std::min<long long>(std::max<long long>(cap / 2, 1 << 18), 1LL << 26);
+    dalloc(ctx.dreq, (size_t)ctx.req_cap * kRing);
     if (!ctx.devcount) dalloc(ctx.devcount, 1);
     if (ctx.ev_cap < 4096) {
-        ctx.ev_cap = 1 << 16;
+        ctx.ev_cap = 1 << 18;
         dalloc(ctx.dev, (size_t)ctx.ev_cap);
     }
     ctx.cap = cap;
@@ -190,16 +219,15 @@ void download_vec3(const double* src, double* dst) {
         for (int k = 0; k < 3; k++) dst[3 * j + k] = j == 0 ? 0.0 : ctx.stage[(size_t)k * n + j];
 }
 
-void timed_launch_begin(int kind, hipEvent_t& start) {
+void timed_launch_begin(hipStream_t st, hipEvent_t& start) {
     if (!ctx.timing.on) return;
     start = ctx.timing.get();
-    HIPCHECK(hipEventRecord(start, ctx.stream));
-    (void)kind;
+    HIPCHECK(hipEventRecord(start, st));
 }
-void timed_launch_end(int kind, hipEvent_t start) {
+void timed_launch_end(int kind, hipStream_t st, hipEvent_t start) {
     if (!ctx.timing.on) return;
     hipEvent_t stop = ctx.timing.get();
-    HIPCHECK(hipEventRecord(stop, ctx.stream));
+    HIPCHECK(hipEventRecord(stop, st));
     ctx.timing.marks.push_back({kind, {start, stop}});
 }
 
@@ -215,54 +243,160 @@ void build_sys(double h, int smax, double rcen) {
     d.mu = c.mu; d.h = h; d.hby2 = .5 * h; d.rcen = rcen;
     d.j2 = c.jcen[0]; d.j4 = c.jcen[1]; d.j6 = c.jcen[2];
     d.obl = (c.jcen[0] != 0 || c.jcen[1] != 0 || c.jcen[2] != 0);
-    ensure_table(d.start + start_stride(np));
+    c.slot_doubles = (d.start + start_stride(np) + 63) / 64 * 64;
+    ensure_table(c.slot_doubles * kRing);
     if (c.sys_valid && std::memcmp(&d, &c.sys, sizeof(Sys)) == 0) return;
     c.sys = d;
     HIPCHECK(hipMemcpy(c.dsys, &c.sys, sizeof(Sys), hipMemcpyHostToDevice));
     c.sys_valid = true;
 }
 
-// One fused launch sequence of `steps` MVS steps of the single system:
-// planets -> fast particles -> parked particles.
-void mvs_launch(double time, double h, int steps, bool reset, bool screen, bool save, double rcen) {
+// Fast kernel specialised on the planet count for the fused (non-saving)
+// launches of systems with up to 12 planets; generic otherwise.
+template <bool SCREEN, int NP>
+bool launch_fast_np(int np, int occ, dim3 grid, dim3 block, hipStream_t st, const ParticleArgs& args) {
+    if constexpr (NP > 12) {
+        return false;
+    } else {
+        if (np != NP) return launch_fast_np<SCREEN, NP + 1>(np, occ, grid, block, st, args);
+        if (occ >= 6) hipLaunchKernelGGL((mvs_particles_fast<SCREEN, false, NP, 6>), grid, block, 0, st, args);
+        else if (occ == 5) hipLaunchKernelGGL((mvs_particles_fast<SCREEN, false, NP, 5>), grid, block, 0, st, args);
+        else hipLaunchKernelGGL((mvs_particles_fast<SCREEN, false, NP, 1>), grid, block, 0, st, args);
+        return true;
+    }
+}
+
+void launch_fast(bool screen, bool save, int np, dim3 grid, dim3 block, hipStream_t st, const ParticleArgs& args) {
+    static const bool generic_only = std::getenv("MERCURY_HIP_GENERIC_FAST") != nullptr;
+    static const int occ = std::getenv("MERCURY_HIP_FAST_OCC") ? std::atoi(std::getenv("MERCURY_HIP_FAST_OCC")) : 1;
+    bool done = false;
+    if (!save && !generic_only) done = screen ? launch_fast_np<true, 0>(np, occ, grid, block, st, args)
+                                              : launch_fast_np<false, 0>(np, occ, grid, block, st, args);
+    if (!done) {
+        if (screen && save) hipLaunchKernelGGL((mvs_particles_fast<true, true>), grid, block, 0, st, args);
+        else if (screen) hipLaunchKernelGGL((mvs_particles_fast<true, false>), grid, block, 0, st, args);
+        else if (save) hipLaunchKernelGGL((mvs_particles_fast<false, true>), grid, block, 0, st, args);
+        else hipLaunchKernelGGL((mvs_particles_fast<false, false>), grid, block, 0, st, args);
+    }
+    HIPCHECK(hipGetLastError());
+}
+void launch_general(bool screen, bool save, dim3 grid, dim3 block, hipStream_t st, const ParticleArgs& args) {
+    // a single system: uniform variant
+    if (screen && save) hipLaunchKernelGGL((mvs_particles_general<true, true, true>), grid, block, 0, st, args);
+    else if (screen) hipLaunchKernelGGL((mvs_particles_general<true, false, true>), grid, block, 0, st, args);
+    else if (save) hipLaunchKernelGGL((mvs_particles_general<false, true, true>), grid, block, 0, st, args);
+    else hipLaunchKernelGGL((mvs_particles_general<false, false, true>), grid, block, 0, st, args);
+    HIPCHECK(hipGetLastError());
+}
+
+// Launch lengths for nsteps with at most smax per launch: a short ramp at the
+// start (the first particle launch waits only for a short planet table) and at
+// the end (the last slow-set launch, which follows the last fast launch, is
+// short).  Launch boundaries do not change results.
+std::vector<int> chunk_plan(int nsteps, int smax) {
+    std::vector<int> head, tail, plan;
+    if (smax >= 16 && nsteps >= 4 * smax) {
+        for (int c = 4; c < smax; c *= 2) { head.push_back(c); tail.push_back(c); }
+    }
+    int rest = nsteps;
+    for (int c : head) rest -= c;
+    for (int c : tail) rest -= c;
+    plan = head;
+    while (rest > 0) { int c = std::min(smax, rest); plan.push_back(c); rest -= c; }
+    for (auto it = tail.rbegin(); it != tail.rend(); ++it) plan.push_back(*it);
+    return plan;
+}
+
+// nsteps MVS steps of the single system in launches of at most `smax` steps.
+// Per launch k (ring slot k % kRing):
+//   planet stream:   planets (table of the launch's steps)
+//   main stream:     fast particle kernel, then the slow-set count snapshot
+//   general stream:  slow-set particles, then the deferred screening details
+// The planet stream runs up to kRing launches ahead; the general kernel of
+// launch k overlaps the fast kernel of launch k+1 (disjoint particles).
+void mvs_run(double time, double h, int nsteps, bool reset, bool screen, bool save, double rcen) {
     Context& c = ctx;
-    build_sys(h, std::max(c.steps_per_launch, steps), rcen);
+    const int smax = save ? 1 : c.steps_per_launch;
+    build_sys(h, smax, rcen);
     State st = device_state();
-    EventSink es{c.dev, c.devcount, c.ev_cap};
-    hipEvent_t t0 = nullptr;
-    // planets (a system with no Big bodies still needs the table header)
-    PlanetArgs pa{c.dsys, st, c.tab, c.tcur, es, time, 1, steps, reset ? 1 : 0, screen ? 1 : 0,
-                  save ? 1 : 0, c.dparkcount};
-    timed_launch_begin(K_PLANET, t0);
-    hipLaunchKernelGGL(mvs_planets, dim3(1), dim3(64), 0, c.stream, pa);
+    EventSink es{c.dev, c.devcount, c.ev_cap, c.fault};
+    const long long nb = c.n;
+    hipLaunchKernelGGL(zero_bytes, dim3((unsigned)std::min<long long>((nb + 255) / 256, 4096)), dim3(256), 0,
+                       c.stream, c.dslow, nb);
+    hipLaunchKernelGGL(zero_ints, dim3(1), dim3(64), 0, c.stream, c.dcnt, 1 + 2 * kRing);
     HIPCHECK(hipGetLastError());
-    timed_launch_end(K_PLANET, t0);
-    if (c.ntiles == 0) return;
-    ParticleArgs args{c.dsys, c.dtiles, st, c.tab, es, ParkList{c.dpark, c.dparkcount, c.park_cap},
-                      steps, reset ? 1 : 0, screen ? 1 : 0, save ? 1 : 0};
-    timed_launch_begin(K_PARTICLE, t0);
-    if (screen && save) hipLaunchKernelGGL((mvs_particles_fast<true, true>), dim3(c.ntiles), dim3(c.block), 0, c.stream, args);
-    else if (screen) hipLaunchKernelGGL((mvs_particles_fast<true, false>), dim3(c.ntiles), dim3(c.block), 0, c.stream, args);
-    else if (save) hipLaunchKernelGGL((mvs_particles_fast<false, true>), dim3(c.ntiles), dim3(c.block), 0, c.stream, args);
-    else hipLaunchKernelGGL((mvs_particles_fast<false, false>), dim3(c.ntiles), dim3(c.block), 0, c.stream, args);
-    HIPCHECK(hipGetLastError());
-    timed_launch_end(K_PARTICLE, t0);
-    int npart = c.n - c.nbig;
-    int gblocks = std::max(1, std::min((npart + 63) / 64, 4096));
-    timed_launch_begin(K_GENERAL, t0);
-    if (screen && save) hipLaunchKernelGGL((mvs_particles_general<true, true>), dim3(gblocks), dim3(64), 0, c.stream, args);
-    else if (screen) hipLaunchKernelGGL((mvs_particles_general<true, false>), dim3(gblocks), dim3(64), 0, c.stream, args);
-    else if (save) hipLaunchKernelGGL((mvs_particles_general<false, true>), dim3(gblocks), dim3(64), 0, c.stream, args);
-    else hipLaunchKernelGGL((mvs_particles_general<false, false>), dim3(gblocks), dim3(64), 0, c.stream, args);
-    HIPCHECK(hipGetLastError());
-    timed_launch_end(K_GENERAL, t0);
+    // the other streams start after the counters are cleared
+    HIPCHECK(hipEventRecord(c.ev_fast[0], c.stream));
+    HIPCHECK(hipStreamWaitEvent(c.sp, c.ev_fast[0], 0));
+    HIPCHECK(hipStreamWaitEvent(c.sg, c.ev_fast[0], 0));
+    const int npart = c.n - c.nbig;
+    const int gblocks = std::max(1, std::min((npart + c.gen_per_wave - 1) / c.gen_per_wave, 4096));
+    const int rblocks = std::max(1, std::min((c.req_cap + 255) / 256, 1024));
+    double t = time;
+    int done = 0;
+    const std::vector<int> plan = chunk_plan(nsteps, smax);
+    for (int k = 0; k < (int)plan.size(); k++) {
+        const int steps = plan[k];
+        const int slot = k % kRing;
+        double* tab = c.tab + (long long)slot * c.slot_doubles;
+        const bool rs0 = reset && k == 0;
+        int* snap = c.dcnt + 1 + slot;
+        int* reqcount = c.dcnt + 1 + kRing + slot;
+        hipEvent_t t0 = nullptr;
+        if (k >= kRing) HIPCHECK(hipStreamWaitEvent(c.sp, c.ev_done[slot], 0));
+        PlanetArgs pa{c.dsys, st, tab, c.tcur, es, t, 1, steps, rs0 ? 1 : 0, screen ? 1 : 0, save ? 1 : 0,
+                      reqcount, 1, done};
+        timed_launch_begin(c.sp, t0);
+        hipLaunchKernelGGL(mvs_planets, dim3(1), dim3(64), 0, c.sp, pa);
+        HIPCHECK(hipGetLastError());
+        timed_launch_end(K_PLANET, c.sp, t0);
+        HIPCHECK(hipEventRecord(c.ev_plan[slot], c.sp));
+        if (c.ntiles > 0) {
+            ReqSink rs{c.dreq + (size_t)slot * c.req_cap, reqcount, c.req_cap, c.fault};
+            ParkList pl{c.dpark, c.dcnt, snap, c.dslow, c.park_cap, k, c.gen_per_wave};
+            ParticleArgs args{c.dsys, c.dtiles, st, tab, es, pl, rs, c.timing.on ? c.work : nullptr,
+                              steps, rs0 ? 1 : 0, screen ? 1 : 0, save ? 1 : 0, done};
+            HIPCHECK(hipStreamWaitEvent(c.stream, c.ev_plan[slot], 0));
+            timed_launch_begin(c.stream, t0);
+            launch_fast(screen, save, c.nbig - 1, dim3(c.ntiles), dim3(c.block), c.stream, args);
+            timed_launch_end(K_PARTICLE, c.stream, t0);
+            hipLaunchKernelGGL(copy_int, dim3(1), dim3(64), 0, c.stream, c.dcnt, snap);
+            HIPCHECK(hipEventRecord(c.ev_fast[slot], c.stream));
+            HIPCHECK(hipStreamWaitEvent(c.sg, c.ev_fast[slot], 0));
+            timed_launch_begin(c.sg, t0);
+            launch_general(screen, save, dim3(gblocks), dim3(64), c.sg, args);
+            timed_launch_end(K_GENERAL, c.sg, t0);
+            if (screen) {
+                RequestArgs ra{c.dsys, st, tab, es, rs, done};
+                hipLaunchKernelGGL(screen_requests, dim3(rblocks), dim3(256), 0, c.sg, ra);
+                HIPCHECK(hipGetLastError());
+            }
+        } else {
+            HIPCHECK(hipStreamWaitEvent(c.sg, c.ev_plan[slot], 0));
+        }
+        HIPCHECK(hipEventRecord(c.ev_done[slot], c.sg));
+        for (int q = 0; q < steps; q++) t = t + h;
+        done += steps;
+    }
+    sync_all();
 }
 
 int read_fault() {
     int bad = 0;
-    HIPCHECK(hipMemcpyAsync(&bad, ctx.fault, sizeof(int), hipMemcpyDeviceToHost, ctx.stream));
-    HIPCHECK(hipStreamSynchronize(ctx.stream));
+    HIPCHECK(hipMemcpy(&bad, ctx.fault, sizeof(int), hipMemcpyDeviceToHost));
     return bad;
+}
+
+void check_fault(const char* what) {
+    int f = read_fault();
+    if (f & FAULT_FORCE) throw std::runtime_error(std::string(what) + ": non-finite acceleration");
+    if (f & FAULT_REQ_OVERFLOW)
+        throw std::runtime_error("screening request buffer overflow: use fewer steps per launch");
+    if (f & FAULT_EVENT_OVERFLOW) throw std::runtime_error("event buffer overflow: advance fewer steps per call");
+}
+
+void clear_fault() {
+    HIPCHECK(hipMemset(ctx.fault, 0, sizeof(int)));
 }
 
 void require_state() {
@@ -363,10 +497,10 @@ int mercury_hip_step(double time, double* h, double* hdid, double tol, int64_t* 
         require_state();
         *rejected = 0;
         if (!mvs_family()) throw std::runtime_error("algorithm not provided by the HIP path");
-        HIPCHECK(hipMemsetAsync(ctx.fault, 0, sizeof(int), ctx.stream));
+        clear_fault();
         bool reset = !ctx.history;
-        mvs_launch(time, *h, 1, reset, false, true, 0.0);
-        if (read_fault()) throw std::runtime_error("MVS force failed");
+        mvs_run(time, *h, 1, reset, false, true, 0.0);
+        check_fault("MVS force failed");
         ctx.history = true;
         ctx.saved_valid = true;
         *hdid = *h;
@@ -393,7 +527,7 @@ int mercury_hip_events(double time, double h, double rcen, const MercuryEvent** 
         int n = ctx.n;
         for (;;) {
             HIPCHECK(hipMemsetAsync(ctx.devcount, 0, sizeof(int), ctx.stream));
-            SavedScreenArgs sa{ctx.dsys, device_state(), ctx.tab, EventSink{ctx.dev, ctx.devcount, ctx.ev_cap},
+            SavedScreenArgs sa{ctx.dsys, device_state(), ctx.tab, EventSink{ctx.dev, ctx.devcount, ctx.ev_cap, ctx.fault},
                                time, h, rcen, 1};
             if (n > 1) hipLaunchKernelGGL(screen_saved, dim3((n - 1 + 255) / 256), dim3(256), 0, ctx.stream, sa);
             HIPCHECK(hipGetLastError());
@@ -460,50 +594,34 @@ int mercury_hip_advance(double time, double h, int nsteps, double rcen, int scre
         if (!mvs_family()) throw std::runtime_error("advance needs MVS");
         if (nsteps < 0) throw std::runtime_error("negative step count");
         ensure_stream();
-        HIPCHECK(hipMemsetAsync(ctx.fault, 0, sizeof(int), ctx.stream));
-        HIPCHECK(hipMemsetAsync(ctx.devcount, 0, sizeof(int), ctx.stream));
-        int64_t nf = 0;
+        clear_fault();
+        HIPCHECK(hipMemset(ctx.devcount, 0, sizeof(int)));
         ctx.adv_events.clear();
-        ctx.parked = 0;
-        double t = time;
-        int done = 0;
-        while (done < nsteps) {
-            int s = std::min(ctx.steps_per_launch, nsteps - done);
-            bool reset = !ctx.history;
-            mvs_launch(t, h, s, reset, screen != 0, false, rcen);
-            nf += s + (reset ? 1 : 0);
+        bool reset = !ctx.history;
+        if (nsteps > 0) {
+            mvs_run(time, h, nsteps, reset, screen != 0, false, rcen);
             ctx.history = true;
-            // the next launch starts from the time the table's last step reached
-            for (int k = 0; k < s; k++) t = t + h;
-            if (screen) {
-                int cnt = 0, pk = 0;
-                HIPCHECK(hipMemcpyAsync(&cnt, ctx.devcount, sizeof(int), hipMemcpyDeviceToHost, ctx.stream));
-                HIPCHECK(hipMemcpyAsync(&pk, ctx.dparkcount, sizeof(int), hipMemcpyDeviceToHost, ctx.stream));
-                HIPCHECK(hipStreamSynchronize(ctx.stream));
-                ctx.parked += pk;
-                if (cnt > ctx.ev_cap) throw std::runtime_error("event buffer overflow in fused advance");
-                if (cnt) {
-                    fetch_events(cnt);
-                    for (auto& e : ctx.hev) {
-                        MercuryHipEvent r;
-                        r.e = to_public(e);
-                        r.step = done + e.step;
-                        r.system = e.sys;
-                        ctx.adv_events.push_back(r);
-                    }
-                    HIPCHECK(hipMemsetAsync(ctx.devcount, 0, sizeof(int), ctx.stream));
-                }
-            }
-            done += s;
         }
-        if (read_fault()) throw std::runtime_error("MVS force failed");
+        check_fault("MVS force failed");
+        int cnt = 0, parked = 0;
+        HIPCHECK(hipMemcpy(&cnt, ctx.devcount, sizeof(int), hipMemcpyDeviceToHost));
+        HIPCHECK(hipMemcpy(&parked, ctx.dcnt, sizeof(int), hipMemcpyDeviceToHost));
+        ctx.parked = parked;
+        fetch_events(std::min(cnt, ctx.ev_cap));
+        for (auto& e : ctx.hev) {
+            MercuryHipEvent r;
+            r.e = to_public(e);
+            r.step = e.step;
+            r.system = e.sys;
+            ctx.adv_events.push_back(r);
+        }
         ctx.saved_valid = false;
         std::sort(ctx.adv_events.begin(), ctx.adv_events.end(), [](const MercuryHipEvent& a, const MercuryHipEvent& b) {
             if (a.system != b.system) return a.system < b.system;
             if (a.step != b.step) return a.step < b.step;
             return pair_less(a.e.i, a.e.j, b.e.i, b.e.j);
         });
-        *forces = nf;
+        *forces = nsteps + ((reset && nsteps > 0) ? 1 : 0);
         return 0;
     } catch (const std::exception& e) { return fail(e); }
 }
@@ -519,7 +637,7 @@ int mercury_hip_set_tuning(int steps_per_launch, int block_size) {
         if (steps_per_launch < 0 || block_size < 0) throw std::runtime_error("invalid tuning");
         if (steps_per_launch) ctx.steps_per_launch = steps_per_launch;
         if (block_size) {
-            if (block_size % 64 || block_size > 1024) throw std::runtime_error("block size must be 64..1024, multiple of 64");
+            if (block_size % 64 || block_size > 256) throw std::runtime_error("block size must be 64..256, multiple of 64");
             ctx.block = block_size;
             if (ctx.have_state) build_tiles();
         }
@@ -531,6 +649,10 @@ int mercury_hip_timing(int enable) {
     try {
         ctx.timing.clear();
         ctx.timing.on = enable != 0;
+        if (ctx.timing.on) {
+            if (!ctx.work) dalloc(ctx.work, 2);
+            HIPCHECK(hipMemset(ctx.work, 0, 2 * sizeof(unsigned long long)));
+        }
         return 0;
     } catch (const std::exception& e) { return fail(e); }
 }
@@ -555,5 +677,15 @@ int mercury_hip_timing_read(double* particle_ms, int* particle_launches, double*
 }
 
 int64_t mercury_hip_parked(void) { return ctx.parked; }
+
+int mercury_hip_work_read(int64_t* fast_steps, int64_t* general_steps) {
+    try {
+        unsigned long long w[2] = {0, 0};
+        if (ctx.work) HIPCHECK(hipMemcpy(w, ctx.work, sizeof(w), hipMemcpyDeviceToHost));
+        *fast_steps = (int64_t)w[0];
+        *general_steps = (int64_t)w[1];
+        return 0;
+    } catch (const std::exception& e) { return fail(e); }
+}
 
 }  // extern "C"

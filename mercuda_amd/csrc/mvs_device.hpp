@@ -78,25 +78,72 @@ struct DevEvent {
     int step, sys;
 };
 
+// Fault and overflow flags (bit set in State::fault / the sinks' flags word).
+enum { FAULT_FORCE = 1, FAULT_REQ_OVERFLOW = 2, FAULT_EVENT_OVERFLOW = 4 };
+
 struct EventSink {
     DevEvent* buf;
     int* count;
     int cap;
+    int* flags;
 };
 
+// A particle that left the fast kernel: it joins the slow set for the rest of
+// the call and is advanced by the general kernel from `step` of launch `chunk`
+// (from step 0 of every later launch).
 struct Park {
-    int j, step, sys, pad;
+    int j, step, sys, chunk;
 };
 
 struct ParkList {
     Park* buf;
-    int* count;
+    int* count;              // entries appended so far in this call
+    const int* snap;         // entry count when this launch's fast kernel finished
+    unsigned char* slow;     // per-body slow-set flag
     int cap;
+    int chunk;               // index of this launch within the call
+    int per_wave;            // general kernel: entries per 64-lane wave (spreads the latency-bound
+                             // slow set over more SIMDs)
 };
 
-__device__ __forceinline__ void push_event(const EventSink& es, const DevEvent& e) {
+// Screening work the fast particle kernel defers: a box overlap with planet p
+// (w = state after the first kick, then after the drift) or a body past the
+// central-impact gate (w = state before the step, then after it).  The detail
+// (mce_min, classification, impact time) runs in screen_requests, which keeps
+// the event-building code out of the fast kernel's register allocation.
+struct ScreenReq {
+    int kind;      // 0 pair, 1 central
+    int g;         // global body index
+    int jb;        // local body index
+    int step;
+    int p;         // planet index (kind 0)
+    int sys;
+    int pad[2];
+    double w[12];
+};
+
+struct ReqSink {
+    ScreenReq* buf;
+    int* count;
+    int cap;
+    int* flags;
+};
+
+__device__ __forceinline__ void push_req(const ReqSink& rs, int kind, int g, int jb, int step, int p, int sys,
+                                         double a0, double a1, double a2, double a3, double a4, double a5,
+                                         double b0, double b1, double b2, double b3, double b4, double b5) {
+    int slot = atomicAdd(rs.count, 1);
+    if (slot >= rs.cap) { atomicOr(rs.flags, FAULT_REQ_OVERFLOW); return; }
+    ScreenReq* r = rs.buf + slot;
+    r->kind = kind; r->g = g; r->jb = jb; r->step = step; r->p = p; r->sys = sys;
+    r->w[0] = a0; r->w[1] = a1; r->w[2] = a2; r->w[3] = a3; r->w[4] = a4; r->w[5] = a5;
+    r->w[6] = b0; r->w[7] = b1; r->w[8] = b2; r->w[9] = b3; r->w[10] = b4; r->w[11] = b5;
+}
+
+__device__ __forceinline__ int reserve_event(const EventSink& es) {
     int slot = atomicAdd(es.count, 1);
-    if (slot < es.cap) es.buf[slot] = e;
+    if (slot >= es.cap) { atomicOr(es.flags, FAULT_EVENT_OVERFLOW); return -1; }
+    return slot;
 }
 
 // ---------------------------------------------------------------------------
@@ -127,12 +174,17 @@ __device__ __forceinline__ void obl_body(double mu, double j2, double j4, double
 // a = a0tp + sum_i -(m_i / d^3) (x - X_i) in planet order, plus oblateness.
 // T points at one step's table (or the start table; both share the header
 // and force-block layout).
+// NP >= 0: the planet count is a compile-time constant (the loop unrolls and
+// the table reads batch into wide scalar loads); NP < 0: runtime count np.
+template <int NP = -1>
 __device__ __forceinline__ void particle_accel(const Sys& d, const double* __restrict__ T, int np,
                                                double x, double y, double z,
                                                double& ax, double& ay, double& az) {
     const double* F = T + kHdr;
+    const int npl = NP >= 0 ? NP : np;
     double a3x = 0.0, a3y = 0.0, a3z = 0.0;
-    for (int p = 0; p < np; p++) {
+#pragma unroll
+    for (int p = 0; p < npl; p++) {
         double dx = x - F[kF * p + 0];
         double dy = y - F[kF * p + 1];
         double dz = z - F[kF * p + 2];
@@ -197,10 +249,11 @@ struct Ends {   // start (after the first kick) and end (after the drift) of one
 };
 
 // The pair part of mce_stat (mercury6_2.for:1820-1925) for Big body i (0-based
-// local index ib, data in S) against body j (0-based local index jb).
-__device__ inline void screen_pair(const double* __restrict__ X1, const double* __restrict__ Sc, int ib, int jb,
-                                   bool j_big, double mi, double mj, const Ends& e, double rce_j, double rphys_j,
-                                   double h, double time_end, int step, int sys, const EventSink& es) {
+// local index ib) against body j (local index jb), for x-y boxes that overlap.
+__device__ inline void screen_pair(const double* __restrict__ X1, const double* __restrict__ Sc, int ib,
+                                         int jb, bool j_big, double mi, double mj, Ends e, double rce_j,
+                                         double rphys_j, double h, double time_end, int step, int sys,
+                                         EventSink es) {
     double dx0 = Sc[SP + 0] - e.x0, dy0 = Sc[SP + 1] - e.y0, dz0 = Sc[SP + 2] - e.z0;
     double du0 = Sc[SV0 + 0] - e.u0, dv0 = Sc[SV0 + 1] - e.v0, dw0 = Sc[SV0 + 2] - e.w0;
     double d0t = (dx0 * du0 + dy0 * dv0 + dz0 * dw0) * 2.0;
@@ -219,29 +272,35 @@ __device__ inline void screen_pair(const double* __restrict__ X1, const double* 
     bool clo = (d2min <= d2ce && d0t * h <= 0.0 && d1t * h >= 0.0) || (d2min <= d2hit);
     bool near = d2min <= d2near;
     if (!clo && !near) return;
-    DevEvent ev;
+    int slot = reserve_event(es);
+    if (slot < 0) return;
+    DevEvent* ev = es.buf + slot;
     bool swap = j_big && mj > mi;
-    ev.i = (swap ? jb : ib) + 1;
-    ev.j = (swap ? ib : jb) + 1;
-    ev.kind = (clo ? 1 : 0) | (near ? 2 : 0) | (d2min <= d2hit ? 4 : 0);
-    ev.now = d1 <= d2hit;
-    ev.distance = sqrt(fmax(0.0, d2min));
-    ev.time = tmin + time_end;
+    ev->i = (swap ? jb : ib) + 1;
+    ev->j = (swap ? ib : jb) + 1;
+    ev->kind = (clo ? 1 : 0) | (near ? 2 : 0) | (d2min <= d2hit ? 4 : 0);
+    ev->now = d1 <= d2hit;
+    ev->distance = sqrt(fmax(0.0, d2min));
+    ev->time = tmin + time_end;
+    ev->step = step;
+    ev->sys = sys;
     double hm1 = 1.0 / h;
     double tmp0 = -tmin * hm1;
     double tmp1 = 1.0 + tmin * hm1;
-    double pi[6] = {tmp0 * Sc[SP] + tmp1 * X1[0], tmp0 * Sc[SP + 1] + tmp1 * X1[1],
-                    tmp0 * Sc[SP + 2] + tmp1 * X1[2], tmp0 * Sc[SV0] + tmp1 * Sc[SV1],
-                    tmp0 * Sc[SV0 + 1] + tmp1 * Sc[SV1 + 1], tmp0 * Sc[SV0 + 2] + tmp1 * Sc[SV1 + 2]};
-    double pj[6] = {tmp0 * e.x0 + tmp1 * e.x1, tmp0 * e.y0 + tmp1 * e.y1, tmp0 * e.z0 + tmp1 * e.z1,
-                    tmp0 * e.u0 + tmp1 * e.u1, tmp0 * e.v0 + tmp1 * e.v1, tmp0 * e.w0 + tmp1 * e.w1};
-    for (int k = 0; k < 6; k++) {
-        ev.xi[k] = swap ? pj[k] : pi[k];
-        ev.xj[k] = swap ? pi[k] : pj[k];
-    }
-    ev.step = step;
-    ev.sys = sys;
-    push_event(es, ev);
+    double* big = swap ? ev->xj : ev->xi;
+    double* other = swap ? ev->xi : ev->xj;
+    big[0] = tmp0 * Sc[SP] + tmp1 * X1[0];
+    big[1] = tmp0 * Sc[SP + 1] + tmp1 * X1[1];
+    big[2] = tmp0 * Sc[SP + 2] + tmp1 * X1[2];
+    big[3] = tmp0 * Sc[SV0] + tmp1 * Sc[SV1];
+    big[4] = tmp0 * Sc[SV0 + 1] + tmp1 * Sc[SV1 + 1];
+    big[5] = tmp0 * Sc[SV0 + 2] + tmp1 * Sc[SV1 + 2];
+    other[0] = tmp0 * e.x0 + tmp1 * e.x1;
+    other[1] = tmp0 * e.y0 + tmp1 * e.y1;
+    other[2] = tmp0 * e.z0 + tmp1 * e.z1;
+    other[3] = tmp0 * e.u0 + tmp1 * e.u1;
+    other[4] = tmp0 * e.v0 + tmp1 * e.v1;
+    other[5] = tmp0 * e.w0 + tmp1 * e.w1;
 }
 
 // Encounter screen of body jb against the Big bodies 1..min(jb,nbig)-1 of one
@@ -267,18 +326,12 @@ __device__ inline void screen_body(const double* __restrict__ T, int np, int jb,
     }
 }
 
-// mce_cent (mercury6_2.for:1059-1185) for one body: (xs,vs) before the step,
-// (xe,ve) after it.
-__device__ inline void central_check(double mcen, double mj, double rcen, double h, double time_end,
-                                     double xs, double ys, double zs, double us, double vs, double ws,
-                                     double xe, double ye, double ze, double ue, double ve, double we,
-                                     int jb, int step, int sys, const EventSink& es) {
-    double rcen2 = rcen * rcen;
-    double rr0 = xs * xs + ys * ys + zs * zs;
-    double rr1 = xe * xe + ye * ye + ze * ze;
-    double rv0 = us * xs + vs * ys + ws * zs;
-    double rv1 = ue * xe + ve * ye + we * ze;
-    if (!((rv0 * h <= 0.0 && rv1 * h >= 0.0) || fmin(rr0, rr1) <= rcen2)) return;
+// The part of mce_cent (mercury6_2.for:1100-1180) past its gate: two-body
+// pericentre and, for an impact, the entry-time estimate.
+__device__ inline void central_event(double mcen, double mj, double rcen, double h, double time_end,
+                                           double xs, double ys, double zs, double us, double vs, double ws,
+                                           double rr0, double rr1, double rv0, int jb, int step, int sys,
+                                           EventSink es) {
     double hx = ys * ws - zs * vs;
     double hy = zs * us - xs * ws;
     double hz = xs * vs - ys * us;
@@ -312,17 +365,33 @@ __device__ inline void central_check(double mcen, double mj, double rcen, double
                                                         - (u0 * u0 * u0) / 3.0);
     }
     if (q == 0.0 && rr1 != rr0) temp = h * fmax(0.0, fmin(1.0, (r0 - rcen) / (r0 - sqrt(rr1))));
-    DevEvent ev;
-    ev.i = 1;
-    ev.j = jb + 1;
-    ev.kind = 8;
-    ev.now = 0;
-    ev.distance = rcen;
-    ev.time = time_end - h + temp;
-    for (int k = 0; k < 6; k++) { ev.xi[k] = 0.0; ev.xj[k] = 0.0; }
-    ev.step = step;
-    ev.sys = sys;
-    push_event(es, ev);
+    int slot = reserve_event(es);
+    if (slot < 0) return;
+    DevEvent* ev = es.buf + slot;
+    ev->i = 1;
+    ev->j = jb + 1;
+    ev->kind = 8;
+    ev->now = 0;
+    ev->distance = rcen;
+    ev->time = time_end - h + temp;
+    for (int k = 0; k < 6; k++) { ev->xi[k] = 0.0; ev->xj[k] = 0.0; }
+    ev->step = step;
+    ev->sys = sys;
+}
+
+// mce_cent (mercury6_2.for:1059-1185) for one body: (xs,vs) before the step,
+// (xe,ve) after it.
+__device__ __forceinline__ void central_check(double mcen, double mj, double rcen, double h, double time_end,
+                                              double xs, double ys, double zs, double us, double vs, double ws,
+                                              double xe, double ye, double ze, double ue, double ve, double we,
+                                              int jb, int step, int sys, const EventSink& es) {
+    double rcen2 = rcen * rcen;
+    double rr0 = xs * xs + ys * ys + zs * zs;
+    double rr1 = xe * xe + ye * ye + ze * ze;
+    double rv0 = us * xs + vs * ys + ws * zs;
+    double rv1 = ue * xe + ve * ye + we * ze;
+    if (!((rv0 * h <= 0.0 && rv1 * h >= 0.0) || fmin(rr0, rr1) <= rcen2)) return;
+    central_event(mcen, mj, rcen, h, time_end, xs, ys, zs, us, vs, ws, rr0, rr1, rv0, jb, step, sys, es);
 }
 
 // ---------------------------------------------------------------------------
@@ -339,7 +408,9 @@ struct PlanetArgs {
     int reset;        // compute accelerations from positions first (dtflag != 2)
     int screen;       // screen planet pairs and central impacts in-kernel
     int save;         // write step-start copies (single-step mode)
-    int* zero;        // counter cleared by the launch (park count), may be null
+    int* zero;        // counters cleared by the launch (park and request counts), may be null
+    int nzero;
+    int step0;        // steps of the call done before this launch (event step numbers)
 };
 
 __device__ inline void h2j_serial(int np, const double* m, const double* jdx, const double* X, const double* V,
@@ -537,7 +608,7 @@ __global__ void __launch_bounds__(64) mvs_planets(PlanetArgs args) {
         }
     }
     double time_end = args.set_time ? args.time0 : args.tcur[sid];
-    if (args.zero && sid == 0 && lane == 0) *args.zero = 0;
+    if (args.zero && sid == 0 && lane < args.nzero) args.zero[lane] = 0;
     const double h = d.h, hby2 = d.hby2;
     for (int s = 0; s < args.steps; s++) {
         double* T = args.tab + d.tab + (long long)s * ts;
@@ -608,10 +679,10 @@ __global__ void __launch_bounds__(64) mvs_planets(PlanetArgs args) {
                    X[3 * p], X[3 * p + 1], X[3 * p + 2], T[kHdr + kF * np + kScr * p + SV1],
                    T[kHdr + kF * np + kScr * p + SV1 + 1], T[kHdr + kF * np + kScr * p + SV1 + 2]};
             screen_body(T, np, jb, d.nbig, m[jb], e, args.st.rce[b0 + jb], args.st.rphys[b0 + jb], h,
-                        time_end, s, sid, args.es);
+                        time_end, s + args.step0, sid, args.es);
             central_check(m[0], m[jb], d.rcen, h, time_end, xs[0], xs[1], xs[2], vs[0], vs[1], vs[2],
                           X[3 * p], X[3 * p + 1], X[3 * p + 2], V[3 * p], V[3 * p + 1], V[3 * p + 2], jb,
-                          s, sid, args.es);
+                          s + args.step0, sid, args.es);
         }
         __syncthreads();
     }
@@ -633,11 +704,21 @@ struct ParticleArgs {
     const double* tab;
     EventSink es;
     ParkList park;
+    ReqSink req;
+    unsigned long long* work;   // [0] particle-steps done by the fast kernel, [1] by the general kernel
     int steps;
     int reset;
     int screen;
     int save;
+    int step0;
 };
+
+// Adds each lane's completed particle-steps into *counter, one atomic per wave.
+__device__ __forceinline__ void count_work(unsigned long long* counter, int steps) {
+    int v = steps;
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    if (__lane_id() == 0 && v) atomicAdd(counter, (unsigned long long)v);
+}
 
 // Step-start copies for the host-driven event call (single-step mode).
 __device__ __forceinline__ void save_step(const State& st, int g, const double xs[6], const Orbit& k0,
@@ -652,6 +733,48 @@ __device__ __forceinline__ void save_step(const State& st, int g, const double x
 // Rest of one step after the drift: screen, force, second kick, central check.
 // A non-finite acceleration raises the fault flag (the step then fails, as
 // mvs_acceleration's check does in the reference, mercury_cuda_symplectic.cuh:78).
+// As finish_step, with the screening details deferred to screen_requests:
+// inline are only the x-y box tests of mce_stat and the gate of mce_cent.
+template <bool SCREEN, int NP>
+__device__ __forceinline__ void finish_step_deferred(const Sys& d, const double* __restrict__ T, int np, int g,
+                                                     int jb, double rce_j, int s, int sid, const ReqSink& rs,
+                                                     int* fault, const double xs[6], const Orbit& k0, Orbit& o,
+                                                     double& ax, double& ay, double& az) {
+    if (SCREEN) {
+        double xmin, xmax, ymin, ymax;
+        box_axis(d.h, k0.x, k0.vx, o.x, o.vx, xmin, xmax);
+        box_axis(d.h, k0.y, k0.vy, o.y, o.vy, ymin, ymax);
+        double temp = rce_j * 1.2;
+        xmin = xmin - temp; xmax = xmax + temp;
+        ymin = ymin - temp; ymax = ymax + temp;
+        const int npl = NP >= 0 ? NP : np;
+        const double* Sc = T + kHdr + kF * npl;
+        // particles (jb >= nbig) test every planet; this kernel never sees Big bodies
+#pragma unroll
+        for (int p = 0; p < npl; p++) {
+            const double* B = Sc + kScr * p;
+            if (B[SX1] >= xmin && xmax >= B[SX0] && B[SY1] >= ymin && ymax >= B[SY0])
+                push_req(rs, 0, g, jb, s, p, sid, k0.x, k0.y, k0.z, k0.vx, k0.vy, k0.vz,
+                         o.x, o.y, o.z, o.vx, o.vy, o.vz);
+        }
+    }
+    particle_accel<NP>(d, T, np, o.x, o.y, o.z, ax, ay, az);
+    if (!isfinite(ax) || !isfinite(ay) || !isfinite(az)) atomicOr(fault, FAULT_FORCE);
+    o.vx = o.vx + d.hby2 * ax;
+    o.vy = o.vy + d.hby2 * ay;
+    o.vz = o.vz + d.hby2 * az;
+    if (SCREEN) {
+        double rcen2 = d.rcen * d.rcen;
+        double rr0 = xs[0] * xs[0] + xs[1] * xs[1] + xs[2] * xs[2];
+        double rr1 = o.x * o.x + o.y * o.y + o.z * o.z;
+        double rv0 = xs[3] * xs[0] + xs[4] * xs[1] + xs[5] * xs[2];
+        double rv1 = o.vx * o.x + o.vy * o.y + o.vz * o.z;
+        if ((rv0 * d.h <= 0.0 && rv1 * d.h >= 0.0) || fmin(rr0, rr1) <= rcen2)
+            push_req(rs, 1, g, jb, s, 0, sid, xs[0], xs[1], xs[2], xs[3], xs[4], xs[5],
+                     o.x, o.y, o.z, o.vx, o.vy, o.vz);
+    }
+}
+
 template <bool SCREEN>
 __device__ __forceinline__ void finish_step(const Sys& d, const double* __restrict__ T, int np, int jb,
                                             double mj, double rce_j, double rphys_j, int s, int sid,
@@ -662,7 +785,7 @@ __device__ __forceinline__ void finish_step(const Sys& d, const double* __restri
         screen_body(T, np, jb, d.nbig, mj, e, rce_j, rphys_j, d.h, T[6], s, sid, es);
     }
     particle_accel(d, T, np, o.x, o.y, o.z, ax, ay, az);
-    if (!isfinite(ax) || !isfinite(ay) || !isfinite(az)) *fault = 1;
+    if (!isfinite(ax) || !isfinite(ay) || !isfinite(az)) atomicOr(fault, FAULT_FORCE);
     o.vx = o.vx + d.hby2 * ax;
     o.vy = o.vy + d.hby2 * ay;
     o.vz = o.vz + d.hby2 * az;
@@ -671,29 +794,34 @@ __device__ __forceinline__ void finish_step(const Sys& d, const double* __restri
                       o.vx, o.vy, o.vz, jb, s, sid, es);
 }
 
-template <bool SCREEN, bool SAVE>
-__global__ void __launch_bounds__(256) mvs_particles_fast(ParticleArgs args) {
+template <bool SCREEN, bool SAVE, int NP = -1, int MINW = 1>
+__global__ void __launch_bounds__(256, MINW) mvs_particles_fast(ParticleArgs args) {
     const int2 tile = args.tiles[blockIdx.x];
     const int sid = tile.x;
     const Sys d = args.sys[sid];
     const int g = tile.y + threadIdx.x;            // global body index
     const int jb = g - d.base;                     // local body index
-    if (jb >= d.n) return;
+    // lanes past the system's end, and slow-set members, idle to the end
+    const bool live = jb < d.n && !args.park.slow[g];
     const long long cap = args.st.cap;
     const int np = d.nbig - 1;
-    Orbit o{args.st.x[g], args.st.x[cap + g], args.st.x[2 * cap + g],
-            args.st.v[g], args.st.v[cap + g], args.st.v[2 * cap + g]};
-    double ax, ay, az;
-    if (args.reset) {
-        particle_accel(d, args.tab + d.start, np, o.x, o.y, o.z, ax, ay, az);
-    } else {
-        ax = args.st.a[g]; ay = args.st.a[cap + g]; az = args.st.a[2 * cap + g];
+    Orbit o{0, 0, 0, 0, 0, 0};
+    double ax = 0, ay = 0, az = 0;
+    double rce_j = 0.0;
+    if (live) {
+        o = Orbit{args.st.x[g], args.st.x[cap + g], args.st.x[2 * cap + g],
+                  args.st.v[g], args.st.v[cap + g], args.st.v[2 * cap + g]};
+        if (args.reset) {
+            particle_accel<NP>(d, args.tab + d.start, np, o.x, o.y, o.z, ax, ay, az);
+        } else {
+            ax = args.st.a[g]; ay = args.st.a[cap + g]; az = args.st.a[2 * cap + g];
+        }
+        if (SCREEN) rce_j = args.st.rce[g];
     }
-    double mj = 0.0, rce_j = 0.0, rphys_j = 0.0;
-    if (SCREEN || SAVE) { mj = args.st.m[g]; rce_j = args.st.rce[g]; rphys_j = args.st.rphys[g]; }
     const double* tab = args.tab + d.tab;
+    const int nsteps = live ? args.steps : 0;
     int s = 0;
-    for (; s < args.steps; s++) {
+    for (; s < nsteps; s++) {
         const double* T = tab + (long long)s * d.ts;
         double xs[6] = {o.x, o.y, o.z, o.vx, o.vy, o.vz};
         Orbit w = o;
@@ -712,41 +840,67 @@ __global__ void __launch_bounds__(256) mvs_particles_fast(ParticleArgs args) {
             if (lane == leader) base = atomicAdd(args.park.count, __popcll(mask));
             base = __shfl(base, leader);
             int slot = base + __popcll(mask & ((1ull << lane) - 1ull));
-            if (slot < args.park.cap) args.park.buf[slot] = Park{g, s, sid, 0};
+            if (slot < args.park.cap) args.park.buf[slot] = Park{g, s, sid, args.park.chunk};
+            args.park.slow[g] = 1;
             break;
         }
         o = w;
         if (SAVE) save_step(args.st, g, xs, k0, o);
-        finish_step<SCREEN>(d, T, np, jb, mj, rce_j, rphys_j, s, sid, args.es, args.st.fault, xs, k0, o,
-                            ax, ay, az);
+        finish_step_deferred<SCREEN, NP>(d, T, np, g, jb, rce_j, s, sid, args.req, args.st.fault, xs, k0, o,
+                                     ax, ay, az);
     }
+    if (args.work) count_work(args.work, s);
+    if (!live) return;
     // parked lanes store their pre-step state and step-start acceleration
     args.st.x[g] = o.x; args.st.x[cap + g] = o.y; args.st.x[2 * cap + g] = o.z;
     args.st.v[g] = o.vx; args.st.v[cap + g] = o.vy; args.st.v[2 * cap + g] = o.vz;
     args.st.a[g] = ax; args.st.a[cap + g] = ay; args.st.a[2 * cap + g] = az;
 }
 
-
 // Parked lanes: resume each from its recorded step with the whole drift chain.
 // Lanes of one wave may start at different steps and belong to different
 // systems, so system and table reads here are per lane.
-template <bool SCREEN, bool SAVE>
+// UNIFORM: every entry belongs to system 0, so the wave walks the steps in
+// lockstep from the earliest start among its lanes and reads each step's table
+// with wave-uniform (scalar) loads; lanes join at their own start step.
+template <bool SCREEN, bool SAVE, bool UNIFORM>
 __global__ void __launch_bounds__(64) mvs_particles_general(ParticleArgs args) {
-    const int count = min(*args.park.count, args.park.cap);
+    const int count = min(*args.park.snap, args.park.cap);
     const long long cap = args.st.cap;
-    for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < count; idx += gridDim.x * blockDim.x) {
-        const Park pk = args.park.buf[idx];
-        const Sys d = args.sys[pk.sys];
+    const int per_wave = args.park.per_wave;
+    const int lane = threadIdx.x;          // one 64-lane wave per workgroup
+    for (int base = blockIdx.x * per_wave; base < count; base += gridDim.x * per_wave) {
+        const int idx = base + lane;
+        bool act = lane < per_wave && idx < count;
+        Park pk{0, 0, 0, 0};
+        int start = args.steps;
+        if (act) {
+            pk = args.park.buf[idx];
+            if (pk.chunk > args.park.chunk) act = false;
+            else start = (pk.chunk < args.park.chunk) ? 0 : pk.step;
+        }
+        if (!act) start = args.steps;
+        const Sys d = args.sys[UNIFORM ? 0 : pk.sys];
         const int g = pk.j;
         const int jb = g - d.base;
         const int np = d.nbig - 1;
-        Orbit o{args.st.x[g], args.st.x[cap + g], args.st.x[2 * cap + g],
-                args.st.v[g], args.st.v[cap + g], args.st.v[2 * cap + g]};
-        double ax = args.st.a[g], ay = args.st.a[cap + g], az = args.st.a[2 * cap + g];
+        Orbit o{0, 0, 0, 0, 0, 0};
+        double ax = 0, ay = 0, az = 0;
         double mj = 0.0, rce_j = 0.0, rphys_j = 0.0;
-        if (SCREEN || SAVE) { mj = args.st.m[g]; rce_j = args.st.rce[g]; rphys_j = args.st.rphys[g]; }
+        if (act) {
+            o = Orbit{args.st.x[g], args.st.x[cap + g], args.st.x[2 * cap + g],
+                      args.st.v[g], args.st.v[cap + g], args.st.v[2 * cap + g]};
+            ax = args.st.a[g]; ay = args.st.a[cap + g]; az = args.st.a[2 * cap + g];
+            if (SCREEN || SAVE) { mj = args.st.m[g]; rce_j = args.st.rce[g]; rphys_j = args.st.rphys[g]; }
+        }
         const double* tab = args.tab + d.tab;
-        for (int s = pk.step; s < args.steps; s++) {
+        int s0 = start;
+        if (UNIFORM) {
+            for (int off = 32; off > 0; off >>= 1) s0 = min(s0, __shfl_xor(s0, off));
+            s0 = __builtin_amdgcn_readfirstlane(s0);
+        }
+        for (int s = s0; s < args.steps; s++) {
+            if (s < start) continue;
             const double* T = tab + (long long)s * d.ts;
             double xs[6] = {o.x, o.y, o.z, o.vx, o.vy, o.vz};
             o.vx = o.vx + d.hby2 * ax;
@@ -755,12 +909,15 @@ __global__ void __launch_bounds__(64) mvs_particles_general(ParticleArgs args) {
             Orbit k0 = o;
             drift_one(d.mu, o, d.h);
             if (SAVE) save_step(args.st, g, xs, k0, o);
-            finish_step<SCREEN>(d, T, np, jb, mj, rce_j, rphys_j, s, pk.sys, args.es, args.st.fault, xs,
-                                k0, o, ax, ay, az);
+            finish_step<SCREEN>(d, T, np, jb, mj, rce_j, rphys_j, s + args.step0, UNIFORM ? 0 : pk.sys,
+                                args.es, args.st.fault, xs, k0, o, ax, ay, az);
         }
-        args.st.x[g] = o.x; args.st.x[cap + g] = o.y; args.st.x[2 * cap + g] = o.z;
-        args.st.v[g] = o.vx; args.st.v[cap + g] = o.vy; args.st.v[2 * cap + g] = o.vz;
-        args.st.a[g] = ax; args.st.a[cap + g] = ay; args.st.a[2 * cap + g] = az;
+        if (act) {
+            args.st.x[g] = o.x; args.st.x[cap + g] = o.y; args.st.x[2 * cap + g] = o.z;
+            args.st.v[g] = o.vx; args.st.v[cap + g] = o.vy; args.st.v[2 * cap + g] = o.vz;
+            args.st.a[g] = ax; args.st.a[cap + g] = ay; args.st.a[2 * cap + g] = az;
+            if (args.work) atomicAdd(args.work + 1, (unsigned long long)(args.steps - start));
+        }
     }
 }
 
@@ -798,9 +955,52 @@ __global__ void __launch_bounds__(256) screen_saved(SavedScreenArgs args) {
                   args.st.v[2 * cap + g], jb, 0, 0, args.es);
 }
 
+// Details of the fast kernel's deferred screening work (one lane per request).
+struct RequestArgs {
+    const Sys* sys;
+    State st;
+    const double* tab;
+    EventSink es;
+    ReqSink req;
+    int step0;
+};
+
+__global__ void __launch_bounds__(256) screen_requests(RequestArgs args) {
+    const int count = min(*args.req.count, args.req.cap);
+    for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < count; idx += gridDim.x * blockDim.x) {
+        const ScreenReq r = args.req.buf[idx];
+        const Sys d = args.sys[r.sys];
+        const int np = d.nbig - 1;
+        const double* T = args.tab + d.tab + (long long)r.step * d.ts;
+        const double mj = args.st.m[r.g];
+        if (r.kind == 0) {
+            const double* F = T + kHdr;
+            const double* B = F + kF * np + kScr * r.p;
+            Ends e{r.w[0], r.w[1], r.w[2], r.w[3], r.w[4], r.w[5], r.w[6], r.w[7], r.w[8], r.w[9], r.w[10], r.w[11]};
+            screen_pair(F + kF * r.p, B, r.p + 1, r.jb, r.jb < d.nbig, F[kF * r.p + 3], mj, e,
+                        args.st.rce[r.g], args.st.rphys[r.g], d.h, T[6], r.step + args.step0, r.sys, args.es);
+        } else {
+            double rr0 = r.w[0] * r.w[0] + r.w[1] * r.w[1] + r.w[2] * r.w[2];
+            double rr1 = r.w[6] * r.w[6] + r.w[7] * r.w[7] + r.w[8] * r.w[8];
+            double rv0 = r.w[3] * r.w[0] + r.w[4] * r.w[1] + r.w[5] * r.w[2];
+            central_event(d.mu, mj, d.rcen, d.h, T[6], r.w[0], r.w[1], r.w[2], r.w[3], r.w[4], r.w[5],
+                          rr0, rr1, rv0, r.jb, r.step + args.step0, r.sys, args.es);
+        }
+    }
+}
+
 __global__ void zero_ints(int* p, int n) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0;
+}
+
+__global__ void zero_bytes(unsigned char* p, long long n) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        p[i] = 0;
+}
+
+__global__ void copy_int(const int* src, int* dst) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *dst = *src;
 }
 
 }  // namespace mhip

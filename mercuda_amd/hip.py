@@ -61,6 +61,7 @@ SIGNATURES = {
     "mercury_hip_timing": (C.c_int, [C.c_int]),
     "mercury_hip_timing_read": (C.c_int, [_dp, _ip, _dp, _ip, _dp, _ip]),
     "mercury_hip_parked": (C.c_int64, []),
+    "mercury_hip_work_read": (C.c_int, [_i64p, _i64p]),
 }
 
 
@@ -180,6 +181,11 @@ class MercuryHip:
                                                 C.byref(gm), C.byref(gl)), "timing_read")
         return dict(particle_ms=pm.value, particle_launches=pl.value, planet_ms=qm.value,
                     planet_launches=ql.value, general_ms=gm.value, general_launches=gl.value)
+
+    def work_read(self):
+        f = C.c_int64(); g = C.c_int64()
+        self._ok(self.L.mercury_hip_work_read(C.byref(f), C.byref(g)), "work_read")
+        return f.value, g.value
 
     def parked(self) -> int:
         return int(self.L.mercury_hip_parked())

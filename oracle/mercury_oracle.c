@@ -768,8 +768,10 @@ void orc_mdt_mvs(double time, double h0, double rcen, const double *jcen,
     double *xj = calloc(n3, sizeof(double)), *vj = calloc(n3, sizeof(double));
     double *x0 = malloc(n3 * sizeof(double)), *v0 = malloc(n3 * sizeof(double));
     double *gm = calloc((size_t)nbod, sizeof(double));
-    ev->nclo = 0;
-    ev->overflow = 0;
+    if (ev) {
+        ev->nclo = 0;
+        ev->overflow = 0;
+    }
     if (*dtflag != 2) {
         *dtflag = 2;
         orc_mco_h2j(nbig, m, x, v, xj, vj);
