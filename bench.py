@@ -87,6 +87,38 @@ def _cpu_worker(job):
     return out["t"]
 
 
+def ensemble_ratio(hip, steps, spl, block):
+    """Batched ensemble (512 systems, 2..12 planets, log-U[100, 5e4] particles,
+    dt 4 d) against one 8-planet system with the same total particle count,
+    both through mercury_hip_advance with screening on."""
+    from mercuda_amd import systems
+    log("ensemble: generating 512 systems")
+    ens = systems.ensemble(seed=4)
+    ntot = sum(len(q["m"]) - q["nbig"] for q in ens)
+    hip.configure(1)
+    hip.ensemble_upload(ens)
+    hip.set_tuning(steps_per_launch=spl, block_size=block)
+    hip.ensemble_advance(0.0, 16, screen=True)
+    tic = time.perf_counter()
+    hip.ensemble_advance(16 * 4.0, steps, screen=True)
+    t_ens = time.perf_counter() - tic
+    log(f"ensemble: single system with {ntot} particles")
+    one = systems.config_large(seed=2, nparticles=ntot)
+    hip.configure(1)
+    hip.upload_system(one)
+    hip.set_tuning(steps_per_launch=spl, block_size=block)
+    hip.advance(0.0, one["h"], 16, one["rcen"], screen=True)
+    tic = time.perf_counter()
+    hip.advance(16 * one["h"], one["h"], steps, one["rcen"], screen=True)
+    t_one = time.perf_counter() - tic
+    r_ens = ntot * steps / t_ens
+    r_one = ntot * steps / t_one
+    return {"systems": len(ens), "particles": ntot, "steps": steps,
+            "ensemble_particle_steps_per_s": round(r_ens, 1),
+            "single_system_particle_steps_per_s": round(r_one, 1),
+            "ratio": round(r_ens / r_one, 3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,6 +130,7 @@ def main():
     ap.add_argument("--no-screen", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-procs", type=int, default=16)
+    ap.add_argument("--no-extras", action="store_true", help="skip the ensemble/single ratio")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the massless-particle path has no multi-GPU mode in this build "
@@ -154,6 +187,9 @@ def main():
         "time_share": {"fast_ms": round(tm["particle_ms"], 2), "general_ms": round(tm["general_ms"], 2),
                        "planet_ms": round(tm["planet_ms"], 2), "wall_ms": round(sec * 1e3, 2)},
     }
+    extras = {}
+    if not args.no_extras:
+        extras["ensemble"] = ensemble_ratio(hip, steps=64, spl=args.spl, block=args.block)
     cpu = None
     if not args.no_cpu:
         log(f"cpu baseline: reference mdt_mvs on {args.cpu_procs} processes")
@@ -185,6 +221,7 @@ def main():
                    "steps_per_launch": args.spl, "block": args.block, "parallelism": "1 GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "extras": extras,
     }
     print(json.dumps(line), flush=True)
 

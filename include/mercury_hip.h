@@ -98,6 +98,19 @@ typedef struct MercuryHipEvent {
  * Same per-step arithmetic as mercury_hip_step. */
 int mercury_hip_advance(double time, double h, int nsteps, double rcen, int screen,
                         int64_t *forces);
+/* Batched ensemble: nsys independent systems in one context.  Arrays are the
+ * systems' MERCURY-layout arrays concatenated (system s has n[s] bodies, its
+ * first the central body, nbig[s] Big bodies including it); h[s] and rcen[s]
+ * are per system.  Every launch of the advance covers all systems (planet
+ * tables per system, workgroups never spanning two systems).  Replaces the
+ * single-system state; free/configure/upload as for one system. */
+int mercury_hip_ensemble_upload(int nsys, const int *n, const int *nbig, const double *m,
+                                const double *x, const double *v, const double *rce,
+                                const double *rphys, const double *h, const double *rcen);
+/* nsteps MVS steps of every system of the ensemble from `time`. */
+int mercury_hip_ensemble_advance(double time, int nsteps, int screen, int64_t *forces);
+/* States of all systems, concatenated as uploaded. */
+int mercury_hip_ensemble_download(double *x, double *v);
 /* Events of the last mercury_hip_advance / ensemble advance, sorted by
  * (system, step, min index, max index). */
 int mercury_hip_advance_events(const MercuryHipEvent **events, int *count);

@@ -94,10 +94,13 @@ struct Context {
     std::vector<DevEvent> hev;
     std::vector<MercuryEvent> events;
     std::vector<MercuryHipEvent> adv_events;
-    Sys sys{};                       // host copy of the single-system descriptor
+    struct HostSys { int base, n, nbig; double mu, h, rcen, j2, j4, j6; };
+    std::vector<HostSys> systems;    // one entry, or the members of an ensemble
+    std::vector<Sys> hsys;           // device descriptors last uploaded
     bool sys_valid = false;
-    double mu = 0;
+    int sys_cap = 0;
     double* tcur = nullptr;          // device per-system time
+    int npart = 0;                   // particles over all systems
     hipStream_t stream = nullptr;
     int steps_per_launch = 32;
     int block = 128;
@@ -133,7 +136,7 @@ void release_all() {
     Context& c = ctx;
     for (double** p : {&c.x, &c.v, &c.a, &c.m, &c.rce, &c.rphys, &c.ox, &c.ov, &c.v0k, &c.v1, &c.tab})
         dfree(*p);
-    dfree(c.fault); dfree(c.dsys); dfree(c.tcur); c.sys_valid = false; dfree(c.dtiles); dfree(c.dev); dfree(c.devcount);
+    dfree(c.fault); dfree(c.dsys); dfree(c.tcur); c.sys_valid = false; c.sys_cap = 0; c.systems.clear(); dfree(c.dtiles); dfree(c.dev); dfree(c.devcount);
     dfree(c.dpark); dfree(c.dcnt); dfree(c.dslow); dfree(c.work); dfree(c.dreq); c.req_cap = 0;
     c.cap = 0; c.tab_cap = 0; c.ev_cap = 0; c.park_cap = 0; c.ntiles = 0;
     c.n = 0; c.nbig = 0; c.history = false; c.saved_valid = false; c.have_state = false;
@@ -156,8 +159,6 @@ void ensure_capacity(int n) {
     for (double** p : {&ctx.x, &ctx.v, &ctx.a, &ctx.ox, &ctx.ov, &ctx.v0k, &ctx.v1}) dalloc(*p, 3 * cap);
     for (double** p : {&ctx.m, &ctx.rce, &ctx.rphys}) dalloc(*p, cap);
     if (!ctx.fault) dalloc(ctx.fault, 1);
-    if (!ctx.dsys) dalloc(ctx.dsys, 1);
-    if (!ctx.tcur) dalloc(ctx.tcur, 1);
     ctx.park_cap = (int)std::min<long long>(cap, 1LL << 30);
     dalloc(ctx.dpark, (size_t)ctx.park_cap);
     if (!ctx.dcnt) dalloc(ctx.dcnt, 1 + 2 * kRing);
@@ -186,16 +187,30 @@ void ensure_events(int count) {
     ctx.ev_cap = cap;
 }
 
-// Tiles of one workgroup over the particles of the single system.
+void ensure_systems(int nsys) {
+    if (nsys <= ctx.sys_cap) return;
+    dalloc(ctx.dsys, (size_t)nsys);
+    dalloc(ctx.tcur, (size_t)nsys);
+    ctx.sys_cap = nsys;
+    ctx.sys_valid = false;
+}
+
+// Workgroup tiles over the particles of every system; a tile never spans two
+// systems, so each workgroup reads one system's descriptor and table.
 void build_tiles() {
-    int npart = ctx.n - ctx.nbig;
     int b = ctx.block;
-    int nt = npart > 0 ? (npart + b - 1) / b : 0;
-    std::vector<int2> t(nt);
-    for (int k = 0; k < nt; k++) t[k] = make_int2(0, ctx.nbig + k * b);
-    dalloc(ctx.dtiles, (size_t)std::max(nt, 1));
-    if (nt) HIPCHECK(hipMemcpy(ctx.dtiles, t.data(), sizeof(int2) * nt, hipMemcpyHostToDevice));
-    ctx.ntiles = nt;
+    std::vector<int2> t;
+    int npart = 0;
+    for (size_t s = 0; s < ctx.systems.size(); s++) {
+        const auto& hs = ctx.systems[s];
+        for (int j = hs.base + hs.nbig; j < hs.base + hs.n; j += b) t.push_back(make_int2((int)s, j));
+        npart += hs.n - hs.nbig;
+    }
+    dalloc(ctx.dtiles, std::max<size_t>(t.size(), 1));
+    if (!t.empty())
+        HIPCHECK(hipMemcpy(ctx.dtiles, t.data(), sizeof(int2) * t.size(), hipMemcpyHostToDevice));
+    ctx.ntiles = (int)t.size();
+    ctx.npart = npart;
 }
 
 // Fortran x(3,n) -> device SoA [3][cap]
@@ -216,7 +231,7 @@ void download_vec3(const double* src, double* dst) {
         HIPCHECK(hipMemcpy(ctx.stage.data() + (size_t)k * n, src + k * ctx.cap, sizeof(double) * n,
                            hipMemcpyDeviceToHost));
     for (int j = 0; j < n; j++)
-        for (int k = 0; k < 3; k++) dst[3 * j + k] = j == 0 ? 0.0 : ctx.stage[(size_t)k * n + j];
+        for (int k = 0; k < 3; k++) dst[3 * j + k] = ctx.stage[(size_t)k * n + j];
 }
 
 void timed_launch_begin(hipStream_t st, hipEvent_t& start) {
@@ -233,26 +248,45 @@ void timed_launch_end(int kind, hipStream_t st, hipEvent_t start) {
 
 enum { K_PARTICLE = 0, K_PLANET = 1, K_GENERAL = 2 };
 
-// Descriptor of the single system for step h and launches of at most smax steps.
-void build_sys(double h, int smax, double rcen) {
+// Device descriptors of every system for launches of at most smax steps.
+// A single uploaded system takes h and rcen from the call (h_call > 0 flag);
+// ensemble members carry their own.  Table block of one launch: every
+// system's step tables, then every system's start table.
+void build_systems(int smax, bool use_call, double h_call, double rcen_call) {
     Context& c = ctx;
-    int np = std::max(c.nbig - 1, 0);
-    Sys d{};
-    d.base = 0; d.n = c.n; d.nbig = c.nbig; d.ts = table_stride(np);
-    d.tab = 0; d.start = (long long)d.ts * smax;
-    d.mu = c.mu; d.h = h; d.hby2 = .5 * h; d.rcen = rcen;
-    d.j2 = c.jcen[0]; d.j4 = c.jcen[1]; d.j6 = c.jcen[2];
-    d.obl = (c.jcen[0] != 0 || c.jcen[1] != 0 || c.jcen[2] != 0);
-    c.slot_doubles = (d.start + start_stride(np) + 63) / 64 * 64;
+    const int nsys = (int)c.systems.size();
+    std::vector<Sys> ds(nsys);
+    long long off = 0;
+    for (int s = 0; s < nsys; s++) {
+        const auto& hs = c.systems[s];
+        int np = std::max(hs.nbig - 1, 0);
+        Sys d{};
+        d.base = hs.base; d.n = hs.n; d.nbig = hs.nbig; d.ts = table_stride(np);
+        d.tab = off;
+        off += (long long)d.ts * smax;
+        d.mu = hs.mu;
+        d.h = use_call ? h_call : hs.h;
+        d.hby2 = .5 * d.h;
+        d.rcen = use_call ? rcen_call : hs.rcen;
+        d.j2 = hs.j2; d.j4 = hs.j4; d.j6 = hs.j6;
+        d.obl = (hs.j2 != 0 || hs.j4 != 0 || hs.j6 != 0);
+        ds[s] = d;
+    }
+    for (int s = 0; s < nsys; s++) {
+        ds[s].start = off;
+        off += start_stride(std::max(c.systems[s].nbig - 1, 0));
+    }
+    c.slot_doubles = (off + 63) / 64 * 64;
     ensure_table(c.slot_doubles * kRing);
-    if (c.sys_valid && std::memcmp(&d, &c.sys, sizeof(Sys)) == 0) return;
-    c.sys = d;
-    HIPCHECK(hipMemcpy(c.dsys, &c.sys, sizeof(Sys), hipMemcpyHostToDevice));
+    ensure_systems(nsys);
+    if (c.sys_valid && c.hsys.size() == ds.size() &&
+        std::memcmp(c.hsys.data(), ds.data(), sizeof(Sys) * ds.size()) == 0)
+        return;
+    c.hsys = ds;
+    HIPCHECK(hipMemcpy(c.dsys, c.hsys.data(), sizeof(Sys) * ds.size(), hipMemcpyHostToDevice));
     c.sys_valid = true;
 }
 
-// Fast kernel specialised on the planet count for the fused (non-saving)
-// launches of systems with up to 12 planets; generic otherwise.
 template <bool SCREEN, int NP>
 bool launch_fast_np(int np, int occ, dim3 grid, dim3 block, hipStream_t st, const ParticleArgs& args) {
     if constexpr (NP > 12) {
@@ -280,12 +314,17 @@ void launch_fast(bool screen, bool save, int np, dim3 grid, dim3 block, hipStrea
     }
     HIPCHECK(hipGetLastError());
 }
-void launch_general(bool screen, bool save, dim3 grid, dim3 block, hipStream_t st, const ParticleArgs& args) {
-    // a single system: uniform variant
-    if (screen && save) hipLaunchKernelGGL((mvs_particles_general<true, true, true>), grid, block, 0, st, args);
-    else if (screen) hipLaunchKernelGGL((mvs_particles_general<true, false, true>), grid, block, 0, st, args);
-    else if (save) hipLaunchKernelGGL((mvs_particles_general<false, true, true>), grid, block, 0, st, args);
-    else hipLaunchKernelGGL((mvs_particles_general<false, false, true>), grid, block, 0, st, args);
+void launch_general(bool screen, bool save, bool uniform, dim3 grid, dim3 block, hipStream_t st,
+                    const ParticleArgs& args) {
+    if (uniform) {   // one system: wave-uniform table reads
+        if (screen && save) hipLaunchKernelGGL((mvs_particles_general<true, true, true>), grid, block, 0, st, args);
+        else if (screen) hipLaunchKernelGGL((mvs_particles_general<true, false, true>), grid, block, 0, st, args);
+        else if (save) hipLaunchKernelGGL((mvs_particles_general<false, true, true>), grid, block, 0, st, args);
+        else hipLaunchKernelGGL((mvs_particles_general<false, false, true>), grid, block, 0, st, args);
+    } else {
+        if (screen) hipLaunchKernelGGL((mvs_particles_general<true, false, false>), grid, block, 0, st, args);
+        else hipLaunchKernelGGL((mvs_particles_general<false, false, false>), grid, block, 0, st, args);
+    }
     HIPCHECK(hipGetLastError());
 }
 
@@ -317,7 +356,10 @@ std::vector<int> chunk_plan(int nsteps, int smax) {
 void mvs_run(double time, double h, int nsteps, bool reset, bool screen, bool save, double rcen) {
     Context& c = ctx;
     const int smax = save ? 1 : c.steps_per_launch;
-    build_sys(h, smax, rcen);
+    const int nsys = (int)c.systems.size();
+    const bool single = nsys == 1;
+    if (save && !single) throw std::runtime_error("single-step mode needs one system");
+    build_systems(smax, single, h, rcen);
     State st = device_state();
     EventSink es{c.dev, c.devcount, c.ev_cap, c.fault};
     const long long nb = c.n;
@@ -329,7 +371,7 @@ void mvs_run(double time, double h, int nsteps, bool reset, bool screen, bool sa
     HIPCHECK(hipEventRecord(c.ev_fast[0], c.stream));
     HIPCHECK(hipStreamWaitEvent(c.sp, c.ev_fast[0], 0));
     HIPCHECK(hipStreamWaitEvent(c.sg, c.ev_fast[0], 0));
-    const int npart = c.n - c.nbig;
+    const int npart = c.npart;
     const int gblocks = std::max(1, std::min((npart + c.gen_per_wave - 1) / c.gen_per_wave, 4096));
     const int rblocks = std::max(1, std::min((c.req_cap + 255) / 256, 1024));
     double t = time;
@@ -344,10 +386,11 @@ void mvs_run(double time, double h, int nsteps, bool reset, bool screen, bool sa
         int* reqcount = c.dcnt + 1 + kRing + slot;
         hipEvent_t t0 = nullptr;
         if (k >= kRing) HIPCHECK(hipStreamWaitEvent(c.sp, c.ev_done[slot], 0));
-        PlanetArgs pa{c.dsys, st, tab, c.tcur, es, t, 1, steps, rs0 ? 1 : 0, screen ? 1 : 0, save ? 1 : 0,
-                      reqcount, 1, done};
+        // one system follows the caller's clock; ensemble members carry theirs on the device
+        PlanetArgs pa{c.dsys, st, tab, c.tcur, es, t, (single || k == 0) ? 1 : 0, steps, rs0 ? 1 : 0,
+                      screen ? 1 : 0, save ? 1 : 0, reqcount, 1, done};
         timed_launch_begin(c.sp, t0);
-        hipLaunchKernelGGL(mvs_planets, dim3(1), dim3(64), 0, c.sp, pa);
+        hipLaunchKernelGGL(mvs_planets, dim3(nsys), dim3(64), 0, c.sp, pa);
         HIPCHECK(hipGetLastError());
         timed_launch_end(K_PLANET, c.sp, t0);
         HIPCHECK(hipEventRecord(c.ev_plan[slot], c.sp));
@@ -358,13 +401,14 @@ void mvs_run(double time, double h, int nsteps, bool reset, bool screen, bool sa
                               steps, rs0 ? 1 : 0, screen ? 1 : 0, save ? 1 : 0, done};
             HIPCHECK(hipStreamWaitEvent(c.stream, c.ev_plan[slot], 0));
             timed_launch_begin(c.stream, t0);
-            launch_fast(screen, save, c.nbig - 1, dim3(c.ntiles), dim3(c.block), c.stream, args);
+            launch_fast(screen, save, single ? c.systems[0].nbig - 1 : -1, dim3(c.ntiles), dim3(c.block), c.stream,
+                        args);
             timed_launch_end(K_PARTICLE, c.stream, t0);
             hipLaunchKernelGGL(copy_int, dim3(1), dim3(64), 0, c.stream, c.dcnt, snap);
             HIPCHECK(hipEventRecord(c.ev_fast[slot], c.stream));
             HIPCHECK(hipStreamWaitEvent(c.sg, c.ev_fast[slot], 0));
             timed_launch_begin(c.sg, t0);
-            launch_general(screen, save, dim3(gblocks), dim3(64), c.sg, args);
+            launch_general(screen, save, single, dim3(gblocks), dim3(64), c.sg, args);
             timed_launch_end(K_GENERAL, c.sg, t0);
             if (screen) {
                 RequestArgs ra{c.dsys, st, tab, es, rs, done};
@@ -472,9 +516,10 @@ int mercury_hip_upload(int n, int nbig, int pn, int ngflag, const double* m, con
             if (!std::isfinite(x[j]) || !std::isfinite(v[j])) throw std::runtime_error("nonfinite state");
         ensure_stream();
         ensure_capacity(n);
-        ctx.n = n; ctx.nbig = nbig; ctx.pn = pn; ctx.ngflag = ngflag; ctx.mu = m[0];
-        ctx.sys_valid = false;
+        ctx.n = n; ctx.nbig = nbig; ctx.pn = pn; ctx.ngflag = ngflag;
         for (int k = 0; k < 3; k++) ctx.jcen[k] = jcen[k];
+        ctx.systems.assign(1, Context::HostSys{0, n, nbig, m[0], 0.0, 0.0, jcen[0], jcen[1], jcen[2]});
+        ctx.sys_valid = false;
         HIPCHECK(hipMemcpy(ctx.m, m, sizeof(double) * n, hipMemcpyHostToDevice));
         HIPCHECK(hipMemcpy(ctx.rce, rce, sizeof(double) * n, hipMemcpyHostToDevice));
         HIPCHECK(hipMemcpy(ctx.rphys, rphys, sizeof(double) * n, hipMemcpyHostToDevice));
@@ -593,6 +638,7 @@ int mercury_hip_advance(double time, double h, int nsteps, double rcen, int scre
         require_state();
         if (!mvs_family()) throw std::runtime_error("advance needs MVS");
         if (nsteps < 0) throw std::runtime_error("negative step count");
+        if (ctx.systems.size() == 1 && (!(h != 0) || !std::isfinite(h))) throw std::runtime_error("invalid step");
         ensure_stream();
         clear_fault();
         HIPCHECK(hipMemset(ctx.devcount, 0, sizeof(int)));
@@ -625,6 +671,53 @@ int mercury_hip_advance(double time, double h, int nsteps, double rcen, int scre
         return 0;
     } catch (const std::exception& e) { return fail(e); }
 }
+
+int mercury_hip_ensemble_upload(int nsys, const int* n, const int* nbig, const double* m, const double* x,
+                                const double* v, const double* rce, const double* rphys, const double* h,
+                                const double* rcen) {
+    try {
+        if (nsys < 1) throw std::runtime_error("no systems");
+        std::vector<Context::HostSys> sys(nsys);
+        long long total = 0;
+        for (int s = 0; s < nsys; s++) {
+            if (n[s] < 1 || nbig[s] < 1 || nbig[s] > n[s]) throw std::runtime_error("invalid body counts");
+            if (nbig[s] > kMaxBig) throw std::runtime_error("more Big bodies than the HIP path supports (64)");
+            if (!(m[total] > 0)) throw std::runtime_error("central mass must be positive");
+            if (!(h[s] != 0) || !std::isfinite(h[s])) throw std::runtime_error("invalid step");
+            sys[s] = Context::HostSys{(int)total, n[s], nbig[s], m[total], h[s], rcen[s], 0.0, 0.0, 0.0};
+            total += n[s];
+        }
+        if (total > (1LL << 30)) throw std::runtime_error("too many bodies");
+        for (long long j = 0; j < 3 * total; j++)
+            if (!std::isfinite(x[j]) || !std::isfinite(v[j])) throw std::runtime_error("nonfinite state");
+        if (ctx.algorithm != 1 && ctx.algorithm != 9) mercury_hip_configure(1);
+        ensure_stream();
+        ensure_capacity((int)total);
+        ctx.n = (int)total; ctx.nbig = 0; ctx.pn = 0; ctx.ngflag = 0;
+        for (int k = 0; k < 3; k++) ctx.jcen[k] = 0.0;
+        ctx.systems = sys;
+        ctx.sys_valid = false;
+        HIPCHECK(hipMemcpy(ctx.m, m, sizeof(double) * total, hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(ctx.rce, rce, sizeof(double) * total, hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(ctx.rphys, rphys, sizeof(double) * total, hipMemcpyHostToDevice));
+        upload_vec3(x, ctx.x);
+        upload_vec3(v, ctx.v);
+        HIPCHECK(hipMemset(ctx.a, 0, sizeof(double) * 3 * ctx.cap));
+        HIPCHECK(hipMemset(ctx.fault, 0, sizeof(int)));
+        build_tiles();
+        ctx.history = false;
+        ctx.saved_valid = false;
+        ctx.have_state = true;
+        HIPCHECK(hipDeviceSynchronize());
+        return 0;
+    } catch (const std::exception& e) { return fail(e); }
+}
+
+int mercury_hip_ensemble_advance(double time, int nsteps, int screen, int64_t* forces) {
+    return mercury_hip_advance(time, 0.0, nsteps, 0.0, screen, forces);
+}
+
+int mercury_hip_ensemble_download(double* x, double* v) { return mercury_hip_download(x, v, 0); }
 
 int mercury_hip_advance_events(const MercuryHipEvent** events, int* count) {
     *events = ctx.adv_events.data();

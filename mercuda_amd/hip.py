@@ -62,6 +62,9 @@ SIGNATURES = {
     "mercury_hip_timing_read": (C.c_int, [_dp, _ip, _dp, _ip, _dp, _ip]),
     "mercury_hip_parked": (C.c_int64, []),
     "mercury_hip_work_read": (C.c_int, [_i64p, _i64p]),
+    "mercury_hip_ensemble_upload": (C.c_int, [C.c_int, _ip, _ip, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
+    "mercury_hip_ensemble_advance": (C.c_int, [C.c_double, C.c_int, C.c_int, _i64p]),
+    "mercury_hip_ensemble_download": (C.c_int, [_dp, _dp]),
 }
 
 
@@ -189,6 +192,35 @@ class MercuryHip:
 
     def parked(self) -> int:
         return int(self.L.mercury_hip_parked())
+
+    # ---- batched ensembles
+    def ensemble_upload(self, systems_list):
+        """Upload a list of system dicts (mercuda_amd.systems) as one ensemble."""
+        ns = len(systems_list)
+        n = np.array([len(q["m"]) for q in systems_list], np.int32)
+        nbig = np.array([q["nbig"] for q in systems_list], np.int32)
+        cat = lambda key, shape: np.ascontiguousarray(np.concatenate([np.asarray(q[key], np.float64).reshape(shape)
+                                                                      for q in systems_list]))
+        m = cat("m", (-1,)); x = cat("x", (-1, 3)); v = cat("v", (-1, 3))
+        rce = cat("rce", (-1,)); rphys = cat("rphys", (-1,))
+        h = np.array([q["h"] for q in systems_list], np.float64)
+        rcen = np.array([q["rcen"] for q in systems_list], np.float64)
+        ip = lambda a: a.ctypes.data_as(_ip)
+        self._ok(self.L.mercury_hip_ensemble_upload(ns, ip(n), ip(nbig), _d(m), _d(x), _d(v), _d(rce), _d(rphys),
+                                                    _d(h), _d(rcen)), "ensemble_upload")
+        self.n = int(n.sum()); self.nbig = 0
+        self.ens_offsets = np.concatenate([[0], np.cumsum(n)])
+
+    def ensemble_advance(self, time, nsteps, screen=True):
+        nf = C.c_int64(0)
+        self._ok(self.L.mercury_hip_ensemble_advance(time, nsteps, int(screen), C.byref(nf)), "ensemble_advance")
+        return nf.value
+
+    def ensemble_download(self):
+        x = np.zeros((self.n, 3)); v = np.zeros((self.n, 3))
+        self._ok(self.L.mercury_hip_ensemble_download(_d(x), _d(v)), "ensemble_download")
+        o = self.ens_offsets
+        return [(x[o[k]:o[k + 1]], v[o[k]:o[k + 1]]) for k in range(len(o) - 1)]
 
     def free(self):
         self.L.mercury_hip_free()

@@ -34,6 +34,8 @@ def kepler_E(M, e):
     for _ in range(60):
         f = E - e * np.sin(E) - M
         E = E - f / (1.0 - e * np.cos(E))
+        if f.size == 0 or np.max(np.abs(f)) < 1e-15:
+            break
     return E
 
 
@@ -70,22 +72,21 @@ def osculating_a(gm, x, v):
 
 
 def draw_planets(rng, nplanets, mlo=1e-7, mhi=1e-3, alo=0.4, ahi=30.0, emax=0.05,
-                 imax_deg=2.0, hill_sep=10.0, max_tries=100000):
+                 imax_deg=2.0, hill_sep=10.0, max_tries=2000, batch=512):
     """Planet masses (solar) and elements, redrawn until every pair is at least
-    hill_sep mutual Hill radii apart."""
+    hill_sep mutual Hill radii apart (candidate sets are drawn in batches and
+    the first acceptable one is kept)."""
     for _ in range(max_tries):
-        m = 10.0 ** rng.uniform(math.log10(mlo), math.log10(mhi), nplanets)
-        a = 10.0 ** rng.uniform(math.log10(alo), math.log10(ahi), nplanets)
-        ok = True
+        m = 10.0 ** rng.uniform(math.log10(mlo), math.log10(mhi), (batch, nplanets))
+        a = 10.0 ** rng.uniform(math.log10(alo), math.log10(ahi), (batch, nplanets))
+        ok = np.ones(batch, bool)
         for i in range(nplanets):
             for j in range(i + 1, nplanets):
-                rh = ((m[i] + m[j]) / 3.0) ** (1.0 / 3.0) * (a[i] + a[j]) / 2.0
-                if abs(a[i] - a[j]) < hill_sep * rh:
-                    ok = False
-                    break
-            if not ok:
-                break
-        if ok:
+                rh = ((m[:, i] + m[:, j]) / 3.0) ** (1.0 / 3.0) * (a[:, i] + a[:, j]) / 2.0
+                ok &= np.abs(a[:, i] - a[:, j]) >= hill_sep * rh
+        idx = np.flatnonzero(ok)
+        if len(idx):
+            m, a = m[idx[0]], a[idx[0]]
             order = np.argsort(a)
             m, a = m[order], a[order]
             e = rng.uniform(0.0, emax, nplanets)
