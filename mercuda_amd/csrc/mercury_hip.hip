@@ -70,6 +70,7 @@ struct Context {
     // device state
     double *x = nullptr, *v = nullptr, *a = nullptr, *m = nullptr, *rce = nullptr, *rphys = nullptr;
     double *ox = nullptr, *ov = nullptr, *v0k = nullptr, *v1 = nullptr;
+    double *cx = nullptr, *cv = nullptr;   // corrector output (allocated on first export)
     int* fault = nullptr;
     double* tab = nullptr;
     long long tab_cap = 0;           // doubles
@@ -134,7 +135,7 @@ void sync_all() {
 
 void release_all() {
     Context& c = ctx;
-    for (double** p : {&c.x, &c.v, &c.a, &c.m, &c.rce, &c.rphys, &c.ox, &c.ov, &c.v0k, &c.v1, &c.tab})
+    for (double** p : {&c.x, &c.v, &c.a, &c.m, &c.rce, &c.rphys, &c.ox, &c.ov, &c.v0k, &c.v1, &c.tab, &c.cx, &c.cv})
         dfree(*p);
     dfree(c.fault); dfree(c.dsys); dfree(c.tcur); c.sys_valid = false; c.sys_cap = 0; c.systems.clear(); dfree(c.dtiles); dfree(c.dev); dfree(c.devcount);
     dfree(c.dpark); dfree(c.dcnt); dfree(c.dslow); dfree(c.work); dfree(c.dreq); c.req_cap = 0;
@@ -157,6 +158,8 @@ void ensure_capacity(int n) {
     HIPCHECK(hipMemGetInfo(&free_b, &total_b));
     if ((size_t)cap * 8 * 40 > free_b) throw std::runtime_error("insufficient device memory");
     for (double** p : {&ctx.x, &ctx.v, &ctx.a, &ctx.ox, &ctx.ov, &ctx.v0k, &ctx.v1}) dalloc(*p, 3 * cap);
+    dfree(ctx.cx);
+    dfree(ctx.cv);
     for (double** p : {&ctx.m, &ctx.rce, &ctx.rphys}) dalloc(*p, cap);
     if (!ctx.fault) dalloc(ctx.fault, 1);
     ctx.park_cap = (int)std::min<long long>(cap, 1LL << 30);
@@ -602,8 +605,25 @@ int mercury_hip_force(double* a) {
 int mercury_hip_export(double h, int physical, double* x, double* v) {
     try {
         require_state();
-        (void)h;
-        if (physical && ctx.algorithm == 1) throw std::runtime_error("symplectic corrector not available yet");
+        if (physical && ctx.algorithm == 1) {
+            // mco_mvs2h on scratch (mercury6_2.for:2107-2219); the state is untouched
+            if (ctx.systems.size() != 1) throw std::runtime_error("corrector needs one system");
+            ensure_stream();
+            build_systems(std::max(ctx.steps_per_launch, 4), true, h, 0.0);
+            if (!ctx.cx) { dalloc(ctx.cx, 3 * ctx.cap); dalloc(ctx.cv, 3 * ctx.cap); }
+            double rt10 = std::sqrt(10.0);
+            CorrectorArgs ca{ctx.dsys, device_state(), ctx.tab, ctx.cx, ctx.cv,
+                             {h * rt10 * 3.0 / 10.0, h * rt10 / 5.0}, {-h * rt10 / 72.0, h * rt10 / 24.0}};
+            hipLaunchKernelGGL(corrector_planets, dim3(1), dim3(64), 0, ctx.stream, ca);
+            int npart = ctx.n - ctx.nbig;
+            if (npart > 0)
+                hipLaunchKernelGGL(corrector_particles, dim3((npart + 127) / 128), dim3(128), 0, ctx.stream, ca);
+            HIPCHECK(hipGetLastError());
+            HIPCHECK(hipStreamSynchronize(ctx.stream));
+            download_vec3(ctx.cx, x);
+            download_vec3(ctx.cv, v);
+            return 0;
+        }
         download_vec3(ctx.x, x);
         download_vec3(ctx.v, v);
         return 0;

@@ -696,6 +696,120 @@ __global__ void __launch_bounds__(64) mvs_planets(PlanetArgs args) {
 }
 
 // ---------------------------------------------------------------------------
+// Symplectic corrector mco_mvs2h (mercury6_2.for:2107-2219) on a copy of the
+// state: for k = 1, 2: drift ha(k), kick -hb(k), drift -2 ha(k), kick +hb(k),
+// drift ha(k).  The planet wave writes the four force points' tables (header
+// and force block) and the corrected planet states; the particle kernel then
+// applies the same sequence to every particle.
+struct CorrectorArgs {
+    const Sys* sys;
+    State st;
+    double* tab;       // 4 tables of stride Sys::ts
+    double* xo;        // corrected positions [3][cap]
+    double* vo;        // corrected velocities [3][cap]
+    double ha[2], hb[2];
+};
+
+__global__ void __launch_bounds__(64) corrector_planets(CorrectorArgs args) {
+    const Sys d = args.sys[0];
+    const int np = d.nbig - 1;
+    const int lane = threadIdx.x;
+    __shared__ double m[kMaxBig], minside[kMaxBig], jdx[kMaxBig], jhx[kMaxBig], gm[kMaxBig];
+    __shared__ double X[3 * kMaxBig], V[3 * kMaxBig], A[3 * kMaxBig], XJ[3 * kMaxBig], VJ[3 * kMaxBig];
+    __shared__ double t0[3 * kMaxBig], t2[3 * kMaxBig], a1[3 * kMaxBig], aob[3 * kMaxBig], hdr[8];
+    const long long cap = args.st.cap;
+    const int b0 = d.base;
+    for (int q = lane; q < d.nbig; q += 64) m[q] = args.st.m[b0 + q];
+    for (int p = lane; p < np; p += 64)
+        for (int c = 0; c < 3; c++) {
+            X[3 * p + c] = args.st.x[c * cap + b0 + 1 + p];
+            V[3 * p + c] = args.st.v[c * cap + b0 + 1 + p];
+        }
+    __syncthreads();
+    if (lane == 0) {
+        double ms = 0.0;
+        for (int p = 0; p < np; p++) { minside[p] = ms; ms = ms + m[p + 1]; }
+        double mtot = (np >= 1) ? m[1] : 0.0;
+        for (int p = 1; p < np; p++) { jdx[p] = 1.0 / (mtot + m[0]); mtot = mtot + m[p + 1]; }
+        mtot = (np >= 1) ? m[1] : 0.0;
+        if (np >= 1) jhx[0] = m[1] / (mtot + m[0]);
+        for (int p = 1; p < np; p++) { mtot = mtot + m[p + 1]; jhx[p] = m[p + 1] / (mtot + m[0]); }
+        double mins = m[0];
+        for (int p = 0; p < np; p++) {
+            double msofar = mins + m[p + 1];
+            gm[p] = m[0] * msofar / mins;
+            mins = msofar;
+        }
+    }
+    __syncthreads();
+    int fp = 0;   // force point 0..3
+    for (int k = 0; k < 2; k++) {
+        const double dts[3] = {args.ha[k], -2.0 * args.ha[k], args.ha[k]};
+        for (int phase = 0; phase < 3; phase++) {
+            if (lane == 0) h2j_serial(np, m, jdx, X, V, XJ, VJ);
+            __syncthreads();
+            if (lane < np) {
+                Orbit o{XJ[3 * lane], XJ[3 * lane + 1], XJ[3 * lane + 2], VJ[3 * lane], VJ[3 * lane + 1],
+                        VJ[3 * lane + 2]};
+                drift_one(gm[lane], o, dts[phase]);
+                XJ[3 * lane] = o.x; XJ[3 * lane + 1] = o.y; XJ[3 * lane + 2] = o.z;
+                VJ[3 * lane] = o.vx; VJ[3 * lane + 1] = o.vy; VJ[3 * lane + 2] = o.vz;
+            }
+            __syncthreads();
+            if (lane == 0) j2h_serial(np, jhx, XJ, VJ, X, V);
+            __syncthreads();
+            if (phase == 2) break;
+            big_forces(d, np, m, minside, X, XJ, A, t0, t2, a1, aob, hdr);
+            double* T = args.tab + (long long)fp * d.ts;
+            if (lane < 8) T[lane] = (lane < 6) ? hdr[lane] : 0.0;
+            for (int p = lane; p < np; p += 64) {
+                for (int c = 0; c < 3; c++) T[kHdr + kF * p + c] = X[3 * p + c];
+                T[kHdr + kF * p + 3] = m[p + 1];
+            }
+            if (lane < np)
+                for (int c = 0; c < 3; c++) {
+                    if (phase == 0) V[3 * lane + c] = V[3 * lane + c] - args.hb[k] * A[3 * lane + c];
+                    else V[3 * lane + c] = V[3 * lane + c] + args.hb[k] * A[3 * lane + c];
+                }
+            __syncthreads();
+            fp++;
+        }
+    }
+    for (int p = lane; p < np; p += 64)
+        for (int c = 0; c < 3; c++) {
+            args.xo[c * cap + b0 + 1 + p] = X[3 * p + c];
+            args.vo[c * cap + b0 + 1 + p] = V[3 * p + c];
+        }
+    if (lane < 3) { args.xo[lane * cap + b0] = 0.0; args.vo[lane * cap + b0] = 0.0; }
+}
+
+__global__ void __launch_bounds__(128) corrector_particles(CorrectorArgs args) {
+    const Sys d = args.sys[0];
+    const int jb = d.nbig + blockIdx.x * blockDim.x + threadIdx.x;
+    if (jb >= d.n) return;
+    const long long cap = args.st.cap;
+    const int g = d.base + jb;
+    const int np = d.nbig - 1;
+    Orbit o{args.st.x[g], args.st.x[cap + g], args.st.x[2 * cap + g],
+            args.st.v[g], args.st.v[cap + g], args.st.v[2 * cap + g]};
+    int fp = 0;
+    for (int k = 0; k < 2; k++) {
+        drift_one(d.mu, o, args.ha[k]);
+        double ax, ay, az;
+        particle_accel(d, args.tab + (long long)fp * d.ts, np, o.x, o.y, o.z, ax, ay, az);
+        o.vx = o.vx - args.hb[k] * ax; o.vy = o.vy - args.hb[k] * ay; o.vz = o.vz - args.hb[k] * az;
+        fp++;
+        drift_one(d.mu, o, -2.0 * args.ha[k]);
+        particle_accel(d, args.tab + (long long)fp * d.ts, np, o.x, o.y, o.z, ax, ay, az);
+        o.vx = o.vx + args.hb[k] * ax; o.vy = o.vy + args.hb[k] * ay; o.vz = o.vz + args.hb[k] * az;
+        fp++;
+        drift_one(d.mu, o, args.ha[k]);
+    }
+    args.xo[g] = o.x; args.xo[cap + g] = o.y; args.xo[2 * cap + g] = o.z;
+    args.vo[g] = o.vx; args.vo[cap + g] = o.vy; args.vo[2 * cap + g] = o.vz;
+}
+
+// ---------------------------------------------------------------------------
 // Particle passes.
 struct ParticleArgs {
     const Sys* sys;

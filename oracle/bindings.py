@@ -103,6 +103,10 @@ class Oracle:
         L.orc_mdt_mvs.argtypes = [C.c_double, C.c_double, C.c_double, _dp, C.c_int, C.c_int,
                                   _dp, _dp, _dp, _dp, _ip, _dp, _dp, C.POINTER(_Events)]
         L.orc_mco_mvs2h.argtypes = [C.c_double, _dp, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp]
+        L.orc_mfo_all.argtypes = [_dp, C.c_int, C.c_int, _dp, _dp, _dp, C.c_int, _dp]
+        L.orc_ra15_run.restype = C.c_long
+        L.orc_ra15_run.argtypes = [C.c_double, C.c_double, C.c_double, _dp, C.c_int, C.c_int, _dp, _dp, _dp,
+                                   C.c_int, C.c_int, _dp, _dp]
         L.orc_mvs_run.restype = C.c_int
         L.orc_mvs_run.argtypes = [C.c_double, C.c_double, C.c_double, _dp, C.c_int, C.c_int,
                                   _dp, _dp, _dp, _dp, _dp, C.c_int, C.c_int, C.c_int, _ip, _dp]
@@ -172,6 +176,27 @@ class Oracle:
             raise RuntimeError("oracle event capacity exceeded")
         return x, v, evi[:k].copy(), evd[:k].copy()
 
+    def mfo_all(self, sysd, x=None, v=None, pn=0):
+        m = np.ascontiguousarray(sysd["m"], np.float64)
+        x = np.ascontiguousarray(sysd["x"] if x is None else x, np.float64)
+        v = np.ascontiguousarray(sysd["v"] if v is None else v, np.float64)
+        jc = np.ascontiguousarray(sysd.get("jcen", (0.0, 0.0, 0.0)), np.float64)
+        a = np.zeros_like(x)
+        self.L.orc_mfo_all(_d(jc), len(m), sysd["nbig"], _d(m), _d(x), _d(v), int(pn), _d(a))
+        return a
+
+    def ra15_run(self, sysd, nsteps, h0=None, tol=1e-12, pn=0, time0=0.0):
+        """nsteps RA15 sequences of a whole system: final x, v, the sizes done,
+        the next proposal and the force-evaluation count."""
+        m = np.ascontiguousarray(sysd["m"], np.float64)
+        x = np.ascontiguousarray(sysd["x"], np.float64).copy()
+        v = np.ascontiguousarray(sysd["v"], np.float64).copy()
+        jc = np.ascontiguousarray(sysd.get("jcen", (0.0, 0.0, 0.0)), np.float64)
+        hdid = np.zeros(max(nsteps, 1)); hn = np.zeros(1)
+        nf = self.L.orc_ra15_run(time0, sysd["h"] if h0 is None else h0, tol, _d(jc), len(m), sysd["nbig"],
+                                 _d(m), _d(x), _d(v), int(pn), nsteps, _d(hdid), _d(hn))
+        return x, v, hdid[:nsteps].copy(), float(hn[0]), int(nf)
+
     def mvs2h(self, sysd, x, v):
         m = np.ascontiguousarray(sysd["m"], np.float64)
         jc = np.ascontiguousarray(sysd.get("jcen", (0.0, 0.0, 0.0)), np.float64)
@@ -231,6 +256,42 @@ class Reference:
         self.L.mfo_mvs_(_d(jc), C.byref(C.c_int(n)), C.byref(C.c_int(nbig)), _d(m), _d(x),
                         _d(xj), _d(a), _i(stat))
         return a
+
+    def mfo_all(self, sysd, x=None, v=None, pn=0):
+        m = np.ascontiguousarray(sysd["m"], np.float64)
+        x = np.ascontiguousarray(sysd["x"] if x is None else x, np.float64)
+        v = np.ascontiguousarray(sysd["v"] if v is None else v, np.float64)
+        n = len(m)
+        jc = np.ascontiguousarray(sysd.get("jcen", (0.0, 0.0, 0.0)), np.float64)
+        a = np.zeros_like(x); s = np.zeros((n, 3)); rc = np.zeros(n); st = np.zeros(n, np.int32)
+        ngf = np.zeros((n, 5)); opt = np.array([0, 1, 1, 2, 0, 1, int(pn), 0], np.int32)
+        self.L.mfo_all_(self._r(0.0), _d(jc), C.byref(C.c_int(n)), C.byref(C.c_int(sysd["nbig"])), _d(m),
+                        _d(x), _d(v), _d(s), _d(rc), _d(a), _i(st), _d(ngf), C.byref(C.c_int(0)), _i(opt),
+                        C.byref(C.c_int(0)), _i(np.zeros(1, np.int32)), _i(np.zeros(1, np.int32)))
+        return a
+
+    def ra15_run(self, sysd, nsteps, h0=None, tol=1e-12, pn=0, time0=0.0):
+        """nsteps calls of the reference mdt_ra15 with mfo_all as the force."""
+        m = np.ascontiguousarray(sysd["m"], np.float64)
+        n = len(m)
+        x = np.ascontiguousarray(sysd["x"], np.float64).copy()
+        v = np.ascontiguousarray(sysd["v"], np.float64).copy()
+        jc = np.ascontiguousarray(sysd.get("jcen", (0.0, 0.0, 0.0)), np.float64)
+        s = np.zeros((n, 3)); rphys = np.zeros(n); rc = np.zeros(n); st = np.zeros(n, np.int32)
+        ngf = np.zeros((n, 5)); opt = np.array([0, 1, 1, 2, 0, 1, int(pn), 0], np.int32)
+        force = C.cast(self.L.mfo_all_, C.c_void_p)
+        t = C.c_double(sysd["h"] if h0 is None else h0); tdid = C.c_double(0.0)
+        dtflag = C.c_int(0)
+        hd = []
+        time = time0
+        for _ in range(nsteps):
+            self.L.mdt_ra15_(self._r(time), C.byref(t), C.byref(tdid), self._r(tol), _d(jc), C.byref(C.c_int(n)),
+                             C.byref(C.c_int(sysd["nbig"])), _d(m), _d(x), _d(v), _d(s), _d(rphys), _d(rc),
+                             _d(ngf), _i(st), C.byref(dtflag), C.byref(C.c_int(0)), _i(opt),
+                             C.byref(C.c_int(0)), _i(np.zeros(1, np.int32)), _i(np.zeros(1, np.int32)), force)
+            hd.append(tdid.value)
+            time = time + tdid.value
+        return x, v, np.array(hd), t.value
 
     def mce_min(self, d0, d1, d0t, d1t, h):
         a = C.c_double(); b = C.c_double()

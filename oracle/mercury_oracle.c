@@ -923,3 +923,335 @@ int orc_mvs_run(double time0, double h0, double rcen, const double *jcen,
     free(ev.ihit); free(ev.jhit); free(ev.chit); free(ev.dhit); free(ev.thit);
     return bad ? -1 : nev;
 }
+
+/* ------------------------------------------------------------------------ */
+/* mfo_grav  mercury6_2.for:4593-4707 */
+static void mfo_grav(int nbod, int nbig, const double *m, const double *x, const double *v,
+                     double *a, int pn)
+{
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    double *r3 = malloc(sizeof(double) * (size_t)(nbod > 0 ? nbod : 1));
+    double *ri = malloc(sizeof(double) * (size_t)(nbod > 0 ? nbod : 1));
+    for (int i = 1; i < nbod; i++) {
+        a[3 * i] = a[3 * i + 1] = a[3 * i + 2] = 0.0;
+        double s2 = x[3 * i] * x[3 * i] + x[3 * i + 1] * x[3 * i + 1] + x[3 * i + 2] * x[3 * i + 2];
+        double s_1 = 1.0 / sqrt(s2);
+        ri[i] = s_1;
+        r3[i] = s_1 * s_1 * s_1;
+    }
+    for (int i = 1; i < nbod; i++) {
+        double tmp1 = m[i] * r3[i];
+        sx = sx - tmp1 * x[3 * i];
+        sy = sy - tmp1 * x[3 * i + 1];
+        sz = sz - tmp1 * x[3 * i + 2];
+    }
+    for (int i = 1; i < nbig; i++) {
+        for (int j = i + 1; j < nbod; j++) {
+            double dx = x[3 * j] - x[3 * i];
+            double dy = x[3 * j + 1] - x[3 * i + 1];
+            double dz = x[3 * j + 2] - x[3 * i + 2];
+            double s2 = dx * dx + dy * dy + dz * dz;
+            double s_1 = 1.0 / sqrt(s2);
+            double s_3 = s_1 * s_1 * s_1;
+            double tmp1 = s_3 * m[i];
+            double tmp2 = s_3 * m[j];
+            a[3 * j] = a[3 * j] - tmp1 * dx;
+            a[3 * j + 1] = a[3 * j + 1] - tmp1 * dy;
+            a[3 * j + 2] = a[3 * j + 2] - tmp1 * dz;
+            a[3 * i] = a[3 * i] + tmp2 * dx;
+            a[3 * i + 1] = a[3 * i + 1] + tmp2 * dy;
+            a[3 * i + 2] = a[3 * i + 2] + tmp2 * dz;
+        }
+    }
+    if (pn == 0) {
+        for (int i = 1; i < nbod; i++) {
+            double tmp1 = m[0] * r3[i];
+            a[3 * i] = a[3 * i] + sx - tmp1 * x[3 * i];
+            a[3 * i + 1] = a[3 * i + 1] + sy - tmp1 * x[3 * i + 1];
+            a[3 * i + 2] = a[3 * i + 2] + sz - tmp1 * x[3 * i + 2];
+        }
+    } else {
+        /* (c in AU/day)**2; AU*.01d0 is the AU in metres */
+        const double cl = 299792458.0 * 86400.0 / (1.495978707e13 * .01);
+        const double light2 = cl * cl;
+        for (int i = 1; i < nbod; i++) {
+            const double *xi = x + 3 * i, *vi = v + 3 * i;
+            double tmp1 = m[0] * r3[i];
+            double rv = xi[0] * vi[0] + xi[1] * vi[1] + xi[2] * vi[2];
+            double v2 = vi[0] * vi[0] + vi[1] * vi[1] + vi[2] * vi[2];
+            double f = tmp1 / light2;
+            double radial = 4.0 * m[0] * ri[i] - v2;
+            a[3 * i] = a[3 * i] + sx - tmp1 * xi[0] + f * (radial * xi[0] + 4.0 * rv * vi[0]);
+            a[3 * i + 1] = a[3 * i + 1] + sy - tmp1 * xi[1] + f * (radial * xi[1] + 4.0 * rv * vi[1]);
+            a[3 * i + 2] = a[3 * i + 2] + sz - tmp1 * xi[2] + f * (radial * xi[2] + 4.0 * rv * vi[2]);
+        }
+    }
+    free(r3);
+    free(ri);
+}
+
+/* mfo_all  mercury6_2.for:4506-4592 (gravity, 1PN and oblateness) */
+void orc_mfo_all(const double *jcen, int nbod, int nbig, const double *m,
+                 const double *x, const double *v, int pn, double *a)
+{
+    mfo_grav(nbod, nbig, m, x, v, a, pn);
+    if (jcen[0] != 0 || jcen[1] != 0 || jcen[2] != 0) {
+        double *acor = calloc((size_t)3 * nbod, sizeof(double));
+        double acen[3];
+        orc_mfo_obl(jcen, nbod, m, x, acor, acen);
+        for (int j = 1; j < nbod; j++)
+            for (int c = 0; c < 3; c++)
+                a[3 * j + c] = a[3 * j + c] + (acor[3 * j + c] - acen[c]);
+        free(acor);
+    }
+}
+
+/* mdt_ra15 constants (mercury6_2.for:4162-4199) */
+static const double RA_H[8] = {0.0, .0562625605369221, .1802406917368924, .3526247171131696,
+                               .5471536263305554, .7342101772154105, .8853209468390958,
+                               .9775206135612875};
+static const double RA_XC[8] = {.5, .1666666666666667, .08333333333333333, .05, .03333333333333333,
+                                .02380952380952381, .01785714285714286, .01388888888888889};
+static const double RA_VC[7] = {.5, .3333333333333333, .25, .2, .1666666666666667,
+                                .1428571428571429, .125};
+
+static void ra_constants(double *r, double *c, double *d)
+{
+    const double *h = RA_H;
+    int n = 0;
+    for (int j = 1; j < 8; j++)
+        for (int k = 0; k < j; k++)
+            r[n++] = 1.0 / (h[j] - h[k]);
+    /* c, d: Fortran 1-based recurrences, stored 0-based */
+    c[0] = -h[1];
+    d[0] = h[1];
+    n = 1;   /* Fortran n */
+    for (int j = 3; j <= 7; j++) {
+        n = n + 1;
+        c[n - 1] = -h[j - 1] * c[n - j + 2 - 1];
+        d[n - 1] = h[1] * d[n - j + 2 - 1];
+        for (int k = 3; k <= j - 1; k++) {
+            n = n + 1;
+            c[n - 1] = c[n - j + 1 - 1] - h[j - 1] * c[n - j + 2 - 1];
+            d[n - 1] = d[n - j + 1 - 1] + h[k - 1] * d[n - j + 2 - 1];
+        }
+        n = n + 1;
+        c[n - 1] = c[n - j + 1 - 1] - h[j - 1];
+        d[n - 1] = d[n - j + 1 - 1] + h[j - 1];
+    }
+}
+
+/* mdt_ra15  mercury6_2.for:4133-4500 */
+long orc_mdt_ra15(double time, double *t, double *tdid, double tol, const double *jcen,
+                  int nbod, int nbig, const double *m, double *x1, double *v1, int pn,
+                  orc_ra15 *st, int *redo)
+{
+    (void)time;
+    const double *h = RA_H;
+    double r[28], c[21], d[21], s[9];
+    ra_constants(r, c, d);
+    if (st->dtflag == 0) st->dtflag = 1;
+    const int nv = 3 * nbod;
+    double *x = calloc((size_t)nv, sizeof(double)), *v = calloc((size_t)nv, sizeof(double));
+    double *a = calloc((size_t)nv, sizeof(double)), *a1 = calloc((size_t)nv, sizeof(double));
+    double *g = calloc((size_t)7 * nv, sizeof(double));
+    double *b = st->b, *e = st->e;
+    long nforce = 0;
+    *redo = 0;
+    for (;;) {
+        int niter;
+        if (st->dtflag == 1) {
+            niter = 6;
+            for (int k = 3; k < nv; k++)
+                for (int q = 0; q < 7; q++) { b[7 * k + q] = 0.0; e[7 * k + q] = 0.0; }
+        } else {
+            niter = 2;
+        }
+        orc_mfo_all(jcen, nbod, nbig, m, x1, v1, pn, a1);
+        nforce++;
+        for (int k = 3; k < nv; k++) {
+            double *bk = b + 7 * k, *gk = g + 7 * k;
+            gk[0] = bk[6] * d[15] + bk[5] * d[10] + bk[4] * d[6] + bk[3] * d[3] + bk[2] * d[1] + bk[1] * d[0] + bk[0];
+            gk[1] = bk[6] * d[16] + bk[5] * d[11] + bk[4] * d[7] + bk[3] * d[4] + bk[2] * d[2] + bk[1];
+            gk[2] = bk[6] * d[17] + bk[5] * d[12] + bk[4] * d[8] + bk[3] * d[5] + bk[2];
+            gk[3] = bk[6] * d[18] + bk[5] * d[13] + bk[4] * d[9] + bk[3];
+            gk[4] = bk[6] * d[19] + bk[5] * d[14] + bk[4];
+            gk[5] = bk[6] * d[20] + bk[5];
+            gk[6] = bk[6];
+        }
+        const int vpred = pn != 0;
+        for (int n = 1; n <= niter; n++) {
+            for (int j = 2; j <= 8; j++) {
+                double hj = h[j - 1];
+                s[0] = *t * hj;
+                s[1] = s[0] * s[0] * .5;
+                s[2] = s[1] * hj * .3333333333333333;
+                s[3] = s[2] * hj * .5;
+                s[4] = s[3] * hj * .6;
+                s[5] = s[4] * hj * .6666666666666667;
+                s[6] = s[5] * hj * .7142857142857143;
+                s[7] = s[6] * hj * .75;
+                s[8] = s[7] * hj * .7777777777777778;
+                for (int k = 3; k < nv; k++) {
+                    const double *bk = b + 7 * k;
+                    x[k] = s[8] * bk[6] + s[7] * bk[5] + s[6] * bk[4] + s[5] * bk[3] + s[4] * bk[2]
+                         + s[3] * bk[1] + s[2] * bk[0] + s[1] * a1[k] + s[0] * v1[k] + x1[k];
+                }
+                if (vpred) {
+                    s[0] = *t * hj;
+                    s[1] = s[0] * hj * .5;
+                    s[2] = s[1] * hj * .6666666666666667;
+                    s[3] = s[2] * hj * .75;
+                    s[4] = s[3] * hj * .8;
+                    s[5] = s[4] * hj * .8333333333333333;
+                    s[6] = s[5] * hj * .8571428571428571;
+                    s[7] = s[6] * hj * .875;
+                    for (int k = 3; k < nv; k++) {
+                        const double *bk = b + 7 * k;
+                        v[k] = s[7] * bk[6] + s[6] * bk[5] + s[5] * bk[4] + s[4] * bk[3] + s[3] * bk[2]
+                             + s[2] * bk[1] + s[1] * bk[0] + s[0] * a1[k] + v1[k];
+                    }
+                }
+                orc_mfo_all(jcen, nbod, nbig, m, x, v, pn, a);
+                nforce++;
+                for (int k = 3; k < nv; k++) {
+                    double *bk = b + 7 * k, *gk = g + 7 * k;
+                    double temp, gkk = a[k] - a1[k];
+                    switch (j) {
+                    case 2:
+                        temp = gk[0];
+                        gk[0] = (a[k] - a1[k]) * r[0];
+                        bk[0] = bk[0] + gk[0] - temp;
+                        break;
+                    case 3:
+                        temp = gk[1];
+                        gk[1] = (gkk * r[1] - gk[0]) * r[2];
+                        temp = gk[1] - temp;
+                        bk[0] = bk[0] + temp * c[0];
+                        bk[1] = bk[1] + temp;
+                        break;
+                    case 4:
+                        temp = gk[2];
+                        gk[2] = ((gkk * r[3] - gk[0]) * r[4] - gk[1]) * r[5];
+                        temp = gk[2] - temp;
+                        bk[0] = bk[0] + temp * c[1];
+                        bk[1] = bk[1] + temp * c[2];
+                        bk[2] = bk[2] + temp;
+                        break;
+                    case 5:
+                        temp = gk[3];
+                        gk[3] = (((gkk * r[6] - gk[0]) * r[7] - gk[1]) * r[8] - gk[2]) * r[9];
+                        temp = gk[3] - temp;
+                        bk[0] = bk[0] + temp * c[3];
+                        bk[1] = bk[1] + temp * c[4];
+                        bk[2] = bk[2] + temp * c[5];
+                        bk[3] = bk[3] + temp;
+                        break;
+                    case 6:
+                        temp = gk[4];
+                        gk[4] = ((((gkk * r[10] - gk[0]) * r[11] - gk[1]) * r[12] - gk[2]) * r[13] - gk[3]) * r[14];
+                        temp = gk[4] - temp;
+                        bk[0] = bk[0] + temp * c[6];
+                        bk[1] = bk[1] + temp * c[7];
+                        bk[2] = bk[2] + temp * c[8];
+                        bk[3] = bk[3] + temp * c[9];
+                        bk[4] = bk[4] + temp;
+                        break;
+                    case 7:
+                        temp = gk[5];
+                        gk[5] = (((((gkk * r[15] - gk[0]) * r[16] - gk[1]) * r[17] - gk[2]) * r[18] - gk[3]) * r[19]
+                                 - gk[4]) * r[20];
+                        temp = gk[5] - temp;
+                        bk[0] = bk[0] + temp * c[10];
+                        bk[1] = bk[1] + temp * c[11];
+                        bk[2] = bk[2] + temp * c[12];
+                        bk[3] = bk[3] + temp * c[13];
+                        bk[4] = bk[4] + temp * c[14];
+                        bk[5] = bk[5] + temp;
+                        break;
+                    default:
+                        temp = gk[6];
+                        gk[6] = ((((((gkk * r[21] - gk[0]) * r[22] - gk[1]) * r[23] - gk[2]) * r[24] - gk[3]) * r[25]
+                                  - gk[4]) * r[26] - gk[5]) * r[27];
+                        temp = gk[6] - temp;
+                        bk[0] = bk[0] + temp * c[15];
+                        bk[1] = bk[1] + temp * c[16];
+                        bk[2] = bk[2] + temp * c[17];
+                        bk[3] = bk[3] + temp * c[18];
+                        bk[4] = bk[4] + temp * c[19];
+                        bk[5] = bk[5] + temp * c[20];
+                        bk[6] = bk[6] + temp;
+                        break;
+                    }
+                }
+            }
+        }
+        double temp = 0.0;
+        for (int k = 3; k < nv; k++) temp = fmax(temp, fabs(b[7 * k + 6]));
+        temp = temp / (72.0 * pow(fabs(*t), 7.0));
+        *tdid = *t;
+        if (temp == 0.0)
+            *t = *tdid * 1.4;
+        else
+            *t = fsign(pow(tol / temp, 1.0 / 9.0), *tdid);
+        if (st->dtflag == 1 && fabs(*t / *tdid) < 1.0) {
+            *t = *t * .8;
+            (*redo)++;
+            continue;
+        }
+        if (fabs(*t / *tdid) > 1.4) *t = *tdid * 1.4;
+        break;
+    }
+    double temp = *tdid * *tdid;
+    for (int k = 3; k < nv; k++) {
+        const double *bk = b + 7 * k;
+        x1[k] = (RA_XC[7] * bk[6] + RA_XC[6] * bk[5] + RA_XC[5] * bk[4] + RA_XC[4] * bk[3] + RA_XC[3] * bk[2]
+                 + RA_XC[2] * bk[1] + RA_XC[1] * bk[0] + RA_XC[0] * a1[k]) * temp + v1[k] * *tdid + x1[k];
+        v1[k] = (RA_VC[6] * bk[6] + RA_VC[5] * bk[5] + RA_VC[4] * bk[4] + RA_VC[3] * bk[3] + RA_VC[2] * bk[2]
+                 + RA_VC[1] * bk[1] + RA_VC[0] * bk[0] + a1[k]) * *tdid + v1[k];
+    }
+    double q = *t / *tdid;
+    double q2 = q * q, q3 = q * q2, q4 = q2 * q2, q5 = q2 * q3, q6 = q3 * q3, q7 = q3 * q4;
+    for (int k = 3; k < nv; k++) {
+        double *bk = b + 7 * k, *ek = e + 7 * k;
+        double sd[7];
+        for (int i = 0; i < 7; i++) sd[i] = bk[i] - ek[i];
+        if (st->dtflag == 1)
+            for (int i = 0; i < 7; i++) sd[i] = 0.0;
+        ek[0] = q * (bk[6] * 7.0 + bk[5] * 6.0 + bk[4] * 5.0 + bk[3] * 4.0 + bk[2] * 3.0 + bk[1] * 2.0 + bk[0]);
+        ek[1] = q2 * (bk[6] * 21.0 + bk[5] * 15.0 + bk[4] * 10.0 + bk[3] * 6.0 + bk[2] * 3.0 + bk[1]);
+        ek[2] = q3 * (bk[6] * 35.0 + bk[5] * 20.0 + bk[4] * 10.0 + bk[3] * 4.0 + bk[2]);
+        ek[3] = q4 * (bk[6] * 35.0 + bk[5] * 15.0 + bk[4] * 5.0 + bk[3]);
+        ek[4] = q5 * (bk[6] * 21.0 + bk[5] * 6.0 + bk[4]);
+        ek[5] = q6 * (bk[6] * 7.0 + bk[5]);
+        ek[6] = q7 * bk[6];
+        for (int i = 0; i < 7; i++) bk[i] = ek[i] + sd[i];
+    }
+    st->dtflag = 2;
+    free(x); free(v); free(a); free(a1); free(g);
+    return nforce;
+}
+
+long orc_ra15_run(double time0, double h0, double tol, const double *jcen, int nbod,
+                  int nbig, const double *m, double *x, double *v, int pn, int nsteps,
+                  double *hdid, double *hnext)
+{
+    orc_ra15 st;
+    st.nbod = nbod;
+    st.dtflag = 0;
+    st.b = calloc((size_t)21 * nbod, sizeof(double));
+    st.e = calloc((size_t)21 * nbod, sizeof(double));
+    double t = h0, time = time0;
+    long nf = 0;
+    for (int s = 0; s < nsteps; s++) {
+        double tdid;
+        int redo;
+        nf += orc_mdt_ra15(time, &t, &tdid, tol, jcen, nbod, nbig, m, x, v, pn, &st, &redo);
+        hdid[s] = tdid;
+        time = time + tdid;
+    }
+    *hnext = t;
+    free(st.b);
+    free(st.e);
+    return nf;
+}

@@ -86,6 +86,32 @@ void orc_mco_mvs2h(double h, const double *jcen, int nbod, int nbig,
                    const double *m, const double *x, const double *v,
                    double *xh, double *vh);
 
+/* Heliocentric accelerations of mfo_all without non-gravitational or user
+ * forces: mfo_grav (pn = 1 adds the central 1PN term) + mfo_obl. */
+void orc_mfo_all(const double *jcen, int nbod, int nbig, const double *m,
+                 const double *x, const double *v, int pn, double *a);
+
+/* RA15 state kept between calls (mdt_ra15's saved b, e). */
+typedef struct {
+    int nbod;
+    int dtflag;           /* 0 first call, 1 reset, 2 normal (as mdt_ra15) */
+    double *b, *e;        /* [3*nbod][7] */
+} orc_ra15;
+
+/* One RA15 sequence (mdt_ra15 mercury6_2.for:4133-4500) with mfo_all forces.
+ * *t in: trial sequence size, out: proposal for the next call; *tdid: size
+ * done.  Returns the number of force evaluations; *redo counts the
+ * shrink-and-redo passes of a reset call. */
+long orc_mdt_ra15(double time, double *t, double *tdid, double tol, const double *jcen,
+                  int nbod, int nbig, const double *m, double *x1, double *v1, int pn,
+                  orc_ra15 *st, int *redo);
+
+/* nsteps RA15 sequences starting from trial size h0; hdid[] receives each
+ * size done.  Returns force evaluations. */
+long orc_ra15_run(double time0, double h0, double tol, const double *jcen, int nbod,
+                  int nbig, const double *m, double *x, double *v, int pn, int nsteps,
+                  double *hdid, double *hnext);
+
 /* Convenience for tests and the CPU baseline: nsteps MVS steps of a whole
  * system with the driver's central-impact check after each, event lists
  * flattened into caller arrays.  Returns the number of events written

@@ -1,0 +1,74 @@
+"""The drop-in: the unchanged MERCURY6 driver linked against libmercury_hip
+through mercuda_amd/fortran/mercury_gpu.f90 (built by `make -C oracle dropin`).
+
+CPU part: with `execution backend = cpu` the drop-in binary is the reference
+CPU path (identical dumps); requesting the device without one is a hard
+failure, never a silent fallback (reference tests/test_edges.py:59-67).
+GPU part: with the device selected, the dumps, encounter file and compressed
+output agree with the reference CPU run on identical inputs."""
+import numpy as np
+import pytest
+
+import mercury_run as R
+from helpers import REL_TOL
+from mercuda_amd import systems
+
+needs_dropin = pytest.mark.skipif(not (R.HIP_BIN.exists() and R.CPU_BIN.exists()),
+                                  reason="drop-in or reference driver not built (make -C oracle ref dropin)")
+
+
+def _pair(tmp_path, sysd, nsteps, backend_b, **kw):
+    a = R.write_run(tmp_path / "cpu", sysd, nsteps=nsteps, backend="cpu", **kw)
+    b = R.write_run(tmp_path / "dev", sysd, nsteps=nsteps, backend=backend_b, **kw)
+    ra = R.run(a, R.CPU_BIN)
+    assert ra.returncode == 0, ra.stdout + ra.stderr
+    rb = R.run(b, R.HIP_BIN)
+    return a, b, rb
+
+
+def _compare_dumps(a, b):
+    worst = 0.0
+    for f in ("big.dmp", "small.dmp"):
+        da, db = R.read_dump(a, f), R.read_dump(b, f)
+        assert da.keys() == db.keys()
+        for k in da:
+            for u, w in zip(da[k], db[k]):
+                worst = max(worst, np.linalg.norm(u - w) / max(np.linalg.norm(u), 1e-300))
+    return worst
+
+
+@needs_dropin
+def test_dropin_cpu_backend_is_the_reference(tmp_path):
+    s = systems.config_small(seed=6, nparticles=200)
+    a, b, rb = _pair(tmp_path, s, 200, "cpu")
+    assert rb.returncode == 0, rb.stdout + rb.stderr
+    assert _compare_dumps(a, b) == 0.0
+    assert (a / "ce.out").read_bytes() == (b / "ce.out").read_bytes()
+    assert any("Execution backend: CPU" in ln for ln in R.info_lines(b))
+
+
+@needs_dropin
+def test_dropin_device_request_without_device_fails(tmp_path):
+    from mercuda_amd.hip import MercuryHip
+    if MercuryHip().available():
+        pytest.skip("a HIP device is present")
+    s = systems.config_small(seed=6, nparticles=20)
+    a, b, rb = _pair(tmp_path, s, 20, "cuda")
+    assert rb.returncode != 0
+    assert "HIP was requested but no HIP device" in rb.stdout + rb.stderr
+
+
+@needs_dropin
+@pytest.mark.gpu
+@pytest.mark.parametrize("algorithm", ["MVS", "TEST"])
+def test_dropin_on_device_matches_reference(tmp_path, algorithm):
+    s = systems.config_small(seed=1, nparticles=1000)
+    print(f"case: drop-in {algorithm}, 8 planets + 1000 particles, 456 steps", flush=True)
+    a, b, rb = _pair(tmp_path, s, 456, "cuda", algorithm=algorithm, interval=8.0 * 57)
+    assert rb.returncode == 0, rb.stdout + rb.stderr
+    assert any("Execution backend: HIP" in ln for ln in R.info_lines(b)), R.info_lines(b)
+    worst = _compare_dumps(a, b)
+    print(f"max relative dump difference {worst:.3e}", flush=True)
+    assert worst <= REL_TOL
+    assert (a / "ce.out").read_bytes() == (b / "ce.out").read_bytes()
+    assert (a / "xv.out").read_bytes() == (b / "xv.out").read_bytes()

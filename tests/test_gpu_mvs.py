@@ -167,3 +167,19 @@ def test_full_size_sampled(hip, oracle):
         sub[k] = sysd[k][keep]
     xo, vo, evi, evd = oracle.mvs_run(sub, 48)
     check_state(x[keep], v[keep], xo, vo, "1M sampled")
+
+
+def test_export_corrector(hip, oracle):
+    """export(physical=1) applies the symplectic corrector mco_mvs2h
+    (mercury6_2.for:2107-2219) to a copy; the state itself is untouched."""
+    sysd = systems.config_small(seed=1, nparticles=500)
+    say("case: corrector export after 50 steps")
+    x, v, _ = run_stepwise(hip, sysd, 50)
+    xc, vc = hip.export(sysd["h"], physical=True)
+    xo, vo, _, _ = oracle.mvs_run(sysd, 50)
+    xh, vh = oracle.mvs2h(sysd, xo, vo)
+    check_state(xc, vc, xh, vh, "corrector")
+    x2, v2 = hip.download()
+    assert np.array_equal(x2, x) and np.array_equal(v2, v)
+    xp, vp = hip.export(sysd["h"], physical=False)
+    assert np.array_equal(xp, x) and np.array_equal(vp, v)
