@@ -15,6 +15,7 @@
 
 #include "../../include/mercury_hip.h"
 #include "mvs_device.hpp"
+#include "radau_device.hpp"
 
 using namespace mhip;
 
@@ -109,6 +110,10 @@ struct Context {
     int64_t parked = 0;
     unsigned long long* work = nullptr;   // device particle-step counters (timing mode)
     Timing timing;
+    // RA15 state (algorithm 4): b, e [21][cap], a1 [3][cap], max |b7|
+    double *rb = nullptr, *re = nullptr, *ra1 = nullptr;
+    unsigned long long* bmax = nullptr;
+    long long ra_cap = 0;
 };
 
 constexpr int kRing = 4;   // launches in flight: table, counter and request slots
@@ -139,6 +144,7 @@ void release_all() {
         dfree(*p);
     dfree(c.fault); dfree(c.dsys); dfree(c.tcur); c.sys_valid = false; c.sys_cap = 0; c.systems.clear(); dfree(c.dtiles); dfree(c.dev); dfree(c.devcount);
     dfree(c.dpark); dfree(c.dcnt); dfree(c.dslow); dfree(c.work); dfree(c.dreq); c.req_cap = 0;
+    dfree(c.rb); dfree(c.re); dfree(c.ra1); dfree(c.bmax); c.ra_cap = 0;
     c.cap = 0; c.tab_cap = 0; c.ev_cap = 0; c.park_cap = 0; c.ntiles = 0;
     c.n = 0; c.nbig = 0; c.history = false; c.saved_valid = false; c.have_state = false;
     c.events.clear(); c.adv_events.clear();
@@ -428,6 +434,129 @@ void mvs_run(double time, double h, int nsteps, bool reset, bool screen, bool sa
     sync_all();
 }
 
+// ---- RA15 (algorithm 4) -----------------------------------------------------
+
+// mdt_ra15's constants (mercury6_2.for:4162-4199): Gauss-Radau spacings h,
+// the position/velocity finishing weights xc, vc, and the r, c, d tables
+// built by the Fortran's recurrences in its order.
+RaConst ra_constants() {
+    static const double h[8] = {0.0, .0562625605369221, .1802406917368924, .3526247171131696,
+                                .5471536263305554, .7342101772154105, .8853209468390958, .9775206135612875};
+    static const double xc[8] = {.5, .1666666666666667, .08333333333333333, .05, .03333333333333333,
+                                 .02380952380952381, .01785714285714286, .01388888888888889};
+    static const double vc[7] = {.5, .3333333333333333, .25, .2, .1666666666666667, .1428571428571429, .125};
+    RaConst k{};
+    for (int i = 0; i < 8; i++) { k.h[i] = h[i]; k.xc[i] = xc[i]; }
+    for (int i = 0; i < 7; i++) k.vc[i] = vc[i];
+    int n = 0;
+    for (int j = 1; j < 8; j++)
+        for (int q = 0; q < j; q++) k.r[n++] = 1.0 / (h[j] - h[q]);
+    // 1-based Fortran indices: c(n), d(n) stored at [n - 1]
+    k.c[0] = -h[1];
+    k.d[0] = h[1];
+    n = 1;
+    for (int j = 3; j <= 7; j++) {
+        n++;
+        k.c[n - 1] = -h[j - 1] * k.c[n - j + 1];
+        k.d[n - 1] = h[1] * k.d[n - j + 1];
+        for (int q = 3; q <= j - 1; q++) {
+            n++;
+            k.c[n - 1] = k.c[n - j] - h[j - 1] * k.c[n - j + 1];
+            k.d[n - 1] = k.d[n - j] + h[q - 1] * k.d[n - j + 1];
+        }
+        n++;
+        k.c[n - 1] = k.c[n - j] - h[j - 1];
+        k.d[n - 1] = k.d[n - j] + h[j - 1];
+    }
+    return k;
+}
+
+// (speed of light in AU/day)^2 of mfo_grav's 1PN term: C = 299792458 m/s,
+// AU = 1.495978707e11 m (mercury.inc constants, as the Fortran evaluates them)
+double light_speed2() {
+    const double cl = 299792458.0 * 86400.0 / (1.495978707e13 * .01);
+    return cl * cl;
+}
+
+// RA15 arrays sized to the body capacity; true when they were (re)allocated,
+// i.e. no RA15 history survives.
+bool ensure_radau() {
+    Context& c = ctx;
+    if (c.ra_cap == c.cap && c.rb) return false;
+    dalloc(c.rb, (size_t)21 * c.cap);
+    dalloc(c.re, (size_t)21 * c.cap);
+    dalloc(c.ra1, (size_t)3 * c.cap);
+    if (!c.bmax) dalloc(c.bmax, 1);
+    c.ra_cap = c.cap;
+    return true;
+}
+
+// One force evaluation (niter = 0: a1 only) or one RA15 iteration sweep over
+// every body; returns max |b7| (0 for niter = 0).
+double radau_pass(double t, int niter, bool reset) {
+    Context& c = ctx;
+    State st = device_state();
+    static const RaConst K = ra_constants();
+    RaArgs args{c.dsys, st, RaState{c.rb, c.re, c.ra1}, c.tab, c.bmax, K, t, niter, reset ? 1 : 0, c.pn,
+                light_speed2()};
+    HIPCHECK(hipMemsetAsync(c.bmax, 0, sizeof(unsigned long long), c.stream));
+    if (c.pn) hipLaunchKernelGGL(ra_planets<true>, dim3(1), dim3(64), 0, c.stream, args);
+    else hipLaunchKernelGGL(ra_planets<false>, dim3(1), dim3(64), 0, c.stream, args);
+    const int npart = c.n - c.nbig;
+    if (npart > 0) {
+        dim3 grid((npart + 127) / 128);
+        if (c.pn) hipLaunchKernelGGL(ra_particles<true>, grid, dim3(128), 0, c.stream, args);
+        else hipLaunchKernelGGL(ra_particles<false>, grid, dim3(128), 0, c.stream, args);
+    }
+    HIPCHECK(hipGetLastError());
+    unsigned long long bits = 0;
+    HIPCHECK(hipMemcpyAsync(&bits, c.bmax, sizeof(bits), hipMemcpyDeviceToHost, c.stream));
+    HIPCHECK(hipStreamSynchronize(c.stream));
+    double v;
+    std::memcpy(&v, &bits, sizeof(v));
+    return v;
+}
+
+void check_fault(const char* what);
+
+// mdt_ra15's sequence with its step-size law on the host (libm pow, as the
+// CPU path): shrink-and-redo while a reset sequence would shrink, cap 1.4x.
+// The iteration runs on the device; only max |b7| comes back per pass.
+void radau_step(double time, double* h, double* hdid, double tol, int64_t* forces, int64_t* rejected) {
+    Context& c = ctx;
+    if (c.systems.size() != 1) throw std::runtime_error("RADAU needs one system");
+    if (ensure_radau()) c.history = false;
+    build_systems(kRaEval, true, *h, 0.0);
+    const bool reset = !c.history;
+    const int niter = reset ? 6 : 2;
+    double t = *h, tdid = t;
+    for (;;) {
+        if (time + t == time) throw std::runtime_error("RADAU step cannot advance time");
+        double temp = radau_pass(t, niter, reset);
+        *forces += 1 + 7 * niter;
+        check_fault("Nonfinite RADAU force or predictor");
+        temp = temp / (72.0 * std::pow(std::fabs(t), 7.0));
+        tdid = t;
+        if (temp == 0.0) t = tdid * 1.4;
+        else t = std::copysign(std::pow(tol / temp, 1.0 / 9.0), tdid);
+        if (reset && std::fabs(t / tdid) < 1.0) {
+            t = t * .8;
+            ++*rejected;
+            continue;
+        }
+        if (std::fabs(t / tdid) > 1.4) t = tdid * 1.4;
+        break;
+    }
+    static const RaConst K = ra_constants();
+    RaFinishArgs fa{c.dsys, device_state(), RaState{c.rb, c.re, c.ra1}, K, tdid, t / tdid, reset ? 1 : 0};
+    if (c.n > 1) hipLaunchKernelGGL(ra_finish, dim3((c.n - 1 + 127) / 128), dim3(128), 0, c.stream, fa);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(c.stream));
+    *hdid = tdid;
+    *h = t;
+    c.history = true;
+}
+
 int read_fault() {
     int bad = 0;
     HIPCHECK(hipMemcpy(&bad, ctx.fault, sizeof(int), hipMemcpyDeviceToHost));
@@ -483,7 +612,7 @@ int mercury_hip_available(void) {
 }
 
 int mercury_hip_configure(int algorithm) {
-    if (algorithm != 1 && algorithm != 9) return 1;
+    if (algorithm != 1 && algorithm != 4 && algorithm != 9) return 1;
     if (algorithm != ctx.algorithm) release_all();
     ctx.algorithm = algorithm;
     ctx.history = false;
@@ -511,10 +640,8 @@ int mercury_hip_upload(int n, int nbig, int pn, int ngflag, const double* m, con
         if (!(m[0] > 0)) throw std::runtime_error("central mass must be positive");
         if (ngflag != 0) throw std::runtime_error("non-gravitational forces are not provided by the HIP path");
         if (pn != 0 && mvs_family()) throw std::runtime_error("MVS does not take post-Newtonian forces");
-        bool obl = jcen[0] != 0 || jcen[1] != 0 || jcen[2] != 0;
-        if (obl)
-            for (int j = nbig; j < n; j++)
-                if (m[j] != 0) throw std::runtime_error("oblateness with massive Small bodies is not provided");
+        for (int j = nbig; j < n; j++)
+            if (m[j] != 0) throw std::runtime_error("Small bodies with mass are not provided by the HIP path");
         for (int j = 0; j < 3 * n; j++)
             if (!std::isfinite(x[j]) || !std::isfinite(v[j])) throw std::runtime_error("nonfinite state");
         ensure_stream();
@@ -529,6 +656,8 @@ int mercury_hip_upload(int n, int nbig, int pn, int ngflag, const double* m, con
         upload_vec3(x, ctx.x);
         upload_vec3(v, ctx.v);
         HIPCHECK(hipMemset(ctx.a, 0, sizeof(double) * 3 * ctx.cap));
+        HIPCHECK(hipMemset(ctx.ox, 0, sizeof(double) * 3 * ctx.cap));
+        HIPCHECK(hipMemset(ctx.ov, 0, sizeof(double) * 3 * ctx.cap));
         HIPCHECK(hipMemset(ctx.fault, 0, sizeof(int)));
         build_tiles();
         ctx.history = false;
@@ -544,6 +673,13 @@ int mercury_hip_step(double time, double* h, double* hdid, double tol, int64_t* 
         (void)tol;
         require_state();
         *rejected = 0;
+        *forces = 0;
+        if (ctx.algorithm == 4) {
+            clear_fault();
+            radau_step(time, h, hdid, tol, forces, rejected);
+            ctx.saved_valid = true;
+            return 0;
+        }
         if (!mvs_family()) throw std::runtime_error("algorithm not provided by the HIP path");
         clear_fault();
         bool reset = !ctx.history;
@@ -571,12 +707,19 @@ int mercury_hip_events(double time, double h, double rcen, const MercuryEvent** 
     try {
         require_state();
         if (!ctx.saved_valid) throw std::runtime_error("no step to screen");
-        if (!mvs_family()) throw std::runtime_error("algorithm not provided by the HIP path");
+        const bool ra = ctx.algorithm == 4;
+        if (!mvs_family() && !ra) throw std::runtime_error("algorithm not provided by the HIP path");
         int n = ctx.n;
+        if (ra && ctx.nbig > 1) {
+            // planet boxes and end positions from the step-start and end states
+            hipLaunchKernelGGL(planet_screen_block, dim3(1), dim3(64), 0, ctx.stream, ctx.dsys, device_state(),
+                               ctx.tab + ctx.hsys[0].tab, h);
+            HIPCHECK(hipGetLastError());
+        }
         for (;;) {
             HIPCHECK(hipMemsetAsync(ctx.devcount, 0, sizeof(int), ctx.stream));
             SavedScreenArgs sa{ctx.dsys, device_state(), ctx.tab, EventSink{ctx.dev, ctx.devcount, ctx.ev_cap, ctx.fault},
-                               time, h, rcen, 1};
+                               time, h, rcen, 1, ra ? ctx.ov : ctx.v0k, ra ? ctx.v : ctx.v1};
             if (n > 1) hipLaunchKernelGGL(screen_saved, dim3((n - 1 + 255) / 256), dim3(256), 0, ctx.stream, sa);
             HIPCHECK(hipGetLastError());
             HIPCHECK(hipMemcpyAsync(count, ctx.devcount, sizeof(int), hipMemcpyDeviceToHost, ctx.stream));
@@ -597,8 +740,17 @@ int mercury_hip_events(double time, double h, double rcen, const MercuryEvent** 
 
 int mercury_hip_force(double* a) {
     try {
-        (void)a;
-        throw std::runtime_error("heliocentric force interface requires algorithm 4");
+        require_state();
+        if (ctx.systems.size() != 1) throw std::runtime_error("force needs one system");
+        ensure_stream();
+        clear_fault();
+        if (ensure_radau() && ctx.algorithm == 4) ctx.history = false;
+        build_systems(kRaEval, true, 1.0, 0.0);
+        radau_pass(0.0, 0, false);
+        check_fault("Nonfinite force");
+        download_vec3(ctx.ra1, a);
+        a[0] = a[1] = a[2] = 0.0;
+        return 0;
     } catch (const std::exception& e) { return fail(e); }
 }
 
@@ -723,6 +875,8 @@ int mercury_hip_ensemble_upload(int nsys, const int* n, const int* nbig, const d
         upload_vec3(x, ctx.x);
         upload_vec3(v, ctx.v);
         HIPCHECK(hipMemset(ctx.a, 0, sizeof(double) * 3 * ctx.cap));
+        HIPCHECK(hipMemset(ctx.ox, 0, sizeof(double) * 3 * ctx.cap));
+        HIPCHECK(hipMemset(ctx.ov, 0, sizeof(double) * 3 * ctx.cap));
         HIPCHECK(hipMemset(ctx.fault, 0, sizeof(int)));
         build_tiles();
         ctx.history = false;

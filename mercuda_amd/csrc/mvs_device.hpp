@@ -1044,7 +1044,9 @@ struct SavedScreenArgs {
     const double* tab;
     EventSink es;
     double time, h, rcen;
-    int pairs;        // 0: central impacts only
+    int pairs;          // 0: central impacts only
+    const double* va;   // pair-screen velocities at the start and end of the step
+    const double* vb;   // (MVS: after the first kick / after the drift; others: ov, v)
 };
 
 __global__ void __launch_bounds__(256) screen_saved(SavedScreenArgs args) {
@@ -1059,8 +1061,8 @@ __global__ void __launch_bounds__(256) screen_saved(SavedScreenArgs args) {
     double x1 = args.st.x[g], y1 = args.st.x[cap + g], z1 = args.st.x[2 * cap + g];
     double xs = args.st.ox[g], ys = args.st.ox[cap + g], zs = args.st.ox[2 * cap + g];
     if (args.pairs) {
-        Ends e{xs, ys, zs, args.st.v0k[g], args.st.v0k[cap + g], args.st.v0k[2 * cap + g],
-               x1, y1, z1, args.st.v1[g], args.st.v1[cap + g], args.st.v1[2 * cap + g]};
+        Ends e{xs, ys, zs, args.va[g], args.va[cap + g], args.va[2 * cap + g],
+               x1, y1, z1, args.vb[g], args.vb[cap + g], args.vb[2 * cap + g]};
         screen_body(T, np, jb, d.nbig, mj, e, args.st.rce[g], args.st.rphys[g], args.h, args.time, 0, 0,
                     args.es);
     }

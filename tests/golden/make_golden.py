@@ -2,7 +2,9 @@
 routines (oracle/_ref/libmercury_ref.so, compiled from mercury6_2.for by
 oracle/Makefile).  Run here, where /root/reference exists:
 
-    make -C oracle ref && python tests/golden/make_golden.py
+    make -C oracle ref && python tests/golden/make_golden.py [radau]
+
+(`radau` rewrites only the mfo_all / mdt_ra15 fixtures.)
 
 Each fixture holds inputs and the reference's outputs only (data, no code).
 """
@@ -112,11 +114,43 @@ def main():
     hits = R.mce_cent(4.0, 4.0, 0.005, m, X0, V0, X1, V1)
     np.savez_compressed(HERE / "mce_cent.npz", m=m, x0=X0, v0=V0, x1=X1, v1=V1,
                         hits=np.array(hits, float).reshape(-1, 3))
+    radau(R)
     print("wrote", sorted(p.name for p in HERE.glob("*.npz")))
+
+
+def radau(R=None):
+    """mfo_all (gravity, 1PN, oblateness) and whole mdt_ra15 runs."""
+    R = R or Reference()
+    jc = (1e-6 * 0.0046 ** 2, -2e-7 * 0.0046 ** 4, 5e-8 * 0.0046 ** 6)
+    cases = {}
+    for c, (seed, npl, npart, pn, obl) in enumerate([(300, 8, 40, 0, False), (301, 8, 40, 1, False),
+                                                     (302, 3, 25, 1, True), (303, 12, 30, 0, True),
+                                                     (304, 0, 10, 1, False)]):
+        s = systems.make_system(seed, npart, nplanets=npl)
+        s["jcen"] = np.array(jc if obl else (0.0, 0.0, 0.0))
+        cases[f"m{c}"] = s["m"]; cases[f"x{c}"] = s["x"]; cases[f"v{c}"] = s["v"]
+        cases[f"nbig{c}"] = s["nbig"]; cases[f"pn{c}"] = pn; cases[f"jcen{c}"] = s["jcen"]
+        cases[f"a{c}"] = R.mfo_all(s, pn=pn)
+    np.savez_compressed(HERE / "mfo_all.npz", **cases)
+
+    runs = {}
+    for c, (seed, npart, nsteps, pn, h0, obl) in enumerate([(1, 120, 150, 1, 8.0, False),
+                                                            (2, 80, 100, 0, 8.0, False),
+                                                            (3, 60, 40, 1, 200.0, False),
+                                                            (4, 60, 80, 1, 4.0, True)]):
+        s = systems.config_small(seed, npart)
+        s["jcen"] = np.array(jc if obl else (0.0, 0.0, 0.0))
+        x, v, hd, hn = R.ra15_run(s, nsteps, h0=h0, tol=1e-12, pn=pn)
+        for key in ("m", "x", "v", "jcen"):
+            runs[f"{key}{c}"] = s[key]
+        runs[f"meta{c}"] = np.array([s["nbig"], nsteps, h0, pn, 1e-12])
+        runs[f"xout{c}"] = x; runs[f"vout{c}"] = v; runs[f"hdid{c}"] = hd; runs[f"hnext{c}"] = np.array([hn])
+    np.savez_compressed(HERE / "ra15_runs.npz", **runs)
 
 
 if __name__ == "__main__":
     threading.stack_size(512 << 20)
-    t = threading.Thread(target=main)
+    only = sys.argv[1] if len(sys.argv) > 1 else None
+    t = threading.Thread(target=radau if only == "radau" else main)
     t.start()
     t.join()

@@ -296,3 +296,137 @@ def test_mce_cent_impact_time(oracle):
     Ehit = -np.arccos((1 - rcen / a_) / e_)
     t_hit = ((Ehit - e_ * np.sin(Ehit)) - (E0 - e_ * np.sin(E0))) / n
     assert abs(hits[0][1] - t_hit) < 1e-9 * h
+
+
+# ---------------------------------------------------------------- RA15 and mfo_all
+JC = (1e-6 * 0.0046 ** 2, -2e-7 * 0.0046 ** 4, 5e-8 * 0.0046 ** 6)
+
+
+def test_golden_mfo_all(oracle):
+    g = np.load(GOLD / "mfo_all.npz")
+    for c in range(5):
+        s = dict(m=g[f"m{c}"], x=g[f"x{c}"], v=g[f"v{c}"], nbig=int(g[f"nbig{c}"]), jcen=g[f"jcen{c}"])
+        a = oracle.mfo_all(s, pn=int(g[f"pn{c}"]))
+        assert np.array_equal(a[1:], g[f"a{c}"][1:]), f"case {c}"
+
+
+def test_golden_ra15_runs(oracle):
+    """Whole mdt_ra15 runs (reset sequence with redo, steady sequences, 1PN,
+    oblateness): states and every step size done, bit for bit."""
+    g = np.load(GOLD / "ra15_runs.npz")
+    for c in range(4):
+        nbig, nsteps, h0, pn, tol = g[f"meta{c}"]
+        s = dict(m=g[f"m{c}"], x=g[f"x{c}"], v=g[f"v{c}"], nbig=int(nbig), jcen=g[f"jcen{c}"], h=h0)
+        x, v, hd, hn, _ = oracle.ra15_run(s, int(nsteps), tol=tol, pn=int(pn))
+        assert np.array_equal(hd, g[f"hdid{c}"]) and hn == g[f"hnext{c}"][0], f"run {c}"
+        assert np.array_equal(x, g[f"xout{c}"]) and np.array_equal(v, g[f"vout{c}"]), f"run {c}"
+
+
+@pytest.mark.parametrize("pn", [0, 1])
+def test_reference_ra15_run(oracle, reference, pn):
+    s = systems.config_small(seed=9, nparticles=150)
+    s["jcen"] = np.array(JC)
+    x, v, hd, hn, _ = oracle.ra15_run(s, 120, h0=60.0, tol=1e-12, pn=pn)
+    xr, vr, hr, hnr = run_big_stack(reference.ra15_run, s, 120, h0=60.0, tol=1e-12, pn=pn)
+    assert np.array_equal(hd, hr) and hn == hnr
+    assert np.array_equal(x, xr) and np.array_equal(v, vr)
+
+
+def test_mfo_all_brute_force(oracle):
+    """mfo_grav's heliocentric accelerations from the barycentric physics,
+    summed in extended precision: a_j = -(m0 + m_j) x_j / r_j^3
+    + sum_{k Big, k != j} m_k ((x_k - x_j)/|x_k - x_j|^3 - x_k / r_k^3)."""
+    s = systems.make_system(8, 30, nplanets=6)
+    a = oracle.mfo_all(s)
+    m = s["m"].astype(np.longdouble); x = s["x"].astype(np.longdouble)
+    nb = s["nbig"]
+    for j in range(1, len(m)):
+        rj = np.linalg.norm(x[j])
+        want = -(m[0] + m[j]) * x[j] / rj ** 3
+        for k in range(1, nb):
+            if k == j:
+                continue
+            d = x[k] - x[j]
+            want = want + m[k] * (d / np.linalg.norm(d) ** 3 - x[k] / np.linalg.norm(x[k]) ** 3)
+        w = want.astype(float)
+        assert np.max(np.abs(a[j] - w)) <= 1e-13 * np.linalg.norm(w), j
+
+
+def test_mfo_obl_is_potential_gradient(oracle):
+    """Oblateness term = -grad of the zonal potential
+    Phi = (mu/r) sum_n J_n R^n / r^n P_n(z/r), n = 2, 4, 6 (jcen holds J_n R^n),
+    by 30-digit central differences, with the central body's reaction
+    -sum_k (m_k/m0) a_k added to every body as mfo_all does."""
+    import mpmath as mp
+    mp.mp.dps = 30
+    s = systems.make_system(12, 6, nplanets=3)
+    jc = (3e-4, -4e-6, 2e-7)     # large enough to dominate rounding
+    s["jcen"] = np.array(jc)
+    a_obl = oracle.mfo_all(s) - oracle.mfo_all(dict(s, jcen=np.zeros(3)))
+    mu = mp.mpf(float(s["m"][0]))
+    P = {2: lambda u: (3 * u ** 2 - 1) / 2, 4: lambda u: (35 * u ** 4 - 30 * u ** 2 + 3) / 8,
+         6: lambda u: (231 * u ** 6 - 315 * u ** 4 + 105 * u ** 2 - 5) / 16}
+
+    def phi(p):
+        r = mp.sqrt(sum(q * q for q in p))
+        u = p[2] / r
+        return mu / r * sum(mp.mpf(j) / r ** n * P[n](u) for j, n in zip(jc, (2, 4, 6)))
+
+    def accel(xj):
+        p = [mp.mpf(float(q)) for q in xj]
+        eps = mp.mpf(10) ** -12
+        out = []
+        for k in range(3):
+            hi = list(p); lo = list(p)
+            hi[k] += eps; lo[k] -= eps
+            out.append(-(phi(hi) - phi(lo)) / (2 * eps))
+        return np.array([float(q) for q in out])
+
+    acc = {j: accel(s["x"][j]) for j in range(1, len(s["m"]))}
+    acen = -sum(s["m"][k] / s["m"][0] * acc[k] for k in range(1, len(s["m"])))
+    for j in range(1, len(s["m"])):
+        want = acc[j] - acen
+        assert np.max(np.abs(a_obl[j] - want)) <= 1e-9 * np.linalg.norm(want), j
+
+
+def test_ra15_circular_orbit_known_answer(oracle):
+    """Circular test-particle orbits about the central body alone: the phase
+    after ~1000 d of adaptive RA15 is sqrt(mu/r^3) t to 1e-12 AU (the
+    reference's known answer, tests/test_release.py:35-48)."""
+    r = np.array([0.7, 1.0, 1.6, 3.3]); th0 = np.array([0.0, 1.0, 2.5, 4.0]); k = len(r)
+    m = np.concatenate([[MU], np.zeros(k)]); x = np.zeros((k + 1, 3)); v = np.zeros((k + 1, 3))
+    w = np.sqrt(MU / r ** 3)
+    x[1:, 0] = r * np.cos(th0); x[1:, 1] = r * np.sin(th0)
+    v[1:, 0] = -r * w * np.sin(th0); v[1:, 1] = r * w * np.cos(th0)
+    xo, vo, hd, hn, nf = oracle.ra15_run({"m": m, "x": x, "v": v, "nbig": 1, "h": 10.0}, 145, tol=1e-12)
+    t = 0.0
+    for q in hd:
+        t = t + q
+    want = np.stack([r * np.cos(th0 + w * t), r * np.sin(th0 + w * t)], 1)
+    assert t > 990.0 and np.abs(xo[1:, :2] - want).max() < 1e-12
+
+
+def test_ra15_pn_perihelion_advance(oracle):
+    """The 1PN term advances the pericentre by 6 pi mu / (c^2 a (1 - e^2)) per
+    orbit (Einstein's perihelion shift).  An eccentric orbit at 0.05 AU for
+    ~1000 orbits: the Laplace-Runge-Lenz direction turns by that amount to
+    1e-3 relative (short-period 1PN wobble ~1e-7 rad)."""
+    a, e = 0.05, 0.5
+    rp = a * (1 - e)
+    vp = np.sqrt(MU * (1 + e) / rp)
+    s = {"m": np.array([MU, 0.0]), "x": np.array([[0, 0, 0], [rp, 0, 0]], float),
+         "v": np.array([[0, 0, 0], [0, vp, 0]], float), "nbig": 1, "h": 0.2}
+    per = 2 * np.pi * np.sqrt(a ** 3 / MU)
+    xo, vo, hd, hn, nf = oracle.ra15_run(s, 40000, tol=1e-12, pn=1)
+    T = float(np.sum(hd))
+    rr, vv = xo[1], vo[1]
+    lrl = np.cross(vv, np.cross(rr, vv)) / MU - rr / np.linalg.norm(rr)
+    turned = np.arctan2(lrl[1], lrl[0])
+    c2 = (299792458.0 * 86400.0 / 1.495978707e11) ** 2
+    want = 6 * np.pi * MU / (c2 * a * (1 - e * e)) * (T / per)
+    assert T / per > 500
+    assert abs(turned - want) <= 1e-3 * want, (turned, want)
+    # and the Newtonian run does not turn
+    xn, vn, *_ = oracle.ra15_run(s, 4000, tol=1e-12, pn=0)
+    lrl = np.cross(vn[1], np.cross(xn[1], vn[1])) / MU - xn[1] / np.linalg.norm(xn[1])
+    assert abs(np.arctan2(lrl[1], lrl[0])) < 1e-9
