@@ -35,6 +35,11 @@ FLOP_PER_PSTEP = 351.6
 # Algorithmic HBM bytes per particle per launch of the fast kernel: read
 # x, v, a (72 B) and rce (8 B), write x, v, a (72 B).
 BYTES_PER_PARTICLE_LAUNCH = 152.0
+# HBM bytes per particle per launch of the fast kernel as rocprofv3 measured
+# them on this workload (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, the factors
+# a 256 MiB device copy reads on the same counters; 1M particles, 9 launches):
+# profiles/mvs_particles_fast_1M_hbm_traffic.md.
+TRAFFIC_PER_PARTICLE_LAUNCH = 172.2
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 
@@ -43,7 +48,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(sysd, nproc=16, shard=6250, nsteps=384):
+def cpu_baseline(sysd, nproc=16, shard=31250, nsteps=1536):
     """The reference CPU path (flang build of mdt_mvs + mce_cent) on `nproc`
     disjoint particle shards in parallel processes.  Massless particles do not
     interact, so the shards are exact.  Returns particle-steps/s."""
@@ -134,7 +139,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the massless-particle path has no multi-GPU mode in this build "
-              "(particles would shard across GPUs with a replicated planet table; see DESIGN.md).",
+              "(particles would shard across GPUs with a replicated planet table).",
               file=sys.stderr)
         sys.exit(2)
 
@@ -173,17 +178,22 @@ def main():
     flop_per_launch = FLOP_PER_PSTEP * fast_steps / launches
     achieved = flop_per_launch / avg_s / 1e12
     roofline = {
-        "bound": "fp64-valu",
+        # compute roof: the FP64 peak (78.6 TFLOP/s, the same for the vector
+        # ALUs and the FP64 matrix cores on MI355X); the kernel itself runs on
+        # the FP64 VALU -- nothing here is a dense contraction
+        "bound": "mfma",
+        "compute_unit": "fp64 valu",
         "achieved": round(achieved, 3),
         "peak": PEAK_FP64_TFLOPS,
         "unit": "TFLOP/s",
         "frac": round(achieved / PEAK_FP64_TFLOPS, 4),
-        "traffic": None,
+        "traffic": round(TRAFFIC_PER_PARTICLE_LAUNCH * npart),
         "kernel": "mvs_particles_fast",
         "avg_launch_ms": round(avg_s * 1e3, 4),
         "flop_per_particle_step": FLOP_PER_PSTEP,
         "particle_steps_per_launch": round(fast_steps / launches, 1),
         "hbm_gbs_algorithmic": round(BYTES_PER_PARTICLE_LAUNCH * npart / avg_s / 1e9, 1),
+        "hbm_bytes_algorithmic_per_launch": round(BYTES_PER_PARTICLE_LAUNCH * npart),
         "time_share": {"fast_ms": round(tm["particle_ms"], 2), "general_ms": round(tm["general_ms"], 2),
                        "planet_ms": round(tm["planet_ms"], 2), "wall_ms": round(sec * 1e3, 2)},
     }
@@ -198,12 +208,12 @@ def main():
             rate, tmax = r
             cpu = {"value": round(rate, 1), "unit": "particle-steps/s", "cores": args.cpu_procs,
                    "kind": "reference",
-                   "sample": f"{args.cpu_procs} processes x 6250 particles x 384 steps of this workload "
+                   "sample": f"{args.cpu_procs} processes x 31250 particles x 1536 steps of this workload "
                              f"(reference mdt_mvs + mce_cent, flang -O3 -ffp-contract=off), "
                              f"slowest process {tmax:.2f} s",
                    "gpu_over_cpu": round(value / rate, 1)}
     line = {
-        "metric": "FP64 particle-steps/s, MVS 8pl+1M TP",
+        "metric": "FP64 particle-steps/s, MVS 8pl+1M TP; batched ensemble & adjoint/fwd cost",
         "value": round(value, 1),
         "unit": "particle-steps/s",
         "n_gpus": 1,

@@ -299,6 +299,19 @@ class Reference:
                         C.byref(a), C.byref(b))
         return a.value, b.value
 
+    def mvs2h(self, sysd, x, v):
+        """The reference's symplectic corrector mco_mvs2h (mercury6_2.for:2107-2219)."""
+        m = np.ascontiguousarray(sysd["m"], np.float64)
+        n = len(m)
+        jc = np.ascontiguousarray(sysd.get("jcen", (0.0, 0.0, 0.0)), np.float64)
+        x = np.ascontiguousarray(x, np.float64); v = np.ascontiguousarray(v, np.float64)
+        xh = np.zeros_like(x); vh = np.zeros_like(v)
+        ngf = np.zeros((n, 5)); opt = np.zeros(8, np.int32)
+        self.L.mco_mvs2h_(self._r(0.0), _d(jc), C.byref(C.c_int(n)), C.byref(C.c_int(sysd["nbig"])),
+                          self._r(sysd["h"]), _d(m), _d(x), _d(v), _d(xh), _d(vh), _d(ngf),
+                          C.byref(C.c_int(0)), _i(opt))
+        return xh, vh
+
     def mce_cent(self, time, h, rcen, m, x0, v0, x1, v1, i0=2):
         arrs = [np.ascontiguousarray(a, np.float64) for a in (m, x0, v0, x1, v1)]
         n = len(arrs[0])

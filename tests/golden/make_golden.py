@@ -4,7 +4,7 @@ oracle/Makefile).  Run here, where /root/reference exists:
 
     make -C oracle ref && python tests/golden/make_golden.py [radau]
 
-(`radau` rewrites only the mfo_all / mdt_ra15 fixtures.)
+(`radau` rewrites only the mfo_all / mdt_ra15 / mco_mvs2h fixtures.)
 
 Each fixture holds inputs and the reference's outputs only (data, no code).
 """
@@ -119,7 +119,7 @@ def main():
 
 
 def radau(R=None):
-    """mfo_all (gravity, 1PN, oblateness) and whole mdt_ra15 runs."""
+    """mfo_all (gravity, 1PN, oblateness), whole mdt_ra15 runs, mco_mvs2h."""
     R = R or Reference()
     jc = (1e-6 * 0.0046 ** 2, -2e-7 * 0.0046 ** 4, 5e-8 * 0.0046 ** 6)
     cases = {}
@@ -146,6 +146,19 @@ def radau(R=None):
         runs[f"meta{c}"] = np.array([s["nbig"], nsteps, h0, pn, 1e-12])
         runs[f"xout{c}"] = x; runs[f"vout{c}"] = v; runs[f"hdid{c}"] = hd; runs[f"hnext{c}"] = np.array([hn])
     np.savez_compressed(HERE / "ra15_runs.npz", **runs)
+
+    # the symplectic corrector mco_mvs2h on evolved MVS states
+    cor = {}
+    for c, (seed, npart, jcv) in enumerate([(21, 60, (0.0, 0.0, 0.0)), (22, 40, jc)]):
+        s = systems.config_small(seed, npart)
+        s["jcen"] = np.array(jcv)
+        x, v, _, _ = R.mdt_mvs_run(s, 50)
+        xh, vh = R.mvs2h(s, x, v)
+        for key in ("m", "jcen"):
+            cor[f"{key}{c}"] = s[key]
+        cor[f"meta{c}"] = np.array([s["nbig"], s["h"]])
+        cor[f"x{c}"] = x; cor[f"v{c}"] = v; cor[f"xh{c}"] = xh; cor[f"vh{c}"] = vh
+    np.savez_compressed(HERE / "mvs2h.npz", **cor)
 
 
 if __name__ == "__main__":

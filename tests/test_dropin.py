@@ -53,9 +53,10 @@ def test_dropin_device_request_without_device_fails(tmp_path):
     if MercuryHip().available():
         pytest.skip("a HIP device is present")
     s = systems.config_small(seed=6, nparticles=20)
-    a, b, rb = _pair(tmp_path, s, 20, "cuda")
-    assert rb.returncode != 0
-    assert "HIP was requested but no HIP device" in rb.stdout + rb.stderr
+    for spelling in ("hip", "cuda"):
+        a, b, rb = _pair(tmp_path / spelling, s, 20, spelling)
+        assert rb.returncode != 0
+        assert "HIP was requested but no HIP device" in rb.stdout + rb.stderr
 
 
 @needs_dropin
@@ -64,7 +65,27 @@ def test_dropin_device_request_without_device_fails(tmp_path):
 def test_dropin_on_device_matches_reference(tmp_path, algorithm):
     s = systems.config_small(seed=1, nparticles=1000)
     print(f"case: drop-in {algorithm}, 8 planets + 1000 particles, 456 steps", flush=True)
-    a, b, rb = _pair(tmp_path, s, 456, "cuda", algorithm=algorithm, interval=8.0 * 57)
+    a, b, rb = _pair(tmp_path, s, 456, "hip", algorithm=algorithm, interval=8.0 * 57)
+    assert rb.returncode == 0, rb.stdout + rb.stderr
+    assert any("Execution backend: HIP" in ln for ln in R.info_lines(b)), R.info_lines(b)
+    worst = _compare_dumps(a, b)
+    print(f"max relative dump difference {worst:.3e}", flush=True)
+    assert worst <= REL_TOL
+    assert (a / "ce.out").read_bytes() == (b / "ce.out").read_bytes()
+    assert (a / "xv.out").read_bytes() == (b / "xv.out").read_bytes()
+
+
+@needs_dropin
+@pytest.mark.gpu
+@pytest.mark.parametrize("pn", [False, True])
+def test_dropin_radau_on_device_matches_reference(tmp_path, pn):
+    """RADAU through the unchanged adaptive driver (mal_hvar): clipped steps at
+    every output time reset the RA15 history; dumps, encounters and output
+    agree with the reference CPU run."""
+    s = systems.config_encounters(seed=3, nparticles=600)
+    s["rce"][1:s["nbig"]] *= 3.0
+    print(f"case: drop-in RADAU pn={pn}, 8 planets + 600 particles, 600 d, outputs every 50 d", flush=True)
+    a, b, rb = _pair(tmp_path, s, 300, "hip", algorithm="RADAU", interval=50.0, pn=pn)
     assert rb.returncode == 0, rb.stdout + rb.stderr
     assert any("Execution backend: HIP" in ln for ln in R.info_lines(b)), R.info_lines(b)
     worst = _compare_dumps(a, b)

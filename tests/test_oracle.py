@@ -430,3 +430,23 @@ def test_ra15_pn_perihelion_advance(oracle):
     xn, vn, *_ = oracle.ra15_run(s, 4000, tol=1e-12, pn=0)
     lrl = np.cross(vn[1], np.cross(xn[1], vn[1])) / MU - xn[1] / np.linalg.norm(xn[1])
     assert abs(np.arctan2(lrl[1], lrl[0])) < 1e-9
+
+
+# ---------------------------------------------------------------- corrector
+def test_golden_mvs2h(oracle):
+    """mco_mvs2h (symplectic -> physical coordinates) on evolved MVS states,
+    with and without oblateness, bit for bit."""
+    g = np.load(GOLD / "mvs2h.npz")
+    for c in range(2):
+        nbig, h = g[f"meta{c}"]
+        s = dict(m=g[f"m{c}"], jcen=g[f"jcen{c}"], nbig=int(nbig), h=h)
+        xh, vh = oracle.mvs2h(s, g[f"x{c}"], g[f"v{c}"])
+        assert np.array_equal(xh[1:], g[f"xh{c}"][1:]) and np.array_equal(vh[1:], g[f"vh{c}"][1:]), c
+
+
+def test_reference_mvs2h(oracle, reference):
+    s = systems.config_small(seed=23, nparticles=80)
+    x, v, _, _ = oracle.mvs_run(s, 30)
+    a = oracle.mvs2h(s, x, v)
+    b = run_big_stack(reference.mvs2h, s, x, v)
+    assert np.array_equal(a[0][1:], b[0][1:]) and np.array_equal(a[1][1:], b[1][1:])
