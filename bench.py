@@ -124,6 +124,43 @@ def ensemble_ratio(hip, steps, spl, block):
             "ratio": round(r_ens / r_one, 3)}
 
 
+def adjoint_cost(hip, nsteps=10000, nparticles=4096):
+    """Orbit-fitting workload: 8 planets + 4,096 massless particles, 10^4 MVS
+    steps of 4 d, loss = sum of squared final position offsets to targets
+    (initial positions + N(0, 1e-3 AU)).  Forward = the plain MVS integration
+    (mercury_hip_advance, no screening); adjoint = the taped forward pass plus
+    the reverse pass giving gradients for every initial state and the 8
+    masses."""
+    import numpy as np
+    from mercuda_amd import systems
+    s = systems.make_system(7, nparticles, nplanets=8, a_range=(0.5, 40.0), emax=0.3, imax_deg=20.0, h=4.0)
+    s["jcen"] = np.zeros(3)
+    nb = s["nbig"]
+    target = s["x"][nb:] + np.random.default_rng(11).normal(0, 1e-3, (nparticles, 3))
+    h = s["h"]
+    hip.configure(1)
+    hip.upload_system(s)
+    hip.advance(0.0, h, 256, s["rcen"], screen=False)          # warm-up
+    hip.upload_system(s)
+    tic = time.perf_counter()
+    hip.advance(0.0, h, nsteps, s["rcen"], screen=False)
+    t_fwd = time.perf_counter() - tic
+    hip.upload_system(s)
+    xT, vT = hip.adjoint_forward(256, h)                      # warm-up
+    hip.adjoint_backward(np.zeros_like(xT), np.zeros_like(vT))
+    tic = time.perf_counter()
+    xT, vT = hip.adjoint_forward(nsteps, h)
+    gx = np.zeros_like(xT)
+    gx[nb:] = 2.0 * (xT[nb:] - target)
+    gx0, gv0, gm = hip.adjoint_backward(gx, np.zeros_like(vT))
+    t_adj = time.perf_counter() - tic
+    return {"workload": f"MVS, 8 planets + {nparticles} particles, {nsteps} steps of 4 d; gradients of "
+                        "sum |x_T - target|^2 w.r.t. all initial states and the 8 masses",
+            "forward_s": round(t_fwd, 4), "adjoint_s": round(t_adj, 4),
+            "adjoint_over_forward": round(t_adj / t_fwd, 3),
+            "checkpoint_interval": int(np.ceil(np.sqrt(nsteps)))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -200,6 +237,8 @@ def main():
     extras = {}
     if not args.no_extras:
         extras["ensemble"] = ensemble_ratio(hip, steps=64, spl=args.spl, block=args.block)
+        log("adjoint: orbit-fitting workload")
+        extras["adjoint"] = adjoint_cost(hip)
     cpu = None
     if not args.no_cpu:
         log(f"cpu baseline: reference mdt_mvs on {args.cpu_procs} processes")

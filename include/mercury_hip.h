@@ -130,6 +130,28 @@ int64_t mercury_hip_parked(void);
  * since timing was last enabled. */
 int mercury_hip_work_read(int64_t *fast_steps, int64_t *general_steps);
 
+/* ---- Differentiation of the MVS map (no counterpart in the reference) ----
+ * For orbit fitting: gradients of a function of the final states with
+ * respect to the initial states of every body and the Big-body masses.
+ * Single uploaded MVS system, jcen = 0, at most 12 Big bodies besides the
+ * central body.  The uploaded state is left unchanged.
+ *
+ * Forward pass of nsteps MVS steps of size h from the uploaded state, keeping
+ * a tape: the planet tables of every step and the particles' states every
+ * `segment` steps (0: ceil(sqrt(nsteps))).  Writes the final states x[3n],
+ * v[3n] (the same numbers mercury_hip_advance produces). */
+int mercury_hip_adjoint_forward(int nsteps, double h, int segment, double *x, double *v);
+/* Reverse pass over the tape: given cotangents gx[3n], gv[3n] of the final
+ * states (rows of Big bodies must be zero), the gradient with respect to the
+ * initial states (gx0[3n], gv0[3n]) and the masses (gm[n]; Big bodies only,
+ * zero elsewhere). */
+int mercury_hip_adjoint_backward(const double *gx, const double *gv, double *gx0, double *gv0,
+                                 double *gm);
+/* Tangent-linear pass over the tape: the derivative of the final states
+ * along the direction (dx0[3n], dv0[3n], dm[n]) of initial states and
+ * Big-body masses, into dx[3n], dv[3n]. */
+int mercury_hip_tangent(const double *dx0, const double *dv0, const double *dm, double *dx, double *dv);
+
 #ifdef __cplusplus
 }
 #endif

@@ -16,6 +16,7 @@
 #include "../../include/mercury_hip.h"
 #include "mvs_device.hpp"
 #include "radau_device.hpp"
+#include "tangent.hpp"
 
 using namespace mhip;
 
@@ -114,6 +115,17 @@ struct Context {
     double *rb = nullptr, *re = nullptr, *ra1 = nullptr;
     unsigned long long* bmax = nullptr;
     long long ra_cap = 0;
+    // differentiation tape (mercury_hip_adjoint_forward)
+    struct Tape {
+        bool valid = false;
+        int steps = 0, seg = 0, nseg = 0, npart = 0, np = 0, nblk = 0;
+        Sys hs{};
+        Sys* dsys = nullptr;
+        double *tab = nullptr, *ck = nullptr, *rec = nullptr, *lam = nullptr, *part = nullptr, *src = nullptr;
+        double *grad = nullptr, *rows = nullptr, *tan = nullptr, *fin = nullptr, *seed = nullptr;
+        double *psave = nullptr;
+        std::vector<double> xfin, vfin;   // final states (host, Fortran layout)
+    } tape;
 };
 
 constexpr int kRing = 4;   // launches in flight: table, counter and request slots
@@ -140,11 +152,17 @@ void sync_all() {
 
 void release_all() {
     Context& c = ctx;
+    if (c.sg) (void)hipStreamSynchronize(c.sg);   // a tape's tangent pass may still run
     for (double** p : {&c.x, &c.v, &c.a, &c.m, &c.rce, &c.rphys, &c.ox, &c.ov, &c.v0k, &c.v1, &c.tab, &c.cx, &c.cv})
         dfree(*p);
     dfree(c.fault); dfree(c.dsys); dfree(c.tcur); c.sys_valid = false; c.sys_cap = 0; c.systems.clear(); dfree(c.dtiles); dfree(c.dev); dfree(c.devcount);
     dfree(c.dpark); dfree(c.dcnt); dfree(c.dslow); dfree(c.work); dfree(c.dreq); c.req_cap = 0;
     dfree(c.rb); dfree(c.re); dfree(c.ra1); dfree(c.bmax); c.ra_cap = 0;
+    for (double** p : {&c.tape.tab, &c.tape.ck, &c.tape.rec, &c.tape.lam, &c.tape.part, &c.tape.src, &c.tape.grad,
+                       &c.tape.rows, &c.tape.tan, &c.tape.fin, &c.tape.seed, &c.tape.psave})
+        dfree(*p);
+    dfree(c.tape.dsys);
+    c.tape.valid = false;
     c.cap = 0; c.tab_cap = 0; c.ev_cap = 0; c.park_cap = 0; c.ntiles = 0;
     c.n = 0; c.nbig = 0; c.history = false; c.saved_valid = false; c.have_state = false;
     c.events.clear(); c.adv_events.clear();
@@ -557,6 +575,275 @@ void radau_step(double time, double* h, double* hdid, double tol, int64_t* force
     c.history = true;
 }
 
+// ---- differentiation tape -------------------------------------------------
+
+// Planet rows (Big bodies 1..nbig-1) of x, v, a <-> psave [9][np].
+void planet_rows_copy(bool save) {
+    Context& c = ctx;
+    const int np = c.nbig - 1;
+    if (np < 1) return;
+    double* arr[3] = {c.x, c.v, c.a};
+    for (int a = 0; a < 3; a++)
+        for (int k = 0; k < 3; k++) {
+            double* dev = arr[a] + (long long)k * c.cap + 1;
+            double* sv = c.tape.psave + (long long)(3 * a + k) * np;
+            HIPCHECK(hipMemcpyAsync(save ? sv : dev, save ? dev : sv, sizeof(double) * np,
+                                    hipMemcpyDeviceToDevice, c.stream));
+        }
+}
+
+AdjArgs adj_args(int k0, int k1) {
+    Context::Tape& t = ctx.tape;
+    AdjArgs a{};
+    a.sys = t.dsys;
+    a.st = device_state();
+    a.tab = t.tab;
+    a.steps = t.steps;
+    a.k0 = k0;
+    a.k1 = k1;
+    a.npart = t.npart;
+    return a;
+}
+
+double* ck_at(int j) { return ctx.tape.ck + (long long)j * 6 * ctx.tape.npart; }
+
+void adjoint_forward(int nsteps, double h, int seg, double* xo, double* vo) {
+    Context& c = ctx;
+    Context::Tape& t = c.tape;
+    if (c.algorithm != 1 && c.algorithm != 9) throw std::runtime_error("differentiation needs the MVS algorithm");
+    if (c.systems.size() != 1) throw std::runtime_error("differentiation needs one system");
+    if (c.jcen[0] != 0 || c.jcen[1] != 0 || c.jcen[2] != 0)
+        throw std::runtime_error("differentiation with oblateness is not provided");
+    if (c.nbig - 1 > 12) throw std::runtime_error("differentiation supports at most 12 Big bodies");
+    if (nsteps < 1 || !(h != 0) || !std::isfinite(h)) throw std::runtime_error("invalid step count or size");
+    t.valid = false;
+    const int np = c.nbig - 1;
+    const int npart = c.n - c.nbig;
+    const int K = seg > 0 ? std::min(seg, nsteps) : (int)std::ceil(std::sqrt((double)nsteps));
+    const int nseg = (nsteps + K - 1) / K;
+    const int R = row_len(np);
+    Sys d{};
+    d.base = 0; d.n = c.n; d.nbig = c.nbig; d.ts = table_stride(np);
+    d.tab = 0; d.start = (long long)nsteps * d.ts;
+    d.mu = c.systems[0].mu; d.h = h; d.hby2 = .5 * h;
+    ensure_stream();
+    ensure_systems(1);
+    // wait for anything still running from an earlier tape
+    HIPCHECK(hipStreamSynchronize(c.sg));
+    dfree(t.tab); dfree(t.ck); dfree(t.rec); dfree(t.lam); dfree(t.part); dfree(t.src);
+    dfree(t.grad); dfree(t.rows); dfree(t.tan); dfree(t.fin); dfree(t.seed);
+    if (!t.dsys) dalloc(t.dsys, 1);
+    if (!t.psave) dalloc(t.psave, 9 * kMaxBig);
+    dalloc(t.tab, (size_t)(nsteps + 1) * d.ts);
+    dalloc(t.ck, (size_t)(nseg + 1) * 6 * std::max(npart, 1));
+    if (np > 0) dalloc(t.rows, (size_t)7 * np * (nsteps + 1) * R);
+    HIPCHECK(hipMemcpy(t.dsys, &d, sizeof(Sys), hipMemcpyHostToDevice));
+    t.hs = d; t.steps = nsteps; t.seg = K; t.nseg = nseg; t.npart = npart; t.np = np;
+    t.nblk = (npart + 127) / 128;
+    planet_rows_copy(true);
+    HIPCHECK(hipEventRecord(c.ev_plan[0], c.stream));
+    // the planets' tangent rows of every unit direction, on their own stream
+    if (np > 0) {
+        HIPCHECK(hipStreamWaitEvent(c.sg, c.ev_plan[0], 0));
+        TlmArgs ta{t.dsys, c.m, t.psave, t.psave + 3 * np, nsteps, nullptr, t.rows, nullptr};
+        hipLaunchKernelGGL(tlm_planets, dim3(7 * np), dim3(64), 0, c.sg, ta);
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipEventRecord(c.ev_done[0], c.sg));
+    }
+    // planets: the tables, one launch per checkpoint segment, on the planet
+    // stream; the particle segment j runs once tables up to its end exist
+    HIPCHECK(hipStreamWaitEvent(c.sp, c.ev_plan[0], 0));
+    State st = device_state();
+    EventSink es{c.dev, c.devcount, c.ev_cap, c.fault};
+    for (int j = 0; j < nseg; j++) {
+        const int k0 = j * K, steps = std::min(nsteps, (j + 1) * K) - k0;
+        PlanetArgs pa{t.dsys, st, t.tab + (long long)k0 * d.ts, c.tcur, es, 0.0, 1, steps, k0 == 0 ? 1 : 0,
+                      0, 0, nullptr, 0, k0};
+        hipLaunchKernelGGL(mvs_planets, dim3(1), dim3(64), 0, c.sp, pa);
+        HIPCHECK(hipGetLastError());
+        hipEvent_t ev = c.ev_fast[j % kRing];
+        HIPCHECK(hipEventRecord(ev, c.sp));
+        if (npart > 0) {
+            HIPCHECK(hipStreamWaitEvent(c.stream, ev, 0));
+            AdjArgs a = adj_args(k0, k0 + steps);
+            a.ck_in = j == 0 ? nullptr : ck_at(j);
+            a.ck_out = ck_at(j + 1);
+            hipLaunchKernelGGL(adj_forward, dim3(t.nblk), dim3(128), 0, c.stream, a);
+            HIPCHECK(hipGetLastError());
+        }
+    }
+    HIPCHECK(hipStreamSynchronize(c.sp));
+    // final planet states before the restore
+    t.xfin.assign((size_t)3 * c.n, 0.0);
+    t.vfin.assign((size_t)3 * c.n, 0.0);
+    std::vector<double> row((size_t)std::max(np, 1));
+    for (int k = 0; k < 3 && np > 0; k++) {
+        HIPCHECK(hipMemcpy(row.data(), c.x + (long long)k * c.cap + 1, sizeof(double) * np, hipMemcpyDeviceToHost));
+        for (int p = 0; p < np; p++) t.xfin[3 * (p + 1) + k] = row[p];
+        HIPCHECK(hipMemcpy(row.data(), c.v + (long long)k * c.cap + 1, sizeof(double) * np, hipMemcpyDeviceToHost));
+        for (int p = 0; p < np; p++) t.vfin[3 * (p + 1) + k] = row[p];
+    }
+    planet_rows_copy(false);
+    if (npart > 0) {
+        std::vector<double> fin((size_t)6 * npart);
+        HIPCHECK(hipMemcpyAsync(fin.data(), ck_at(nseg), sizeof(double) * 6 * npart, hipMemcpyDeviceToHost,
+                                c.stream));
+        HIPCHECK(hipStreamSynchronize(c.stream));
+        for (int i = 0; i < npart; i++)
+            for (int k = 0; k < 3; k++) {
+                t.xfin[3 * (c.nbig + i) + k] = fin[(size_t)k * npart + i];
+                t.vfin[3 * (c.nbig + i) + k] = fin[(size_t)(3 + k) * npart + i];
+            }
+    }
+    HIPCHECK(hipStreamSynchronize(c.stream));
+    std::memcpy(xo, t.xfin.data(), sizeof(double) * 3 * c.n);
+    std::memcpy(vo, t.vfin.data(), sizeof(double) * 3 * c.n);
+    t.valid = true;
+}
+
+void require_tape() {
+    if (!ctx.tape.valid) throw std::runtime_error("no tape: call mercury_hip_adjoint_forward first");
+}
+
+void adjoint_backward(const double* gx, const double* gv, double* gx0, double* gv0, double* gm) {
+    Context& c = ctx;
+    Context::Tape& t = c.tape;
+    require_tape();
+    const int np = t.np, npart = t.npart, R = row_len(np), T = t.steps;
+    for (int j = 0; j < c.nbig; j++)
+        for (int k = 0; k < 3; k++)
+            if (gx[3 * j + k] != 0 || gv[3 * j + k] != 0)
+                throw std::runtime_error("cotangents of Big-body final states are not provided");
+    dfree(t.src); dfree(t.grad);
+    dalloc(t.src, (size_t)(T + 1) * R);
+    dalloc(t.grad, (size_t)std::max(7 * np, 1));
+    std::vector<double> lam0((size_t)6 * std::max(npart, 1));
+    if (npart > 0) {
+        if (!t.lam) dalloc(t.lam, (size_t)6 * npart);
+        if (!t.rec) dalloc(t.rec, (size_t)t.seg * 7 * npart);
+        if (!t.part) dalloc(t.part, (size_t)(T + 1) * t.nblk * R);
+        if (!t.fin) dalloc(t.fin, (size_t)6 * npart);
+        for (int i = 0; i < npart; i++)
+            for (int k = 0; k < 3; k++) {
+                lam0[(size_t)k * npart + i] = gx[3 * (c.nbig + i) + k];
+                lam0[(size_t)(3 + k) * npart + i] = gv[3 * (c.nbig + i) + k];
+            }
+        HIPCHECK(hipMemcpyAsync(t.lam, lam0.data(), sizeof(double) * 6 * npart, hipMemcpyHostToDevice, c.stream));
+        for (int j = t.nseg - 1; j >= 0; j--) {
+            const int k0 = j * t.seg, k1 = std::min(T, (j + 1) * t.seg);
+            AdjArgs a = adj_args(k0, k1);
+            a.ck_in = j == 0 ? nullptr : ck_at(j);
+            a.rec = t.rec;
+            hipLaunchKernelGGL(adj_forward, dim3(t.nblk), dim3(128), 0, c.stream, a);
+            a.lam = t.lam;
+            a.xfin = ck_at(t.nseg);
+            a.fin_ell = t.fin;
+            hipLaunchKernelGGL(adj_reverse, dim3(t.nblk), dim3(128), 0, c.stream, a);
+            a.part = t.part;
+            hipLaunchKernelGGL(adj_rows, dim3(k1 - k0 + (k1 == T ? 1 : 0), t.nblk), dim3(128), 0, c.stream, a);
+            HIPCHECK(hipGetLastError());
+        }
+        long long tot = (long long)(T + 1) * R;
+        hipLaunchKernelGGL(adj_sum_rows, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c.stream, t.part, t.src,
+                           T + 1, t.nblk, R);
+        HIPCHECK(hipGetLastError());
+    } else {
+        HIPCHECK(hipMemsetAsync(t.src, 0, sizeof(double) * (T + 1) * R, c.stream));
+    }
+    std::vector<double> grad((size_t)std::max(7 * np, 1), 0.0);
+    if (np > 0) {
+        HIPCHECK(hipStreamWaitEvent(c.stream, c.ev_done[0], 0));
+        hipLaunchKernelGGL(adj_contract, dim3(7 * np), dim3(256), 0, c.stream, t.src, t.rows, t.grad, T + 1, R);
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipMemcpyAsync(grad.data(), t.grad, sizeof(double) * 7 * np, hipMemcpyDeviceToHost, c.stream));
+    }
+    if (npart > 0)
+        HIPCHECK(hipMemcpyAsync(lam0.data(), t.lam, sizeof(double) * 6 * npart, hipMemcpyDeviceToHost, c.stream));
+    HIPCHECK(hipStreamSynchronize(c.stream));
+    std::fill(gx0, gx0 + 3 * c.n, 0.0);
+    std::fill(gv0, gv0 + 3 * c.n, 0.0);
+    std::fill(gm, gm + c.n, 0.0);
+    for (int p = 0; p < np; p++) {
+        for (int k = 0; k < 3; k++) {
+            gx0[3 * (p + 1) + k] = grad[3 * p + k];
+            gv0[3 * (p + 1) + k] = grad[3 * np + 3 * p + k];
+        }
+        gm[p + 1] = grad[6 * np + p];
+    }
+    for (int i = 0; i < npart; i++)
+        for (int k = 0; k < 3; k++) {
+            gx0[3 * (c.nbig + i) + k] = lam0[(size_t)k * npart + i];
+            gv0[3 * (c.nbig + i) + k] = lam0[(size_t)(3 + k) * npart + i];
+        }
+}
+
+void tangent_pass(const double* dx0, const double* dv0, const double* dm, double* dx, double* dv) {
+    Context& c = ctx;
+    Context::Tape& t = c.tape;
+    require_tape();
+    const int np = t.np, npart = t.npart, R = row_len(np), T = t.steps;
+    std::fill(dx, dx + 3 * c.n, 0.0);
+    std::fill(dv, dv + 3 * c.n, 0.0);
+    double* rows1 = nullptr;
+    dalloc(rows1, (size_t)(T + 1) * R);
+    try {
+        if (np > 0) {
+            std::vector<double> seed((size_t)7 * np);
+            for (int p = 0; p < np; p++) {
+                for (int k = 0; k < 3; k++) {
+                    seed[3 * p + k] = dx0[3 * (p + 1) + k];
+                    seed[3 * np + 3 * p + k] = dv0[3 * (p + 1) + k];
+                }
+                seed[6 * np + p] = dm[p + 1];
+            }
+            if (!t.seed) dalloc(t.seed, 7 * kMaxBig);
+            double* fin = nullptr;
+            dalloc(fin, 6 * kMaxBig);
+            HIPCHECK(hipMemcpyAsync(t.seed, seed.data(), sizeof(double) * 7 * np, hipMemcpyHostToDevice, c.stream));
+            TlmArgs ta{t.dsys, c.m, t.psave, t.psave + 3 * np, T, t.seed, rows1, fin};
+            hipLaunchKernelGGL(tlm_planets, dim3(1), dim3(64), 0, c.stream, ta);
+            HIPCHECK(hipGetLastError());
+            std::vector<double> fh((size_t)6 * np);
+            HIPCHECK(hipMemcpyAsync(fh.data(), fin, sizeof(double) * 6 * np, hipMemcpyDeviceToHost, c.stream));
+            HIPCHECK(hipStreamSynchronize(c.stream));
+            dfree(fin);
+            for (int p = 0; p < np; p++)
+                for (int k = 0; k < 3; k++) {
+                    dx[3 * (p + 1) + k] = fh[3 * p + k];
+                    dv[3 * (p + 1) + k] = fh[3 * np + 3 * p + k];
+                }
+        } else {
+            HIPCHECK(hipMemsetAsync(rows1, 0, sizeof(double) * (T + 1) * R, c.stream));
+        }
+        if (npart > 0) {
+            std::vector<double> tv((size_t)6 * npart);
+            for (int i = 0; i < npart; i++)
+                for (int k = 0; k < 3; k++) {
+                    tv[(size_t)k * npart + i] = dx0[3 * (c.nbig + i) + k];
+                    tv[(size_t)(3 + k) * npart + i] = dv0[3 * (c.nbig + i) + k];
+                }
+            if (!t.tan) dalloc(t.tan, (size_t)6 * npart);
+            HIPCHECK(hipMemcpyAsync(t.tan, tv.data(), sizeof(double) * 6 * npart, hipMemcpyHostToDevice, c.stream));
+            AdjArgs a = adj_args(0, T);
+            a.rows = rows1;
+            a.tan = t.tan;
+            hipLaunchKernelGGL(tlm_particles, dim3(t.nblk), dim3(128), 0, c.stream, a);
+            HIPCHECK(hipGetLastError());
+            HIPCHECK(hipMemcpyAsync(tv.data(), t.tan, sizeof(double) * 6 * npart, hipMemcpyDeviceToHost, c.stream));
+            HIPCHECK(hipStreamSynchronize(c.stream));
+            for (int i = 0; i < npart; i++)
+                for (int k = 0; k < 3; k++) {
+                    dx[3 * (c.nbig + i) + k] = tv[(size_t)k * npart + i];
+                    dv[3 * (c.nbig + i) + k] = tv[(size_t)(3 + k) * npart + i];
+                }
+        }
+    } catch (...) {
+        dfree(rows1);
+        throw;
+    }
+    dfree(rows1);
+}
+
 int read_fault() {
     int bad = 0;
     HIPCHECK(hipMemcpy(&bad, ctx.fault, sizeof(int), hipMemcpyDeviceToHost));
@@ -663,6 +950,7 @@ int mercury_hip_upload(int n, int nbig, int pn, int ngflag, const double* m, con
         ctx.history = false;
         ctx.saved_valid = false;
         ctx.have_state = true;
+        ctx.tape.valid = false;
         HIPCHECK(hipDeviceSynchronize());
         return 0;
     } catch (const std::exception& e) { return fail(e); }
@@ -676,12 +964,14 @@ int mercury_hip_step(double time, double* h, double* hdid, double tol, int64_t* 
         *forces = 0;
         if (ctx.algorithm == 4) {
             clear_fault();
+            ctx.tape.valid = false;
             radau_step(time, h, hdid, tol, forces, rejected);
             ctx.saved_valid = true;
             return 0;
         }
         if (!mvs_family()) throw std::runtime_error("algorithm not provided by the HIP path");
         clear_fault();
+        ctx.tape.valid = false;
         bool reset = !ctx.history;
         mvs_run(time, *h, 1, reset, false, true, 0.0);
         check_fault("MVS force failed");
@@ -813,6 +1103,7 @@ int mercury_hip_advance(double time, double h, int nsteps, double rcen, int scre
         if (ctx.systems.size() == 1 && (!(h != 0) || !std::isfinite(h))) throw std::runtime_error("invalid step");
         ensure_stream();
         clear_fault();
+        ctx.tape.valid = false;
         HIPCHECK(hipMemset(ctx.devcount, 0, sizeof(int)));
         ctx.adv_events.clear();
         bool reset = !ctx.history;
@@ -882,6 +1173,7 @@ int mercury_hip_ensemble_upload(int nsys, const int* n, const int* nbig, const d
         ctx.history = false;
         ctx.saved_valid = false;
         ctx.have_state = true;
+        ctx.tape.valid = false;
         HIPCHECK(hipDeviceSynchronize());
         return 0;
     } catch (const std::exception& e) { return fail(e); }
@@ -944,6 +1236,32 @@ int mercury_hip_timing_read(double* particle_ms, int* particle_launches, double*
 }
 
 int64_t mercury_hip_parked(void) { return ctx.parked; }
+
+int mercury_hip_adjoint_forward(int nsteps, double h, int segment, double* x, double* v) {
+    try {
+        require_state();
+        clear_fault();
+        adjoint_forward(nsteps, h, segment, x, v);
+        check_fault("MVS force failed");
+        return 0;
+    } catch (const std::exception& e) { return fail(e); }
+}
+
+int mercury_hip_adjoint_backward(const double* gx, const double* gv, double* gx0, double* gv0, double* gm) {
+    try {
+        require_state();
+        adjoint_backward(gx, gv, gx0, gv0, gm);
+        return 0;
+    } catch (const std::exception& e) { return fail(e); }
+}
+
+int mercury_hip_tangent(const double* dx0, const double* dv0, const double* dm, double* dx, double* dv) {
+    try {
+        require_state();
+        tangent_pass(dx0, dv0, dm, dx, dv);
+        return 0;
+    } catch (const std::exception& e) { return fail(e); }
+}
 
 int mercury_hip_work_read(int64_t* fast_steps, int64_t* general_steps) {
     try {

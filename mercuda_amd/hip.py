@@ -65,6 +65,9 @@ SIGNATURES = {
     "mercury_hip_ensemble_upload": (C.c_int, [C.c_int, _ip, _ip, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
     "mercury_hip_ensemble_advance": (C.c_int, [C.c_double, C.c_int, C.c_int, _i64p]),
     "mercury_hip_ensemble_download": (C.c_int, [_dp, _dp]),
+    "mercury_hip_adjoint_forward": (C.c_int, [C.c_int, C.c_double, C.c_int, _dp, _dp]),
+    "mercury_hip_adjoint_backward": (C.c_int, [_dp, _dp, _dp, _dp, _dp]),
+    "mercury_hip_tangent": (C.c_int, [_dp, _dp, _dp, _dp, _dp]),
 }
 
 
@@ -221,6 +224,28 @@ class MercuryHip:
         self._ok(self.L.mercury_hip_ensemble_download(_d(x), _d(v)), "ensemble_download")
         o = self.ens_offsets
         return [(x[o[k]:o[k + 1]], v[o[k]:o[k + 1]]) for k in range(len(o) - 1)]
+
+    # ---- differentiation (tape of the last adjoint_forward) ----
+    def adjoint_forward(self, nsteps, h, segment=0):
+        """Final (x, v) after nsteps MVS steps of the uploaded state; keeps the tape."""
+        x = np.zeros((self.n, 3)); v = np.zeros((self.n, 3))
+        self._ok(self.L.mercury_hip_adjoint_forward(int(nsteps), float(h), int(segment), _d(x), _d(v)),
+                 "adjoint_forward")
+        return x, v
+
+    def adjoint_backward(self, gx, gv):
+        """Gradient of <gx, x_T> + <gv, v_T> w.r.t. initial x, v and masses."""
+        gx = _f64(gx, (self.n, 3)); gv = _f64(gv, (self.n, 3))
+        gx0 = np.zeros((self.n, 3)); gv0 = np.zeros((self.n, 3)); gm = np.zeros(self.n)
+        self._ok(self.L.mercury_hip_adjoint_backward(_d(gx), _d(gv), _d(gx0), _d(gv0), _d(gm)), "adjoint_backward")
+        return gx0, gv0, gm
+
+    def tangent(self, dx0, dv0, dm):
+        """Derivative of the final (x, v) along (dx0, dv0, dm)."""
+        dx0 = _f64(dx0, (self.n, 3)); dv0 = _f64(dv0, (self.n, 3)); dm = _f64(dm, (self.n,))
+        dx = np.zeros((self.n, 3)); dv = np.zeros((self.n, 3))
+        self._ok(self.L.mercury_hip_tangent(_d(dx0), _d(dv0), _d(dm), _d(dx), _d(dv)), "tangent")
+        return dx, dv
 
     def free(self):
         self.L.mercury_hip_free()
