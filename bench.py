@@ -95,33 +95,47 @@ def _cpu_worker(job):
 def ensemble_ratio(hip, steps, spl, block):
     """Batched ensemble (512 systems, 2..12 planets, log-U[100, 5e4] particles,
     dt 4 d) against one 8-planet system with the same total particle count,
-    both through mercury_hip_advance with screening on."""
+    both through mercury_hip_advance with screening on.  Each side is
+    autotuned (steps per launch, workgroup size) on its own state and runs
+    as a replayed hipGraph; the timed call is a replay."""
     from mercuda_amd import systems
     log("ensemble: generating 512 systems")
     ens = systems.ensemble(seed=4)
     ntot = sum(len(q["m"]) - q["nbig"] for q in ens)
+
+    def timed(advance):
+        advance(0.0, 16)                          # first call (acceleration reset)
+        advance(16 * 4.0, steps)                  # captures the steady shape
+        hip.graph_stats()
+        tic = time.perf_counter()
+        advance((16 + steps) * 4.0, steps)        # replay
+        return time.perf_counter() - tic
+
     hip.configure(1)
     hip.ensemble_upload(ens)
     hip.set_tuning(steps_per_launch=spl, block_size=block)
-    hip.ensemble_advance(0.0, 16, screen=True)
-    tic = time.perf_counter()
-    hip.ensemble_advance(16 * 4.0, steps, screen=True)
-    t_ens = time.perf_counter() - tic
+    tune_e = hip.autotune(probe_steps=32)
+    hip.set_graph(True)
+    t_ens = timed(lambda t, n: hip.ensemble_advance(t, n, screen=True))
+    hip.set_graph(False)
     log(f"ensemble: single system with {ntot} particles")
     one = systems.config_large(seed=2, nparticles=ntot)
     hip.configure(1)
     hip.upload_system(one)
     hip.set_tuning(steps_per_launch=spl, block_size=block)
-    hip.advance(0.0, one["h"], 16, one["rcen"], screen=True)
-    tic = time.perf_counter()
-    hip.advance(16 * one["h"], one["h"], steps, one["rcen"], screen=True)
-    t_one = time.perf_counter() - tic
+    tune_1 = hip.autotune(h=one["h"], probe_steps=32)
+    hip.set_graph(True)
+    t_one = timed(lambda t, n: hip.advance(t, one["h"], n, one["rcen"], screen=True))
+    hip.set_graph(False)
     r_ens = ntot * steps / t_ens
     r_one = ntot * steps / t_one
     return {"systems": len(ens), "particles": ntot, "steps": steps,
             "ensemble_particle_steps_per_s": round(r_ens, 1),
             "single_system_particle_steps_per_s": round(r_one, 1),
-            "ratio": round(r_ens / r_one, 3)}
+            "ratio": round(r_ens / r_one, 3),
+            "ensemble_tuning": {"steps_per_launch": tune_e["steps_per_launch"], "block": tune_e["block"]},
+            "single_tuning": {"steps_per_launch": tune_1["steps_per_launch"], "block": tune_1["block"]},
+            "graph": hip.graph_stats()}
 
 
 def adjoint_cost(hip, nsteps=10000, nparticles=4096):
@@ -173,6 +187,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-procs", type=int, default=16)
     ap.add_argument("--no-extras", action="store_true", help="skip the ensemble/single ratio")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="use --spl/--block as given instead of the autotuner's choice")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the massless-particle path has no multi-GPU mode in this build "
@@ -193,6 +209,12 @@ def main():
     hip.set_tuning(steps_per_launch=args.spl, block_size=args.block)
     screen = not args.no_screen
     h = sysd["h"]
+    spl, block = args.spl, args.block
+    if not args.no_autotune:
+        log("autotuning steps per launch and workgroup size (32-step probes, state restored)")
+        tune = hip.autotune(h=h, probe_steps=32)
+        spl, block = tune["steps_per_launch"], tune["block"]
+        log(f"autotuner: {tune}")
     log(f"warmup {args.warmup} steps")
     hip.advance(0.0, h, args.warmup, sysd["rcen"], screen=screen)
     t0 = args.warmup * h
@@ -236,7 +258,7 @@ def main():
     }
     extras = {}
     if not args.no_extras:
-        extras["ensemble"] = ensemble_ratio(hip, steps=64, spl=args.spl, block=args.block)
+        extras["ensemble"] = ensemble_ratio(hip, steps=64, spl=spl, block=block)
         log("adjoint: orbit-fitting workload")
         extras["adjoint"] = adjoint_cost(hip)
     cpu = None
@@ -267,7 +289,8 @@ def main():
         "config": {"workload": f"MVS, 8 planets + {npart} massless particles, dt 4 d, "
                                f"encounter screening {'on' if screen else 'off'}",
                    "particles": npart, "planets": 8, "dt_days": h,
-                   "steps_per_launch": args.spl, "block": args.block, "parallelism": "1 GPU"},
+                   "steps_per_launch": spl, "block": block,
+                   "tuning": "autotuned" if not args.no_autotune else "fixed", "parallelism": "1 GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "extras": extras,

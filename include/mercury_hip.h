@@ -130,6 +130,22 @@ int64_t mercury_hip_parked(void);
  * since timing was last enabled. */
 int mercury_hip_work_read(int64_t *fast_steps, int64_t *general_steps);
 
+/* hipGraph mode for mercury_hip_advance / ensemble advance (no counterpart in
+ * the reference): enable = 1 captures a call's whole launch sequence (all
+ * streams) into a graph and replays it for later calls with the same step
+ * count, flags, tuning and buffers; results are bitwise those of the eager
+ * launches.  Off by default; ignored while kernel timing is on. */
+int mercury_hip_set_graph(int enable);
+/* Graphs captured and replays made since the library was loaded. */
+int mercury_hip_graph_stats(int *captures, int *replays);
+/* Autotuner (no counterpart in the reference): times probe_steps MVS steps of
+ * the uploaded system or ensemble for every steps-per-launch in {8,16,32,64}
+ * and workgroup size in {64,128,256} (h: the step of a single system; ensemble
+ * members use their own), keeps the fastest (mercury_hip_set_tuning) and
+ * leaves the state as it was.  Returns the choice and its particle-steps/s. */
+int mercury_hip_autotune(double h, int probe_steps, int *steps_per_launch, int *block_size,
+                         double *particle_steps_per_s);
+
 /* ---- Differentiation of the MVS map (no counterpart in the reference) ----
  * For orbit fitting: gradients of a function of the final states with
  * respect to the initial states of every body and the Big-body masses.

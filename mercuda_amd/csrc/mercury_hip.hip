@@ -5,6 +5,7 @@
 // bodies, body 0 the central body, Big bodies 1..nbig-1, particles after.
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -131,13 +132,37 @@ struct Context {
 constexpr int kRing = 4;   // launches in flight: table, counter and request slots
 Context ctx;
 
+// What a captured advance depends on: replay only when all of it is unchanged.
+struct GraphKey {
+    int nsteps, screen, reset, spl, block, gpw, ntiles, nsys, ev_cap, req_cap, park_cap;
+    long long cap, slot;
+    const void* ptr[9];
+    bool operator==(const GraphKey& o) const { return std::memcmp(this, &o, sizeof(GraphKey)) == 0; }
+};
+
+struct GraphCache {
+    bool enabled = false;
+    bool valid = false;
+    GraphKey key{};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int captures = 0, replays = 0;
+    void drop() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+        valid = false;
+    }
+} graphs;
+
 void ensure_stream() {
     if (ctx.stream) return;
     if (const char* e = std::getenv("MERCURY_HIP_GEN_PER_WAVE")) ctx.gen_per_wave = std::max(1, std::min(64, std::atoi(e)));
     HIPCHECK(hipStreamCreateWithFlags(&ctx.stream, hipStreamNonBlocking));
     HIPCHECK(hipStreamCreateWithFlags(&ctx.sp, hipStreamNonBlocking));
     HIPCHECK(hipStreamCreateWithFlags(&ctx.sg, hipStreamNonBlocking));
-    for (int k = 0; k < kRing; k++) {
+    for (int k = 0; k <= kRing; k++) {
         HIPCHECK(hipEventCreateWithFlags(&ctx.ev_plan[k], hipEventDisableTiming));
         HIPCHECK(hipEventCreateWithFlags(&ctx.ev_fast[k], hipEventDisableTiming));
         HIPCHECK(hipEventCreateWithFlags(&ctx.ev_done[k], hipEventDisableTiming));
@@ -153,6 +178,7 @@ void sync_all() {
 void release_all() {
     Context& c = ctx;
     if (c.sg) (void)hipStreamSynchronize(c.sg);   // a tape's tangent pass may still run
+    graphs.drop();
     for (double** p : {&c.x, &c.v, &c.a, &c.m, &c.rce, &c.rphys, &c.ox, &c.ov, &c.v0k, &c.v1, &c.tab, &c.cx, &c.cv})
         dfree(*p);
     dfree(c.fault); dfree(c.dsys); dfree(c.tcur); c.sys_valid = false; c.sys_cap = 0; c.systems.clear(); dfree(c.dtiles); dfree(c.dev); dfree(c.devcount);
@@ -380,13 +406,14 @@ std::vector<int> chunk_plan(int nsteps, int smax) {
 //   general stream:  slow-set particles, then the deferred screening details
 // The planet stream runs up to kRing launches ahead; the general kernel of
 // launch k overlaps the fast kernel of launch k+1 (disjoint particles).
-void mvs_run(double time, double h, int nsteps, bool reset, bool screen, bool save, double rcen) {
+// Enqueue only (no host synchronisation): usable under stream capture.
+// device_time: every planet launch reads the systems' clocks from the device
+// (set beforehand) instead of taking the call's time as an argument.
+void mvs_enqueue(double time, double h, int nsteps, bool reset, bool screen, bool save, bool device_time) {
     Context& c = ctx;
     const int smax = save ? 1 : c.steps_per_launch;
     const int nsys = (int)c.systems.size();
     const bool single = nsys == 1;
-    if (save && !single) throw std::runtime_error("single-step mode needs one system");
-    build_systems(smax, single, h, rcen);
     State st = device_state();
     EventSink es{c.dev, c.devcount, c.ev_cap, c.fault};
     const long long nb = c.n;
@@ -414,8 +441,8 @@ void mvs_run(double time, double h, int nsteps, bool reset, bool screen, bool sa
         hipEvent_t t0 = nullptr;
         if (k >= kRing) HIPCHECK(hipStreamWaitEvent(c.sp, c.ev_done[slot], 0));
         // one system follows the caller's clock; ensemble members carry theirs on the device
-        PlanetArgs pa{c.dsys, st, tab, c.tcur, es, t, (single || k == 0) ? 1 : 0, steps, rs0 ? 1 : 0,
-                      screen ? 1 : 0, save ? 1 : 0, reqcount, 1, done};
+        PlanetArgs pa{c.dsys, st, tab, c.tcur, es, t, (!device_time && (single || k == 0)) ? 1 : 0, steps,
+                      rs0 ? 1 : 0, screen ? 1 : 0, save ? 1 : 0, reqcount, 1, done};
         timed_launch_begin(c.sp, t0);
         hipLaunchKernelGGL(mvs_planets, dim3(nsys), dim3(64), 0, c.sp, pa);
         HIPCHECK(hipGetLastError());
@@ -449,7 +476,68 @@ void mvs_run(double time, double h, int nsteps, bool reset, bool screen, bool sa
         for (int q = 0; q < steps; q++) t = t + h;
         done += steps;
     }
-    sync_all();
+}
+
+// The planet and general streams' work joins the main stream.
+void join_streams() {
+    Context& c = ctx;
+    HIPCHECK(hipEventRecord(c.ev_plan[kRing], c.sp));
+    HIPCHECK(hipEventRecord(c.ev_fast[kRing], c.sg));
+    HIPCHECK(hipStreamWaitEvent(c.stream, c.ev_plan[kRing], 0));
+    HIPCHECK(hipStreamWaitEvent(c.stream, c.ev_fast[kRing], 0));
+}
+
+__global__ void set_clocks(double* tcur, int nsys, double time) {
+    for (int s = threadIdx.x; s < nsys; s += blockDim.x) tcur[s] = time;
+}
+
+
+void mvs_run(double time, double h, int nsteps, bool reset, bool screen, bool save, double rcen) {
+    Context& c = ctx;
+    const int smax = save ? 1 : c.steps_per_launch;
+    const int nsys = (int)c.systems.size();
+    const bool single = nsys == 1;
+    if (save && !single) throw std::runtime_error("single-step mode needs one system");
+    build_systems(smax, single, h, rcen);
+    if (!graphs.enabled || save || c.timing.on) {
+        mvs_enqueue(time, h, nsteps, reset, screen, save, false);
+        sync_all();
+        return;
+    }
+    // hipGraph: capture the call's whole launch sequence once, replay it for
+    // later calls of the same shape; the clocks come from the device
+    GraphKey k;
+    std::memset(&k, 0, sizeof(k));
+    k.nsteps = nsteps; k.screen = screen; k.reset = reset; k.spl = c.steps_per_launch; k.block = c.block;
+    k.gpw = c.gen_per_wave; k.ntiles = c.ntiles; k.nsys = nsys; k.ev_cap = c.ev_cap; k.req_cap = c.req_cap;
+    k.park_cap = c.park_cap; k.cap = c.cap; k.slot = c.slot_doubles;
+    const void* ptrs[9] = {c.dsys, c.dtiles, c.tab, c.dev, c.dreq, c.dpark, c.x, c.tcur, c.dslow};
+    for (int q = 0; q < 9; q++) k.ptr[q] = ptrs[q];
+    hipLaunchKernelGGL(set_clocks, dim3(1), dim3(256), 0, c.stream, c.tcur, nsys, time);
+    HIPCHECK(hipGetLastError());
+    if (!(graphs.valid && graphs.key == k)) {
+        graphs.drop();
+        HIPCHECK(hipStreamSynchronize(c.stream));
+        HIPCHECK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+        try {
+            mvs_enqueue(time, h, nsteps, reset, screen, false, true);
+            join_streams();
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(c.stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        HIPCHECK(hipStreamEndCapture(c.stream, &graphs.graph));
+        HIPCHECK(hipGraphInstantiate(&graphs.exec, graphs.graph, nullptr, nullptr, 0));
+        graphs.key = k;
+        graphs.valid = true;
+        graphs.captures++;
+    } else {
+        graphs.replays++;
+    }
+    HIPCHECK(hipGraphLaunch(graphs.exec, c.stream));
+    HIPCHECK(hipStreamSynchronize(c.stream));
 }
 
 // ---- RA15 (algorithm 4) -----------------------------------------------------
@@ -1261,6 +1349,89 @@ int mercury_hip_tangent(const double* dx0, const double* dv0, const double* dm, 
         tangent_pass(dx0, dv0, dm, dx, dv);
         return 0;
     } catch (const std::exception& e) { return fail(e); }
+}
+
+int mercury_hip_set_graph(int enable) {
+    graphs.enabled = enable != 0;
+    if (!graphs.enabled) graphs.drop();
+    return 0;
+}
+
+int mercury_hip_graph_stats(int* captures, int* replays) {
+    *captures = graphs.captures;
+    *replays = graphs.replays;
+    return 0;
+}
+
+int mercury_hip_autotune(double h, int probe_steps, int* spl_out, int* block_out, double* rate_out) {
+    Context& c = ctx;
+    double* save = nullptr;
+    double* tsave = nullptr;
+    try {
+        require_state();
+        if (!mvs_family()) throw std::runtime_error("autotuning needs the MVS algorithm");
+        if (probe_steps < 1) throw std::runtime_error("probe needs at least one step");
+        const bool single = c.systems.size() == 1;
+        if (single && (!(h != 0) || !std::isfinite(h))) throw std::runtime_error("invalid step");
+        ensure_stream();
+        const int nsys = (int)c.systems.size();
+        ensure_systems(nsys);
+        // the state is put back after every probe
+        dalloc(save, (size_t)9 * c.cap);
+        dalloc(tsave, (size_t)nsys);
+        double* arr[3] = {c.x, c.v, c.a};
+        for (int q = 0; q < 3; q++)
+            HIPCHECK(hipMemcpy(save + (size_t)q * 3 * c.cap, arr[q], sizeof(double) * 3 * c.cap, hipMemcpyDeviceToDevice));
+        HIPCHECK(hipMemcpy(tsave, c.tcur, sizeof(double) * nsys, hipMemcpyDeviceToDevice));
+        const bool hist = c.history;
+        const int spl0 = c.steps_per_launch, block0 = c.block;
+        auto restore = [&]() {
+            for (int q = 0; q < 3; q++)
+                HIPCHECK(hipMemcpy(arr[q], save + (size_t)q * 3 * c.cap, sizeof(double) * 3 * c.cap,
+                                   hipMemcpyDeviceToDevice));
+            HIPCHECK(hipMemcpy(c.tcur, tsave, sizeof(double) * nsys, hipMemcpyDeviceToDevice));
+            c.history = hist;
+        };
+        const int spls[4] = {8, 16, 32, 64};
+        const int blocks[3] = {64, 128, 256};
+        double best = -1.0;
+        int bspl = spl0, bblock = block0;
+        const double rc = single ? c.systems[0].rcen : 0.0;
+        for (int bi = 0; bi < 3; bi++)
+            for (int si = 0; si < 4; si++) {
+                c.steps_per_launch = spls[si];
+                c.block = blocks[bi];
+                build_tiles();
+                double sec = 0.0;
+                for (int rep = 0; rep < 2; rep++) {   // the first run warms up (and captures a graph)
+                    restore();
+                    clear_fault();
+                    HIPCHECK(hipMemset(c.devcount, 0, sizeof(int)));
+                    HIPCHECK(hipDeviceSynchronize());
+                    auto t0 = std::chrono::steady_clock::now();
+                    mvs_run(0.0, h, probe_steps, !c.history, true, false, rc);
+                    sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+                }
+                check_fault("MVS force failed");
+                double rate = (double)c.npart * probe_steps / sec;
+                if (rate > best) { best = rate; bspl = spls[si]; bblock = blocks[bi]; }
+            }
+        c.steps_per_launch = bspl;
+        c.block = bblock;
+        build_tiles();
+        restore();
+        HIPCHECK(hipDeviceSynchronize());
+        dfree(save);
+        dfree(tsave);
+        *spl_out = bspl;
+        *block_out = bblock;
+        *rate_out = best;
+        return 0;
+    } catch (const std::exception& e) {
+        dfree(save);
+        dfree(tsave);
+        return fail(e);
+    }
 }
 
 int mercury_hip_work_read(int64_t* fast_steps, int64_t* general_steps) {

@@ -65,6 +65,9 @@ SIGNATURES = {
     "mercury_hip_ensemble_upload": (C.c_int, [C.c_int, _ip, _ip, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
     "mercury_hip_ensemble_advance": (C.c_int, [C.c_double, C.c_int, C.c_int, _i64p]),
     "mercury_hip_ensemble_download": (C.c_int, [_dp, _dp]),
+    "mercury_hip_set_graph": (C.c_int, [C.c_int]),
+    "mercury_hip_graph_stats": (C.c_int, [_ip, _ip]),
+    "mercury_hip_autotune": (C.c_int, [C.c_double, C.c_int, _ip, _ip, _dp]),
     "mercury_hip_adjoint_forward": (C.c_int, [C.c_int, C.c_double, C.c_int, _dp, _dp]),
     "mercury_hip_adjoint_backward": (C.c_int, [_dp, _dp, _dp, _dp, _dp]),
     "mercury_hip_tangent": (C.c_int, [_dp, _dp, _dp, _dp, _dp]),
@@ -224,6 +227,20 @@ class MercuryHip:
         self._ok(self.L.mercury_hip_ensemble_download(_d(x), _d(v)), "ensemble_download")
         o = self.ens_offsets
         return [(x[o[k]:o[k + 1]], v[o[k]:o[k + 1]]) for k in range(len(o) - 1)]
+
+    def set_graph(self, enable=True):
+        self._ok(self.L.mercury_hip_set_graph(int(enable)), "set_graph")
+
+    def graph_stats(self):
+        a = C.c_int(0); b = C.c_int(0)
+        self._ok(self.L.mercury_hip_graph_stats(C.byref(a), C.byref(b)), "graph_stats")
+        return {"captures": a.value, "replays": b.value}
+
+    def autotune(self, h=0.0, probe_steps=32):
+        spl = C.c_int(0); blk = C.c_int(0); rate = C.c_double(0.0)
+        self._ok(self.L.mercury_hip_autotune(float(h), int(probe_steps), C.byref(spl), C.byref(blk), C.byref(rate)),
+                 "autotune")
+        return {"steps_per_launch": spl.value, "block": blk.value, "particle_steps_per_s": rate.value}
 
     # ---- differentiation (tape of the last adjoint_forward) ----
     def adjoint_forward(self, nsteps, h, segment=0):
