@@ -110,6 +110,17 @@ class Oracle:
         L.orc_mvs_run.restype = C.c_int
         L.orc_mvs_run.argtypes = [C.c_double, C.c_double, C.c_double, _dp, C.c_int, C.c_int,
                                   _dp, _dp, _dp, _dp, _dp, C.c_int, C.c_int, C.c_int, _ip, _dp]
+        L.orc_hy_run.restype = C.c_int
+        L.orc_hy_run.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, _dp, C.c_int, C.c_int,
+                                 _dp, _dp, _dp, _dp, _dp, _dp, C.c_int, C.c_int, _ip, _dp, C.POINTER(C.c_long)]
+        L.orc_mfo_hy.argtypes = [_dp, C.c_int, C.c_int, _dp, _dp, _dp, _dp]
+        L.orc_mco_dh2h.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp]
+        L.orc_mfo_hkce.argtypes = [C.c_int, _dp, _dp, _dp, _dp, C.c_int, _ip, _ip]
+        L.orc_mdt_bs2.restype = C.c_int
+        L.orc_mdt_bs2.argtypes = [C.c_double, _dp, _dp, C.c_double, C.c_int, _dp, _dp, _dp, _dp, C.c_int, _ip, _ip,
+                                  C.POINTER(C.c_long)]
+        L.orc_mce_snif.restype = C.c_int
+        L.orc_mce_snif.argtypes = [C.c_double, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _ip, C.c_int, _ip, _ip]
 
     def drift_one(self, mu, xv, dt):
         w = np.array(xv, dtype=np.float64).copy()
@@ -196,6 +207,57 @@ class Oracle:
         nf = self.L.orc_ra15_run(time0, sysd["h"] if h0 is None else h0, tol, _d(jc), len(m), sysd["nbig"],
                                  _d(m), _d(x), _d(v), int(pn), nsteps, _d(hdid), _d(hn))
         return x, v, hdid[:nsteps].copy(), float(hn[0]), int(nf)
+
+    def hy_run(self, sysd, nsteps, time0=0.0, tol=1e-12, cap=200000):
+        """nsteps HYBRID steps (mdt_hy + the driver's mce_cent on physical
+        coordinates), collisions not allowed.  Returns x, v (democratic
+        heliocentric), event rows, force-evaluation count."""
+        m = np.ascontiguousarray(sysd["m"], np.float64)
+        x = np.ascontiguousarray(sysd["x"], np.float64).copy()
+        v = np.ascontiguousarray(sysd["v"], np.float64).copy()
+        jc = np.ascontiguousarray(sysd.get("jcen", (0.0, 0.0, 0.0)), np.float64)
+        arrs = [np.ascontiguousarray(sysd[k], np.float64) for k in ("rcrit", "rce", "rphys")]
+        evi = np.zeros((cap, 4), np.int32); evd = np.zeros((cap, 2)); nf = C.c_long(0)
+        k = self.L.orc_hy_run(time0, sysd["h"], tol, sysd["rcen"], _d(jc), len(m), sysd["nbig"], _d(m), _d(x),
+                              _d(v), *[_d(a) for a in arrs], nsteps, cap, _i(evi), _d(evd), C.byref(nf))
+        if k < 0:
+            raise RuntimeError("oracle HYBRID run failed (capacity or BS2 stall)")
+        return x, v, evi[:k].copy(), evd[:k].copy(), nf.value
+
+    def mfo_hkce(self, m, x, rcrit, pairs):
+        m = np.ascontiguousarray(m, np.float64); x = np.ascontiguousarray(x, np.float64)
+        rc = np.ascontiguousarray(rcrit, np.float64)
+        pi = np.ascontiguousarray([p[0] for p in pairs], np.int32); pj = np.ascontiguousarray([p[1] for p in pairs], np.int32)
+        a = np.zeros_like(x)
+        self.L.orc_mfo_hkce(len(m), _d(m), _d(x), _d(rc), _d(a), len(pairs), _i(pi), _i(pj))
+        return a
+
+    def bs2(self, m, x, v, rcrit, pairs, h0, tol=1e-12, time=0.0):
+        """One mdt_bs2 call: (x, v, h done, next h, halvings, force evaluations)."""
+        m = np.ascontiguousarray(m, np.float64)
+        x = np.ascontiguousarray(x, np.float64).copy(); v = np.ascontiguousarray(v, np.float64).copy()
+        rc = np.ascontiguousarray(rcrit, np.float64)
+        pi = np.ascontiguousarray([p[0] for p in pairs] or [0], np.int32)
+        pj = np.ascontiguousarray([p[1] for p in pairs] or [0], np.int32)
+        h = C.c_double(h0); hd = C.c_double(0.0); nf = C.c_long(0)
+        r = self.L.orc_mdt_bs2(time, C.byref(h), C.byref(hd), tol, len(m), _d(m), _d(x), _d(v), _d(rc), len(pairs),
+                               _i(pi), _i(pj), C.byref(nf))
+        return x, v, hd.value, h.value, r, nf.value
+
+    def mfo_hy(self, sysd, x=None):
+        m = np.ascontiguousarray(sysd["m"], np.float64)
+        x = np.ascontiguousarray(sysd["x"] if x is None else x, np.float64)
+        jc = np.ascontiguousarray(sysd.get("jcen", (0.0, 0.0, 0.0)), np.float64)
+        a = np.zeros_like(x)
+        self.L.orc_mfo_hy(_d(jc), len(m), sysd["nbig"], _d(m), _d(x), _d(np.ascontiguousarray(sysd["rcrit"])), _d(a))
+        return a
+
+    def dh2h(self, sysd, x, v):
+        m = np.ascontiguousarray(sysd["m"], np.float64)
+        x = np.ascontiguousarray(x, np.float64); v = np.ascontiguousarray(v, np.float64)
+        xh = np.zeros_like(x); vh = np.zeros_like(v)
+        self.L.orc_mco_dh2h(len(m), _d(m), _d(x), _d(v), _d(xh), _d(vh))
+        return xh, vh
 
     def mvs2h(self, sysd, x, v):
         m = np.ascontiguousarray(sysd["m"], np.float64)
@@ -298,6 +360,62 @@ class Reference:
         self.L.mce_min_(self._r(d0), self._r(d1), self._r(d0t), self._r(d1t), self._r(h),
                         C.byref(a), C.byref(b))
         return a.value, b.value
+
+    def mdt_hy_run(self, sysd, nsteps, time0=0.0, tol=1e-12):
+        """nsteps calls of the reference mdt_hy (collisions not allowed) plus the
+        driver's mce_cent on physical coordinates (mco_dh2h) after each step, as
+        mal_hcon does for algorithm 10.  Rows as Oracle.hy_run."""
+        m = np.ascontiguousarray(sysd["m"], np.float64)
+        n = len(m)
+        nbig = sysd["nbig"]
+        x = np.ascontiguousarray(sysd["x"], np.float64).copy()
+        v = np.ascontiguousarray(sysd["v"], np.float64).copy()
+        rce = np.ascontiguousarray(sysd["rce"], np.float64)
+        rphys = np.ascontiguousarray(sysd["rphys"], np.float64)
+        rcrit = np.ascontiguousarray(sysd["rcrit"], np.float64)
+        jc = np.ascontiguousarray(sysd.get("jcen", (0.0, 0.0, 0.0)), np.float64)
+        h = sysd["h"]; rcen = sysd["rcen"]
+        cm = self.cmax
+        s = np.zeros((n, 3)); stat = np.zeros(n, np.int32)
+        ngf = np.zeros((n, 5)); opt = np.array([0, 0, 0, 2, 0, 1, 0, 0], np.int32)
+        ids = C.create_string_buffer(b" " * (25 * n))
+        outfile = C.create_string_buffer(b"/dev/null".ljust(80) * 3)
+        mem = C.create_string_buffer(b" " * (80 * self.NMESS))
+        lmem = np.ones(self.NMESS, np.int32)
+        iclo = np.zeros(cm, np.int32); jclo = np.zeros(cm, np.int32)
+        dclo = np.zeros(cm); tclo = np.zeros(cm)
+        ixv = np.zeros((cm, 6)); jxv = np.zeros((cm, 6))
+        nclo = C.c_int(0); colflag = C.c_int(0); dtflag = C.c_int(0)
+        en = np.zeros(3); am = np.zeros(3)
+        rows_i = []; rows_d = []
+        time = time0
+        xh0 = np.zeros_like(x); vh0 = np.zeros_like(v); xh = np.zeros_like(x); vh = np.zeros_like(v)
+        zero = C.c_int(0)
+
+        def dh2h(xa, va, xo, vo):
+            self.L.mco_dh2h_(self._r(time), _d(jc), C.byref(C.c_int(n)), C.byref(C.c_int(nbig)), self._r(h), _d(m),
+                             _d(xa), _d(va), _d(xo), _d(vo), _d(ngf), C.byref(zero), _i(opt))
+
+        for step in range(nsteps):
+            dh2h(x, v, xh0, vh0)
+            self.L.mdt_hy_(self._r(time), self._r(time0), self._r(h), self._r(tol),
+                           self._r(100.0), _d(en), _d(am), _d(jc), self._r(rcen),
+                           C.byref(C.c_int(n)), C.byref(C.c_int(nbig)), _d(m), _d(x), _d(v),
+                           _d(s), _d(rphys), _d(rcrit), _d(rce), _i(stat), ids,
+                           _d(ngf), C.byref(C.c_int(10)), _i(opt), C.byref(dtflag),
+                           C.byref(C.c_int(0)), C.byref(C.c_int(-1)), C.byref(colflag),
+                           C.byref(nclo), _i(iclo), _i(jclo), _d(dclo), _d(tclo), _d(ixv), _d(jxv),
+                           outfile, mem, _i(lmem),
+                           C.c_size_t(25), C.c_size_t(80), C.c_size_t(80))
+            time = time + h
+            for k in range(nclo.value):
+                rows_i.append((step, 1, int(iclo[k]), int(jclo[k])))
+                rows_d.append((float(dclo[k]), float(tclo[k])))
+            dh2h(x, v, xh, vh)
+            xh0[0] = vh0[0] = xh[0] = vh[0] = 0.0
+            for (j, t, d) in self.mce_cent(time, h, rcen, m, xh0, vh0, xh, vh):
+                rows_i.append((step, 8, 1, j)); rows_d.append((d, t))
+        return x, v, np.array(rows_i, np.int32).reshape(-1, 4), np.array(rows_d).reshape(-1, 2)
 
     def mvs2h(self, sysd, x, v):
         """The reference's symplectic corrector mco_mvs2h (mercury6_2.for:2107-2219)."""

@@ -1255,3 +1255,452 @@ long orc_ra15_run(double time0, double h0, double tol, const double *jcen, int n
     free(st.e);
     return nf;
 }
+
+/* ------------------------------------------------------------------------ */
+/* HYBRID (algorithm 10) */
+
+/* mfo_drct  mercury6_2.for:4708-4780 (i0 = 2, changeover weighting K) */
+static void mfo_drct(int nbod, int nbig, const double *m, const double *x, const double *rcrit, double *a)
+{
+    for (int i = 1; i < nbig; i++) {
+        for (int j = i + 1; j < nbod; j++) {
+            double dx = x[3 * j] - x[3 * i];
+            double dy = x[3 * j + 1] - x[3 * i + 1];
+            double dz = x[3 * j + 2] - x[3 * i + 2];
+            double s2 = dx * dx + dy * dy + dz * dz;
+            double rc = fmax(rcrit[i], rcrit[j]);
+            double rc2 = rc * rc;
+            double tmp2;
+            if (s2 >= rc2) {
+                double s_1 = 1.0 / sqrt(s2);
+                tmp2 = s_1 * s_1 * s_1;
+            } else if (s2 <= 0.01 * rc2) {
+                tmp2 = 0.0;
+            } else {
+                double s_1 = 1.0 / sqrt(s2);
+                double s = 1.0 / s_1;
+                double s_3 = s_1 * s_1 * s_1;
+                double q = (s - 0.1 * rc) / (0.9 * rc);
+                double q2 = q * q, q3 = q * q2, q4 = q2 * q2, q5 = q2 * q3;
+                tmp2 = (10.0 * q3 - 15.0 * q4 + 6.0 * q5) * s_3;
+            }
+            double faci = tmp2 * m[i];
+            double facj = tmp2 * m[j];
+            a[3 * j] = a[3 * j] - faci * dx;
+            a[3 * j + 1] = a[3 * j + 1] - faci * dy;
+            a[3 * j + 2] = a[3 * j + 2] - faci * dz;
+            a[3 * i] = a[3 * i] + facj * dx;
+            a[3 * i + 1] = a[3 * i + 1] + facj * dy;
+            a[3 * i + 2] = a[3 * i + 2] + facj * dz;
+        }
+    }
+}
+
+/* mfo_hy  mercury6_2.for:4781-4835 */
+void orc_mfo_hy(const double *jcen, int nbod, int nbig, const double *m, const double *x,
+                const double *rcrit, double *a)
+{
+    for (int k = 0; k < 3 * nbod; k++) a[k] = 0.0;
+    mfo_drct(nbod, nbig, m, x, rcrit, a);
+    if (jcen[0] != 0 || jcen[1] != 0 || jcen[2] != 0) {
+        double *aobl = calloc((size_t)3 * nbod, sizeof(double));
+        double acen[3];
+        orc_mfo_obl(jcen, nbod, m, x, aobl, acen);
+        for (int k = 1; k < nbod; k++)
+            for (int c = 0; c < 3; c++) a[3 * k + c] = a[3 * k + c] + aobl[3 * k + c] - acen[c];
+        free(aobl);
+    }
+}
+
+/* mce_snif  mercury6_2.for:1648-1764 (i0 = 2).  ice/jce 1-based, in loop
+ * order; returns nce, or -1 when more than cap pairs would be listed. */
+int orc_mce_snif(double h, int nbod, int nbig, const double *x0, const double *v0, const double *x1,
+                 const double *v1, const double *rcrit, int *ce, int cap, int *ice, int *jce)
+{
+    int nce = 0, over = 0;
+    for (int j = 1; j < nbod; j++) ce[j] = 0;
+    double *xmin = malloc(sizeof(double) * (size_t)nbod), *xmax = malloc(sizeof(double) * (size_t)nbod);
+    double *ymin = malloc(sizeof(double) * (size_t)nbod), *ymax = malloc(sizeof(double) * (size_t)nbod);
+    orc_mce_box(nbod, h, x0, v0, x1, v1, xmin, xmax, ymin, ymax);
+    for (int j = 1; j < nbig; j++) {
+        xmin[j] = xmin[j] - rcrit[j];
+        xmax[j] = xmax[j] + rcrit[j];
+        ymin[j] = ymin[j] - rcrit[j];
+        ymax[j] = ymax[j] + rcrit[j];
+    }
+    for (int i = 1; i < nbig; i++) {
+        for (int j = i + 1; j < nbod; j++) {
+            if (!(xmax[i] >= xmin[j] && xmax[j] >= xmin[i] && ymax[i] >= ymin[j] && ymax[j] >= ymin[i]))
+                continue;
+            double rc = fmax(rcrit[i], rcrit[j]);
+            double rc2 = rc * rc;
+            double dx0 = x0[3 * i] - x0[3 * j], dy0 = x0[3 * i + 1] - x0[3 * j + 1], dz0 = x0[3 * i + 2] - x0[3 * j + 2];
+            double dx1 = x1[3 * i] - x1[3 * j], dy1 = x1[3 * i + 1] - x1[3 * j + 1], dz1 = x1[3 * i + 2] - x1[3 * j + 2];
+            double d0 = dx0 * dx0 + dy0 * dy0 + dz0 * dz0;
+            double d1 = dx1 * dx1 + dy1 * dy1 + dz1 * dz1;
+            double du0 = v0[3 * i] - v0[3 * j], dv0 = v0[3 * i + 1] - v0[3 * j + 1], dw0 = v0[3 * i + 2] - v0[3 * j + 2];
+            double du1 = v1[3 * i] - v1[3 * j], dv1 = v1[3 * i + 1] - v1[3 * j + 1], dw1 = v1[3 * i + 2] - v1[3 * j + 2];
+            double d0t = (dx0 * du0 + dy0 * dv0 + dz0 * dw0) * 2.0;
+            double d1t = (dx1 * du1 + dy1 * dv1 + dz1 * dw1) * 2.0;
+            double d2min = HUGEV, tmin;
+            if (d0t * h <= 0 && d1t * h >= 0) orc_mce_min(d0, d1, d0t, d1t, h, &d2min, &tmin);
+            double temp = fmin(fmin(d0, d1), d2min);
+            if (temp <= rc2) {
+                ce[i] = 2;
+                ce[j] = 2;
+                if (nce >= cap) { over = 1; continue; }
+                ice[nce] = i + 1;
+                jce[nce] = j + 1;
+                nce++;
+            }
+        }
+    }
+    free(xmin); free(xmax); free(ymin); free(ymax);
+    return over ? -1 : nce;
+}
+
+/* mfo_hkce  mercury6_2.for (Keplerian part with close-encounter terms):
+ * pair terms weighted 1 - K over the listed pairs, then the solar terms. */
+static void mfo_hkce(int nbod, const double *m, const double *x, const double *rcrit, double *a,
+                     int nce, const int *ice, const int *jce)
+{
+    for (int k = 0; k < 3 * nbod; k++) a[k] = 0.0;
+    for (int k = 0; k < nce; k++) {
+        int i = ice[k] - 1, j = jce[k] - 1;
+        double dx = x[3 * j] - x[3 * i];
+        double dy = x[3 * j + 1] - x[3 * i + 1];
+        double dz = x[3 * j + 2] - x[3 * i + 2];
+        double s2 = dx * dx + dy * dy + dz * dz;
+        double rc = fmax(rcrit[i], rcrit[j]);
+        double rc2 = rc * rc;
+        if (s2 < rc2) {
+            double s_1 = 1.0 / sqrt(s2);
+            double s_3 = s_1 * s_1 * s_1;
+            double tmp2;
+            if (s2 <= 0.01 * rc2) {
+                tmp2 = s_3;
+            } else {
+                double s = 1.0 / s_1;
+                double q = (s - 0.1 * rc) / (0.9 * rc);
+                double q2 = q * q, q3 = q * q2, q4 = q2 * q2, q5 = q2 * q3;
+                tmp2 = (1.0 - 10.0 * q3 + 15.0 * q4 - 6.0 * q5) * s_3;
+            }
+            double faci = tmp2 * m[i];
+            double facj = tmp2 * m[j];
+            a[3 * j] = a[3 * j] - faci * dx;
+            a[3 * j + 1] = a[3 * j + 1] - faci * dy;
+            a[3 * j + 2] = a[3 * j + 2] - faci * dz;
+            a[3 * i] = a[3 * i] + facj * dx;
+            a[3 * i + 1] = a[3 * i + 1] + facj * dy;
+            a[3 * i + 2] = a[3 * i + 2] + facj * dz;
+        }
+    }
+    for (int i = 1; i < nbod; i++) {
+        double s2 = x[3 * i] * x[3 * i] + x[3 * i + 1] * x[3 * i + 1] + x[3 * i + 2] * x[3 * i + 2];
+        double s_1 = 1.0 / sqrt(s2);
+        double tmp2 = m[0] * s_1 * s_1 * s_1;
+        a[3 * i] = a[3 * i] - tmp2 * x[3 * i];
+        a[3 * i + 1] = a[3 * i + 1] - tmp2 * x[3 * i + 1];
+        a[3 * i + 2] = a[3 * i + 2] - tmp2 * x[3 * i + 2];
+    }
+}
+
+void orc_mfo_hkce(int nbod, const double *m, const double *x, const double *rcrit, double *a, int nce,
+                  const int *ice, const int *jce)
+{
+    mfo_hkce(nbod, m, x, rcrit, a, nce, ice, jce);
+}
+
+/* mdt_bs2  mercury6_2.for:3428-3617 with mfo_hkce forces.  Returns the
+ * number of halvings (rejected attempts); -1 if time cannot advance. */
+int orc_mdt_bs2(double time, double *h0, double *hdid, double tol, int nbod, const double *m, double *x0,
+                double *v0, const double *rcrit, int nce, const int *ice, const int *jce, long *nforce)
+{
+    const double SHRINK = .55, GROW = 1.3;
+    const double tol2 = tol * tol;
+    const size_t nv = (size_t)3 * nbod;
+    double *xend = calloc(nv, sizeof(double)), *b = calloc(nv, sizeof(double)), *c = calloc(nv, sizeof(double));
+    double *a = calloc(nv, sizeof(double)), *a0 = calloc(nv, sizeof(double));
+    double *d = calloc((size_t)6 * nbod * 12, sizeof(double));
+    double *xscal = calloc((size_t)nbod, sizeof(double)), *vscal = calloc((size_t)nbod, sizeof(double));
+    double h2[12];
+    int rejected = 0;
+#define D(q, k, n) d[((size_t)(n) * nbod + (k)) * 6 + (q)]
+    for (int k = 1; k < nbod; k++) {
+        double tmp1 = x0[3 * k] * x0[3 * k] + x0[3 * k + 1] * x0[3 * k + 1] + x0[3 * k + 2] * x0[3 * k + 2];
+        double tmp2 = v0[3 * k] * v0[3 * k] + v0[3 * k + 1] * v0[3 * k + 1] + v0[3 * k + 2] * v0[3 * k + 2];
+        xscal[k] = 1.0 / fmax(tmp1, 1e-300);
+        vscal[k] = 1.0 / fmax(tmp2, 1e-300);
+    }
+    mfo_hkce(nbod, m, x0, rcrit, a0, nce, ice, jce);
+    (*nforce)++;
+    for (;;) {
+        if (time + *h0 == time) { rejected = -1; break; }
+        int done = 0;
+        for (int n = 1; n <= 12 && !done; n++) {
+            double h = *h0 / (double)n;
+            double hby2 = .5 * h;
+            h2[n - 1] = h * h;
+            double h2by2 = .5 * h2[n - 1];
+            for (int k = 1; k < nbod; k++)
+                for (int q = 0; q < 3; q++) {
+                    b[3 * k + q] = .5 * a0[3 * k + q];
+                    c[3 * k + q] = 0.0;
+                    xend[3 * k + q] = h2by2 * a0[3 * k + q] + h * v0[3 * k + q] + x0[3 * k + q];
+                }
+            for (int j = 2; j <= n; j++) {
+                mfo_hkce(nbod, m, xend, rcrit, a, nce, ice, jce);
+                (*nforce)++;
+                double tmp0 = h * (double)j;
+                for (int k = 1; k < nbod; k++)
+                    for (int q = 0; q < 3; q++) {
+                        b[3 * k + q] = b[3 * k + q] + a[3 * k + q];
+                        c[3 * k + q] = c[3 * k + q] + b[3 * k + q];
+                        xend[3 * k + q] = h2[n - 1] * c[3 * k + q] + h2by2 * a0[3 * k + q] + tmp0 * v0[3 * k + q]
+                                        + x0[3 * k + q];
+                    }
+            }
+            mfo_hkce(nbod, m, xend, rcrit, a, nce, ice, jce);
+            (*nforce)++;
+            for (int k = 1; k < nbod; k++)
+                for (int q = 0; q < 3; q++) {
+                    D(q, k, n - 1) = xend[3 * k + q];
+                    D(3 + q, k, n - 1) = h * b[3 * k + q] + hby2 * a[3 * k + q] + v0[3 * k + q];
+                }
+            for (int j = n - 1; j >= 1; j--) {
+                int j1 = j + 1;
+                double tmp0 = 1.0 / (h2[j - 1] - h2[n - 1]);
+                double tmp1 = tmp0 * h2[j1 - 1];
+                double tmp2 = tmp0 * h2[n - 1];
+                for (int k = 1; k < nbod; k++)
+                    for (int q = 0; q < 6; q++) D(q, k, j - 1) = tmp1 * D(q, k, j1 - 1) - tmp2 * D(q, k, j - 1);
+            }
+            if (n > 3) {
+                double errmax = 0.0;
+                for (int k = 1; k < nbod; k++) {
+                    double tmp1 = fmax(fmax(D(0, k, 0) * D(0, k, 0), D(1, k, 0) * D(1, k, 0)), D(2, k, 0) * D(2, k, 0));
+                    double tmp2 = fmax(fmax(D(3, k, 0) * D(3, k, 0), D(4, k, 0) * D(4, k, 0)), D(5, k, 0) * D(5, k, 0));
+                    errmax = fmax(errmax, fmax(tmp1 * xscal[k], tmp2 * vscal[k]));
+                }
+                if (errmax <= tol2) {
+                    for (int k = 1; k < nbod; k++)
+                        for (int q = 0; q < 3; q++) {
+                            x0[3 * k + q] = D(q, k, 0);
+                            v0[3 * k + q] = D(3 + q, k, 0);
+                        }
+                    for (int j = 2; j <= n; j++)
+                        for (int k = 1; k < nbod; k++)
+                            for (int q = 0; q < 3; q++) {
+                                x0[3 * k + q] = x0[3 * k + q] + D(q, k, j - 1);
+                                v0[3 * k + q] = v0[3 * k + q] + D(3 + q, k, j - 1);
+                            }
+                    *hdid = *h0;
+                    if (n >= 8) *h0 = *h0 * SHRINK;
+                    if (n < 7) *h0 = *h0 * GROW;
+                    done = 1;
+                }
+            }
+        }
+        if (done) break;
+        rejected++;
+        *h0 = *h0 * .5;
+    }
+#undef D
+    free(xend); free(b); free(c); free(a); free(a0); free(d); free(xscal); free(vscal);
+    return rejected;
+}
+
+/* mdt_hkce  mercury6_2.for:3798-3960, collisions not allowed: the encounter
+ * members are integrated by BS2 substeps over h0, mce_stat after each. */
+static int hkce(double time, double h0, double *hrec, double tol, double rcen, int nbod, int nbig,
+                const double *m, double *x, double *v, const double *rcrit, const double *rce,
+                const double *rphys, const int *ce, int nce, const int *ice, const int *jce,
+                orc_events *ev, long *nforce)
+{
+    int nbs = 1, nbsbig = 0;
+    int *index = calloc((size_t)nbod + 1, sizeof(int)), *iback = calloc((size_t)nbod + 1, sizeof(int));
+    double *mbs = calloc((size_t)nbod + 1, sizeof(double)), *xbs = calloc((size_t)3 * (nbod + 1), sizeof(double));
+    double *vbs = calloc((size_t)3 * (nbod + 1), sizeof(double)), *rcb = calloc((size_t)nbod + 1, sizeof(double));
+    double *rceb = calloc((size_t)nbod + 1, sizeof(double)), *rphb = calloc((size_t)nbod + 1, sizeof(double));
+    double *x0 = calloc((size_t)3 * (nbod + 1), sizeof(double)), *v0 = calloc((size_t)3 * (nbod + 1), sizeof(double));
+    int *ibs = calloc((size_t)(nce > 0 ? nce : 1), sizeof(int)), *jbs = calloc((size_t)(nce > 0 ? nce : 1), sizeof(int));
+    int rej = 0;
+    mbs[0] = m[0];
+    for (int j = 1; j < nbod; j++) {
+        if (ce[j] == 0) continue;
+        int k = nbs++;
+        if (j < nbig) nbsbig = nbs;
+        mbs[k] = m[j];
+        for (int q = 0; q < 3; q++) { xbs[3 * k + q] = x[3 * j + q]; vbs[3 * k + q] = v[3 * j + q]; }
+        rcb[k] = rcrit[j]; rceb[k] = rce[j]; rphb[k] = rphys[j];
+        index[k] = j;
+        iback[j] = k;
+    }
+    for (int k = 0; k < nce; k++) { ibs[k] = iback[ice[k] - 1] + 1; jbs[k] = iback[jce[k] - 1] + 1; }
+    int nclo0 = ev->nclo;
+    double tlocal = 0.0;
+    double hlocal = fsign(*hrec, h0);
+    do {
+        double tmp0 = fabs(h0) - fabs(tlocal);
+        *hrec = hlocal;
+        if (fabs(hlocal) > tmp0) hlocal = fsign(tmp0, h0);
+        memcpy(x0, xbs, sizeof(double) * 3 * nbs);
+        memcpy(v0, vbs, sizeof(double) * 3 * nbs);
+        double hdid;
+        int r = orc_mdt_bs2(tlocal, &hlocal, &hdid, tol, nbs, mbs, xbs, vbs, rcb, nce, ibs, jbs, nforce);
+        if (r < 0) { rej = -1; break; }
+        rej += r;
+        tlocal = tlocal + hdid;
+        double temp = time + tlocal;
+        orc_mce_stat(temp, hdid, rcen, nbs, nbsbig, mbs, x0, v0, xbs, vbs, rceb, rphb, ev);
+    } while ((tlocal - h0) * h0 < 0.0);
+    for (int k = 1; k < nbs; k++) {
+        int j = index[k];
+        for (int q = 0; q < 3; q++) { x[3 * j + q] = xbs[3 * k + q]; v[3 * j + q] = vbs[3 * k + q]; }
+    }
+    for (int k = nclo0; k < ev->nclo; k++) {
+        ev->iclo[k] = index[ev->iclo[k] - 1] + 1;
+        ev->jclo[k] = index[ev->jclo[k] - 1] + 1;
+    }
+    free(index); free(iback); free(mbs); free(xbs); free(vbs); free(rcb); free(rceb); free(rphb);
+    free(x0); free(v0); free(ibs); free(jbs);
+    return rej;
+}
+
+/* mdt_hy  mercury6_2.for:3618-3797 (no user or non-gravitational forces).
+ * st->a carries the interaction accelerations; st->dtflag, st->hrec as the
+ * routine's saved state.  Returns the encounter pairs found (nce). */
+int orc_mdt_hy(double time, double h0, double tol, const double *jcen, double rcen, int nbod, int nbig,
+               const double *m, double *x, double *v, const double *rcrit, const double *rce,
+               const double *rphys, orc_hybrid *st, orc_events *ev)
+{
+    const double hby2 = h0 * .5;
+    ev->nclo = 0;
+    if (st->dtflag != 2) {
+        if (st->dtflag == 0) st->hrec = h0;
+        orc_mfo_hy(jcen, nbod, nbig, m, x, rcrit, st->a);
+        st->nforce++;
+        st->dtflag = 2;
+    }
+    for (int j = 1; j < nbod; j++)
+        for (int q = 0; q < 3; q++) v[3 * j + q] = v[3 * j + q] + hby2 * (0.0 + 0.0 + st->a[3 * j + q]);
+    double mv[3] = {0, 0, 0};
+    for (int j = 1; j < nbod; j++)
+        for (int q = 0; q < 3; q++) mv[q] = mv[q] + m[j] * v[3 * j + q];
+    double temp = hby2 / m[0];
+    for (int q = 0; q < 3; q++) mv[q] = temp * mv[q];
+    for (int j = 1; j < nbod; j++)
+        for (int q = 0; q < 3; q++) x[3 * j + q] = x[3 * j + q] + mv[q];
+    double *x0 = malloc(sizeof(double) * 3 * (size_t)nbod), *v0 = malloc(sizeof(double) * 3 * (size_t)nbod);
+    memcpy(x0, x, sizeof(double) * 3 * (size_t)nbod);
+    memcpy(v0, v, sizeof(double) * 3 * (size_t)nbod);
+    for (int j = 1; j < nbod; j++) {
+        double xv[6] = {x[3 * j], x[3 * j + 1], x[3 * j + 2], v[3 * j], v[3 * j + 1], v[3 * j + 2]};
+        orc_drift_one(m[0], xv, h0);
+        for (int q = 0; q < 3; q++) { x[3 * j + q] = xv[q]; v[3 * j + q] = xv[3 + q]; }
+    }
+    int *ce = calloc((size_t)nbod, sizeof(int));
+    int cap = st->cap;
+    int nce = orc_mce_snif(h0, nbod, nbig, x0, v0, x, v, rcrit, ce, cap, st->ice, st->jce);
+    if (nce < 0) { free(x0); free(v0); free(ce); return -1; }
+    if (nce > 0) {
+        for (int j = 1; j < nbod; j++)
+            if (ce[j] != 0)
+                for (int q = 0; q < 3; q++) { x[3 * j + q] = x0[3 * j + q]; v[3 * j + q] = v0[3 * j + q]; }
+        if (hkce(time, h0, &st->hrec, tol, rcen, nbod, nbig, m, x, v, rcrit, rce, rphys, ce, nce, st->ice,
+                 st->jce, ev, &st->nforce) < 0) {
+            free(x0); free(v0); free(ce);
+            return -1;
+        }
+    }
+    mv[0] = mv[1] = mv[2] = 0.0;
+    for (int j = 1; j < nbod; j++)
+        for (int q = 0; q < 3; q++) mv[q] = mv[q] + m[j] * v[3 * j + q];
+    for (int q = 0; q < 3; q++) mv[q] = temp * mv[q];
+    for (int j = 1; j < nbod; j++)
+        for (int q = 0; q < 3; q++) x[3 * j + q] = x[3 * j + q] + mv[q];
+    orc_mfo_hy(jcen, nbod, nbig, m, x, rcrit, st->a);
+    st->nforce++;
+    for (int j = 1; j < nbod; j++)
+        for (int q = 0; q < 3; q++) v[3 * j + q] = v[3 * j + q] + hby2 * (0.0 + 0.0 + st->a[3 * j + q]);
+    free(x0); free(v0); free(ce);
+    return nce;
+}
+
+/* mco_dh2h  mercury6_2.for:2007-2063: democratic heliocentric -> heliocentric */
+void orc_mco_dh2h(int nbod, const double *m, const double *x, const double *v, double *xh, double *vh)
+{
+    double mv[3] = {0, 0, 0};
+    for (int j = 1; j < nbod; j++)
+        for (int q = 0; q < 3; q++) {
+            xh[3 * j + q] = x[3 * j + q];
+            mv[q] = mv[q] + m[j] * v[3 * j + q];
+        }
+    double temp = 1.0 / m[0];
+    for (int q = 0; q < 3; q++) mv[q] = temp * mv[q];
+    for (int j = 1; j < nbod; j++)
+        for (int q = 0; q < 3; q++) vh[3 * j + q] = v[3 * j + q] + mv[q];
+    for (int q = 0; q < 3; q++) { xh[q] = 0.0; vh[q] = 0.0; }
+}
+
+/* nsteps HYBRID steps as the driver takes them (mal_hcon with algorithm 10,
+ * collisions not allowed): mdt_hy, then mce_cent on the physical states.
+ * Event rows as orc_mvs_run (kind 1 from the encounter substeps, 8 central). */
+int orc_hy_run(double time0, double h0, double tol, double rcen, const double *jcen, int nbod, int nbig,
+               const double *m, double *x, double *v, const double *rcrit, const double *rce,
+               const double *rphys, int nsteps, int cap, int *evi, double *evd, long *nforce)
+{
+    orc_hybrid st;
+    st.a = calloc((size_t)3 * nbod, sizeof(double));
+    st.dtflag = 0;
+    st.hrec = h0;
+    st.nforce = 0;
+    st.cap = 20000;
+    st.ice = calloc((size_t)st.cap, sizeof(int));
+    st.jce = calloc((size_t)st.cap, sizeof(int));
+    orc_events ev;
+    memset(&ev, 0, sizeof(ev));
+    ev.cap = 20000;
+    ev.iclo = calloc((size_t)ev.cap, sizeof(int)); ev.jclo = calloc((size_t)ev.cap, sizeof(int));
+    ev.dclo = calloc((size_t)ev.cap, sizeof(double)); ev.tclo = calloc((size_t)ev.cap, sizeof(double));
+    ev.ixvclo = calloc((size_t)6 * ev.cap, sizeof(double)); ev.jxvclo = calloc((size_t)6 * ev.cap, sizeof(double));
+    ev.ihit = calloc((size_t)ev.cap, sizeof(int)); ev.jhit = calloc((size_t)ev.cap, sizeof(int));
+    ev.chit = calloc((size_t)ev.cap, sizeof(int)); ev.dhit = calloc((size_t)ev.cap, sizeof(double));
+    ev.thit = calloc((size_t)ev.cap, sizeof(double));
+    double *xh0 = calloc((size_t)3 * nbod, sizeof(double)), *vh0 = calloc((size_t)3 * nbod, sizeof(double));
+    double *xh = calloc((size_t)3 * nbod, sizeof(double)), *vh = calloc((size_t)3 * nbod, sizeof(double));
+    int *cj = calloc((size_t)nbod, sizeof(int));
+    double *ct = calloc((size_t)nbod, sizeof(double)), *cd = calloc((size_t)nbod, sizeof(double));
+    double time = time0;
+    int nev = 0, bad = 0;
+    for (int s = 0; s < nsteps && !bad; s++) {
+        orc_mco_dh2h(nbod, m, x, v, xh0, vh0);
+        ev.nhit = 0;
+        if (orc_mdt_hy(time, h0, tol, jcen, rcen, nbod, nbig, m, x, v, rcrit, rce, rphys, &st, &ev) < 0) {
+            bad = 1;
+            break;
+        }
+        time = time + h0;
+        if (ev.overflow) bad = 1;
+        for (int k = 0; k < ev.nclo; k++) {
+            if (nev >= cap) { bad = 1; break; }
+            evi[4 * nev] = s; evi[4 * nev + 1] = 1; evi[4 * nev + 2] = ev.iclo[k]; evi[4 * nev + 3] = ev.jclo[k];
+            evd[2 * nev] = ev.dclo[k]; evd[2 * nev + 1] = ev.tclo[k];
+            nev++;
+        }
+        orc_mco_dh2h(nbod, m, x, v, xh, vh);
+        int nc = orc_mce_cent(time, h0, rcen, 2, nbod, m, xh0, vh0, xh, vh, nbod, cj, ct, cd);
+        for (int k = 0; k < nc; k++) {
+            if (nev >= cap) { bad = 1; break; }
+            evi[4 * nev] = s; evi[4 * nev + 1] = 8; evi[4 * nev + 2] = 1; evi[4 * nev + 3] = cj[k];
+            evd[2 * nev] = cd[k]; evd[2 * nev + 1] = ct[k];
+            nev++;
+        }
+    }
+    *nforce = st.nforce;
+    free(st.a); free(st.ice); free(st.jce);
+    free(ev.iclo); free(ev.jclo); free(ev.dclo); free(ev.tclo); free(ev.ixvclo); free(ev.jxvclo);
+    free(ev.ihit); free(ev.jhit); free(ev.chit); free(ev.dhit); free(ev.thit);
+    free(xh0); free(vh0); free(xh); free(vh); free(cj); free(ct); free(cd);
+    return bad ? -1 : nev;
+}

@@ -112,6 +112,41 @@ long orc_ra15_run(double time0, double h0, double tol, const double *jcen, int n
                   int nbig, const double *m, double *x, double *v, int pn, int nsteps,
                   double *hdid, double *hnext);
 
+/* ---- HYBRID (algorithm 10) ---- */
+/* Interaction accelerations mfo_hy: changeover-weighted direct terms + oblateness. */
+void orc_mfo_hy(const double *jcen, int nbod, int nbig, const double *m, const double *x,
+                const double *rcrit, double *a);
+/* mce_snif: encounter pairs (1-based, loop order) of one step; -1 on overflow. */
+int orc_mce_snif(double h, int nbod, int nbig, const double *x0, const double *v0, const double *x1,
+                 const double *v1, const double *rcrit, int *ce, int cap, int *ice, int *jce);
+/* mfo_hkce: the encounter set's Keplerian-part accelerations (listed pair
+ * terms weighted 1 - K, solar terms); pairs 1-based. */
+void orc_mfo_hkce(int nbod, const double *m, const double *x, const double *rcrit, double *a, int nce,
+                  const int *ice, const int *jce);
+/* mdt_bs2 with mfo_hkce forces on a (compact) set; pairs 1-based. */
+int orc_mdt_bs2(double time, double *h0, double *hdid, double tol, int nbod, const double *m, double *x0,
+                double *v0, const double *rcrit, int nce, const int *ice, const int *jce, long *nforce);
+typedef struct {
+    double *a;            /* interaction accelerations carried between calls */
+    int dtflag;           /* 0 first call, 1 reset, 2 normal */
+    double hrec;          /* BS2 step recommendation kept between encounters */
+    long nforce;
+    int cap;              /* capacity of ice/jce */
+    int *ice, *jce;
+} orc_hybrid;
+/* One HYBRID step (mdt_hy, collisions not allowed); events of the encounter
+ * substeps in ev (close-encounter list, global 1-based indices).  Returns
+ * the number of encounter pairs, -1 on failure. */
+int orc_mdt_hy(double time, double h0, double tol, const double *jcen, double rcen, int nbod, int nbig,
+               const double *m, double *x, double *v, const double *rcrit, const double *rce,
+               const double *rphys, orc_hybrid *st, orc_events *ev);
+void orc_mco_dh2h(int nbod, const double *m, const double *x, const double *v, double *xh, double *vh);
+/* nsteps HYBRID steps as the driver takes them, with mce_cent on the
+ * physical states after each (rows as orc_mvs_run). */
+int orc_hy_run(double time0, double h0, double tol, double rcen, const double *jcen, int nbod, int nbig,
+               const double *m, double *x, double *v, const double *rcrit, const double *rce,
+               const double *rphys, int nsteps, int cap, int *evi, double *evd, long *nforce);
+
 /* Convenience for tests and the CPU baseline: nsteps MVS steps of a whole
  * system with the driver's central-impact check after each, event lists
  * flattened into caller arrays.  Returns the number of events written

@@ -2,9 +2,10 @@
 routines (oracle/_ref/libmercury_ref.so, compiled from mercury6_2.for by
 oracle/Makefile).  Run here, where /root/reference exists:
 
-    make -C oracle ref && python tests/golden/make_golden.py [radau]
+    make -C oracle ref && python tests/golden/make_golden.py [radau|hybrid]
 
-(`radau` rewrites only the mfo_all / mdt_ra15 / mco_mvs2h fixtures.)
+(`radau` rewrites only the mfo_all / mdt_ra15 / mco_mvs2h fixtures, `hybrid`
+only the mdt_hy runs.)
 
 Each fixture holds inputs and the reference's outputs only (data, no code).
 """
@@ -115,6 +116,7 @@ def main():
     np.savez_compressed(HERE / "mce_cent.npz", m=m, x0=X0, v0=V0, x1=X1, v1=V1,
                         hits=np.array(hits, float).reshape(-1, 3))
     radau(R)
+    hybrid(R)
     print("wrote", sorted(p.name for p in HERE.glob("*.npz")))
 
 
@@ -161,9 +163,25 @@ def radau(R=None):
     np.savez_compressed(HERE / "mvs2h.npz", **cor)
 
 
+def hybrid(R=None):
+    """Whole HYBRID runs (mdt_hy with BS2 encounter substeps, the driver's
+    mce_cent on physical coordinates), collisions not allowed."""
+    R = R or Reference()
+    runs = {}
+    for c, (seed, npart, nsteps, near) in enumerate([(3, 150, 120, 0.5), (6, 80, 80, 0.0), (7, 200, 100, 0.5)]):
+        s = systems.make_system(seed, npart, a_range=(0.5, 40.0), emax=0.3, imax_deg=20.0, h=2.0,
+                                near_fraction=near)
+        x, v, evi, evd = R.mdt_hy_run(s, nsteps)
+        for key in ("m", "x", "v", "rce", "rphys", "rcrit"):
+            runs[f"{key}{c}"] = s[key]
+        runs[f"meta{c}"] = np.array([s["nbig"], nsteps, s["h"], s["rcen"]], float)
+        runs[f"xout{c}"] = x; runs[f"vout{c}"] = v; runs[f"evi{c}"] = evi; runs[f"evd{c}"] = evd
+    np.savez_compressed(HERE / "hy_runs.npz", **runs)
+
+
 if __name__ == "__main__":
     threading.stack_size(512 << 20)
     only = sys.argv[1] if len(sys.argv) > 1 else None
-    t = threading.Thread(target=radau if only == "radau" else main)
+    t = threading.Thread(target={"radau": radau, "hybrid": hybrid}.get(only, main))
     t.start()
     t.join()

@@ -450,3 +450,90 @@ def test_reference_mvs2h(oracle, reference):
     a = oracle.mvs2h(s, x, v)
     b = run_big_stack(reference.mvs2h, s, x, v)
     assert np.array_equal(a[0][1:], b[0][1:]) and np.array_equal(a[1][1:], b[1][1:])
+
+
+# ---------------------------------------------------------------- HYBRID
+def test_golden_hy_runs(oracle):
+    """Whole HYBRID runs (mdt_hy, BS2 encounter substeps, mce_stat and the
+    driver's mce_cent) against the reference's, bit for bit."""
+    g = np.load(GOLD / "hy_runs.npz")
+    for c in range(3):
+        nbig, nsteps, h, rcen = g[f"meta{c}"]
+        s = dict(m=g[f"m{c}"], x=g[f"x{c}"], v=g[f"v{c}"], rce=g[f"rce{c}"], rphys=g[f"rphys{c}"],
+                 rcrit=g[f"rcrit{c}"], nbig=int(nbig), h=h, rcen=rcen, jcen=np.zeros(3))
+        x, v, evi, evd, nf = oracle.hy_run(s, int(nsteps))
+        assert np.array_equal(x, g[f"xout{c}"]) and np.array_equal(v, g[f"vout{c}"]), f"run {c}"
+        assert np.array_equal(evi, g[f"evi{c}"]) and np.array_equal(evd, g[f"evd{c}"]), f"run {c}"
+
+
+def test_reference_hy_run(oracle, reference):
+    s = systems.make_system(8, 250, a_range=(0.5, 40.0), emax=0.3, imax_deg=20.0, h=2.0, near_fraction=0.5)
+    x, v, evi, evd, nf = oracle.hy_run(s, 150)
+    xr, vr, ri, rd = run_big_stack(reference.mdt_hy_run, s, 150)
+    assert np.array_equal(x, xr) and np.array_equal(v, vr)
+    assert np.array_equal(evi, ri) and np.array_equal(evd, rd)
+    assert len(evi) > 0
+
+
+def test_changeover_partition_of_unity(oracle):
+    """The regular part (mfo_hy, weight K) and the encounter part (mfo_hkce,
+    weight 1 - K) of a pair force add to the full Newtonian pull at every
+    separation; K rises from 0 at 0.1 rc to 1 at rc, smoothly (the quintic's
+    value, slope and curvature at both ends)."""
+    mu = MU
+    mp = 1e-3 * MU
+    rc = 0.05
+    seps = np.concatenate([np.linspace(0.02 * rc, 1.5 * rc, 400), [0.1 * rc, rc]])
+    k_vals = []
+    for sep in seps:
+        m = np.array([mu, mp, 0.0])
+        x = np.array([[0, 0, 0], [5.0, 0.0, 0.0], [5.0 + sep * 0.6, sep * 0.8, 0.0]])
+        rcrit = np.array([0.0, rc, 0.0])
+        s = {"m": m, "x": x, "nbig": 2, "rcrit": rcrit, "jcen": np.zeros(3)}
+        a_reg = oracle.mfo_hy(s)[2]
+        a_enc = oracle.mfo_hkce(m, x, rcrit, [(2, 3)])[2]
+        r3 = np.linalg.norm(x[2]) ** 3
+        a_sun = -mu * x[2] / r3
+        d = x[2] - x[1]
+        full = -mp * d / np.linalg.norm(d) ** 3
+        # mfo_hkce also holds the solar term; mfo_hy only pair terms
+        assert np.allclose(a_reg + (a_enc - a_sun), full, rtol=1e-13, atol=1e-13 * np.linalg.norm(full))
+        k_vals.append(np.dot(a_reg, full) / np.dot(full, full))
+    k = np.array(k_vals)
+    assert abs(k[-2]) < 1e-15 and abs(k[-1] - 1.0) < 1e-15
+    inside = (seps[:-2] > 0.1 * rc) & (seps[:-2] < rc)
+    assert np.all(np.diff(k[:-2][inside]) > 0)
+    assert np.all(k[:-2][seps[:-2] <= 0.1 * rc] == 0.0)
+    assert np.all(np.abs(k[:-2][seps[:-2] >= rc] - 1.0) < 1e-14)
+    # smoothness: one-sided differences vanish at both ends to O(eps^2)
+    q = np.array([1e-4, 2e-4])
+    for end, sgn in ((0.1 * rc, 1.0), (rc, -1.0)):
+        vals = []
+        for dq in q:
+            sep = end + sgn * dq * 0.9 * rc
+            m = np.array([mu, mp, 0.0])
+            x = np.array([[0, 0, 0], [5.0, 0.0, 0.0], [5.0 + sep, 0.0, 0.0]])
+            s = {"m": m, "x": x, "nbig": 2, "rcrit": np.array([0.0, rc, 0.0]), "jcen": np.zeros(3)}
+            a = oracle.mfo_hy(s)[2][0]
+            vals.append(abs(a / (-mp / sep ** 2) - (0.0 if sgn > 0 else 1.0)))
+        # K - K(end) ~ 10 dq^3: doubling dq multiplies the gap by ~8
+        assert 6.0 < vals[1] / vals[0] < 10.0
+
+
+def test_bs2_kepler_known_answer(oracle):
+    """mdt_bs2 on pure two-body motion (no encounter pairs: only the solar
+    term) reproduces the exact Kepler drift to its tolerance."""
+    rng = np.random.default_rng(17)
+    for _ in range(20):
+        r = 10 ** rng.uniform(-0.5, 1.0)
+        vel = np.sqrt(MU / r) * rng.uniform(0.6, 1.3)
+        xd = rng.normal(size=3); xd /= np.linalg.norm(xd)
+        vd = np.cross(xd, rng.normal(size=3)); vd /= np.linalg.norm(vd)
+        m = np.array([MU, 0.0])
+        x = np.array([[0, 0, 0], xd * r]); v = np.array([[0, 0, 0], vd * vel])
+        h = rng.uniform(0.5, 20.0)
+        x1, v1, hd, hn, rej, nf = oracle.bs2(m, x, v, np.zeros(2), [], h, tol=1e-12)
+        want = _exact_two_body(MU, np.concatenate([x[1], v[1]]), hd)
+        assert hd != 0.0 and nf > 0
+        assert np.linalg.norm(x1[1] - want[:3]) < 1e-10 * r
+        assert np.linalg.norm(v1[1] - want[3:]) < 1e-10 * vel
