@@ -18,6 +18,7 @@
 #include "mvs_device.hpp"
 #include "radau_device.hpp"
 #include "tangent.hpp"
+#include "hybrid_device.hpp"
 
 using namespace mhip;
 
@@ -116,6 +117,23 @@ struct Context {
     double *rb = nullptr, *re = nullptr, *ra1 = nullptr;
     unsigned long long* bmax = nullptr;
     long long ra_cap = 0;
+    // HYBRID (algorithm 10): changeover radii, pre-drift states, tables, pairs
+    double *rcrit = nullptr, *hx0 = nullptr, *hv0 = nullptr, *hytab = nullptr, *hpack = nullptr;
+    int *hpairs = nullptr, *hnpairs = nullptr, *hids = nullptr;
+    int hpair_cap = 0, hpack_cap = 0;
+    long long hy_cap = 0;
+    std::vector<int> members;          // encounter members of the current step (body indices)
+    std::vector<double> packed;
+    // compact encounter context (mdt_hkce's BS2 set)
+    struct Enc {
+        bool active = false;
+        int n = 0, nbig = 0, ncap = 0;
+        double *m = nullptr, *rcrit = nullptr, *x = nullptr, *v = nullptr, *dbuf = nullptr, *cand = nullptr;
+        int *bbp = nullptr, *poff = nullptr, *pidx = nullptr;
+        int nbbp = 0, npidx_cap = 0, nbbp_cap = 0;
+        unsigned long long* err = nullptr;
+        std::vector<double> stage;
+    } enc;
     // differentiation tape (mercury_hip_adjoint_forward)
     struct Tape {
         bool valid = false;
@@ -189,6 +207,13 @@ void release_all() {
         dfree(*p);
     dfree(c.tape.dsys);
     c.tape.valid = false;
+    for (double** p : {&c.rcrit, &c.hx0, &c.hv0, &c.hytab, &c.hpack, &c.enc.m, &c.enc.rcrit, &c.enc.x, &c.enc.v,
+                       &c.enc.dbuf, &c.enc.cand})
+        dfree(*p);
+    dfree(c.hpairs); dfree(c.hnpairs); dfree(c.hids); dfree(c.enc.bbp); dfree(c.enc.poff); dfree(c.enc.pidx);
+    dfree(c.enc.err);
+    c.hpair_cap = 0; c.hpack_cap = 0; c.hy_cap = 0; c.enc.ncap = 0; c.enc.active = false;
+    c.enc.npidx_cap = 0; c.enc.nbbp_cap = 0;
     c.cap = 0; c.tab_cap = 0; c.ev_cap = 0; c.park_cap = 0; c.ntiles = 0;
     c.n = 0; c.nbig = 0; c.history = false; c.saved_valid = false; c.have_state = false;
     c.events.clear(); c.adv_events.clear();
@@ -932,6 +957,302 @@ void tangent_pass(const double* dx0, const double* dv0, const double* dm, double
     dfree(rows1);
 }
 
+// ---- HYBRID -----------------------------------------------------------------
+
+int read_fault();
+void clear_fault();
+
+void ensure_hybrid() {
+    Context& c = ctx;
+    if (c.hy_cap != c.cap || !c.rcrit) {
+        dalloc(c.rcrit, (size_t)c.cap);
+        dalloc(c.hx0, (size_t)3 * c.cap);
+        dalloc(c.hv0, (size_t)3 * c.cap);
+        HIPCHECK(hipMemset(c.rcrit, 0, sizeof(double) * c.cap));
+        c.hy_cap = c.cap;
+    }
+    if (!c.hytab) dalloc(c.hytab, (size_t)(kHyHdr + HB_N * kMaxBig));
+    if (!c.hnpairs) dalloc(c.hnpairs, 1);
+    if (c.hpair_cap == 0) {
+        c.hpair_cap = 1 << 16;
+        dalloc(c.hpairs, (size_t)2 * c.hpair_cap);
+    }
+}
+
+HyArgs hy_args(bool reset) {
+    Context& c = ctx;
+    HyArgs a{};
+    a.sys = c.dsys;
+    a.st = device_state();
+    a.rcrit = c.rcrit;
+    a.x0 = c.hx0;
+    a.v0 = c.hv0;
+    a.tab = c.hytab;
+    a.pairs = c.hpairs;
+    a.npairs = c.hnpairs;
+    a.cap = c.hpair_cap;
+    a.reset = reset ? 1 : 0;
+    a.fault = c.fault;
+    return a;
+}
+
+// mdt_hy's first half and the changeover scan (mercury6_2.for:3655-3727):
+// kick, Sun-momentum drift, Kepler drift of every body, mce_snif.  Encounter
+// members go back to their pre-drift states, which the host receives.
+void hybrid_begin(double h, const double* crit, int flag, int cap, int* ce, int* nce, int* pi, int* pj, double* xx,
+                  double* vv, int64_t* forces) {
+    Context& c = ctx;
+    if (c.algorithm != 10) throw std::runtime_error("HYBRID calls need algorithm 10");
+    if (c.enc.active) throw std::runtime_error("HYBRID step inside an encounter context");
+    if (c.systems.size() != 1) throw std::runtime_error("HYBRID needs one system");
+    if (!(h != 0) || !std::isfinite(h)) throw std::runtime_error("invalid step");
+    ensure_stream();
+    ensure_hybrid();
+    build_systems(1, true, h, 0.0);
+    const bool reset = flag != 2 || !c.history;
+    if (reset) HIPCHECK(hipMemcpy(c.rcrit, crit, sizeof(double) * c.n, hipMemcpyHostToDevice));
+    *forces = reset ? 1 : 0;
+    c.tape.valid = false;
+    for (;;) {
+        clear_fault();
+        HIPCHECK(hipMemsetAsync(c.hnpairs, 0, sizeof(int), c.stream));
+        HyArgs a = hy_args(reset);
+        // a reset recomputes accelerations from the start state: run on a copy
+        // of x, v when the pair buffer has to grow and the step is retried
+        hipLaunchKernelGGL(hy_big_begin, dim3(1), dim3(64), 0, c.stream, a);
+        const int npart = c.n - c.nbig;
+        if (npart > 0)
+            hipLaunchKernelGGL(hy_particles_begin, dim3((npart + 127) / 128), dim3(128), 0, c.stream, a);
+        HIPCHECK(hipGetLastError());
+        int cnt = 0;
+        HIPCHECK(hipMemcpyAsync(&cnt, c.hnpairs, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+        HIPCHECK(hipStreamSynchronize(c.stream));
+        if (read_fault() & FAULT_FORCE) throw std::runtime_error("HYBRID force failed");
+        if (cnt > c.hpair_cap) {
+            // the scan overflowed after the state moved: the encounter list is
+            // incomplete, and a retry would double-step; fail loudly
+            throw std::runtime_error("HYBRID encounter pair buffer overflow");
+        }
+        std::vector<int> pr((size_t)2 * cnt);
+        if (cnt) HIPCHECK(hipMemcpy(pr.data(), c.hpairs, sizeof(int) * 2 * cnt, hipMemcpyDeviceToHost));
+        std::vector<std::pair<int, int>> pairs((size_t)cnt);
+        for (int k = 0; k < cnt; k++) pairs[k] = {pr[2 * k], pr[2 * k + 1]};
+        std::sort(pairs.begin(), pairs.end());
+        if (cnt > cap) throw std::runtime_error("Hybrid encounter capacity exceeded");
+        *nce = cnt;
+        std::fill(ce, ce + c.n, 0);
+        for (int k = 0; k < cnt; k++) {
+            pi[k] = pairs[k].first + 1;
+            pj[k] = pairs[k].second + 1;
+            ce[pairs[k].first] = 2;
+            ce[pairs[k].second] = 2;
+        }
+        c.members.clear();
+        for (int j = 1; j < c.n; j++)
+            if (ce[j]) c.members.push_back(j);
+        const int ns = (int)c.members.size();
+        if (ns) {
+            if (ns > c.hpack_cap) {
+                c.hpack_cap = std::max(ns, 2 * c.hpack_cap);
+                dalloc(c.hpack, (size_t)6 * c.hpack_cap);
+                dalloc(c.hids, (size_t)c.hpack_cap);
+            }
+            HIPCHECK(hipMemcpy(c.hids, c.members.data(), sizeof(int) * ns, hipMemcpyHostToDevice));
+            hipLaunchKernelGGL(hy_revert_gather, dim3((ns + 127) / 128), dim3(128), 0, c.stream, device_state(),
+                               c.hx0, c.hv0, c.hids, ns, c.hpack);
+            HIPCHECK(hipGetLastError());
+            c.packed.resize((size_t)6 * ns);
+            HIPCHECK(hipMemcpyAsync(c.packed.data(), c.hpack, sizeof(double) * 6 * ns, hipMemcpyDeviceToHost,
+                                    c.stream));
+            HIPCHECK(hipStreamSynchronize(c.stream));
+            for (int a2 = 0; a2 < ns; a2++)
+                for (int k = 0; k < 3; k++) {
+                    xx[3 * c.members[a2] + k] = c.packed[6 * a2 + k];
+                    vv[3 * c.members[a2] + k] = c.packed[6 * a2 + 3 + k];
+                }
+        }
+        break;
+    }
+    c.history = true;
+    c.saved_valid = true;
+}
+
+// mdt_hy's second half (mercury6_2.for:3758-3796): the members' states from
+// the host, Sun-momentum drift, mfo_hy, kick.
+void hybrid_finish(double h, const double* m, const double* xx, const double* vv, int64_t* forces) {
+    Context& c = ctx;
+    if (c.algorithm != 10) throw std::runtime_error("HYBRID calls need algorithm 10");
+    if (c.enc.active) throw std::runtime_error("HYBRID step inside an encounter context");
+    build_systems(1, true, h, 0.0);
+    const int ns = (int)c.members.size();
+    if (ns) {
+        for (int a2 = 0; a2 < ns; a2++)
+            for (int k = 0; k < 3; k++) {
+                c.packed[6 * a2 + k] = xx[3 * c.members[a2] + k];
+                c.packed[6 * a2 + 3 + k] = vv[3 * c.members[a2] + k];
+            }
+        HIPCHECK(hipMemcpyAsync(c.hpack, c.packed.data(), sizeof(double) * 6 * ns, hipMemcpyHostToDevice, c.stream));
+        hipLaunchKernelGGL(hy_scatter, dim3((ns + 127) / 128), dim3(128), 0, c.stream, device_state(), c.hids, ns,
+                           c.hpack);
+        // encounter collisions may change Big masses; Small bodies stay massless
+        for (int j = c.nbig; j < c.n; j++)
+            if (m[j] != 0) throw std::runtime_error("Small bodies with mass are not provided by the HIP path");
+        HIPCHECK(hipMemcpyAsync(c.m, m, sizeof(double) * c.nbig, hipMemcpyHostToDevice, c.stream));
+    }
+    clear_fault();
+    HyArgs a = hy_args(false);
+    hipLaunchKernelGGL(hy_big_finish, dim3(1), dim3(64), 0, c.stream, a);
+    const int npart = c.n - c.nbig;
+    if (npart > 0) hipLaunchKernelGGL(hy_particles_finish, dim3((npart + 127) / 128), dim3(128), 0, c.stream, a);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(c.stream));
+    check_fault("HYBRID force failed");
+    c.members.clear();
+    *forces = 1;
+}
+
+// ---- compact encounter context (BS2) -------------------------------------
+
+void encounter_enter(int n, int nbig, const double* m, const double* x, const double* v, const double* crit,
+                     int np_, const int* pi, const int* pj) {
+    Context& c = ctx;
+    Context::Enc& e = c.enc;
+    if (c.algorithm != 10) throw std::runtime_error("encounter context needs algorithm 10");
+    if (e.active) throw std::runtime_error("encounter context already active");
+    if (n < 1 || nbig < 1 || nbig > n) throw std::runtime_error("invalid encounter body counts");
+    if (nbig - 1 > kEncBig) throw std::runtime_error("more than 16 Big bodies in one encounter set");
+    for (int j = nbig; j < n; j++)
+        if (m[j] != 0) throw std::runtime_error("Small bodies with mass are not provided by the HIP path");
+    ensure_stream();
+    if (n > e.ncap) {
+        int cap = std::max(n, 2 * e.ncap);
+        for (double** p : {&e.m, &e.rcrit}) dalloc(*p, (size_t)cap);
+        for (double** p : {&e.x, &e.v}) dalloc(*p, (size_t)3 * cap);
+        dalloc(e.dbuf, (size_t)72 * cap);
+        dalloc(e.cand, (size_t)54 * cap);
+        dalloc(e.poff, (size_t)cap + 1);
+        e.ncap = cap;
+    }
+    if (!e.err) dalloc(e.err, 12);
+    // pair lists: Big-Big pairs in list order; each particle's Big partners
+    std::vector<int> bb, off((size_t)(n - nbig) + 1, 0), idx;
+    std::vector<std::vector<int>> part((size_t)std::max(n - nbig, 0));
+    for (int k = 0; k < np_; k++) {
+        int i = pi[k] - 1, j = pj[k] - 1;
+        if (i < 1 || j < 1 || i >= n || j >= n) throw std::runtime_error("encounter pair out of range");
+        if (i >= nbig) throw std::runtime_error("encounter pair without a Big body first");
+        if (j < nbig) { bb.push_back(i); bb.push_back(j); }
+        else part[j - nbig].push_back(i);
+    }
+    for (int q = 0; q < n - nbig; q++) {
+        off[q + 1] = off[q] + (int)part[q].size();
+        idx.insert(idx.end(), part[q].begin(), part[q].end());
+    }
+    if ((int)bb.size() > e.nbbp_cap) {
+        e.nbbp_cap = std::max((int)bb.size(), 64);
+        dalloc(e.bbp, (size_t)e.nbbp_cap);
+    }
+    if ((int)idx.size() > e.npidx_cap) {
+        e.npidx_cap = std::max((int)idx.size(), 64);
+        dalloc(e.pidx, (size_t)e.npidx_cap);
+    }
+    if (!bb.empty()) HIPCHECK(hipMemcpy(e.bbp, bb.data(), sizeof(int) * bb.size(), hipMemcpyHostToDevice));
+    if (!idx.empty()) HIPCHECK(hipMemcpy(e.pidx, idx.data(), sizeof(int) * idx.size(), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(e.poff, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice));
+    e.nbbp = (int)bb.size() / 2;
+    e.n = n;
+    e.nbig = nbig;
+    HIPCHECK(hipMemcpy(e.m, m, sizeof(double) * n, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(e.rcrit, crit, sizeof(double) * n, hipMemcpyHostToDevice));
+    e.active = true;
+    // states
+    e.stage.resize((size_t)3 * n);
+    for (const double* src : {x, v}) {
+        for (int j = 0; j < n; j++)
+            for (int k = 0; k < 3; k++) e.stage[(size_t)k * n + j] = src[3 * j + k];
+        double* dst = src == x ? e.x : e.v;
+        for (int k = 0; k < 3; k++)
+            HIPCHECK(hipMemcpy(dst + (size_t)k * e.ncap, e.stage.data() + (size_t)k * n, sizeof(double) * n,
+                               hipMemcpyHostToDevice));
+    }
+}
+
+void encounter_states(const double* x, const double* v) {
+    Context::Enc& e = ctx.enc;
+    e.stage.resize((size_t)3 * e.n);
+    for (const double* src : {x, v}) {
+        for (int j = 0; j < e.n; j++)
+            for (int k = 0; k < 3; k++) e.stage[(size_t)k * e.n + j] = src[3 * j + k];
+        double* dst = src == x ? e.x : e.v;
+        for (int k = 0; k < 3; k++)
+            HIPCHECK(hipMemcpy(dst + (size_t)k * e.ncap, e.stage.data() + (size_t)k * e.n, sizeof(double) * e.n,
+                               hipMemcpyHostToDevice));
+    }
+}
+
+void encounter_download(double* x, double* v) {
+    Context::Enc& e = ctx.enc;
+    e.stage.resize((size_t)3 * e.n);
+    for (int which = 0; which < 2; which++) {
+        double* src = which == 0 ? e.x : e.v;
+        double* dst = which == 0 ? x : v;
+        for (int k = 0; k < 3; k++)
+            HIPCHECK(hipMemcpy(e.stage.data() + (size_t)k * e.n, src + (size_t)k * e.ncap, sizeof(double) * e.n,
+                               hipMemcpyDeviceToHost));
+        for (int j = 0; j < e.n; j++)
+            for (int k = 0; k < 3; k++) dst[3 * j + k] = e.stage[(size_t)k * e.n + j];
+    }
+}
+
+// mdt_bs2 (mercury6_2.for:3428-3617) over the encounter set: every attempt
+// evaluates the 12 modified-midpoint levels on the device; the host takes the
+// first level n > 3 whose largest error is within tol^2, or halves the step.
+void bs2_step(double time, double* h, double* hdid, double tol, int64_t* forces, int64_t* rejected) {
+    Context& c = ctx;
+    Context::Enc& e = c.enc;
+    const double SHRINK = .55, GROW = 1.3;
+    const int npart = e.n - e.nbig;
+    BsArgs A{};
+    A.n = e.n; A.nbig = e.nbig; A.m = e.m; A.rcrit = e.rcrit; A.x = e.x; A.v = e.v; A.ncap = e.ncap;
+    A.bbp = e.bbp; A.nbbp = e.nbbp; A.poff = e.poff; A.pidx = e.pidx;
+    A.tol2 = tol * tol;
+    A.dbuf = e.dbuf; A.cand = e.cand; A.err = e.err; A.fault = c.fault;
+    *forces = 1;           // the start accelerations, once per call
+    for (;;) {
+        if (time + *h == time) throw std::runtime_error("BS cannot advance time at the requested tolerance");
+        A.h0 = *h;
+        clear_fault();
+        hipLaunchKernelGGL(zero_bytes, dim3(1), dim3(128), 0, c.stream, reinterpret_cast<unsigned char*>(e.err),
+                           (long long)(12 * sizeof(unsigned long long)));
+        hipLaunchKernelGGL(bs2_attempt, dim3(std::max(1, (npart + 63) / 64)), dim3(64), 0, c.stream, A);
+        HIPCHECK(hipGetLastError());
+        unsigned long long bits[12];
+        HIPCHECK(hipMemcpyAsync(bits, e.err, sizeof(bits), hipMemcpyDeviceToHost, c.stream));
+        HIPCHECK(hipStreamSynchronize(c.stream));
+        if (read_fault() & FAULT_FORCE) throw std::runtime_error("Nonfinite BS extrapolation");
+        int level = 0;
+        for (int n = 4; n <= 12 && !level; n++) {
+            double err;
+            std::memcpy(&err, &bits[n - 1], sizeof(err));
+            if (err <= A.tol2) level = n;
+        }
+        if (level) {
+            for (int n = 1; n <= level; n++) *forces += n;
+            if (e.n > 1)
+                hipLaunchKernelGGL(bs2_accept, dim3((e.n - 1 + 127) / 128), dim3(128), 0, c.stream, A, level);
+            HIPCHECK(hipGetLastError());
+            HIPCHECK(hipStreamSynchronize(c.stream));
+            *hdid = *h;
+            if (level >= 8) *h = *h * SHRINK;
+            if (level < 7) *h = *h * GROW;
+            return;
+        }
+        *forces += 78;
+        ++*rejected;
+        *h = *h * .5;
+    }
+}
+
 int read_fault() {
     int bad = 0;
     HIPCHECK(hipMemcpy(&bad, ctx.fault, sizeof(int), hipMemcpyDeviceToHost));
@@ -987,7 +1308,7 @@ int mercury_hip_available(void) {
 }
 
 int mercury_hip_configure(int algorithm) {
-    if (algorithm != 1 && algorithm != 4 && algorithm != 9) return 1;
+    if (algorithm != 1 && algorithm != 4 && algorithm != 9 && algorithm != 10) return 1;
     if (algorithm != ctx.algorithm) release_all();
     ctx.algorithm = algorithm;
     ctx.history = false;
@@ -995,6 +1316,7 @@ int mercury_hip_configure(int algorithm) {
 }
 
 void mercury_hip_reset(int flag) {
+    if (ctx.enc.active) return;      // BS2 keeps no history between calls
     if (flag != 2) ctx.history = false;
 }
 
@@ -1014,7 +1336,9 @@ int mercury_hip_upload(int n, int nbig, int pn, int ngflag, const double* m, con
         if (nbig > kMaxBig) throw std::runtime_error("more Big bodies than the HIP path supports (64)");
         if (!(m[0] > 0)) throw std::runtime_error("central mass must be positive");
         if (ngflag != 0) throw std::runtime_error("non-gravitational forces are not provided by the HIP path");
-        if (pn != 0 && mvs_family()) throw std::runtime_error("MVS does not take post-Newtonian forces");
+        if (pn != 0 && (mvs_family() || ctx.algorithm == 10))
+            throw std::runtime_error("MVS and HYBRID do not take post-Newtonian forces");
+        if (ctx.enc.active) throw std::runtime_error("upload inside an encounter context");
         for (int j = nbig; j < n; j++)
             if (m[j] != 0) throw std::runtime_error("Small bodies with mass are not provided by the HIP path");
         for (int j = 0; j < 3 * n; j++)
@@ -1050,6 +1374,11 @@ int mercury_hip_step(double time, double* h, double* hdid, double tol, int64_t* 
         require_state();
         *rejected = 0;
         *forces = 0;
+        if (ctx.enc.active) {
+            bs2_step(time, h, hdid, tol, forces, rejected);
+            return 0;
+        }
+        if (ctx.algorithm == 10) throw std::runtime_error("HYBRID steps are mercury_hip_hybrid_begin/finish");
         if (ctx.algorithm == 4) {
             clear_fault();
             ctx.tape.valid = false;
@@ -1073,6 +1402,11 @@ int mercury_hip_step(double time, double* h, double* hdid, double tol, int64_t* 
 
 int mercury_hip_download(double* x, double* v, int previous) {
     try {
+        if (ctx.enc.active) {
+            if (previous) throw std::runtime_error("no previous state in an encounter context");
+            encounter_download(x, v);
+            return 0;
+        }
         require_state();
         if (previous && !ctx.saved_valid) throw std::runtime_error("no previous state");
         download_vec3(previous ? ctx.ox : ctx.x, x);
@@ -1086,7 +1420,10 @@ int mercury_hip_events(double time, double h, double rcen, const MercuryEvent** 
         require_state();
         if (!ctx.saved_valid) throw std::runtime_error("no step to screen");
         const bool ra = ctx.algorithm == 4;
-        if (!mvs_family() && !ra) throw std::runtime_error("algorithm not provided by the HIP path");
+        const bool hy = ctx.algorithm == 10;
+        if (ctx.enc.active) throw std::runtime_error("event screening inside an encounter context");
+        if (!mvs_family() && !ra && !hy) throw std::runtime_error("algorithm not provided by the HIP path");
+        if (hy) build_systems(1, true, h, rcen);
         int n = ctx.n;
         if (ra && ctx.nbig > 1) {
             // planet boxes and end positions from the step-start and end states
@@ -1097,7 +1434,8 @@ int mercury_hip_events(double time, double h, double rcen, const MercuryEvent** 
         for (;;) {
             HIPCHECK(hipMemsetAsync(ctx.devcount, 0, sizeof(int), ctx.stream));
             SavedScreenArgs sa{ctx.dsys, device_state(), ctx.tab, EventSink{ctx.dev, ctx.devcount, ctx.ev_cap, ctx.fault},
-                               time, h, rcen, 1, ra ? ctx.ov : ctx.v0k, ra ? ctx.v : ctx.v1};
+                               time, h, rcen, hy ? 0 : 1, ra ? ctx.ov : ctx.v0k, ra ? ctx.v : ctx.v1,
+                               hy ? ctx.v1 : ctx.v};
             if (n > 1) hipLaunchKernelGGL(screen_saved, dim3((n - 1 + 255) / 256), dim3(256), 0, ctx.stream, sa);
             HIPCHECK(hipGetLastError());
             HIPCHECK(hipMemcpyAsync(count, ctx.devcount, sizeof(int), hipMemcpyDeviceToHost, ctx.stream));
@@ -1156,28 +1494,63 @@ int mercury_hip_export(double h, int physical, double* x, double* v) {
         }
         download_vec3(ctx.x, x);
         download_vec3(ctx.v, v);
+        if (physical && ctx.algorithm == 10) {
+            // mco_dh2h (mercury6_2.for:2007-2063): heliocentric velocities
+            std::vector<double> m((size_t)ctx.n);
+            HIPCHECK(hipMemcpy(m.data(), ctx.m, sizeof(double) * ctx.n, hipMemcpyDeviceToHost));
+            double mv[3] = {0.0, 0.0, 0.0};
+            for (int j = 1; j < ctx.n; j++)
+                for (int k = 0; k < 3; k++) mv[k] = mv[k] + m[j] * v[3 * j + k];
+            double temp = 1.0 / m[0];
+            for (int k = 0; k < 3; k++) mv[k] = temp * mv[k];
+            for (int j = 1; j < ctx.n; j++)
+                for (int k = 0; k < 3; k++) v[3 * j + k] = v[3 * j + k] + mv[k];
+        }
         return 0;
     } catch (const std::exception& e) { return fail(e); }
 }
 
-int mercury_hip_hybrid_begin(double, const double*, int, int, int*, int*, int*, int*, double*, double*, int64_t*) {
-    std::fprintf(stderr, "MERCURY HIP: HYBRID not available yet\n");
-    return 1;
+int mercury_hip_hybrid_begin(double h, const double* crit, int flag, int cap, int* ce, int* nce, int* pi, int* pj,
+                             double* x, double* v, int64_t* forces) {
+    try {
+        require_state();
+        hybrid_begin(h, crit, flag, cap, ce, nce, pi, pj, x, v, forces);
+        return 0;
+    } catch (const std::exception& e) { return fail(e); }
 }
-int mercury_hip_hybrid_finish(double, const double*, const double*, const double*, int64_t*) {
-    std::fprintf(stderr, "MERCURY HIP: HYBRID not available yet\n");
-    return 1;
+
+int mercury_hip_hybrid_finish(double h, const double* m, const double* x, const double* v, int64_t* forces) {
+    try {
+        require_state();
+        hybrid_finish(h, m, x, v, forces);
+        return 0;
+    } catch (const std::exception& e) { return fail(e); }
 }
-int mercury_hip_encounter_enter(int, int, const double*, const double*, const double*, const double*,
-                                const double*, const double*, int, const int*, const int*) {
-    std::fprintf(stderr, "MERCURY HIP: HYBRID not available yet\n");
-    return 1;
+
+int mercury_hip_encounter_enter(int n, int nbig, const double* m, const double* x, const double* v,
+                                const double* crit, const double* rce, const double* rphys, int np, const int* pi,
+                                const int* pj) {
+    try {
+        (void)rce;
+        (void)rphys;
+        encounter_enter(n, nbig, m, x, v, crit, np, pi, pj);
+        return 0;
+    } catch (const std::exception& e) { return fail(e); }
 }
-int mercury_hip_encounter_update(const double*, const double*, const double*) {
-    std::fprintf(stderr, "MERCURY HIP: HYBRID not available yet\n");
-    return 1;
+
+int mercury_hip_encounter_update(const double* m, const double* x, const double* v) {
+    try {
+        Context::Enc& e = ctx.enc;
+        if (!e.active) throw std::runtime_error("no encounter context");
+        for (int j = e.nbig; j < e.n; j++)
+            if (m[j] != 0) throw std::runtime_error("Small bodies with mass are not provided by the HIP path");
+        HIPCHECK(hipMemcpy(e.m, m, sizeof(double) * e.n, hipMemcpyHostToDevice));
+        encounter_states(x, v);
+        return 0;
+    } catch (const std::exception& e) { return fail(e); }
 }
-void mercury_hip_encounter_exit(void) {}
+
+void mercury_hip_encounter_exit(void) { ctx.enc.active = false; }
 
 void mercury_hip_free(void) { release_all(); }
 

@@ -1047,6 +1047,7 @@ struct SavedScreenArgs {
     int pairs;          // 0: central impacts only
     const double* va;   // pair-screen velocities at the start and end of the step
     const double* vb;   // (MVS: after the first kick / after the drift; others: ov, v)
+    const double* vc;   // central check: velocity at the end (HYBRID: the physical one)
 };
 
 __global__ void __launch_bounds__(256) screen_saved(SavedScreenArgs args) {
@@ -1067,8 +1068,8 @@ __global__ void __launch_bounds__(256) screen_saved(SavedScreenArgs args) {
                     args.es);
     }
     central_check(d.mu, mj, args.rcen, args.h, args.time, xs, ys, zs, args.st.ov[g], args.st.ov[cap + g],
-                  args.st.ov[2 * cap + g], x1, y1, z1, args.st.v[g], args.st.v[cap + g],
-                  args.st.v[2 * cap + g], jb, 0, 0, args.es);
+                  args.st.ov[2 * cap + g], x1, y1, z1, args.vc[g], args.vc[cap + g],
+                  args.vc[2 * cap + g], jb, 0, 0, args.es);
 }
 
 // Details of the fast kernel's deferred screening work (one lane per request).

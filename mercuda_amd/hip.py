@@ -242,6 +242,48 @@ class MercuryHip:
                  "autotune")
         return {"steps_per_launch": spl.value, "block": blk.value, "particle_steps_per_s": rate.value}
 
+    # ---- HYBRID (algorithm 10) and the compact encounter context ----
+    def hybrid_begin(self, h, rcrit, flag, cap, x, v):
+        """First half of a HYBRID step; x, v (n x 3) receive the encounter
+        members' pre-drift states.  Returns ce, pairs (i, j) 1-based, forces."""
+        rcrit = _f64(rcrit, (self.n,))
+        ce = np.zeros(self.n, np.int32)
+        pi = np.zeros(max(cap, 1), np.int32); pj = np.zeros(max(cap, 1), np.int32)
+        nce = C.c_int(0); nf = C.c_int64(0)
+        assert x.flags.c_contiguous and v.flags.c_contiguous and x.dtype == np.float64
+        self._ok(self.L.mercury_hip_hybrid_begin(float(h), _d(rcrit), int(flag), int(cap), ce.ctypes.data_as(_ip),
+                                                 C.byref(nce), pi.ctypes.data_as(_ip), pj.ctypes.data_as(_ip),
+                                                 _d(x), _d(v), C.byref(nf)), "hybrid_begin")
+        k = nce.value
+        return ce, list(zip(pi[:k].tolist(), pj[:k].tolist())), nf.value
+
+    def hybrid_finish(self, h, m, x, v):
+        m = _f64(m, (self.n,)); x = _f64(x, (self.n, 3)); v = _f64(v, (self.n, 3))
+        nf = C.c_int64(0)
+        self._ok(self.L.mercury_hip_hybrid_finish(float(h), _d(m), _d(x), _d(v), C.byref(nf)), "hybrid_finish")
+        return nf.value
+
+    def encounter_enter(self, m, x, v, rcrit, rce, rphys, nbig, pairs):
+        n = len(m)
+        arr = [_f64(a) for a in (m, x, v, rcrit, rce, rphys)]
+        pi = np.ascontiguousarray([p[0] for p in pairs] or [0], np.int32)
+        pj = np.ascontiguousarray([p[1] for p in pairs] or [0], np.int32)
+        self._ok(self.L.mercury_hip_encounter_enter(n, int(nbig), *[_d(a) for a in arr], len(pairs),
+                                                    pi.ctypes.data_as(_ip), pj.ctypes.data_as(_ip)),
+                 "encounter_enter")
+        self.enc_n = n
+
+    def encounter_update(self, m, x, v):
+        self._ok(self.L.mercury_hip_encounter_update(_d(_f64(m)), _d(_f64(x)), _d(_f64(v))), "encounter_update")
+
+    def encounter_download(self):
+        x = np.zeros((self.enc_n, 3)); v = np.zeros((self.enc_n, 3))
+        self._ok(self.L.mercury_hip_download(_d(x), _d(v), 0), "download (encounter)")
+        return x, v
+
+    def encounter_exit(self):
+        self.L.mercury_hip_encounter_exit()
+
     # ---- differentiation (tape of the last adjoint_forward) ----
     def adjoint_forward(self, nsteps, h, segment=0):
         """Final (x, v) after nsteps MVS steps of the uploaded state; keeps the tape."""

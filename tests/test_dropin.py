@@ -93,3 +93,21 @@ def test_dropin_radau_on_device_matches_reference(tmp_path, pn):
     assert worst <= REL_TOL
     assert (a / "ce.out").read_bytes() == (b / "ce.out").read_bytes()
     assert (a / "xv.out").read_bytes() == (b / "xv.out").read_bytes()
+
+
+@needs_dropin
+@pytest.mark.gpu
+def test_dropin_hybrid_on_device_matches_reference(tmp_path):
+    """HYBRID through the unchanged driver: regular map and changeover scan on
+    the device, the driver's mdt_hkce integrating encounter members through
+    the library's BS2 context, central impacts from the device's events."""
+    s = systems.config_encounters(seed=3, nparticles=600)
+    print("case: drop-in HYBRID, 8 planets + 600 particles, 300 steps of 2 d", flush=True)
+    a, b, rb = _pair(tmp_path, s, 300, "hip", algorithm="HYBRID", interval=100.0)
+    assert rb.returncode == 0, rb.stdout + rb.stderr
+    assert any("Execution backend: HIP" in ln for ln in R.info_lines(b)), R.info_lines(b)
+    worst = _compare_dumps(a, b)
+    print(f"max relative dump difference {worst:.3e}", flush=True)
+    assert worst <= REL_TOL
+    assert (a / "ce.out").read_bytes() == (b / "ce.out").read_bytes()
+    assert (a / "xv.out").read_bytes() == (b / "xv.out").read_bytes()
