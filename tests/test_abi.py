@@ -53,7 +53,9 @@ def test_host_side_errors_without_device(lib, capfd):
     returns with a message, and no device call is needed to reach them."""
     import ctypes as C
     assert lib.mercury_hip_configure(2) != 0          # BS: not served
-    assert lib.mercury_hip_configure(10) != 0         # HYBRID: not served
+    assert lib.mercury_hip_configure(3) != 0          # BS2 (outside an encounter context): not served
+    assert lib.mercury_hip_configure(10) == 0         # HYBRID
+    assert lib.mercury_hip_encounter_update(None, None, None) != 0   # no encounter context
     h = C.c_double(4.0); hd = C.c_double(); nf = C.c_int64(); nr = C.c_int64()
     lib.mercury_hip_free()
     assert lib.mercury_hip_step(0.0, C.byref(h), C.byref(hd), 1e-12, C.byref(nf), C.byref(nr)) != 0
@@ -63,7 +65,7 @@ def test_host_side_errors_without_device(lib, capfd):
     assert lib.mercury_hip_set_tuning(0, 512) != 0
     assert lib.mercury_hip_set_tuning(0, 96) != 0
     err = capfd.readouterr().err
-    assert "no state uploaded" in err and "block size" in err
+    assert "no state uploaded" in err and "block size" in err and "no encounter context" in err
 
 
 def test_no_device_means_not_available(lib):
