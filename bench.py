@@ -175,6 +175,60 @@ def adjoint_cost(hip, nsteps=10000, nparticles=4096):
             "checkpoint_interval": int(np.ceil(np.sqrt(nsteps)))}
 
 
+def radau_rate(hip, nparticles=1_000_000, nseq=12):
+    """RA15 (tol 1e-12, central 1PN) on 8 planets + 1M particles of the MVS
+    workload's generator, first trial step 4 d: accepted sequences x
+    particles per second through mercury_hip_step (one host round trip per
+    sequence for the step-size law), after 3 warm-up sequences."""
+    from mercuda_amd import systems
+    s = systems.config_large(seed=2, nparticles=nparticles)
+    hip.configure(4)
+    hip.upload_system(s, pn=1)
+    h, t = s["h"], 0.0
+    for _ in range(3):
+        h, hd, _, _ = hip.step(t, h, 1e-12)
+        t += hd
+    forces = 0
+    days = 0.0
+    tic = time.perf_counter()
+    for _ in range(nseq):
+        h, hd, nf, _ = hip.step(t, h, 1e-12)
+        t += hd
+        days += hd
+        forces += nf
+    sec = time.perf_counter() - tic
+    return {"workload": f"RADAU tol 1e-12 with 1PN, 8 planets + {nparticles} particles",
+            "sequences": nseq, "particle_steps_per_s": round(nparticles * nseq / sec, 1),
+            "particle_force_evaluations_per_s": round(nparticles * forces / sec, 1),
+            "ms_per_sequence": round(sec * 1e3 / nseq, 3), "mean_step_days": round(days / nseq, 4)}
+
+
+def hybrid_rate(hip, nparticles=200_000, nsteps=3):
+    """HYBRID on 8 planets + 200k particles, half within 0.1 AU of a planet's
+    orbit (changeover 3 Hill radii, dt 2 d): every step's device work --
+    regular map, changeover scan, encounter members' BS2 substeps through the
+    compact context -- driven by mercuda_amd.hybrid as the driver drives it.
+    The driver's own host bookkeeping (encounter listing, output) is not in
+    the timed loop."""
+    from mercuda_amd import hybrid, systems
+    s = systems.config_encounters(seed=3, nparticles=nparticles)
+    hip.configure(10)
+    hip.upload_system(s)
+    hybrid.hy_run(hip, s, 1, upload=False)
+    hip.configure(10)
+    hip.upload_system(s)
+    hybrid.mdt_hkce.substeps = 0
+    tic = time.perf_counter()
+    _, _, most = hybrid.hy_run(hip, s, nsteps, upload=False,
+                               progress=lambda k, npairs: log(f"hybrid step {k}: {npairs} encounter pairs, "
+                                                              f"{hybrid.mdt_hkce.substeps} BS2 substeps so far"))
+    sec = time.perf_counter() - tic
+    return {"workload": f"HYBRID, 8 planets + {nparticles} particles (half near planet orbits), dt 2 d",
+            "steps": nsteps, "particle_steps_per_s": round(nparticles * nsteps / sec, 1),
+            "ms_per_step": round(sec * 1e3 / nsteps, 3), "most_encounter_pairs_in_a_step": most,
+            "bs2_substeps": hybrid.mdt_hkce.substeps}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -261,6 +315,10 @@ def main():
         extras["ensemble"] = ensemble_ratio(hip, steps=64, spl=spl, block=block)
         log("adjoint: orbit-fitting workload")
         extras["adjoint"] = adjoint_cost(hip)
+        log("radau: 8 planets + 1M particles")
+        extras["radau"] = radau_rate(hip)
+        log("hybrid: 8 planets + 200k particles")
+        extras["hybrid"] = hybrid_rate(hip)
     cpu = None
     if not args.no_cpu:
         log(f"cpu baseline: reference mdt_mvs on {args.cpu_procs} processes")

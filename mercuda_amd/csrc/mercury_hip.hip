@@ -973,8 +973,13 @@ void ensure_hybrid() {
     }
     if (!c.hytab) dalloc(c.hytab, (size_t)(kHyHdr + HB_N * kMaxBig));
     if (!c.hnpairs) dalloc(c.hnpairs, 1);
-    if (c.hpair_cap == 0) {
-        c.hpair_cap = 1 << 16;
+    // room for every possible pair (each particle with every Big body), so
+    // the scan can never overflow after the state has moved
+    const long long np = std::max(c.nbig - 1, 1);
+    const long long need = std::max<long long>(1 << 16, (long long)(c.n - c.nbig) * np + np * np);
+    if (need > (1LL << 31) - 1) throw std::runtime_error("too many possible HYBRID pairs");
+    if (c.hpair_cap < need) {
+        c.hpair_cap = (int)need;
         dalloc(c.hpairs, (size_t)2 * c.hpair_cap);
     }
 }

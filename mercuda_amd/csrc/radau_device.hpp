@@ -400,7 +400,7 @@ __global__ void __launch_bounds__(128) ra_particles(RaArgs args) {
             for (int c = 0; c < 3; c++) ra_g_from_b(args.k, b[c], g[c]);
         int entry = 1;
         for (int n = 1; n <= args.niter; n++) {
-#pragma unroll
+#pragma unroll 1
             for (int j = 2; j <= 8; j++, entry++) {
                 const double hj = args.k.h[j - 1];
                 double s[9], x[3], v[3] = {0, 0, 0};
