@@ -130,6 +130,14 @@ int64_t mercury_hip_parked(void);
  * since timing was last enabled. */
 int mercury_hip_work_read(int64_t *fast_steps, int64_t *general_steps);
 
+/* Device self-check of the arithmetic (no counterpart in the reference): the
+ * fast MVS kernel divides and takes square roots with shorter sequences that
+ * are exact for operands inside the normal range.  This draws n random
+ * operand sets from `seed` over those ranges and writes, for the divide, the
+ * reciprocal, the square root and the divide by a constant, how many results
+ * differ in any bit from the compiler's IEEE expansions (mismatches[4]). */
+int mercury_hip_check_arith(int64_t n, uint64_t seed, int64_t *mismatches);
+
 /* hipGraph mode for mercury_hip_advance / ensemble advance (no counterpart in
  * the reference): enable = 1 captures a call's whole launch sequence (all
  * streams) into a graph and replays it for later calls with the same step

@@ -17,6 +17,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "rn_ops.hpp"
+
 namespace mhip {
 
 constexpr double kDanbyB = 1.0e-13;          // swift.inc:23
@@ -174,27 +176,30 @@ __device__ inline int kepu(double dt, double r0, double mu, double alpha, double
     return iflg;
 }
 
-// drift_kepmd (mercury6_2.for:7754-7808)
+// drift_kepmd (mercury6_2.for:7754-7808).  Ops: IeeeOps, or RangeOps where
+// the caller has bounded the orbit (drift_fast).
+template <class Ops = IeeeOps>
 __device__ __forceinline__ void kepmd(double dm, double es, double ec, double& x, double& s, double& c) {
     const double A0 = 39916800.0, A1 = 6652800.0, A2 = 332640.0, A3 = 7920.0, A4 = 110.0;
-    double fac1 = 1.0 / (1.0 - ec);
+    constexpr double rA0 = 1.0 / 39916800.0, r3 = 1.0 / 3.0;
+    double fac1 = Ops::rcp(1.0 - ec);
     double q = fac1 * dm;
-    double fac2 = es * es * fac1 - ec / 3.0;
+    double fac2 = es * es * fac1 - Ops::div_c(ec, 3.0, r3);
     x = q * (1.0 - 0.5 * fac1 * q * (es - q * fac2));
     double y = x * x;
-    s = x * (A0 - y * (A1 - y * (A2 - y * (A3 - y * (A4 - y))))) / A0;
-    c = sqrt(1.0 - s * s);
+    s = Ops::div_c(x * (A0 - y * (A1 - y * (A2 - y * (A3 - y * (A4 - y))))), A0, rA0);
+    c = Ops::sqrt(1.0 - s * s);
     double f = x - ec * s + es * (1.0 - c) - dm;
     double fp = 1. - ec * c + es * s;
     double fpp = ec * s + es * c;
     double fppp = ec * c - es * s;
-    double dx = -f / fp;
-    dx = -f / (fp + 0.5 * dx * fpp);
-    dx = -f / (fp + 0.5 * dx * fpp + 0.16666666666666666 * dx * dx * fppp);
+    double dx = Ops::div(-f, fp);
+    dx = Ops::div(-f, fp + 0.5 * dx * fpp);
+    dx = Ops::div(-f, fp + 0.5 * dx * fpp + 0.16666666666666666 * dx * dx * fppp);
     x = x + dx;
     y = x * x;
-    s = x * (A0 - y * (A1 - y * (A2 - y * (A3 - y * (A4 - y))))) / A0;
-    c = sqrt(1.0 - s * s);
+    s = Ops::div_c(x * (A0 - y * (A1 - y * (A2 - y * (A3 - y * (A4 - y))))), A0, rA0);
+    c = Ops::sqrt(1.0 - s * s);
 }
 
 struct Orbit { double x, y, z, vx, vy, vz; };
@@ -202,30 +207,43 @@ struct Orbit { double x, y, z, vx, vy, vz; };
 // The kepmd branch of drift_dan (mercury6_2.for:7628-7690).  Returns true and
 // advances the state when drift_dan(dt) takes that branch and passes its
 // fchk test; otherwise returns false with the state untouched.
+//
+// Divides and square roots use RangeOps (rn_ops.hpp).  Their operands stay in
+// its exact range when r0^2 lies in [2^-200, 2^200] and mu, |dt| in
+// [2^-100, 2^100] (the caller's gate): a lies in [r0/2, 2^53 r0] because a
+// positive alpha = 2mu/r0 - v^2 is at least 2^-54 * 2mu/r0, so every
+// quotient below stays within 2^+-600; inside the kepmd branch |ec|, |es| <=
+// 0.6, |dm| <= 0.4.  The one unbounded numerator, u in es = u / (en a^2),
+// keeps the IEEE divide.  A state outside the gate returns false, so the
+// general kernel's IEEE chain takes it.
 __device__ __forceinline__ bool drift_fast(double mu, Orbit& o, double dt) {
-    double r0 = sqrt(o.x * o.x + o.y * o.y + o.z * o.z);
+    using Ops = RangeOps;
+    constexpr double rTwoPi = 1.0 / kTwoPi;
+    double rr = o.x * o.x + o.y * o.y + o.z * o.z;
+    if (!(rr >= 0x1p-200 && rr <= 0x1p200)) return false;
+    double r0 = Ops::sqrt(rr);
     double v0s = o.vx * o.vx + o.vy * o.vy + o.vz * o.vz;
     double u = o.x * o.vx + o.y * o.vy + o.z * o.vz;
-    double alpha = 2.0 * mu / r0 - v0s;
+    double alpha = Ops::div(2.0 * mu, r0) - v0s;
     if (!(alpha > 0.0)) return false;
-    double a = mu / alpha;
+    double a = Ops::div(mu, alpha);
     double asq = a * a;
-    double en = sqrt(mu / (a * asq));
-    double ec = 1.0 - r0 / a;
+    double en = Ops::sqrt(Ops::div(mu, a * asq));
+    double ec = 1.0 - Ops::div(r0, a);
     double es = u / (en * asq);
     double esq = ec * ec + es * es;
-    double dm = dt * en - (double)((int)(dt * en / kTwoPi)) * kTwoPi;
-    double dtr = dm / en;
+    double dm = dt * en - (double)((int)Ops::div_c(dt * en, kTwoPi, rTwoPi)) * kTwoPi;
+    double dtr = Ops::div(dm, en);
     if ((dm * dm > 0.16) || (esq > 0.36) || !(esq * dm * dm < 0.0016)) return false;
     double xkep, s, c;
-    kepmd(dm, es, ec, xkep, s, c);
+    kepmd<Ops>(dm, es, ec, xkep, s, c);
     double fchk = (xkep - ec * s + es * (1.0 - c) - dm);
     if (fchk * fchk > kDanbyB) return false;
     double fp = 1.0 - ec * c + es * s;
-    double f = (a / r0) * (c - 1.0) + 1.0;
-    double g = dtr + (s - xkep) / en;
-    double fdot = -(a / (r0 * fp)) * en * s;
-    double gdot = (c - 1.) / fp + 1.;
+    double f = Ops::div(a, r0) * (c - 1.0) + 1.0;
+    double g = dtr + Ops::div(s - xkep, en);
+    double fdot = -Ops::div(a, r0 * fp) * en * s;
+    double gdot = Ops::div(c - 1., fp) + 1.;
     double nx = o.x * f + o.vx * g, ny = o.y * f + o.vy * g, nz = o.z * f + o.vz * g;
     double nvx = o.x * fdot + o.vx * gdot, nvy = o.y * fdot + o.vy * gdot, nvz = o.z * fdot + o.vz * gdot;
     o.x = nx; o.y = ny; o.z = nz; o.vx = nvx; o.vy = nvy; o.vz = nvz;

@@ -1812,6 +1812,27 @@ int mercury_hip_autotune(double h, int probe_steps, int* spl_out, int* block_out
     }
 }
 
+int mercury_hip_check_arith(int64_t n, uint64_t seed, int64_t* mismatches) {
+    unsigned long long* bad = nullptr;
+    try {
+        if (n < 0) throw std::runtime_error("negative sample count");
+        dalloc(bad, 4);
+        HIPCHECK(hipMemset(bad, 0, 4 * sizeof(unsigned long long)));
+        const long long blocks = std::min<long long>(std::max<long long>((n + 255) / 256, 1), 8192);
+        hipLaunchKernelGGL(check_arith, dim3((unsigned)blocks), dim3(256), 0, 0, (long long)n,
+                           (unsigned long long)seed, bad);
+        HIPCHECK(hipGetLastError());
+        unsigned long long h[4];
+        HIPCHECK(hipMemcpy(h, bad, sizeof(h), hipMemcpyDeviceToHost));
+        dfree(bad);
+        for (int q = 0; q < 4; q++) mismatches[q] = (int64_t)h[q];
+        return 0;
+    } catch (const std::exception& e) {
+        dfree(bad);
+        return fail(e);
+    }
+}
+
 int mercury_hip_work_read(int64_t* fast_steps, int64_t* general_steps) {
     try {
         unsigned long long w[2] = {0, 0};

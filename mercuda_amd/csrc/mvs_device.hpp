@@ -149,10 +149,11 @@ __device__ __forceinline__ int reserve_event(const EventSink& es) {
 // ---------------------------------------------------------------------------
 // mfo_obl for one body (mercury6_2.for:5191-5217): acceleration before the
 // central reaction is subtracted.
+template <class Ops = IeeeOps>
 __device__ __forceinline__ void obl_body(double mu, double j2, double j4, double j6, double x, double y,
                                          double z, double& ax, double& ay, double& az) {
     double r2 = x * x + y * y + z * z;
-    double r_1 = 1.0 / sqrt(r2);
+    double r_1 = Ops::rcp(Ops::sqrt(r2));
     double r_2 = r_1 * r_1;
     double r_3 = r_2 * r_1;
     double jr2 = j2 * r_2;
@@ -176,7 +177,10 @@ __device__ __forceinline__ void obl_body(double mu, double j2, double j4, double
 // and force-block layout).
 // NP >= 0: the planet count is a compile-time constant (the loop unrolls and
 // the table reads batch into wide scalar loads); NP < 0: runtime count np.
-template <int NP = -1>
+// Ops = RangeOps (fast kernel): 1/sqrt(s2) is exact for s2 >= 2^-767; below
+// that s^-3 overflows, so both this and the IEEE form end in the non-finite
+// force fault.
+template <int NP = -1, class Ops = IeeeOps>
 __device__ __forceinline__ void particle_accel(const Sys& d, const double* __restrict__ T, int np,
                                                double x, double y, double z,
                                                double& ax, double& ay, double& az) {
@@ -189,7 +193,7 @@ __device__ __forceinline__ void particle_accel(const Sys& d, const double* __res
         double dy = y - F[kF * p + 1];
         double dz = z - F[kF * p + 2];
         double s2 = dx * dx + dy * dy + dz * dz;
-        double s_1 = 1.0 / sqrt(s2);
+        double s_1 = Ops::rcp(Ops::sqrt(s2));
         double s_3 = s_1 * s_1 * s_1;
         double faci = F[kF * p + 3] * s_3;
         a3x = a3x - faci * dx;
@@ -201,7 +205,7 @@ __device__ __forceinline__ void particle_accel(const Sys& d, const double* __res
     az = T[2] + a3z;
     if (d.obl) {
         double ox, oy, oz;
-        obl_body(d.mu, d.j2, d.j4, d.j6, x, y, z, ox, oy, oz);
+        obl_body<Ops>(d.mu, d.j2, d.j4, d.j6, x, y, z, ox, oy, oz);
         ax = ax + (ox - T[3]);
         ay = ay + (oy - T[4]);
         az = az + (oz - T[5]);
@@ -872,7 +876,7 @@ __device__ __forceinline__ void finish_step_deferred(const Sys& d, const double*
                          o.x, o.y, o.z, o.vx, o.vy, o.vz);
         }
     }
-    particle_accel<NP>(d, T, np, o.x, o.y, o.z, ax, ay, az);
+    particle_accel<NP, RangeOps>(d, T, np, o.x, o.y, o.z, ax, ay, az);
     if (!isfinite(ax) || !isfinite(ay) || !isfinite(az)) atomicOr(fault, FAULT_FORCE);
     o.vx = o.vx + d.hby2 * ax;
     o.vy = o.vy + d.hby2 * ay;
@@ -926,7 +930,7 @@ __global__ void __launch_bounds__(256, MINW) mvs_particles_fast(ParticleArgs arg
         o = Orbit{args.st.x[g], args.st.x[cap + g], args.st.x[2 * cap + g],
                   args.st.v[g], args.st.v[cap + g], args.st.v[2 * cap + g]};
         if (args.reset) {
-            particle_accel<NP>(d, args.tab + d.start, np, o.x, o.y, o.z, ax, ay, az);
+            particle_accel<NP, RangeOps>(d, args.tab + d.start, np, o.x, o.y, o.z, ax, ay, az);
         } else {
             ax = args.st.a[g]; ay = args.st.a[cap + g]; az = args.st.a[2 * cap + g];
         }
@@ -934,6 +938,9 @@ __global__ void __launch_bounds__(256, MINW) mvs_particles_fast(ParticleArgs arg
     }
     const double* tab = args.tab + d.tab;
     const int nsteps = live ? args.steps : 0;
+    // drift_fast's exact range needs mu and |h| in [2^-100, 2^100]; a system
+    // outside it parks every particle at its first step
+    const bool in_range = d.mu >= 0x1p-100 && d.mu <= 0x1p100 && fabs(d.h) >= 0x1p-100 && fabs(d.h) <= 0x1p100;
     int s = 0;
     for (; s < nsteps; s++) {
         const double* T = tab + (long long)s * d.ts;
@@ -943,7 +950,7 @@ __global__ void __launch_bounds__(256, MINW) mvs_particles_fast(ParticleArgs arg
         w.vy = w.vy + d.hby2 * ay;
         w.vz = w.vz + d.hby2 * az;
         Orbit k0 = w;
-        bool ok = drift_fast(d.mu, w, d.h);
+        bool ok = in_range && drift_fast(d.mu, w, d.h);
         // wave-ballot compaction: lanes whose drift leaves the kepmd branch
         // park here and finish in the general kernel
         unsigned long long mask = __ballot(!ok);

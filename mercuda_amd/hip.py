@@ -62,6 +62,7 @@ SIGNATURES = {
     "mercury_hip_timing_read": (C.c_int, [_dp, _ip, _dp, _ip, _dp, _ip]),
     "mercury_hip_parked": (C.c_int64, []),
     "mercury_hip_work_read": (C.c_int, [_i64p, _i64p]),
+    "mercury_hip_check_arith": (C.c_int, [C.c_int64, C.c_uint64, _i64p]),
     "mercury_hip_ensemble_upload": (C.c_int, [C.c_int, _ip, _ip, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
     "mercury_hip_ensemble_advance": (C.c_int, [C.c_double, C.c_int, C.c_int, _i64p]),
     "mercury_hip_ensemble_download": (C.c_int, [_dp, _dp]),
@@ -195,6 +196,13 @@ class MercuryHip:
         f = C.c_int64(); g = C.c_int64()
         self._ok(self.L.mercury_hip_work_read(C.byref(f), C.byref(g)), "work_read")
         return f.value, g.value
+
+    def check_arith(self, n: int, seed: int = 1) -> dict:
+        """Bit mismatches of the fast kernel's divide, reciprocal, square root
+        and constant divide against the IEEE expansions over n random draws."""
+        out = (C.c_int64 * 4)()
+        self._ok(self.L.mercury_hip_check_arith(int(n), int(seed) & (2**64 - 1), out), "check_arith")
+        return dict(div=out[0], rcp=out[1], sqrt=out[2], div_const=out[3])
 
     def parked(self) -> int:
         return int(self.L.mercury_hip_parked())
