@@ -37,9 +37,11 @@ FLOP_PER_PSTEP = 351.6
 BYTES_PER_PARTICLE_LAUNCH = 152.0
 # HBM bytes per particle per launch of the fast kernel as rocprofv3 measured
 # them on this workload (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, the factors
-# a 256 MiB device copy reads on the same counters; 1M particles, 9 launches):
-# profiles/mvs_particles_fast_1M_hbm_traffic.md.
-TRAFFIC_PER_PARTICLE_LAUNCH = 172.2
+# a 256 MiB device copy reads on the same counters; 1M particles, the 68
+# launches of the default timed run): profiles/mvs_particles_fast_1M_hbm_traffic.md.
+# It includes the one write-back per launch of the kernel's 28 B of register
+# spill per lane.
+TRAFFIC_PER_PARTICLE_LAUNCH = 200.3
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 
@@ -114,7 +116,7 @@ def ensemble_ratio(hip, steps, spl, block):
     hip.configure(1)
     hip.ensemble_upload(ens)
     hip.set_tuning(steps_per_launch=spl, block_size=block)
-    tune_e = hip.autotune(probe_steps=32)
+    tune_e = hip.autotune(probe_steps=64)
     hip.set_graph(True)
     t_ens = timed(lambda t, n: hip.ensemble_advance(t, n, screen=True))
     hip.set_graph(False)
@@ -123,7 +125,7 @@ def ensemble_ratio(hip, steps, spl, block):
     hip.configure(1)
     hip.upload_system(one)
     hip.set_tuning(steps_per_launch=spl, block_size=block)
-    tune_1 = hip.autotune(h=one["h"], probe_steps=32)
+    tune_1 = hip.autotune(h=one["h"], probe_steps=64)
     hip.set_graph(True)
     t_one = timed(lambda t, n: hip.advance(t, one["h"], n, one["rcen"], screen=True))
     hip.set_graph(False)
@@ -232,8 +234,8 @@ def hybrid_rate(hip, nparticles=200_000, nsteps=3):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=512)
-    ap.add_argument("--warmup", type=int, default=64)
+    ap.add_argument("--steps", type=int, default=2048)
+    ap.add_argument("--warmup", type=int, default=128)
     ap.add_argument("--particles", type=int, default=1_000_000)
     ap.add_argument("--spl", type=int, default=32, help="steps fused per particle launch")
     ap.add_argument("--block", type=int, default=128)
@@ -241,8 +243,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-procs", type=int, default=16)
     ap.add_argument("--no-extras", action="store_true", help="skip the ensemble/single ratio")
-    ap.add_argument("--no-autotune", action="store_true",
-                    help="use --spl/--block as given instead of the autotuner's choice")
+    ap.add_argument("--autotune", action="store_true",
+                    help="let the autotuner pick steps per launch and workgroup size instead of --spl/--block "
+                         "(the defaults are its usual choice on MI355X)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the massless-particle path has no multi-GPU mode in this build "
@@ -264,9 +267,9 @@ def main():
     screen = not args.no_screen
     h = sysd["h"]
     spl, block = args.spl, args.block
-    if not args.no_autotune:
-        log("autotuning steps per launch and workgroup size (32-step probes, state restored)")
-        tune = hip.autotune(h=h, probe_steps=32)
+    if args.autotune:
+        log("autotuning steps per launch and workgroup size (128-step probes, state restored)")
+        tune = hip.autotune(h=h, probe_steps=128)
         spl, block = tune["steps_per_launch"], tune["block"]
         log(f"autotuner: {tune}")
     log(f"warmup {args.warmup} steps")
@@ -348,7 +351,7 @@ def main():
                                f"encounter screening {'on' if screen else 'off'}",
                    "particles": npart, "planets": 8, "dt_days": h,
                    "steps_per_launch": spl, "block": block,
-                   "tuning": "autotuned" if not args.no_autotune else "fixed", "parallelism": "1 GPU"},
+                   "tuning": "autotuned" if args.autotune else "fixed", "parallelism": "1 GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "extras": extras,

@@ -63,6 +63,8 @@ struct Timing {
     void clear() { marks.clear(); used = 0; }
 };
 
+constexpr int kRing = 8;   // launches in flight: table, counter and request slots
+
 struct Context {
     int algorithm = 1;
     int n = 0, nbig = 0, pn = 0, ngflag = 0;
@@ -93,7 +95,7 @@ struct Context {
     int req_cap = 0;
     long long slot_doubles = 0;      // table doubles per ring slot
     hipStream_t sp = nullptr, sg = nullptr;   // planet and general-kernel streams
-    hipEvent_t ev_plan[8] = {}, ev_fast[8] = {}, ev_done[8] = {};
+    hipEvent_t ev_plan[kRing + 1] = {}, ev_fast[kRing + 1] = {}, ev_done[kRing + 1] = {};
     // host
     std::vector<double> stage;
     std::vector<DevEvent> hev;
@@ -147,7 +149,6 @@ struct Context {
     } tape;
 };
 
-constexpr int kRing = 4;   // launches in flight: table, counter and request slots
 Context ctx;
 
 // What a captured advance depends on: replay only when all of it is unchanged.
@@ -365,30 +366,24 @@ void build_systems(int smax, bool use_call, double h_call, double rcen_call) {
     c.sys_valid = true;
 }
 
-template <bool SCREEN, int NP>
-bool launch_fast_np(int np, int occ, dim3 grid, dim3 block, hipStream_t st, const ParticleArgs& args) {
-    if constexpr (NP > 12) {
-        return false;
+// The fast kernel specialises on each workgroup's planet count itself (see
+// mvs_particles_fast); np is not needed here.
+void launch_fast(bool screen, bool save, int, dim3 grid, dim3 block, hipStream_t st, const ParticleArgs& args) {
+    // waves per SIMD the fast kernel's registers are capped for: 5 measured
+    // best on MI355X (a few spills, more latency hidden); 1 leaves the
+    // allocation to the compiler
+    static const int occ = std::getenv("MERCURY_HIP_FAST_OCC") ? std::atoi(std::getenv("MERCURY_HIP_FAST_OCC")) : 5;
+    const size_t shm = screen ? sizeof(double) * kEnds * block.x : 0;
+    if (occ == 1) {
+        if (screen && save) hipLaunchKernelGGL((mvs_particles_fast<true, true, 1>), grid, block, shm, st, args);
+        else if (screen) hipLaunchKernelGGL((mvs_particles_fast<true, false, 1>), grid, block, shm, st, args);
+        else if (save) hipLaunchKernelGGL((mvs_particles_fast<false, true, 1>), grid, block, shm, st, args);
+        else hipLaunchKernelGGL((mvs_particles_fast<false, false, 1>), grid, block, shm, st, args);
     } else {
-        if (np != NP) return launch_fast_np<SCREEN, NP + 1>(np, occ, grid, block, st, args);
-        if (occ >= 6) hipLaunchKernelGGL((mvs_particles_fast<SCREEN, false, NP, 6>), grid, block, 0, st, args);
-        else if (occ == 5) hipLaunchKernelGGL((mvs_particles_fast<SCREEN, false, NP, 5>), grid, block, 0, st, args);
-        else hipLaunchKernelGGL((mvs_particles_fast<SCREEN, false, NP, 1>), grid, block, 0, st, args);
-        return true;
-    }
-}
-
-void launch_fast(bool screen, bool save, int np, dim3 grid, dim3 block, hipStream_t st, const ParticleArgs& args) {
-    static const bool generic_only = std::getenv("MERCURY_HIP_GENERIC_FAST") != nullptr;
-    static const int occ = std::getenv("MERCURY_HIP_FAST_OCC") ? std::atoi(std::getenv("MERCURY_HIP_FAST_OCC")) : 1;
-    bool done = false;
-    if (!save && !generic_only) done = screen ? launch_fast_np<true, 0>(np, occ, grid, block, st, args)
-                                              : launch_fast_np<false, 0>(np, occ, grid, block, st, args);
-    if (!done) {
-        if (screen && save) hipLaunchKernelGGL((mvs_particles_fast<true, true>), grid, block, 0, st, args);
-        else if (screen) hipLaunchKernelGGL((mvs_particles_fast<true, false>), grid, block, 0, st, args);
-        else if (save) hipLaunchKernelGGL((mvs_particles_fast<false, true>), grid, block, 0, st, args);
-        else hipLaunchKernelGGL((mvs_particles_fast<false, false>), grid, block, 0, st, args);
+        if (screen && save) hipLaunchKernelGGL((mvs_particles_fast<true, true, 5>), grid, block, shm, st, args);
+        else if (screen) hipLaunchKernelGGL((mvs_particles_fast<true, false, 5>), grid, block, shm, st, args);
+        else if (save) hipLaunchKernelGGL((mvs_particles_fast<false, true, 5>), grid, block, shm, st, args);
+        else hipLaunchKernelGGL((mvs_particles_fast<false, false, 5>), grid, block, shm, st, args);
     }
     HIPCHECK(hipGetLastError());
 }

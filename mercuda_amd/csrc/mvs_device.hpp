@@ -563,7 +563,10 @@ __device__ inline void big_forces(const Sys& d, int np, const double* m, const d
     __syncthreads();
 }
 
+// One wave per system, latency-bound; raised issue priority keeps it ahead of
+// the particle waves it shares a SIMD with.
 __global__ void __launch_bounds__(64) mvs_planets(PlanetArgs args) {
+    __builtin_amdgcn_s_setprio(3);
     const int sid = blockIdx.x;
     const Sys d = args.sys[sid];
     const int np = d.nbig - 1;
@@ -853,18 +856,24 @@ __device__ __forceinline__ void save_step(const State& st, int g, const double x
 // mvs_acceleration's check does in the reference, mercury_cuda_symplectic.cuh:78).
 // As finish_step, with the screening details deferred to screen_requests:
 // inline are only the x-y box tests of mce_stat and the gate of mce_cent.
+//
+// The states a request records stay out of registers: E (the lane's slots in
+// LDS, stride `ls`) holds the state after the first kick (E[0..5]) and the
+// step-start velocities (E[6..8]; the step-start position is E[0..2]); the
+// central gate needs only rr0 = |x|^2 and rv0 = v.x of the step start.
+constexpr int kEnds = 9;       // doubles per lane in the fast kernel's LDS
+
 template <bool SCREEN, int NP>
 __device__ __forceinline__ void finish_step_deferred(const Sys& d, const double* __restrict__ T, int np, int g,
-                                                     int jb, double rce_j, int s, int sid, const ReqSink& rs,
-                                                     int* fault, const double xs[6], const Orbit& k0, Orbit& o,
-                                                     double& ax, double& ay, double& az) {
+                                                     int jb, double pad, int s, int sid, const ReqSink& rs,
+                                                     int* fault, double rr0, double rv0, const double* E, int ls,
+                                                     Orbit& o, double& ax, double& ay, double& az) {
     if (SCREEN) {
         double xmin, xmax, ymin, ymax;
-        box_axis(d.h, k0.x, k0.vx, o.x, o.vx, xmin, xmax);
-        box_axis(d.h, k0.y, k0.vy, o.y, o.vy, ymin, ymax);
-        double temp = rce_j * 1.2;
-        xmin = xmin - temp; xmax = xmax + temp;
-        ymin = ymin - temp; ymax = ymax + temp;
+        box_axis(d.h, E[0], E[3 * ls], o.x, o.vx, xmin, xmax);
+        box_axis(d.h, E[ls], E[4 * ls], o.y, o.vy, ymin, ymax);
+        xmin = xmin - pad; xmax = xmax + pad;
+        ymin = ymin - pad; ymax = ymax + pad;
         const int npl = NP >= 0 ? NP : np;
         const double* Sc = T + kHdr + kF * npl;
         // particles (jb >= nbig) test every planet; this kernel never sees Big bodies
@@ -872,7 +881,7 @@ __device__ __forceinline__ void finish_step_deferred(const Sys& d, const double*
         for (int p = 0; p < npl; p++) {
             const double* B = Sc + kScr * p;
             if (B[SX1] >= xmin && xmax >= B[SX0] && B[SY1] >= ymin && ymax >= B[SY0])
-                push_req(rs, 0, g, jb, s, p, sid, k0.x, k0.y, k0.z, k0.vx, k0.vy, k0.vz,
+                push_req(rs, 0, g, jb, s, p, sid, E[0], E[ls], E[2 * ls], E[3 * ls], E[4 * ls], E[5 * ls],
                          o.x, o.y, o.z, o.vx, o.vy, o.vz);
         }
     }
@@ -883,12 +892,10 @@ __device__ __forceinline__ void finish_step_deferred(const Sys& d, const double*
     o.vz = o.vz + d.hby2 * az;
     if (SCREEN) {
         double rcen2 = d.rcen * d.rcen;
-        double rr0 = xs[0] * xs[0] + xs[1] * xs[1] + xs[2] * xs[2];
         double rr1 = o.x * o.x + o.y * o.y + o.z * o.z;
-        double rv0 = xs[3] * xs[0] + xs[4] * xs[1] + xs[5] * xs[2];
         double rv1 = o.vx * o.x + o.vy * o.y + o.vz * o.z;
         if ((rv0 * d.h <= 0.0 && rv1 * d.h >= 0.0) || fmin(rr0, rr1) <= rcen2)
-            push_req(rs, 1, g, jb, s, 0, sid, xs[0], xs[1], xs[2], xs[3], xs[4], xs[5],
+            push_req(rs, 1, g, jb, s, 0, sid, E[0], E[ls], E[2 * ls], E[6 * ls], E[7 * ls], E[8 * ls],
                      o.x, o.y, o.z, o.vx, o.vy, o.vz);
     }
 }
@@ -912,8 +919,11 @@ __device__ __forceinline__ void finish_step(const Sys& d, const double* __restri
                       o.vx, o.vy, o.vz, jb, s, sid, es);
 }
 
-template <bool SCREEN, bool SAVE, int NP = -1, int MINW = 1>
-__global__ void __launch_bounds__(256, MINW) mvs_particles_fast(ParticleArgs args) {
+// One workgroup's launch: S steps of its tile.  NP >= 0: the planet count as a
+// compile-time constant (the force and box loops unroll and the table reads
+// batch into wide scalar loads); NP < 0: read at run time.
+template <bool SCREEN, bool SAVE, int NP>
+__device__ __forceinline__ void fast_tile(const ParticleArgs& args, double* lds_ends) {
     const int2 tile = args.tiles[blockIdx.x];
     const int sid = tile.x;
     const Sys d = args.sys[sid];
@@ -936,6 +946,9 @@ __global__ void __launch_bounds__(256, MINW) mvs_particles_fast(ParticleArgs arg
         }
         if (SCREEN) rce_j = args.st.rce[g];
     }
+    const double pad = rce_j * 1.2;                 // mce_stat's box padding
+    const int ls = blockDim.x;
+    double* E = lds_ends + threadIdx.x;
     const double* tab = args.tab + d.tab;
     const int nsteps = live ? args.steps : 0;
     // drift_fast's exact range needs mu and |h| in [2^-100, 2^100]; a system
@@ -950,6 +963,14 @@ __global__ void __launch_bounds__(256, MINW) mvs_particles_fast(ParticleArgs arg
         w.vy = w.vy + d.hby2 * ay;
         w.vz = w.vz + d.hby2 * az;
         Orbit k0 = w;
+        double rr0 = 0.0, rv0 = 0.0;
+        if (SCREEN) {
+            rr0 = o.x * o.x + o.y * o.y + o.z * o.z;
+            rv0 = o.vx * o.x + o.vy * o.y + o.vz * o.z;
+            E[0] = w.x; E[ls] = w.y; E[2 * ls] = w.z;
+            E[3 * ls] = w.vx; E[4 * ls] = w.vy; E[5 * ls] = w.vz;
+            E[6 * ls] = o.vx; E[7 * ls] = o.vy; E[8 * ls] = o.vz;
+        }
         bool ok = in_range && drift_fast(d.mu, w, d.h);
         // wave-ballot compaction: lanes whose drift leaves the kepmd branch
         // park here and finish in the general kernel
@@ -967,8 +988,8 @@ __global__ void __launch_bounds__(256, MINW) mvs_particles_fast(ParticleArgs arg
         }
         o = w;
         if (SAVE) save_step(args.st, g, xs, k0, o);
-        finish_step_deferred<SCREEN, NP>(d, T, np, g, jb, rce_j, s, sid, args.req, args.st.fault, xs, k0, o,
-                                     ax, ay, az);
+        finish_step_deferred<SCREEN, NP>(d, T, np, g, jb, pad, s, sid, args.req, args.st.fault, rr0, rv0, E, ls,
+                                         o, ax, ay, az);
     }
     if (args.work) count_work(args.work, s);
     if (!live) return;
@@ -978,14 +999,49 @@ __global__ void __launch_bounds__(256, MINW) mvs_particles_fast(ParticleArgs arg
     args.st.a[g] = ax; args.st.a[cap + g] = ay; args.st.a[2 * cap + g] = az;
 }
 
+// The fast particle kernel.  A tile never spans two systems, so each
+// workgroup branches once, on its system's planet count, into the body built
+// for that count: single systems and ensembles of mixed planet counts run the
+// same specialised code.  SAVE (single-step mode) keeps the run-time count.
+// MINW: waves per SIMD the register allocation is capped for.
+// SCREEN: launched with kEnds * blockDim.x doubles of dynamic LDS.
+template <bool SCREEN, bool SAVE, int MINW>
+__global__ void __launch_bounds__(256, MINW) mvs_particles_fast(ParticleArgs args) {
+    extern __shared__ double lds_ends[];
+    if (SAVE) {
+        fast_tile<SCREEN, SAVE, -1>(args, lds_ends);
+        return;
+    }
+    switch (args.sys[args.tiles[blockIdx.x].x].nbig - 1) {
+        case 0: fast_tile<SCREEN, SAVE, 0>(args, lds_ends); break;
+        case 1: fast_tile<SCREEN, SAVE, 1>(args, lds_ends); break;
+        case 2: fast_tile<SCREEN, SAVE, 2>(args, lds_ends); break;
+        case 3: fast_tile<SCREEN, SAVE, 3>(args, lds_ends); break;
+        case 4: fast_tile<SCREEN, SAVE, 4>(args, lds_ends); break;
+        case 5: fast_tile<SCREEN, SAVE, 5>(args, lds_ends); break;
+        case 6: fast_tile<SCREEN, SAVE, 6>(args, lds_ends); break;
+        case 7: fast_tile<SCREEN, SAVE, 7>(args, lds_ends); break;
+        case 8: fast_tile<SCREEN, SAVE, 8>(args, lds_ends); break;
+        case 9: fast_tile<SCREEN, SAVE, 9>(args, lds_ends); break;
+        case 10: fast_tile<SCREEN, SAVE, 10>(args, lds_ends); break;
+        case 11: fast_tile<SCREEN, SAVE, 11>(args, lds_ends); break;
+        case 12: fast_tile<SCREEN, SAVE, 12>(args, lds_ends); break;
+        default: fast_tile<SCREEN, SAVE, -1>(args, lds_ends); break;
+    }
+}
+
 // Parked lanes: resume each from its recorded step with the whole drift chain.
 // Lanes of one wave may start at different steps and belong to different
 // systems, so system and table reads here are per lane.
 // UNIFORM: every entry belongs to system 0, so the wave walks the steps in
 // lockstep from the earliest start among its lanes and reads each step's table
 // with wave-uniform (scalar) loads; lanes join at their own start step.
+//
+// These few latency-bound waves share SIMDs with the fast kernel's; raised
+// issue priority keeps them from trailing the fast launches they overlap.
 template <bool SCREEN, bool SAVE, bool UNIFORM>
 __global__ void __launch_bounds__(64) mvs_particles_general(ParticleArgs args) {
+    __builtin_amdgcn_s_setprio(3);
     const int count = min(*args.park.snap, args.park.cap);
     const long long cap = args.st.cap;
     const int per_wave = args.park.per_wave;
