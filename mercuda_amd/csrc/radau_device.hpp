@@ -51,8 +51,11 @@ struct RaArgs {
 
 // Heliocentric acceleration of a massless body (mfo_grav for j > nbig, plus
 // oblateness): direct terms from the Big bodies in order, then the indirect
-// and central terms, then 1PN (pn) and oblateness (d.obl).
-template <bool PN>
+// and central terms, then 1PN (pn) and oblateness (d.obl).  Ops: the particle
+// kernel's RangeOps give the IEEE bits for every squared distance sq_ok
+// admits and raise the force fault for the rest, where s^-3 (or s^2) has
+// overflowed in the IEEE form as well.
+template <bool PN, class Ops = IeeeOps>
 __device__ __forceinline__ void grav_small(const Sys& d, const double* __restrict__ T, int np, double light2,
                                            double x, double y, double z, double vx, double vy, double vz,
                                            double& ax, double& ay, double& az, int* fault) {
@@ -61,8 +64,8 @@ __device__ __forceinline__ void grav_small(const Sys& d, const double* __restric
     for (int p = 0; p < np; p++) {
         double dx = x - F[kF * p], dy = y - F[kF * p + 1], dz = z - F[kF * p + 2];
         double s2 = dx * dx + dy * dy + dz * dz;
-        if (!(s2 > 0.0)) atomicOr(fault, FAULT_FORCE);
-        double s_1 = 1.0 / sqrt(s2);
+        if (!Ops::sq_ok(s2)) atomicOr(fault, FAULT_FORCE);
+        double s_1 = Ops::rcp(Ops::sqrt(s2));
         double s_3 = s_1 * s_1 * s_1;
         double tmp1 = s_3 * F[kF * p + 3];
         ax = ax - tmp1 * dx;
@@ -70,8 +73,8 @@ __device__ __forceinline__ void grav_small(const Sys& d, const double* __restric
         az = az - tmp1 * dz;
     }
     double s2 = x * x + y * y + z * z;
-    if (!(s2 > 0.0)) atomicOr(fault, FAULT_FORCE);
-    double s_1 = 1.0 / sqrt(s2);
+    if (!Ops::sq_ok(s2)) atomicOr(fault, FAULT_FORCE);
+    double s_1 = Ops::rcp(Ops::sqrt(s2));
     double r3 = s_1 * s_1 * s_1;
     double tmp1 = d.mu * r3;
     if (!PN) {
@@ -89,7 +92,7 @@ __device__ __forceinline__ void grav_small(const Sys& d, const double* __restric
     }
     if (d.obl) {
         double ox, oy, oz;
-        obl_body(d.mu, d.j2, d.j4, d.j6, x, y, z, ox, oy, oz);
+        obl_body<Ops>(d.mu, d.j2, d.j4, d.j6, x, y, z, ox, oy, oz);
         ax = ax + (ox - T[3]);
         ay = ay + (oy - T[4]);
         az = az + (oz - T[5]);
@@ -372,6 +375,8 @@ __global__ void __launch_bounds__(64) ra_planets(RaArgs args) {
     if (iter) merge_bmax(args.bmax, bm, args.st.fault);
 }
 
+// Uncapped registers (151-153 VGPRs, 3 waves per SIMD): caps for 4 or 5 waves
+// spill b and g and measured 1.2x and 2.4x slower.
 template <bool PN>
 __global__ void __launch_bounds__(128) ra_particles(RaArgs args) {
     const Sys d = args.sys[0];
@@ -394,8 +399,8 @@ __global__ void __launch_bounds__(128) ra_particles(RaArgs args) {
                     if (args.reset) args.ra.e[(c * 7 + q) * cap + g_] = 0.0;
                 }
         }
-        grav_small<PN>(d, args.tab, np, args.light2, x1[0], x1[1], x1[2], v1[0], v1[1], v1[2], a1[0], a1[1], a1[2],
-                       args.st.fault);
+        grav_small<PN, RangeOps>(d, args.tab, np, args.light2, x1[0], x1[1], x1[2], v1[0], v1[1], v1[2], a1[0],
+                                 a1[1], a1[2], args.st.fault);
         if (iter)
             for (int c = 0; c < 3; c++) ra_g_from_b(args.k, b[c], g[c]);
         int entry = 1;
@@ -411,8 +416,8 @@ __global__ void __launch_bounds__(128) ra_particles(RaArgs args) {
                     for (int c = 0; c < 3; c++) v[c] = ra_vpred(s, b[c], a1[c], v1[c]);
                 }
                 double a[3];
-                grav_small<PN>(d, args.tab + (long long)entry * d.ts, np, args.light2, x[0], x[1], x[2], v[0], v[1],
-                               v[2], a[0], a[1], a[2], args.st.fault);
+                grav_small<PN, RangeOps>(d, args.tab + (long long)entry * d.ts, np, args.light2, x[0], x[1], x[2],
+                                         v[0], v[1], v[2], a[0], a[1], a[2], args.st.fault);
                 for (int c = 0; c < 3; c++) ra_update(args.k, j, a[c], a1[c], g[c], b[c]);
             }
         }

@@ -72,7 +72,10 @@ __device__ __forceinline__ double divc_rn(double a, double b, double y) {
 // Operation sets for code shared by the fast kernel (RangeOps, operands
 // bounded as above) and every other path (IeeeOps, the compiler's IEEE
 // expansions).  div_c(a, b, y) divides by the constant b with y = RN(1/b).
+// sq_ok(s2): a squared distance whose 1/sqrt stays exact; outside it (RangeOps)
+// s^-3 has overflowed or s^2 itself has, as it would in the IEEE form too.
 struct IeeeOps {
+    static __device__ __forceinline__ bool sq_ok(double s2) { return s2 > 0.0; }
     static __device__ __forceinline__ double div(double a, double b) { return a / b; }
     static __device__ __forceinline__ double rcp(double b) { return 1.0 / b; }
     static __device__ __forceinline__ double sqrt(double x) { return ::sqrt(x); }
@@ -80,6 +83,7 @@ struct IeeeOps {
 };
 
 struct RangeOps {
+    static __device__ __forceinline__ bool sq_ok(double s2) { return s2 > 0x1p-767 && s2 < 0x1p1000; }
     static __device__ __forceinline__ double div(double a, double b) { return div_rn(a, b); }
     static __device__ __forceinline__ double rcp(double b) { return rcp_rn(b); }
     static __device__ __forceinline__ double sqrt(double x) { return sqrt_rn(x); }
