@@ -130,6 +130,16 @@ int64_t mercury_hip_parked(void);
  * since timing was last enabled. */
 int mercury_hip_work_read(int64_t *fast_steps, int64_t *general_steps);
 
+/* The driver's encounter loop mdt_hkce (mercury6_2.for:3883-3905) over one step
+ * h0 in the active encounter context, with mdt_bs2's step control on the device
+ * (no counterpart in the reference, whose driver calls mercury_hip_step once per
+ * substep): for callers that do not list encounters after every substep.  hrec:
+ * in, the BS step recommendation kept from the last encounter; out, the new
+ * one.  Returns the substeps taken, force evaluations and rejected attempts;
+ * the states are those of the per-substep calls, bit for bit. */
+int mercury_hip_encounter_advance(double h0, double *hrec, double tol, int64_t *substeps,
+                                  int64_t *forces, int64_t *rejected);
+
 /* Device self-check of the arithmetic (no counterpart in the reference): the
  * fast MVS kernel divides and takes square roots with shorter sequences that
  * are exact for operands inside the normal range.  This draws n random

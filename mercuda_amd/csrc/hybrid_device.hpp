@@ -385,6 +385,21 @@ __global__ void __launch_bounds__(128) hy_particles_finish(HyArgs args) {
 constexpr int kEncBig = 16;     // Big members of one encounter set
 constexpr int kBsPoints = 78;   // force points of the 12 modified-midpoint levels
 
+// Step control of the driver's mdt_hkce loop (mercury6_2.for:3883-3905) with
+// mdt_bs2's step rule (mercury6_2.for:3428-3617) on the device, for callers
+// that do not list encounters after every substep: bs2_big and bs2_particles read their
+// step from here, bs2_control decides, bs2_accept applies the chosen level.
+struct BsCtl {
+    double h0;                  // the step the encounter set is integrated over
+    double tlocal;              // time done so far
+    double hlocal;              // step of the next attempt
+    double hrec;                // mdt_hkce's recommendation for the next encounter
+    double tol2;
+    long long forces, rejected, substeps;
+    int state;                  // 0 running, 1 done, 2 step below the time's resolution, 3 non-finite
+    int level;                  // level the last control accepted (0: none)
+};
+
 struct BsArgs {
     int n, nbig;                // members (index 0: the central body), Big members 1..nbig-1
     const double* m;            // [n]
@@ -397,10 +412,11 @@ struct BsArgs {
     const int* poff;            // particle partners: CSR offsets [n - nbig + 1]
     const int* pidx;            // partner Big members (compact index)
     double h0, tol2;
-    double* dbuf;               // per particle d table [72][ncap]
+    double* tp;                 // Big positions at every force point [kBsPoints][3 * kEncBig] (bs2_big)
     double* cand;               // accepted states per level n = 4..12: [9][6][ncap]
     unsigned long long* err;    // max error per level [12] (bits of non-negative doubles)
     int* fault;
+    BsCtl* ctl;                 // device step control (nullptr: h0 as given, the host decides)
 };
 
 // mfo_hkce pair factor (1 - K) / s^3 inside rc, else 0 (mercury6_2.for, mfo_hkce).
@@ -429,96 +445,31 @@ __device__ __forceinline__ void merge_err(unsigned long long* slot, double e) {
     if (__lane_id() == 0) atomicMax(slot, bits);
 }
 
-// Big members' forces at positions P (LDS, [3][kEncBig]), lane k = member k+1.
-__device__ inline void bs_big_force(const BsArgs& a, int nb, const double* P, double* F) {
-    const int k = threadIdx.x;
-    if (k < nb) {
-        double acc[3] = {0.0, 0.0, 0.0};
-        const int me = k + 1;
-        for (int q = 0; q < a.nbbp; q++) {
-            int i = a.bbp[2 * q], j = a.bbp[2 * q + 1];
-            if (i != me && j != me) continue;
-            double dx = P[3 * (j - 1)] - P[3 * (i - 1)];
-            double dy = P[3 * (j - 1) + 1] - P[3 * (i - 1) + 1];
-            double dz = P[3 * (j - 1) + 2] - P[3 * (i - 1) + 2];
-            double tmp2;
-            if (!hkce_factor(dx * dx + dy * dy + dz * dz, fmax(a.rcrit[i], a.rcrit[j]), tmp2)) continue;
-            if (j == me) {
-                double faci = tmp2 * a.m[i];
-                acc[0] = acc[0] - faci * dx; acc[1] = acc[1] - faci * dy; acc[2] = acc[2] - faci * dz;
-            } else {
-                double facj = tmp2 * a.m[j];
-                acc[0] = acc[0] + facj * dx; acc[1] = acc[1] + facj * dy; acc[2] = acc[2] + facj * dz;
-            }
-        }
-        const double* x = P + 3 * k;
-        double s2 = x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
-        double s_1 = 1.0 / sqrt(s2);
-        double tmp2 = a.m[0] * s_1 * s_1 * s_1;
-        for (int c = 0; c < 3; c++) F[3 * k + c] = acc[c] - tmp2 * x[c];
-    }
-}
-
-// One member's 12-level modified-midpoint sequence with polynomial
-// extrapolation (mdt_bs2 mercury6_2.for:3470-3570): per level n > 3 its
-// error and accepted state.  force(point, xend, a) gives the acceleration at
-// force point `point`; D is this member's d table (6 x 12, strided).
-template <class Force>
-__device__ inline void bs_member(const BsArgs& A, double h0, const double* x0, const double* v0, const double* a0,
-                                 Force force, double* D, long long ds, double xscal, double vscal, double* errs,
-                                 double* cands, long long cs) {
-    double h2[12];
-    int point = 0;
-    for (int n = 1; n <= 12; n++) {
-        double h = h0 / (double)n;
-        double hby2 = .5 * h;
-        h2[n - 1] = h * h;
-        double h2by2 = .5 * h2[n - 1];
-        double b[3], c[3], xend[3], a[3];
-        for (int q = 0; q < 3; q++) {
-            b[q] = .5 * a0[q];
-            c[q] = 0.0;
-            xend[q] = h2by2 * a0[q] + h * v0[q] + x0[q];
-        }
-        for (int j = 2; j <= n; j++) {
-            force(point++, xend, a);
-            double tmp0 = h * (double)j;
-            for (int q = 0; q < 3; q++) {
-                b[q] = b[q] + a[q];
-                c[q] = c[q] + b[q];
-                xend[q] = h2[n - 1] * c[q] + h2by2 * a0[q] + tmp0 * v0[q] + x0[q];
-            }
-        }
-        force(point++, xend, a);
-        for (int q = 0; q < 3; q++) {
-            D[(long long)(6 * (n - 1) + q) * ds] = xend[q];
-            D[(long long)(6 * (n - 1) + 3 + q) * ds] = h * b[q] + hby2 * a[q] + v0[q];
-        }
-        for (int j = n - 1; j >= 1; j--) {
-            int j1 = j + 1;
-            double tmp0 = 1.0 / (h2[j - 1] - h2[n - 1]);
-            double tmp1 = tmp0 * h2[j1 - 1];
-            double tmp2 = tmp0 * h2[n - 1];
-            for (int q = 0; q < 6; q++)
-                D[(long long)(6 * (j - 1) + q) * ds] = tmp1 * D[(long long)(6 * (j1 - 1) + q) * ds]
-                                                     - tmp2 * D[(long long)(6 * (j - 1) + q) * ds];
-        }
-        if (n > 3) {
-            double d1[6];
-            for (int q = 0; q < 6; q++) d1[q] = D[(long long)q * ds];
-            bool fin = true;
-            for (int q = 0; q < 6; q++) fin = fin && isfinite(d1[q]);
-            double tmp1 = fmax(fmax(d1[0] * d1[0], d1[1] * d1[1]), d1[2] * d1[2]);
-            double tmp2 = fmax(fmax(d1[3] * d1[3], d1[4] * d1[4]), d1[5] * d1[5]);
-            if (!fin) atomicOr(A.fault, FAULT_FORCE);
-            errs[n - 4] = fin ? fmax(tmp1 * xscal, tmp2 * vscal) : 0.0;
-            for (int q = 0; q < 6; q++) {
-                double s = d1[q];
-                for (int j = 2; j <= n; j++) s = s + D[(long long)(6 * (j - 1) + q) * ds];
-                cands[(long long)((n - 4) * 6 + q) * cs] = s;
-            }
+// Force on Big member me (1..nb) from the Big positions P ([3][kEncBig]):
+// mfo_hkce's Big-Big pairs in list order, then the central term.
+__device__ inline void big_accel(const BsArgs& a, int me, const double* P, double* acc) {
+    acc[0] = 0.0; acc[1] = 0.0; acc[2] = 0.0;
+    for (int q = 0; q < a.nbbp; q++) {
+        int i = a.bbp[2 * q], j = a.bbp[2 * q + 1];
+        if (i != me && j != me) continue;
+        double dx = P[3 * (j - 1)] - P[3 * (i - 1)];
+        double dy = P[3 * (j - 1) + 1] - P[3 * (i - 1) + 1];
+        double dz = P[3 * (j - 1) + 2] - P[3 * (i - 1) + 2];
+        double tmp2;
+        if (!hkce_factor(dx * dx + dy * dy + dz * dz, fmax(a.rcrit[i], a.rcrit[j]), tmp2)) continue;
+        if (j == me) {
+            double faci = tmp2 * a.m[i];
+            acc[0] = acc[0] - faci * dx; acc[1] = acc[1] - faci * dy; acc[2] = acc[2] - faci * dz;
+        } else {
+            double facj = tmp2 * a.m[j];
+            acc[0] = acc[0] + facj * dx; acc[1] = acc[1] + facj * dy; acc[2] = acc[2] + facj * dz;
         }
     }
+    const double* x = P + 3 * (me - 1);
+    double s2 = x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
+    double s_1 = 1.0 / sqrt(s2);
+    double tmp2 = a.m[0] * s_1 * s_1 * s_1;
+    for (int c = 0; c < 3; c++) acc[c] = acc[c] - tmp2 * x[c];
 }
 
 __device__ __forceinline__ void bs_scales(const double* x0, const double* v0, double& xs, double& vs) {
@@ -528,68 +479,193 @@ __device__ __forceinline__ void bs_scales(const double* x0, const double* v0, do
     vs = 1.0 / fmax(t2, 1e-300);
 }
 
-// One BS2 attempt of the whole encounter set.  Every workgroup first runs the
-// Big members' sequence (one lane each, identical in every workgroup) into an
-// LDS table of their positions at the 78 force points; then each lane runs one
-// particle member's sequence against that table.
-__global__ void __launch_bounds__(64) bs2_attempt(BsArgs A) {
+// mdt_bs2's 12 modified-midpoint levels (mercury6_2.for:3470-3530) are
+// independent until the extrapolation, so each level runs in its own thread;
+// level n makes its n force evaluations at the force points n(n-1)/2 ..
+// n(n+1)/2 - 1 of the sequential order.  bs_level_* follow one level of one
+// member: start, then per evaluation the update, then the level's result.
+constexpr int kLevels = 12;
+__host__ __device__ constexpr int bs_point(int n, int r) { return n * (n - 1) / 2 + r - 1; }   // r = 1..n
+
+struct BsLevel {
+    double h, hby2, h2, h2by2, tmp0;
+    double b[3], c[3], xend[3];
+};
+
+__device__ __forceinline__ void bs_level_start(BsLevel& L, int n, double h0, const double* x0, const double* v0,
+                                               const double* a0) {
+    L.h = h0 / (double)n;
+    L.hby2 = .5 * L.h;
+    L.h2 = L.h * L.h;
+    L.h2by2 = .5 * L.h2;
+    for (int q = 0; q < 3; q++) {
+        L.b[q] = .5 * a0[q];
+        L.c[q] = 0.0;
+        L.xend[q] = L.h2by2 * a0[q] + L.h * v0[q] + x0[q];
+    }
+}
+
+// After the force a at evaluation r of n (r < n): advance xend to the next point.
+__device__ __forceinline__ void bs_level_step(BsLevel& L, int r, const double* a, const double* x0,
+                                              const double* v0, const double* a0) {
+    const int j = r + 1;
+    double tmp0 = L.h * (double)j;
+    for (int q = 0; q < 3; q++) {
+        L.b[q] = L.b[q] + a[q];
+        L.c[q] = L.c[q] + L.b[q];
+        L.xend[q] = L.h2 * L.c[q] + L.h2by2 * a0[q] + tmp0 * v0[q] + x0[q];
+    }
+}
+
+// The level's result from the force a at its last evaluation: D row n-1.
+__device__ __forceinline__ void bs_level_result(const BsLevel& L, const double* a, const double* v0, double* R) {
+    for (int q = 0; q < 3; q++) {
+        R[q] = L.xend[q];
+        R[3 + q] = L.h * L.b[q] + L.hby2 * a[q] + v0[q];
+    }
+}
+
+// Extrapolation of one component q of one member over the levels in
+// mdt_bs2's order (mercury6_2.for:3531-3570), given every level's result
+// already in D (row n-1 holds level n's: no earlier level writes it).  Per
+// level n > 3 it keeps d(1) in E[n-4] and writes the accepted value.
+__device__ inline void bs_extrapolate(double h0, double* D, int ds, double* E, double* cand, long long cs) {
+    double h2[kLevels];
+    for (int n = 1; n <= kLevels; n++) {
+        double h = h0 / (double)n;
+        h2[n - 1] = h * h;
+    }
+    for (int n = 1; n <= kLevels; n++) {
+        for (int j = n - 1; j >= 1; j--) {
+            int j1 = j + 1;
+            double tmp0 = 1.0 / (h2[j - 1] - h2[n - 1]);
+            double tmp1 = tmp0 * h2[j1 - 1];
+            double tmp2 = tmp0 * h2[n - 1];
+            D[(j - 1) * 6 * ds] = tmp1 * D[(j1 - 1) * 6 * ds] - tmp2 * D[(j - 1) * 6 * ds];
+        }
+        if (n > 3) {
+            double d1 = D[0];
+            E[n - 4] = d1;
+            double s = d1;
+            for (int j = 2; j <= n; j++) s = s + D[(j - 1) * 6 * ds];
+            cand[(long long)(n - 4) * 6 * cs] = s;
+        }
+    }
+}
+
+// Level error of one member from its six d(1) values (mercury6_2.for:3544-3556).
+__device__ __forceinline__ double bs_level_err(const double* d1, double xs, double vs, int* fault) {
+    bool fin = true;
+    for (int q = 0; q < 6; q++) fin = fin && isfinite(d1[q]);
+    double tmp1 = fmax(fmax(d1[0] * d1[0], d1[1] * d1[1]), d1[2] * d1[2]);
+    double tmp2 = fmax(fmax(d1[3] * d1[3], d1[4] * d1[4]), d1[5] * d1[5]);
+    if (!fin) atomicOr(fault, FAULT_FORCE);
+    return fin ? fmax(tmp1 * xs, tmp2 * vs) : 0.0;
+}
+
+__device__ __forceinline__ double bs_step_of(const BsArgs& A, bool& stop) {
+    stop = false;
+    if (!A.ctl) return A.h0;
+    if (A.ctl->state != 0) stop = true;      // the loop has ended (uniform)
+    return A.ctl->hlocal;
+}
+
+// One BS2 attempt, Big members: one block of kLevels x kEncBig threads,
+// thread (level n, member k+1).  All levels advance together, one force
+// evaluation per round, with a block barrier between writing the positions
+// and reading them for the pair forces; every position is also written to
+// tp (the Big positions at each force point) for bs2_particles.  Then the
+// per-component extrapolation, the level errors and the accepted states.
+__global__ void __launch_bounds__(kLevels * kEncBig) bs2_big(BsArgs A) {
+    bool stop;
+    const double h0 = bs_step_of(A, stop);
+    if (stop) return;
     const int nb = A.nbig - 1;
-    const int lane = threadIdx.x;
-    __shared__ double TP[kBsPoints][3 * kEncBig];
-    __shared__ double P[3 * kEncBig], F[3 * kEncBig];
-    // d tables and accepted states of the Big members; column kEncBig is
-    // scratch for the lanes past the Big members
-    constexpr int kCol = kEncBig + 1;
-    __shared__ double BD[72 * kCol];
-    __shared__ double BC[54 * kCol];
-    // --- Big members: every force point is a lockstep of all Big lanes
-    {
-        const int k = lane;
-        double x0[3] = {0, 0, 0}, v0[3] = {0, 0, 0}, a0[3] = {0, 0, 0};
-        if (k < nb)
+    const int t = threadIdx.x;
+    const int n = t / kEncBig + 1;
+    const int k = t % kEncBig;
+    const bool mem = k < nb;
+    __shared__ double P[kLevels][3 * kEncBig];
+    __shared__ double X0[3 * kEncBig];
+    __shared__ double D[72 * kEncBig];       // [6 * level + component][member]
+    __shared__ double E[kEncBig][6][9];
+    double x0[3] = {0, 0, 0}, v0[3] = {0, 0, 0}, a0[3] = {0, 0, 0};
+    if (mem)
+        for (int c = 0; c < 3; c++) {
+            x0[c] = A.x[c * A.ncap + 1 + k];
+            v0[c] = A.v[c * A.ncap + 1 + k];
+            if (n == 1) X0[3 * k + c] = x0[c];
+        }
+    __syncthreads();
+    if (mem) big_accel(A, k + 1, X0, a0);
+    BsLevel L;
+    bs_level_start(L, n, h0, x0, v0, a0);
+    double a[3] = {0, 0, 0};
+    for (int r = 1; r <= kLevels; r++) {
+        const bool act = mem && r <= n;
+        if (act)
             for (int c = 0; c < 3; c++) {
-                x0[c] = A.x[c * A.ncap + 1 + k];
-                v0[c] = A.v[c * A.ncap + 1 + k];
-                P[3 * k + c] = x0[c];
+                P[n - 1][3 * k + c] = L.xend[c];
+                A.tp[bs_point(n, r) * 3 * kEncBig + 3 * k + c] = L.xend[c];
             }
         __syncthreads();
-        bs_big_force(A, nb, P, F);
+        if (act) big_accel(A, k + 1, P[n - 1], a);
         __syncthreads();
-        if (k < nb)
-            for (int c = 0; c < 3; c++) a0[c] = F[3 * k + c];
-        __syncthreads();
-        double xs = 0, vs = 0;
-        if (k < nb) bs_scales(x0, v0, xs, vs);
-        double errs[9];
-        // the sequence with a force that synchronises the wave at every point
-        auto force = [&](int point, const double* xe, double* a) {
-            __syncthreads();
-            if (k < nb)
-                for (int c = 0; c < 3; c++) { P[3 * k + c] = xe[c]; TP[point][3 * k + c] = xe[c]; }
-            __syncthreads();
-            bs_big_force(A, nb, P, F);
-            __syncthreads();
-            for (int c = 0; c < 3; c++) a[c] = k < nb ? F[3 * k + c] : 0.0;
-        };
-        // lanes past nb run the same control flow on dummy data
-        double dummy0[3] = {1, 0, 0}, dummyv[3] = {0, 1, 0};
-        const double* mx0 = k < nb ? x0 : dummy0;
-        const double* mv0 = k < nb ? v0 : dummyv;
-        const int col = k < nb ? k : kEncBig;
-        bs_member(A, A.h0, mx0, mv0, a0, force, BD + col, kCol, xs, vs, errs, BC + col, kCol);
-        if (k < nb) {
-            for (int q = 0; q < 9; q++) atomicMax(A.err + 3 + q, (unsigned long long)__double_as_longlong(errs[q]));
-            if (blockIdx.x == 0)
-                for (int q = 0; q < 54; q++) A.cand[(long long)q * A.ncap + 1 + k] = BC[q * kCol + k];
-        }
-        __syncthreads();
+        if (act && r < n) bs_level_step(L, r, a, x0, v0, a0);
     }
-    // --- particle members
+    if (mem) {
+        double R[6];
+        bs_level_result(L, a, v0, R);
+        for (int q = 0; q < 6; q++) D[(6 * (n - 1) + q) * kEncBig + k] = R[q];
+    }
+    __syncthreads();
+    // extrapolation: thread (member k, component q)
+    if (t < 6 * kEncBig) {
+        const int km = t / 6, q = t % 6;
+        if (km < nb) {
+            double e[9];
+            bs_extrapolate(h0, D + q * kEncBig + km, kEncBig, e, A.cand + (long long)q * A.ncap + 1 + km, A.ncap);
+            for (int l = 0; l < 9; l++) E[km][q][l] = e[l];
+        }
+    }
+    __syncthreads();
+    // level errors: thread (member k, level 4 + l)
+    if (t < 9 * kEncBig) {
+        const int km = t / 9, l = t % 9;
+        if (km < nb) {
+            double xs, vs, xm[3], vm[3];
+            for (int c = 0; c < 3; c++) { xm[c] = A.x[c * A.ncap + 1 + km]; vm[c] = A.v[c * A.ncap + 1 + km]; }
+            bs_scales(xm, vm, xs, vs);
+            double d1[6];
+            for (int q = 0; q < 6; q++) d1[q] = E[km][q][l];
+            double err = bs_level_err(d1, xs, vs, A.fault);
+            atomicMax(A.err + 3 + l, (unsigned long long)__double_as_longlong(err));
+        }
+    }
+}
+
+// One BS2 attempt, particle members (after bs2_big): kBsPart particles per
+// block, thread (level n, particle slot); each level's evaluations use the
+// Big positions bs2_big recorded at the same force points.  The d tables live
+// in LDS; then the per-component extrapolation and the level errors, merged
+// over the block before one atomicMax per level.
+constexpr int kBsPart = 16;
+__global__ void __launch_bounds__(kLevels * kBsPart) bs2_particles(BsArgs A) {
+    bool stop;
+    const double h0 = bs_step_of(A, stop);
+    if (stop) return;
+    const int t = threadIdx.x;
+    const int n = t / kBsPart + 1;
+    const int sl = t % kBsPart;
     const int np = A.n - A.nbig;
-    const int pi_ = blockIdx.x * 64 + lane;
+    const int pi_ = blockIdx.x * kBsPart + sl;
     const bool live = pi_ < np;
     const int me = A.nbig + pi_;
-    double x0[3] = {1, 0, 0}, v0[3] = {0, 1, 0}, a0[3];
+    __shared__ double D[72 * kBsPart];       // [6 * level + component][slot]
+    __shared__ double E[kBsPart][6][9];
+    __shared__ unsigned long long EM[9];
+    if (t < 9) EM[t] = 0ull;
+    double x0[3] = {1, 0, 0}, v0[3] = {0, 1, 0}, a0[3] = {0, 0, 0};
     double rcj = 0.0;
     int p0 = 0, p1 = 0;
     if (live) {
@@ -598,11 +674,13 @@ __global__ void __launch_bounds__(64) bs2_attempt(BsArgs A) {
         p0 = A.poff[pi_];
         p1 = A.poff[pi_ + 1];
     }
-    auto pforce = [&](int point, const double* xe, double* a) {
+    // mfo_hkce for a particle: pairs with its Big partners, then the central term;
+    // Xb(i) gives Big member i's position at this evaluation
+    auto pforce = [&](const double* xe, double* a, auto Xb) {
         double acc[3] = {0.0, 0.0, 0.0};
         for (int q = p0; q < p1; q++) {
             int i = A.pidx[q];
-            const double* Xi = point >= 0 ? &TP[point][3 * (i - 1)] : &P[3 * (i - 1)];
+            const double* Xi = Xb(i);
             double dx = xe[0] - Xi[0], dy = xe[1] - Xi[1], dz = xe[2] - Xi[2];
             double tmp2;
             if (!hkce_factor(dx * dx + dy * dy + dz * dz, fmax(A.rcrit[i], rcj), tmp2)) continue;
@@ -614,27 +692,100 @@ __global__ void __launch_bounds__(64) bs2_attempt(BsArgs A) {
         double tmp2 = A.m[0] * s_1 * s_1 * s_1;
         for (int c = 0; c < 3; c++) a[c] = acc[c] - tmp2 * xe[c];
     };
-    // start positions of the Big members for a0
-    if (lane < nb)
-        for (int c = 0; c < 3; c++) P[3 * lane + c] = A.x[c * A.ncap + 1 + lane];
-    __syncthreads();
-    if (live) pforce(-1, x0, a0);
-    double errs[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     if (live) {
-        double xs, vs;
-        bs_scales(x0, v0, xs, vs);
-        bs_member(A, A.h0, x0, v0, a0, pforce, A.dbuf + pi_, A.ncap, xs, vs, errs, A.cand + me, A.ncap);
+        double xb[3];
+        pforce(x0, a0, [&](int i) {
+            for (int c = 0; c < 3; c++) xb[c] = A.x[c * A.ncap + i];
+            return (const double*)xb;
+        });
+        BsLevel L;
+        bs_level_start(L, n, h0, x0, v0, a0);
+        double a[3];
+        for (int r = 1; r <= n; r++) {
+            const double* T = A.tp + bs_point(n, r) * 3 * kEncBig;
+            pforce(L.xend, a, [&](int i) { return T + 3 * (i - 1); });
+            if (r < n) bs_level_step(L, r, a, x0, v0, a0);
+        }
+        double R[6];
+        bs_level_result(L, a, v0, R);
+        for (int q = 0; q < 6; q++) D[(6 * (n - 1) + q) * kBsPart + sl] = R[q];
     }
-    for (int q = 0; q < 9; q++) merge_err(A.err + 3 + q, errs[q]);
+    __syncthreads();
+    // extrapolation: thread (slot, component)
+    if (t < 6 * kBsPart) {
+        const int s = t / 6, q = t % 6;
+        const int ps = blockIdx.x * kBsPart + s;
+        if (ps < np) {
+            double e[9];
+            bs_extrapolate(h0, D + q * kBsPart + s, kBsPart, e, A.cand + (long long)q * A.ncap + A.nbig + ps,
+                           A.ncap);
+            for (int l = 0; l < 9; l++) E[s][q][l] = e[l];
+        }
+    }
+    __syncthreads();
+    // level errors: thread (slot, level 4 + l), merged per level over the block
+    if (t < 9 * kBsPart) {
+        const int s = t / 9, l = t % 9;
+        const int ps = blockIdx.x * kBsPart + s;
+        if (ps < np) {
+            const int mp = A.nbig + ps;
+            double xs, vs, xm[3], vm[3];
+            for (int c = 0; c < 3; c++) { xm[c] = A.x[c * A.ncap + mp]; vm[c] = A.v[c * A.ncap + mp]; }
+            bs_scales(xm, vm, xs, vs);
+            double d1[6];
+            for (int q = 0; q < 6; q++) d1[q] = E[s][q][l];
+            double err = bs_level_err(d1, xs, vs, A.fault);
+            atomicMax(&EM[l], (unsigned long long)__double_as_longlong(err));
+        }
+    }
+    __syncthreads();
+    if (t < 9 && EM[t]) atomicMax(A.err + 3 + t, EM[t]);
 }
 
 __global__ void bs2_accept(BsArgs A, int level) {
+    if (A.ctl) level = A.ctl->level;
+    if (level < 4) return;
     const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
     if (k >= A.n) return;
     for (int q = 0; q < 3; q++) {
         A.x[q * A.ncap + k] = A.cand[(long long)((level - 4) * 6 + q) * A.ncap + k];
         A.v[q * A.ncap + k] = A.cand[(long long)((level - 4) * 6 + 3 + q) * A.ncap + k];
     }
+}
+
+// One thread: after an attempt, take the first level n > 3 whose largest error
+// is within tol^2 (mdt_bs2), or halve the step; after an accepted substep
+// advance the local time and, unless h0 is done, clamp the next step to what
+// is left (mdt_hkce).  Clears the level errors for the next attempt.  The
+// arithmetic is the host loop's, operation for operation.
+__global__ void bs2_control(BsArgs A) {
+    BsCtl& c = *A.ctl;
+    if (c.state != 0) { c.level = 0; return; }
+    const double SHRINK = .55, GROW = 1.3;
+    if (*A.fault & FAULT_FORCE) { c.state = 3; c.level = 0; return; }
+    int level = 0;
+    for (int n = 4; n <= 12 && !level; n++)
+        if (__longlong_as_double((long long)A.err[n - 1]) <= c.tol2) level = n;
+    for (int q = 0; q < 12; q++) A.err[q] = 0ull;
+    c.level = level;
+    if (level) {
+        for (int n = 1; n <= level; n++) c.forces += n;
+        const double hdid = c.hlocal;
+        if (level >= 8) c.hlocal = c.hlocal * SHRINK;
+        if (level < 7) c.hlocal = c.hlocal * GROW;
+        c.tlocal = c.tlocal + hdid;
+        c.substeps++;
+        if (!((c.tlocal - c.h0) * c.h0 < 0.0)) { c.state = 1; return; }
+        c.forces += 1;                       // the next substep's start accelerations
+        const double tmp0 = fabs(c.h0) - fabs(c.tlocal);
+        c.hrec = c.hlocal;
+        if (fabs(c.hlocal) > tmp0) c.hlocal = copysign(tmp0, c.h0);
+    } else {
+        c.forces += 78;
+        c.rejected++;
+        c.hlocal = c.hlocal * .5;
+    }
+    if (c.tlocal + c.hlocal == c.tlocal) c.state = 2;
 }
 
 }  // namespace mhip

@@ -130,10 +130,11 @@ struct Context {
     struct Enc {
         bool active = false;
         int n = 0, nbig = 0, ncap = 0;
-        double *m = nullptr, *rcrit = nullptr, *x = nullptr, *v = nullptr, *dbuf = nullptr, *cand = nullptr;
+        double *m = nullptr, *rcrit = nullptr, *x = nullptr, *v = nullptr, *tp = nullptr, *cand = nullptr;
         int *bbp = nullptr, *poff = nullptr, *pidx = nullptr;
         int nbbp = 0, npidx_cap = 0, nbbp_cap = 0;
         unsigned long long* err = nullptr;
+        BsCtl* ctl = nullptr;          // device step control (encounter_advance)
         std::vector<double> stage;
     } enc;
     // differentiation tape (mercury_hip_adjoint_forward)
@@ -209,10 +210,10 @@ void release_all() {
     dfree(c.tape.dsys);
     c.tape.valid = false;
     for (double** p : {&c.rcrit, &c.hx0, &c.hv0, &c.hytab, &c.hpack, &c.enc.m, &c.enc.rcrit, &c.enc.x, &c.enc.v,
-                       &c.enc.dbuf, &c.enc.cand})
+                       &c.enc.tp, &c.enc.cand})
         dfree(*p);
     dfree(c.hpairs); dfree(c.hnpairs); dfree(c.hids); dfree(c.enc.bbp); dfree(c.enc.poff); dfree(c.enc.pidx);
-    dfree(c.enc.err);
+    dfree(c.enc.err); dfree(c.enc.ctl);
     c.hpair_cap = 0; c.hpack_cap = 0; c.hy_cap = 0; c.enc.ncap = 0; c.enc.active = false;
     c.enc.npidx_cap = 0; c.enc.nbbp_cap = 0;
     c.cap = 0; c.tab_cap = 0; c.ev_cap = 0; c.park_cap = 0; c.ntiles = 0;
@@ -1128,12 +1129,12 @@ void encounter_enter(int n, int nbig, const double* m, const double* x, const do
         int cap = std::max(n, 2 * e.ncap);
         for (double** p : {&e.m, &e.rcrit}) dalloc(*p, (size_t)cap);
         for (double** p : {&e.x, &e.v}) dalloc(*p, (size_t)3 * cap);
-        dalloc(e.dbuf, (size_t)72 * cap);
         dalloc(e.cand, (size_t)54 * cap);
         dalloc(e.poff, (size_t)cap + 1);
         e.ncap = cap;
     }
     if (!e.err) dalloc(e.err, 12);
+    if (!e.tp) dalloc(e.tp, (size_t)kBsPoints * 3 * kEncBig);
     // pair lists: Big-Big pairs in list order; each particle's Big partners
     std::vector<int> bb, off((size_t)(n - nbig) + 1, 0), idx;
     std::vector<std::vector<int>> part((size_t)std::max(n - nbig, 0));
@@ -1216,7 +1217,7 @@ void bs2_step(double time, double* h, double* hdid, double tol, int64_t* forces,
     A.n = e.n; A.nbig = e.nbig; A.m = e.m; A.rcrit = e.rcrit; A.x = e.x; A.v = e.v; A.ncap = e.ncap;
     A.bbp = e.bbp; A.nbbp = e.nbbp; A.poff = e.poff; A.pidx = e.pidx;
     A.tol2 = tol * tol;
-    A.dbuf = e.dbuf; A.cand = e.cand; A.err = e.err; A.fault = c.fault;
+    A.tp = e.tp; A.cand = e.cand; A.err = e.err; A.fault = c.fault;
     *forces = 1;           // the start accelerations, once per call
     for (;;) {
         if (time + *h == time) throw std::runtime_error("BS cannot advance time at the requested tolerance");
@@ -1224,7 +1225,10 @@ void bs2_step(double time, double* h, double* hdid, double tol, int64_t* forces,
         clear_fault();
         hipLaunchKernelGGL(zero_bytes, dim3(1), dim3(128), 0, c.stream, reinterpret_cast<unsigned char*>(e.err),
                            (long long)(12 * sizeof(unsigned long long)));
-        hipLaunchKernelGGL(bs2_attempt, dim3(std::max(1, (npart + 63) / 64)), dim3(64), 0, c.stream, A);
+        hipLaunchKernelGGL(bs2_big, dim3(1), dim3(kLevels * kEncBig), 0, c.stream, A);
+        if (npart > 0)
+            hipLaunchKernelGGL(bs2_particles, dim3((npart + kBsPart - 1) / kBsPart), dim3(kLevels * kBsPart), 0,
+                               c.stream, A);
         HIPCHECK(hipGetLastError());
         unsigned long long bits[12];
         HIPCHECK(hipMemcpyAsync(bits, e.err, sizeof(bits), hipMemcpyDeviceToHost, c.stream));
@@ -1251,6 +1255,63 @@ void bs2_step(double time, double* h, double* hdid, double tol, int64_t* forces,
         ++*rejected;
         *h = *h * .5;
     }
+}
+
+// The driver's mdt_hkce loop over one step h0 (mercury6_2.for:3883-3905) with
+// the step control on the device (bs2_control): batches of (attempt, control,
+// accept) are queued without host synchronisation, each later launch a no-op
+// once the loop has ended; the host checks the control block between batches.
+// Same sequence of steps and states as calling bs2_step per substep; the
+// encounter listing the driver makes after each substep is not made.
+void encounter_advance(double h0, double* hrec, double tol, int64_t* substeps, int64_t* forces,
+                       int64_t* rejected) {
+    Context& c = ctx;
+    Context::Enc& e = c.enc;
+    if (!e.active) throw std::runtime_error("no encounter context");
+    if (!(h0 != 0) || !std::isfinite(h0) || !std::isfinite(*hrec) || !(tol > 0))
+        throw std::runtime_error("invalid encounter step");
+    if (!e.ctl) dalloc(e.ctl, 1);
+    BsCtl k{};
+    k.h0 = h0;
+    k.tlocal = 0.0;
+    k.tol2 = tol * tol;
+    double hlocal = std::copysign(*hrec, h0);
+    const double tmp0 = std::fabs(h0) - std::fabs(k.tlocal);
+    k.hrec = hlocal;
+    if (std::fabs(hlocal) > tmp0) hlocal = std::copysign(tmp0, h0);
+    k.hlocal = hlocal;
+    k.forces = 1;
+    if (k.tlocal + k.hlocal == k.tlocal) throw std::runtime_error("BS cannot advance time at the requested tolerance");
+    const int npart = e.n - e.nbig;
+    BsArgs A{};
+    A.n = e.n; A.nbig = e.nbig; A.m = e.m; A.rcrit = e.rcrit; A.x = e.x; A.v = e.v; A.ncap = e.ncap;
+    A.bbp = e.bbp; A.nbbp = e.nbbp; A.poff = e.poff; A.pidx = e.pidx;
+    A.tol2 = k.tol2;
+    A.tp = e.tp; A.cand = e.cand; A.err = e.err; A.fault = c.fault; A.ctl = e.ctl;
+    clear_fault();
+    HIPCHECK(hipMemcpyAsync(e.ctl, &k, sizeof(k), hipMemcpyHostToDevice, c.stream));
+    hipLaunchKernelGGL(zero_bytes, dim3(1), dim3(128), 0, c.stream, reinterpret_cast<unsigned char*>(e.err),
+                       (long long)(12 * sizeof(unsigned long long)));
+    HIPCHECK(hipGetLastError());
+    const dim3 gp((npart + kBsPart - 1) / kBsPart), gc((std::max(e.n - 1, 1) + 127) / 128);
+    for (int batch = 8;; batch = std::min(2 * batch, 256)) {
+        for (int q = 0; q < batch; q++) {
+            hipLaunchKernelGGL(bs2_big, dim3(1), dim3(kLevels * kEncBig), 0, c.stream, A);
+            if (npart > 0) hipLaunchKernelGGL(bs2_particles, gp, dim3(kLevels * kBsPart), 0, c.stream, A);
+            hipLaunchKernelGGL(bs2_control, dim3(1), dim3(1), 0, c.stream, A);
+            hipLaunchKernelGGL(bs2_accept, gc, dim3(128), 0, c.stream, A, 0);
+        }
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipMemcpyAsync(&k, e.ctl, sizeof(k), hipMemcpyDeviceToHost, c.stream));
+        HIPCHECK(hipStreamSynchronize(c.stream));
+        if (k.state == 3) throw std::runtime_error("Nonfinite BS extrapolation");
+        if (k.state == 2) throw std::runtime_error("BS cannot advance time at the requested tolerance");
+        if (k.state == 1) break;
+    }
+    *hrec = k.hrec;
+    *substeps = k.substeps;
+    *forces = k.forces;
+    *rejected = k.rejected;
 }
 
 int read_fault() {
@@ -1551,6 +1612,14 @@ int mercury_hip_encounter_update(const double* m, const double* x, const double*
 }
 
 void mercury_hip_encounter_exit(void) { ctx.enc.active = false; }
+
+int mercury_hip_encounter_advance(double h0, double* hrec, double tol, int64_t* substeps, int64_t* forces,
+                                  int64_t* rejected) {
+    try {
+        encounter_advance(h0, hrec, tol, substeps, forces, rejected);
+        return 0;
+    } catch (const std::exception& e) { return fail(e); }
+}
 
 void mercury_hip_free(void) { release_all(); }
 

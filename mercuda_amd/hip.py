@@ -54,6 +54,7 @@ SIGNATURES = {
                                               C.c_int, _ip, _ip]),
     "mercury_hip_encounter_update": (C.c_int, [_dp, _dp, _dp]),
     "mercury_hip_encounter_exit": (None, []),
+    "mercury_hip_encounter_advance": (C.c_int, [C.c_double, _dp, C.c_double, _i64p, _i64p, _i64p]),
     "mercury_hip_free": (None, []),
     "mercury_hip_advance": (C.c_int, [C.c_double, C.c_double, C.c_int, C.c_double, C.c_int, _i64p]),
     "mercury_hip_advance_events": (C.c_int, [C.POINTER(C.POINTER(MercuryHipEvent)), _ip]),
@@ -291,6 +292,15 @@ class MercuryHip:
 
     def encounter_exit(self):
         self.L.mercury_hip_encounter_exit()
+
+    def encounter_advance(self, h0, hrec, tol):
+        """mdt_hkce's whole BS2 loop over h0 with the step control on the device.
+        Returns (hrec, substeps, forces, rejected)."""
+        hr = C.c_double(hrec)
+        ns = C.c_int64(); nf = C.c_int64(); nr = C.c_int64()
+        self._ok(self.L.mercury_hip_encounter_advance(float(h0), C.byref(hr), float(tol), C.byref(ns), C.byref(nf),
+                                                       C.byref(nr)), "encounter_advance")
+        return hr.value, ns.value, nf.value, nr.value
 
     # ---- differentiation (tape of the last adjoint_forward) ----
     def adjoint_forward(self, nsteps, h, segment=0):

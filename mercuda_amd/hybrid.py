@@ -17,9 +17,12 @@ import math
 import numpy as np
 
 
-def mdt_hkce(hip, time, h0, hrec, tol, s, x, v, ce, pairs, on_substep=None):
+def mdt_hkce(hip, time, h0, hrec, tol, s, x, v, ce, pairs, on_substep=None, device_loop=True):
     """Integrate the encounter members over h0 with BS2 substeps; returns the
-    BS2 step recommendation kept for the next encounter (hrec)."""
+    BS2 step recommendation kept for the next encounter (hrec).  Without an
+    on_substep callback the loop runs on the device (mercury_hip_encounter_advance)
+    unless device_loop is False; with one, each substep is a mercury_hip_step
+    call whose states the callback receives."""
     m = s["m"]
     nbig = s["nbig"]
     index = [0]
@@ -41,6 +44,15 @@ def mdt_hkce(hip, time, h0, hrec, tol, s, x, v, ce, pairs, on_substep=None):
     hip.encounter_enter(mbs, xbs, vbs, rcb, rceb, rphb, nbsbig, bp)
     substeps = 0
     try:
+        if on_substep is None and device_loop:
+            # no per-substep listing: the whole loop with its step control on the device
+            hrec, substeps, _, _ = hip.encounter_advance(h0, hrec, tol)
+            mdt_hkce.substeps = getattr(mdt_hkce, "substeps", 0) + substeps
+            xbs, vbs = hip.encounter_download()
+            for k in range(1, len(index)):
+                x[index[k]] = xbs[k]
+                v[index[k]] = vbs[k]
+            return hrec
         tlocal = 0.0
         hlocal = math.copysign(hrec, h0)
         while True:
@@ -70,7 +82,7 @@ def mdt_hkce(hip, time, h0, hrec, tol, s, x, v, ce, pairs, on_substep=None):
 
 
 def hy_run(hip, s, nsteps, tol=1e-12, cap=None, time0=0.0, on_substep=None, on_step=None, upload=True,
-           progress=None):
+           progress=None, device_loop=True):
     """nsteps HYBRID steps of system s (democratic heliocentric x, v; keys m,
     x, v, nbig, h, rcrit, rce, rphys, rcen).  Returns the final x, v and the
     largest number of encounter pairs met in one step.  on_step(step, time)
@@ -97,7 +109,7 @@ def hy_run(hip, s, nsteps, tol=1e-12, cap=None, time0=0.0, on_substep=None, on_s
             sub = None
             if on_substep is not None:
                 sub = lambda *a, _s=step: on_substep(_s, *a)
-            hrec = mdt_hkce(hip, time, h, hrec, tol, s, x, v, ce, pairs, sub)
+            hrec = mdt_hkce(hip, time, h, hrec, tol, s, x, v, ce, pairs, sub, device_loop)
         hip.hybrid_finish(h, s["m"], x, v)
         time = time + h
         if on_step is not None:

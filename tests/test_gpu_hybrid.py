@@ -87,3 +87,23 @@ def test_hybrid_ragged_counts(hip, oracle, count):
     xo, vo, evi, evd, nf = oracle.hy_run(s, 40)
     check(x, v, xo, vo, f"HYBRID count {count}")
     compare_rows(rows, oracle_rows(evi, evd))
+
+
+def test_device_encounter_loop_equals_per_substep_calls(hip, oracle):
+    """mercury_hip_encounter_advance (mdt_hkce's loop with the step control on
+    the device) against the same loop driven by one mercury_hip_step per
+    substep: the same substeps and bitwise the same states; and the oracle."""
+    from mercuda_amd import hybrid
+    s = hybrid_system(5, 2000)
+    say("case: HYBRID encounter loop on the device vs per-substep calls, 8 planets + 2000 particles, 60 steps")
+    hybrid.mdt_hkce.substeps = 0
+    xd, vd, most = hybrid.hy_run(hip, s, 60)
+    nd = hybrid.mdt_hkce.substeps
+    hybrid.mdt_hkce.substeps = 0
+    xh, vh, _ = hybrid.hy_run(hip, s, 60, device_loop=False)
+    nh = hybrid.mdt_hkce.substeps
+    say(f"encounter pairs in one step, at most {most}; BS2 substeps device {nd}, per-call {nh}")
+    assert most > 0 and nd == nh > 0
+    assert np.array_equal(xd, xh) and np.array_equal(vd, vh)
+    xo, vo, _, _, _ = oracle.hy_run(s, 60)
+    check(xd, vd, xo, vo, "HYBRID device encounter loop")
