@@ -119,6 +119,9 @@ class Oracle:
         L.orc_mdt_bs2.restype = C.c_int
         L.orc_mdt_bs2.argtypes = [C.c_double, _dp, _dp, C.c_double, C.c_int, _dp, _dp, _dp, _dp, C.c_int, _ip, _ip,
                                   C.POINTER(C.c_long)]
+        L.orc_bs_run.restype = C.c_long
+        L.orc_bs_run.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, _dp, C.c_int, C.c_int, _dp, _dp,
+                                 _dp, C.c_int, C.c_int, _dp, _dp, C.POINTER(C.c_long)]
         L.orc_mce_snif.restype = C.c_int
         L.orc_mce_snif.argtypes = [C.c_double, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _ip, C.c_int, _ip, _ip]
 
@@ -207,6 +210,21 @@ class Oracle:
         nf = self.L.orc_ra15_run(time0, sysd["h"] if h0 is None else h0, tol, _d(jc), len(m), sysd["nbig"],
                                  _d(m), _d(x), _d(v), int(pn), nsteps, _d(hdid), _d(hn))
         return x, v, hdid[:nsteps].copy(), float(hn[0]), int(nf)
+
+    def bs_run(self, algor, sysd, nsteps, h0=None, tol=1e-12, pn=0, time0=0.0):
+        """nsteps steps of algorithm 2 (mdt_bs1) or 3 (mdt_bs2) with mfo_all on a
+        whole system: final x, v, the sizes done, the next proposal, the halvings
+        and the force-evaluation count."""
+        m = np.ascontiguousarray(sysd["m"], np.float64)
+        x = np.ascontiguousarray(sysd["x"], np.float64).copy()
+        v = np.ascontiguousarray(sysd["v"], np.float64).copy()
+        jc = np.ascontiguousarray(sysd.get("jcen", (0.0, 0.0, 0.0)), np.float64)
+        hdid = np.zeros(max(nsteps, 1)); hn = np.zeros(1); rej = C.c_long(0)
+        nf = self.L.orc_bs_run(int(algor), time0, sysd["h"] if h0 is None else h0, tol, _d(jc), len(m),
+                               sysd["nbig"], _d(m), _d(x), _d(v), int(pn), nsteps, _d(hdid), _d(hn), C.byref(rej))
+        if nf < 0:
+            raise RuntimeError(f"oracle BS run failed ({nf})")
+        return x, v, hdid[:nsteps].copy(), float(hn[0]), int(rej.value), int(nf)
 
     def hy_run(self, sysd, nsteps, time0=0.0, tol=1e-12, cap=200000):
         """nsteps HYBRID steps (mdt_hy + the driver's mce_cent on physical
@@ -351,6 +369,31 @@ class Reference:
                              C.byref(C.c_int(sysd["nbig"])), _d(m), _d(x), _d(v), _d(s), _d(rphys), _d(rc),
                              _d(ngf), _i(st), C.byref(dtflag), C.byref(C.c_int(0)), _i(opt),
                              C.byref(C.c_int(0)), _i(np.zeros(1, np.int32)), _i(np.zeros(1, np.int32)), force)
+            hd.append(tdid.value)
+            time = time + tdid.value
+        return x, v, np.array(hd), t.value
+
+    def bs_run(self, algor, sysd, nsteps, h0=None, tol=1e-12, pn=0, time0=0.0):
+        """nsteps calls of the reference mdt_bs1 (algor 2) or mdt_bs2 (algor 3)
+        with mfo_all as the force, as mal_hvar makes them."""
+        m = np.ascontiguousarray(sysd["m"], np.float64)
+        n = len(m)
+        x = np.ascontiguousarray(sysd["x"], np.float64).copy()
+        v = np.ascontiguousarray(sysd["v"], np.float64).copy()
+        jc = np.ascontiguousarray(sysd.get("jcen", (0.0, 0.0, 0.0)), np.float64)
+        s = np.zeros((n, 3)); rphys = np.zeros(n); rc = np.zeros(n); st = np.zeros(n, np.int32)
+        ngf = np.zeros((n, 5)); opt = np.array([0, 1, 1, 2, 0, 1, int(pn), 0], np.int32)
+        force = C.cast(self.L.mfo_all_, C.c_void_p)
+        fn = self.L.mdt_bs1_ if algor == 2 else self.L.mdt_bs2_
+        t = C.c_double(sysd["h"] if h0 is None else h0); tdid = C.c_double(0.0)
+        dtflag = C.c_int(2)
+        hd = []
+        time = time0
+        for _ in range(nsteps):
+            fn(self._r(time), C.byref(t), C.byref(tdid), self._r(tol), _d(jc), C.byref(C.c_int(n)),
+               C.byref(C.c_int(sysd["nbig"])), _d(m), _d(x), _d(v), _d(s), _d(rphys), _d(rc), _d(ngf), _i(st),
+               C.byref(dtflag), C.byref(C.c_int(0)), _i(opt), C.byref(C.c_int(0)), _i(np.zeros(1, np.int32)),
+               _i(np.zeros(1, np.int32)), force)
             hd.append(tdid.value)
             time = time + tdid.value
         return x, v, np.array(hd), t.value

@@ -1411,10 +1411,30 @@ void orc_mfo_hkce(int nbod, const double *m, const double *x, const double *rcri
     mfo_hkce(nbod, m, x, rcrit, a, nce, ice, jce);
 }
 
-/* mdt_bs2  mercury6_2.for:3428-3617 with mfo_hkce forces.  Returns the
- * number of halvings (rejected attempts); -1 if time cannot advance. */
-int orc_mdt_bs2(double time, double *h0, double *hdid, double tol, int nbod, const double *m, double *x0,
-                double *v0, const double *rcrit, int nce, const int *ice, const int *jce, long *nforce)
+/* Force of a BS step: a[3*nbod] at positions x (and velocities v). */
+typedef void (*bs_force_fn)(void *ctx, const double *x, const double *v, double *a);
+
+struct hkce_ctx { int nbod; const double *m, *rcrit; int nce; const int *ice, *jce; };
+static void hkce_force(void *p, const double *x, const double *v, double *a)
+{
+    (void)v;
+    struct hkce_ctx *c = (struct hkce_ctx *)p;
+    mfo_hkce(c->nbod, c->m, x, c->rcrit, a, c->nce, c->ice, c->jce);
+}
+
+struct all_ctx { const double *jcen; int nbod, nbig; const double *m; int pn; };
+static void all_force(void *p, const double *x, const double *v, double *a)
+{
+    struct all_ctx *c = (struct all_ctx *)p;
+    orc_mfo_all(c->jcen, c->nbod, c->nbig, c->m, x, v, c->pn, a);
+}
+
+/* mdt_bs2  mercury6_2.for:3428-3617: 12 modified-midpoint levels of a
+ * conservative system (every force call gets the start velocities v0).
+ * Returns the number of halvings (rejected attempts); -1 if time cannot
+ * advance, -2 on a non-finite extrapolation. */
+static int bs2_core(double time, double *h0, double *hdid, double tol, int nbod, double *x0, double *v0,
+                    bs_force_fn force, void *fc, long *nforce)
 {
     const double SHRINK = .55, GROW = 1.3;
     const double tol2 = tol * tol;
@@ -1432,7 +1452,7 @@ int orc_mdt_bs2(double time, double *h0, double *hdid, double tol, int nbod, con
         xscal[k] = 1.0 / fmax(tmp1, 1e-300);
         vscal[k] = 1.0 / fmax(tmp2, 1e-300);
     }
-    mfo_hkce(nbod, m, x0, rcrit, a0, nce, ice, jce);
+    force(fc, x0, v0, a0);
     (*nforce)++;
     for (;;) {
         if (time + *h0 == time) { rejected = -1; break; }
@@ -1449,7 +1469,7 @@ int orc_mdt_bs2(double time, double *h0, double *hdid, double tol, int nbod, con
                     xend[3 * k + q] = h2by2 * a0[3 * k + q] + h * v0[3 * k + q] + x0[3 * k + q];
                 }
             for (int j = 2; j <= n; j++) {
-                mfo_hkce(nbod, m, xend, rcrit, a, nce, ice, jce);
+                force(fc, xend, v0, a);
                 (*nforce)++;
                 double tmp0 = h * (double)j;
                 for (int k = 1; k < nbod; k++)
@@ -1460,7 +1480,7 @@ int orc_mdt_bs2(double time, double *h0, double *hdid, double tol, int nbod, con
                                         + x0[3 * k + q];
                     }
             }
-            mfo_hkce(nbod, m, xend, rcrit, a, nce, ice, jce);
+            force(fc, xend, v0, a);
             (*nforce)++;
             for (int k = 1; k < nbod; k++)
                 for (int q = 0; q < 3; q++) {
@@ -1477,11 +1497,14 @@ int orc_mdt_bs2(double time, double *h0, double *hdid, double tol, int nbod, con
             }
             if (n > 3) {
                 double errmax = 0.0;
+                int bad = 0;
                 for (int k = 1; k < nbod; k++) {
+                    for (int q = 0; q < 6; q++) bad |= !isfinite(D(q, k, 0));
                     double tmp1 = fmax(fmax(D(0, k, 0) * D(0, k, 0), D(1, k, 0) * D(1, k, 0)), D(2, k, 0) * D(2, k, 0));
                     double tmp2 = fmax(fmax(D(3, k, 0) * D(3, k, 0), D(4, k, 0) * D(4, k, 0)), D(5, k, 0) * D(5, k, 0));
                     errmax = fmax(errmax, fmax(tmp1 * xscal[k], tmp2 * vscal[k]));
                 }
+                if (bad) { rejected = -2; done = 2; break; }
                 if (errmax <= tol2) {
                     for (int k = 1; k < nbod; k++)
                         for (int q = 0; q < 3; q++) {
@@ -1508,6 +1531,158 @@ int orc_mdt_bs2(double time, double *h0, double *hdid, double tol, int nbod, con
 #undef D
     free(xend); free(b); free(c); free(a); free(a0); free(d); free(xscal); free(vscal);
     return rejected;
+}
+
+/* mdt_bs2  mercury6_2.for:3428-3617 with mfo_hkce forces (the encounter set). */
+int orc_mdt_bs2(double time, double *h0, double *hdid, double tol, int nbod, const double *m, double *x0,
+                double *v0, const double *rcrit, int nce, const int *ice, const int *jce, long *nforce)
+{
+    struct hkce_ctx c = {nbod, m, rcrit, nce, ice, jce};
+    return bs2_core(time, h0, hdid, tol, nbod, x0, v0, hkce_force, &c, nforce);
+}
+
+/* mdt_bs1  mercury6_2.for:3227-3426: 8 modified-midpoint levels of 2n
+ * substeps for a general (velocity-dependent) force.  Returns as bs2_core. */
+static int bs1_core(double time, double *h0, double *hdid, double tol, int nbod, double *x0, double *v0,
+                    bs_force_fn force, void *fc, long *nforce)
+{
+    const double SHRINK = .55, GROW = 1.3;
+    const double tol2 = tol * tol;
+    const size_t nv = (size_t)3 * nbod;
+    double *x = calloc(nv, sizeof(double)), *v = calloc(nv, sizeof(double));
+    double *xend = calloc(nv, sizeof(double)), *vend = calloc(nv, sizeof(double));
+    double *a = calloc(nv, sizeof(double)), *a0 = calloc(nv, sizeof(double));
+    double *d = calloc((size_t)6 * nbod * 8, sizeof(double));
+    double *xscal = calloc((size_t)nbod, sizeof(double)), *vscal = calloc((size_t)nbod, sizeof(double));
+    double h2[8];
+    int rejected = 0;
+#define D(q, k, n) d[((size_t)(n) * nbod + (k)) * 6 + (q)]
+    for (int k = 1; k < nbod; k++) {
+        double tmp1 = x0[3 * k] * x0[3 * k] + x0[3 * k + 1] * x0[3 * k + 1] + x0[3 * k + 2] * x0[3 * k + 2];
+        double tmp2 = v0[3 * k] * v0[3 * k] + v0[3 * k + 1] * v0[3 * k + 1] + v0[3 * k + 2] * v0[3 * k + 2];
+        xscal[k] = 1.0 / fmax(tmp1, 1e-300);
+        vscal[k] = 1.0 / fmax(tmp2, 1e-300);
+    }
+    force(fc, x0, v0, a0);
+    (*nforce)++;
+    for (;;) {
+        if (time + *h0 == time) { rejected = -1; break; }
+        int done = 0;
+        for (int n = 1; n <= 8 && !done; n++) {
+            double h = *h0 / (2.0 * (double)n);
+            h2[n - 1] = .25 / (double)(n * n);
+            double hx2 = h * 2.0;
+            for (int k = 1; k < nbod; k++)
+                for (int q = 0; q < 3; q++) {
+                    x[3 * k + q] = x0[3 * k + q] + h * v0[3 * k + q];
+                    v[3 * k + q] = v0[3 * k + q] + h * a0[3 * k + q];
+                }
+            force(fc, x, v, a);
+            (*nforce)++;
+            for (int k = 1; k < nbod; k++)
+                for (int q = 0; q < 3; q++) {
+                    xend[3 * k + q] = x0[3 * k + q] + hx2 * v[3 * k + q];
+                    vend[3 * k + q] = v0[3 * k + q] + hx2 * a[3 * k + q];
+                }
+            for (int j = 2; j <= n; j++) {
+                force(fc, xend, vend, a);
+                (*nforce)++;
+                for (int k = 1; k < nbod; k++)
+                    for (int q = 0; q < 3; q++) {
+                        x[3 * k + q] = x[3 * k + q] + hx2 * vend[3 * k + q];
+                        v[3 * k + q] = v[3 * k + q] + hx2 * a[3 * k + q];
+                    }
+                force(fc, x, v, a);
+                (*nforce)++;
+                for (int k = 1; k < nbod; k++)
+                    for (int q = 0; q < 3; q++) {
+                        xend[3 * k + q] = xend[3 * k + q] + hx2 * v[3 * k + q];
+                        vend[3 * k + q] = vend[3 * k + q] + hx2 * a[3 * k + q];
+                    }
+            }
+            force(fc, xend, vend, a);
+            (*nforce)++;
+            for (int k = 1; k < nbod; k++)
+                for (int q = 0; q < 3; q++) {
+                    D(q, k, n - 1) = .5 * (xend[3 * k + q] + x[3 * k + q] + h * vend[3 * k + q]);
+                    D(3 + q, k, n - 1) = .5 * (vend[3 * k + q] + v[3 * k + q] + h * a[3 * k + q]);
+                }
+            for (int j = n - 1; j >= 1; j--) {
+                int j1 = j + 1;
+                double tmp0 = 1.0 / (h2[j - 1] - h2[n - 1]);
+                double tmp1 = tmp0 * h2[j1 - 1];
+                double tmp2 = tmp0 * h2[n - 1];
+                for (int k = 1; k < nbod; k++)
+                    for (int q = 0; q < 6; q++) D(q, k, j - 1) = tmp1 * D(q, k, j1 - 1) - tmp2 * D(q, k, j - 1);
+            }
+            if (n > 3) {
+                double errmax = 0.0;
+                int bad = 0;
+                for (int k = 1; k < nbod; k++) {
+                    for (int q = 0; q < 6; q++) bad |= !isfinite(D(q, k, 0));
+                    double tmp1 = fmax(fmax(D(0, k, 0) * D(0, k, 0), D(1, k, 0) * D(1, k, 0)), D(2, k, 0) * D(2, k, 0));
+                    double tmp2 = fmax(fmax(D(3, k, 0) * D(3, k, 0), D(4, k, 0) * D(4, k, 0)), D(5, k, 0) * D(5, k, 0));
+                    errmax = fmax(errmax, fmax(tmp1 * xscal[k], tmp2 * vscal[k]));
+                }
+                if (bad) { rejected = -2; done = 2; break; }
+                if (errmax <= tol2) {
+                    for (int k = 1; k < nbod; k++)
+                        for (int q = 0; q < 3; q++) {
+                            x0[3 * k + q] = D(q, k, 0);
+                            v0[3 * k + q] = D(3 + q, k, 0);
+                        }
+                    for (int j = 2; j <= n; j++)
+                        for (int k = 1; k < nbod; k++)
+                            for (int q = 0; q < 3; q++) {
+                                x0[3 * k + q] = x0[3 * k + q] + D(q, k, j - 1);
+                                v0[3 * k + q] = v0[3 * k + q] + D(3 + q, k, j - 1);
+                            }
+                    *hdid = *h0;
+                    if (n == 8) *h0 = *h0 * SHRINK;
+                    if (n < 7) *h0 = *h0 * GROW;
+                    done = 1;
+                }
+            }
+        }
+        if (done) break;
+        rejected++;
+        *h0 = *h0 * .5;
+    }
+#undef D
+    free(x); free(v); free(xend); free(vend); free(a); free(a0); free(d); free(xscal); free(vscal);
+    return rejected;
+}
+
+/* One step of algorithm 2 (mdt_bs1) or 3 (mdt_bs2) over the whole system with
+ * mfo_all (gravity, 1PN if pn, oblateness if jcen) as the driver passes it. */
+int orc_mdt_bs(int algor, double time, double *h0, double *hdid, double tol, const double *jcen, int nbod,
+               int nbig, const double *m, double *x0, double *v0, int pn, long *nforce)
+{
+    struct all_ctx c = {jcen, nbod, nbig, m, pn};
+    if (algor == 2) return bs1_core(time, h0, hdid, tol, nbod, x0, v0, all_force, &c, nforce);
+    return bs2_core(time, h0, hdid, tol, nbod, x0, v0, all_force, &c, nforce);
+}
+
+/* nsteps steps of algorithm 2 or 3; the sizes done in hdid[nsteps], the next
+ * proposal in *hnext, halvings in *rejected.  Returns the force count, or a
+ * negative bs*_core code. */
+long orc_bs_run(int algor, double time0, double h0, double tol, const double *jcen, int nbod, int nbig,
+                const double *m, double *x, double *v, int pn, int nsteps, double *hdid, double *hnext,
+                long *rejected)
+{
+    long nforce = 0;
+    double time = time0, h = h0;
+    *rejected = 0;
+    for (int s = 0; s < nsteps; s++) {
+        double hd = 0.0;
+        int r = orc_mdt_bs(algor, time, &h, &hd, tol, jcen, nbod, nbig, m, x, v, pn, &nforce);
+        if (r < 0) return r;
+        *rejected += r;
+        hdid[s] = hd;
+        time = time + hd;
+    }
+    *hnext = h;
+    return nforce;
 }
 
 /* mdt_hkce  mercury6_2.for:3798-3960, collisions not allowed: the encounter

@@ -126,6 +126,12 @@ void orc_mfo_hkce(int nbod, const double *m, const double *x, const double *rcri
 /* mdt_bs2 with mfo_hkce forces on a (compact) set; pairs 1-based. */
 int orc_mdt_bs2(double time, double *h0, double *hdid, double tol, int nbod, const double *m, double *x0,
                 double *v0, const double *rcrit, int nce, const int *ice, const int *jce, long *nforce);
+/* algorithm 2 (mdt_bs1) or 3 (mdt_bs2) with mfo_all: one step; a run of steps */
+int orc_mdt_bs(int algor, double time, double *h0, double *hdid, double tol, const double *jcen, int nbod,
+               int nbig, const double *m, double *x0, double *v0, int pn, long *nforce);
+long orc_bs_run(int algor, double time0, double h0, double tol, const double *jcen, int nbod, int nbig,
+                const double *m, double *x, double *v, int pn, int nsteps, double *hdid, double *hnext,
+                long *rejected);
 typedef struct {
     double *a;            /* interaction accelerations carried between calls */
     int dtflag;           /* 0 first call, 1 reset, 2 normal */

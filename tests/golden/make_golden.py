@@ -116,6 +116,7 @@ def main():
     np.savez_compressed(HERE / "mce_cent.npz", m=m, x0=X0, v0=V0, x1=X1, v1=V1,
                         hits=np.array(hits, float).reshape(-1, 3))
     radau(R)
+    bs(R)
     hybrid(R)
     print("wrote", sorted(p.name for p in HERE.glob("*.npz")))
 
@@ -163,6 +164,26 @@ def radau(R=None):
     np.savez_compressed(HERE / "mvs2h.npz", **cor)
 
 
+def bs(R=None):
+    """Whole runs of algorithm 2 (mdt_bs1) and 3 (mdt_bs2) with mfo_all:
+    growing steps, halvings from a too-large first step, 1PN, oblateness."""
+    R = R or Reference()
+    jc = (1e-6 * 0.0046 ** 2, -2e-7 * 0.0046 ** 4, 5e-8 * 0.0046 ** 6)
+    runs = {}
+    for c, (algor, seed, npart, nsteps, pn, h0, obl) in enumerate([(2, 11, 60, 40, 0, 4.0, False),
+                                                                   (2, 12, 40, 25, 1, 300.0, True),
+                                                                   (3, 13, 60, 40, 0, 4.0, True),
+                                                                   (3, 14, 40, 25, 1, 300.0, False)]):
+        s = systems.config_small(seed, npart)
+        s["jcen"] = np.array(jc if obl else (0.0, 0.0, 0.0))
+        x, v, hd, hn = R.bs_run(algor, s, nsteps, h0=h0, tol=1e-12, pn=pn)
+        for key in ("m", "x", "v", "jcen"):
+            runs[f"{key}{c}"] = s[key]
+        runs[f"meta{c}"] = np.array([algor, s["nbig"], nsteps, h0, pn, 1e-12])
+        runs[f"xout{c}"] = x; runs[f"vout{c}"] = v; runs[f"hdid{c}"] = hd; runs[f"hnext{c}"] = np.array([hn])
+    np.savez_compressed(HERE / "bs_runs.npz", **runs)
+
+
 def hybrid(R=None):
     """Whole HYBRID runs (mdt_hy with BS2 encounter substeps, the driver's
     mce_cent on physical coordinates), collisions not allowed."""
@@ -182,6 +203,6 @@ def hybrid(R=None):
 if __name__ == "__main__":
     threading.stack_size(512 << 20)
     only = sys.argv[1] if len(sys.argv) > 1 else None
-    t = threading.Thread(target={"radau": radau, "hybrid": hybrid}.get(only, main))
+    t = threading.Thread(target={"radau": radau, "bs": bs, "hybrid": hybrid}.get(only, main))
     t.start()
     t.join()

@@ -537,3 +537,45 @@ def test_bs2_kepler_known_answer(oracle):
         assert hd != 0.0 and nf > 0
         assert np.linalg.norm(x1[1] - want[:3]) < 1e-10 * r
         assert np.linalg.norm(v1[1] - want[3:]) < 1e-10 * vel
+
+
+def test_golden_bs_runs(oracle):
+    """Whole runs of algorithm 2 (mdt_bs1) and 3 (mdt_bs2) with mfo_all,
+    including halvings from a too-large first step, 1PN and oblateness: states
+    and every step size done, bit for bit."""
+    g = np.load(GOLD / "bs_runs.npz")
+    for c in range(4):
+        algor, nbig, nsteps, h0, pn, tol = g[f"meta{c}"]
+        s = dict(m=g[f"m{c}"], x=g[f"x{c}"], v=g[f"v{c}"], nbig=int(nbig), jcen=g[f"jcen{c}"], h=h0)
+        x, v, hd, hn, rej, _ = oracle.bs_run(int(algor), s, int(nsteps), tol=tol, pn=int(pn))
+        assert np.array_equal(hd, g[f"hdid{c}"]) and hn == g[f"hnext{c}"][0], f"run {c}"
+        assert np.array_equal(x, g[f"xout{c}"]) and np.array_equal(v, g[f"vout{c}"]), f"run {c}"
+        if h0 > 100:
+            assert rej > 0, f"run {c} should halve its first step"
+
+
+@pytest.mark.parametrize("algor", [2, 3])
+def test_reference_bs_run(oracle, reference, algor):
+    s = systems.config_small(seed=19, nparticles=120)
+    s["jcen"] = np.array(JC)
+    x, v, hd, hn, _, _ = oracle.bs_run(algor, s, 60, h0=90.0, tol=1e-11, pn=1)
+    xr, vr, hr, hnr = run_big_stack(reference.bs_run, algor, s, 60, h0=90.0, tol=1e-11, pn=1)
+    assert np.array_equal(hd, hr) and hn == hnr
+    assert np.array_equal(x, xr) and np.array_equal(v, vr)
+
+
+@pytest.mark.parametrize("algor", [2, 3])
+def test_bs_circular_orbit_known_answer(oracle, algor):
+    """Circular orbits about the central body alone: after ~500 d of adaptive
+    BS the phase is sqrt(mu/r^3) t to 1e-10 AU (closed form)."""
+    r = np.array([0.7, 1.0, 1.6, 3.3]); th0 = np.array([0.0, 1.0, 2.5, 4.0]); k = len(r)
+    m = np.concatenate([[MU], np.zeros(k)]); x = np.zeros((k + 1, 3)); v = np.zeros((k + 1, 3))
+    w = np.sqrt(MU / r ** 3)
+    x[1:, 0] = r * np.cos(th0); x[1:, 1] = r * np.sin(th0)
+    v[1:, 0] = -r * w * np.sin(th0); v[1:, 1] = r * w * np.cos(th0)
+    xo, vo, hd, hn, rej, nf = oracle.bs_run(algor, {"m": m, "x": x, "v": v, "nbig": 1, "h": 5.0}, 40, tol=1e-12)
+    t = 0.0
+    for q in hd:
+        t = t + q
+    want = np.stack([r * np.cos(th0 + w * t), r * np.sin(th0 + w * t)], 1)
+    assert t > 400.0 and np.abs(xo[1:, :2] - want).max() < 1e-10
