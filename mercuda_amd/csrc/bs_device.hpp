@@ -1,0 +1,440 @@
+// Bulirsch-Stoer integration of a whole system for gfx950, FP64: algorithm 2
+// (mdt_bs1, mercury6_2.for:3227-3426) and algorithm 3 (mdt_bs2,
+// mercury6_2.for:3428-3617), each with mfo_all as the force (mfo_grav with the
+// optional central 1PN term, plus oblateness; mercury6_2.for:4506-4707).
+//
+// Massless particles do not act on the Big bodies, so one attempt runs as
+//   bsw_big        one block, a thread per (level, planet): every
+//                  modified-midpoint level advances in lockstep, one force
+//                  evaluation per round, and each evaluation tabulates what a
+//                  particle's force at that point needs (indirect term,
+//                  oblateness reaction, planet positions and masses, the RA15
+//                  table layout); then the planets' extrapolation, level
+//                  errors and accepted states;
+//   bsw_particles  a thread per (level, particle), the force read from those
+//                  tables; the same extrapolation per particle;
+//   bsw_accept     the level the host chose (first n > 3 with the largest
+//                  error within tol^2), the step-start states kept in ox, ov.
+// The levels are independent until the extrapolation, which replays the
+// Fortran's order per (body, component), so the bits are the sequential
+// loop's.  mdt_bs1 passes each point's own velocities to the force, mdt_bs2
+// the step-start velocities v0 (its force is meant to be conservative).
+#pragma once
+#include "radau_device.hpp"
+
+namespace mhip {
+
+constexpr int kBsLanes = 16;      // planets one level of bsw_big can hold
+constexpr int kBsMaxLevels = 12;
+
+template <int ALG> struct BsShape;
+template <> struct BsShape<2> { static constexpr int L = 8; static constexpr int R = 16; static constexpr int P = 72; };
+template <> struct BsShape<3> { static constexpr int L = 12; static constexpr int R = 12; static constexpr int P = 78; };
+
+// force evaluations of level n, and the table slot of its r-th (slot 0: the
+// step start; then the evaluations in the sequential order)
+template <int ALG> __host__ __device__ constexpr int bsw_evals(int n) { return ALG == 2 ? 2 * n : n; }
+template <int ALG> __host__ __device__ constexpr int bsw_slot(int n, int r) {
+    return 1 + (ALG == 2 ? n * (n - 1) : n * (n - 1) / 2) + r - 1;
+}
+
+struct BswArgs {
+    const Sys* sys;
+    State st;
+    double* tab;                // 1 + P tables of stride Sys::ts
+    double* cand;               // accepted states per level n > 3: [L - 3][6][cap]
+    unsigned long long* err;    // max error per level [12] (bits of non-negative doubles)
+    double h0;
+    double light2;
+};
+
+// One level of one body: h2 of the extrapolation, then the running sums.
+// ALG 2: x, v and xe, ve (xend, vend); ALG 3: b, c and xe (xend).
+template <int ALG>
+struct BswLevel {
+    double h, hby2, h2, h2by2, hx2;
+    double x[3], v[3], xe[3], ve[3];
+};
+
+template <int ALG>
+__device__ __forceinline__ double bsw_h2(double h0, int n) {
+    if (ALG == 2) return .25 / (double)(n * n);
+    double h = h0 / (double)n;
+    return h * h;
+}
+
+template <int ALG>
+__device__ __forceinline__ void bsw_start(BswLevel<ALG>& L, int n, double h0, const double* x0, const double* v0,
+                                          const double* a0) {
+    if (ALG == 2) {
+        L.h = h0 / (2.0 * (double)n);
+        L.hx2 = L.h * 2.0;
+        for (int q = 0; q < 3; q++) {
+            L.x[q] = x0[q] + L.h * v0[q];
+            L.v[q] = v0[q] + L.h * a0[q];
+        }
+    } else {
+        L.h = h0 / (double)n;
+        L.hby2 = .5 * L.h;
+        L.h2 = L.h * L.h;
+        L.h2by2 = .5 * L.h2;
+        for (int q = 0; q < 3; q++) {
+            L.x[q] = .5 * a0[q];       // b
+            L.v[q] = 0.0;              // c
+            L.xe[q] = L.h2by2 * a0[q] + L.h * v0[q] + x0[q];
+        }
+    }
+}
+
+// Where evaluation r of level n is taken: position and the velocity the force sees.
+template <int ALG>
+__device__ __forceinline__ void bsw_point(const BswLevel<ALG>& L, int r, const double* v0, const double*& xp,
+                                          const double*& vp) {
+    if (ALG == 2) {
+        if (r & 1) { xp = L.x; vp = L.v; }
+        else { xp = L.xe; vp = L.ve; }
+    } else {
+        xp = L.xe; vp = v0;
+    }
+}
+
+// After the force a of evaluation r < evals(n): advance to the next point.
+template <int ALG>
+__device__ __forceinline__ void bsw_advance(BswLevel<ALG>& L, int r, const double* a, const double* x0,
+                                            const double* v0, const double* a0) {
+    if (ALG == 2) {
+        if (r == 1) {
+            for (int q = 0; q < 3; q++) {
+                L.xe[q] = x0[q] + L.hx2 * L.v[q];
+                L.ve[q] = v0[q] + L.hx2 * a[q];
+            }
+        } else if (r & 1) {
+            for (int q = 0; q < 3; q++) {
+                L.xe[q] = L.xe[q] + L.hx2 * L.v[q];
+                L.ve[q] = L.ve[q] + L.hx2 * a[q];
+            }
+        } else {
+            for (int q = 0; q < 3; q++) {
+                L.x[q] = L.x[q] + L.hx2 * L.ve[q];
+                L.v[q] = L.v[q] + L.hx2 * a[q];
+            }
+        }
+    } else {
+        const int j = r + 1;
+        double tmp0 = L.h * (double)j;
+        for (int q = 0; q < 3; q++) {
+            L.x[q] = L.x[q] + a[q];
+            L.v[q] = L.v[q] + L.x[q];
+            L.xe[q] = L.h2 * L.v[q] + L.h2by2 * a0[q] + tmp0 * v0[q] + x0[q];
+        }
+    }
+}
+
+// The level's result from the force a at its last evaluation.
+template <int ALG>
+__device__ __forceinline__ void bsw_result(const BswLevel<ALG>& L, const double* a, const double* v0, double* R) {
+    if (ALG == 2) {
+        for (int q = 0; q < 3; q++) {
+            R[q] = .5 * (L.xe[q] + L.x[q] + L.h * L.ve[q]);
+            R[3 + q] = .5 * (L.ve[q] + L.v[q] + L.h * a[q]);
+        }
+    } else {
+        for (int q = 0; q < 3; q++) {
+            R[q] = L.xe[q];
+            R[3 + q] = L.h * L.x[q] + L.hby2 * a[q] + v0[q];
+        }
+    }
+}
+
+// Extrapolation of one component of one body over the levels in the
+// Fortran's order, every level's result already in D (row n-1): per level
+// n > 3, d(1) in E[n-4] and the accepted value in cand.
+template <int ALG>
+__device__ inline void bsw_extrapolate(double h0, double* D, int ds, double* E, double* cand, long long cs) {
+    constexpr int L = BsShape<ALG>::L;
+    double h2[L];
+    for (int n = 1; n <= L; n++) h2[n - 1] = bsw_h2<ALG>(h0, n);
+    for (int n = 1; n <= L; n++) {
+        for (int j = n - 1; j >= 1; j--) {
+            int j1 = j + 1;
+            double tmp0 = 1.0 / (h2[j - 1] - h2[n - 1]);
+            double tmp1 = tmp0 * h2[j1 - 1];
+            double tmp2 = tmp0 * h2[n - 1];
+            D[(j - 1) * 6 * ds] = tmp1 * D[(j1 - 1) * 6 * ds] - tmp2 * D[(j - 1) * 6 * ds];
+        }
+        if (n > 3) {
+            double d1 = D[0];
+            E[n - 4] = d1;
+            double s = d1;
+            for (int j = 2; j <= n; j++) s = s + D[(j - 1) * 6 * ds];
+            cand[(long long)(n - 4) * 6 * cs] = s;
+        }
+    }
+}
+
+// Level errors of one body (mdt_bs1 mercury6_2.for:3348-3362 / mdt_bs2), merged by max.
+__device__ __forceinline__ double bsw_level_err(const double* d1, double xs, double vs, int* fault) {
+    bool fin = true;
+    for (int q = 0; q < 6; q++) fin = fin && isfinite(d1[q]);
+    double tmp1 = fmax(fmax(d1[0] * d1[0], d1[1] * d1[1]), d1[2] * d1[2]);
+    double tmp2 = fmax(fmax(d1[3] * d1[3], d1[4] * d1[4]), d1[5] * d1[5]);
+    if (!fin) atomicOr(fault, FAULT_FORCE);
+    return fin ? fmax(tmp1 * xs, tmp2 * vs) : 0.0;
+}
+
+__device__ __forceinline__ void bsw_scales(const double* x0, const double* v0, double& xs, double& vs) {
+    double t1 = x0[0] * x0[0] + x0[1] * x0[1] + x0[2] * x0[2];
+    double t2 = v0[0] * v0[0] + v0[1] * v0[1] + v0[2] * v0[2];
+    xs = 1.0 / fmax(t1, 1e-300);
+    vs = 1.0 / fmax(t2, 1e-300);
+}
+
+// Big-body accelerations of mfo_all (mfo_grav mercury6_2.for:4616-4690, 1PN,
+// oblateness) for one level's planets at X, V ([3][kBsLanes] each), the
+// arithmetic of ra_planet_forces: thread (level, k) computes planet k+1; lane 0
+// of the level forms the indirect term and the oblateness reaction.  When act,
+// the level's table T for this point is written too.  Every thread of the
+// block calls it (block barriers inside).
+template <bool PN>
+__device__ void bsw_planet_forces(const Sys& d, int np, const double* m, const double* X, const double* V, double* A,
+                                  double* aob, double* hdr, double light2, double* T, int* fault, int k, bool act) {
+    if (act && k < np) {
+        const double* xb = X + 3 * k;
+        double a[3] = {0.0, 0.0, 0.0};
+        for (int i = 0; i < k; i++) {
+            double dx = xb[0] - X[3 * i], dy = xb[1] - X[3 * i + 1], dz = xb[2] - X[3 * i + 2];
+            double s2 = dx * dx + dy * dy + dz * dz;
+            if (!(s2 > 0.0)) atomicOr(fault, FAULT_FORCE);
+            double s_1 = 1.0 / sqrt(s2);
+            double s_3 = s_1 * s_1 * s_1;
+            double tmp1 = s_3 * m[i + 1];
+            a[0] = a[0] - tmp1 * dx; a[1] = a[1] - tmp1 * dy; a[2] = a[2] - tmp1 * dz;
+        }
+        for (int j = k + 1; j < np; j++) {
+            double dx = X[3 * j] - xb[0], dy = X[3 * j + 1] - xb[1], dz = X[3 * j + 2] - xb[2];
+            double s2 = dx * dx + dy * dy + dz * dz;
+            if (!(s2 > 0.0)) atomicOr(fault, FAULT_FORCE);
+            double s_1 = 1.0 / sqrt(s2);
+            double s_3 = s_1 * s_1 * s_1;
+            double tmp2 = s_3 * m[j + 1];
+            a[0] = a[0] + tmp2 * dx; a[1] = a[1] + tmp2 * dy; a[2] = a[2] + tmp2 * dz;
+        }
+        for (int c = 0; c < 3; c++) A[3 * k + c] = a[c];
+        if (d.obl) obl_body(d.mu, d.j2, d.j4, d.j6, xb[0], xb[1], xb[2], aob[3 * k], aob[3 * k + 1], aob[3 * k + 2]);
+    }
+    __syncthreads();
+    if (act && k == 0) {
+        double sx = 0.0, sy = 0.0, sz = 0.0;
+        for (int i = 0; i < np; i++) {
+            const double* xi = X + 3 * i;
+            double s2 = xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2];
+            if (!(s2 > 0.0)) atomicOr(fault, FAULT_FORCE);
+            double s_1 = 1.0 / sqrt(s2);
+            double tmp1 = m[i + 1] * (s_1 * s_1 * s_1);
+            sx = sx - tmp1 * xi[0];
+            sy = sy - tmp1 * xi[1];
+            sz = sz - tmp1 * xi[2];
+        }
+        double acen[3] = {0.0, 0.0, 0.0};
+        if (d.obl)
+            for (int i = 0; i < np; i++) {
+                double tmp4 = m[i + 1] / m[0];
+                for (int c = 0; c < 3; c++) acen[c] = acen[c] - tmp4 * aob[3 * i + c];
+            }
+        hdr[0] = sx; hdr[1] = sy; hdr[2] = sz;
+        hdr[3] = acen[0]; hdr[4] = acen[1]; hdr[5] = acen[2];
+        for (int q = 0; q < 6; q++) T[q] = hdr[q];
+        T[6] = 0.0; T[7] = 0.0;
+    }
+    __syncthreads();
+    if (act && k < np) {
+        const double* xb = X + 3 * k;
+        const double* vb = V + 3 * k;
+        double s2 = xb[0] * xb[0] + xb[1] * xb[1] + xb[2] * xb[2];
+        double s_1 = 1.0 / sqrt(s2);
+        double r3 = s_1 * s_1 * s_1;
+        double tmp1 = m[0] * r3;
+        double* a = A + 3 * k;
+        if (!PN) {
+            for (int c = 0; c < 3; c++) a[c] = a[c] + hdr[c] - tmp1 * xb[c];
+        } else {
+            double rv = xb[0] * vb[0] + xb[1] * vb[1] + xb[2] * vb[2];
+            double v2 = vb[0] * vb[0] + vb[1] * vb[1] + vb[2] * vb[2];
+            double f = tmp1 / light2;
+            double radial = 4.0 * m[0] * s_1 - v2;
+            for (int c = 0; c < 3; c++) a[c] = a[c] + hdr[c] - tmp1 * xb[c] + f * (radial * xb[c] + 4.0 * rv * vb[c]);
+        }
+        if (d.obl)
+            for (int c = 0; c < 3; c++) a[c] = a[c] + (aob[3 * k + c] - hdr[3 + c]);
+        for (int c = 0; c < 3; c++) T[kHdr + kF * k + c] = xb[c];
+        T[kHdr + kF * k + 3] = m[k + 1];
+    }
+    __syncthreads();
+}
+
+// One attempt, planets: one block of L x kBsLanes threads, thread (level n, planet k+1).
+template <int ALG, bool PN>
+__global__ void __launch_bounds__(BsShape<ALG>::L * kBsLanes) bsw_big(BswArgs A) {
+    constexpr int L = BsShape<ALG>::L;
+    constexpr int R = BsShape<ALG>::R;
+    const Sys d = A.sys[0];
+    const int np = d.nbig - 1;
+    const int t = threadIdx.x;
+    const int n = t / kBsLanes + 1;
+    const int k = t % kBsLanes;
+    const bool mem = k < np;
+    const long long cap = A.st.cap;
+    __shared__ double m[kBsLanes + 1];
+    __shared__ double X[L][3 * kBsLanes], V[L][3 * kBsLanes], F[L][3 * kBsLanes], OB[L][3 * kBsLanes], HD[L][8];
+    __shared__ double D[72 * kBsLanes];
+    __shared__ double E[kBsLanes][6][L - 3];
+    if (t <= np) m[t] = A.st.m[d.base + t];
+    double x0[3] = {0, 0, 0}, v0[3] = {0, 0, 0}, a0[3] = {0, 0, 0};
+    if (mem)
+        for (int c = 0; c < 3; c++) {
+            x0[c] = A.st.x[c * cap + d.base + 1 + k];
+            v0[c] = A.st.v[c * cap + d.base + 1 + k];
+            if (n == 1) { X[0][3 * k + c] = x0[c]; V[0][3 * k + c] = v0[c]; }
+        }
+    __syncthreads();
+    // the step start: one evaluation (level 1's lanes), shared by every level
+    bsw_planet_forces<PN>(d, np, m, X[0], V[0], F[0], OB[0], HD[0], A.light2, A.tab + d.tab, A.st.fault, k, n == 1);
+    if (mem)
+        for (int c = 0; c < 3; c++) a0[c] = F[0][3 * k + c];
+    __syncthreads();
+    BswLevel<ALG> Lv;
+    bsw_start<ALG>(Lv, n, A.h0, x0, v0, a0);
+    double a[3] = {0, 0, 0};
+    for (int r = 1; r <= R; r++) {
+        const bool act = r <= bsw_evals<ALG>(n);
+        if (act && mem) {
+            const double *xp, *vp;
+            bsw_point<ALG>(Lv, r, v0, xp, vp);
+            for (int c = 0; c < 3; c++) { X[n - 1][3 * k + c] = xp[c]; V[n - 1][3 * k + c] = vp[c]; }
+        }
+        __syncthreads();
+        bsw_planet_forces<PN>(d, np, m, X[n - 1], V[n - 1], F[n - 1], OB[n - 1], HD[n - 1], A.light2,
+                              A.tab + d.tab + (long long)bsw_slot<ALG>(n, act ? r : 1) * d.ts, A.st.fault, k, act);
+        if (act && mem) {
+            for (int c = 0; c < 3; c++) a[c] = F[n - 1][3 * k + c];
+            if (r < bsw_evals<ALG>(n)) bsw_advance<ALG>(Lv, r, a, x0, v0, a0);
+        }
+    }
+    if (mem) {
+        double Rr[6];
+        bsw_result<ALG>(Lv, a, v0, Rr);
+        for (int q = 0; q < 6; q++) D[(6 * (n - 1) + q) * kBsLanes + k] = Rr[q];
+    }
+    __syncthreads();
+    if (t < 6 * kBsLanes) {
+        const int km = t / 6, q = t % 6;
+        if (km < np) {
+            double e[L - 3];
+            bsw_extrapolate<ALG>(A.h0, D + q * kBsLanes + km, kBsLanes, e,
+                                 A.cand + (long long)q * cap + d.base + 1 + km, cap);
+            for (int l = 0; l < L - 3; l++) E[km][q][l] = e[l];
+        }
+    }
+    __syncthreads();
+    if (t < (L - 3) * kBsLanes) {
+        const int km = t / (L - 3), l = t % (L - 3);
+        if (km < np) {
+            double xm[3], vm[3], xs, vs, d1[6];
+            for (int c = 0; c < 3; c++) {
+                xm[c] = A.st.x[c * cap + d.base + 1 + km];
+                vm[c] = A.st.v[c * cap + d.base + 1 + km];
+            }
+            bsw_scales(xm, vm, xs, vs);
+            for (int q = 0; q < 6; q++) d1[q] = E[km][q][l];
+            double err = bsw_level_err(d1, xs, vs, A.st.fault);
+            atomicMax(A.err + 3 + l, (unsigned long long)__double_as_longlong(err));
+        }
+    }
+}
+
+// One attempt, particles (after bsw_big): kBsPart particles per block, thread
+// (level n, particle slot); the force is the RA15 particle force on the
+// tables bsw_big wrote at the same points.
+template <int ALG, bool PN>
+__global__ void __launch_bounds__(BsShape<ALG>::L * 16) bsw_particles(BswArgs A) {
+    constexpr int L = BsShape<ALG>::L;
+    constexpr int NS = 16;
+    const Sys d = A.sys[0];
+    const int np = d.nbig - 1;
+    const int t = threadIdx.x;
+    const int n = t / NS + 1;
+    const int sl = t % NS;
+    const int npart = d.n - d.nbig;
+    const int pi_ = blockIdx.x * NS + sl;
+    const bool live = pi_ < npart;
+    const long long cap = A.st.cap;
+    const int g = d.base + d.nbig + pi_;
+    __shared__ double D[72 * NS];
+    __shared__ double E[NS][6][L - 3];
+    __shared__ unsigned long long EM[L - 3];
+    if (t < L - 3) EM[t] = 0ull;
+    if (live) {
+        double x0[3], v0[3], a0[3];
+        for (int c = 0; c < 3; c++) { x0[c] = A.st.x[c * cap + g]; v0[c] = A.st.v[c * cap + g]; }
+        grav_small<PN, RangeOps>(d, A.tab + d.tab, np, A.light2, x0[0], x0[1], x0[2], v0[0], v0[1], v0[2], a0[0],
+                                 a0[1], a0[2], A.st.fault);
+        BswLevel<ALG> Lv;
+        bsw_start<ALG>(Lv, n, A.h0, x0, v0, a0);
+        double a[3];
+        const int ne = bsw_evals<ALG>(n);
+        for (int r = 1; r <= ne; r++) {
+            const double *xp, *vp;
+            bsw_point<ALG>(Lv, r, v0, xp, vp);
+            grav_small<PN, RangeOps>(d, A.tab + d.tab + (long long)bsw_slot<ALG>(n, r) * d.ts, np, A.light2, xp[0],
+                                     xp[1], xp[2], vp[0], vp[1], vp[2], a[0], a[1], a[2], A.st.fault);
+            if (r < ne) bsw_advance<ALG>(Lv, r, a, x0, v0, a0);
+        }
+        double Rr[6];
+        bsw_result<ALG>(Lv, a, v0, Rr);
+        for (int q = 0; q < 6; q++) D[(6 * (n - 1) + q) * NS + sl] = Rr[q];
+    }
+    __syncthreads();
+    if (t < 6 * NS) {
+        const int s = t / 6, q = t % 6;
+        const int ps = blockIdx.x * NS + s;
+        if (ps < npart) {
+            double e[L - 3];
+            bsw_extrapolate<ALG>(A.h0, D + q * NS + s, NS, e, A.cand + (long long)q * cap + d.base + d.nbig + ps, cap);
+            for (int l = 0; l < L - 3; l++) E[s][q][l] = e[l];
+        }
+    }
+    __syncthreads();
+    if (t < (L - 3) * NS) {
+        const int s = t / (L - 3), l = t % (L - 3);
+        const int ps = blockIdx.x * NS + s;
+        if (ps < npart) {
+            const int gp = d.base + d.nbig + ps;
+            double xm[3], vm[3], xs, vs, d1[6];
+            for (int c = 0; c < 3; c++) { xm[c] = A.st.x[c * cap + gp]; vm[c] = A.st.v[c * cap + gp]; }
+            bsw_scales(xm, vm, xs, vs);
+            for (int q = 0; q < 6; q++) d1[q] = E[s][q][l];
+            double err = bsw_level_err(d1, xs, vs, A.st.fault);
+            atomicMax(&EM[l], (unsigned long long)__double_as_longlong(err));
+        }
+    }
+    __syncthreads();
+    if (t < L - 3 && EM[t]) atomicMax(A.err + 3 + t, EM[t]);
+}
+
+// The accepted level's states for every body j >= 1, the step-start states
+// kept in ox, ov (the driver's impact redo and the event screen).
+__global__ void bsw_accept(BswArgs A, int level) {
+    const Sys d = A.sys[0];
+    const int j = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    if (j >= d.n) return;
+    const long long cap = A.st.cap;
+    const int g = d.base + j;
+    for (int c = 0; c < 3; c++) {
+        A.st.ox[c * cap + g] = A.st.x[c * cap + g];
+        A.st.ov[c * cap + g] = A.st.v[c * cap + g];
+        A.st.x[c * cap + g] = A.cand[(long long)((level - 4) * 6 + c) * cap + g];
+        A.st.v[c * cap + g] = A.cand[(long long)((level - 4) * 6 + 3 + c) * cap + g];
+    }
+}
+
+}  // namespace mhip

@@ -19,6 +19,7 @@
 #include "radau_device.hpp"
 #include "tangent.hpp"
 #include "hybrid_device.hpp"
+#include "bs_device.hpp"
 
 using namespace mhip;
 
@@ -117,6 +118,10 @@ struct Context {
     Timing timing;
     // RA15 state (algorithm 4): b, e [21][cap], a1 [3][cap], max |b7|
     double *rb = nullptr, *re = nullptr, *ra1 = nullptr;
+    // whole-system Bulirsch-Stoer (algorithms 2, 3): accepted states per level, level errors
+    double* bscand = nullptr;
+    long long bs_cap = 0;
+    unsigned long long* bserr = nullptr;
     unsigned long long* bmax = nullptr;
     long long ra_cap = 0;
     // HYBRID (algorithm 10): changeover radii, pre-drift states, tables, pairs
@@ -204,6 +209,7 @@ void release_all() {
     dfree(c.fault); dfree(c.dsys); dfree(c.tcur); c.sys_valid = false; c.sys_cap = 0; c.systems.clear(); dfree(c.dtiles); dfree(c.dev); dfree(c.devcount);
     dfree(c.dpark); dfree(c.dcnt); dfree(c.dslow); dfree(c.work); dfree(c.dreq); c.req_cap = 0;
     dfree(c.rb); dfree(c.re); dfree(c.ra1); dfree(c.bmax); c.ra_cap = 0;
+    dfree(c.bscand); dfree(c.bserr); c.bs_cap = 0;
     for (double** p : {&c.tape.tab, &c.tape.ck, &c.tape.rec, &c.tape.lam, &c.tape.part, &c.tape.src, &c.tape.grad,
                        &c.tape.rows, &c.tape.tan, &c.tape.fin, &c.tape.seed, &c.tape.psave})
         dfree(*p);
@@ -682,6 +688,80 @@ void radau_step(double time, double* h, double* hdid, double tol, int64_t* force
     *hdid = tdid;
     *h = t;
     c.history = true;
+}
+
+// ---- Bulirsch-Stoer over the whole system (algorithms 2 and 3) ------------
+
+int read_fault();
+void clear_fault();
+
+// One attempt of mdt_bs1 (ALG 2) or mdt_bs2 (ALG 3) at step h0: every level's
+// error comes back; returns the first level n > 3 within tol^2, 0 if none.
+template <int ALG>
+int bs_attempt(double h0, double tol2) {
+    Context& c = ctx;
+    constexpr int L = BsShape<ALG>::L;
+    BswArgs A{c.dsys, device_state(), c.tab, c.bscand, c.bserr, h0, light_speed2()};
+    hipLaunchKernelGGL(zero_bytes, dim3(1), dim3(128), 0, c.stream, reinterpret_cast<unsigned char*>(c.bserr),
+                       (long long)(12 * sizeof(unsigned long long)));
+    if (c.pn) hipLaunchKernelGGL((bsw_big<ALG, true>), dim3(1), dim3(L * kBsLanes), 0, c.stream, A);
+    else hipLaunchKernelGGL((bsw_big<ALG, false>), dim3(1), dim3(L * kBsLanes), 0, c.stream, A);
+    const int npart = c.n - c.nbig;
+    if (npart > 0) {
+        dim3 grid((npart + 15) / 16);
+        if (c.pn) hipLaunchKernelGGL((bsw_particles<ALG, true>), grid, dim3(L * 16), 0, c.stream, A);
+        else hipLaunchKernelGGL((bsw_particles<ALG, false>), grid, dim3(L * 16), 0, c.stream, A);
+    }
+    HIPCHECK(hipGetLastError());
+    unsigned long long bits[12];
+    HIPCHECK(hipMemcpyAsync(bits, c.bserr, sizeof(bits), hipMemcpyDeviceToHost, c.stream));
+    HIPCHECK(hipStreamSynchronize(c.stream));
+    if (read_fault() & FAULT_FORCE) throw std::runtime_error("Nonfinite BS extrapolation");
+    for (int n = 4; n <= L; n++) {
+        double err;
+        std::memcpy(&err, &bits[n - 1], sizeof(err));
+        if (err <= tol2) return n;
+    }
+    return 0;
+}
+
+// mdt_bs1 / mdt_bs2 with mfo_all: attempts at h, halving until a level
+// converges; then the Fortran's step recommendation (x .55 after the last
+// level, x 1.3 before level 7).  The levels run on the device; the host reads
+// the 12 level errors per attempt.
+void bs_step(int alg, double time, double* h, double* hdid, double tol, int64_t* forces, int64_t* rejected) {
+    Context& c = ctx;
+    const double SHRINK = .55, GROW = 1.3;
+    if (c.systems.size() != 1) throw std::runtime_error("BS needs one system");
+    if (c.nbig - 1 > kBsLanes) throw std::runtime_error("more than 16 Big bodies besides the central body");
+    const int P = alg == 2 ? BsShape<2>::P : BsShape<3>::P;
+    build_systems(1 + P, true, *h, 0.0);
+    if (c.bs_cap != c.cap) {
+        dalloc(c.bscand, (size_t)9 * 6 * c.cap);       // levels 4..12
+        if (!c.bserr) dalloc(c.bserr, 12);
+        c.bs_cap = c.cap;
+    }
+    const double tol2 = tol * tol;
+    *forces = 1;
+    for (;;) {
+        if (time + *h == time) throw std::runtime_error("BS cannot advance time at the requested tolerance");
+        clear_fault();
+        const int level = alg == 2 ? bs_attempt<2>(*h, tol2) : bs_attempt<3>(*h, tol2);
+        if (level) {
+            for (int n = 1; n <= level; n++) *forces += alg == 2 ? 2 * n : n;
+            BswArgs A{c.dsys, device_state(), c.tab, c.bscand, c.bserr, *h, light_speed2()};
+            if (c.n > 1) hipLaunchKernelGGL(bsw_accept, dim3((c.n - 1 + 127) / 128), dim3(128), 0, c.stream, A, level);
+            HIPCHECK(hipGetLastError());
+            HIPCHECK(hipStreamSynchronize(c.stream));
+            *hdid = *h;
+            if (level >= 8) *h = *h * SHRINK;
+            if (level < 7) *h = *h * GROW;
+            return;
+        }
+        *forces += P;
+        ++*rejected;
+        *h = *h * .5;
+    }
 }
 
 // ---- differentiation tape -------------------------------------------------
@@ -1369,7 +1449,9 @@ int mercury_hip_available(void) {
 }
 
 int mercury_hip_configure(int algorithm) {
-    if (algorithm != 1 && algorithm != 4 && algorithm != 9 && algorithm != 10) return 1;
+    if (algorithm < 1 || algorithm > 4) {
+        if (algorithm != 9 && algorithm != 10) return 1;
+    }
     if (algorithm != ctx.algorithm) release_all();
     ctx.algorithm = algorithm;
     ctx.history = false;
@@ -1447,6 +1529,13 @@ int mercury_hip_step(double time, double* h, double* hdid, double tol, int64_t* 
             ctx.saved_valid = true;
             return 0;
         }
+        if (ctx.algorithm == 2 || ctx.algorithm == 3) {
+            ensure_stream();
+            ctx.tape.valid = false;
+            bs_step(ctx.algorithm, time, h, hdid, tol, forces, rejected);
+            ctx.saved_valid = true;
+            return 0;
+        }
         if (!mvs_family()) throw std::runtime_error("algorithm not provided by the HIP path");
         clear_fault();
         ctx.tape.valid = false;
@@ -1480,7 +1569,8 @@ int mercury_hip_events(double time, double h, double rcen, const MercuryEvent** 
     try {
         require_state();
         if (!ctx.saved_valid) throw std::runtime_error("no step to screen");
-        const bool ra = ctx.algorithm == 4;
+        // single-system integrators that keep the step-start states in ox, ov
+        const bool ra = ctx.algorithm == 4 || ctx.algorithm == 2 || ctx.algorithm == 3;
         const bool hy = ctx.algorithm == 10;
         if (ctx.enc.active) throw std::runtime_error("event screening inside an encounter context");
         if (!mvs_family() && !ra && !hy) throw std::runtime_error("algorithm not provided by the HIP path");

@@ -52,8 +52,10 @@ def test_host_side_errors_without_device(lib, capfd):
     """Unsupported algorithms, a step before any upload, bad tuning: nonzero
     returns with a message, and no device call is needed to reach them."""
     import ctypes as C
-    assert lib.mercury_hip_configure(2) != 0          # BS: not served
-    assert lib.mercury_hip_configure(3) != 0          # BS2 (outside an encounter context): not served
+    for alg in (0, 5, 6, 7, 8, 11, -1):
+        assert lib.mercury_hip_configure(alg) != 0    # not integrators of the reference's path
+    for alg in (1, 2, 3, 4, 9):
+        assert lib.mercury_hip_configure(alg) == 0    # MVS, BS, BS2, RADAU, TEST
     assert lib.mercury_hip_configure(10) == 0         # HYBRID
     assert lib.mercury_hip_encounter_update(None, None, None) != 0   # no encounter context
     h = C.c_double(4.0); hd = C.c_double(); nf = C.c_int64(); nr = C.c_int64()

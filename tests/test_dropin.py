@@ -97,6 +97,26 @@ def test_dropin_radau_on_device_matches_reference(tmp_path, pn):
 
 @needs_dropin
 @pytest.mark.gpu
+@pytest.mark.parametrize("algorithm", ["BS", "BS2"])
+def test_dropin_bs_on_device_matches_reference(tmp_path, algorithm):
+    """Bulirsch-Stoer (mdt_bs1 / mdt_bs2 with mfo_all) through the unchanged
+    adaptive driver (mal_hvar), steps clipped at every output time; dumps,
+    encounters and output agree with the reference CPU run."""
+    s = systems.config_encounters(seed=3, nparticles=600)
+    s["rce"][1:s["nbig"]] *= 3.0
+    print(f"case: drop-in {algorithm}, 8 planets + 600 particles, 600 d, outputs every 50 d", flush=True)
+    a, b, rb = _pair(tmp_path, s, 300, "hip", algorithm=algorithm, interval=50.0)
+    assert rb.returncode == 0, rb.stdout + rb.stderr
+    assert any("Execution backend: HIP" in ln for ln in R.info_lines(b)), R.info_lines(b)
+    worst = _compare_dumps(a, b)
+    print(f"max relative dump difference {worst:.3e}", flush=True)
+    assert worst <= REL_TOL
+    assert (a / "ce.out").read_bytes() == (b / "ce.out").read_bytes()
+    assert (a / "xv.out").read_bytes() == (b / "xv.out").read_bytes()
+
+
+@needs_dropin
+@pytest.mark.gpu
 def test_dropin_hybrid_on_device_matches_reference(tmp_path):
     """HYBRID through the unchanged driver: regular map and changeover scan on
     the device, the driver's mdt_hkce integrating encounter members through
