@@ -93,6 +93,76 @@ def _cpu_worker(job):
     return out["t"]
 
 
+def _in_big_thread(fn):
+    """Run fn in a thread with a large stack (the Fortran keeps big locals)."""
+    import threading
+    out = {}
+
+    def run():
+        out["r"] = fn()
+
+    threading.stack_size(256 << 20)
+    th = threading.Thread(target=run)
+    th.start()
+    th.join()
+    return out["r"]
+
+
+def _cpu_ra15_worker(job):
+    sub, nseq = job
+
+    def run():
+        from oracle.bindings import Reference
+        R = Reference(nmax=len(sub["m"]) + 16, cmax=len(sub["m"]) + 16)
+        R.ra15_run(sub, 1, pn=1)                       # warm
+        t = time.perf_counter()
+        R.ra15_run(sub, nseq, pn=1)
+        return time.perf_counter() - t
+    return _in_big_thread(run)
+
+
+def cpu_radau(sysd, nproc=16, shard=62500, nseq=8):
+    """The reference mdt_ra15 with mfo_all (tol 1e-12, 1PN) on `nproc` disjoint
+    particle shards of the 1M system in parallel processes.  Each shard takes
+    its own step sizes (the RA15 law uses max |b7| over the bodies it
+    integrates), close to the whole system's.  Returns particle-steps/s."""
+    import multiprocessing as mp
+    import numpy as np
+    from oracle.bindings import REF_PATH
+    if not REF_PATH.exists():
+        return None
+    nb = sysd["nbig"]
+    jobs = []
+    for p in range(nproc):
+        idx = np.concatenate([np.arange(nb), nb + p * shard + np.arange(shard)])
+        sub = {k: sysd[k][idx] for k in ("m", "x", "v", "rce", "rphys")}
+        sub.update(nbig=nb, h=sysd["h"], rcen=sysd["rcen"], jcen=np.zeros(3))
+        jobs.append((sub, nseq))
+    with mp.get_context("spawn").Pool(nproc) as pool:
+        times = pool.map(_cpu_ra15_worker, jobs)
+    return nproc * shard * nseq / max(times), max(times)
+
+
+def cpu_hybrid(nparticles=20_000, nsteps=1):
+    """The reference mdt_hy (with its mdt_hkce BS2 encounter loop) on the same
+    20k-particle encounter system as the GPU's hybrid_rate(20000), one process:
+    the reference integrates one system serially, and splitting its particles
+    would change the encounter set that shares a BS2 step."""
+    from oracle.bindings import REF_PATH, Reference
+    from mercuda_amd import systems
+    if not REF_PATH.exists():
+        return None
+    s = systems.config_encounters(seed=3, nparticles=nparticles)
+
+    def run():
+        R = Reference(nmax=len(s["m"]) + 16, cmax=len(s["m"]) * 8 + 16)
+        t = time.perf_counter()
+        R.mdt_hy_run(s, nsteps)
+        return time.perf_counter() - t
+    sec = _in_big_thread(run)
+    return nparticles * nsteps / sec, sec
+
+
 def ensemble_ratio(hip, steps, spl, block):
     """Batched ensemble (512 systems, 2..12 planets, log-U[100, 5e4] particles,
     dt 4 d) against one 8-planet system with the same total particle count,
@@ -202,6 +272,70 @@ def radau_rate(hip, nparticles=1_000_000, nseq=12):
             "sequences": nseq, "particle_steps_per_s": round(nparticles * nseq / sec, 1),
             "particle_force_evaluations_per_s": round(nparticles * forces / sec, 1),
             "ms_per_sequence": round(sec * 1e3 / nseq, 3), "mean_step_days": round(days / nseq, 4)}
+
+
+def bs_rate(hip, algor, nparticles=1_000_000, nsteps=8):
+    """Algorithm 2 (mdt_bs1) or 3 (mdt_bs2) with mfo_all, tol 1e-12, on the
+    MVS workload's 8 planets + 1M particles, first step 4 d: accepted steps x
+    particles per second through mercury_hip_step (one host round trip per
+    attempt for the step control), after 2 warm-up steps."""
+    from mercuda_amd import systems
+    s = systems.config_large(seed=2, nparticles=nparticles)
+    hip.configure(algor)
+    hip.upload_system(s)
+    h, t = s["h"], 0.0
+    for _ in range(2):
+        h, hd, _, _ = hip.step(t, h, 1e-12)
+        t += hd
+    days = 0.0
+    rej = 0
+    tic = time.perf_counter()
+    for _ in range(nsteps):
+        h, hd, _, r = hip.step(t, h, 1e-12)
+        t += hd
+        days += hd
+        rej += r
+    sec = time.perf_counter() - tic
+    return {"workload": f"{'BS' if algor == 2 else 'BS2'} (algorithm {algor}) tol 1e-12, 8 planets + "
+                        f"{nparticles} particles", "steps": nsteps,
+            "particle_steps_per_s": round(nparticles * nsteps / sec, 1),
+            "ms_per_step": round(sec * 1e3 / nsteps, 3), "mean_step_days": round(days / nsteps, 4),
+            "halvings": rej}
+
+
+def _cpu_bs_worker(job):
+    sub, algor, nsteps = job
+
+    def run():
+        from oracle.bindings import Reference
+        R = Reference(nmax=len(sub["m"]) + 16, cmax=len(sub["m"]) + 16)
+        R.bs_run(algor, sub, 1)                         # warm
+        t = time.perf_counter()
+        R.bs_run(algor, sub, nsteps)
+        return time.perf_counter() - t
+    return _in_big_thread(run)
+
+
+def cpu_bs(sysd, algor, nproc=16, shard=62500, nsteps=8):
+    """The reference mdt_bs1 / mdt_bs2 with mfo_all on `nproc` disjoint particle
+    shards of the 1M system in parallel processes (each shard takes its own
+    step sizes, as the error control spans the bodies it integrates).  Returns
+    particle-steps/s."""
+    import multiprocessing as mp
+    import numpy as np
+    from oracle.bindings import REF_PATH
+    if not REF_PATH.exists():
+        return None
+    nb = sysd["nbig"]
+    jobs = []
+    for p in range(nproc):
+        idx = np.concatenate([np.arange(nb), nb + p * shard + np.arange(shard)])
+        sub = {k: sysd[k][idx] for k in ("m", "x", "v", "rce", "rphys")}
+        sub.update(nbig=nb, h=sysd["h"], rcen=sysd["rcen"], jcen=np.zeros(3))
+        jobs.append((sub, algor, nsteps))
+    with mp.get_context("spawn").Pool(nproc) as pool:
+        times = pool.map(_cpu_bs_worker, jobs)
+    return nproc * shard * nsteps / max(times), max(times)
 
 
 def hybrid_rate(hip, nparticles=200_000, nsteps=3):
@@ -319,8 +453,13 @@ def main():
         extras["adjoint"] = adjoint_cost(hip)
         log("radau: 8 planets + 1M particles")
         extras["radau"] = radau_rate(hip)
+        for algor in (2, 3):
+            log(f"bs: algorithm {algor}, 8 planets + 1M particles")
+            extras[f"bs{algor}"] = bs_rate(hip, algor)
         log("hybrid: 8 planets + 200k particles")
         extras["hybrid"] = hybrid_rate(hip)
+        log("hybrid: 8 planets + 20k particles")
+        extras["hybrid_20k"] = hybrid_rate(hip, nparticles=20_000, nsteps=5)
     cpu = None
     if not args.no_cpu:
         log(f"cpu baseline: reference mdt_mvs on {args.cpu_procs} processes")
@@ -333,6 +472,36 @@ def main():
                              f"(reference mdt_mvs + mce_cent, flang -O3 -ffp-contract=off), "
                              f"slowest process {tmax:.2f} s",
                    "gpu_over_cpu": round(value / rate, 1)}
+        if extras.get("radau"):
+            log(f"cpu baseline: reference mdt_ra15 on {args.cpu_procs} processes")
+            r = cpu_radau(sysd, nproc=args.cpu_procs)
+            if r:
+                extras["radau"]["cpu_baseline"] = {
+                    "value": round(r[0], 1), "unit": "particle-steps/s", "cores": args.cpu_procs, "kind": "reference",
+                    "sample": f"{args.cpu_procs} processes x 62500 particles x 8 RA15 sequences of this system "
+                              f"(tol 1e-12, 1PN; each shard takes its own step sizes), slowest process {r[1]:.2f} s",
+                    "gpu_over_cpu": round(extras["radau"]["particle_steps_per_s"] / r[0], 1)}
+        for algor in (2, 3):
+            if extras.get(f"bs{algor}"):
+                log(f"cpu baseline: reference algorithm {algor} on {args.cpu_procs} processes")
+                r = cpu_bs(sysd, algor, nproc=args.cpu_procs)
+                if r:
+                    extras[f"bs{algor}"]["cpu_baseline"] = {
+                        "value": round(r[0], 1), "unit": "particle-steps/s", "cores": args.cpu_procs,
+                        "kind": "reference",
+                        "sample": f"{args.cpu_procs} processes x 62500 particles x 8 steps of this system "
+                                  f"(mdt_bs{'1' if algor == 2 else '2'} with mfo_all, tol 1e-12; each shard takes "
+                                  f"its own step sizes), slowest process {r[1]:.2f} s",
+                        "gpu_over_cpu": round(extras[f"bs{algor}"]["particle_steps_per_s"] / r[0], 1)}
+        if extras.get("hybrid_20k"):
+            log("cpu baseline: reference mdt_hy on 20k particles, one process")
+            r = cpu_hybrid()
+            if r:
+                extras["hybrid_20k"]["cpu_baseline"] = {
+                    "value": round(r[0], 1), "unit": "particle-steps/s", "cores": 1, "kind": "reference",
+                    "sample": f"1 HYBRID step of the same 20k-particle system (the reference integrates a system "
+                              f"serially), {r[1]:.2f} s",
+                    "gpu_over_cpu": round(extras["hybrid_20k"]["particle_steps_per_s"] / r[0], 1)}
     line = {
         "metric": "FP64 particle-steps/s, MVS 8pl+1M TP; batched ensemble & adjoint/fwd cost",
         "value": round(value, 1),
