@@ -97,6 +97,7 @@ struct RangeOps {
 //   rcp_rn:  |b| in [2^-700, 2^700]
 //   sqrt_rn: x in [2^-767, 2^1000]
 //   divc_rn: |a| in [2^-900, 2^900] over 3, 2*pi and 39916800
+// A quarter of the draws take edge significands (see draw).
 __device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
     z += 0x9e3779b97f4a7c15ull;
     z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
@@ -104,10 +105,23 @@ __device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
     return z ^ (z >> 31);
 }
 
+// A random exponent in [emin, emax]; the significand random, or for one draw
+// in four one of the edge patterns where a Newton reciprocal can round to the
+// wrong neighbour: near all ones, near a power of two, few low bits.
 __device__ __forceinline__ double draw(unsigned long long r, int emin, int emax, bool sign) {
     unsigned long long e = (unsigned long long)(emin + (int)((r >> 52) % (unsigned long long)(emax - emin + 1)) + 1023);
-    unsigned long long bits = (e << 52) | (r & ((1ull << 52) - 1));
-    double v = __longlong_as_double((long long)bits);
+    const unsigned long long mask = (1ull << 52) - 1;
+    unsigned long long f = r & mask;
+    switch ((r >> 62) == 3 ? (int)((r >> 32) & 7) : -1) {
+        case 0: f = mask; break;                               // 1.111...1
+        case 1: f = mask - (r & 0xffff); break;                // just below 2
+        case 2: f = 0; break;                                  // a power of two
+        case 3: f = r & 0xffff; break;                         // just above a power of two
+        case 4: f = (r & 0xff) << 44; break;                   // few leading bits
+        case 5: f = mask ^ ((r & 0xff) << 44); break;          // ones with a few holes
+        default: break;
+    }
+    double v = __longlong_as_double((long long)((e << 52) | f));
     return sign ? -v : v;
 }
 
