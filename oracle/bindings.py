@@ -40,6 +40,8 @@ def build(reference: bool = True) -> None:
     subprocess.run(["make", "-s", "-C", str(HERE), "all"], check=True)
     if reference and REFERENCE_SRC.is_dir():
         subprocess.run(["make", "-s", "-C", str(HERE), "-j4", "ref"], check=True)
+        # The unchanged driver over libmercury_hip (needs the library built first).
+        subprocess.run(["make", "-s", "-C", str(HERE), "-j4", "dropin"], check=True)
 
 
 class _Events(C.Structure):
