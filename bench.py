@@ -368,7 +368,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2048)
-    ap.add_argument("--warmup", type=int, default=128)
+    ap.add_argument("--warmup", type=int, default=1024)
     ap.add_argument("--particles", type=int, default=1_000_000)
     ap.add_argument("--spl", type=int, default=32, help="steps fused per particle launch")
     ap.add_argument("--block", type=int, default=128)

@@ -19,6 +19,7 @@ import numpy as np
 
 HERE = Path(__file__).resolve().parent
 ORC_PATH = HERE / "liborc.so"
+ORC_FMA_PATH = HERE / "liborc_fma.so"   # the same restatement with multiply-adds fused
 REF_PATH = HERE / "_ref" / "libmercury_ref.so"
 REF_BIN = HERE / "_ref" / "mercury6"
 REFERENCE_SRC = Path("/root/reference")

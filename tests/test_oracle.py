@@ -579,3 +579,33 @@ def test_bs_circular_orbit_known_answer(oracle, algor):
         t = t + q
     want = np.stack([r * np.cos(th0 + w * t), r * np.sin(th0 + w * t)], 1)
     assert t > 400.0 and np.abs(xo[1:, :2] - want).max() < 1e-10
+
+
+# ------------------------------------------------- rounding-order sensitivity
+
+def test_fused_multiply_add_breaks_the_parity_bar(oracle):
+    """Why every kernel keeps the Fortran's operation order (-ffp-contract=off,
+    correctly rounded divides and square roots): the oracle rebuilt with
+    multiply-adds fused (oracle/liborc_fma.so) leaves the IEEE-order run by far
+    more than the 1e-11 relative bar after 10^4 MVS steps of the 1,000-particle
+    workload's system (200 of its particles here).  The planets already drift
+    apart by ~1e-10, and close approaches amplify the particles' last-bit
+    differences to order one.  Only bitwise-faithful arithmetic meets the bar."""
+    import subprocess
+    from oracle.bindings import HERE, ORC_FMA_PATH, Oracle
+    if not ORC_FMA_PATH.exists():
+        r = subprocess.run(["make", "-s", "-C", str(HERE), "liborc_fma.so"], capture_output=True, text=True)
+        if r.returncode != 0:
+            pytest.skip("no FMA-capable host compiler: " + r.stderr[-200:])
+    fused = Oracle(ORC_FMA_PATH)
+    s = systems.config_small(seed=1, nparticles=200)
+    x0, v0, _, _ = oracle.mvs_run(s, 10000)
+    x1, v1, _, _ = fused.mvs_run(s, 10000)
+    rel = np.linalg.norm(x1 - x0, axis=1)[1:] / np.linalg.norm(x0, axis=1)[1:]
+    nb = s["nbig"]
+    planets, particles = rel[:nb - 1], rel[nb - 1:]
+    print(f"fused vs IEEE order after 1e4 steps: planets max {planets.max():.2e}, particles max "
+          f"{particles.max():.2e}, {int((particles > 1e-11).sum())}/{particles.size} particles beyond 1e-11")
+    assert planets.max() > 1e-11
+    assert (particles > 1e-11).sum() > particles.size // 10
+
