@@ -387,8 +387,7 @@ constexpr int kBsPoints = 78;   // force points of the 12 modified-midpoint leve
 
 // Step control of the driver's mdt_hkce loop (mercury6_2.for:3883-3905) with
 // mdt_bs2's step rule (mercury6_2.for:3428-3617) on the device, for callers
-// that do not list encounters after every substep: bs2_big and bs2_particles read their
-// step from here, bs2_control decides, bs2_accept applies the chosen level.
+// that do not list encounters after every substep (bs2_loop).
 struct BsCtl {
     double h0;                  // the step the encounter set is integrated over
     double tlocal;              // time done so far
@@ -397,7 +396,6 @@ struct BsCtl {
     double tol2;
     long long forces, rejected, substeps;
     int state;                  // 0 running, 1 done, 2 step below the time's resolution, 3 non-finite
-    int level;                  // level the last control accepted (0: none)
 };
 
 struct BsArgs {
@@ -416,7 +414,6 @@ struct BsArgs {
     double* cand;               // accepted states per level n = 4..12: [9][6][ncap]
     unsigned long long* err;    // max error per level [12] (bits of non-negative doubles)
     int* fault;
-    BsCtl* ctl;                 // device step control (nullptr: h0 as given, the host decides)
 };
 
 // mfo_hkce pair factor (1 - K) / s^3 inside rc, else 0 (mercury6_2.for, mfo_hkce).
@@ -563,23 +560,14 @@ __device__ __forceinline__ double bs_level_err(const double* d1, double xs, doub
     return fin ? fmax(tmp1 * xs, tmp2 * vs) : 0.0;
 }
 
-__device__ __forceinline__ double bs_step_of(const BsArgs& A, bool& stop) {
-    stop = false;
-    if (!A.ctl) return A.h0;
-    if (A.ctl->state != 0) stop = true;      // the loop has ended (uniform)
-    return A.ctl->hlocal;
-}
-
 // One BS2 attempt, Big members: one block of kLevels x kEncBig threads,
 // thread (level n, member k+1).  All levels advance together, one force
 // evaluation per round, with a block barrier between writing the positions
 // and reading them for the pair forces; every position is also written to
 // tp (the Big positions at each force point) for bs2_particles.  Then the
 // per-component extrapolation, the level errors and the accepted states.
-__global__ void __launch_bounds__(kLevels * kEncBig) bs2_big(BsArgs A) {
-    bool stop;
-    const double h0 = bs_step_of(A, stop);
-    if (stop) return;
+__device__ inline void bs2_big_body(const BsArgs& A) {
+    const double h0 = A.h0;
     const int nb = A.nbig - 1;
     const int t = threadIdx.x;
     const int n = t / kEncBig + 1;
@@ -649,11 +637,11 @@ __global__ void __launch_bounds__(kLevels * kEncBig) bs2_big(BsArgs A) {
 // Big positions bs2_big recorded at the same force points.  The d tables live
 // in LDS; then the per-component extrapolation and the level errors, merged
 // over the block before one atomicMax per level.
+__global__ void __launch_bounds__(kLevels * kEncBig) bs2_big(BsArgs A) { bs2_big_body(A); }
+
 constexpr int kBsPart = 16;
-__global__ void __launch_bounds__(kLevels * kBsPart) bs2_particles(BsArgs A) {
-    bool stop;
-    const double h0 = bs_step_of(A, stop);
-    if (stop) return;
+__device__ inline void bs2_particles_body(const BsArgs& A) {
+    const double h0 = A.h0;
     const int t = threadIdx.x;
     const int n = t / kBsPart + 1;
     const int sl = t % kBsPart;
@@ -742,8 +730,9 @@ __global__ void __launch_bounds__(kLevels * kBsPart) bs2_particles(BsArgs A) {
     if (t < 9 && EM[t]) atomicMax(A.err + 3 + t, EM[t]);
 }
 
+__global__ void __launch_bounds__(kLevels * kBsPart) bs2_particles(BsArgs A) { bs2_particles_body(A); }
+
 __global__ void bs2_accept(BsArgs A, int level) {
-    if (A.ctl) level = A.ctl->level;
     if (level < 4) return;
     const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
     if (k >= A.n) return;
@@ -753,21 +742,18 @@ __global__ void bs2_accept(BsArgs A, int level) {
     }
 }
 
-// One thread: after an attempt, take the first level n > 3 whose largest error
-// is within tol^2 (mdt_bs2), or halve the step; after an accepted substep
+// mdt_hkce's step control after one attempt (mercury6_2.for:3883-3905 with
+// mdt_bs2's level choice, 3571-3605): take the first level n > 3 whose
+// largest error is within tol^2, or halve the step; after an accepted substep
 // advance the local time and, unless h0 is done, clamp the next step to what
-// is left (mdt_hkce).  Clears the level errors for the next attempt.  The
-// arithmetic is the host loop's, operation for operation.
-__global__ void bs2_control(BsArgs A) {
-    BsCtl& c = *A.ctl;
-    if (c.state != 0) { c.level = 0; return; }
+// is left.  Returns the accepted level (0: none).  The arithmetic is the host
+// loop's (bs2_step), operation for operation.
+__device__ inline int bs_control_step(BsCtl& c, const unsigned long long* err, bool faulted) {
     const double SHRINK = .55, GROW = 1.3;
-    if (*A.fault & FAULT_FORCE) { c.state = 3; c.level = 0; return; }
+    if (faulted) { c.state = 3; return 0; }
     int level = 0;
     for (int n = 4; n <= 12 && !level; n++)
-        if (__longlong_as_double((long long)A.err[n - 1]) <= c.tol2) level = n;
-    for (int q = 0; q < 12; q++) A.err[q] = 0ull;
-    c.level = level;
+        if (__longlong_as_double((long long)err[n - 1]) <= c.tol2) level = n;
     if (level) {
         for (int n = 1; n <= level; n++) c.forces += n;
         const double hdid = c.hlocal;
@@ -775,7 +761,7 @@ __global__ void bs2_control(BsArgs A) {
         if (level < 7) c.hlocal = c.hlocal * GROW;
         c.tlocal = c.tlocal + hdid;
         c.substeps++;
-        if (!((c.tlocal - c.h0) * c.h0 < 0.0)) { c.state = 1; return; }
+        if (!((c.tlocal - c.h0) * c.h0 < 0.0)) { c.state = 1; return level; }
         c.forces += 1;                       // the next substep's start accelerations
         const double tmp0 = fabs(c.h0) - fabs(c.tlocal);
         c.hrec = c.hlocal;
@@ -786,6 +772,234 @@ __global__ void bs2_control(BsArgs A) {
         c.hlocal = c.hlocal * .5;
     }
     if (c.tlocal + c.hlocal == c.tlocal) c.state = 2;
+    return level;
+}
+
+// Rotating buffers of the device encounter loop.  The state is stored, and
+// the next error set cleared, by bs2_loop_big or block 0 of bs2_loop_fused.
+struct BsLoop {
+    BsCtl* ctl;                  // [2]: attempt q reads ctl[q & 1] and stores ctl[(q + 1) & 1]
+    unsigned long long* err;     // [3][12]: attempt q accumulates into err[q % 3], clears err[(q + 1) % 3]
+    double* cand;                // [2][9][6][ncap]: attempt q writes cand[q & 1]
+};
+
+// The whole mdt_hkce loop runs as two launches per BS2 attempt q, queued in
+// batches with no host synchronisation: bs2_loop_big (one block), then
+// bs2_loop_particles.  Each first makes attempt q-1's control decision from
+// the same inputs -- the state attempt q-1 ran with (ctl[q & 1]), its level
+// errors (err[(q - 1) % 3]) and the fault flag -- so every block of both
+// kernels takes the same one, and applies its accept to its own members (the
+// Big members, or the block's particles); bs2_loop_big alone stores the new
+// state.  Then each runs attempt q exactly as bs2_big / bs2_particles do.
+// Once a decision ends the loop, later launches only carry the state forward.
+__device__ inline int bs_loop_decide(const BsArgs& A, const BsLoop& B, int q, BsCtl& c) {
+    c = B.ctl[q & 1];
+    if (c.state != 0 || q == 0) return 0;
+    return bs_control_step(c, B.err + 12 * ((q - 1) % 3), (*A.fault & FAULT_FORCE) != 0);
+}
+
+// Member j takes its state at attempt q-1's accepted level.
+__device__ inline void bs_loop_accept(const BsArgs& A, const BsLoop& B, int q, int level, int j) {
+    const long long nc = A.ncap;
+    const double* cp = B.cand + (long long)((q + 1) & 1) * 54 * nc + (long long)(level - 4) * 6 * nc;
+    for (int qc = 0; qc < 3; qc++) {
+        A.x[qc * nc + j] = cp[qc * nc + j];
+        A.v[qc * nc + j] = cp[(3 + qc) * nc + j];
+    }
+}
+
+// Attempt q's step, candidate set and error set.
+__device__ inline BsArgs bs_loop_attempt(const BsArgs& A, const BsLoop& B, int q, const BsCtl& c) {
+    BsArgs a = A;
+    a.h0 = c.hlocal;
+    a.cand = B.cand + (long long)(q & 1) * 54 * A.ncap;
+    a.err = B.err + 12 * (q % 3);
+    return a;
+}
+
+__global__ void __launch_bounds__(kLevels * kEncBig) bs2_loop_big(BsArgs A, BsLoop B, int q) {
+    BsCtl c;
+    const int level = bs_loop_decide(A, B, q, c);
+    const int t = threadIdx.x;
+    if (t == 0) B.ctl[(q + 1) & 1] = c;
+    if (t < 12) B.err[12 * ((q + 1) % 3) + t] = 0ull;
+    if (level >= 4 && t < A.nbig - 1) bs_loop_accept(A, B, q, level, 1 + t);
+    if (c.state != 0) return;
+    __syncthreads();                       // accepted states before the attempt reads them
+    bs2_big_body(bs_loop_attempt(A, B, q, c));
+}
+
+__global__ void __launch_bounds__(kLevels * kBsPart) bs2_loop_particles(BsArgs A, BsLoop B, int q) {
+    BsCtl c;
+    const int level = bs_loop_decide(A, B, q, c);
+    const int t = threadIdx.x;
+    const int pi_ = blockIdx.x * kBsPart + t;
+    if (level >= 4 && t < kBsPart && pi_ < A.n - A.nbig) bs_loop_accept(A, B, q, level, A.nbig + pi_);
+    if (c.state != 0) return;
+    __syncthreads();
+    bs2_particles_body(bs_loop_attempt(A, B, q, c));
+}
+
+// The same loop in one launch per attempt, for encounter sets whose particle
+// blocks all fit on the device at once (bs2_loop_fused): every block advances
+// the Big members' 12 levels itself, in LDS, round by round beside its
+// particles -- thread (level n, slot k) is Big member k+1 and particle slot k
+// -- so no launch waits on another for the Big positions.  Block 0 alone
+// applies the Big members' accept, extrapolates them and stores the state;
+// the other blocks read the Big members' accepted states from the candidates
+// (never from x, v, which block 0 may be writing).  Results are bs2_big's and
+// bs2_particles', bit for bit.
+__global__ void __launch_bounds__(kLevels * kBsPart) bs2_loop_fused(BsArgs A, BsLoop B, int q) {
+    static_assert(kBsPart == kEncBig, "slot k is both Big member k+1 and particle slot k");
+    const int t = threadIdx.x;
+    const int n = t / kBsPart + 1;
+    const int k = t % kBsPart;
+    const int nb = A.nbig - 1;
+    const int np = A.n - A.nbig;
+    const bool lead = blockIdx.x == 0;
+    const long long nc = A.ncap;
+    BsCtl c;
+    const int level = bs_loop_decide(A, B, q, c);
+    if (lead && t == 0) B.ctl[(q + 1) & 1] = c;
+    if (lead && t < 12) B.err[12 * ((q + 1) % 3) + t] = 0ull;
+    // the start of attempt q: attempt q-1's accepted states (level >= 4) or the unchanged ones
+    const double* cp =
+        level >= 4 ? B.cand + (long long)((q + 1) & 1) * 54 * nc + (long long)(level - 4) * 6 * nc : nullptr;
+    auto start = [&](int j, int comp) {
+        return level >= 4 ? cp[comp * nc + j] : (comp < 3 ? A.x[comp * nc + j] : A.v[(comp - 3) * nc + j]);
+    };
+    const int pi_ = blockIdx.x * kBsPart + k;
+    const bool live = pi_ < np;
+    const int me = A.nbig + pi_;
+    const bool mem = k < nb;
+    if (level >= 4) {
+        if (n == 1 && live) bs_loop_accept(A, B, q, level, me);
+        if (n == 2 && lead && mem) bs_loop_accept(A, B, q, level, 1 + k);
+    }
+    if (c.state != 0) return;
+    const BsArgs a = bs_loop_attempt(A, B, q, c);
+    const double h0 = a.h0;
+    __shared__ double P[kLevels][3 * kEncBig];
+    __shared__ double X0[3 * kEncBig];
+    __shared__ double D[72 * kBsPart];       // [6 * level + component][slot]
+    __shared__ double E[kBsPart][6][9];
+    __shared__ unsigned long long EM[9];
+    if (t < 9) EM[t] = 0ull;
+    double bx0[3] = {0, 0, 0}, bv0[3] = {0, 0, 0}, ba0[3] = {0, 0, 0};
+    if (mem)
+        for (int qc = 0; qc < 3; qc++) {
+            bx0[qc] = start(1 + k, qc);
+            bv0[qc] = start(1 + k, 3 + qc);
+            if (n == 1) X0[3 * k + qc] = bx0[qc];
+        }
+    double x0[3] = {1, 0, 0}, v0[3] = {0, 1, 0}, a0[3] = {0, 0, 0};
+    double rcj = 0.0;
+    int p0 = 0, p1 = 0;
+    if (live) {
+        for (int qc = 0; qc < 3; qc++) { x0[qc] = start(me, qc); v0[qc] = start(me, 3 + qc); }
+        rcj = A.rcrit[me];
+        p0 = A.poff[pi_];
+        p1 = A.poff[pi_ + 1];
+    }
+    // mfo_hkce for a particle (as bs2_particles); Xb holds the Big members' positions
+    auto pforce = [&](const double* xe, double* acc_out, const double* Xb) {
+        double acc[3] = {0.0, 0.0, 0.0};
+        for (int qq = p0; qq < p1; qq++) {
+            int i = A.pidx[qq];
+            const double* Xi = Xb + 3 * (i - 1);
+            double dx = xe[0] - Xi[0], dy = xe[1] - Xi[1], dz = xe[2] - Xi[2];
+            double tmp2;
+            if (!hkce_factor(dx * dx + dy * dy + dz * dz, fmax(A.rcrit[i], rcj), tmp2)) continue;
+            double faci = tmp2 * A.m[i];
+            acc[0] = acc[0] - faci * dx; acc[1] = acc[1] - faci * dy; acc[2] = acc[2] - faci * dz;
+        }
+        double s2 = xe[0] * xe[0] + xe[1] * xe[1] + xe[2] * xe[2];
+        double s_1 = 1.0 / sqrt(s2);
+        double tmp2 = A.m[0] * s_1 * s_1 * s_1;
+        for (int qc = 0; qc < 3; qc++) acc_out[qc] = acc[qc] - tmp2 * xe[qc];
+    };
+    __syncthreads();
+    if (mem) big_accel(A, k + 1, X0, ba0);
+    if (live) pforce(x0, a0, X0);
+    BsLevel LB, LP;
+    bs_level_start(LB, n, h0, bx0, bv0, ba0);
+    bs_level_start(LP, n, h0, x0, v0, a0);
+    double ab[3] = {0, 0, 0}, ap[3] = {0, 0, 0};
+    for (int r = 1; r <= kLevels; r++) {
+        const bool act = r <= n;
+        if (act && mem)
+            for (int qc = 0; qc < 3; qc++) P[n - 1][3 * k + qc] = LB.xend[qc];
+        __syncthreads();
+        if (act && mem) big_accel(A, k + 1, P[n - 1], ab);
+        if (act && live) pforce(LP.xend, ap, P[n - 1]);
+        __syncthreads();
+        if (act && r < n) {
+            if (mem) bs_level_step(LB, r, ab, bx0, bv0, ba0);
+            if (live) bs_level_step(LP, r, ap, x0, v0, a0);
+        }
+    }
+    // particles: d rows, extrapolation (thread (slot, component)), level errors
+    if (live) {
+        double R[6];
+        bs_level_result(LP, ap, v0, R);
+        for (int qc = 0; qc < 6; qc++) D[(6 * (n - 1) + qc) * kBsPart + k] = R[qc];
+    }
+    __syncthreads();
+    if (t < 6 * kBsPart) {
+        const int sl = t / 6, qc = t % 6;
+        const int ps = blockIdx.x * kBsPart + sl;
+        if (ps < np) {
+            double e[9];
+            bs_extrapolate(h0, D + qc * kBsPart + sl, kBsPart, e, a.cand + (long long)qc * nc + A.nbig + ps, nc);
+            for (int l = 0; l < 9; l++) E[sl][qc][l] = e[l];
+        }
+    }
+    __syncthreads();
+    if (t < 9 * kBsPart) {
+        const int sl = t / 9, l = t % 9;
+        const int ps = blockIdx.x * kBsPart + sl;
+        if (ps < np) {
+            const int mp = A.nbig + ps;
+            double xs, vs, xm[3], vm[3];
+            for (int qc = 0; qc < 3; qc++) { xm[qc] = start(mp, qc); vm[qc] = start(mp, 3 + qc); }
+            bs_scales(xm, vm, xs, vs);
+            double d1[6];
+            for (int qc = 0; qc < 6; qc++) d1[qc] = E[sl][qc][l];
+            double err = bs_level_err(d1, xs, vs, A.fault);
+            atomicMax(&EM[l], (unsigned long long)__double_as_longlong(err));
+        }
+    }
+    __syncthreads();
+    if (t < 9 && EM[t]) atomicMax(a.err + 3 + t, EM[t]);
+    if (!lead) return;
+    // block 0: the Big members' d rows, extrapolation and level errors (as bs2_big)
+    if (mem) {
+        double R[6];
+        bs_level_result(LB, ab, bv0, R);
+        for (int qc = 0; qc < 6; qc++) D[(6 * (n - 1) + qc) * kEncBig + k] = R[qc];
+    }
+    __syncthreads();
+    if (t < 6 * kEncBig) {
+        const int km = t / 6, qc = t % 6;
+        if (km < nb) {
+            double e[9];
+            bs_extrapolate(h0, D + qc * kEncBig + km, kEncBig, e, a.cand + (long long)qc * nc + 1 + km, nc);
+            for (int l = 0; l < 9; l++) E[km][qc][l] = e[l];
+        }
+    }
+    __syncthreads();
+    if (t < 9 * kEncBig) {
+        const int km = t / 9, l = t % 9;
+        if (km < nb) {
+            double xs, vs, xm[3], vm[3];
+            for (int qc = 0; qc < 3; qc++) { xm[qc] = start(1 + km, qc); vm[qc] = start(1 + km, 3 + qc); }
+            bs_scales(xm, vm, xs, vs);
+            double d1[6];
+            for (int qc = 0; qc < 6; qc++) d1[qc] = E[km][qc][l];
+            double err = bs_level_err(d1, xs, vs, A.fault);
+            atomicMax(a.err + 3 + l, (unsigned long long)__double_as_longlong(err));
+        }
+    }
 }
 
 }  // namespace mhip

@@ -1209,11 +1209,11 @@ void encounter_enter(int n, int nbig, const double* m, const double* x, const do
         int cap = std::max(n, 2 * e.ncap);
         for (double** p : {&e.m, &e.rcrit}) dalloc(*p, (size_t)cap);
         for (double** p : {&e.x, &e.v}) dalloc(*p, (size_t)3 * cap);
-        dalloc(e.cand, (size_t)54 * cap);
+        dalloc(e.cand, (size_t)108 * cap);   // two rotating sets of 9 levels x 6 components
         dalloc(e.poff, (size_t)cap + 1);
         e.ncap = cap;
     }
-    if (!e.err) dalloc(e.err, 12);
+    if (!e.err) dalloc(e.err, 36);        // three rotating sets of 12 level errors
     if (!e.tp) dalloc(e.tp, (size_t)kBsPoints * 3 * kEncBig);
     // pair lists: Big-Big pairs in list order; each particle's Big partners
     std::vector<int> bb, off((size_t)(n - nbig) + 1, 0), idx;
@@ -1338,9 +1338,12 @@ void bs2_step(double time, double* h, double* hdid, double tol, int64_t* forces,
 }
 
 // The driver's mdt_hkce loop over one step h0 (mercury6_2.for:3883-3905) with
-// the step control on the device (bs2_control): batches of (attempt, control,
-// accept) are queued without host synchronisation, each later launch a no-op
-// once the loop has ended; the host checks the control block between batches.
+// the step control on the device: per attempt one bs2_loop_fused launch when
+// all its blocks are resident at once, else bs2_loop_big and
+// bs2_loop_particles (leaner blocks, more members per round); queued in
+// batches without host synchronisation (launches after the loop's end only
+// carry the state forward); the host reads the state between batches.
+// MERCURY_HIP_ENC_LOOP=fused or =split pins a variant (both give the same bits).
 // Same sequence of steps and states as calling bs2_step per substep; the
 // encounter listing the driver makes after each substep is not made.
 void encounter_advance(double h0, double* hrec, double tol, int64_t* substeps, int64_t* forces,
@@ -1350,7 +1353,7 @@ void encounter_advance(double h0, double* hrec, double tol, int64_t* substeps, i
     if (!e.active) throw std::runtime_error("no encounter context");
     if (!(h0 != 0) || !std::isfinite(h0) || !std::isfinite(*hrec) || !(tol > 0))
         throw std::runtime_error("invalid encounter step");
-    if (!e.ctl) dalloc(e.ctl, 1);
+    if (!e.ctl) dalloc(e.ctl, 2);
     BsCtl k{};
     k.h0 = h0;
     k.tlocal = 0.0;
@@ -1367,22 +1370,36 @@ void encounter_advance(double h0, double* hrec, double tol, int64_t* substeps, i
     A.n = e.n; A.nbig = e.nbig; A.m = e.m; A.rcrit = e.rcrit; A.x = e.x; A.v = e.v; A.ncap = e.ncap;
     A.bbp = e.bbp; A.nbbp = e.nbbp; A.poff = e.poff; A.pidx = e.pidx;
     A.tol2 = k.tol2;
-    A.tp = e.tp; A.cand = e.cand; A.err = e.err; A.fault = c.fault; A.ctl = e.ctl;
+    A.tp = e.tp; A.cand = e.cand; A.err = e.err; A.fault = c.fault;
+    const BsLoop B{e.ctl, e.err, e.cand};
     clear_fault();
     HIPCHECK(hipMemcpyAsync(e.ctl, &k, sizeof(k), hipMemcpyHostToDevice, c.stream));
     hipLaunchKernelGGL(zero_bytes, dim3(1), dim3(128), 0, c.stream, reinterpret_cast<unsigned char*>(e.err),
-                       (long long)(12 * sizeof(unsigned long long)));
+                       (long long)(36 * sizeof(unsigned long long)));
     HIPCHECK(hipGetLastError());
-    const dim3 gp((npart + kBsPart - 1) / kBsPart), gc((std::max(e.n - 1, 1) + 127) / 128);
+    const dim3 gp((unsigned)std::max((npart + kBsPart - 1) / kBsPart, 1));
+    int per_cu = 0, ncu = 0, dev = 0;
+    HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&bs2_loop_fused),
+                                                          kLevels * kBsPart, 0));
+    HIPCHECK(hipGetDevice(&dev));
+    HIPCHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    bool fused = (long long)gp.x <= (long long)per_cu * ncu;
+    if (const char* v = std::getenv("MERCURY_HIP_ENC_LOOP")) {
+        if (!std::strcmp(v, "fused")) fused = true;
+        else if (!std::strcmp(v, "split")) fused = false;
+    }
+    int q = 0;
     for (int batch = 8;; batch = std::min(2 * batch, 256)) {
-        for (int q = 0; q < batch; q++) {
-            hipLaunchKernelGGL(bs2_big, dim3(1), dim3(kLevels * kEncBig), 0, c.stream, A);
-            if (npart > 0) hipLaunchKernelGGL(bs2_particles, gp, dim3(kLevels * kBsPart), 0, c.stream, A);
-            hipLaunchKernelGGL(bs2_control, dim3(1), dim3(1), 0, c.stream, A);
-            hipLaunchKernelGGL(bs2_accept, gc, dim3(128), 0, c.stream, A, 0);
+        for (int j = 0; j < batch; j++, q++) {
+            if (fused) {
+                hipLaunchKernelGGL(bs2_loop_fused, gp, dim3(kLevels * kBsPart), 0, c.stream, A, B, q);
+                continue;
+            }
+            hipLaunchKernelGGL(bs2_loop_big, dim3(1), dim3(kLevels * kEncBig), 0, c.stream, A, B, q);
+            if (npart > 0) hipLaunchKernelGGL(bs2_loop_particles, gp, dim3(kLevels * kBsPart), 0, c.stream, A, B, q);
         }
         HIPCHECK(hipGetLastError());
-        HIPCHECK(hipMemcpyAsync(&k, e.ctl, sizeof(k), hipMemcpyDeviceToHost, c.stream));
+        HIPCHECK(hipMemcpyAsync(&k, e.ctl + (q & 1), sizeof(k), hipMemcpyDeviceToHost, c.stream));
         HIPCHECK(hipStreamSynchronize(c.stream));
         if (k.state == 3) throw std::runtime_error("Nonfinite BS extrapolation");
         if (k.state == 2) throw std::runtime_error("BS cannot advance time at the requested tolerance");

@@ -89,13 +89,18 @@ def test_hybrid_ragged_counts(hip, oracle, count):
     compare_rows(rows, oracle_rows(evi, evd))
 
 
-def test_device_encounter_loop_equals_per_substep_calls(hip, oracle):
+@pytest.mark.parametrize("variant", ["fused", "split"])
+def test_device_encounter_loop_equals_per_substep_calls(hip, oracle, variant, monkeypatch):
     """mercury_hip_encounter_advance (mdt_hkce's loop with the step control on
     the device) against the same loop driven by one mercury_hip_step per
-    substep: the same substeps and bitwise the same states; and the oracle."""
+    substep: the same substeps and bitwise the same states; and the oracle.
+    Both device variants: one launch per attempt (every block keeps the Big
+    members itself), and the Big members' launch followed by the particles'."""
     from mercuda_amd import hybrid
+    monkeypatch.setenv("MERCURY_HIP_ENC_LOOP", variant)
     s = hybrid_system(5, 2000)
-    say("case: HYBRID encounter loop on the device vs per-substep calls, 8 planets + 2000 particles, 60 steps")
+    say(f"case: HYBRID encounter loop on the device ({variant}) vs per-substep calls, 8 planets + 2000 particles, "
+        "60 steps")
     hybrid.mdt_hkce.substeps = 0
     xd, vd, most = hybrid.hy_run(hip, s, 60)
     nd = hybrid.mdt_hkce.substeps
