@@ -453,8 +453,12 @@ void mvs_enqueue(double time, double h, int nsteps, bool reset, bool screen, boo
     HIPCHECK(hipStreamWaitEvent(c.sp, c.ev_fast[0], 0));
     HIPCHECK(hipStreamWaitEvent(c.sg, c.ev_fast[0], 0));
     const int npart = c.npart;
-    const int gblocks = std::max(1, std::min((npart + c.gen_per_wave - 1) / c.gen_per_wave, 4096));
-    const int rblocks = std::max(1, std::min((c.req_cap + 255) / 256, 1024));
+    // The slow-set and screening kernels grid-stride over counts only the device
+    // knows (a few hundred slow-set members and tens of thousands of requests per
+    // launch at 1M particles); small grids keep their idle workgroups from
+    // competing with the fast kernel's for dispatch.
+    const int gblocks = std::max(1, std::min((npart + c.gen_per_wave - 1) / c.gen_per_wave, 512));
+    const int rblocks = std::max(1, std::min((c.req_cap + 255) / 256, 256));
     double t = time;
     int done = 0;
     const std::vector<int> plan = chunk_plan(nsteps, smax);
