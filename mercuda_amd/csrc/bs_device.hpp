@@ -352,24 +352,32 @@ __global__ void __launch_bounds__(BsShape<ALG>::L * kBsLanes) bsw_big(BswArgs A)
     }
 }
 
-// One attempt, particles (after bsw_big): kBsPart particles per block, thread
-// (level n, particle slot); the force is the RA15 particle force on the
-// tables bsw_big wrote at the same points.
+// One attempt, particles (after bsw_big).  Level n makes evals(n) force
+// evaluations (2n for ALG 2, n for ALG 3), so thread (pair p, particle slot)
+// runs the levels p+1 and L-p one after the other: every thread then makes the
+// same number of evaluations (2(L+1) or L+1) and no lane idles while a longer
+// level of its wave runs.  The force is the RA15 particle force on the tables
+// bsw_big wrote at the same points; each level's arithmetic and its d row are
+// those of a thread of its own.
+constexpr int kBswSlots = 32;      // particles per block of bsw_particles
+template <int ALG> __host__ __device__ constexpr int bsw_threads() { return BsShape<ALG>::L / 2 * kBswSlots; }
+
 template <int ALG, bool PN>
-__global__ void __launch_bounds__(BsShape<ALG>::L * 16) bsw_particles(BswArgs A) {
+__global__ void __launch_bounds__(bsw_threads<ALG>()) bsw_particles(BswArgs A) {
     constexpr int L = BsShape<ALG>::L;
-    constexpr int NS = 16;
+    constexpr int NS = kBswSlots;
+    static_assert(L % 2 == 0, "levels pair up");
     const Sys d = A.sys[0];
     const int np = d.nbig - 1;
     const int t = threadIdx.x;
-    const int n = t / NS + 1;
+    const int pair = t / NS;
     const int sl = t % NS;
     const int npart = d.n - d.nbig;
     const int pi_ = blockIdx.x * NS + sl;
     const bool live = pi_ < npart;
     const long long cap = A.st.cap;
     const int g = d.base + d.nbig + pi_;
-    __shared__ double D[72 * NS];
+    __shared__ double D[6 * L * NS];
     __shared__ double E[NS][6][L - 3];
     __shared__ unsigned long long EM[L - 3];
     if (t < L - 3) EM[t] = 0ull;
@@ -378,34 +386,47 @@ __global__ void __launch_bounds__(BsShape<ALG>::L * 16) bsw_particles(BswArgs A)
         for (int c = 0; c < 3; c++) { x0[c] = A.st.x[c * cap + g]; v0[c] = A.st.v[c * cap + g]; }
         grav_small<PN, RangeOps>(d, A.tab + d.tab, np, A.light2, x0[0], x0[1], x0[2], v0[0], v0[1], v0[2], a0[0],
                                  a0[1], a0[2], A.st.fault);
-        BswLevel<ALG> Lv;
-        bsw_start<ALG>(Lv, n, A.h0, x0, v0, a0);
-        double a[3];
-        const int ne = bsw_evals<ALG>(n);
-        for (int r = 1; r <= ne; r++) {
-            const double *xp, *vp;
-            bsw_point<ALG>(Lv, r, v0, xp, vp);
-            grav_small<PN, RangeOps>(d, A.tab + d.tab + (long long)bsw_slot<ALG>(n, r) * d.ts, np, A.light2, xp[0],
-                                     xp[1], xp[2], vp[0], vp[1], vp[2], a[0], a[1], a[2], A.st.fault);
-            if (r < ne) bsw_advance<ALG>(Lv, r, a, x0, v0, a0);
+#pragma unroll 1
+        for (int k = 0; k < 2; k++) {
+            const int n = k == 0 ? pair + 1 : L - pair;
+            BswLevel<ALG> Lv;
+            bsw_start<ALG>(Lv, n, A.h0, x0, v0, a0);
+            double a[3];
+            const int ne = bsw_evals<ALG>(n);
+            for (int r = 1; r <= ne; r++) {
+                const double *xp, *vp;
+                bsw_point<ALG>(Lv, r, v0, xp, vp);
+                grav_small<PN, RangeOps>(d, A.tab + d.tab + (long long)bsw_slot<ALG>(n, r) * d.ts, np, A.light2,
+                                         xp[0], xp[1], xp[2], vp[0], vp[1], vp[2], a[0], a[1], a[2], A.st.fault);
+                if (r < ne) bsw_advance<ALG>(Lv, r, a, x0, v0, a0);
+            }
+            double Rr[6];
+            bsw_result<ALG>(Lv, a, v0, Rr);
+            for (int q = 0; q < 6; q++) D[(6 * (n - 1) + q) * NS + sl] = Rr[q];
         }
-        double Rr[6];
-        bsw_result<ALG>(Lv, a, v0, Rr);
-        for (int q = 0; q < 6; q++) D[(6 * (n - 1) + q) * NS + sl] = Rr[q];
     }
     __syncthreads();
-    if (t < 6 * NS) {
-        const int s = t / 6, q = t % 6;
+    // extrapolation: work item (slot, component).  One pass where the block has
+    // a thread per item (a loop there would let the compiler hoist the
+    // extrapolation's coefficients into registers across iterations).
+    auto extrapolate = [&](int w) {
+        const int s = w / 6, q = w % 6;
         const int ps = blockIdx.x * NS + s;
         if (ps < npart) {
             double e[L - 3];
             bsw_extrapolate<ALG>(A.h0, D + q * NS + s, NS, e, A.cand + (long long)q * cap + d.base + d.nbig + ps, cap);
             for (int l = 0; l < L - 3; l++) E[s][q][l] = e[l];
         }
+    };
+    if constexpr (6 * NS <= bsw_threads<ALG>()) {
+        if (t < 6 * NS) extrapolate(t);
+    } else {
+        for (int w = t; w < 6 * NS; w += bsw_threads<ALG>()) extrapolate(w);
     }
     __syncthreads();
-    if (t < (L - 3) * NS) {
-        const int s = t / (L - 3), l = t % (L - 3);
+    // level errors: work item (slot, level 4 + l), merged per level over the block
+    for (int w = t; w < (L - 3) * NS; w += bsw_threads<ALG>()) {
+        const int s = w / (L - 3), l = w % (L - 3);
         const int ps = blockIdx.x * NS + s;
         if (ps < npart) {
             const int gp = d.base + d.nbig + ps;

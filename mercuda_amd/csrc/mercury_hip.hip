@@ -712,9 +712,9 @@ int bs_attempt(double h0, double tol2) {
     else hipLaunchKernelGGL((bsw_big<ALG, false>), dim3(1), dim3(L * kBsLanes), 0, c.stream, A);
     const int npart = c.n - c.nbig;
     if (npart > 0) {
-        dim3 grid((npart + 15) / 16);
-        if (c.pn) hipLaunchKernelGGL((bsw_particles<ALG, true>), grid, dim3(L * 16), 0, c.stream, A);
-        else hipLaunchKernelGGL((bsw_particles<ALG, false>), grid, dim3(L * 16), 0, c.stream, A);
+        dim3 grid((npart + kBswSlots - 1) / kBswSlots);
+        if (c.pn) hipLaunchKernelGGL((bsw_particles<ALG, true>), grid, dim3(bsw_threads<ALG>()), 0, c.stream, A);
+        else hipLaunchKernelGGL((bsw_particles<ALG, false>), grid, dim3(bsw_threads<ALG>()), 0, c.stream, A);
     }
     HIPCHECK(hipGetLastError());
     unsigned long long bits[12];
