@@ -632,25 +632,32 @@ __device__ inline void bs2_big_body(const BsArgs& A) {
     }
 }
 
-// One BS2 attempt, particle members (after bs2_big): kBsPart particles per
-// block, thread (level n, particle slot); each level's evaluations use the
-// Big positions bs2_big recorded at the same force points.  The d tables live
-// in LDS; then the per-component extrapolation and the level errors, merged
-// over the block before one atomicMax per level.
 __global__ void __launch_bounds__(kLevels * kEncBig) bs2_big(BsArgs A) { bs2_big_body(A); }
 
-constexpr int kBsPart = 16;
+// One BS2 attempt, particle members (after bs2_big): kBsPairSlots particles
+// per block.  Level n makes n force evaluations, so thread (pair p, slot) runs
+// the levels p+1 and 12-p one after the other (13 evaluations each): no lane
+// idles while a longer level of its wave runs, and a block holds twice the
+// particles.  Each level's evaluations use the Big positions bs2_big recorded
+// at the same force points; its arithmetic and d row are those of a thread of
+// its own.  The d tables live in LDS; then the per-component extrapolation
+// and the level errors, merged over the block before one atomicMax per level.
+constexpr int kBsPart = 16;                 // particle slots of bs2_loop_fused's blocks
+constexpr int kBsPairSlots = 32;            // particles per block of bs2_particles
+constexpr int kBsPairThreads = kLevels / 2 * kBsPairSlots;
 __device__ inline void bs2_particles_body(const BsArgs& A) {
+    constexpr int NS = kBsPairSlots;
+    static_assert(6 * NS == kBsPairThreads, "one extrapolation item per thread");
     const double h0 = A.h0;
     const int t = threadIdx.x;
-    const int n = t / kBsPart + 1;
-    const int sl = t % kBsPart;
+    const int pair = t / NS;
+    const int sl = t % NS;
     const int np = A.n - A.nbig;
-    const int pi_ = blockIdx.x * kBsPart + sl;
+    const int pi_ = blockIdx.x * NS + sl;
     const bool live = pi_ < np;
     const int me = A.nbig + pi_;
-    __shared__ double D[72 * kBsPart];       // [6 * level + component][slot]
-    __shared__ double E[kBsPart][6][9];
+    __shared__ double D[72 * NS];            // [6 * level + component][slot]
+    __shared__ double E[NS][6][9];
     __shared__ unsigned long long EM[9];
     if (t < 9) EM[t] = 0ull;
     double x0[3] = {1, 0, 0}, v0[3] = {0, 1, 0}, a0[3] = {0, 0, 0};
@@ -686,35 +693,38 @@ __device__ inline void bs2_particles_body(const BsArgs& A) {
             for (int c = 0; c < 3; c++) xb[c] = A.x[c * A.ncap + i];
             return (const double*)xb;
         });
-        BsLevel L;
-        bs_level_start(L, n, h0, x0, v0, a0);
-        double a[3];
-        for (int r = 1; r <= n; r++) {
-            const double* T = A.tp + bs_point(n, r) * 3 * kEncBig;
-            pforce(L.xend, a, [&](int i) { return T + 3 * (i - 1); });
-            if (r < n) bs_level_step(L, r, a, x0, v0, a0);
+#pragma unroll 1
+        for (int k = 0; k < 2; k++) {
+            const int n = k == 0 ? pair + 1 : kLevels - pair;
+            BsLevel L;
+            bs_level_start(L, n, h0, x0, v0, a0);
+            double a[3];
+            for (int r = 1; r <= n; r++) {
+                const double* T = A.tp + bs_point(n, r) * 3 * kEncBig;
+                pforce(L.xend, a, [&](int i) { return T + 3 * (i - 1); });
+                if (r < n) bs_level_step(L, r, a, x0, v0, a0);
+            }
+            double R[6];
+            bs_level_result(L, a, v0, R);
+            for (int q = 0; q < 6; q++) D[(6 * (n - 1) + q) * NS + sl] = R[q];
         }
-        double R[6];
-        bs_level_result(L, a, v0, R);
-        for (int q = 0; q < 6; q++) D[(6 * (n - 1) + q) * kBsPart + sl] = R[q];
     }
     __syncthreads();
     // extrapolation: thread (slot, component)
-    if (t < 6 * kBsPart) {
+    {
         const int s = t / 6, q = t % 6;
-        const int ps = blockIdx.x * kBsPart + s;
+        const int ps = blockIdx.x * NS + s;
         if (ps < np) {
             double e[9];
-            bs_extrapolate(h0, D + q * kBsPart + s, kBsPart, e, A.cand + (long long)q * A.ncap + A.nbig + ps,
-                           A.ncap);
+            bs_extrapolate(h0, D + q * NS + s, NS, e, A.cand + (long long)q * A.ncap + A.nbig + ps, A.ncap);
             for (int l = 0; l < 9; l++) E[s][q][l] = e[l];
         }
     }
     __syncthreads();
-    // level errors: thread (slot, level 4 + l), merged per level over the block
-    if (t < 9 * kBsPart) {
-        const int s = t / 9, l = t % 9;
-        const int ps = blockIdx.x * kBsPart + s;
+    // level errors: item (slot, level 4 + l), merged per level over the block
+    for (int w = t; w < 9 * NS; w += kBsPairThreads) {
+        const int s = w / 9, l = w % 9;
+        const int ps = blockIdx.x * NS + s;
         if (ps < np) {
             const int mp = A.nbig + ps;
             double xs, vs, xm[3], vm[3];
@@ -730,7 +740,7 @@ __device__ inline void bs2_particles_body(const BsArgs& A) {
     if (t < 9 && EM[t]) atomicMax(A.err + 3 + t, EM[t]);
 }
 
-__global__ void __launch_bounds__(kLevels * kBsPart) bs2_particles(BsArgs A) { bs2_particles_body(A); }
+__global__ void __launch_bounds__(kBsPairThreads) bs2_particles(BsArgs A) { bs2_particles_body(A); }
 
 __global__ void bs2_accept(BsArgs A, int level) {
     if (level < 4) return;
@@ -829,12 +839,12 @@ __global__ void __launch_bounds__(kLevels * kEncBig) bs2_loop_big(BsArgs A, BsLo
     bs2_big_body(bs_loop_attempt(A, B, q, c));
 }
 
-__global__ void __launch_bounds__(kLevels * kBsPart) bs2_loop_particles(BsArgs A, BsLoop B, int q) {
+__global__ void __launch_bounds__(kBsPairThreads) bs2_loop_particles(BsArgs A, BsLoop B, int q) {
     BsCtl c;
     const int level = bs_loop_decide(A, B, q, c);
     const int t = threadIdx.x;
-    const int pi_ = blockIdx.x * kBsPart + t;
-    if (level >= 4 && t < kBsPart && pi_ < A.n - A.nbig) bs_loop_accept(A, B, q, level, A.nbig + pi_);
+    const int pi_ = blockIdx.x * kBsPairSlots + t;
+    if (level >= 4 && t < kBsPairSlots && pi_ < A.n - A.nbig) bs_loop_accept(A, B, q, level, A.nbig + pi_);
     if (c.state != 0) return;
     __syncthreads();
     bs2_particles_body(bs_loop_attempt(A, B, q, c));

@@ -1311,8 +1311,8 @@ void bs2_step(double time, double* h, double* hdid, double tol, int64_t* forces,
                            (long long)(12 * sizeof(unsigned long long)));
         hipLaunchKernelGGL(bs2_big, dim3(1), dim3(kLevels * kEncBig), 0, c.stream, A);
         if (npart > 0)
-            hipLaunchKernelGGL(bs2_particles, dim3((npart + kBsPart - 1) / kBsPart), dim3(kLevels * kBsPart), 0,
-                               c.stream, A);
+            hipLaunchKernelGGL(bs2_particles, dim3((npart + kBsPairSlots - 1) / kBsPairSlots), dim3(kBsPairThreads),
+                               0, c.stream, A);
         HIPCHECK(hipGetLastError());
         unsigned long long bits[12];
         HIPCHECK(hipMemcpyAsync(bits, e.err, sizeof(bits), hipMemcpyDeviceToHost, c.stream));
@@ -1381,13 +1381,14 @@ void encounter_advance(double h0, double* hrec, double tol, int64_t* substeps, i
     hipLaunchKernelGGL(zero_bytes, dim3(1), dim3(128), 0, c.stream, reinterpret_cast<unsigned char*>(e.err),
                        (long long)(36 * sizeof(unsigned long long)));
     HIPCHECK(hipGetLastError());
-    const dim3 gp((unsigned)std::max((npart + kBsPart - 1) / kBsPart, 1));
+    const dim3 gf((unsigned)std::max((npart + kBsPart - 1) / kBsPart, 1));
+    const dim3 gp((unsigned)std::max((npart + kBsPairSlots - 1) / kBsPairSlots, 1));
     int per_cu = 0, ncu = 0, dev = 0;
     HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&bs2_loop_fused),
                                                           kLevels * kBsPart, 0));
     HIPCHECK(hipGetDevice(&dev));
     HIPCHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    bool fused = (long long)gp.x <= (long long)per_cu * ncu;
+    bool fused = (long long)gf.x <= (long long)per_cu * ncu;
     if (const char* v = std::getenv("MERCURY_HIP_ENC_LOOP")) {
         if (!std::strcmp(v, "fused")) fused = true;
         else if (!std::strcmp(v, "split")) fused = false;
@@ -1396,11 +1397,11 @@ void encounter_advance(double h0, double* hrec, double tol, int64_t* substeps, i
     for (int batch = 8;; batch = std::min(2 * batch, 256)) {
         for (int j = 0; j < batch; j++, q++) {
             if (fused) {
-                hipLaunchKernelGGL(bs2_loop_fused, gp, dim3(kLevels * kBsPart), 0, c.stream, A, B, q);
+                hipLaunchKernelGGL(bs2_loop_fused, gf, dim3(kLevels * kBsPart), 0, c.stream, A, B, q);
                 continue;
             }
             hipLaunchKernelGGL(bs2_loop_big, dim3(1), dim3(kLevels * kEncBig), 0, c.stream, A, B, q);
-            if (npart > 0) hipLaunchKernelGGL(bs2_loop_particles, gp, dim3(kLevels * kBsPart), 0, c.stream, A, B, q);
+            if (npart > 0) hipLaunchKernelGGL(bs2_loop_particles, gp, dim3(kBsPairThreads), 0, c.stream, A, B, q);
         }
         HIPCHECK(hipGetLastError());
         HIPCHECK(hipMemcpyAsync(&k, e.ctl + (q & 1), sizeof(k), hipMemcpyDeviceToHost, c.stream));
