@@ -59,6 +59,57 @@ def test_bs2_compact_matches_oracle(hip, oracle):
         assert np.array_equal(xd[1:], xo[1:]) and np.array_equal(vd[1:], vo[1:])
 
 
+@pytest.mark.parametrize("variant", ["fused", "split"])
+@pytest.mark.parametrize("nparticles", [0, 40])
+def test_device_encounter_loop_compact_sets(hip, variant, nparticles, monkeypatch):
+    """The device encounter loop on hand-made sets with a Big-Big pair: two
+    planets inside each other's changeover radius, with no particles (a set of
+    Big members only) or with 40 particles around the first.  Same substeps,
+    step recommendation and bitwise states as mdt_hkce's loop of one
+    mercury_hip_step per substep (mercury6_2.for:3883-3905)."""
+    import math
+    monkeypatch.setenv("MERCURY_HIP_ENC_LOOP", variant)
+    s = hybrid_system(31, 0, near=0.0)
+    rng = np.random.default_rng(5)
+    n = 3 + nparticles
+    m = np.concatenate([s["m"][:3], np.zeros(nparticles)])
+    x = np.zeros((n, 3)); v = np.zeros((n, 3))
+    x[1] = s["x"][1]; v[1] = s["v"][1]
+    x[2] = x[1] + np.array([0.01, -0.004, 0.002]); v[2] = v[1] + np.array([1e-4, 2e-4, -5e-5])
+    rc = np.zeros(n)
+    rc[1:3] = 0.05
+    rc[3:] = 0.01
+    if nparticles:
+        x[3:] = x[1] + rng.normal(0, 0.02, (nparticles, 3))
+        v[3:] = v[1] + rng.normal(0, 2e-3, (nparticles, 3))
+    pairs = [(2, 3)] + [(2, k) for k in range(4, n + 1)]
+    hip.configure(10)
+    hip.upload_system(s)
+    h0, hrec0, tol = 40.0, 40.0, 1e-12
+    say(f"case: compact encounter set, 2 planets + {nparticles} particles, loop {variant}, h0 {h0}")
+    hip.encounter_enter(m, x, v, rc, np.zeros(n), np.zeros(n), 3, pairs)
+    hrec_d, subs_d, _, _ = hip.encounter_advance(h0, hrec0, tol)
+    xd, vd = hip.encounter_download()
+    hip.encounter_exit()
+    hip.encounter_enter(m, x, v, rc, np.zeros(n), np.zeros(n), 3, pairs)
+    tlocal, hlocal, hrec, subs = 0.0, math.copysign(hrec0, h0), hrec0, 0
+    while True:
+        tmp0 = abs(h0) - abs(tlocal)
+        hrec = hlocal
+        if abs(hlocal) > tmp0:
+            hlocal = math.copysign(tmp0, h0)
+        hlocal, hdid, _, _ = hip.step(tlocal, hlocal, tol)
+        tlocal += hdid
+        subs += 1
+        if not ((tlocal - h0) * h0 < 0.0):
+            break
+    xh, vh = hip.encounter_download()
+    hip.encounter_exit()
+    say(f"substeps device {subs_d}, per call {subs}; recommendation {hrec_d} / {hrec}")
+    assert subs_d == subs > 1 and hrec_d == hrec
+    assert np.array_equal(xd, xh) and np.array_equal(vd, vh)
+
+
 def test_hybrid_encounters_match_oracle(hip, oracle):
     s = hybrid_system(3, 300)
     say("case: HYBRID, 8 planets + 300 particles (half near planet orbits), 150 steps of 2 d")
