@@ -225,14 +225,72 @@ __device__ __forceinline__ void merge_bmax(unsigned long long* bmax, double v, i
     if (__lane_id() == 0) atomicMax(bmax, bits);
 }
 
+// Pair terms of the planets' mutual forces computed once per pair, in
+// parallel over the wave, for systems of up to kPairPlanets planets: s^-3 and
+// the separation x_j - x_i (i < j), which is the dx of both bodies' terms in
+// mfo_grav, so the sums below give the same bits as computing each in its lane.
+constexpr int kPairPlanets = 16;
+constexpr int kPairMax = kPairPlanets * (kPairPlanets - 1) / 2;
+
+struct RaPairs {
+    double* s3;            // [kPairMax]
+    double* dx;            // [3][kPairMax]
+    double* rc;            // [kMaxBig] 1/|x| of each planet (indirect and central terms)
+    const unsigned char* pi;   // pair q = (pi[q], pj[q]), i < j, in row order
+    const unsigned char* pj;
+};
+
+__device__ __forceinline__ int ra_pair_index(int np, int i, int j) { return i * np - i * (i + 1) / 2 + (j - i - 1); }
+
 // Big-body accelerations of mfo_grav (mercury6_2.for:4616-4690) for the
 // planets at X (LDS): lane p computes body p+1; lane 0 also writes the
 // table header (indirect term, oblateness reaction) and the force block.
+// With pair terms (np <= kPairPlanets) the separations, s^-3 and 1/|x| come
+// from one parallel pass; each lane then sums its partners in the same order
+// (i < p subtracted, then j > p added) with the same operations.
 template <bool PN>
 __device__ void ra_planet_forces(const Sys& d, int np, const double* m, const double* X, const double* V,
-                                 double* A, double* hdr, double* aob, double light2, double* T, int* fault) {
+                                 double* A, double* hdr, double* aob, double light2, double* T, int* fault,
+                                 const RaPairs* P = nullptr) {
     const int lane = threadIdx.x;
-    if (lane < np) {
+    if (P) {
+        const int npairs = np * (np - 1) / 2;
+        for (int q = lane; q < npairs; q += blockDim.x) {
+            const int i = P->pi[q], j = P->pj[q];
+            double dx = X[3 * j] - X[3 * i], dy = X[3 * j + 1] - X[3 * i + 1], dz = X[3 * j + 2] - X[3 * i + 2];
+            double s2 = dx * dx + dy * dy + dz * dz;
+            if (!(s2 > 0.0)) atomicOr(fault, FAULT_FORCE);
+            double s_1 = 1.0 / sqrt(s2);
+            P->s3[q] = s_1 * s_1 * s_1;
+            P->dx[q] = dx; P->dx[kPairMax + q] = dy; P->dx[2 * kPairMax + q] = dz;
+        }
+        if (lane < np) {
+            const double* xb = X + 3 * lane;
+            double s2 = xb[0] * xb[0] + xb[1] * xb[1] + xb[2] * xb[2];
+            if (!(s2 > 0.0)) atomicOr(fault, FAULT_FORCE);
+            P->rc[lane] = 1.0 / sqrt(s2);
+        }
+        __syncthreads();
+        if (lane < np) {
+            const double* xb = X + 3 * lane;
+            double a[3] = {0.0, 0.0, 0.0};
+            for (int i = 0; i < lane; i++) {
+                const int q = ra_pair_index(np, i, lane);
+                double tmp1 = P->s3[q] * m[i + 1];
+                a[0] = a[0] - tmp1 * P->dx[q]; a[1] = a[1] - tmp1 * P->dx[kPairMax + q];
+                a[2] = a[2] - tmp1 * P->dx[2 * kPairMax + q];
+            }
+            for (int j = lane + 1; j < np; j++) {
+                const int q = ra_pair_index(np, lane, j);
+                double tmp2 = P->s3[q] * m[j + 1];
+                a[0] = a[0] + tmp2 * P->dx[q]; a[1] = a[1] + tmp2 * P->dx[kPairMax + q];
+                a[2] = a[2] + tmp2 * P->dx[2 * kPairMax + q];
+            }
+            for (int c = 0; c < 3; c++) A[3 * lane + c] = a[c];
+            if (d.obl) obl_body(d.mu, d.j2, d.j4, d.j6, xb[0], xb[1], xb[2], aob[3 * lane], aob[3 * lane + 1],
+                                aob[3 * lane + 2]);
+        }
+    } else if (lane < np) {
         const double* xb = X + 3 * lane;
         double a[3] = {0.0, 0.0, 0.0};
         for (int i = 0; i < lane; i++) {
@@ -262,9 +320,14 @@ __device__ void ra_planet_forces(const Sys& d, int np, const double* m, const do
         double sx = 0.0, sy = 0.0, sz = 0.0;
         for (int i = 0; i < np; i++) {
             const double* xi = X + 3 * i;
-            double s2 = xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2];
-            if (!(s2 > 0.0)) atomicOr(fault, FAULT_FORCE);
-            double s_1 = 1.0 / sqrt(s2);
+            double s_1;
+            if (P) {
+                s_1 = P->rc[i];
+            } else {
+                double s2 = xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2];
+                if (!(s2 > 0.0)) atomicOr(fault, FAULT_FORCE);
+                s_1 = 1.0 / sqrt(s2);
+            }
             double tmp1 = m[i + 1] * (s_1 * s_1 * s_1);
             sx = sx - tmp1 * xi[0];
             sy = sy - tmp1 * xi[1];
@@ -285,8 +348,13 @@ __device__ void ra_planet_forces(const Sys& d, int np, const double* m, const do
     if (lane < np) {
         const double* xb = X + 3 * lane;
         const double* vb = V + 3 * lane;
-        double s2 = xb[0] * xb[0] + xb[1] * xb[1] + xb[2] * xb[2];
-        double s_1 = 1.0 / sqrt(s2);
+        double s_1;
+        if (P) {
+            s_1 = P->rc[lane];
+        } else {
+            double s2 = xb[0] * xb[0] + xb[1] * xb[1] + xb[2] * xb[2];
+            s_1 = 1.0 / sqrt(s2);
+        }
         double r3 = s_1 * s_1 * s_1;
         double tmp1 = m[0] * r3;
         double* a = A + 3 * lane;
@@ -315,7 +383,14 @@ __global__ void __launch_bounds__(64) ra_planets(RaArgs args) {
     const long long cap = args.st.cap;
     __shared__ double m[kMaxBig], X[3 * kMaxBig], V[3 * kMaxBig], A[3 * kMaxBig], aob[3 * kMaxBig], hdr[8];
     __shared__ double X1[3 * kMaxBig], V1[3 * kMaxBig], A1[3 * kMaxBig];
+    __shared__ double ps3[kPairMax], pdx[3 * kPairMax], prc[kMaxBig];
+    __shared__ unsigned char ppi[kPairMax], ppj[kPairMax];
     for (int q = lane; q < d.nbig; q += 64) m[q] = args.st.m[d.base + q];
+    if (lane == 0 && np <= kPairPlanets)
+        for (int i = 0, q = 0; i < np; i++)
+            for (int j = i + 1; j < np; j++, q++) { ppi[q] = (unsigned char)i; ppj[q] = (unsigned char)j; }
+    const RaPairs pairs{ps3, pdx, prc, ppi, ppj};
+    const RaPairs* P = np <= kPairPlanets ? &pairs : nullptr;
     double b[3][7], g[3][7];
     const int gb = d.base + 1 + lane;
     const bool iter = args.niter > 0;
@@ -333,7 +408,7 @@ __global__ void __launch_bounds__(64) ra_planets(RaArgs args) {
         }
     }
     __syncthreads();
-    ra_planet_forces<PN>(d, np, m, X, V, A, hdr, aob, args.light2, args.tab, args.st.fault);
+    ra_planet_forces<PN>(d, np, m, X, V, A, hdr, aob, args.light2, args.tab, args.st.fault, P);
     if (lane < np) {
         for (int c = 0; c < 3; c++) {
             A1[3 * lane + c] = A[3 * lane + c];
@@ -342,7 +417,7 @@ __global__ void __launch_bounds__(64) ra_planets(RaArgs args) {
     }
     int entry = 1;
     for (int n = 1; n <= args.niter; n++) {
-#pragma unroll
+#pragma unroll 1
         for (int j = 2; j <= 8; j++, entry++) {
             const double hj = args.k.h[j - 1];
             if (lane < np) {
@@ -357,7 +432,7 @@ __global__ void __launch_bounds__(64) ra_planets(RaArgs args) {
             }
             __syncthreads();
             ra_planet_forces<PN>(d, np, m, X, V, A, hdr, aob, args.light2, args.tab + (long long)entry * d.ts,
-                                 args.st.fault);
+                                 args.st.fault, P);
             if (lane < np)
                 for (int c = 0; c < 3; c++) ra_update(args.k, j, A[3 * lane + c], A1[3 * lane + c], g[c], b[c]);
         }
