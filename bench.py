@@ -376,6 +376,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-procs", type=int, default=16)
     ap.add_argument("--no-extras", action="store_true", help="skip the ensemble/single ratio")
+    ap.add_argument("--extras", default="all",
+                    help="comma list of extra workloads to run: ensemble, adjoint, radau, bs2, bs3, hybrid, "
+                         "hybrid_20k (default all)")
     ap.add_argument("--autotune", action="store_true",
                     help="let the autotuner pick steps per launch and workgroup size instead of --spl/--block "
                          "(the defaults are its usual choice on MI355X)")
@@ -447,17 +450,26 @@ def main():
                        "planet_ms": round(tm["planet_ms"], 2), "wall_ms": round(sec * 1e3, 2)},
     }
     extras = {}
-    if not args.no_extras:
+    names = ("ensemble", "adjoint", "radau", "bs2", "bs3", "hybrid", "hybrid_20k")
+    want = set() if args.no_extras else (set(names) if args.extras == "all" else set(args.extras.split(",")))
+    if want - set(names):
+        raise SystemExit(f"unknown extras {sorted(want - set(names))}; choose from {names}")
+    if "ensemble" in want:
         extras["ensemble"] = ensemble_ratio(hip, steps=64, spl=spl, block=block)
+    if "adjoint" in want:
         log("adjoint: orbit-fitting workload")
         extras["adjoint"] = adjoint_cost(hip)
+    if "radau" in want:
         log("radau: 8 planets + 1M particles")
         extras["radau"] = radau_rate(hip)
-        for algor in (2, 3):
+    for algor in (2, 3):
+        if f"bs{algor}" in want:
             log(f"bs: algorithm {algor}, 8 planets + 1M particles")
             extras[f"bs{algor}"] = bs_rate(hip, algor)
+    if "hybrid" in want:
         log("hybrid: 8 planets + 200k particles")
         extras["hybrid"] = hybrid_rate(hip)
+    if "hybrid_20k" in want:
         log("hybrid: 8 planets + 20k particles")
         extras["hybrid_20k"] = hybrid_rate(hip, nparticles=20_000, nsteps=5)
     cpu = None

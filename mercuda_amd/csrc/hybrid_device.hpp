@@ -409,6 +409,8 @@ struct BsArgs {
     int nbbp;
     const int* poff;            // particle partners: CSR offsets [n - nbig + 1]
     const int* pidx;            // partner Big members (compact index)
+    const int4* pinfo;          // per particle: partners held in registers (or -1), their indices
+    const double4* pcon;        // per particle: their max(rcrit) and masses (pack_partners)
     double h0, tol2;
     double* tp;                 // Big positions at every force point [kBsPoints][3 * kEncBig] (bs2_big)
     double* cand;               // accepted states per level n = 4..12: [9][6][ncap]
@@ -442,30 +444,44 @@ __device__ __forceinline__ void merge_err(unsigned long long* slot, double e) {
     if (__lane_id() == 0) atomicMax(slot, bits);
 }
 
+// The Big members' pair list, rcrit and masses, staged in LDS once per
+// launch: the force rounds then read no global memory between barriers.
+constexpr int kBbMax = 256;     // Big-Big pairs one encounter set may list
+struct BigShared {
+    int bb[2 * kBbMax];
+    double rc[kEncBig + 1], m[kEncBig + 1];
+};
+
+// Threads t of nt copy the lists; a barrier must follow before big_accel.
+__device__ __forceinline__ void stage_big(const BsArgs& a, BigShared& S, int t, int nt) {
+    for (int w = t; w < 2 * a.nbbp; w += nt) S.bb[w] = a.bbp[w];
+    for (int w = t; w < a.nbig; w += nt) { S.rc[w] = a.rcrit[w]; S.m[w] = a.m[w]; }
+}
+
 // Force on Big member me (1..nb) from the Big positions P ([3][kEncBig]):
 // mfo_hkce's Big-Big pairs in list order, then the central term.
-__device__ inline void big_accel(const BsArgs& a, int me, const double* P, double* acc) {
+__device__ inline void big_accel(const BigShared& S, int nbbp, int me, const double* P, double* acc) {
     acc[0] = 0.0; acc[1] = 0.0; acc[2] = 0.0;
-    for (int q = 0; q < a.nbbp; q++) {
-        int i = a.bbp[2 * q], j = a.bbp[2 * q + 1];
+    for (int q = 0; q < nbbp; q++) {
+        int i = S.bb[2 * q], j = S.bb[2 * q + 1];
         if (i != me && j != me) continue;
         double dx = P[3 * (j - 1)] - P[3 * (i - 1)];
         double dy = P[3 * (j - 1) + 1] - P[3 * (i - 1) + 1];
         double dz = P[3 * (j - 1) + 2] - P[3 * (i - 1) + 2];
         double tmp2;
-        if (!hkce_factor(dx * dx + dy * dy + dz * dz, fmax(a.rcrit[i], a.rcrit[j]), tmp2)) continue;
+        if (!hkce_factor(dx * dx + dy * dy + dz * dz, fmax(S.rc[i], S.rc[j]), tmp2)) continue;
         if (j == me) {
-            double faci = tmp2 * a.m[i];
+            double faci = tmp2 * S.m[i];
             acc[0] = acc[0] - faci * dx; acc[1] = acc[1] - faci * dy; acc[2] = acc[2] - faci * dz;
         } else {
-            double facj = tmp2 * a.m[j];
+            double facj = tmp2 * S.m[j];
             acc[0] = acc[0] + facj * dx; acc[1] = acc[1] + facj * dy; acc[2] = acc[2] + facj * dz;
         }
     }
     const double* x = P + 3 * (me - 1);
     double s2 = x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
     double s_1 = 1.0 / sqrt(s2);
-    double tmp2 = a.m[0] * s_1 * s_1 * s_1;
+    double tmp2 = S.m[0] * s_1 * s_1 * s_1;
     for (int c = 0; c < 3; c++) acc[c] = acc[c] - tmp2 * x[c];
 }
 
@@ -522,22 +538,37 @@ __device__ __forceinline__ void bs_level_result(const BsLevel& L, const double* 
     }
 }
 
+// mdt_bs2's extrapolation weights for the step h0 (mercury6_2.for:3533-3536):
+// for level n and j = n-1 .. 1, tmp0 = 1/(h2(j) - h2(n)), W = (tmp0 * h2(j+1),
+// tmp0 * h2(n)), with h2(n) = (h0/n)^2.  They depend on h0 alone, so a block
+// forms them once (threads t of nt; a barrier must follow) for all its members.
+constexpr int kExPairs = kLevels * (kLevels - 1) / 2;
+__host__ __device__ constexpr int ex_index(int n, int j) { return (n - 1) * (n - 2) / 2 + j - 1; }
+__device__ inline void ex_weights(double h0, double (*W)[2], int t, int nt) {
+    for (int w = t; w < kExPairs; w += nt) {
+        int n = 2;
+        while (n * (n - 1) / 2 <= w) n++;
+        const int j = w - (n - 1) * (n - 2) / 2 + 1;
+        const double hn = h0 / (double)n, hj = h0 / (double)j, hj1 = h0 / (double)(j + 1);
+        const double h2n = hn * hn, h2j = hj * hj, h2j1 = hj1 * hj1;
+        const double tmp0 = 1.0 / (h2j - h2n);
+        W[w][0] = tmp0 * h2j1;
+        W[w][1] = tmp0 * h2n;
+    }
+}
+
 // Extrapolation of one component q of one member over the levels in
 // mdt_bs2's order (mercury6_2.for:3531-3570), given every level's result
-// already in D (row n-1 holds level n's: no earlier level writes it).  Per
-// level n > 3 it keeps d(1) in E[n-4] and writes the accepted value.
-__device__ inline void bs_extrapolate(double h0, double* D, int ds, double* E, double* cand, long long cs) {
-    double h2[kLevels];
-    for (int n = 1; n <= kLevels; n++) {
-        double h = h0 / (double)n;
-        h2[n - 1] = h * h;
-    }
+// already in D (row n-1 holds level n's: no earlier level writes it) and the
+// weights W.  Per level n > 3 it keeps d(1) in E[n-4] and writes the accepted
+// value.
+__device__ inline void bs_extrapolate(const double (*W)[2], double* D, int ds, double* E, double* cand,
+                                      long long cs) {
     for (int n = 1; n <= kLevels; n++) {
         for (int j = n - 1; j >= 1; j--) {
-            int j1 = j + 1;
-            double tmp0 = 1.0 / (h2[j - 1] - h2[n - 1]);
-            double tmp1 = tmp0 * h2[j1 - 1];
-            double tmp2 = tmp0 * h2[n - 1];
+            const int j1 = j + 1;
+            const double tmp1 = W[ex_index(n, j)][0];
+            const double tmp2 = W[ex_index(n, j)][1];
             D[(j - 1) * 6 * ds] = tmp1 * D[(j1 - 1) * 6 * ds] - tmp2 * D[(j - 1) * 6 * ds];
         }
         if (n > 3) {
@@ -577,6 +608,10 @@ __device__ inline void bs2_big_body(const BsArgs& A) {
     __shared__ double X0[3 * kEncBig];
     __shared__ double D[72 * kEncBig];       // [6 * level + component][member]
     __shared__ double E[kEncBig][6][9];
+    __shared__ BigShared S;
+    __shared__ double W[kExPairs][2];
+    stage_big(A, S, t, kLevels * kEncBig);
+    ex_weights(h0, W, t, kLevels * kEncBig);
     double x0[3] = {0, 0, 0}, v0[3] = {0, 0, 0}, a0[3] = {0, 0, 0};
     if (mem)
         for (int c = 0; c < 3; c++) {
@@ -585,7 +620,7 @@ __device__ inline void bs2_big_body(const BsArgs& A) {
             if (n == 1) X0[3 * k + c] = x0[c];
         }
     __syncthreads();
-    if (mem) big_accel(A, k + 1, X0, a0);
+    if (mem) big_accel(S, A.nbbp, k + 1, X0, a0);
     BsLevel L;
     bs_level_start(L, n, h0, x0, v0, a0);
     double a[3] = {0, 0, 0};
@@ -597,7 +632,7 @@ __device__ inline void bs2_big_body(const BsArgs& A) {
                 A.tp[bs_point(n, r) * 3 * kEncBig + 3 * k + c] = L.xend[c];
             }
         __syncthreads();
-        if (act) big_accel(A, k + 1, P[n - 1], a);
+        if (act) big_accel(S, A.nbbp, k + 1, P[n - 1], a);
         __syncthreads();
         if (act && r < n) bs_level_step(L, r, a, x0, v0, a0);
     }
@@ -612,7 +647,7 @@ __device__ inline void bs2_big_body(const BsArgs& A) {
         const int km = t / 6, q = t % 6;
         if (km < nb) {
             double e[9];
-            bs_extrapolate(h0, D + q * kEncBig + km, kEncBig, e, A.cand + (long long)q * A.ncap + 1 + km, A.ncap);
+            bs_extrapolate(W, D + q * kEncBig + km, kEncBig, e, A.cand + (long long)q * A.ncap + 1 + km, A.ncap);
             for (int l = 0; l < 9; l++) E[km][q][l] = e[l];
         }
     }
@@ -642,10 +677,82 @@ __global__ void __launch_bounds__(kLevels * kEncBig) bs2_big(BsArgs A) { bs2_big
 // at the same force points; its arithmetic and d row are those of a thread of
 // its own.  The d tables live in LDS; then the per-component extrapolation
 // and the level errors, merged over the block before one atomicMax per level.
+// The attempt starts from member j's state at xs[c * ncap + j], vs[c * ncap + j]
+// (the state arrays, or the previous attempt's accepted candidates); RP holds
+// the thread's particle's partners, loaded by the caller (load_partners).
 constexpr int kBsPart = 16;                 // particle slots of bs2_loop_fused's blocks
 constexpr int kBsPairSlots = 32;            // particles per block of bs2_particles
 constexpr int kBsPairThreads = kLevels / 2 * kBsPairSlots;
-__device__ inline void bs2_particles_body(const BsArgs& A) {
+
+// A particle's Big partners in registers -- the usual encounter set gives a
+// particle one or two -- so that a force evaluation reads no pair list from
+// memory: compact index, max(rcrit_i, rcrit_j) and mass, in list order.
+// n = -1 when the particle has more partners (they are read from the lists).
+struct RegPartners {
+    int n;
+    int i0, i1;
+    double rc0, rc1, m0, m1;
+};
+
+__device__ __forceinline__ RegPartners reg_partners(const BsArgs& A, int p0, int p1, double rcj) {
+    RegPartners R{p1 - p0 <= 2 ? p1 - p0 : -1, 1, 1, 0.0, 0.0, 0.0, 0.0};
+    if (R.n > 0) { R.i0 = A.pidx[p0]; R.rc0 = fmax(A.rcrit[R.i0], rcj); R.m0 = A.m[R.i0]; }
+    if (R.n > 1) { R.i1 = A.pidx[p0 + 1]; R.rc1 = fmax(A.rcrit[R.i1], rcj); R.m1 = A.m[R.i1]; }
+    return R;
+}
+
+// Particle p's partners from the packed arrays pack_partners wrote.
+__device__ __forceinline__ RegPartners load_partners(const BsArgs& A, int p) {
+    const int4 f = A.pinfo[p];
+    const double4 g = A.pcon[p];
+    return RegPartners{f.x, f.y, f.z, g.x, g.y, g.z, g.w};
+}
+
+// Packs every particle's RegPartners once per encounter set: the pair lists
+// and masses stay fixed while the set is integrated.
+__global__ void pack_partners(BsArgs A, int4* info, double4* con) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= A.n - A.nbig) return;
+    const RegPartners R = reg_partners(A, A.poff[p], A.poff[p + 1], A.rcrit[A.nbig + p]);
+    info[p] = make_int4(R.n, R.i0, R.i1, 0);
+    con[p] = make_double4(R.rc0, R.rc1, R.m0, R.m1);
+}
+
+// One pair term of mfo_hkce for a particle at xe and a partner at X.
+__device__ __forceinline__ void pair_term(const double* xe, const double* X, double rc, double m, double* acc) {
+    double dx = xe[0] - X[0], dy = xe[1] - X[1], dz = xe[2] - X[2];
+    double tmp2;
+    if (!hkce_factor(dx * dx + dy * dy + dz * dz, rc, tmp2)) return;
+    double faci = tmp2 * m;
+    acc[0] = acc[0] - faci * dx; acc[1] = acc[1] - faci * dy; acc[2] = acc[2] - faci * dz;
+}
+
+// mfo_hkce for a particle whose partners are in R, at positions X0 and X1:
+// the pair terms in list order, then the central term (mc = central mass),
+// the same operations as the list loop.
+__device__ __forceinline__ void pforce_reg(const RegPartners& R, const double* xe, const double* X0,
+                                           const double* X1, double mc, double* out) {
+    double acc[3] = {0.0, 0.0, 0.0};
+    if (R.n > 0) pair_term(xe, X0, R.rc0, R.m0, acc);
+    if (R.n > 1) pair_term(xe, X1, R.rc1, R.m1, acc);
+    double s2 = xe[0] * xe[0] + xe[1] * xe[1] + xe[2] * xe[2];
+    double s_1 = 1.0 / sqrt(s2);
+    double tmp2 = mc * s_1 * s_1 * s_1;
+    for (int c = 0; c < 3; c++) out[c] = acc[c] - tmp2 * xe[c];
+}
+
+// Partner positions at force point `pt` from bs2_big's table.
+__device__ __forceinline__ void partner_positions(const RegPartners& R, const double* tp, int pt, double* X0,
+                                                  double* X1) {
+    const double* T = tp + pt * 3 * kEncBig;
+    if (R.n > 0)
+        for (int c = 0; c < 3; c++) X0[c] = T[3 * (R.i0 - 1) + c];
+    if (R.n > 1)
+        for (int c = 0; c < 3; c++) X1[c] = T[3 * (R.i1 - 1) + c];
+}
+
+__device__ inline void bs2_particles_body(const BsArgs& A, const double* xs, const double* vs,
+                                         const RegPartners& RP) {
     constexpr int NS = kBsPairSlots;
     static_assert(6 * NS == kBsPairThreads, "one extrapolation item per thread");
     const double h0 = A.h0;
@@ -658,17 +765,24 @@ __device__ inline void bs2_particles_body(const BsArgs& A) {
     const int me = A.nbig + pi_;
     __shared__ double D[72 * NS];            // [6 * level + component][slot]
     __shared__ double E[NS][6][9];
+    __shared__ double XS[NS][2];             // the slot's error scales
+    __shared__ double W[kExPairs][2];
     __shared__ unsigned long long EM[9];
     if (t < 9) EM[t] = 0ull;
+    ex_weights(h0, W, t, kBsPairThreads);
     double x0[3] = {1, 0, 0}, v0[3] = {0, 1, 0}, a0[3] = {0, 0, 0};
     double rcj = 0.0;
     int p0 = 0, p1 = 0;
     if (live) {
-        for (int c = 0; c < 3; c++) { x0[c] = A.x[c * A.ncap + me]; v0[c] = A.v[c * A.ncap + me]; }
-        rcj = A.rcrit[me];
-        p0 = A.poff[pi_];
-        p1 = A.poff[pi_ + 1];
+        for (int c = 0; c < 3; c++) { x0[c] = xs[c * A.ncap + me]; v0[c] = vs[c * A.ncap + me]; }
+        if (RP.n < 0) {
+            rcj = A.rcrit[me];
+            p0 = A.poff[pi_];
+            p1 = A.poff[pi_ + 1];
+        }
+        if (pair == 0) bs_scales(x0, v0, XS[sl][0], XS[sl][1]);
     }
+    const double m0 = A.m[0];
     // mfo_hkce for a particle: pairs with its Big partners, then the central term;
     // Xb(i) gives Big member i's position at this evaluation
     auto pforce = [&](const double* xe, double* a, auto Xb) {
@@ -688,21 +802,44 @@ __device__ inline void bs2_particles_body(const BsArgs& A) {
         for (int c = 0; c < 3; c++) a[c] = acc[c] - tmp2 * xe[c];
     };
     if (live) {
-        double xb[3];
-        pforce(x0, a0, [&](int i) {
-            for (int c = 0; c < 3; c++) xb[c] = A.x[c * A.ncap + i];
-            return (const double*)xb;
-        });
+        if (RP.n >= 0) {
+            double X0[3] = {0, 0, 0}, X1[3] = {0, 0, 0};
+            for (int c = 0; c < 3; c++) {
+                if (RP.n > 0) X0[c] = A.x[c * A.ncap + RP.i0];
+                if (RP.n > 1) X1[c] = A.x[c * A.ncap + RP.i1];
+            }
+            pforce_reg(RP, x0, X0, X1, m0, a0);
+        } else {
+            double xb[3];
+            pforce(x0, a0, [&](int i) {
+                for (int c = 0; c < 3; c++) xb[c] = A.x[c * A.ncap + i];
+                return (const double*)xb;
+            });
+        }
 #pragma unroll 1
         for (int k = 0; k < 2; k++) {
             const int n = k == 0 ? pair + 1 : kLevels - pair;
             BsLevel L;
             bs_level_start(L, n, h0, x0, v0, a0);
             double a[3];
-            for (int r = 1; r <= n; r++) {
-                const double* T = A.tp + bs_point(n, r) * 3 * kEncBig;
-                pforce(L.xend, a, [&](int i) { return T + 3 * (i - 1); });
-                if (r < n) bs_level_step(L, r, a, x0, v0, a0);
+            if (RP.n >= 0) {
+                // the next point's partner positions load while this one computes
+                double X0[3] = {0, 0, 0}, X1[3] = {0, 0, 0}, Y0[3] = {0, 0, 0}, Y1[3] = {0, 0, 0};
+                partner_positions(RP, A.tp, bs_point(n, 1), X0, X1);
+                for (int r = 1; r <= n; r++) {
+                    if (r < n) partner_positions(RP, A.tp, bs_point(n, r + 1), Y0, Y1);
+                    pforce_reg(RP, L.xend, X0, X1, m0, a);
+                    if (r < n) {
+                        bs_level_step(L, r, a, x0, v0, a0);
+                        for (int c = 0; c < 3; c++) { X0[c] = Y0[c]; X1[c] = Y1[c]; }
+                    }
+                }
+            } else {
+                for (int r = 1; r <= n; r++) {
+                    const double* T = A.tp + bs_point(n, r) * 3 * kEncBig;
+                    pforce(L.xend, a, [&](int i) { return T + 3 * (i - 1); });
+                    if (r < n) bs_level_step(L, r, a, x0, v0, a0);
+                }
             }
             double R[6];
             bs_level_result(L, a, v0, R);
@@ -716,7 +853,7 @@ __device__ inline void bs2_particles_body(const BsArgs& A) {
         const int ps = blockIdx.x * NS + s;
         if (ps < np) {
             double e[9];
-            bs_extrapolate(h0, D + q * NS + s, NS, e, A.cand + (long long)q * A.ncap + A.nbig + ps, A.ncap);
+            bs_extrapolate(W, D + q * NS + s, NS, e, A.cand + (long long)q * A.ncap + A.nbig + ps, A.ncap);
             for (int l = 0; l < 9; l++) E[s][q][l] = e[l];
         }
     }
@@ -726,13 +863,9 @@ __device__ inline void bs2_particles_body(const BsArgs& A) {
         const int s = w / 9, l = w % 9;
         const int ps = blockIdx.x * NS + s;
         if (ps < np) {
-            const int mp = A.nbig + ps;
-            double xs, vs, xm[3], vm[3];
-            for (int c = 0; c < 3; c++) { xm[c] = A.x[c * A.ncap + mp]; vm[c] = A.v[c * A.ncap + mp]; }
-            bs_scales(xm, vm, xs, vs);
             double d1[6];
             for (int q = 0; q < 6; q++) d1[q] = E[s][q][l];
-            double err = bs_level_err(d1, xs, vs, A.fault);
+            double err = bs_level_err(d1, XS[s][0], XS[s][1], A.fault);
             atomicMax(&EM[l], (unsigned long long)__double_as_longlong(err));
         }
     }
@@ -740,7 +873,11 @@ __device__ inline void bs2_particles_body(const BsArgs& A) {
     if (t < 9 && EM[t]) atomicMax(A.err + 3 + t, EM[t]);
 }
 
-__global__ void __launch_bounds__(kBsPairThreads) bs2_particles(BsArgs A) { bs2_particles_body(A); }
+__global__ void __launch_bounds__(kBsPairThreads) bs2_particles(BsArgs A) {
+    const int p = blockIdx.x * kBsPairSlots + threadIdx.x % kBsPairSlots;
+    const RegPartners RP = p < A.n - A.nbig ? load_partners(A, p) : RegPartners{0, 1, 1, 0.0, 0.0, 0.0, 0.0};
+    bs2_particles_body(A, A.x, A.v, RP);
+}
 
 __global__ void bs2_accept(BsArgs A, int level) {
     if (level < 4) return;
@@ -840,14 +977,19 @@ __global__ void __launch_bounds__(kLevels * kEncBig) bs2_loop_big(BsArgs A, BsLo
 }
 
 __global__ void __launch_bounds__(kBsPairThreads) bs2_loop_particles(BsArgs A, BsLoop B, int q) {
+    const int t = threadIdx.x;
+    const int p = blockIdx.x * kBsPairSlots + t % kBsPairSlots;
+    const RegPartners RP = p < A.n - A.nbig ? load_partners(A, p) : RegPartners{0, 1, 1, 0.0, 0.0, 0.0, 0.0};
     BsCtl c;
     const int level = bs_loop_decide(A, B, q, c);
-    const int t = threadIdx.x;
-    const int pi_ = blockIdx.x * kBsPairSlots + t;
-    if (level >= 4 && t < kBsPairSlots && pi_ < A.n - A.nbig) bs_loop_accept(A, B, q, level, A.nbig + pi_);
+    if (level >= 4 && t < kBsPairSlots && p < A.n - A.nbig) bs_loop_accept(A, B, q, level, A.nbig + p);
     if (c.state != 0) return;
-    __syncthreads();
-    bs2_particles_body(bs_loop_attempt(A, B, q, c));
+    // the attempt starts from the accepted candidates (those just stored into
+    // x, v by the threads above) or from the unchanged state
+    const long long nc = A.ncap;
+    const double* cp =
+        level >= 4 ? B.cand + (long long)((q + 1) & 1) * 54 * nc + (long long)(level - 4) * 6 * nc : nullptr;
+    bs2_particles_body(bs_loop_attempt(A, B, q, c), level >= 4 ? cp : A.x, level >= 4 ? cp + 3 * nc : A.v, RP);
 }
 
 // The same loop in one launch per attempt, for encounter sets whose particle
@@ -894,7 +1036,11 @@ __global__ void __launch_bounds__(kLevels * kBsPart) bs2_loop_fused(BsArgs A, Bs
     __shared__ double D[72 * kBsPart];       // [6 * level + component][slot]
     __shared__ double E[kBsPart][6][9];
     __shared__ unsigned long long EM[9];
+    __shared__ BigShared S;
+    __shared__ double W[kExPairs][2];
     if (t < 9) EM[t] = 0ull;
+    stage_big(A, S, t, kLevels * kBsPart);
+    ex_weights(h0, W, t, kLevels * kBsPart);
     double bx0[3] = {0, 0, 0}, bv0[3] = {0, 0, 0}, ba0[3] = {0, 0, 0};
     if (mem)
         for (int qc = 0; qc < 3; qc++) {
@@ -905,11 +1051,14 @@ __global__ void __launch_bounds__(kLevels * kBsPart) bs2_loop_fused(BsArgs A, Bs
     double x0[3] = {1, 0, 0}, v0[3] = {0, 1, 0}, a0[3] = {0, 0, 0};
     double rcj = 0.0;
     int p0 = 0, p1 = 0;
+    const RegPartners RP = live ? load_partners(A, pi_) : RegPartners{0, 1, 1, 0.0, 0.0, 0.0, 0.0};
     if (live) {
         for (int qc = 0; qc < 3; qc++) { x0[qc] = start(me, qc); v0[qc] = start(me, 3 + qc); }
-        rcj = A.rcrit[me];
-        p0 = A.poff[pi_];
-        p1 = A.poff[pi_ + 1];
+        if (RP.n < 0) {
+            rcj = A.rcrit[me];
+            p0 = A.poff[pi_];
+            p1 = A.poff[pi_ + 1];
+        }
     }
     // mfo_hkce for a particle (as bs2_particles); Xb holds the Big members' positions
     auto pforce = [&](const double* xe, double* acc_out, const double* Xb) {
@@ -928,9 +1077,13 @@ __global__ void __launch_bounds__(kLevels * kBsPart) bs2_loop_fused(BsArgs A, Bs
         double tmp2 = A.m[0] * s_1 * s_1 * s_1;
         for (int qc = 0; qc < 3; qc++) acc_out[qc] = acc[qc] - tmp2 * xe[qc];
     };
+    const double m0 = A.m[0];
     __syncthreads();
-    if (mem) big_accel(A, k + 1, X0, ba0);
-    if (live) pforce(x0, a0, X0);
+    if (mem) big_accel(S, A.nbbp, k + 1, X0, ba0);
+    if (live) {
+        if (RP.n >= 0) pforce_reg(RP, x0, X0 + 3 * (RP.i0 - 1), X0 + 3 * (RP.i1 - 1), m0, a0);
+        else pforce(x0, a0, X0);
+    }
     BsLevel LB, LP;
     bs_level_start(LB, n, h0, bx0, bv0, ba0);
     bs_level_start(LP, n, h0, x0, v0, a0);
@@ -940,8 +1093,16 @@ __global__ void __launch_bounds__(kLevels * kBsPart) bs2_loop_fused(BsArgs A, Bs
         if (act && mem)
             for (int qc = 0; qc < 3; qc++) P[n - 1][3 * k + qc] = LB.xend[qc];
         __syncthreads();
-        if (act && mem) big_accel(A, k + 1, P[n - 1], ab);
-        if (act && live) pforce(LP.xend, ap, P[n - 1]);
+        if (act && mem) big_accel(S, A.nbbp, k + 1, P[n - 1], ab);
+        if (act && live) {
+            if (RP.n >= 0) {
+                const double* X0 = P[n - 1] + 3 * (RP.i0 - 1);
+                const double* X1 = P[n - 1] + 3 * (RP.i1 - 1);
+                pforce_reg(RP, LP.xend, X0, X1, m0, ap);
+            } else {
+                pforce(LP.xend, ap, P[n - 1]);
+            }
+        }
         __syncthreads();
         if (act && r < n) {
             if (mem) bs_level_step(LB, r, ab, bx0, bv0, ba0);
@@ -960,7 +1121,7 @@ __global__ void __launch_bounds__(kLevels * kBsPart) bs2_loop_fused(BsArgs A, Bs
         const int ps = blockIdx.x * kBsPart + sl;
         if (ps < np) {
             double e[9];
-            bs_extrapolate(h0, D + qc * kBsPart + sl, kBsPart, e, a.cand + (long long)qc * nc + A.nbig + ps, nc);
+            bs_extrapolate(W, D + qc * kBsPart + sl, kBsPart, e, a.cand + (long long)qc * nc + A.nbig + ps, nc);
             for (int l = 0; l < 9; l++) E[sl][qc][l] = e[l];
         }
     }
@@ -993,7 +1154,7 @@ __global__ void __launch_bounds__(kLevels * kBsPart) bs2_loop_fused(BsArgs A, Bs
         const int km = t / 6, qc = t % 6;
         if (km < nb) {
             double e[9];
-            bs_extrapolate(h0, D + qc * kEncBig + km, kEncBig, e, a.cand + (long long)qc * nc + 1 + km, nc);
+            bs_extrapolate(W, D + qc * kEncBig + km, kEncBig, e, a.cand + (long long)qc * nc + 1 + km, nc);
             for (int l = 0; l < 9; l++) E[km][qc][l] = e[l];
         }
     }

@@ -137,6 +137,8 @@ struct Context {
         int n = 0, nbig = 0, ncap = 0;
         double *m = nullptr, *rcrit = nullptr, *x = nullptr, *v = nullptr, *tp = nullptr, *cand = nullptr;
         int *bbp = nullptr, *poff = nullptr, *pidx = nullptr;
+        int4* pinfo = nullptr;         // packed partners per particle (pack_partners)
+        double4* pcon = nullptr;
         int nbbp = 0, npidx_cap = 0, nbbp_cap = 0;
         unsigned long long* err = nullptr;
         BsCtl* ctl = nullptr;          // device step control (encounter_advance)
@@ -219,7 +221,7 @@ void release_all() {
                        &c.enc.tp, &c.enc.cand})
         dfree(*p);
     dfree(c.hpairs); dfree(c.hnpairs); dfree(c.hids); dfree(c.enc.bbp); dfree(c.enc.poff); dfree(c.enc.pidx);
-    dfree(c.enc.err); dfree(c.enc.ctl);
+    dfree(c.enc.err); dfree(c.enc.ctl); dfree(c.enc.pinfo); dfree(c.enc.pcon);
     c.hpair_cap = 0; c.hpack_cap = 0; c.hy_cap = 0; c.enc.ncap = 0; c.enc.active = false;
     c.enc.npidx_cap = 0; c.enc.nbbp_cap = 0;
     c.cap = 0; c.tab_cap = 0; c.ev_cap = 0; c.park_cap = 0; c.ntiles = 0;
@@ -1198,6 +1200,19 @@ void hybrid_finish(double h, const double* m, const double* xx, const double* vv
 
 // ---- compact encounter context (BS2) -------------------------------------
 
+// Each particle's partners, their max(rcrit) and masses, packed for the BS2
+// kernels; again whenever the masses change.
+void encounter_pack() {
+    Context& c = ctx;
+    Context::Enc& e = c.enc;
+    if (e.n <= e.nbig) return;
+    BsArgs A{};
+    A.n = e.n; A.nbig = e.nbig; A.m = e.m; A.rcrit = e.rcrit; A.poff = e.poff; A.pidx = e.pidx;
+    hipLaunchKernelGGL(pack_partners, dim3((e.n - e.nbig + 255) / 256), dim3(256), 0, c.stream, A, e.pinfo, e.pcon);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(c.stream));
+}
+
 void encounter_enter(int n, int nbig, const double* m, const double* x, const double* v, const double* crit,
                      int np_, const int* pi, const int* pj) {
     Context& c = ctx;
@@ -1215,6 +1230,8 @@ void encounter_enter(int n, int nbig, const double* m, const double* x, const do
         for (double** p : {&e.x, &e.v}) dalloc(*p, (size_t)3 * cap);
         dalloc(e.cand, (size_t)108 * cap);   // two rotating sets of 9 levels x 6 components
         dalloc(e.poff, (size_t)cap + 1);
+        dalloc(e.pinfo, (size_t)cap);
+        dalloc(e.pcon, (size_t)cap);
         e.ncap = cap;
     }
     if (!e.err) dalloc(e.err, 36);        // three rotating sets of 12 level errors
@@ -1233,6 +1250,7 @@ void encounter_enter(int n, int nbig, const double* m, const double* x, const do
         off[q + 1] = off[q] + (int)part[q].size();
         idx.insert(idx.end(), part[q].begin(), part[q].end());
     }
+    if ((int)bb.size() > 2 * kBbMax) throw std::runtime_error("more than 256 Big-Big pairs in one encounter set");
     if ((int)bb.size() > e.nbbp_cap) {
         e.nbbp_cap = std::max((int)bb.size(), 64);
         dalloc(e.bbp, (size_t)e.nbbp_cap);
@@ -1249,6 +1267,7 @@ void encounter_enter(int n, int nbig, const double* m, const double* x, const do
     e.nbig = nbig;
     HIPCHECK(hipMemcpy(e.m, m, sizeof(double) * n, hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(e.rcrit, crit, sizeof(double) * n, hipMemcpyHostToDevice));
+    encounter_pack();
     e.active = true;
     // states
     e.stage.resize((size_t)3 * n);
@@ -1299,7 +1318,7 @@ void bs2_step(double time, double* h, double* hdid, double tol, int64_t* forces,
     const int npart = e.n - e.nbig;
     BsArgs A{};
     A.n = e.n; A.nbig = e.nbig; A.m = e.m; A.rcrit = e.rcrit; A.x = e.x; A.v = e.v; A.ncap = e.ncap;
-    A.bbp = e.bbp; A.nbbp = e.nbbp; A.poff = e.poff; A.pidx = e.pidx;
+    A.bbp = e.bbp; A.nbbp = e.nbbp; A.poff = e.poff; A.pidx = e.pidx; A.pinfo = e.pinfo; A.pcon = e.pcon;
     A.tol2 = tol * tol;
     A.tp = e.tp; A.cand = e.cand; A.err = e.err; A.fault = c.fault;
     *forces = 1;           // the start accelerations, once per call
@@ -1372,7 +1391,7 @@ void encounter_advance(double h0, double* hrec, double tol, int64_t* substeps, i
     const int npart = e.n - e.nbig;
     BsArgs A{};
     A.n = e.n; A.nbig = e.nbig; A.m = e.m; A.rcrit = e.rcrit; A.x = e.x; A.v = e.v; A.ncap = e.ncap;
-    A.bbp = e.bbp; A.nbbp = e.nbbp; A.poff = e.poff; A.pidx = e.pidx;
+    A.bbp = e.bbp; A.nbbp = e.nbbp; A.poff = e.poff; A.pidx = e.pidx; A.pinfo = e.pinfo; A.pcon = e.pcon;
     A.tol2 = k.tol2;
     A.tp = e.tp; A.cand = e.cand; A.err = e.err; A.fault = c.fault;
     const BsLoop B{e.ctl, e.err, e.cand};
@@ -1718,6 +1737,7 @@ int mercury_hip_encounter_update(const double* m, const double* x, const double*
         for (int j = e.nbig; j < e.n; j++)
             if (m[j] != 0) throw std::runtime_error("Small bodies with mass are not provided by the HIP path");
         HIPCHECK(hipMemcpy(e.m, m, sizeof(double) * e.n, hipMemcpyHostToDevice));
+        encounter_pack();
         encounter_states(x, v);
         return 0;
     } catch (const std::exception& e) { return fail(e); }
