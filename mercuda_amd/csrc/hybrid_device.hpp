@@ -597,7 +597,8 @@ __device__ __forceinline__ double bs_level_err(const double* d1, double xs, doub
 // and reading them for the pair forces; every position is also written to
 // tp (the Big positions at each force point) for bs2_particles.  Then the
 // per-component extrapolation, the level errors and the accepted states.
-__device__ inline void bs2_big_body(const BsArgs& A) {
+// The attempt starts from member j's state at xs[c * ncap + j], vs[c * ncap + j].
+__device__ inline void bs2_big_body(const BsArgs& A, const double* xs, const double* vs) {
     const double h0 = A.h0;
     const int nb = A.nbig - 1;
     const int t = threadIdx.x;
@@ -610,15 +611,18 @@ __device__ inline void bs2_big_body(const BsArgs& A) {
     __shared__ double E[kEncBig][6][9];
     __shared__ BigShared S;
     __shared__ double W[kExPairs][2];
+    __shared__ double XS[kEncBig][2];        // the member's error scales
     stage_big(A, S, t, kLevels * kEncBig);
     ex_weights(h0, W, t, kLevels * kEncBig);
     double x0[3] = {0, 0, 0}, v0[3] = {0, 0, 0}, a0[3] = {0, 0, 0};
-    if (mem)
+    if (mem) {
         for (int c = 0; c < 3; c++) {
-            x0[c] = A.x[c * A.ncap + 1 + k];
-            v0[c] = A.v[c * A.ncap + 1 + k];
+            x0[c] = xs[c * A.ncap + 1 + k];
+            v0[c] = vs[c * A.ncap + 1 + k];
             if (n == 1) X0[3 * k + c] = x0[c];
         }
+        if (n == 1) bs_scales(x0, v0, XS[k][0], XS[k][1]);
+    }
     __syncthreads();
     if (mem) big_accel(S, A.nbbp, k + 1, X0, a0);
     BsLevel L;
@@ -656,18 +660,15 @@ __device__ inline void bs2_big_body(const BsArgs& A) {
     if (t < 9 * kEncBig) {
         const int km = t / 9, l = t % 9;
         if (km < nb) {
-            double xs, vs, xm[3], vm[3];
-            for (int c = 0; c < 3; c++) { xm[c] = A.x[c * A.ncap + 1 + km]; vm[c] = A.v[c * A.ncap + 1 + km]; }
-            bs_scales(xm, vm, xs, vs);
             double d1[6];
             for (int q = 0; q < 6; q++) d1[q] = E[km][q][l];
-            double err = bs_level_err(d1, xs, vs, A.fault);
+            double err = bs_level_err(d1, XS[km][0], XS[km][1], A.fault);
             atomicMax(A.err + 3 + l, (unsigned long long)__double_as_longlong(err));
         }
     }
 }
 
-__global__ void __launch_bounds__(kLevels * kEncBig) bs2_big(BsArgs A) { bs2_big_body(A); }
+__global__ void __launch_bounds__(kLevels * kEncBig) bs2_big(BsArgs A) { bs2_big_body(A, A.x, A.v); }
 
 // One BS2 attempt, particle members (after bs2_big): kBsPairSlots particles
 // per block.  Level n makes n force evaluations, so thread (pair p, slot) runs
@@ -939,19 +940,42 @@ struct BsLoop {
 // Big members, or the block's particles); bs2_loop_big alone stores the new
 // state.  Then each runs attempt q exactly as bs2_big / bs2_particles do.
 // Once a decision ends the loop, later launches only carry the state forward.
+// Every input of the decision is loaded up front, in one round trip: the
+// state, the 12 level errors of attempt q-1 (err[(q + 2) % 3], a valid set
+// also for q = 0) and the fault flag; a loop that stopped at the first
+// accepted level would wait on one load per level.
 __device__ inline int bs_loop_decide(const BsArgs& A, const BsLoop& B, int q, BsCtl& c) {
     c = B.ctl[q & 1];
+    const ulonglong2* ep = reinterpret_cast<const ulonglong2*>(B.err + 12 * ((q + 2) % 3));
+    unsigned long long ev[12];
+    for (int w = 0; w < 6; w++) {
+        const ulonglong2 e2 = ep[w];
+        ev[2 * w] = e2.x;
+        ev[2 * w + 1] = e2.y;
+    }
+    const int fault = *A.fault;
+    // an empty asm that takes every loaded value keeps the compiler from
+    // sinking the loads behind the branches below
+    unsigned long long all = (unsigned long long)fault ^ (unsigned long long)c.state ^
+                             (unsigned long long)__double_as_longlong(c.hlocal) ^
+                             (unsigned long long)__double_as_longlong(c.tlocal) ^
+                             (unsigned long long)__double_as_longlong(c.tol2);
+    for (int w = 0; w < 12; w++) all ^= ev[w];
+    asm volatile("" : : "s"(all));
     if (c.state != 0 || q == 0) return 0;
-    return bs_control_step(c, B.err + 12 * ((q - 1) % 3), (*A.fault & FAULT_FORCE) != 0);
+    return bs_control_step(c, ev, (fault & FAULT_FORCE) != 0);
 }
 
-// Member j takes its state at attempt q-1's accepted level.
+// Member j takes its state at attempt q-1's accepted level (all six loads
+// issued before the stores).
 __device__ inline void bs_loop_accept(const BsArgs& A, const BsLoop& B, int q, int level, int j) {
     const long long nc = A.ncap;
     const double* cp = B.cand + (long long)((q + 1) & 1) * 54 * nc + (long long)(level - 4) * 6 * nc;
+    double st[6];
+    for (int qc = 0; qc < 6; qc++) st[qc] = cp[qc * nc + j];
     for (int qc = 0; qc < 3; qc++) {
-        A.x[qc * nc + j] = cp[qc * nc + j];
-        A.v[qc * nc + j] = cp[(3 + qc) * nc + j];
+        A.x[qc * nc + j] = st[qc];
+        A.v[qc * nc + j] = st[3 + qc];
     }
 }
 
@@ -972,8 +996,12 @@ __global__ void __launch_bounds__(kLevels * kEncBig) bs2_loop_big(BsArgs A, BsLo
     if (t < 12) B.err[12 * ((q + 1) % 3) + t] = 0ull;
     if (level >= 4 && t < A.nbig - 1) bs_loop_accept(A, B, q, level, 1 + t);
     if (c.state != 0) return;
-    __syncthreads();                       // accepted states before the attempt reads them
-    bs2_big_body(bs_loop_attempt(A, B, q, c));
+    // the attempt starts from the accepted candidates (being stored into x, v
+    // above for bs2_loop_particles) or from the unchanged state
+    const long long nc = A.ncap;
+    const double* cp =
+        level >= 4 ? B.cand + (long long)((q + 1) & 1) * 54 * nc + (long long)(level - 4) * 6 * nc : nullptr;
+    bs2_big_body(bs_loop_attempt(A, B, q, c), level >= 4 ? cp : A.x, level >= 4 ? cp + 3 * nc : A.v);
 }
 
 __global__ void __launch_bounds__(kBsPairThreads) bs2_loop_particles(BsArgs A, BsLoop B, int q) {
