@@ -1020,6 +1020,187 @@ __global__ void __launch_bounds__(kBsPairThreads) bs2_loop_particles(BsArgs A, B
     bs2_particles_body(bs_loop_attempt(A, B, q, c), level >= 4 ? cp : A.x, level >= 4 ? cp + 3 * nc : A.v, RP);
 }
 
+// The loop in one launch per attempt with bs2_particles' level pairing: every
+// block first advances the Big members' 12 levels in LDS (thread (level,
+// member), one force evaluation per round) and keeps their positions at all
+// 78 force points (TB, dynamic LDS of 78 x 3 x nb doubles); then its
+// kBsPairSlots particles run their paired levels from that table.  Block 0
+// alone applies the Big members' accept, extrapolates them and stores the
+// state.  Results are bs2_big's and bs2_particles', bit for bit.
+__global__ void __launch_bounds__(kBsPairThreads) bs2_loop_paired(BsArgs A, BsLoop B, int q) {
+    constexpr int NS = kBsPairSlots;
+    static_assert(kLevels * kEncBig == kBsPairThreads, "thread (level, Big member) in the Big rounds");
+    extern __shared__ double TB[];
+    const int t = threadIdx.x;
+    const int nb = A.nbig - 1;
+    const int np = A.n - A.nbig;
+    const bool lead = blockIdx.x == 0;
+    const long long nc = A.ncap;
+    const int sl = t % NS, pair = t / NS;
+    const int pi_ = blockIdx.x * NS + sl;
+    const bool live = pi_ < np;
+    const int me = A.nbig + pi_;
+    const RegPartners RP = live ? load_partners(A, pi_) : RegPartners{0, 1, 1, 0.0, 0.0, 0.0, 0.0};
+    BsCtl c;
+    const int level = bs_loop_decide(A, B, q, c);
+    if (lead && t == 0) B.ctl[(q + 1) & 1] = c;
+    if (lead && t < 12) B.err[12 * ((q + 1) % 3) + t] = 0ull;
+    if (level >= 4) {
+        if (pair == 0 && live) bs_loop_accept(A, B, q, level, me);
+        if (pair == 1 && lead && sl < nb) bs_loop_accept(A, B, q, level, 1 + sl);
+    }
+    if (c.state != 0) return;
+    const BsArgs a = bs_loop_attempt(A, B, q, c);
+    const double h0 = a.h0;
+    // the attempt starts from the accepted candidates or from the unchanged state
+    const double* cp =
+        level >= 4 ? B.cand + (long long)((q + 1) & 1) * 54 * nc + (long long)(level - 4) * 6 * nc : nullptr;
+    const double* xs = level >= 4 ? cp : A.x;
+    const double* vs = level >= 4 ? cp + 3 * nc : A.v;
+    __shared__ double X0[3 * kEncBig];
+    __shared__ double D[72 * NS];            // [6 * level + component][slot], then [..][Big member]
+    static_assert(6 * 9 * NS <= 72 * NS, "the level errors' d(1) table fits in D");
+    __shared__ double XS[NS][2];
+    __shared__ double BXS[kEncBig][2];
+    __shared__ unsigned long long EM[9];
+    __shared__ BigShared S;
+    __shared__ double W[kExPairs][2];
+    if (t < 9) EM[t] = 0ull;
+    stage_big(A, S, t, kBsPairThreads);
+    ex_weights(h0, W, t, kBsPairThreads);
+    // Big member k+1 at level n
+    const int n = t / kEncBig + 1, k = t % kEncBig;
+    const bool mem = k < nb;
+    double bx0[3] = {0, 0, 0}, bv0[3] = {0, 0, 0}, ba0[3] = {0, 0, 0};
+    if (mem) {
+        for (int qc = 0; qc < 3; qc++) {
+            bx0[qc] = xs[qc * nc + 1 + k];
+            bv0[qc] = vs[qc * nc + 1 + k];
+            if (n == 1) X0[3 * k + qc] = bx0[qc];
+        }
+        if (n == 1) bs_scales(bx0, bv0, BXS[k][0], BXS[k][1]);
+    }
+    double x0[3] = {1, 0, 0}, v0[3] = {0, 1, 0}, a0[3] = {0, 0, 0};
+    double rcj = 0.0;
+    int p0 = 0, p1 = 0;
+    if (live) {
+        for (int qc = 0; qc < 3; qc++) { x0[qc] = xs[qc * nc + me]; v0[qc] = vs[qc * nc + me]; }
+        if (RP.n < 0) {
+            rcj = A.rcrit[me];
+            p0 = A.poff[pi_];
+            p1 = A.poff[pi_ + 1];
+        }
+        if (pair == 0) bs_scales(x0, v0, XS[sl][0], XS[sl][1]);
+    }
+    const double m0 = A.m[0];
+    __syncthreads();
+    if (mem) big_accel(S, A.nbbp, k + 1, X0, ba0);
+    BsLevel LB;
+    bs_level_start(LB, n, h0, bx0, bv0, ba0);
+    double ab[3] = {0, 0, 0};
+    for (int r = 1; r <= kLevels; r++) {
+        const bool act = mem && r <= n;
+        double* Pt = TB + bs_point(n, r) * 3 * nb;
+        if (act)
+            for (int qc = 0; qc < 3; qc++) Pt[3 * k + qc] = LB.xend[qc];
+        __syncthreads();
+        if (act) big_accel(S, A.nbbp, k + 1, Pt, ab);
+        __syncthreads();
+        if (act && r < n) bs_level_step(LB, r, ab, bx0, bv0, ba0);
+    }
+    // particles: mfo_hkce with the Big positions Xb(i)
+    auto pforce = [&](const double* xe, double* acc_out, auto Xb) {
+        double acc[3] = {0.0, 0.0, 0.0};
+        for (int qq = p0; qq < p1; qq++) {
+            int i = A.pidx[qq];
+            const double* Xi = Xb(i);
+            double dx = xe[0] - Xi[0], dy = xe[1] - Xi[1], dz = xe[2] - Xi[2];
+            double tmp2;
+            if (!hkce_factor(dx * dx + dy * dy + dz * dz, fmax(A.rcrit[i], rcj), tmp2)) continue;
+            double faci = tmp2 * A.m[i];
+            acc[0] = acc[0] - faci * dx; acc[1] = acc[1] - faci * dy; acc[2] = acc[2] - faci * dz;
+        }
+        double s2 = xe[0] * xe[0] + xe[1] * xe[1] + xe[2] * xe[2];
+        double s_1 = 1.0 / sqrt(s2);
+        double tmp2 = A.m[0] * s_1 * s_1 * s_1;
+        for (int qc = 0; qc < 3; qc++) acc_out[qc] = acc[qc] - tmp2 * xe[qc];
+    };
+    if (live) {
+        if (RP.n >= 0) pforce_reg(RP, x0, X0 + 3 * (RP.i0 - 1), X0 + 3 * (RP.i1 - 1), m0, a0);
+        else pforce(x0, a0, [&](int i) { return (const double*)(X0 + 3 * (i - 1)); });
+#pragma unroll 1
+        for (int kk = 0; kk < 2; kk++) {
+            const int nn = kk == 0 ? pair + 1 : kLevels - pair;
+            BsLevel L;
+            bs_level_start(L, nn, h0, x0, v0, a0);
+            double ap[3];
+            for (int r = 1; r <= nn; r++) {
+                const double* T = TB + bs_point(nn, r) * 3 * nb;
+                if (RP.n >= 0) pforce_reg(RP, L.xend, T + 3 * (RP.i0 - 1), T + 3 * (RP.i1 - 1), m0, ap);
+                else pforce(L.xend, ap, [&](int i) { return T + 3 * (i - 1); });
+                if (r < nn) bs_level_step(L, r, ap, x0, v0, a0);
+            }
+            double R[6];
+            bs_level_result(L, ap, v0, R);
+            for (int qc = 0; qc < 6; qc++) D[(6 * (nn - 1) + qc) * NS + sl] = R[qc];
+        }
+    }
+    __syncthreads();
+    // particle extrapolation: thread (slot, component); then d(1) of levels
+    // 4..12 go into D's space, no longer read, for the level errors
+    double* E = D;                           // [slot or Big member][component][level]
+    {
+        const int s = t / 6, qc = t % 6;
+        const int ps = blockIdx.x * NS + s;
+        double e[9];
+        if (ps < np) bs_extrapolate(W, D + qc * NS + s, NS, e, a.cand + (long long)qc * nc + A.nbig + ps, nc);
+        __syncthreads();
+        if (ps < np)
+            for (int l = 0; l < 9; l++) E[(6 * s + qc) * 9 + l] = e[l];
+    }
+    __syncthreads();
+    // level errors: item (slot, level 4 + l), merged per level over the block
+    for (int w = t; w < 9 * NS; w += kBsPairThreads) {
+        const int s = w / 9, l = w % 9;
+        const int ps = blockIdx.x * NS + s;
+        if (ps < np) {
+            double d1[6];
+            for (int qc = 0; qc < 6; qc++) d1[qc] = E[(6 * s + qc) * 9 + l];
+            double err = bs_level_err(d1, XS[s][0], XS[s][1], A.fault);
+            atomicMax(&EM[l], (unsigned long long)__double_as_longlong(err));
+        }
+    }
+    __syncthreads();
+    if (t < 9 && EM[t]) atomicMax(a.err + 3 + t, EM[t]);
+    if (!lead) return;
+    // block 0: the Big members' d rows, extrapolation and level errors
+    if (mem) {
+        double R[6];
+        bs_level_result(LB, ab, bv0, R);
+        for (int qc = 0; qc < 6; qc++) D[(6 * (n - 1) + qc) * kEncBig + k] = R[qc];
+    }
+    __syncthreads();
+    {
+        const int km = t / 6, qc = t % 6;
+        const bool valid = t < 6 * kEncBig && km < nb;
+        double e[9];
+        if (valid) bs_extrapolate(W, D + qc * kEncBig + km, kEncBig, e, a.cand + (long long)qc * nc + 1 + km, nc);
+        __syncthreads();
+        if (valid)
+            for (int l = 0; l < 9; l++) E[(6 * km + qc) * 9 + l] = e[l];
+    }
+    __syncthreads();
+    if (t < 9 * kEncBig) {
+        const int km = t / 9, l = t % 9;
+        if (km < nb) {
+            double d1[6];
+            for (int qc = 0; qc < 6; qc++) d1[qc] = E[(6 * km + qc) * 9 + l];
+            double err = bs_level_err(d1, BXS[km][0], BXS[km][1], A.fault);
+            atomicMax(a.err + 3 + l, (unsigned long long)__double_as_longlong(err));
+        }
+    }
+}
+
 // The same loop in one launch per attempt, for encounter sets whose particle
 // blocks all fit on the device at once (bs2_loop_fused): every block advances
 // the Big members' 12 levels itself, in LDS, round by round beside its

@@ -1362,11 +1362,12 @@ void bs2_step(double time, double* h, double* hdid, double tol, int64_t* forces,
 
 // The driver's mdt_hkce loop over one step h0 (mercury6_2.for:3883-3905) with
 // the step control on the device: per attempt one bs2_loop_fused launch when
-// all its blocks are resident at once, else bs2_loop_big and
-// bs2_loop_particles (leaner blocks, more members per round); queued in
+// all its blocks are resident at once, else one bs2_loop_paired launch when
+// its blocks are (twice the particles per block, levels paired), else
+// bs2_loop_big and bs2_loop_particles; queued in
 // batches without host synchronisation (launches after the loop's end only
 // carry the state forward); the host reads the state between batches.
-// MERCURY_HIP_ENC_LOOP=fused or =split pins a variant (both give the same bits).
+// MERCURY_HIP_ENC_LOOP=fused, =paired or =split pins a variant (all give the same bits).
 // Same sequence of steps and states as calling bs2_step per substep; the
 // encounter listing the driver makes after each substep is not made.
 void encounter_advance(double h0, double* hrec, double tol, int64_t* substeps, int64_t* forces,
@@ -1408,13 +1409,23 @@ void encounter_advance(double h0, double* hrec, double tol, int64_t* substeps, i
     HIPCHECK(hipGetDevice(&dev));
     HIPCHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     bool fused = (long long)gf.x <= (long long)per_cu * ncu;
+    // otherwise one paired launch per attempt when its blocks are resident at once
+    const size_t tb = sizeof(double) * kBsPoints * 3 * (size_t)std::max(e.nbig - 1, 0);
+    int per_cu_p = 0;
+    HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_p, reinterpret_cast<const void*>(&bs2_loop_paired),
+                                                          kBsPairThreads, tb));
+    bool paired = !fused && (long long)gp.x <= (long long)per_cu_p * ncu;
     if (const char* v = std::getenv("MERCURY_HIP_ENC_LOOP")) {
-        if (!std::strcmp(v, "fused")) fused = true;
-        else if (!std::strcmp(v, "split")) fused = false;
+        fused = !std::strcmp(v, "fused");
+        paired = !std::strcmp(v, "paired");
     }
     int q = 0;
     for (int batch = 8;; batch = std::min(2 * batch, 256)) {
         for (int j = 0; j < batch; j++, q++) {
+            if (paired) {
+                hipLaunchKernelGGL(bs2_loop_paired, gp, dim3(kBsPairThreads), tb, c.stream, A, B, q);
+                continue;
+            }
             if (fused) {
                 hipLaunchKernelGGL(bs2_loop_fused, gf, dim3(kLevels * kBsPart), 0, c.stream, A, B, q);
                 continue;

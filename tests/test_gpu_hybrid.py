@@ -59,7 +59,7 @@ def test_bs2_compact_matches_oracle(hip, oracle):
         assert np.array_equal(xd[1:], xo[1:]) and np.array_equal(vd[1:], vo[1:])
 
 
-@pytest.mark.parametrize("variant", ["fused", "split"])
+@pytest.mark.parametrize("variant", ["fused", "split", "paired"])
 @pytest.mark.parametrize("nparticles", [0, 40])
 def test_device_encounter_loop_compact_sets(hip, variant, nparticles, monkeypatch):
     """The device encounter loop on hand-made sets with a Big-Big pair: two
@@ -140,7 +140,7 @@ def test_hybrid_ragged_counts(hip, oracle, count):
     compare_rows(rows, oracle_rows(evi, evd))
 
 
-@pytest.mark.parametrize("variant", ["fused", "split"])
+@pytest.mark.parametrize("variant", ["fused", "split", "paired"])
 def test_device_encounter_loop_equals_per_substep_calls(hip, oracle, variant, monkeypatch):
     """mercury_hip_encounter_advance (mdt_hkce's loop with the step control on
     the device) against the same loop driven by one mercury_hip_step per
