@@ -43,6 +43,7 @@ struct BswArgs {
     State st;
     double* tab;                // 1 + P tables of stride Sys::ts
     double* cand;               // accepted states per level n > 3: [L - 3][6][cap]
+    double* rsave;              // the first pass's level results for the second: [8][6][cap]
     unsigned long long* err;    // max error per level [12] (bits of non-negative doubles)
     double h0;
     double light2;
@@ -146,23 +147,40 @@ __device__ __forceinline__ void bsw_result(const BswLevel<ALG>& L, const double*
     }
 }
 
-// Extrapolation of one component of one body over the levels in the
-// Fortran's order, every level's result already in D (row n-1): per level
-// n > 3, d(1) in E[n-4] and the accepted value in cand.
+// The extrapolation weights of mdt_bs1 / mdt_bs2 for the step h0: for level n
+// and j = n-1 .. 1, tmp0 = 1/(h2(j) - h2(n)) and W = (tmp0 * h2(j+1), tmp0 *
+// h2(n)).  They depend on h0 alone, so a block forms them once (threads t of
+// nt; a barrier must follow) instead of every thread per component.
+template <int ALG> __host__ __device__ constexpr int bsw_pairs() { return BsShape<ALG>::L * (BsShape<ALG>::L - 1) / 2; }
+__host__ __device__ constexpr int bsw_ex(int n, int j) { return (n - 1) * (n - 2) / 2 + j - 1; }
 template <int ALG>
-__device__ inline void bsw_extrapolate(double h0, double* D, int ds, double* E, double* cand, long long cs) {
-    constexpr int L = BsShape<ALG>::L;
-    double h2[L];
-    for (int n = 1; n <= L; n++) h2[n - 1] = bsw_h2<ALG>(h0, n);
-    for (int n = 1; n <= L; n++) {
+__device__ inline void bsw_weights(double h0, double (*W)[2], int t, int nt) {
+    for (int w = t; w < bsw_pairs<ALG>(); w += nt) {
+        int n = 2;
+        while (n * (n - 1) / 2 <= w) n++;
+        const int j = w - (n - 1) * (n - 2) / 2 + 1;
+        const double h2n = bsw_h2<ALG>(h0, n), h2j = bsw_h2<ALG>(h0, j), h2j1 = bsw_h2<ALG>(h0, j + 1);
+        const double tmp0 = 1.0 / (h2j - h2n);
+        W[w][0] = tmp0 * h2j1;
+        W[w][1] = tmp0 * h2n;
+    }
+}
+
+// Extrapolation of one component of one body over the levels 1..NMAX in the
+// Fortran's order, every level's result already in D (row n-1): per level
+// n > 3 and n >= nlo, d(1) in E[n-4] and the accepted value in cand (a
+// second pass redoes the first levels' updates, whose outputs it already has).
+template <int NMAX>
+__device__ inline void bsw_extrapolate(const double (*W)[2], double* D, int ds, double* E, double* cand,
+                                       long long cs, int nlo) {
+    for (int n = 1; n <= NMAX; n++) {
         for (int j = n - 1; j >= 1; j--) {
-            int j1 = j + 1;
-            double tmp0 = 1.0 / (h2[j - 1] - h2[n - 1]);
-            double tmp1 = tmp0 * h2[j1 - 1];
-            double tmp2 = tmp0 * h2[n - 1];
+            const int j1 = j + 1;
+            const double tmp1 = W[bsw_ex(n, j)][0];
+            const double tmp2 = W[bsw_ex(n, j)][1];
             D[(j - 1) * 6 * ds] = tmp1 * D[(j1 - 1) * 6 * ds] - tmp2 * D[(j - 1) * 6 * ds];
         }
-        if (n > 3) {
+        if (n > 3 && n >= nlo) {
             double d1 = D[0];
             E[n - 4] = d1;
             double s = d1;
@@ -272,33 +290,51 @@ __device__ void bsw_planet_forces(const Sys& d, int np, const double* m, const d
     __syncthreads();
 }
 
-// One attempt, planets: one block of L x kBsLanes threads, thread (level n, planet k+1).
-template <int ALG, bool PN>
-__global__ void __launch_bounds__(BsShape<ALG>::L * kBsLanes) bsw_big(BswArgs A) {
+// One attempt over the levels LO..HI, planets: one block of (HI-LO+1) x
+// kBsLanes threads, thread (level n, planet k+1).  An attempt runs levels
+// 1..HI first; the host runs the rest (LO > 1) only when none of those levels
+// converged.  A first pass (HI < L) keeps its levels' results in rsave; the
+// second reads them back, so that its extrapolation is that of all L levels.
+template <int ALG, bool PN, int LO, int HI>
+__global__ void __launch_bounds__((HI - LO + 1) * kBsLanes) bsw_big(BswArgs A) {
     constexpr int L = BsShape<ALG>::L;
-    constexpr int R = BsShape<ALG>::R;
+    constexpr int NL = HI - LO + 1;
+    constexpr int NT = NL * kBsLanes;
+    constexpr int R = bsw_evals<ALG>(HI);
+    constexpr int ELO = LO > 4 ? LO : 4;
+    static_assert(LO == 1 || HI == L, "a first pass or the completing pass");
+    static_assert(LO - 1 <= 8, "rsave holds 8 levels");
     const Sys d = A.sys[0];
     const int np = d.nbig - 1;
     const int t = threadIdx.x;
-    const int n = t / kBsLanes + 1;
+    const int n = LO + t / kBsLanes;
     const int k = t % kBsLanes;
     const bool mem = k < np;
     const long long cap = A.st.cap;
     __shared__ double m[kBsLanes + 1];
-    __shared__ double X[L][3 * kBsLanes], V[L][3 * kBsLanes], F[L][3 * kBsLanes], OB[L][3 * kBsLanes], HD[L][8];
-    __shared__ double D[72 * kBsLanes];
+    __shared__ double X[NL][3 * kBsLanes], V[NL][3 * kBsLanes], F[NL][3 * kBsLanes], OB[NL][3 * kBsLanes],
+        HD[NL][8];
+    __shared__ double D[6 * HI * kBsLanes];
     __shared__ double E[kBsLanes][6][L - 3];
+    __shared__ double W[bsw_pairs<ALG>()][2];
     if (t <= np) m[t] = A.st.m[d.base + t];
+    bsw_weights<ALG>(A.h0, W, t, NT);
+    if constexpr (LO > 1) {
+        for (int w = t; w < (LO - 1) * 6 * kBsLanes; w += NT) {
+            const int row = w / (6 * kBsLanes), q = w / kBsLanes % 6, km = w % kBsLanes;
+            if (km < np) D[(6 * row + q) * kBsLanes + km] = A.rsave[(long long)(6 * row + q) * cap + d.base + 1 + km];
+        }
+    }
     double x0[3] = {0, 0, 0}, v0[3] = {0, 0, 0}, a0[3] = {0, 0, 0};
     if (mem)
         for (int c = 0; c < 3; c++) {
             x0[c] = A.st.x[c * cap + d.base + 1 + k];
             v0[c] = A.st.v[c * cap + d.base + 1 + k];
-            if (n == 1) { X[0][3 * k + c] = x0[c]; V[0][3 * k + c] = v0[c]; }
+            if (n == LO) { X[0][3 * k + c] = x0[c]; V[0][3 * k + c] = v0[c]; }
         }
     __syncthreads();
-    // the step start: one evaluation (level 1's lanes), shared by every level
-    bsw_planet_forces<PN>(d, np, m, X[0], V[0], F[0], OB[0], HD[0], A.light2, A.tab + d.tab, A.st.fault, k, n == 1);
+    // the step start: one evaluation (the first level's lanes), shared by every level
+    bsw_planet_forces<PN>(d, np, m, X[0], V[0], F[0], OB[0], HD[0], A.light2, A.tab + d.tab, A.st.fault, k, n == LO);
     if (mem)
         for (int c = 0; c < 3; c++) a0[c] = F[0][3 * k + c];
     __syncthreads();
@@ -310,13 +346,13 @@ __global__ void __launch_bounds__(BsShape<ALG>::L * kBsLanes) bsw_big(BswArgs A)
         if (act && mem) {
             const double *xp, *vp;
             bsw_point<ALG>(Lv, r, v0, xp, vp);
-            for (int c = 0; c < 3; c++) { X[n - 1][3 * k + c] = xp[c]; V[n - 1][3 * k + c] = vp[c]; }
+            for (int c = 0; c < 3; c++) { X[n - LO][3 * k + c] = xp[c]; V[n - LO][3 * k + c] = vp[c]; }
         }
         __syncthreads();
-        bsw_planet_forces<PN>(d, np, m, X[n - 1], V[n - 1], F[n - 1], OB[n - 1], HD[n - 1], A.light2,
+        bsw_planet_forces<PN>(d, np, m, X[n - LO], V[n - LO], F[n - LO], OB[n - LO], HD[n - LO], A.light2,
                               A.tab + d.tab + (long long)bsw_slot<ALG>(n, act ? r : 1) * d.ts, A.st.fault, k, act);
         if (act && mem) {
-            for (int c = 0; c < 3; c++) a[c] = F[n - 1][3 * k + c];
+            for (int c = 0; c < 3; c++) a[c] = F[n - LO][3 * k + c];
             if (r < bsw_evals<ALG>(n)) bsw_advance<ALG>(Lv, r, a, x0, v0, a0);
         }
     }
@@ -324,20 +360,25 @@ __global__ void __launch_bounds__(BsShape<ALG>::L * kBsLanes) bsw_big(BswArgs A)
         double Rr[6];
         bsw_result<ALG>(Lv, a, v0, Rr);
         for (int q = 0; q < 6; q++) D[(6 * (n - 1) + q) * kBsLanes + k] = Rr[q];
+        if constexpr (HI < L)
+            for (int q = 0; q < 6; q++) A.rsave[(long long)(6 * (n - 1) + q) * cap + d.base + 1 + k] = Rr[q];
     }
-    __syncthreads();
-    if (t < 6 * kBsLanes) {
-        const int km = t / 6, q = t % 6;
-        if (km < np) {
+    // extrapolation: item (planet, component), in passes separated by barriers
+    for (int p = 0; p < (6 * kBsLanes + NT - 1) / NT; p++) {
+        __syncthreads();
+        const int w = p * NT + t;
+        const int km = w / 6, q = w % 6;
+        if (w < 6 * kBsLanes && km < np) {
             double e[L - 3];
-            bsw_extrapolate<ALG>(A.h0, D + q * kBsLanes + km, kBsLanes, e,
-                                 A.cand + (long long)q * cap + d.base + 1 + km, cap);
-            for (int l = 0; l < L - 3; l++) E[km][q][l] = e[l];
+            bsw_extrapolate<HI>(W, D + q * kBsLanes + km, kBsLanes, e, A.cand + (long long)q * cap + d.base + 1 + km,
+                                cap, LO);
+            for (int l = ELO - 4; l <= HI - 4; l++) E[km][q][l] = e[l];
         }
     }
     __syncthreads();
-    if (t < (L - 3) * kBsLanes) {
-        const int km = t / (L - 3), l = t % (L - 3);
+    constexpr int NE = HI - ELO + 1;
+    for (int w = t; w < NE * kBsLanes; w += NT) {
+        const int km = w / NE, l = ELO - 4 + w % NE;
         if (km < np) {
             double xm[3], vm[3], xs, vs, d1[6];
             for (int c = 0; c < 3; c++) {
@@ -359,14 +400,25 @@ __global__ void __launch_bounds__(BsShape<ALG>::L * kBsLanes) bsw_big(BswArgs A)
 // level of its wave runs.  The force is the RA15 particle force on the tables
 // bsw_big wrote at the same points; each level's arithmetic and its d row are
 // those of a thread of its own.
+// bsw_particles<ALG, PN, LO, HI> runs the levels LO..HI with bsw_big's passes
+// and rsave convention: a first pass in pairs (LO+p, HI-p), the completing
+// pass one level per thread (it holds fewer levels, and its blocks need more
+// threads per slot of LDS to stay resident).
 constexpr int kBswSlots = 32;      // particles per block of bsw_particles
-template <int ALG> __host__ __device__ constexpr int bsw_threads() { return BsShape<ALG>::L / 2 * kBswSlots; }
+template <int LO, int HI> __host__ __device__ constexpr bool bsw_paired() { return LO == 1; }
+template <int LO, int HI> __host__ __device__ constexpr int bsw_threads() {
+    return (bsw_paired<LO, HI>() ? (HI - LO + 1) / 2 : HI - LO + 1) * kBswSlots;
+}
 
-template <int ALG, bool PN>
-__global__ void __launch_bounds__(bsw_threads<ALG>()) bsw_particles(BswArgs A) {
+template <int ALG, bool PN, int LO, int HI>
+__global__ void __launch_bounds__((bsw_threads<LO, HI>())) bsw_particles(BswArgs A) {
     constexpr int L = BsShape<ALG>::L;
     constexpr int NS = kBswSlots;
-    static_assert(L % 2 == 0, "levels pair up");
+    constexpr int NT = bsw_threads<LO, HI>();
+    constexpr int ELO = LO > 4 ? LO : 4;
+    constexpr bool PAIR = bsw_paired<LO, HI>();
+    static_assert(!PAIR || (HI - LO + 1) % 2 == 0, "levels pair up");
+    static_assert(LO == 1 || HI == L, "a first pass or the completing pass");
     const Sys d = A.sys[0];
     const int np = d.nbig - 1;
     const int t = threadIdx.x;
@@ -377,18 +429,27 @@ __global__ void __launch_bounds__(bsw_threads<ALG>()) bsw_particles(BswArgs A) {
     const bool live = pi_ < npart;
     const long long cap = A.st.cap;
     const int g = d.base + d.nbig + pi_;
-    __shared__ double D[6 * L * NS];
+    __shared__ double D[6 * HI * NS];
     __shared__ double E[NS][6][L - 3];
+    __shared__ double W[bsw_pairs<ALG>()][2];
     __shared__ unsigned long long EM[L - 3];
     if (t < L - 3) EM[t] = 0ull;
+    bsw_weights<ALG>(A.h0, W, t, NT);
+    if constexpr (LO > 1) {
+        for (int w = t; w < (LO - 1) * 6 * NS; w += NT) {
+            const int row = w / (6 * NS), q = w / NS % 6, s = w % NS;
+            const int ps = blockIdx.x * NS + s;
+            if (ps < npart) D[(6 * row + q) * NS + s] = A.rsave[(long long)(6 * row + q) * cap + d.base + d.nbig + ps];
+        }
+    }
     if (live) {
         double x0[3], v0[3], a0[3];
         for (int c = 0; c < 3; c++) { x0[c] = A.st.x[c * cap + g]; v0[c] = A.st.v[c * cap + g]; }
         grav_small<PN, RangeOps>(d, A.tab + d.tab, np, A.light2, x0[0], x0[1], x0[2], v0[0], v0[1], v0[2], a0[0],
                                  a0[1], a0[2], A.st.fault);
 #pragma unroll 1
-        for (int k = 0; k < 2; k++) {
-            const int n = k == 0 ? pair + 1 : L - pair;
+        for (int k = 0; k < (PAIR ? 2 : 1); k++) {
+            const int n = k == 0 ? LO + pair : HI - pair;
             BswLevel<ALG> Lv;
             bsw_start<ALG>(Lv, n, A.h0, x0, v0, a0);
             double a[3];
@@ -403,30 +464,29 @@ __global__ void __launch_bounds__(bsw_threads<ALG>()) bsw_particles(BswArgs A) {
             double Rr[6];
             bsw_result<ALG>(Lv, a, v0, Rr);
             for (int q = 0; q < 6; q++) D[(6 * (n - 1) + q) * NS + sl] = Rr[q];
+            if constexpr (HI < L)
+                for (int q = 0; q < 6; q++) A.rsave[(long long)(6 * (n - 1) + q) * cap + g] = Rr[q];
         }
     }
-    __syncthreads();
-    // extrapolation: work item (slot, component).  One pass where the block has
-    // a thread per item (a loop there would let the compiler hoist the
-    // extrapolation's coefficients into registers across iterations).
-    auto extrapolate = [&](int w) {
+    // extrapolation: work item (slot, component), in passes separated by
+    // barriers (the weights are read from LDS in each pass, never kept in
+    // registers across passes)
+    for (int p = 0; p < (6 * NS + NT - 1) / NT; p++) {
+        __syncthreads();
+        const int w = p * NT + t;
         const int s = w / 6, q = w % 6;
         const int ps = blockIdx.x * NS + s;
-        if (ps < npart) {
+        if (w < 6 * NS && ps < npart) {
             double e[L - 3];
-            bsw_extrapolate<ALG>(A.h0, D + q * NS + s, NS, e, A.cand + (long long)q * cap + d.base + d.nbig + ps, cap);
-            for (int l = 0; l < L - 3; l++) E[s][q][l] = e[l];
+            bsw_extrapolate<HI>(W, D + q * NS + s, NS, e, A.cand + (long long)q * cap + d.base + d.nbig + ps, cap, LO);
+            for (int l = ELO - 4; l <= HI - 4; l++) E[s][q][l] = e[l];
         }
-    };
-    if constexpr (6 * NS <= bsw_threads<ALG>()) {
-        if (t < 6 * NS) extrapolate(t);
-    } else {
-        for (int w = t; w < 6 * NS; w += bsw_threads<ALG>()) extrapolate(w);
     }
     __syncthreads();
-    // level errors: work item (slot, level 4 + l), merged per level over the block
-    for (int w = t; w < (L - 3) * NS; w += bsw_threads<ALG>()) {
-        const int s = w / (L - 3), l = w % (L - 3);
+    // level errors: work item (slot, level), merged per level over the block
+    constexpr int NE = HI - ELO + 1;
+    for (int w = t; w < NE * NS; w += NT) {
+        const int s = w / NE, l = ELO - 4 + w % NE;
         const int ps = blockIdx.x * NS + s;
         if (ps < npart) {
             const int gp = d.base + d.nbig + ps;
@@ -439,7 +499,7 @@ __global__ void __launch_bounds__(bsw_threads<ALG>()) bsw_particles(BswArgs A) {
         }
     }
     __syncthreads();
-    if (t < L - 3 && EM[t]) atomicMax(A.err + 3 + t, EM[t]);
+    if (t >= ELO - 4 && t <= HI - 4 && EM[t]) atomicMax(A.err + 3 + t, EM[t]);
 }
 
 // The accepted level's states for every body j >= 1, the step-start states

@@ -120,6 +120,7 @@ struct Context {
     double *rb = nullptr, *re = nullptr, *ra1 = nullptr;
     // whole-system Bulirsch-Stoer (algorithms 2, 3): accepted states per level, level errors
     double* bscand = nullptr;
+    double* bsrsave = nullptr;
     long long bs_cap = 0;
     unsigned long long* bserr = nullptr;
     unsigned long long* bmax = nullptr;
@@ -211,7 +212,7 @@ void release_all() {
     dfree(c.fault); dfree(c.dsys); dfree(c.tcur); c.sys_valid = false; c.sys_cap = 0; c.systems.clear(); dfree(c.dtiles); dfree(c.dev); dfree(c.devcount);
     dfree(c.dpark); dfree(c.dcnt); dfree(c.dslow); dfree(c.work); dfree(c.dreq); c.req_cap = 0;
     dfree(c.rb); dfree(c.re); dfree(c.ra1); dfree(c.bmax); c.ra_cap = 0;
-    dfree(c.bscand); dfree(c.bserr); c.bs_cap = 0;
+    dfree(c.bscand); dfree(c.bsrsave); dfree(c.bserr); c.bs_cap = 0;
     for (double** p : {&c.tape.tab, &c.tape.ck, &c.tape.rec, &c.tape.lam, &c.tape.part, &c.tape.src, &c.tape.grad,
                        &c.tape.rows, &c.tape.tan, &c.tape.fin, &c.tape.seed, &c.tape.psave})
         dfree(*p);
@@ -701,29 +702,31 @@ void radau_step(double time, double* h, double* hdid, double tol, int64_t* force
 int read_fault();
 void clear_fault();
 
-// One attempt of mdt_bs1 (ALG 2) or mdt_bs2 (ALG 3) at step h0: every level's
-// error comes back; returns the first level n > 3 within tol^2, 0 if none.
-template <int ALG>
-int bs_attempt(double h0, double tol2) {
+// The kernels of one pass over the levels LO..HI.
+template <int ALG, int LO, int HI>
+void bs_pass(const BswArgs& A) {
     Context& c = ctx;
-    constexpr int L = BsShape<ALG>::L;
-    BswArgs A{c.dsys, device_state(), c.tab, c.bscand, c.bserr, h0, light_speed2()};
-    hipLaunchKernelGGL(zero_bytes, dim3(1), dim3(128), 0, c.stream, reinterpret_cast<unsigned char*>(c.bserr),
-                       (long long)(12 * sizeof(unsigned long long)));
-    if (c.pn) hipLaunchKernelGGL((bsw_big<ALG, true>), dim3(1), dim3(L * kBsLanes), 0, c.stream, A);
-    else hipLaunchKernelGGL((bsw_big<ALG, false>), dim3(1), dim3(L * kBsLanes), 0, c.stream, A);
+    if (c.pn) hipLaunchKernelGGL((bsw_big<ALG, true, LO, HI>), dim3(1), dim3((HI - LO + 1) * kBsLanes), 0, c.stream, A);
+    else hipLaunchKernelGGL((bsw_big<ALG, false, LO, HI>), dim3(1), dim3((HI - LO + 1) * kBsLanes), 0, c.stream, A);
     const int npart = c.n - c.nbig;
     if (npart > 0) {
         dim3 grid((npart + kBswSlots - 1) / kBswSlots);
-        if (c.pn) hipLaunchKernelGGL((bsw_particles<ALG, true>), grid, dim3(bsw_threads<ALG>()), 0, c.stream, A);
-        else hipLaunchKernelGGL((bsw_particles<ALG, false>), grid, dim3(bsw_threads<ALG>()), 0, c.stream, A);
+        if (c.pn)
+            hipLaunchKernelGGL((bsw_particles<ALG, true, LO, HI>), grid, dim3(bsw_threads<LO, HI>()), 0, c.stream, A);
+        else
+            hipLaunchKernelGGL((bsw_particles<ALG, false, LO, HI>), grid, dim3(bsw_threads<LO, HI>()), 0, c.stream, A);
     }
     HIPCHECK(hipGetLastError());
+}
+
+// The first level n > 3 of LO..HI whose largest error is within tol^2, 0 if none.
+int bs_first_level(int lo, int hi, double tol2) {
+    Context& c = ctx;
     unsigned long long bits[12];
     HIPCHECK(hipMemcpyAsync(bits, c.bserr, sizeof(bits), hipMemcpyDeviceToHost, c.stream));
     HIPCHECK(hipStreamSynchronize(c.stream));
     if (read_fault() & FAULT_FORCE) throw std::runtime_error("Nonfinite BS extrapolation");
-    for (int n = 4; n <= L; n++) {
+    for (int n = std::max(lo, 4); n <= hi; n++) {
         double err;
         std::memcpy(&err, &bits[n - 1], sizeof(err));
         if (err <= tol2) return n;
@@ -731,10 +734,31 @@ int bs_attempt(double h0, double tol2) {
     return 0;
 }
 
+// One attempt of mdt_bs1 (ALG 2) or mdt_bs2 (ALG 3) at step h0: the first
+// level n > 3 within tol^2, 0 if none.  The levels are independent up to the
+// extrapolation, whose value at level n needs levels 1..n only, so the device
+// runs levels 1..6 (BS) or 1..8 (BS2) first and the remaining ones only when
+// none of those converged -- the step control keeps most accepted levels in
+// that range (it shrinks the step after level 8 or higher and grows it before
+// level 7).  The sequential reference stops at the same first level.
+template <int ALG>
+int bs_attempt(double h0, double tol2) {
+    Context& c = ctx;
+    constexpr int L = BsShape<ALG>::L;
+    constexpr int LA = ALG == 2 ? 6 : 8;
+    BswArgs A{c.dsys, device_state(), c.tab, c.bscand, c.bsrsave, c.bserr, h0, light_speed2()};
+    hipLaunchKernelGGL(zero_bytes, dim3(1), dim3(128), 0, c.stream, reinterpret_cast<unsigned char*>(c.bserr),
+                       (long long)(12 * sizeof(unsigned long long)));
+    bs_pass<ALG, 1, LA>(A);
+    if (const int n = bs_first_level(1, LA, tol2)) return n;
+    bs_pass<ALG, LA + 1, L>(A);
+    return bs_first_level(LA + 1, L, tol2);
+}
+
 // mdt_bs1 / mdt_bs2 with mfo_all: attempts at h, halving until a level
 // converges; then the Fortran's step recommendation (x .55 after the last
 // level, x 1.3 before level 7).  The levels run on the device; the host reads
-// the 12 level errors per attempt.
+// the level errors after each pass of an attempt.
 void bs_step(int alg, double time, double* h, double* hdid, double tol, int64_t* forces, int64_t* rejected) {
     Context& c = ctx;
     const double SHRINK = .55, GROW = 1.3;
@@ -744,6 +768,7 @@ void bs_step(int alg, double time, double* h, double* hdid, double tol, int64_t*
     build_systems(1 + P, true, *h, 0.0);
     if (c.bs_cap != c.cap) {
         dalloc(c.bscand, (size_t)9 * 6 * c.cap);       // levels 4..12
+        dalloc(c.bsrsave, (size_t)8 * 6 * c.cap);      // the first pass's level results
         if (!c.bserr) dalloc(c.bserr, 12);
         c.bs_cap = c.cap;
     }
@@ -755,7 +780,7 @@ void bs_step(int alg, double time, double* h, double* hdid, double tol, int64_t*
         const int level = alg == 2 ? bs_attempt<2>(*h, tol2) : bs_attempt<3>(*h, tol2);
         if (level) {
             for (int n = 1; n <= level; n++) *forces += alg == 2 ? 2 * n : n;
-            BswArgs A{c.dsys, device_state(), c.tab, c.bscand, c.bserr, *h, light_speed2()};
+            BswArgs A{c.dsys, device_state(), c.tab, c.bscand, c.bsrsave, c.bserr, *h, light_speed2()};
             if (c.n > 1) hipLaunchKernelGGL(bsw_accept, dim3((c.n - 1 + 127) / 128), dim3(128), 0, c.stream, A, level);
             HIPCHECK(hipGetLastError());
             HIPCHECK(hipStreamSynchronize(c.stream));
