@@ -1100,12 +1100,13 @@ __global__ void __launch_bounds__(kBsPairThreads) bs2_loop_paired(BsArgs A, BsLo
     double ab[3] = {0, 0, 0};
     for (int r = 1; r <= kLevels; r++) {
         const bool act = mem && r <= n;
+        // every (level, round) has its own point in TB, so a round's writes never
+        // meet the previous round's reads: one barrier per round
         double* Pt = TB + bs_point(n, r) * 3 * nb;
         if (act)
             for (int qc = 0; qc < 3; qc++) Pt[3 * k + qc] = LB.xend[qc];
         __syncthreads();
         if (act) big_accel(S, A.nbbp, k + 1, Pt, ab);
-        __syncthreads();
         if (act && r < n) bs_level_step(LB, r, ab, bx0, bv0, ba0);
     }
     // particles: mfo_hkce with the Big positions Xb(i)
