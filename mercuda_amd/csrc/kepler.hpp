@@ -250,8 +250,9 @@ __device__ __forceinline__ bool drift_fast(double mu, Orbit& o, double dt) {
     return true;
 }
 
-// drift_dan (mercury6_2.for:7602-7752): returns iflg.
-__device__ inline int drift_dan(double mu, Orbit& o, double dt0) {
+// drift_dan (mercury6_2.for:7602-7752): returns iflg.  *kep (when given)
+// is set when the drift took the kepmd branch and passed its fchk test.
+__device__ inline int drift_dan(double mu, Orbit& o, double dt0, bool* kep = nullptr) {
     double dt = dt0;
     double r0 = sqrt(o.x * o.x + o.y * o.y + o.z * o.z);
     double v0s = o.vx * o.vx + o.vy * o.vy + o.vz * o.vz;
@@ -281,6 +282,7 @@ __device__ inline int drift_dan(double mu, Orbit& o, double dt0) {
             double nvx = o.x * fdot + o.vx * gdot, nvy = o.y * fdot + o.vy * gdot,
                    nvz = o.z * fdot + o.vz * gdot;
             o.x = nx; o.y = ny; o.z = nz; o.vx = nvx; o.vy = nvy; o.vz = nvz;
+            if (kep) *kep = true;
             return 0;
         }
     }
@@ -297,17 +299,25 @@ __device__ inline int drift_dan(double mu, Orbit& o, double dt0) {
     return iflg;
 }
 
-// drift_one (mercury6_2.for:8279-8352): a failed drift is retried as ten
-// drifts of dt/10, stopping at the first failure with partial progress kept.
-__device__ inline int drift_one(double mu, Orbit& o, double dt) {
-    int iflg = drift_dan(mu, o, dt);
-    if (iflg != 0) {
-        for (int i = 1; i <= 10; i++) {
-            double dttmp = dt / 10.0;
-            iflg = drift_dan(mu, o, dttmp);
-            if (iflg != 0) return iflg;
-        }
+// drift_one's retry of a failed drift as ten drifts of dt/10, stopping at the
+// first failure with partial progress kept (not inlined: rarely taken).
+__device__ __noinline__ int drift_retry(double mu, Orbit& o, double dt) {
+    int iflg = 0;
+    for (int i = 1; i <= 10; i++) {
+        double dttmp = dt / 10.0;
+        iflg = drift_dan(mu, o, dttmp);
+        if (iflg != 0) return iflg;
     }
+    return iflg;
+}
+
+// drift_one (mercury6_2.for:8279-8352).  *kep (when given) reports whether the
+// first drift_dan took the kepmd branch (then no retry follows): the drift
+// drift_fast would have made.
+__device__ inline int drift_one(double mu, Orbit& o, double dt, bool* kep = nullptr) {
+    if (kep) *kep = false;
+    int iflg = drift_dan(mu, o, dt, kep);
+    if (iflg != 0) iflg = drift_retry(mu, o, dt);
     return iflg;
 }
 

@@ -65,6 +65,14 @@ struct Timing {
 };
 
 constexpr int kRing = 8;   // launches in flight: table, counter and request slots
+// control words of an MVS call (Context::dcnt)
+constexpr int kCtlCount = 0;               // entries of slow-set list 0 and list 1
+constexpr int kCtlPar = 2;                 // index of the current list
+constexpr int kCtlLeave = 3;               // bodies leaving the slow set after the call
+constexpr int kCtlPers = 4;                // kept members at the call's start
+constexpr int kCtlSnap = 5;                // [kRing] current-list entries after launch k's fast kernel
+constexpr int kCtlReq = 5 + kRing;         // [kRing] screening requests of ring slot k
+constexpr int kCtlInts = 5 + 2 * kRing;
 
 struct Context {
     int algorithm = 1;
@@ -87,16 +95,21 @@ struct Context {
     DevEvent* dev = nullptr;
     int* devcount = nullptr;
     int ev_cap = 0;
-    Park* dpark = nullptr;           // slow-set list of the current call
-    int* dcnt = nullptr;             // [0] slow-set entries, [1+k] entry count after launch k's fast
-                                     // kernel, [1+kRing+k] screening requests of ring slot k
+    Park* dpark = nullptr;           // the two slow-set lists (park_cap entries each)
+    int* dleave = nullptr;           // bodies leaving the slow set
+    int* dcnt = nullptr;             // control words (kCtl*)
     unsigned char* dslow = nullptr;  // slow-set flag per body
     int park_cap = 0;
+    int* hstatus = nullptr;          // pinned host words the call's last kernel writes (call_end)
+    int* dstatus = nullptr;          // their device address
+    unsigned upload_gen = 0;         // bumped by every upload (captured graphs depend on it)
     ScreenReq* dreq = nullptr;       // kRing slots of req_cap requests
     int req_cap = 0;
     long long slot_doubles = 0;      // table doubles per ring slot
     hipStream_t sp = nullptr, sg = nullptr;   // planet and general-kernel streams
+    hipStream_t sm = nullptr;        // the MVS pipeline's particle stream (the main stream)
     hipEvent_t ev_plan[kRing + 1] = {}, ev_fast[kRing + 1] = {}, ev_done[kRing + 1] = {};
+    hipEvent_t ev_gate = nullptr;    // the first fast launch of a call waits for it
     // host
     std::vector<double> stage;
     std::vector<DevEvent> hev;
@@ -104,6 +117,7 @@ struct Context {
     std::vector<MercuryHipEvent> adv_events;
     struct HostSys { int base, n, nbig; double mu, h, rcen, j2, j4, j6; };
     std::vector<HostSys> systems;    // one entry, or the members of an ensemble
+    bool ensemble = false;           // systems came from ensemble_upload (each carries its h, rcen)
     std::vector<Sys> hsys;           // device descriptors last uploaded
     bool sys_valid = false;
     int sys_cap = 0;
@@ -115,6 +129,7 @@ struct Context {
     int gen_per_wave = 4;
     int64_t parked = 0;
     unsigned long long* work = nullptr;   // device particle-step counters (timing mode)
+    long long work_total = 0;             // particle-steps of the MVS calls since timing was enabled
     Timing timing;
     // RA15 state (algorithm 4): b, e [21][cap], a1 [3][cap], max |b7|
     double *rb = nullptr, *re = nullptr, *ra1 = nullptr;
@@ -162,9 +177,10 @@ Context ctx;
 
 // What a captured advance depends on: replay only when all of it is unchanged.
 struct GraphKey {
-    int nsteps, screen, reset, spl, block, gpw, ntiles, nsys, ev_cap, req_cap, park_cap;
+    int nsteps, screen, reset, spl, block, gpw, ntiles, nsys, ev_cap, req_cap, park_cap, n, npart;
+    unsigned gen;
     long long cap, slot;
-    const void* ptr[9];
+    const void* ptr[11];
     bool operator==(const GraphKey& o) const { return std::memcmp(this, &o, sizeof(GraphKey)) == 0; }
 };
 
@@ -188,8 +204,15 @@ void ensure_stream() {
     if (ctx.stream) return;
     if (const char* e = std::getenv("MERCURY_HIP_GEN_PER_WAVE")) ctx.gen_per_wave = std::max(1, std::min(64, std::atoi(e)));
     HIPCHECK(hipStreamCreateWithFlags(&ctx.stream, hipStreamNonBlocking));
+    ctx.sm = ctx.stream;
     HIPCHECK(hipStreamCreateWithFlags(&ctx.sp, hipStreamNonBlocking));
     HIPCHECK(hipStreamCreateWithFlags(&ctx.sg, hipStreamNonBlocking));
+    if (!ctx.hstatus) {
+        HIPCHECK(hipHostMalloc(reinterpret_cast<void**>(&ctx.hstatus), 64, hipHostMallocDefault));
+        std::memset(ctx.hstatus, 0, 64);
+        HIPCHECK(hipHostGetDevicePointer(reinterpret_cast<void**>(&ctx.dstatus), ctx.hstatus, 0));
+    }
+    HIPCHECK(hipEventCreateWithFlags(&ctx.ev_gate, hipEventDisableTiming));
     for (int k = 0; k <= kRing; k++) {
         HIPCHECK(hipEventCreateWithFlags(&ctx.ev_plan[k], hipEventDisableTiming));
         HIPCHECK(hipEventCreateWithFlags(&ctx.ev_fast[k], hipEventDisableTiming));
@@ -198,6 +221,7 @@ void ensure_stream() {
 }
 
 void sync_all() {
+    HIPCHECK(hipStreamSynchronize(ctx.sm));
     HIPCHECK(hipStreamSynchronize(ctx.sp));
     HIPCHECK(hipStreamSynchronize(ctx.stream));
     HIPCHECK(hipStreamSynchronize(ctx.sg));
@@ -210,7 +234,7 @@ void release_all() {
     for (double** p : {&c.x, &c.v, &c.a, &c.m, &c.rce, &c.rphys, &c.ox, &c.ov, &c.v0k, &c.v1, &c.tab, &c.cx, &c.cv})
         dfree(*p);
     dfree(c.fault); dfree(c.dsys); dfree(c.tcur); c.sys_valid = false; c.sys_cap = 0; c.systems.clear(); dfree(c.dtiles); dfree(c.dev); dfree(c.devcount);
-    dfree(c.dpark); dfree(c.dcnt); dfree(c.dslow); dfree(c.work); dfree(c.dreq); c.req_cap = 0;
+    dfree(c.dpark); dfree(c.dleave); dfree(c.dcnt); dfree(c.dslow); dfree(c.work); dfree(c.dreq); c.req_cap = 0;
     dfree(c.rb); dfree(c.re); dfree(c.ra1); dfree(c.bmax); c.ra_cap = 0;
     dfree(c.bscand); dfree(c.bsrsave); dfree(c.bserr); c.bs_cap = 0;
     for (double** p : {&c.tape.tab, &c.tape.ck, &c.tape.rec, &c.tape.lam, &c.tape.part, &c.tape.src, &c.tape.grad,
@@ -248,9 +272,10 @@ void ensure_capacity(int n) {
     dfree(ctx.cv);
     for (double** p : {&ctx.m, &ctx.rce, &ctx.rphys}) dalloc(*p, cap);
     if (!ctx.fault) dalloc(ctx.fault, 1);
-    ctx.park_cap = (int)std::min<long long>(cap, 1LL << 30);
-    dalloc(ctx.dpark, (size_t)ctx.park_cap);
-    if (!ctx.dcnt) dalloc(ctx.dcnt, 1 + 2 * kRing);
+    ctx.park_cap = (int)std::min<long long>(cap, 1LL << 29);
+    dalloc(ctx.dpark, (size_t)2 * ctx.park_cap);
+    dalloc(ctx.dleave, (size_t)ctx.park_cap);
+    if (!ctx.dcnt) dalloc(ctx.dcnt, kCtlInts);
     dalloc(ctx.dslow, (size_t)cap);
     ctx.req_cap = (int)std::min<long long>(std::max<long long>(cap / 2, 1 << 18), 1LL << 26);
     dalloc(ctx.dreq, (size_t)ctx.req_cap * kRing);
@@ -260,6 +285,13 @@ void ensure_capacity(int n) {
         dalloc(ctx.dev, (size_t)ctx.ev_cap);
     }
     ctx.cap = cap;
+}
+
+// Empties the slow set (a new state, or one put back): every particle starts
+// the next call in the fast kernel.
+void clear_slow_set() {
+    if (ctx.dslow) HIPCHECK(hipMemset(ctx.dslow, 0, (size_t)ctx.cap));
+    if (ctx.dcnt) HIPCHECK(hipMemset(ctx.dcnt, 0, sizeof(int) * kCtlInts));
 }
 
 void ensure_table(long long doubles) {
@@ -411,34 +443,44 @@ void launch_general(bool screen, bool save, bool uniform, dim3 grid, dim3 block,
     HIPCHECK(hipGetLastError());
 }
 
-// Launch lengths for nsteps with at most smax per launch: a short ramp at the
-// start (the first particle launch waits only for a short planet table) and at
-// the end (the last slow-set launch, which follows the last fast launch, is
-// short).  Launch boundaries do not change results.
+// Launch lengths for nsteps with at most smax per launch: a short ramp 1, 2
+// (MERCURY_HIP_RAMP levels) at the start, so that the first particle launch
+// waits only for a one-step planet table, and the same ramp down at the end,
+// so that the general launch after the last fast launch covers one step; at
+// every call length.  Launch boundaries do not change results.
 std::vector<int> chunk_plan(int nsteps, int smax) {
-    std::vector<int> head, tail, plan;
-    if (smax >= 16 && nsteps >= 4 * smax) {
-        for (int c = 4; c < smax; c *= 2) { head.push_back(c); tail.push_back(c); }
-    }
+    static const int levels = std::getenv("MERCURY_HIP_RAMP") ? std::atoi(std::getenv("MERCURY_HIP_RAMP")) : 0;
+    std::vector<int> head, plan;
     int rest = nsteps;
-    for (int c : head) rest -= c;
-    for (int c : tail) rest -= c;
+    for (int c = 1; (int)head.size() < levels && c < smax && rest >= 2 * c; c *= 2) {
+        head.push_back(c);
+        rest -= 2 * c;
+    }
     plan = head;
     while (rest > 0) { int c = std::min(smax, rest); plan.push_back(c); rest -= c; }
-    for (auto it = tail.rbegin(); it != tail.rend(); ++it) plan.push_back(*it);
+    for (auto it = head.rbegin(); it != head.rend(); ++it) plan.push_back(*it);
     return plan;
 }
 
-// nsteps MVS steps of the single system in launches of at most `smax` steps.
-// Per launch k (ring slot k % kRing):
-//   planet stream:   planets (table of the launch's steps)
-//   main stream:     fast particle kernel, then the slow-set count snapshot
-//   general stream:  slow-set particles, then the deferred screening details
-// The planet stream runs up to kRing launches ahead; the general kernel of
-// launch k overlaps the fast kernel of launch k+1 (disjoint particles).
-// Enqueue only (no host synchronisation): usable under stream capture.
-// device_time: every planet launch reads the systems' clocks from the device
-// (set beforehand) instead of taking the call's time as an argument.
+// nsteps MVS steps of the single system or ensemble in launches of at most
+// `smax` steps.  First, on the planet stream, call_begin (slow-set rotation,
+// counters).  Per launch k (ring slot k % kRing):
+//   planet stream:   planets
+//   general stream:  the slow set over launch k's steps, once their planet
+//                    tables exist and launch k-1's fast kernel has ended (it
+//                    first finishes launch k-1's parks): beside launch k's fast
+//                    kernel; then launch k-1's deferred screening details
+//   main stream:     the fast particle kernel once the launch's planet tables
+//                    exist, then the slow-set count snapshot
+// A last general launch finishes the last fast kernel's parks.  The
+// latency-bound chains (planets, slow set) thus run beside the fast kernel
+// instead of before or after it; the planet stream runs up to kRing launches
+// ahead.  Last, the streams join the main stream and call_end writes the
+// host's status words.  Enqueue only (no host synchronisation): usable under
+// stream capture.  device_time: every planet launch reads the systems' clocks
+// from the device (set beforehand) instead of taking the call's time as an
+// argument; this is the captured form, whose planet stream first joins the
+// capture.
 void mvs_enqueue(double time, double h, int nsteps, bool reset, bool screen, bool save, bool device_time) {
     Context& c = ctx;
     const int smax = save ? 1 : c.steps_per_launch;
@@ -446,15 +488,17 @@ void mvs_enqueue(double time, double h, int nsteps, bool reset, bool screen, boo
     const bool single = nsys == 1;
     State st = device_state();
     EventSink es{c.dev, c.devcount, c.ev_cap, c.fault};
-    const long long nb = c.n;
-    hipLaunchKernelGGL(zero_bytes, dim3((unsigned)std::min<long long>((nb + 255) / 256, 4096)), dim3(256), 0,
-                       c.stream, c.dslow, nb);
-    hipLaunchKernelGGL(zero_ints, dim3(1), dim3(64), 0, c.stream, c.dcnt, 1 + 2 * kRing);
+    if (device_time) {
+        HIPCHECK(hipEventRecord(c.ev_fast[kRing], c.sm));
+        HIPCHECK(hipStreamWaitEvent(c.sp, c.ev_fast[kRing], 0));
+    }
+    int* const count2 = c.dcnt + kCtlCount;
+    int* const par = c.dcnt + kCtlPar;
+    int* const pers = c.dcnt + kCtlPers;
+    CallBeginArgs cb{c.dslow, c.dleave, count2, par, pers, c.dcnt + kCtlLeave, c.dcnt + kCtlSnap, 2 * kRing,
+                     c.devcount, c.fault};
+    hipLaunchKernelGGL(call_begin, dim3(1), dim3(256), 0, c.sp, cb);
     HIPCHECK(hipGetLastError());
-    // the other streams start after the counters are cleared
-    HIPCHECK(hipEventRecord(c.ev_fast[0], c.stream));
-    HIPCHECK(hipStreamWaitEvent(c.sp, c.ev_fast[0], 0));
-    HIPCHECK(hipStreamWaitEvent(c.sg, c.ev_fast[0], 0));
     const int npart = c.npart;
     // The slow-set and screening kernels grid-stride over counts only the device
     // knows (a few hundred slow-set members and tens of thousands of requests per
@@ -465,60 +509,110 @@ void mvs_enqueue(double time, double h, int nsteps, bool reset, bool screen, boo
     double t = time;
     int done = 0;
     const std::vector<int> plan = chunk_plan(nsteps, smax);
-    for (int k = 0; k < (int)plan.size(); k++) {
+    const int nk = (int)plan.size();
+    const bool parts = c.ntiles > 0;
+    auto slot_tab = [&](int k) { return c.tab + (long long)(k % kRing) * c.slot_doubles; };
+    auto park_list = [&](int chunk, bool last) {
+        const int* snap = c.dcnt + kCtlSnap + ((chunk + kRing - 1) % kRing);   // after fast kernel chunk-1
+        return ParkList{c.dpark, count2, par, pers, snap, c.dleave, c.dcnt + kCtlLeave, c.dslow, c.park_cap,
+                        chunk, c.gen_per_wave, last ? 1 : 0};
+    };
+    auto req_sink = [&](int k) {
+        return ReqSink{c.dreq + (size_t)(k % kRing) * c.req_cap, c.dcnt + kCtlReq + (k % kRing), c.req_cap, c.fault};
+    };
+    auto general = [&](const ParticleArgs& a) {
+        hipEvent_t g0 = nullptr;
+        timed_launch_begin(c.sg, g0);
+        launch_general(screen, save, single, dim3(gblocks), dim3(64), c.sg, a);
+        timed_launch_end(K_GENERAL, c.sg, g0);
+    };
+    auto screen_details = [&](int k, int step0) {
+        if (!screen) return;
+        RequestArgs ra{c.dsys, st, slot_tab(k), es, req_sink(k), step0};
+        hipLaunchKernelGGL(screen_requests, dim3(rblocks), dim3(256), 0, c.sg, ra);
+        HIPCHECK(hipGetLastError());
+    };
+    int prev_done = 0;
+    for (int k = 0; k < nk; k++) {
         const int steps = plan[k];
         const int slot = k % kRing;
-        double* tab = c.tab + (long long)slot * c.slot_doubles;
+        double* tab = slot_tab(k);
         const bool rs0 = reset && k == 0;
-        int* snap = c.dcnt + 1 + slot;
-        int* reqcount = c.dcnt + 1 + kRing + slot;
+        const bool last = k + 1 == nk;
+        int* snap = c.dcnt + kCtlSnap + slot;
+        int* reqcount = c.dcnt + kCtlReq + slot;
         hipEvent_t t0 = nullptr;
         if (k >= kRing) HIPCHECK(hipStreamWaitEvent(c.sp, c.ev_done[slot], 0));
+        ParticleArgs args{c.dsys, c.dtiles, st, tab, es, park_list(k, false), req_sink(k),
+                          c.timing.on ? c.work : nullptr, steps, rs0 ? 1 : 0, screen ? 1 : 0, save ? 1 : 0, done};
         // one system follows the caller's clock; ensemble members carry theirs on the device
-        PlanetArgs pa{c.dsys, st, tab, c.tcur, es, t, (!device_time && (single || k == 0)) ? 1 : 0, steps,
-                      rs0 ? 1 : 0, screen ? 1 : 0, save ? 1 : 0, reqcount, 1, done};
+        const bool set_time = !device_time && ((single && !c.ensemble) || k == 0);
+        PlanetArgs pa{c.dsys, st, tab, c.tcur, es, t, set_time ? 1 : 0, steps, rs0 ? 1 : 0, screen ? 1 : 0,
+                      save ? 1 : 0, reqcount, 1, done};
         timed_launch_begin(c.sp, t0);
         hipLaunchKernelGGL(mvs_planets, dim3(nsys), dim3(64), 0, c.sp, pa);
         HIPCHECK(hipGetLastError());
         timed_launch_end(K_PLANET, c.sp, t0);
+        for (int q = 0; q < steps; q++) t = t + h;
         HIPCHECK(hipEventRecord(c.ev_plan[slot], c.sp));
-        if (c.ntiles > 0) {
-            ReqSink rs{c.dreq + (size_t)slot * c.req_cap, reqcount, c.req_cap, c.fault};
-            ParkList pl{c.dpark, c.dcnt, snap, c.dslow, c.park_cap, k, c.gen_per_wave};
-            ParticleArgs args{c.dsys, c.dtiles, st, tab, es, pl, rs, c.timing.on ? c.work : nullptr,
-                              steps, rs0 ? 1 : 0, screen ? 1 : 0, save ? 1 : 0, done};
-            HIPCHECK(hipStreamWaitEvent(c.stream, c.ev_plan[slot], 0));
-            timed_launch_begin(c.stream, t0);
-            launch_fast(screen, save, single ? c.systems[0].nbig - 1 : -1, dim3(c.ntiles), dim3(c.block), c.stream,
-                        args);
-            timed_launch_end(K_PARTICLE, c.stream, t0);
-            hipLaunchKernelGGL(copy_int, dim3(1), dim3(64), 0, c.stream, c.dcnt, snap);
-            HIPCHECK(hipEventRecord(c.ev_fast[slot], c.stream));
-            HIPCHECK(hipStreamWaitEvent(c.sg, c.ev_fast[slot], 0));
-            timed_launch_begin(c.sg, t0);
-            launch_general(screen, save, single, dim3(gblocks), dim3(64), c.sg, args);
-            timed_launch_end(K_GENERAL, c.sg, t0);
-            if (screen) {
-                RequestArgs ra{c.dsys, st, tab, es, rs, done};
-                hipLaunchKernelGGL(screen_requests, dim3(rblocks), dim3(256), 0, c.sg, ra);
-                HIPCHECK(hipGetLastError());
+        if (parts) {
+            // the slow set over this launch's steps, beside its fast kernel
+            ParticleArgs ga = args;
+            ga.park = park_list(k, last);
+            if (k > 0) {
+                ga.ptab = slot_tab(k - 1);
+                ga.psteps = plan[k - 1];
+                HIPCHECK(hipStreamWaitEvent(c.sg, c.ev_fast[(k - 1) % kRing], 0));
             }
+            HIPCHECK(hipStreamWaitEvent(c.sg, c.ev_plan[slot], 0));
+            // The call's first fast launch starts behind a marker on the general
+            // stream: the slow set's waves, dispatched right after it, then take
+            // their CU slots before the fast kernel's fill every SIMD's registers.
+            hipEvent_t gate = c.ev_plan[slot];
+            if (k == 0) {
+                hipLaunchKernelGGL(marker, dim3(1), dim3(64), 0, c.sg);
+                HIPCHECK(hipGetLastError());
+                HIPCHECK(hipEventRecord(c.ev_gate, c.sg));
+                gate = c.ev_gate;
+            }
+            general(ga);
+            if (k > 0) {
+                // launch k-1's screening details; its table slot is free after them
+                screen_details(k - 1, prev_done);
+                HIPCHECK(hipEventRecord(c.ev_done[(k - 1) % kRing], c.sg));
+            }
+            HIPCHECK(hipStreamWaitEvent(c.sm, gate, 0));
+            timed_launch_begin(c.sm, t0);
+            launch_fast(screen, save, single ? c.systems[0].nbig - 1 : -1, dim3(c.ntiles), dim3(c.block), c.sm,
+                        args);
+            timed_launch_end(K_PARTICLE, c.sm, t0);
+            hipLaunchKernelGGL(snap_count, dim3(1), dim3(64), 0, c.sm, count2, par, snap);
+            HIPCHECK(hipEventRecord(c.ev_fast[slot], c.sm));
         } else {
             HIPCHECK(hipStreamWaitEvent(c.sg, c.ev_plan[slot], 0));
+            HIPCHECK(hipEventRecord(c.ev_done[slot], c.sg));
         }
-        HIPCHECK(hipEventRecord(c.ev_done[slot], c.sg));
-        for (int q = 0; q < steps; q++) t = t + h;
+        prev_done = done;
         done += steps;
     }
-}
-
-// The planet and general streams' work joins the main stream.
-void join_streams() {
-    Context& c = ctx;
+    if (parts && nk > 0) {
+        // the last fast kernel's parks finish their block; then its screening details
+        ParticleArgs fa{c.dsys, c.dtiles, st, slot_tab(nk), es, park_list(nk, true), req_sink(nk - 1),
+                        c.timing.on ? c.work : nullptr, 0, 0, screen ? 1 : 0, save ? 1 : 0, done};
+        fa.ptab = slot_tab(nk - 1);
+        fa.psteps = plan[nk - 1];
+        HIPCHECK(hipStreamWaitEvent(c.sg, c.ev_fast[(nk - 1) % kRing], 0));
+        general(fa);
+        screen_details(nk - 1, prev_done);
+        HIPCHECK(hipEventRecord(c.ev_done[(nk - 1) % kRing], c.sg));
+    }
+    // the planet and general streams join the main stream; then the status words
     HIPCHECK(hipEventRecord(c.ev_plan[kRing], c.sp));
-    HIPCHECK(hipEventRecord(c.ev_fast[kRing], c.sg));
-    HIPCHECK(hipStreamWaitEvent(c.stream, c.ev_plan[kRing], 0));
-    HIPCHECK(hipStreamWaitEvent(c.stream, c.ev_fast[kRing], 0));
+    HIPCHECK(hipEventRecord(c.ev_done[kRing], c.sg));
+    HIPCHECK(hipStreamWaitEvent(c.sm, c.ev_plan[kRing], 0));
+    HIPCHECK(hipStreamWaitEvent(c.sm, c.ev_done[kRing], 0));
+    hipLaunchKernelGGL(call_end, dim3(1), dim3(64), 0, c.sm, count2, par, c.devcount, c.fault, c.dstatus);
+    HIPCHECK(hipGetLastError());
 }
 
 __global__ void set_clocks(double* tcur, int nsys, double time) {
@@ -532,10 +626,12 @@ void mvs_run(double time, double h, int nsteps, bool reset, bool screen, bool sa
     const int nsys = (int)c.systems.size();
     const bool single = nsys == 1;
     if (save && !single) throw std::runtime_error("single-step mode needs one system");
-    build_systems(smax, single, h, rcen);
+    // an uploaded system follows the call's h and rcen; ensemble members carry theirs
+    build_systems(smax, single && !c.ensemble, h, rcen);
+    if (c.timing.on) c.work_total += (long long)c.npart * nsteps;
     if (!graphs.enabled || save || c.timing.on) {
         mvs_enqueue(time, h, nsteps, reset, screen, save, false);
-        sync_all();
+        HIPCHECK(hipStreamSynchronize(c.sm));   // every stream has joined it
         return;
     }
     // hipGraph: capture the call's whole launch sequence once, replay it for
@@ -545,24 +641,24 @@ void mvs_run(double time, double h, int nsteps, bool reset, bool screen, bool sa
     k.nsteps = nsteps; k.screen = screen; k.reset = reset; k.spl = c.steps_per_launch; k.block = c.block;
     k.gpw = c.gen_per_wave; k.ntiles = c.ntiles; k.nsys = nsys; k.ev_cap = c.ev_cap; k.req_cap = c.req_cap;
     k.park_cap = c.park_cap; k.cap = c.cap; k.slot = c.slot_doubles;
-    const void* ptrs[9] = {c.dsys, c.dtiles, c.tab, c.dev, c.dreq, c.dpark, c.x, c.tcur, c.dslow};
-    for (int q = 0; q < 9; q++) k.ptr[q] = ptrs[q];
-    hipLaunchKernelGGL(set_clocks, dim3(1), dim3(256), 0, c.stream, c.tcur, nsys, time);
+    k.n = c.n; k.npart = c.npart; k.gen = c.upload_gen;
+    const void* ptrs[11] = {c.dsys, c.dtiles, c.tab, c.dev, c.dreq, c.dpark, c.x, c.tcur, c.dslow, c.dleave, c.dcnt};
+    for (int q = 0; q < 11; q++) k.ptr[q] = ptrs[q];
+    hipLaunchKernelGGL(set_clocks, dim3(1), dim3(256), 0, c.sm, c.tcur, nsys, time);
     HIPCHECK(hipGetLastError());
     if (!(graphs.valid && graphs.key == k)) {
         graphs.drop();
-        HIPCHECK(hipStreamSynchronize(c.stream));
-        HIPCHECK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+        HIPCHECK(hipStreamSynchronize(c.sm));
+        HIPCHECK(hipStreamBeginCapture(c.sm, hipStreamCaptureModeThreadLocal));
         try {
             mvs_enqueue(time, h, nsteps, reset, screen, false, true);
-            join_streams();
         } catch (...) {
             hipGraph_t g = nullptr;
-            (void)hipStreamEndCapture(c.stream, &g);
+            (void)hipStreamEndCapture(c.sm, &g);
             if (g) (void)hipGraphDestroy(g);
             throw;
         }
-        HIPCHECK(hipStreamEndCapture(c.stream, &graphs.graph));
+        HIPCHECK(hipStreamEndCapture(c.sm, &graphs.graph));
         HIPCHECK(hipGraphInstantiate(&graphs.exec, graphs.graph, nullptr, nullptr, 0));
         graphs.key = k;
         graphs.valid = true;
@@ -570,8 +666,8 @@ void mvs_run(double time, double h, int nsteps, bool reset, bool screen, bool sa
     } else {
         graphs.replays++;
     }
-    HIPCHECK(hipGraphLaunch(graphs.exec, c.stream));
-    HIPCHECK(hipStreamSynchronize(c.stream));
+    HIPCHECK(hipGraphLaunch(graphs.exec, c.sm));
+    HIPCHECK(hipStreamSynchronize(c.sm));
 }
 
 // ---- RA15 (algorithm 4) -----------------------------------------------------
@@ -1568,7 +1664,10 @@ int mercury_hip_upload(int n, int nbig, int pn, int ngflag, const double* m, con
         ctx.n = n; ctx.nbig = nbig; ctx.pn = pn; ctx.ngflag = ngflag;
         for (int k = 0; k < 3; k++) ctx.jcen[k] = jcen[k];
         ctx.systems.assign(1, Context::HostSys{0, n, nbig, m[0], 0.0, 0.0, jcen[0], jcen[1], jcen[2]});
+        ctx.ensemble = false;
         ctx.sys_valid = false;
+        ctx.upload_gen++;
+        clear_slow_set();
         HIPCHECK(hipMemcpy(ctx.m, m, sizeof(double) * n, hipMemcpyHostToDevice));
         HIPCHECK(hipMemcpy(ctx.rce, rce, sizeof(double) * n, hipMemcpyHostToDevice));
         HIPCHECK(hipMemcpy(ctx.rphys, rphys, sizeof(double) * n, hipMemcpyHostToDevice));
@@ -1614,11 +1713,10 @@ int mercury_hip_step(double time, double* h, double* hdid, double tol, int64_t* 
             return 0;
         }
         if (!mvs_family()) throw std::runtime_error("algorithm not provided by the HIP path");
-        clear_fault();
         ctx.tape.valid = false;
         bool reset = !ctx.history;
-        mvs_run(time, *h, 1, reset, false, true, 0.0);
-        check_fault("MVS force failed");
+        mvs_run(time, *h, 1, reset, false, true, 0.0);   // clears and reports the fault flag itself
+        if (ctx.hstatus[0] & FAULT_FORCE) throw std::runtime_error("MVS force failed: non-finite acceleration");
         ctx.history = true;
         ctx.saved_valid = true;
         *hdid = *h;
@@ -1798,22 +1896,25 @@ int mercury_hip_advance(double time, double h, int nsteps, double rcen, int scre
         require_state();
         if (!mvs_family()) throw std::runtime_error("advance needs MVS");
         if (nsteps < 0) throw std::runtime_error("negative step count");
-        if (ctx.systems.size() == 1 && (!(h != 0) || !std::isfinite(h))) throw std::runtime_error("invalid step");
+        if (!ctx.ensemble && (!(h != 0) || !std::isfinite(h))) throw std::runtime_error("invalid step");
         ensure_stream();
-        clear_fault();
         ctx.tape.valid = false;
-        HIPCHECK(hipMemset(ctx.devcount, 0, sizeof(int)));
         ctx.adv_events.clear();
         bool reset = !ctx.history;
+        int cnt = 0;
         if (nsteps > 0) {
+            // the call's first kernel clears the fault flag and the event count;
+            // its last writes them, with the slow-set size, to ctx.hstatus
             mvs_run(time, h, nsteps, reset, screen != 0, false, rcen);
             ctx.history = true;
+            const int f = ctx.hstatus[0];
+            if (f & FAULT_FORCE) throw std::runtime_error("MVS force failed: non-finite acceleration");
+            if (f & FAULT_REQ_OVERFLOW)
+                throw std::runtime_error("screening request buffer overflow: use fewer steps per launch");
+            if (f & FAULT_EVENT_OVERFLOW) throw std::runtime_error("event buffer overflow: advance fewer steps per call");
+            cnt = ctx.hstatus[1];
+            ctx.parked = ctx.hstatus[2];
         }
-        check_fault("MVS force failed");
-        int cnt = 0, parked = 0;
-        HIPCHECK(hipMemcpy(&cnt, ctx.devcount, sizeof(int), hipMemcpyDeviceToHost));
-        HIPCHECK(hipMemcpy(&parked, ctx.dcnt, sizeof(int), hipMemcpyDeviceToHost));
-        ctx.parked = parked;
         fetch_events(std::min(cnt, ctx.ev_cap));
         for (auto& e : ctx.hev) {
             MercuryHipEvent r;
@@ -1857,7 +1958,10 @@ int mercury_hip_ensemble_upload(int nsys, const int* n, const int* nbig, const d
         ctx.n = (int)total; ctx.nbig = 0; ctx.pn = 0; ctx.ngflag = 0;
         for (int k = 0; k < 3; k++) ctx.jcen[k] = 0.0;
         ctx.systems = sys;
+        ctx.ensemble = true;
         ctx.sys_valid = false;
+        ctx.upload_gen++;
+        clear_slow_set();
         HIPCHECK(hipMemcpy(ctx.m, m, sizeof(double) * total, hipMemcpyHostToDevice));
         HIPCHECK(hipMemcpy(ctx.rce, rce, sizeof(double) * total, hipMemcpyHostToDevice));
         HIPCHECK(hipMemcpy(ctx.rphys, rphys, sizeof(double) * total, hipMemcpyHostToDevice));
@@ -1906,6 +2010,7 @@ int mercury_hip_timing(int enable) {
     try {
         ctx.timing.clear();
         ctx.timing.on = enable != 0;
+        ctx.work_total = 0;
         if (ctx.timing.on) {
             if (!ctx.work) dalloc(ctx.work, 2);
             HIPCHECK(hipMemset(ctx.work, 0, 2 * sizeof(unsigned long long)));
@@ -1917,7 +2022,10 @@ int mercury_hip_timing(int enable) {
 int mercury_hip_timing_read(double* particle_ms, int* particle_launches, double* planet_ms, int* planet_launches,
                             double* general_ms, int* general_launches) {
     try {
-        if (ctx.stream) HIPCHECK(hipStreamSynchronize(ctx.stream));
+        if (ctx.stream) {
+            HIPCHECK(hipStreamSynchronize(ctx.stream));
+            HIPCHECK(hipStreamSynchronize(ctx.sm));
+        }
         double ms[3] = {0, 0, 0};
         int cnt[3] = {0, 0, 0};
         for (auto& mk : ctx.timing.marks) {
@@ -1981,7 +2089,7 @@ int mercury_hip_autotune(double h, int probe_steps, int* spl_out, int* block_out
         require_state();
         if (!mvs_family()) throw std::runtime_error("autotuning needs the MVS algorithm");
         if (probe_steps < 1) throw std::runtime_error("probe needs at least one step");
-        const bool single = c.systems.size() == 1;
+        const bool single = !c.ensemble;
         if (single && (!(h != 0) || !std::isfinite(h))) throw std::runtime_error("invalid step");
         ensure_stream();
         const int nsys = (int)c.systems.size();
@@ -2001,6 +2109,7 @@ int mercury_hip_autotune(double h, int probe_steps, int* spl_out, int* block_out
                                    hipMemcpyDeviceToDevice));
             HIPCHECK(hipMemcpy(c.tcur, tsave, sizeof(double) * nsys, hipMemcpyDeviceToDevice));
             c.history = hist;
+            clear_slow_set();   // the probe's slow set belongs to the probe's end state
         };
         const int spls[4] = {8, 16, 32, 64};
         const int blocks[3] = {64, 128, 256};
@@ -2069,7 +2178,7 @@ int mercury_hip_work_read(int64_t* fast_steps, int64_t* general_steps) {
     try {
         unsigned long long w[2] = {0, 0};
         if (ctx.work) HIPCHECK(hipMemcpy(w, ctx.work, sizeof(w), hipMemcpyDeviceToHost));
-        *fast_steps = (int64_t)w[0];
+        *fast_steps = (int64_t)ctx.work_total - (int64_t)w[1];
         *general_steps = (int64_t)w[1];
         return 0;
     } catch (const std::exception& e) { return fail(e); }

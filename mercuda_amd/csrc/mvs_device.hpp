@@ -88,22 +88,37 @@ struct EventSink {
     int* flags;
 };
 
-// A particle that left the fast kernel: it joins the slow set for the rest of
-// the call and is advanced by the general kernel from `step` of launch `chunk`
-// (from step 0 of every later launch).
+// A member of the slow set: a particle the general kernel advances with the
+// whole drift chain, from `step` of launch `chunk` of the current call and from
+// step 0 of every later launch.  Members kept from the previous call carry
+// chunk -1 (every step of the call).
 struct Park {
     int j, step, sys, chunk;
 };
 
+// The slow set persists across calls, so that a particle parked by the fast
+// kernel in one call starts the next one in the general kernel instead of
+// parking again after the fast kernel's first launch.  Two lists of `cap`
+// entries alternate: the current one (list[*par]) holds the members kept from
+// the previous call followed by this call's parks; the last launch of a call
+// appends the members that stay to the other list, and the next call's
+// call_begin makes that list current.  A kept member leaves when none of its
+// drifts in a call left the kepmd branch; its flag is cleared at the next
+// call's start, after every fast launch of this call has read it.
 struct ParkList {
-    Park* buf;
-    int* count;              // entries appended so far in this call
-    const int* snap;         // entry count when this launch's fast kernel finished
-    unsigned char* slow;     // per-body slow-set flag
+    Park* buf;               // list q at buf + q * cap
+    int* count2;             // entries of list 0 and list 1
+    const int* par;          // index of the current list
+    const int* pers;         // kept members at the start of the call (the current list's first entries)
+    const int* snap;         // current-list entries when this launch's fast kernel finished
+    int* leave;              // bodies leaving the slow set after this call
+    int* nleave;
+    unsigned char* slow;     // per body: bit 0 member, bit 1 a kept member left the kepmd branch this call
     int cap;
     int chunk;               // index of this launch within the call
     int per_wave;            // general kernel: entries per 64-lane wave (spreads the latency-bound
                              // slow set over more SIMDs)
+    int last;                // the call's last launch: members stay or leave
 };
 
 // Screening work the fast particle kernel defers: a box overlap with planet p
@@ -254,7 +269,7 @@ struct Ends {   // start (after the first kick) and end (after the drift) of one
 
 // The pair part of mce_stat (mercury6_2.for:1820-1925) for Big body i (0-based
 // local index ib) against body j (local index jb), for x-y boxes that overlap.
-__device__ inline void screen_pair(const double* __restrict__ X1, const double* __restrict__ Sc, int ib,
+__device__ __noinline__ void screen_pair(const double* __restrict__ X1, const double* __restrict__ Sc, int ib,
                                          int jb, bool j_big, double mi, double mj, Ends e, double rce_j,
                                          double rphys_j, double h, double time_end, int step, int sys,
                                          EventSink es) {
@@ -332,7 +347,7 @@ __device__ inline void screen_body(const double* __restrict__ T, int np, int jb,
 
 // The part of mce_cent (mercury6_2.for:1100-1180) past its gate: two-body
 // pericentre and, for an impact, the entry-time estimate.
-__device__ inline void central_event(double mcen, double mj, double rcen, double h, double time_end,
+__device__ __noinline__ void central_event(double mcen, double mj, double rcen, double h, double time_end,
                                            double xs, double ys, double zs, double us, double vs, double ws,
                                            double rr0, double rr1, double rv0, int jb, int step, int sys,
                                            EventSink es) {
@@ -417,13 +432,18 @@ struct PlanetArgs {
     int step0;        // steps of the call done before this launch (event step numbers)
 };
 
-__device__ inline void h2j_serial(int np, const double* m, const double* jdx, const double* X, const double* V,
+// NPL: the planet count as a compile-time constant (loops unroll, so the
+// independent pair terms and LDS reads overlap) or -1 (run-time count np).
+template <int NPL = -1>
+__device__ inline void h2j_serial(int np_rt, const double* m, const double* jdx, const double* X, const double* V,
                                   double* XJ, double* VJ) {
     // mco_h2j (mercury6_2.for:2613-2687); jdx[p] = 1/(mtot+m0) before body p is added
+    const int np = NPL >= 0 ? NPL : np_rt;
     if (np < 1) return;
     for (int k = 0; k < 3; k++) { XJ[k] = X[k]; VJ[k] = V[k]; }
     double mx = m[1] * X[0], my = m[1] * X[1], mz = m[1] * X[2];
     double mu = m[1] * V[0], mv = m[1] * V[1], mw = m[1] * V[2];
+#pragma unroll
     for (int p = 1; p < np; p++) {
         double temp = jdx[p];
         XJ[3 * p + 0] = X[3 * p + 0] - temp * mx;
@@ -444,14 +464,17 @@ __device__ inline void h2j_serial(int np, const double* m, const double* jdx, co
     }
 }
 
-__device__ inline void j2h_serial(int np, const double* jhx, const double* XJ, const double* VJ, double* X,
+template <int NPL = -1>
+__device__ inline void j2h_serial(int np_rt, const double* jhx, const double* XJ, const double* VJ, double* X,
                                   double* V) {
     // mco_j2h (mercury6_2.for:2700-2770); jhx[p] = m_b/(mtot+m0) after body p is added
+    const int np = NPL >= 0 ? NPL : np_rt;
     if (np < 1) return;
     for (int k = 0; k < 3; k++) { X[k] = XJ[k]; V[k] = VJ[k]; }
     double temp = jhx[0];
     double mx = temp * XJ[0], my = temp * XJ[1], mz = temp * XJ[2];
     double mu = temp * VJ[0], mv = temp * VJ[1], mw = temp * VJ[2];
+#pragma unroll
     for (int p = 1; p < np; p++) {
         X[3 * p + 0] = XJ[3 * p + 0] + mx;
         X[3 * p + 1] = XJ[3 * p + 1] + my;
@@ -474,9 +497,11 @@ __device__ inline void j2h_serial(int np, const double* jhx, const double* XJ, c
 // Big-body part of mfo_mvs (mercury6_2.for:4945-5025, 5062-5070) for the
 // planets in LDS.  Lane p computes body p+1; serial recurrences run in lane 0.
 // Writes A (Big accelerations) and the table header terms a0tp and acen.
-__device__ inline void big_forces(const Sys& d, int np, const double* m, const double* minside,
+template <int NPL = -1>
+__device__ inline void big_forces(const Sys& d, int np_rt, const double* m, const double* minside,
                                   const double* X, const double* XJ, double* A, double* t0, double* t2,
                                   double* a1, double* aob, double* hdr) {
+    const int np = NPL >= 0 ? NPL : np_rt;
     int lane = threadIdx.x;
     int p = lane;
     if (p < np) {
@@ -506,24 +531,45 @@ __device__ inline void big_forces(const Sys& d, int np, const double* m, const d
         } else {
             for (int c = 0; c < 3; c++) { t0[c] = 0.0; a1[c] = 0.0; t2[c] = 0.0; }
         }
-        // a3 over Big pairs, in the Fortran's pair order for this body
+        // a3 over Big pairs, in the Fortran's pair order for this body: the
+        // partners before it (subtracted), then those after it (added)
         double a3[3] = {0.0, 0.0, 0.0};
         const double* xb = X + 3 * p;
-        for (int i = 0; i < p; i++) {
-            double dx = xb[0] - X[3 * i], dy = xb[1] - X[3 * i + 1], dz = xb[2] - X[3 * i + 2];
-            double s2 = dx * dx + dy * dy + dz * dz;
-            double s_1 = 1.0 / sqrt(s2);
-            double s_3 = s_1 * s_1 * s_1;
-            double faci = m[i + 1] * s_3;
-            a3[0] = a3[0] - faci * dx; a3[1] = a3[1] - faci * dy; a3[2] = a3[2] - faci * dz;
-        }
-        for (int j = p + 1; j < np; j++) {
-            double dx = X[3 * j] - xb[0], dy = X[3 * j + 1] - xb[1], dz = X[3 * j + 2] - xb[2];
-            double s2 = dx * dx + dy * dy + dz * dz;
-            double s_1 = 1.0 / sqrt(s2);
-            double s_3 = s_1 * s_1 * s_1;
-            double facj = m[j + 1] * s_3;
-            a3[0] = a3[0] + facj * dx; a3[1] = a3[1] + facj * dy; a3[2] = a3[2] + facj * dz;
+        if (NPL >= 0) {
+            // every partner's term is independent of the others until the sum;
+            // the body's own slot is computed and discarded
+#pragma unroll
+            for (int q = 0; q < NPL; q++) {
+                const bool before = q < p;
+                double dx = before ? xb[0] - X[3 * q] : X[3 * q] - xb[0];
+                double dy = before ? xb[1] - X[3 * q + 1] : X[3 * q + 1] - xb[1];
+                double dz = before ? xb[2] - X[3 * q + 2] : X[3 * q + 2] - xb[2];
+                double s2 = dx * dx + dy * dy + dz * dz;
+                double s_1 = 1.0 / sqrt(s2);
+                double s_3 = s_1 * s_1 * s_1;
+                double fac = m[q + 1] * s_3;
+                if (q != p) {
+                    if (before) { a3[0] = a3[0] - fac * dx; a3[1] = a3[1] - fac * dy; a3[2] = a3[2] - fac * dz; }
+                    else { a3[0] = a3[0] + fac * dx; a3[1] = a3[1] + fac * dy; a3[2] = a3[2] + fac * dz; }
+                }
+            }
+        } else {
+            for (int i = 0; i < p; i++) {
+                double dx = xb[0] - X[3 * i], dy = xb[1] - X[3 * i + 1], dz = xb[2] - X[3 * i + 2];
+                double s2 = dx * dx + dy * dy + dz * dz;
+                double s_1 = 1.0 / sqrt(s2);
+                double s_3 = s_1 * s_1 * s_1;
+                double faci = m[i + 1] * s_3;
+                a3[0] = a3[0] - faci * dx; a3[1] = a3[1] - faci * dy; a3[2] = a3[2] - faci * dz;
+            }
+            for (int j = p + 1; j < np; j++) {
+                double dx = X[3 * j] - xb[0], dy = X[3 * j + 1] - xb[1], dz = X[3 * j + 2] - xb[2];
+                double s2 = dx * dx + dy * dy + dz * dz;
+                double s_1 = 1.0 / sqrt(s2);
+                double s_3 = s_1 * s_1 * s_1;
+                double facj = m[j + 1] * s_3;
+                a3[0] = a3[0] + facj * dx; a3[1] = a3[1] + facj * dy; a3[2] = a3[2] + facj * dz;
+            }
         }
         for (int c = 0; c < 3; c++) A[3 * p + c] = a3[c];
         if (d.obl) obl_body(d.mu, d.j2, d.j4, d.j6, xb[0], xb[1], xb[2], aob[3 * p], aob[3 * p + 1],
@@ -533,6 +579,7 @@ __device__ inline void big_forces(const Sys& d, int np, const double* m, const d
     if (lane == 0) {
         double a0[3] = {0.0, 0.0, 0.0};
         double a2[3] = {0.0, 0.0, 0.0};
+#pragma unroll
         for (int q = 1; q < np; q++)
             for (int c = 0; c < 3; c++) a0[c] = a0[c] - t0[3 * q + c];
         double a0tp[3] = {0.0, 0.0, 0.0};
@@ -550,6 +597,7 @@ __device__ inline void big_forces(const Sys& d, int np, const double* m, const d
                 for (int c = 0; c < 3; c++) acen[c] = acen[c] - tmp4 * aob[3 * q + c];
             }
         }
+#pragma unroll
         for (int q = 0; q < np; q++) {
             for (int c = 0; c < 3; c++) {
                 if (q >= 1) a2[c] = a2[c] + t2[3 * q + c];
@@ -563,18 +611,53 @@ __device__ inline void big_forces(const Sys& d, int np, const double* m, const d
     __syncthreads();
 }
 
+// Encounter screen of Big body jb against the Big bodies before it (the
+// planet kernel's part of mce_stat): screen_body with the planet loop
+// unrolled when the planet count is a compile-time constant.
+template <int NPL>
+__device__ inline void screen_planet(const double* __restrict__ T, int np_rt, int jb, double mj, const Ends& e,
+                                     double rce_j, double rphys_j, double h, double time_end, int step, int sys,
+                                     const EventSink& es) {
+    if (NPL < 0) {
+        screen_body(T, np_rt, jb, np_rt + 1, mj, e, rce_j, rphys_j, h, time_end, step, sys, es);
+        return;
+    }
+    double xmin, xmax, ymin, ymax;
+    box_axis(h, e.x0, e.u0, e.x1, e.u1, xmin, xmax);
+    box_axis(h, e.y0, e.v0, e.y1, e.v1, ymin, ymax);
+    double temp = rce_j * 1.2;
+    xmin = xmin - temp; xmax = xmax + temp;
+    ymin = ymin - temp; ymax = ymax + temp;
+    const double* F = T + kHdr;
+    const double* Sc = F + kF * NPL;
+#pragma unroll
+    for (int p = 0; p < NPL; p++) {
+        const double* B = Sc + kScr * p;
+        if (p < jb - 1 && B[SX1] >= xmin && xmax >= B[SX0] && B[SY1] >= ymin && ymax >= B[SY0])
+            screen_pair(F + kF * p, B, p + 1, jb, true, F[kF * p + 3], mj, e, rce_j, rphys_j, h, time_end, step,
+                        sys, es);
+    }
+}
+
 // One wave per system, latency-bound; raised issue priority keeps it ahead of
 // the particle waves it shares a SIMD with.
-__global__ void __launch_bounds__(64) mvs_planets(PlanetArgs args) {
-    __builtin_amdgcn_s_setprio(3);
+struct PlanetLds {
+    double m[kMaxBig], minside[kMaxBig], jdx[kMaxBig], jhx[kMaxBig], gm[kMaxBig];
+    double X[3 * kMaxBig], V[3 * kMaxBig], A[3 * kMaxBig], XJ[3 * kMaxBig], VJ[3 * kMaxBig];
+    double X0[3 * kMaxBig], V0[3 * kMaxBig], t0[3 * kMaxBig], t2[3 * kMaxBig];
+    double a1[3 * kMaxBig], aob[3 * kMaxBig], hdr[8];
+};
+
+template <int NPL>
+__device__ __forceinline__ void planets_run(const PlanetArgs& args, PlanetLds& L) {
     const int sid = blockIdx.x;
     const Sys d = args.sys[sid];
-    const int np = d.nbig - 1;
+    const int np = NPL >= 0 ? NPL : d.nbig - 1;
     const int lane = threadIdx.x;
-    __shared__ double m[kMaxBig], minside[kMaxBig], jdx[kMaxBig], jhx[kMaxBig], gm[kMaxBig];
-    __shared__ double X[3 * kMaxBig], V[3 * kMaxBig], A[3 * kMaxBig], XJ[3 * kMaxBig], VJ[3 * kMaxBig];
-    __shared__ double X0[3 * kMaxBig], V0[3 * kMaxBig], t0[3 * kMaxBig], t2[3 * kMaxBig];
-    __shared__ double a1[3 * kMaxBig], aob[3 * kMaxBig], hdr[8];
+    double* m = L.m; double* minside = L.minside; double* jdx = L.jdx; double* jhx = L.jhx; double* gm = L.gm;
+    double* X = L.X; double* V = L.V; double* A = L.A; double* XJ = L.XJ; double* VJ = L.VJ;
+    double* X0 = L.X0; double* V0 = L.V0; double* t0 = L.t0; double* t2 = L.t2;
+    double* a1 = L.a1; double* aob = L.aob; double* hdr = L.hdr;
     const long long cap = args.st.cap;
     const int b0 = d.base;
     for (int q = lane; q < d.nbig; q += 64) m[q] = args.st.m[b0 + q];
@@ -584,6 +667,9 @@ __global__ void __launch_bounds__(64) mvs_planets(PlanetArgs args) {
             V[3 * p + c] = args.st.v[c * cap + b0 + 1 + p];
             A[3 * p + c] = args.st.a[c * cap + b0 + 1 + p];
         }
+    // this lane's planet: radii read once for the launch
+    double rce_p = 0.0, rphys_p = 0.0;
+    if (lane < np) { rce_p = args.st.rce[b0 + 1 + lane]; rphys_p = args.st.rphys[b0 + 1 + lane]; }
     __syncthreads();
     if (lane == 0) {
         // constants of the Jacobi transforms and drifts (mdt_mvs mercury6_2.for:4041-4046)
@@ -604,9 +690,9 @@ __global__ void __launch_bounds__(64) mvs_planets(PlanetArgs args) {
     __syncthreads();
     const int ts = d.ts;
     if (args.reset) {
-        if (lane == 0) h2j_serial(np, m, jdx, X, V, XJ, VJ);
+        if (lane == 0) h2j_serial<NPL>(np, m, jdx, X, V, XJ, VJ);
         __syncthreads();
-        big_forces(d, np, m, minside, X, XJ, A, t0, t2, a1, aob, hdr);
+        big_forces<NPL>(d, np, m, minside, X, XJ, A, t0, t2, a1, aob, hdr);
         double* St = args.tab + d.start;
         if (lane < 8) St[lane] = (lane < 6) ? hdr[lane] : 0.0;
         for (int p = lane; p < np; p += 64) {
@@ -636,7 +722,7 @@ __global__ void __launch_bounds__(64) mvs_planets(PlanetArgs args) {
             }
         }
         __syncthreads();
-        if (lane == 0) h2j_serial(np, m, jdx, X, V, XJ, VJ);
+        if (lane == 0) h2j_serial<NPL>(np, m, jdx, X, V, XJ, VJ);
         __syncthreads();
         if (p < np) {
             Orbit o{XJ[3 * p], XJ[3 * p + 1], XJ[3 * p + 2], VJ[3 * p], VJ[3 * p + 1], VJ[3 * p + 2]};
@@ -645,15 +731,14 @@ __global__ void __launch_bounds__(64) mvs_planets(PlanetArgs args) {
             VJ[3 * p] = o.vx; VJ[3 * p + 1] = o.vy; VJ[3 * p + 2] = o.vz;
         }
         __syncthreads();
-        if (lane == 0) j2h_serial(np, jhx, XJ, VJ, X, V);
+        if (lane == 0) j2h_serial<NPL>(np, jhx, XJ, VJ, X, V);
         __syncthreads();
         // screening data of this step (velocities after the drift, before the kick)
         if (p < np) {
             double* B = T + kHdr + kF * np + kScr * p;
             double lo, hi;
             box_axis(h, X0[3 * p], V0[3 * p], X[3 * p], V[3 * p], lo, hi);
-            double rce = args.st.rce[b0 + 1 + p];
-            double pad = rce * 1.2;
+            double pad = rce_p * 1.2;
             B[SX0] = lo - pad; B[SX1] = hi + pad;
             box_axis(h, X0[3 * p + 1], V0[3 * p + 1], X[3 * p + 1], V[3 * p + 1], lo, hi);
             B[SY0] = lo - pad; B[SY1] = hi + pad;
@@ -663,8 +748,8 @@ __global__ void __launch_bounds__(64) mvs_planets(PlanetArgs args) {
                 B[SV1 + c] = V[3 * p + c];
                 T[kHdr + kF * p + c] = X[3 * p + c];
             }
-            B[SRCE] = rce;
-            B[SRPH] = args.st.rphys[b0 + 1 + p];
+            B[SRCE] = rce_p;
+            B[SRPH] = rphys_p;
             T[kHdr + kF * p + 3] = m[p + 1];
             if (args.save)
                 for (int c = 0; c < 3; c++) {
@@ -672,7 +757,7 @@ __global__ void __launch_bounds__(64) mvs_planets(PlanetArgs args) {
                     args.st.v1[c * cap + b0 + 1 + p] = V[3 * p + c];
                 }
         }
-        big_forces(d, np, m, minside, X, XJ, A, t0, t2, a1, aob, hdr);
+        big_forces<NPL>(d, np, m, minside, X, XJ, A, t0, t2, a1, aob, hdr);
         if (lane < 8) T[lane] = (lane < 6) ? hdr[lane] : (lane == 6 ? time_end : 0.0);
         if (p < np) {
             for (int c = 0; c < 3; c++) V[3 * p + c] = V[3 * p + c] + hby2 * A[3 * p + c];
@@ -685,8 +770,7 @@ __global__ void __launch_bounds__(64) mvs_planets(PlanetArgs args) {
             Ends e{X0[3 * p], X0[3 * p + 1], X0[3 * p + 2], V0[3 * p], V0[3 * p + 1], V0[3 * p + 2],
                    X[3 * p], X[3 * p + 1], X[3 * p + 2], T[kHdr + kF * np + kScr * p + SV1],
                    T[kHdr + kF * np + kScr * p + SV1 + 1], T[kHdr + kF * np + kScr * p + SV1 + 2]};
-            screen_body(T, np, jb, d.nbig, m[jb], e, args.st.rce[b0 + jb], args.st.rphys[b0 + jb], h,
-                        time_end, s + args.step0, sid, args.es);
+            screen_planet<NPL>(T, np, jb, m[jb], e, rce_p, rphys_p, h, time_end, s + args.step0, sid, args.es);
             central_check(m[0], m[jb], d.rcen, h, time_end, xs[0], xs[1], xs[2], vs[0], vs[1], vs[2],
                           X[3 * p], X[3 * p + 1], X[3 * p + 2], V[3 * p], V[3 * p + 1], V[3 * p + 2], jb,
                           s + args.step0, sid, args.es);
@@ -700,6 +784,30 @@ __global__ void __launch_bounds__(64) mvs_planets(PlanetArgs args) {
             args.st.a[c * cap + b0 + 1 + p] = A[3 * p + c];
         }
     if (lane == 0) args.tcur[sid] = time_end;
+}
+
+// The planet kernel: one 64-lane workgroup per system, lane p owns Big body
+// p+1.  Each workgroup branches once on its system's planet count into the
+// body built for it (0..12); more planets take the run-time-count body.
+__global__ void __launch_bounds__(64) mvs_planets(PlanetArgs args) {
+    __builtin_amdgcn_s_setprio(3);
+    __shared__ PlanetLds L;
+    switch (args.sys[blockIdx.x].nbig - 1) {
+        case 0: planets_run<0>(args, L); break;
+        case 1: planets_run<1>(args, L); break;
+        case 2: planets_run<2>(args, L); break;
+        case 3: planets_run<3>(args, L); break;
+        case 4: planets_run<4>(args, L); break;
+        case 5: planets_run<5>(args, L); break;
+        case 6: planets_run<6>(args, L); break;
+        case 7: planets_run<7>(args, L); break;
+        case 8: planets_run<8>(args, L); break;
+        case 9: planets_run<9>(args, L); break;
+        case 10: planets_run<10>(args, L); break;
+        case 11: planets_run<11>(args, L); break;
+        case 12: planets_run<12>(args, L); break;
+        default: planets_run<-1>(args, L); break;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -826,12 +934,19 @@ struct ParticleArgs {
     EventSink es;
     ParkList park;
     ReqSink req;
-    unsigned long long* work;   // [0] particle-steps done by the fast kernel, [1] by the general kernel
+    unsigned long long* work;   // [1] particle-steps done by the general kernel (timing mode; the
+                                // fast kernel's are the call's total less these)
     int steps;
     int reset;
     int screen;
     int save;
     int step0;
+    // general kernel: the launch runs the steps of its block for the slow set,
+    // after the rest of the previous launch's block for the members that
+    // launch's fast kernel parked (ptab: that block, psteps: its step count;
+    // 0 for the call's first launch)
+    const double* ptab = nullptr;
+    int psteps = 0;
 };
 
 // Adds each lane's completed particle-steps into *counter, one atomic per wave.
@@ -978,11 +1093,12 @@ __device__ __forceinline__ void fast_tile(const ParticleArgs& args, double* lds_
         if (!ok) {
             int leader = __ffsll((long long)mask) - 1;
             int lane = __lane_id();
+            const int par = *args.park.par;
             int base = 0;
-            if (lane == leader) base = atomicAdd(args.park.count, __popcll(mask));
+            if (lane == leader) base = atomicAdd(args.park.count2 + par, __popcll(mask));
             base = __shfl(base, leader);
             int slot = base + __popcll(mask & ((1ull << lane) - 1ull));
-            if (slot < args.park.cap) args.park.buf[slot] = Park{g, s, sid, args.park.chunk};
+            if (slot < args.park.cap) args.park.buf[(long long)par * args.park.cap + slot] = Park{g, s, sid, args.park.chunk};
             args.park.slow[g] = 1;
             break;
         }
@@ -991,7 +1107,6 @@ __device__ __forceinline__ void fast_tile(const ParticleArgs& args, double* lds_
         finish_step_deferred<SCREEN, NP>(d, T, np, g, jb, pad, s, sid, args.req, args.st.fault, rr0, rv0, E, ls,
                                          o, ax, ay, az);
     }
-    if (args.work) count_work(args.work, s);
     if (!live) return;
     // parked lanes store their pre-step state and step-start acceleration
     args.st.x[g] = o.x; args.st.x[cap + g] = o.y; args.st.x[2 * cap + g] = o.z;
@@ -1030,9 +1145,15 @@ __global__ void __launch_bounds__(256, MINW) mvs_particles_fast(ParticleArgs arg
     }
 }
 
-// Parked lanes: resume each from its recorded step with the whole drift chain.
-// Lanes of one wave may start at different steps and belong to different
-// systems, so system and table reads here are per lane.
+// Slow-set members: each runs with the whole drift chain.  The launch for
+// block m (park.chunk) waits for the planet tables of its steps and for block
+// m-1's fast kernel, not for block m's: it runs beside that fast kernel.  Its
+// members are the kept ones and those parked by fast kernels up to block m-1
+// (park.snap, or the kept count for m = 0); a member parked in block m-1 at
+// step s first finishes that block (steps s.. of ptab).  The call ends with a
+// launch of no steps of its own that finishes the last block's parks.
+// Lanes of one wave may belong to different systems, so system and table reads
+// here are per lane.
 // UNIFORM: every entry belongs to system 0, so the wave walks the steps in
 // lockstep from the earliest start among its lanes and reads each step's table
 // with wave-uniform (scalar) loads; lanes join at their own start step.
@@ -1042,21 +1163,31 @@ __global__ void __launch_bounds__(256, MINW) mvs_particles_fast(ParticleArgs arg
 template <bool SCREEN, bool SAVE, bool UNIFORM>
 __global__ void __launch_bounds__(64) mvs_particles_general(ParticleArgs args) {
     __builtin_amdgcn_s_setprio(3);
-    const int count = min(*args.park.snap, args.park.cap);
+    const ParkList& P = args.park;
+    const int par = *P.par;
+    const Park* list = P.buf + (long long)par * P.cap;
+    Park* next = P.buf + (long long)(1 - par) * P.cap;
+    const int count = min(P.chunk == 0 ? *P.pers : *P.snap, P.cap);
     const long long cap = args.st.cap;
-    const int per_wave = args.park.per_wave;
+    const int per_wave = P.per_wave;
     const int lane = threadIdx.x;          // one 64-lane wave per workgroup
+    const int prev = P.chunk - 1;          // block whose parks this launch first catches up
     for (int base = blockIdx.x * per_wave; base < count; base += gridDim.x * per_wave) {
         const int idx = base + lane;
         bool act = lane < per_wave && idx < count;
         Park pk{0, 0, 0, 0};
-        int start = args.steps;
+        int a_from = args.psteps, b_from = args.steps;   // catch-up and own-block first steps (none)
         if (act) {
-            pk = args.park.buf[idx];
-            if (pk.chunk > args.park.chunk) act = false;
-            else start = (pk.chunk < args.park.chunk) ? 0 : pk.step;
+            pk = list[idx];
+            if (pk.chunk < 0 || pk.chunk < prev) {        // kept, or parked before block m-1
+                b_from = 0;
+            } else if (pk.chunk == prev) {                // parked by block m-1's fast kernel
+                a_from = pk.step;
+                b_from = 0;
+            }
+            act = a_from < args.psteps || b_from < args.steps;
         }
-        if (!act) start = args.steps;
+        if (!act) { a_from = args.psteps; b_from = args.steps; }
         const Sys d = args.sys[UNIFORM ? 0 : pk.sys];
         const int g = pk.j;
         const int jb = g - d.base;
@@ -1067,33 +1198,63 @@ __global__ void __launch_bounds__(64) mvs_particles_general(ParticleArgs args) {
         if (act) {
             o = Orbit{args.st.x[g], args.st.x[cap + g], args.st.x[2 * cap + g],
                       args.st.v[g], args.st.v[cap + g], args.st.v[2 * cap + g]};
-            ax = args.st.a[g]; ay = args.st.a[cap + g]; az = args.st.a[2 * cap + g];
+            if (args.reset && pk.chunk < 0) {
+                // a kept member starts a call that recomputes accelerations
+                particle_accel(d, args.tab + d.start, np, o.x, o.y, o.z, ax, ay, az);
+            } else {
+                ax = args.st.a[g]; ay = args.st.a[cap + g]; az = args.st.a[2 * cap + g];
+            }
             if (SCREEN || SAVE) { mj = args.st.m[g]; rce_j = args.st.rce[g]; rphys_j = args.st.rphys[g]; }
         }
-        const double* tab = args.tab + d.tab;
-        int s0 = start;
-        if (UNIFORM) {
-            for (int off = 32; off > 0; off >>= 1) s0 = min(s0, __shfl_xor(s0, off));
-            s0 = __builtin_amdgcn_readfirstlane(s0);
-        }
-        for (int s = s0; s < args.steps; s++) {
-            if (s < start) continue;
-            const double* T = tab + (long long)s * d.ts;
-            double xs[6] = {o.x, o.y, o.z, o.vx, o.vy, o.vz};
-            o.vx = o.vx + d.hby2 * ax;
-            o.vy = o.vy + d.hby2 * ay;
-            o.vz = o.vz + d.hby2 * az;
-            Orbit k0 = o;
-            drift_one(d.mu, o, d.h);
-            if (SAVE) save_step(args.st, g, xs, k0, o);
-            finish_step<SCREEN>(d, T, np, jb, mj, rce_j, rphys_j, s + args.step0, UNIFORM ? 0 : pk.sys,
-                                args.es, args.st.fault, xs, k0, o, ax, ay, az);
-        }
+        // the fast kernel's gate (drift_fast): the drift it would have made
+        const bool in_range = d.mu >= 0x1p-100 && d.mu <= 0x1p100 && fabs(d.h) >= 0x1p-100 && fabs(d.h) <= 0x1p100;
+        bool left_kepmd = false;
+        // steps [from, to) of one table block; step numbers from `first`
+        auto run = [&](const double* blk, int from, int to, int first) {
+            const double* tab = blk + d.tab;
+            int s0 = from;
+            if (UNIFORM) {
+                for (int off = 32; off > 0; off >>= 1) s0 = min(s0, __shfl_xor(s0, off));
+                s0 = __builtin_amdgcn_readfirstlane(s0);
+            }
+            for (int s = s0; s < to; s++) {
+                if (s < from) continue;
+                const double* T = tab + (long long)s * d.ts;
+                double xs[6] = {o.x, o.y, o.z, o.vx, o.vy, o.vz};
+                o.vx = o.vx + d.hby2 * ax;
+                o.vy = o.vy + d.hby2 * ay;
+                o.vz = o.vz + d.hby2 * az;
+                Orbit k0 = o;
+                const double rr = o.x * o.x + o.y * o.y + o.z * o.z;
+                bool kep = false;
+                drift_one(d.mu, o, d.h, &kep);
+                if (!(kep && in_range && rr >= 0x1p-200 && rr <= 0x1p200)) left_kepmd = true;
+                if (SAVE) save_step(args.st, g, xs, k0, o);
+                finish_step<SCREEN>(d, T, np, jb, mj, rce_j, rphys_j, s + first, UNIFORM ? 0 : pk.sys,
+                                    args.es, args.st.fault, xs, k0, o, ax, ay, az);
+            }
+        };
+        if (args.psteps > 0) run(args.ptab, a_from, args.psteps, args.step0 - args.psteps);
+        run(args.tab, b_from, args.steps, args.step0);
         if (act) {
             args.st.x[g] = o.x; args.st.x[cap + g] = o.y; args.st.x[2 * cap + g] = o.z;
             args.st.v[g] = o.vx; args.st.v[cap + g] = o.vy; args.st.v[2 * cap + g] = o.vz;
             args.st.a[g] = ax; args.st.a[cap + g] = ay; args.st.a[2 * cap + g] = az;
-            if (args.work) atomicAdd(args.work + 1, (unsigned long long)(args.steps - start));
+            if (args.work)
+                atomicAdd(args.work + 1, (unsigned long long)(args.psteps - a_from + args.steps - b_from));
+            // members parked in this call stay; kept members stay while they leave the kepmd branch
+            const bool kept = pk.chunk < 0;
+            if (P.last) {
+                if (!kept || left_kepmd || (P.slow[g] & 2)) {
+                    int slot = atomicAdd(P.count2 + (1 - par), 1);
+                    if (slot < P.cap) next[slot] = Park{g, 0, pk.sys, -1};
+                    P.slow[g] = 1;
+                } else {
+                    P.leave[atomicAdd(P.nleave, 1)] = g;
+                }
+            } else if (kept && left_kepmd) {
+                P.slow[g] = 3;
+            }
         }
     }
 }
@@ -1181,6 +1342,63 @@ __global__ void zero_bytes(unsigned char* p, long long n) {
 
 __global__ void copy_int(const int* src, int* dst) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *dst = *src;
+}
+
+// An empty kernel: a point in a stream's order that an event can mark.
+__global__ void marker() {}
+
+// Current slow-set size after a fast launch (the general launch of the same
+// chunk takes entries up to it; later fast launches keep appending).
+__global__ void snap_count(const int* count2, const int* par, int* dst) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *dst = count2[*par];
+}
+
+// Start of a call, before its first planet launch: bodies that left the slow
+// set in the previous call rejoin the fast kernel, the list of kept members
+// becomes current, and the call's counters, event count and fault flag clear.
+struct CallBeginArgs {
+    unsigned char* slow;
+    const int* leave;
+    int* count2;       // entries of list 0 and list 1
+    int* par;          // current list
+    int* pers;         // kept members that start the call
+    int* nleave;
+    int* zero;         // per-launch snapshots and request counts
+    int nzero;
+    int* devcount;
+    int* fault;
+};
+
+__global__ void __launch_bounds__(256) call_begin(CallBeginArgs a) {
+    const int nl = *a.nleave;
+    for (int i = threadIdx.x; i < nl; i += blockDim.x) a.slow[a.leave[i]] = 0;
+    for (int i = threadIdx.x; i < a.nzero; i += blockDim.x) a.zero[i] = 0;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int par = *a.par ^ 1;
+        *a.par = par;
+        *a.pers = a.count2[par];
+        a.count2[1 - par] = 0;
+        *a.nleave = 0;
+        *a.devcount = 0;
+        *a.fault = 0;
+    }
+}
+
+// End of a call, after every stream's work: the counts the host reads, written
+// into pinned host memory so that no copy follows the call.
+// status: [0] fault flags, [1] events, [2] slow-set members in the call,
+// [3] members kept for the next call.
+__global__ void call_end(const int* count2, const int* par, const int* devcount, const int* fault,
+                         int* status) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        const int q = *par;
+        status[0] = *fault;
+        status[1] = *devcount;
+        status[2] = count2[q];
+        status[3] = count2[1 - q];
+        __threadfence_system();
+    }
 }
 
 }  // namespace mhip

@@ -61,3 +61,23 @@ def test_ensemble_equals_single_runs_bitwise(hip):
         hip.advance(0.0, q["h"], nsteps, q["rcen"], screen=True)
         x, v = hip.download()
         assert np.array_equal(x, states[k][0]) and np.array_equal(v, states[k][1]), k
+
+
+def test_ensemble_of_one_system(hip, oracle):
+    """An ensemble may hold a single system: it keeps its own step and
+    central radius (the advance call passes none)."""
+    ens = systems.ensemble(seed=14, nsystems=1, nmin=500, nmax=3000)
+    q = ens[0]
+    q["rce"][1:q["nbig"]] *= 4.0
+    nsteps = 90
+    say(f"case: ensemble of one system, {len(q['m'])} bodies, {nsteps} steps")
+    hip.configure(1)
+    hip.ensemble_upload(ens)
+    hip.set_tuning(steps_per_launch=16)
+    hip.ensemble_advance(0.0, 40, screen=True)
+    hip.ensemble_advance(40 * q["h"], nsteps - 40, screen=True)
+    (x, v), = hip.ensemble_download()
+    xo, vo, evi, evd = oracle.mvs_run(q, nsteps)
+    ex, _ = rel_err(x[1:], xo[1:])
+    ev_, _ = rel_err(v[1:], vo[1:])
+    assert ex <= REL_TOL and ev_ <= REL_TOL, (ex, ev_)

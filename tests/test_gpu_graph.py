@@ -100,3 +100,33 @@ def test_autotune_leaves_state_and_results(hip):
     hip.ensemble_advance(80.0, 50, screen=True)
     plain = hip.download()
     assert np.array_equal(tuned[0], plain[0]) and np.array_equal(tuned[1], plain[1])
+
+
+def test_graph_after_reupload_with_more_particles(hip):
+    """A captured graph is not replayed for a different upload: re-uploading a
+    system with a few more particles (same capacity and tiles) integrates all
+    of them, as the eager launches do."""
+    s = systems.config_small(seed=8, nparticles=1000)
+    more = systems.config_small(seed=8, nparticles=1040)
+    h = s["h"]
+    say("case: graph capture on 1000 particles, then replay attempt after re-upload of 1040")
+
+    def run(graph):
+        hip.set_graph(False)
+        hip.configure(1)
+        hip.set_tuning(steps_per_launch=16, block_size=128)
+        hip.upload_system(s)
+        hip.set_graph(graph)
+        hip.advance(0.0, h, 32, s["rcen"], screen=True)
+        hip.advance(32 * h, h, 32, s["rcen"], screen=True)
+        hip.upload_system(more)
+        hip.advance(0.0, h, 32, more["rcen"], screen=True)
+        hip.advance(32 * h, h, 32, more["rcen"], screen=True)
+        out = hip.download()
+        hip.set_graph(False)
+        return out
+
+    x1, v1 = run(False)
+    x2, v2 = run(True)
+    assert np.array_equal(x1, x2) and np.array_equal(v1, v2)
+    assert not np.array_equal(x1[-40:], more["x"][-40:])

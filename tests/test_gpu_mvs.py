@@ -124,6 +124,56 @@ def test_kepler_regimes(hip, oracle):
     compare_rows(rows, oracle_rows(evi, evd))
 
 
+def regimes_system(seed, k=3000):
+    """Particles at 0.3-2x the local escape speed: many leave the kepmd branch
+    (slow-set members of the general kernel), some for a few steps only."""
+    rng = np.random.default_rng(seed)
+    sysd = systems.config_small(seed=2, nparticles=0)
+    mu = sysd["m"][0]
+    r = 10 ** rng.uniform(-0.5, 1.2, k)
+    vesc = np.sqrt(2 * mu / r)
+    f = rng.choice([0.3, 0.7, 0.9, 0.99, 1.0, 1.2], k)
+    dirx = rng.normal(size=(k, 3)); dirx /= np.linalg.norm(dirx, axis=1)[:, None]
+    dirv = rng.normal(size=(k, 3)); dirv /= np.linalg.norm(dirv, axis=1)[:, None]
+    sysd["m"] = np.concatenate([sysd["m"], np.zeros(k)])
+    sysd["x"] = np.concatenate([sysd["x"], dirx * r[:, None]])
+    sysd["v"] = np.concatenate([sysd["v"], dirv * (f * vesc)[:, None]])
+    systems.finish_limits(sysd)
+    return sysd
+
+
+def test_split_calls_equal_one_call_bitwise(hip):
+    """The slow set persists from call to call (members kept, members leaving
+    when their drifts return to the kepmd branch); calls of any length, and a
+    reset of the acceleration history between them, give bitwise the states
+    and events of one long call."""
+    sysd = regimes_system(17)
+    h = sysd["h"]
+    say("case: split calls 1+5+20+3+37 (reset before the 4th) vs one call of 66 steps")
+    hip.configure(1)
+    hip.upload_system(sysd)
+    hip.set_tuning(steps_per_launch=16)
+    hip.advance(0.0, h, 66, sysd["rcen"], screen=True)
+    x1, v1 = hip.download()
+    r1 = adv_event_rows(hip.advance_events())
+    hip.upload_system(sysd)
+    t, done, r2, members = 0.0, 0, [], []
+    for k, n in enumerate([1, 5, 20, 3, 37]):
+        if k == 3:
+            hip.reset(0)
+        hip.advance(t, h, n, sysd["rcen"], screen=True)
+        members.append(hip.parked())
+        r2 += [(row[0] + done,) + tuple(row[1:]) for row in adv_event_rows(hip.advance_events())]
+        for _ in range(n):
+            t = t + h
+        done += n
+    x2, v2 = hip.download()
+    say(f"slow-set members per call {members}")
+    assert min(members) > 0
+    assert np.array_equal(x1, x2) and np.array_equal(v1, v2)
+    compare_rows(sorted(r2), sorted(r1), rtol=0.0)
+
+
 def test_encounters_fire(hip, oracle):
     """Half the particles near planet orbits: encounter minima, near misses and
     central impacts are listed identically (indices, kind, step)."""
