@@ -364,6 +364,29 @@ def hybrid_rate(hip, nparticles=200_000, nsteps=3):
             "bs2_substeps": hybrid.mdt_hkce.substeps}
 
 
+def copy_probe(mib):
+    """A plain device-to-device copy of known size (three hipMemcpy calls of
+    `mib` MiB), the calibration run for rocprofv3's HBM byte counters
+    (FETCH_SIZE, WRITE_SIZE): the runtime's copy kernel reads and writes
+    exactly `mib` MiB per dispatch."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    for p in (a, b):
+        if hip.hipMalloc(C.byref(p), C.c_size_t(n)) != 0:
+            raise SystemExit("hipMalloc failed")
+    hip.hipMemset(a, 1, C.c_size_t(n))
+    hip.hipDeviceSynchronize()
+    for _ in range(3):
+        if hip.hipMemcpy(b, a, C.c_size_t(n), 3) != 0:      # hipMemcpyDeviceToDevice
+            raise SystemExit("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"copied {mib} MiB device-to-device 3 times")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -382,7 +405,12 @@ def main():
     ap.add_argument("--autotune", action="store_true",
                     help="let the autotuner pick steps per launch and workgroup size instead of --spl/--block "
                          "(the defaults are its usual choice on MI355X)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run the HBM-counter calibration copy of MIB MiB and exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the massless-particle path has no multi-GPU mode in this build "
               "(particles would shard across GPUs with a replicated planet table).",
@@ -430,11 +458,9 @@ def main():
     flop_per_launch = FLOP_PER_PSTEP * fast_steps / launches
     achieved = flop_per_launch / avg_s / 1e12
     roofline = {
-        # compute roof: the FP64 peak (78.6 TFLOP/s, the same for the vector
-        # ALUs and the FP64 matrix cores on MI355X); the kernel itself runs on
-        # the FP64 VALU -- nothing here is a dense contraction
-        "bound": "mfma",
-        "compute_unit": "fp64 valu",
+        # compute roof: the FP64 vector peak (78.6 TFLOP/s on MI355X); the
+        # kernel runs on the FP64 VALU -- nothing here is a dense contraction
+        "bound": "fp64_valu",
         "achieved": round(achieved, 3),
         "peak": PEAK_FP64_TFLOPS,
         "unit": "TFLOP/s",
