@@ -166,6 +166,20 @@ int mercury_hip_graph_stats(int *captures, int *replays);
 int mercury_hip_autotune(double h, int probe_steps, int *steps_per_launch, int *block_size,
                          double *particle_steps_per_s);
 
+/* Allocation-free force (no counterpart in the reference): for callers that
+ * cannot allocate device memory -- code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc.  The accelerations mercury_hip_force
+ * gives (mfo_all: gravity, 1PN when pn != 0, oblateness from jcen), for a
+ * system passed in DEVICE memory in this library's layout: m[n]; x, v and the
+ * output a component-major, x[c*ld + j] for c = 0..2 (v is read only when
+ * pn != 0); jcen[3] is host memory.  Body 0 is the central body (its row of a
+ * is zero).  Runs on the null stream and returns when a is written.  It
+ * allocates nothing: its scratch is a fixed 4 KB device array per device,
+ * which holds at most 64 Big bodies (more is an error), and calls that
+ * overlap on one device race on it. */
+int mercury_hip_force_noalloc(int n, int nbig, int pn, const double *m, const double *x, const double *v,
+                              const double *jcen, double *a, long long ld);
+
 /* ---- Differentiation of the MVS map (no counterpart in the reference) ----
  * For orbit fitting: gradients of a function of the final states with
  * respect to the initial states of every body and the Big-body masses.

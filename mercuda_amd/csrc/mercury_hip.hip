@@ -175,6 +175,31 @@ struct Context {
 
 Context ctx;
 
+// ---- allocation-free force (mercury_hip_force_noalloc) ---------------------
+// Its kernels' scratch lives in this file-scope array, one copy per device,
+// held until the process ends: the system descriptor at byte 0, the fault flag
+// at byte 256, the planets' force table from byte 512 (kNoAllocTable doubles:
+// the header and force block of up to kMaxBig - 1 planets).  Calls on one
+// device share it: calls that overlap on one device race.
+constexpr size_t kNoAllocBytes = 4096;
+constexpr size_t kNoAllocFault = 256, kNoAllocTab = 512;
+constexpr int kNoAllocTable = (int)((kNoAllocBytes - kNoAllocTab) / sizeof(double));
+static_assert(sizeof(Sys) <= kNoAllocFault, "descriptor slot");
+static_assert(kHdr + kF * (kMaxBig - 1) <= kNoAllocTable, "table slot");
+__device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+constexpr int kMaxDevices = 64;
+unsigned char* scratch_ptr[kMaxDevices] = {};   // device address of `scratch` on each device
+
+// The descriptor comes by value (no host-to-device copy); the fault flag and
+// the central body's output row are cleared.
+__global__ void noalloc_setup(Sys d, unsigned char* scr, double* a, long long ld) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        *reinterpret_cast<Sys*>(scr) = d;
+        *reinterpret_cast<int*>(scr + kNoAllocFault) = 0;
+        a[0] = 0.0; a[ld] = 0.0; a[2 * ld] = 0.0;
+    }
+}
+
 // What a captured advance depends on: replay only when all of it is unchanged.
 struct GraphKey {
     int nsteps, screen, reset, spl, block, gpw, ntiles, nsys, ev_cap, req_cap, park_cap, n, npart;
@@ -1788,6 +1813,53 @@ int mercury_hip_force(double* a) {
         check_fault("Nonfinite force");
         download_vec3(ctx.ra1, a);
         a[0] = a[1] = a[2] = 0.0;
+        return 0;
+    } catch (const std::exception& e) { return fail(e); }
+}
+
+int mercury_hip_force_noalloc(int n, int nbig, int pn, const double* m, const double* x, const double* v,
+                              const double* jcen, double* a, long long ld) {
+    try {
+        if (n < 1 || nbig < 1 || nbig > n) throw std::runtime_error("invalid body counts");
+        if (nbig > kMaxBig) throw std::runtime_error("more Big bodies than the allocation-free force's scratch holds (64)");
+        if (ld < n) throw std::runtime_error("leading dimension smaller than the body count");
+        if (!m || !x || !a || (pn && !v) || !jcen) throw std::runtime_error("missing array");
+        int dev = 0;
+        HIPCHECK(hipGetDevice(&dev));
+        if (dev < 0 || dev >= kMaxDevices) throw std::runtime_error("device index beyond the scratch table");
+        if (!scratch_ptr[dev])
+            HIPCHECK(hipGetSymbolAddress(reinterpret_cast<void**>(&scratch_ptr[dev]), HIP_SYMBOL(scratch)));
+        unsigned char* scr = scratch_ptr[dev];
+        double mu = 0.0;
+        HIPCHECK(hipMemcpy(&mu, m, sizeof(double), hipMemcpyDeviceToHost));
+        if (!(mu > 0)) throw std::runtime_error("central mass must be positive");
+        const int np = nbig - 1;
+        Sys d{};
+        d.base = 0; d.n = n; d.nbig = nbig; d.ts = start_stride(np);
+        d.tab = 0; d.start = 0;
+        d.mu = mu; d.h = 1.0; d.hby2 = 0.5; d.rcen = 0.0;
+        d.j2 = jcen[0]; d.j4 = jcen[1]; d.j6 = jcen[2];
+        d.obl = (jcen[0] != 0 || jcen[1] != 0 || jcen[2] != 0);
+        State st{};
+        st.x = const_cast<double*>(x); st.v = const_cast<double*>(pn ? v : x); st.m = m;
+        st.fault = reinterpret_cast<int*>(scr + kNoAllocFault);
+        st.cap = ld;
+        static const RaConst K = ra_constants();
+        RaArgs args{reinterpret_cast<const Sys*>(scr), st, RaState{nullptr, nullptr, a},
+                    reinterpret_cast<double*>(scr + kNoAllocTab), nullptr, K, 0.0, 0, 0, pn ? 1 : 0,
+                    light_speed2()};
+        hipLaunchKernelGGL(noalloc_setup, dim3(1), dim3(64), 0, 0, d, scr, a, ld);
+        if (pn) hipLaunchKernelGGL(ra_planets<true>, dim3(1), dim3(64), 0, 0, args);
+        else hipLaunchKernelGGL(ra_planets<false>, dim3(1), dim3(64), 0, 0, args);
+        if (n > nbig) {
+            dim3 grid((n - nbig + 127) / 128);
+            if (pn) hipLaunchKernelGGL(ra_particles<true>, grid, dim3(128), 0, 0, args);
+            else hipLaunchKernelGGL(ra_particles<false>, grid, dim3(128), 0, 0, args);
+        }
+        HIPCHECK(hipGetLastError());
+        int fault = 0;
+        HIPCHECK(hipMemcpy(&fault, scr + kNoAllocFault, sizeof(int), hipMemcpyDeviceToHost));
+        if (fault & FAULT_FORCE) throw std::runtime_error("Nonfinite force: non-finite acceleration");
         return 0;
     } catch (const std::exception& e) { return fail(e); }
 }

@@ -47,6 +47,8 @@ SIGNATURES = {
     "mercury_hip_events": (C.c_int, [C.c_double, C.c_double, C.c_double,
                                      C.POINTER(C.POINTER(MercuryEvent)), _ip]),
     "mercury_hip_force": (C.c_int, [_dp]),
+    "mercury_hip_force_noalloc": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                            _dp, C.c_void_p, C.c_longlong]),
     "mercury_hip_hybrid_begin": (C.c_int, [C.c_double, _dp, C.c_int, C.c_int, _ip, _ip, _ip, _ip,
                                            _dp, _dp, _i64p]),
     "mercury_hip_hybrid_finish": (C.c_int, [C.c_double, _dp, _dp, _dp, _i64p]),
@@ -169,6 +171,14 @@ class MercuryHip:
         a = np.zeros((self.n, 3))
         self._ok(self.L.mercury_hip_force(_d(a)), "force")
         return a
+
+    def force_noalloc(self, n, nbig, m_dev, x_dev, v_dev, a_dev, ld, *, pn=0, jcen=(0.0, 0.0, 0.0)):
+        """mercury_hip_force_noalloc on device buffers (addresses as ints):
+        m[n], x/v/a component-major with leading dimension ld."""
+        jc = _f64(jcen, (3,))
+        self._ok(self.L.mercury_hip_force_noalloc(n, nbig, pn, C.c_void_p(m_dev), C.c_void_p(x_dev),
+                                                  C.c_void_p(v_dev), _d(jc), C.c_void_p(a_dev), ld),
+                 "force_noalloc")
 
     def advance(self, time, h, nsteps, rcen=0.005, screen=True):
         nf = C.c_int64(0)

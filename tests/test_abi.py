@@ -66,8 +66,15 @@ def test_host_side_errors_without_device(lib, capfd):
                                     x.ctypes.data_as(C.POINTER(C.c_double)), 0) != 0
     assert lib.mercury_hip_set_tuning(0, 512) != 0
     assert lib.mercury_hip_set_tuning(0, 96) != 0
+    jc = np.zeros(3)
+    jp = jc.ctypes.data_as(C.POINTER(C.c_double))
+    dummy = C.c_void_p(1)
+    assert lib.mercury_hip_force_noalloc(0, 1, 0, dummy, dummy, dummy, jp, dummy, 8) != 0
+    assert lib.mercury_hip_force_noalloc(100, 65, 0, dummy, dummy, dummy, jp, dummy, 100) != 0
+    assert lib.mercury_hip_force_noalloc(100, 9, 0, dummy, dummy, dummy, jp, dummy, 99) != 0
     err = capfd.readouterr().err
     assert "no state uploaded" in err and "block size" in err and "no encounter context" in err
+    assert "scratch holds (64)" in err and "leading dimension" in err
 
 
 def test_no_device_means_not_available(lib):
