@@ -170,6 +170,17 @@ class Oracle:
         self.L.orc_mce_stat(time, h, 0.0, len(arrs[0]), nbig, *[_d(a) for a in arrs], C.byref(ev.s))
         return ev
 
+    def mce_stat_lists(self, time, h, m, x0, v0, x1, v1, rce, rphys, nbig, cap=4096):
+        """mce_stat's outputs in the layout Reference.mce_stat returns (1-based
+        indices as the Fortran's)."""
+        ev = self.mce_stat(time, h, m, x0, v0, x1, v1, rce, rphys, nbig, cap)
+        k, q = ev.s.nclo, ev.s.nhit
+        return dict(iclo=ev.iclo[:k].copy(), jclo=ev.jclo[:k].copy(), dclo=ev.dclo[:k].copy(),
+                    tclo=ev.tclo[:k].copy(), ixv=ev.ixv[:k].copy(), jxv=ev.jxv[:k].copy(),
+                    ihit=ev.ihit[:q].copy(), jhit=ev.jhit[:q].copy(), chit=ev.chit[:q].copy(),
+                    dhit=ev.dhit[:q].copy(), thit=ev.thit[:q].copy(), thit1=ev.s.thit1,
+                    nowflag=ev.s.nowflag)
+
     def mce_cent(self, time, h, rcen, m, x0, v0, x1, v1, i0=2):
         arrs = [np.ascontiguousarray(a, np.float64) for a in (m, x0, v0, x1, v1)]
         n = len(arrs[0])
@@ -486,6 +497,34 @@ class Reference:
                          _d(arrs[2]), _d(arrs[3]), _d(arrs[4]), C.byref(nhit), _i(jhit),
                          _d(thit), _d(dhit), C.byref(C.c_int(1)), _d(ngf), C.byref(C.c_int(0)))
         return [(int(jhit[q]), float(thit[q]), float(dhit[q])) for q in range(nhit.value)]
+
+    def mce_stat(self, time, h, m, x0, v0, x1, v1, rce, rphys, nbig, nclo0=0):
+        """The reference mce_stat (mercury6_2.for:1765-1931) on one step's
+        start (x0, v0) and end (x1, v1) states: the close-encounter list (appended
+        after nclo0 entries) and the near-miss/collision list, with thit1 and
+        nowflag."""
+        arrs = [np.ascontiguousarray(a, np.float64) for a in (m, x0, v0, x1, v1, rce, rphys)]
+        n = len(arrs[0])
+        cm = self.cmax
+        iclo = np.zeros(cm, np.int32); jclo = np.zeros(cm, np.int32)
+        dclo = np.zeros(cm); tclo = np.zeros(cm); ixv = np.zeros((cm, 6)); jxv = np.zeros((cm, 6))
+        ihit = np.zeros(cm, np.int32); jhit = np.zeros(cm, np.int32); chit = np.zeros(cm, np.int32)
+        dhit = np.zeros(cm); thit = np.zeros(cm)
+        nclo = C.c_int(nclo0); nhit = C.c_int(0); thit1 = C.c_double(0.0); nowflag = C.c_int(0)
+        stat = np.zeros(n, np.int32)
+        outfile = C.create_string_buffer(b"/dev/null".ljust(80))
+        mem = C.create_string_buffer(b" " * (80 * self.NMESS))
+        lmem = np.ones(self.NMESS, np.int32)
+        self.L.mce_stat_(self._r(time), self._r(h), self._r(0.0), C.byref(C.c_int(n)), C.byref(C.c_int(nbig)),
+                         *[_d(a) for a in arrs], C.byref(nclo), _i(iclo), _i(jclo), _d(dclo), _d(tclo),
+                         _d(ixv), _d(jxv), C.byref(nhit), _i(ihit), _i(jhit), _i(chit), _d(dhit), _d(thit),
+                         C.byref(thit1), C.byref(nowflag), _i(stat), outfile, mem, _i(lmem),
+                         C.c_size_t(80), C.c_size_t(80))
+        k, q = nclo.value, nhit.value
+        return dict(iclo=iclo[nclo0:k].copy(), jclo=jclo[nclo0:k].copy(), dclo=dclo[nclo0:k].copy(),
+                    tclo=tclo[nclo0:k].copy(), ixv=ixv[nclo0:k].copy(), jxv=jxv[nclo0:k].copy(),
+                    ihit=ihit[:q].copy(), jhit=jhit[:q].copy(), chit=chit[:q].copy(), dhit=dhit[:q].copy(),
+                    thit=thit[:q].copy(), thit1=thit1.value, nowflag=nowflag.value)
 
     def mdt_mvs_run(self, sysd, nsteps, time0=0.0, algor=1):
         """nsteps calls of the reference mdt_mvs on a whole system, plus the

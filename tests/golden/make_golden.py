@@ -5,7 +5,8 @@ oracle/Makefile).  Run here, where /root/reference exists:
     make -C oracle ref && python tests/golden/make_golden.py [radau|hybrid]
 
 (`radau` rewrites only the mfo_all / mdt_ra15 / mco_mvs2h fixtures, `hybrid`
-only the mdt_hy runs.)
+only the mdt_hy runs, `bs` only the BS runs, `mce_stat` only mce_stat's
+encounter and near-miss/collision lists.)
 
 Each fixture holds inputs and the reference's outputs only (data, no code).
 """
@@ -118,7 +119,80 @@ def main():
     radau(R)
     bs(R)
     hybrid(R)
+    mce_stat(R)
     print("wrote", sorted(p.name for p in HERE.glob("*.npz")))
+
+
+def stat_case(rng, R, nplanets, nparticles, h, near):
+    """Start and end states of one step for mce_stat: planets on near-circular
+    orbits drifted by h; particles (and, in pairs, the planets after the first)
+    placed near a planet with a straight relative path whose closest approach,
+    as a fraction of the pair's summed physical radii, is drawn around the
+    near-miss (2x) and collision (1x) thresholds."""
+    nb = nplanets + 1
+    n = nb + nparticles
+    m = np.zeros(n)
+    m[0] = MU
+    m[1:nb] = 10 ** rng.uniform(-9, -5, nplanets) * MU
+    rphys = np.zeros(n)
+    rphys[1:] = 10 ** rng.uniform(-4.5, -3, n - 1)
+    rce = np.zeros(n)
+    rce[1:] = 10 ** rng.uniform(-2.5, -1, n - 1)
+    x0 = np.zeros((n, 3)); v0 = np.zeros((n, 3)); x1 = np.zeros((n, 3)); v1 = np.zeros((n, 3))
+    for j in range(1, nb):
+        a = rng.uniform(0.5, 5.0)
+        th = rng.uniform(0, 2 * np.pi)
+        x0[j] = [a * np.cos(th), a * np.sin(th), rng.normal() * 1e-3]
+        vc = np.sqrt(MU / a)
+        v0[j] = [-vc * np.sin(th), vc * np.cos(th), rng.normal() * 1e-5]
+        out, _ = R.drift_one(MU, np.concatenate([x0[j], v0[j]]), h)
+        x1[j], v1[j] = out[:3], out[3:]
+    # bodies near a planet: Big bodies near an earlier planet (Big-Big pairs,
+    # mass order mixed), particles near any planet
+    for j in range(2, n):
+        if j < nb and rng.uniform() > near:
+            continue
+        if j >= nb and rng.uniform() > near:
+            a = rng.uniform(0.5, 5.0); th = rng.uniform(0, 2 * np.pi)
+            x0[j] = [a * np.cos(th), a * np.sin(th), 0.0]
+            vc = np.sqrt(MU / a) * rng.uniform(0.9, 1.1)
+            v0[j] = [-vc * np.sin(th), vc * np.cos(th), 0.0]
+            out, _ = R.drift_one(MU, np.concatenate([x0[j], v0[j]]), h)
+            x1[j], v1[j] = out[:3], out[3:]
+            continue
+        p = int(rng.integers(1, min(j, nb)))
+        dv = rng.normal(size=3); dv *= 10 ** rng.uniform(-5, -3) / np.linalg.norm(dv)
+        tc = rng.uniform(-1.3, 0.3) * h            # time of closest approach relative to the step end
+        e = rng.normal(size=3); e -= e.dot(dv) / dv.dot(dv) * dv; e /= np.linalg.norm(e)
+        b = (rphys[p] + rphys[j]) * rng.choice([0.3, 0.9, 1.0, 1.1, 1.7, 2.0, 2.2, 5.0, 50.0])
+        if rng.uniform() < 0.3:
+            b = rce[j] * rng.uniform(0.2, 1.5)
+        d_end = b * e - dv * tc                     # relative offset at the step end
+        x1[j] = x1[p] + d_end; v1[j] = v1[p] + dv
+        x0[j] = x0[p] + d_end - dv * h; v0[j] = v0[p] + dv
+    return dict(m=m, x0=x0, v0=v0, x1=x1, v1=v1, rce=rce, rphys=rphys, nbig=nb, h=h)
+
+
+def mce_stat(R=None):
+    """mce_stat on constructed steps: close encounters, near misses and
+    collisions of particles and of Big pairs, both step signs."""
+    R = R or Reference()
+    rng = np.random.default_rng(20261204)
+    out = {}
+    shapes = [(8, 200, 4.0, 0.8), (5, 150, -2.0, 0.8), (12, 100, 8.0, 0.6), (3, 300, 1.0, 0.9),
+              (8, 0, 4.0, 1.0), (2, 50, -8.0, 1.0)]
+    for c, (npl, npart, h, near) in enumerate(shapes):
+        q = stat_case(rng, R, npl, npart, h, near)
+        time = float(rng.uniform(-100, 100))
+        r = R.mce_stat(time, h, q["m"], q["x0"], q["v0"], q["x1"], q["v1"], q["rce"], q["rphys"], q["nbig"])
+        for k in ("m", "x0", "v0", "x1", "v1", "rce", "rphys"):
+            out[f"{k}{c}"] = q[k]
+        out[f"meta{c}"] = np.array([q["nbig"], time, h], float)
+        for k, v in r.items():
+            out[f"{k}{c}"] = np.asarray(v)
+        print(f"mce_stat case {c}: {len(r['iclo'])} encounters, {len(r['ihit'])} near misses/collisions "
+              f"({int(np.sum(np.asarray(r['chit']) == 1))} collisions)")
+    np.savez_compressed(HERE / "mce_stat.npz", **out)
 
 
 def radau(R=None):
@@ -203,6 +277,6 @@ def hybrid(R=None):
 if __name__ == "__main__":
     threading.stack_size(512 << 20)
     only = sys.argv[1] if len(sys.argv) > 1 else None
-    t = threading.Thread(target={"radau": radau, "bs": bs, "hybrid": hybrid}.get(only, main))
+    t = threading.Thread(target={"radau": radau, "bs": bs, "hybrid": hybrid, "mce_stat": mce_stat}.get(only, main))
     t.start()
     t.join()

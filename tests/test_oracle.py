@@ -83,7 +83,89 @@ def test_golden_mvs_runs(oracle):
         assert mine == [tuple(r) for r in g[f"evi{c}"]]
 
 
+STAT_KEYS = ("iclo", "jclo", "dclo", "tclo", "ixv", "jxv", "ihit", "jhit", "chit", "dhit", "thit")
+
+
+def _stat_equal(got, want, label):
+    for k in STAT_KEYS:
+        assert np.array_equal(np.asarray(got[k]), np.asarray(want[k])), (label, k)
+    assert got["thit1"] == want["thit1"] and got["nowflag"] == want["nowflag"], label
+
+
+def test_golden_mce_stat(oracle):
+    """mce_stat's whole output -- close encounters (more massive body first,
+    states interpolated to the minimum), near misses and collisions (chit -1 /
+    1), thit1 and nowflag -- bitwise against the reference on constructed
+    steps with Big-Big and Big-particle pairs at the thresholds, both signs of h."""
+    g = np.load(GOLD / "mce_stat.npz")
+    c = 0
+    kinds = set()
+    swapped = 0
+    while f"meta{c}" in g:
+        nbig, time, h = g[f"meta{c}"]
+        got = oracle.mce_stat_lists(time, h, g[f"m{c}"], g[f"x0{c}"], g[f"v0{c}"], g[f"x1{c}"], g[f"v1{c}"],
+                                    g[f"rce{c}"], g[f"rphys{c}"], int(nbig))
+        want = {k: g[f"{k}{c}"] for k in STAT_KEYS + ("thit1", "nowflag")}
+        want["thit1"] = float(want["thit1"]); want["nowflag"] = int(want["nowflag"])
+        _stat_equal(got, want, f"case {c}")
+        kinds |= set(int(q) for q in want["chit"])
+        swapped += int(np.sum(want["iclo"] > want["jclo"]))
+        c += 1
+    assert c >= 6 and kinds == {-1, 1} and swapped > 0
+
+
+def test_mce_stat_classification_closed_form(oracle):
+    """Straight relative paths: the squared separation is quadratic in time,
+    which mce_min's cubic reproduces exactly, so the minimum is the impact
+    parameter b.  b below the summed radii is a collision, below twice them
+    a near miss, beyond that no hit; within the encounter radius (which pads
+    the x-y boxes) the minimum is also a close encounter; the more massive Big
+    body of a Big pair is listed first."""
+    h = 4.0
+    rsum = 1e-3 + 5e-4
+    for b_over, chit in ((0.5, 1), (0.99, 1), (1.5, -1), (1.99, -1), (2.01, None), (10.0, None)):
+        for big_pair in (False, True):
+            m = np.array([MU, 1e-9, 3e-9 if big_pair else 0.0])
+            nbig = 3 if big_pair else 2
+            rphys = np.array([0.0, 1e-3, 5e-4])
+            rce = np.array([0.0, 1e-2, 1e-2])
+            x0 = np.zeros((3, 3)); v0 = np.zeros((3, 3))
+            x0[1] = [1.0, 0.0, 0.0]
+            dv = np.array([0.0, 2e-4, 0.0])
+            tc = -0.3 * h                                # closest approach inside the step
+            x1 = x0.copy(); v1 = v0.copy()
+            x1[2] = x0[1] + np.array([b_over * rsum, 0.0, 0.0]) - dv * tc
+            x0[2] = x1[2] - dv * h
+            v0[2] = dv; v1[2] = dv
+            r = oracle.mce_stat_lists(10.0, h, m, x0, v0, x1, v1, rce, rphys, nbig)
+            b = b_over * rsum
+            first = (3, 2) if big_pair else (2, 3)
+            assert len(r["iclo"]) == (1 if b < rce[1] else 0)
+            if len(r["iclo"]):
+                assert (r["iclo"][0], r["jclo"][0]) == first
+                assert abs(r["dclo"][0] - b) <= 1e-9 * rsum
+                # interpolated states at the minimum: the pair's separation there is b
+                sep = np.linalg.norm(r["ixv"][0][:3] - r["jxv"][0][:3])
+                assert abs(sep - b) <= 1e-9 * rsum
+            if chit is None:
+                assert len(r["ihit"]) == 0
+                continue
+            assert list(r["chit"]) == [chit]
+            assert abs(r["dhit"][0] - b) <= 1e-9 * rsum
+            assert abs(r["thit"][0] - (10.0 + tc)) <= 1e-9
+            assert (r["ihit"][0], r["jhit"][0]) == first
+
+
 # ---------------------------------------------------------------- live reference
+def test_reference_mce_stat_random(oracle, reference):
+    from golden.make_golden import stat_case
+    rng = np.random.default_rng(77)
+    for c, (npl, npart, h, near) in enumerate([(8, 150, 4.0, 0.8), (6, 100, -4.0, 0.9), (10, 80, 2.0, 0.7)]):
+        q = stat_case(rng, reference, npl, npart, h, near)
+        args = (3.5, h, q["m"], q["x0"], q["v0"], q["x1"], q["v1"], q["rce"], q["rphys"], q["nbig"])
+        _stat_equal(oracle.mce_stat_lists(*args), run_big_stack(reference.mce_stat, *args), f"random {c}")
+
+
 def test_reference_drift_random(oracle, reference):
     rng = np.random.default_rng(99)
     from golden.make_golden import drift_cases
