@@ -83,3 +83,56 @@ def read_dump(path, filename):
 
 def info_lines(path):
     return (Path(path) / "info.out").read_text().splitlines()
+
+
+def body(name="PARTICLE", mass=0.0, a=1.0, e=0.1, phase=0.0, **params):
+    """A body at the pericentre of a coplanar orbit about one solar mass (the
+    reference's test bodies, tests/cases.py:12-17): mass in solar masses, extra
+    big.in/small.in parameters (d=, r=, a1=, a2=, a3=, b=, yar=) as keywords."""
+    r = a * (1 - e)
+    speed = np.sqrt(systems.K2 * (1 + e) / r)
+    c, s = np.cos(phase), np.sin(phase)
+    return dict(name=name, mass=mass, x=[r * c, r * s, 0.0], v=[-speed * s, speed * c, 0.0], params=params)
+
+
+def write_case(path, small=None, big=None, *, algorithm="BS", stop=100.0, step=1.0, interval=7.3, tol=1e-12,
+               backend="cpu", collisions=False, pn=False, rcen=0.005, rmax=100.0, periodic=100,
+               dump_every=5000, changeover=3.0, jcen=(0.0, 0.0, 0.0)):
+    """A run directory from named bodies (body()), with the driver's parameters
+    as the reference's own test cases set them (tests/cases.py:20-48)."""
+    path = Path(path)
+    path.mkdir(parents=True, exist_ok=True)
+    for name in ("message.in", "files.in"):
+        shutil.copyfile(REF_DIR / name, path / name)
+    values = [algorithm, 0.0, stop, interval, step, tol, "no", "yes" if collisions else "no", "no", "days", "no",
+              "high", "unused", "yes" if pn else "no", "no", rmax, rcen, 1.0, *jcen, "unused", "unused",
+              changeover, dump_every, periodic]
+    fmt = lambda v: f"{v:.17e}" if isinstance(v, float) else str(v)
+    (path / "param.in").write_text(")O+_06 Integration parameters\n"
+                                   + "\n".join(f" {lab} = {fmt(v)}" for lab, v in zip(PARAM_LABELS, values))
+                                   + f"\n execution backend = {backend}\n")
+    for fname, objs, is_big in (("big.in", big or [], True), ("small.in", small if small is not None else [body()],
+                                                             False)):
+        lines = [")O+_06 Initial data", " style = Cartesian"]
+        if is_big:
+            lines.append(f" epoch = {0.0:.17e}")
+        for o in objs:
+            params = {"m": o["mass"], **o.get("params", {})}
+            lines.append(o["name"] + " " + " ".join(f"{k}={v:.17e}" for k, v in params.items()))
+            lines.append(" ".join(f"{q:.17e}" for q in list(o["x"]) + list(o["v"]) + [0.0] * 3))
+        (path / fname).write_text("\n".join(lines) + "\n")
+    return path
+
+
+def read_dump_full(path, filename="small.dmp"):
+    """name -> dict(mass, x[3], v[3], params) from a dump file."""
+    lines = [ln.strip() for ln in (Path(path) / filename).read_text().splitlines()
+             if ln.strip() and not ln.startswith(")")]
+    lines = lines[2 if filename.startswith("big") else 1:]
+    out = {}
+    for k in range(0, len(lines), 4):
+        tok = lines[k].replace("=", " = ").split()
+        params = {tok[i - 1]: float(tok[i + 1].replace("D", "E")) for i, t in enumerate(tok) if t == "="}
+        vals = [float(t.replace("D", "E")) for ln in lines[k + 1:k + 3] for t in ln.split()[:3]]
+        out[tok[0]] = dict(mass=params.pop("m", 0.0), x=np.array(vals[:3]), v=np.array(vals[3:6]), params=params)
+    return out
