@@ -305,8 +305,11 @@ void ensure_capacity(int n) {
     ctx.req_cap = (int)std::min<long long>(std::max<long long>(cap / 2, 1 << 18), 1LL << 26);
     dalloc(ctx.dreq, (size_t)ctx.req_cap * kRing);
     if (!ctx.devcount) dalloc(ctx.devcount, 1);
-    if (ctx.ev_cap < 4096) {
+    if (!ctx.dev) {
+        // the event list starts at 2^18 records (MERCURY_HIP_EVENT_CAP overrides)
+        // and doubles when a step's events need more (mercury_hip_events)
         ctx.ev_cap = 1 << 18;
+        if (const char* e = std::getenv("MERCURY_HIP_EVENT_CAP")) ctx.ev_cap = std::max(1, std::atoi(e));
         dalloc(ctx.dev, (size_t)ctx.ev_cap);
     }
     ctx.cap = cap;
@@ -1979,7 +1982,10 @@ int mercury_hip_advance(double time, double h, int nsteps, double rcen, int scre
             if (f & FAULT_FORCE) throw std::runtime_error("MVS force failed: non-finite acceleration");
             if (f & FAULT_REQ_OVERFLOW)
                 throw std::runtime_error("screening request buffer overflow: use fewer steps per launch");
-            if (f & FAULT_EVENT_OVERFLOW) throw std::runtime_error("event buffer overflow: advance fewer steps per call");
+            if (f & FAULT_EVENT_OVERFLOW) {
+                ensure_events(ctx.hstatus[1]);   // room for as many in the next call
+                throw std::runtime_error("event buffer overflow: advance fewer steps per call");
+            }
             cnt = ctx.hstatus[1];
             ctx.parked = ctx.hstatus[2];
         }

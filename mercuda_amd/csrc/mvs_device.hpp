@@ -693,6 +693,8 @@ __device__ __forceinline__ void planets_run(const PlanetArgs& args, PlanetLds& L
         if (lane == 0) h2j_serial<NPL>(np, m, jdx, X, V, XJ, VJ);
         __syncthreads();
         big_forces<NPL>(d, np, m, minside, X, XJ, A, t0, t2, a1, aob, hdr);
+        if (lane < np && !(isfinite(A[3 * lane]) && isfinite(A[3 * lane + 1]) && isfinite(A[3 * lane + 2])))
+            atomicOr(args.st.fault, FAULT_FORCE);
         double* St = args.tab + d.start;
         if (lane < 8) St[lane] = (lane < 6) ? hdr[lane] : 0.0;
         for (int p = lane; p < np; p += 64) {
@@ -758,6 +760,8 @@ __device__ __forceinline__ void planets_run(const PlanetArgs& args, PlanetLds& L
                 }
         }
         big_forces<NPL>(d, np, m, minside, X, XJ, A, t0, t2, a1, aob, hdr);
+        if (lane < np && !(isfinite(A[3 * lane]) && isfinite(A[3 * lane + 1]) && isfinite(A[3 * lane + 2])))
+            atomicOr(args.st.fault, FAULT_FORCE);
         if (lane < 8) T[lane] = (lane < 6) ? hdr[lane] : (lane == 6 ? time_end : 0.0);
         if (p < np) {
             for (int c = 0; c < 3; c++) V[3 * p + c] = V[3 * p + c] + hby2 * A[3 * p + c];
@@ -1056,6 +1060,7 @@ __device__ __forceinline__ void fast_tile(const ParticleArgs& args, double* lds_
                   args.st.v[g], args.st.v[cap + g], args.st.v[2 * cap + g]};
         if (args.reset) {
             particle_accel<NP, RangeOps>(d, args.tab + d.start, np, o.x, o.y, o.z, ax, ay, az);
+            if (!isfinite(ax) || !isfinite(ay) || !isfinite(az)) atomicOr(args.st.fault, FAULT_FORCE);
         } else {
             ax = args.st.a[g]; ay = args.st.a[cap + g]; az = args.st.a[2 * cap + g];
         }
@@ -1201,6 +1206,7 @@ __global__ void __launch_bounds__(64) mvs_particles_general(ParticleArgs args) {
             if (args.reset && pk.chunk < 0) {
                 // a kept member starts a call that recomputes accelerations
                 particle_accel(d, args.tab + d.start, np, o.x, o.y, o.z, ax, ay, az);
+                if (!isfinite(ax) || !isfinite(ay) || !isfinite(az)) atomicOr(args.st.fault, FAULT_FORCE);
             } else {
                 ax = args.st.a[g]; ay = args.st.a[cap + g]; az = args.st.a[2 * cap + g];
             }

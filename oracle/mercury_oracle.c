@@ -873,7 +873,7 @@ int orc_mvs_run(double time0, double h0, double rcen, const double *jcen,
     size_t n3 = (size_t)3 * nbod;
     double *a = calloc(n3, sizeof(double));
     double *xs = malloc(n3 * sizeof(double)), *vs = malloc(n3 * sizeof(double));
-    int lcap = 4096;
+    int lcap = 2 * nbod > 4096 ? 2 * nbod : 4096;   /* events of one step */
     orc_events ev;
     memset(&ev, 0, sizeof ev);
     ev.cap = lcap;
@@ -1830,12 +1830,14 @@ int orc_hy_run(double time0, double h0, double tol, double rcen, const double *j
     st.dtflag = 0;
     st.hrec = h0;
     st.nforce = 0;
-    st.cap = 20000;
+    /* encounter-pair and event capacities: at least 20000, and room for two
+       pairs per body (half of a 200k-particle system near planet orbits) */
+    st.cap = nbod * 2 > 20000 ? nbod * 2 : 20000;
     st.ice = calloc((size_t)st.cap, sizeof(int));
     st.jce = calloc((size_t)st.cap, sizeof(int));
     orc_events ev;
     memset(&ev, 0, sizeof(ev));
-    ev.cap = 20000;
+    ev.cap = st.cap;
     ev.iclo = calloc((size_t)ev.cap, sizeof(int)); ev.jclo = calloc((size_t)ev.cap, sizeof(int));
     ev.dclo = calloc((size_t)ev.cap, sizeof(double)); ev.tclo = calloc((size_t)ev.cap, sizeof(double));
     ev.ixvclo = calloc((size_t)6 * ev.cap, sizeof(double)); ev.jxvclo = calloc((size_t)6 * ev.cap, sizeof(double));
