@@ -189,3 +189,31 @@ def ensemble(seed=4, nsystems=512, nmin=100, nmax=50_000, h=4.0):
         out.append(make_system(seed * 100003 + s, npart, nplanets=nbig, a_range=(0.5, 40.0),
                                emax=0.3, imax_deg=20.0, h=h))
     return out
+
+
+def add_nongrav(sysd, seed, fraction=0.5, kinds=("a1", "a2", "a3", "yar", "beta"), big=False):
+    """Non-gravitational parameters (the driver's ngf(5,n): A1, A2, A3 in
+    AU/day^2, beta, yar) on a seeded fraction of the particles (and of the Big
+    bodies when big is true, except beta, which mfo_pr applies to massless
+    bodies only): cometary A values ~1e-9..1e-7, some above the 1e-7 cutoff
+    threshold; yar ~1e-12..1e-10; beta ~0.01..0.3.  Sets sysd["ngf"] and
+    sysd["ngflag"] (1 cometary and Yarkovsky, 2 radiation, 3 both)."""
+    rng = np.random.default_rng(seed)
+    n = len(sysd["m"])
+    ngf = np.zeros((n, 5))
+    first = 1 if big else sysd["nbig"]
+    for j in range(first, n):
+        if rng.random() > fraction:
+            continue
+        for k, name in enumerate(("a1", "a2", "a3")):
+            if name in kinds and rng.random() < 0.6:
+                ngf[j, k] = rng.choice([-1.0, 1.0]) * 10 ** rng.uniform(-9, -6.7)
+        if "beta" in kinds and j >= sysd["nbig"] and rng.random() < 0.5:
+            ngf[j, 3] = 10 ** rng.uniform(-2, -0.5)
+        if "yar" in kinds and rng.random() < 0.5:
+            ngf[j, 4] = rng.choice([-1.0, 1.0]) * 10 ** rng.uniform(-12, -10)
+    comet = bool(np.any(ngf[:, [0, 1, 2, 4]] != 0))
+    rad = bool(np.any(ngf[:, 3] != 0))
+    sysd["ngf"] = ngf
+    sysd["ngflag"] = (1 if comet else 0) + (2 if rad else 0)
+    return sysd

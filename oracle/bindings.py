@@ -10,6 +10,7 @@ Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use these.
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
 import os
 import subprocess
@@ -125,6 +126,10 @@ class Oracle:
         L.orc_bs_run.restype = C.c_long
         L.orc_bs_run.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, _dp, C.c_int, C.c_int, _dp, _dp,
                                  _dp, C.c_int, C.c_int, _dp, _dp, C.POINTER(C.c_long)]
+        L.orc_set_ngf.argtypes = [C.c_int, _dp, C.c_int]
+        L.orc_mfo_ngf.restype = C.c_int
+        L.orc_mfo_ngf.argtypes = [C.c_int, _dp, _dp, _dp, _dp]
+        L.orc_mfo_pr.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp]
         L.orc_mce_snif.restype = C.c_int
         L.orc_mce_snif.argtypes = [C.c_double, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _ip, C.c_int, _ip, _ip]
 
@@ -157,6 +162,36 @@ class Oracle:
         a = np.zeros_like(x)
         jc = np.ascontiguousarray(jcen, np.float64)
         self.L.orc_mfo_mvs(_d(jc), n, nbig, _d(m), _d(x), _d(xj), _d(a))
+        return a
+
+    @contextlib.contextmanager
+    def ngf_model(self, sysd):
+        """The system's non-gravitational model (sysd["ngflag"], sysd["ngf"][n, 5])
+        for the calls inside the block (mfo_all, RA15, BS, MVS)."""
+        flag = int(sysd.get("ngflag", 0))
+        if not flag:
+            yield
+            return
+        ngf = np.ascontiguousarray(sysd["ngf"], np.float64)
+        self.L.orc_set_ngf(flag, _d(ngf), len(ngf))
+        try:
+            yield
+        finally:
+            self.L.orc_set_ngf(0, None, 0)
+
+    def mfo_ngf(self, x, v, ngf):
+        x = np.ascontiguousarray(x, np.float64); v = np.ascontiguousarray(v, np.float64)
+        ngf = np.ascontiguousarray(ngf, np.float64)
+        a = np.zeros_like(x)
+        bad = self.L.orc_mfo_ngf(len(x), _d(x), _d(v), _d(a), _d(ngf))
+        return a, bad
+
+    def mfo_pr(self, m, x, v, ngf):
+        m = np.ascontiguousarray(m, np.float64)
+        x = np.ascontiguousarray(x, np.float64); v = np.ascontiguousarray(v, np.float64)
+        ngf = np.ascontiguousarray(ngf, np.float64)
+        a = np.zeros_like(x)
+        self.L.orc_mfo_pr(len(x), _d(m), _d(x), _d(v), _d(a), _d(ngf))
         return a
 
     def mce_min(self, d0, d1, d0t, d1t, h):
@@ -197,9 +232,10 @@ class Oracle:
         rphys = np.ascontiguousarray(sysd["rphys"], np.float64)
         jc = np.ascontiguousarray(sysd.get("jcen", (0.0, 0.0, 0.0)), np.float64)
         evi = np.zeros((cap, 4), np.int32); evd = np.zeros((cap, 2))
-        k = self.L.orc_mvs_run(time0, sysd["h"], sysd["rcen"], _d(jc), len(m), sysd["nbig"],
-                               _d(m), _d(x), _d(v), _d(rce), _d(rphys), nsteps, int(screen),
-                               cap, _i(evi), _d(evd))
+        with self.ngf_model(sysd):
+            k = self.L.orc_mvs_run(time0, sysd["h"], sysd["rcen"], _d(jc), len(m), sysd["nbig"],
+                                   _d(m), _d(x), _d(v), _d(rce), _d(rphys), nsteps, int(screen),
+                                   cap, _i(evi), _d(evd))
         if k < 0:
             raise RuntimeError("oracle event capacity exceeded")
         return x, v, evi[:k].copy(), evd[:k].copy()
@@ -210,7 +246,8 @@ class Oracle:
         v = np.ascontiguousarray(sysd["v"] if v is None else v, np.float64)
         jc = np.ascontiguousarray(sysd.get("jcen", (0.0, 0.0, 0.0)), np.float64)
         a = np.zeros_like(x)
-        self.L.orc_mfo_all(_d(jc), len(m), sysd["nbig"], _d(m), _d(x), _d(v), int(pn), _d(a))
+        with self.ngf_model(sysd):
+            self.L.orc_mfo_all(_d(jc), len(m), sysd["nbig"], _d(m), _d(x), _d(v), int(pn), _d(a))
         return a
 
     def ra15_run(self, sysd, nsteps, h0=None, tol=1e-12, pn=0, time0=0.0):
@@ -221,8 +258,9 @@ class Oracle:
         v = np.ascontiguousarray(sysd["v"], np.float64).copy()
         jc = np.ascontiguousarray(sysd.get("jcen", (0.0, 0.0, 0.0)), np.float64)
         hdid = np.zeros(max(nsteps, 1)); hn = np.zeros(1)
-        nf = self.L.orc_ra15_run(time0, sysd["h"] if h0 is None else h0, tol, _d(jc), len(m), sysd["nbig"],
-                                 _d(m), _d(x), _d(v), int(pn), nsteps, _d(hdid), _d(hn))
+        with self.ngf_model(sysd):
+            nf = self.L.orc_ra15_run(time0, sysd["h"] if h0 is None else h0, tol, _d(jc), len(m), sysd["nbig"],
+                                     _d(m), _d(x), _d(v), int(pn), nsteps, _d(hdid), _d(hn))
         return x, v, hdid[:nsteps].copy(), float(hn[0]), int(nf)
 
     def bs_run(self, algor, sysd, nsteps, h0=None, tol=1e-12, pn=0, time0=0.0):
@@ -234,8 +272,10 @@ class Oracle:
         v = np.ascontiguousarray(sysd["v"], np.float64).copy()
         jc = np.ascontiguousarray(sysd.get("jcen", (0.0, 0.0, 0.0)), np.float64)
         hdid = np.zeros(max(nsteps, 1)); hn = np.zeros(1); rej = C.c_long(0)
-        nf = self.L.orc_bs_run(int(algor), time0, sysd["h"] if h0 is None else h0, tol, _d(jc), len(m),
-                               sysd["nbig"], _d(m), _d(x), _d(v), int(pn), nsteps, _d(hdid), _d(hn), C.byref(rej))
+        with self.ngf_model(sysd):
+            nf = self.L.orc_bs_run(int(algor), time0, sysd["h"] if h0 is None else h0, tol, _d(jc), len(m),
+                                   sysd["nbig"], _d(m), _d(x), _d(v), int(pn), nsteps, _d(hdid), _d(hn),
+                                   C.byref(rej))
         if nf < 0:
             raise RuntimeError(f"oracle BS run failed ({nf})")
         return x, v, hdid[:nsteps].copy(), float(hn[0]), int(rej.value), int(nf)
@@ -358,9 +398,11 @@ class Reference:
         n = len(m)
         jc = np.ascontiguousarray(sysd.get("jcen", (0.0, 0.0, 0.0)), np.float64)
         a = np.zeros_like(x); s = np.zeros((n, 3)); rc = np.zeros(n); st = np.zeros(n, np.int32)
-        ngf = np.zeros((n, 5)); opt = np.array([0, 1, 1, 2, 0, 1, int(pn), 0], np.int32)
+        ngf = np.ascontiguousarray(sysd.get("ngf", np.zeros((n, 5))), np.float64)
+        ngflag = C.c_int(int(sysd.get("ngflag", 0)))
+        opt = np.array([0, 1, 1, 2, 0, 1, int(pn), 0], np.int32)
         self.L.mfo_all_(self._r(0.0), _d(jc), C.byref(C.c_int(n)), C.byref(C.c_int(sysd["nbig"])), _d(m),
-                        _d(x), _d(v), _d(s), _d(rc), _d(a), _i(st), _d(ngf), C.byref(C.c_int(0)), _i(opt),
+                        _d(x), _d(v), _d(s), _d(rc), _d(a), _i(st), _d(ngf), C.byref(ngflag), _i(opt),
                         C.byref(C.c_int(0)), _i(np.zeros(1, np.int32)), _i(np.zeros(1, np.int32)))
         return a
 
@@ -372,7 +414,9 @@ class Reference:
         v = np.ascontiguousarray(sysd["v"], np.float64).copy()
         jc = np.ascontiguousarray(sysd.get("jcen", (0.0, 0.0, 0.0)), np.float64)
         s = np.zeros((n, 3)); rphys = np.zeros(n); rc = np.zeros(n); st = np.zeros(n, np.int32)
-        ngf = np.zeros((n, 5)); opt = np.array([0, 1, 1, 2, 0, 1, int(pn), 0], np.int32)
+        ngf = np.ascontiguousarray(sysd.get("ngf", np.zeros((n, 5))), np.float64)
+        ngflag = C.c_int(int(sysd.get("ngflag", 0)))
+        opt = np.array([0, 1, 1, 2, 0, 1, int(pn), 0], np.int32)
         force = C.cast(self.L.mfo_all_, C.c_void_p)
         t = C.c_double(sysd["h"] if h0 is None else h0); tdid = C.c_double(0.0)
         dtflag = C.c_int(0)
@@ -381,7 +425,7 @@ class Reference:
         for _ in range(nsteps):
             self.L.mdt_ra15_(self._r(time), C.byref(t), C.byref(tdid), self._r(tol), _d(jc), C.byref(C.c_int(n)),
                              C.byref(C.c_int(sysd["nbig"])), _d(m), _d(x), _d(v), _d(s), _d(rphys), _d(rc),
-                             _d(ngf), _i(st), C.byref(dtflag), C.byref(C.c_int(0)), _i(opt),
+                             _d(ngf), _i(st), C.byref(dtflag), C.byref(ngflag), _i(opt),
                              C.byref(C.c_int(0)), _i(np.zeros(1, np.int32)), _i(np.zeros(1, np.int32)), force)
             hd.append(tdid.value)
             time = time + tdid.value
@@ -396,7 +440,9 @@ class Reference:
         v = np.ascontiguousarray(sysd["v"], np.float64).copy()
         jc = np.ascontiguousarray(sysd.get("jcen", (0.0, 0.0, 0.0)), np.float64)
         s = np.zeros((n, 3)); rphys = np.zeros(n); rc = np.zeros(n); st = np.zeros(n, np.int32)
-        ngf = np.zeros((n, 5)); opt = np.array([0, 1, 1, 2, 0, 1, int(pn), 0], np.int32)
+        ngf = np.ascontiguousarray(sysd.get("ngf", np.zeros((n, 5))), np.float64)
+        ngflag = C.c_int(int(sysd.get("ngflag", 0)))
+        opt = np.array([0, 1, 1, 2, 0, 1, int(pn), 0], np.int32)
         force = C.cast(self.L.mfo_all_, C.c_void_p)
         fn = self.L.mdt_bs1_ if algor == 2 else self.L.mdt_bs2_
         t = C.c_double(sysd["h"] if h0 is None else h0); tdid = C.c_double(0.0)
@@ -406,7 +452,7 @@ class Reference:
         for _ in range(nsteps):
             fn(self._r(time), C.byref(t), C.byref(tdid), self._r(tol), _d(jc), C.byref(C.c_int(n)),
                C.byref(C.c_int(sysd["nbig"])), _d(m), _d(x), _d(v), _d(s), _d(rphys), _d(rc), _d(ngf), _i(st),
-               C.byref(dtflag), C.byref(C.c_int(0)), _i(opt), C.byref(C.c_int(0)), _i(np.zeros(1, np.int32)),
+               C.byref(dtflag), C.byref(ngflag), _i(opt), C.byref(C.c_int(0)), _i(np.zeros(1, np.int32)),
                _i(np.zeros(1, np.int32)), force)
             hd.append(tdid.value)
             time = time + tdid.value
@@ -498,6 +544,25 @@ class Reference:
                          _d(thit), _d(dhit), C.byref(C.c_int(1)), _d(ngf), C.byref(C.c_int(0)))
         return [(int(jhit[q]), float(thit[q]), float(dhit[q])) for q in range(nhit.value)]
 
+    def mfo_ngf(self, x, v, ngf):
+        """The reference mfo_ngf (mercury6_2.for:5106-5152) on every body."""
+        x = np.ascontiguousarray(x, np.float64); v = np.ascontiguousarray(v, np.float64)
+        ngf = np.ascontiguousarray(ngf, np.float64)
+        a = np.zeros_like(x)
+        self.L.mfo_ngf_(C.byref(C.c_int(len(x))), _d(x), _d(v), _d(a), _d(ngf))
+        return a
+
+    def mfo_pr(self, m, nbig, x, v, ngf):
+        """The reference mfo_pr (mercury6_2.for:5307-5357); ngf[n, 5] as the
+        driver holds it (the routine reads the first four of each body's
+        five, so the array is repacked to ngf(4,n))."""
+        m = np.ascontiguousarray(m, np.float64)
+        x = np.ascontiguousarray(x, np.float64); v = np.ascontiguousarray(v, np.float64)
+        ng4 = np.ascontiguousarray(np.asarray(ngf, np.float64)[:, :4])
+        a = np.zeros_like(x)
+        self.L.mfo_pr_(C.byref(C.c_int(len(x))), C.byref(C.c_int(nbig)), _d(m), _d(x), _d(v), _d(a), _d(ng4))
+        return a
+
     def mce_stat(self, time, h, m, x0, v0, x1, v1, rce, rphys, nbig, nclo0=0):
         """The reference mce_stat (mercury6_2.for:1765-1931) on one step's
         start (x0, v0) and end (x1, v1) states: the close-encounter list (appended
@@ -543,7 +608,11 @@ class Reference:
         h = sysd["h"]; rcen = sysd["rcen"]
         cm = self.cmax
         s = np.zeros((n, 3)); rcrit = np.zeros(n); stat = np.zeros(n, np.int32)
-        ngf = np.zeros((n, 5)); opt = np.array([0, 1, 1, 2, 0, 1, 0, 0], np.int32)
+        # the cometary model (mdt_mvs keeps its accelerations in a saved array
+        # that a later call with ngflag 0 does not clear: one model per process)
+        ngf = np.ascontiguousarray(sysd.get("ngf", np.zeros((n, 5))), np.float64)
+        ngflag = int(sysd.get("ngflag", 0))
+        opt = np.array([0, 1, 1, 2, 0, 1, 0, 0], np.int32)
         ids = C.create_string_buffer(b" " * (25 * n))
         outfile = C.create_string_buffer(b"/dev/null".ljust(80) * 3)
         mem = C.create_string_buffer(b" " * (80 * self.NMESS))
@@ -562,7 +631,7 @@ class Reference:
                             C.byref(C.c_int(n)), C.byref(C.c_int(nbig)), _d(m), _d(x), _d(v),
                             _d(s), _d(rphys), _d(rcrit), _d(rce), _i(stat), ids,
                             _d(ngf), C.byref(C.c_int(algor)), _i(opt), C.byref(dtflag),
-                            C.byref(C.c_int(0)), C.byref(C.c_int(-1)), C.byref(colflag),
+                            C.byref(C.c_int(ngflag)), C.byref(C.c_int(-1)), C.byref(colflag),
                             C.byref(nclo), _i(iclo), _i(jclo), _d(dclo), _d(tclo), _d(ixv), _d(jxv),
                             outfile, mem, _i(lmem),
                             C.c_size_t(25), C.c_size_t(80), C.c_size_t(80))

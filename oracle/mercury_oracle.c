@@ -14,6 +14,14 @@
 #include <stdlib.h>
 #include <string.h>
 
+/* Gaussian gravitational constant squared (mercury.inc:37) */
+#define K2 2.959122082855911e-4
+
+/* the non-gravitational model (orc_set_ngf) */
+static int g_ngflag = 0;
+static const double *g_ngf = NULL;
+static double *g_angf = NULL;
+
 /* swift.inc:22-31, mercury.inc:37-44 */
 #define DANBYB 1.0e-13
 #define NLAG2 400
@@ -757,7 +765,8 @@ int orc_mce_cent(double time, double h, double rcen, int i0, int nbod,
 }
 
 /* ------------------------------------------------------------------------ */
-/* mdt_mvs  mercury6_2.for:3987-4097 (no user or non-gravitational forces) */
+/* mdt_mvs  mercury6_2.for:3987-4097 (no user force; cometary forces from the
+ * orc_set_ngf model) */
 void orc_mdt_mvs(double time, double h0, double rcen, const double *jcen,
                  int nbod, int nbig, const double *m, double *x, double *v,
                  double *a, int *dtflag, const double *rce, const double *rphys,
@@ -772,10 +781,15 @@ void orc_mdt_mvs(double time, double h0, double rcen, const double *jcen,
         ev->nclo = 0;
         ev->overflow = 0;
     }
+    /* mdt_mvs' cometary accelerations angf (zero without the model) */
+    double *angf = (g_angf && (g_ngflag == 1 || g_ngflag == 3)) ? g_angf : NULL;
+    double *zero = angf ? NULL : calloc(n3, sizeof(double));
+    double *ang = angf ? angf : zero;
     if (*dtflag != 2) {
         *dtflag = 2;
         orc_mco_h2j(nbig, m, x, v, xj, vj);
         orc_mfo_mvs(jcen, nbod, nbig, m, x, xj, a);
+        if (angf) orc_mfo_ngf(nbod, x, v, angf, g_ngf);
     }
     double minside = m[0];
     for (int j = 1; j < nbig; j++) {
@@ -784,9 +798,9 @@ void orc_mdt_mvs(double time, double h0, double rcen, const double *jcen,
         minside = msofar;
     }
     for (int j = 1; j < nbod; j++) {
-        v[3 * j + 0] = v[3 * j + 0] + hby2 * (0.0 + 0.0 + a[3 * j + 0]);
-        v[3 * j + 1] = v[3 * j + 1] + hby2 * (0.0 + 0.0 + a[3 * j + 1]);
-        v[3 * j + 2] = v[3 * j + 2] + hby2 * (0.0 + 0.0 + a[3 * j + 2]);
+        v[3 * j + 0] = v[3 * j + 0] + hby2 * (ang[3 * j + 0] + 0.0 + a[3 * j + 0]);
+        v[3 * j + 1] = v[3 * j + 1] + hby2 * (ang[3 * j + 1] + 0.0 + a[3 * j + 1]);
+        v[3 * j + 2] = v[3 * j + 2] + hby2 * (ang[3 * j + 2] + 0.0 + a[3 * j + 2]);
     }
     memcpy(x0, x, n3 * sizeof(double));
     memcpy(v0, v, n3 * sizeof(double));
@@ -805,12 +819,13 @@ void orc_mdt_mvs(double time, double h0, double rcen, const double *jcen,
     if (ev)
         orc_mce_stat(time + h0, h0, rcen, nbod, nbig, m, x0, v0, x, v, rce, rphys, ev);
     orc_mfo_mvs(jcen, nbod, nbig, m, x, xj, a);
+    if (angf) orc_mfo_ngf(nbod, x, v, angf, g_ngf);
     for (int j = 1; j < nbod; j++) {
-        v[3 * j + 0] = v[3 * j + 0] + hby2 * (0.0 + 0.0 + a[3 * j + 0]);
-        v[3 * j + 1] = v[3 * j + 1] + hby2 * (0.0 + 0.0 + a[3 * j + 1]);
-        v[3 * j + 2] = v[3 * j + 2] + hby2 * (0.0 + 0.0 + a[3 * j + 2]);
+        v[3 * j + 0] = v[3 * j + 0] + hby2 * (ang[3 * j + 0] + 0.0 + a[3 * j + 0]);
+        v[3 * j + 1] = v[3 * j + 1] + hby2 * (ang[3 * j + 1] + 0.0 + a[3 * j + 1]);
+        v[3 * j + 2] = v[3 * j + 2] + hby2 * (ang[3 * j + 2] + 0.0 + a[3 * j + 2]);
     }
-    free(xj); free(vj); free(x0); free(v0); free(gm);
+    free(xj); free(vj); free(x0); free(v0); free(gm); free(zero);
 }
 
 /* mco_mvs2h  mercury6_2.for:2107-2219 */
@@ -990,20 +1005,125 @@ static void mfo_grav(int nbod, int nbig, const double *m, const double *x, const
     free(ri);
 }
 
-/* mfo_all  mercury6_2.for:4506-4592 (gravity, 1PN and oblateness) */
-void orc_mfo_all(const double *jcen, int nbod, int nbig, const double *m,
-                 const double *x, const double *v, int pn, double *a)
+/* The non-gravitational force model of the calls that follow (the driver's
+ * ngflag and ngf(5,nbod)): ngflag 1 cometary A1..A3 and Yarkovsky, 2 radiation
+ * pressure and Poynting-Robertson drag, 3 both; 0 none.  It stays set until
+ * changed; orc_set_ngf(0, NULL, 0) clears it.  The MVS step's cometary
+ * accelerations (mdt_mvs' angf) are carried in the model, like mdt_mvs'
+ * saved array. */
+void orc_set_ngf(int ngflag, const double *ngf, int nbod)
+{
+    g_ngflag = ngf ? ngflag : 0;
+    g_ngf = ngf;
+    free(g_angf);
+    g_angf = (ngf && nbod > 0) ? calloc((size_t)3 * nbod, sizeof(double)) : NULL;
+}
+
+/* mfo_ngf  mercury6_2.for:5106-5152: cometary A1/A2/A3 with Marsden's g(r)
+ * and the distance cutoff, and the Yarkovsky transverse term.  Returns 1 for
+ * an undefined transverse or normal direction (the Fortran's fail). */
+int orc_mfo_ngf(int nbod, const double *x, const double *v, double *a, const double *ngf)
+{
+    for (int k = 0; k < 3 * nbod; k++) a[k] = 0.0;
+    for (int j = 1; j < nbod; j++) {
+        const double *p = ngf + 5 * j;
+        const double *xj = x + 3 * j, *vj = v + 3 * j;
+        double *aj = a + 3 * j;
+        if (p[0] == 0.0 && p[1] == 0.0 && p[2] == 0.0 && p[4] == 0.0) continue;
+        double r2 = xj[0] * xj[0] + xj[1] * xj[1] + xj[2] * xj[2];
+        double r = sqrt(r2);
+        double rv = (xj[0] * vj[0] + xj[1] * vj[1] + xj[2] * vj[2]) / r2;
+        double tx = vj[0] - rv * xj[0];
+        double ty = vj[1] - rv * xj[1];
+        double tz = vj[2] - rv * xj[2];
+        double tn = sqrt(tx * tx + ty * ty + tz * tz);
+        if ((p[0] != 0.0 || p[1] != 0.0 || p[2] != 0.0) &&
+            (r2 < 88.0 || fabs(p[0]) > 1e-7 || fabs(p[1]) > 1e-7 || fabs(p[2]) > 1e-7)) {
+            double q = r * .3561253561253561;
+            double g = .111262 * pow(q, -2.15) * pow(1.0 + pow(q, 5.093), -4.6142);
+            for (int c = 0; c < 3; c++) aj[c] = p[0] * g / r * xj[c];
+            if (p[1] != 0.0) {
+                if (tn == 0.0) return 1;
+                double f = p[1] * g / tn;
+                aj[0] = aj[0] + f * tx; aj[1] = aj[1] + f * ty; aj[2] = aj[2] + f * tz;
+            }
+            if (p[2] != 0.0) {
+                double nx = xj[1] * vj[2] - xj[2] * vj[1];
+                double ny = xj[2] * vj[0] - xj[0] * vj[2];
+                double nz = xj[0] * vj[1] - xj[1] * vj[0];
+                double nn = sqrt(nx * nx + ny * ny + nz * nz);
+                if (nn == 0.0) return 1;
+                double f = p[2] * g / nn;
+                aj[0] = aj[0] + f * nx; aj[1] = aj[1] + f * ny; aj[2] = aj[2] + f * nz;
+            }
+        }
+        if (p[4] != 0.0) {
+            if (tn == 0.0) return 1;
+            double f = p[4] / (r2 * tn);
+            aj[0] = aj[0] + f * tx; aj[1] = aj[1] + f * ty; aj[2] = aj[2] + f * tz;
+        }
+    }
+    return 0;
+}
+
+/* mfo_pr  mercury6_2.for:5307-5357: radiation pressure and Poynting-Robertson
+ * drag (solar-wind factor 0.3) on massless bodies with beta = ngf(4). */
+void orc_mfo_pr(int nbod, const double *m, const double *x, const double *v, double *a, const double *ngf)
+{
+    const double sw = .3;
+    const double c = 299792458.0 * 86400.0 / (1.495978707e13 * .01);
+    for (int j = 1; j < nbod; j++) {
+        const double *xj = x + 3 * j, *vj = v + 3 * j;
+        double *aj = a + 3 * j;
+        double beta = ngf[5 * j + 3];
+        if (m[j] != 0.0 || beta == 0.0) {
+            aj[0] = 0.0; aj[1] = 0.0; aj[2] = 0.0;
+            continue;
+        }
+        double r = sqrt(xj[0] * xj[0] + xj[1] * xj[1] + xj[2] * xj[2]);
+        double vr = (xj[0] * vj[0] + xj[1] * vj[1] + xj[2] * vj[2]) / r;
+        double f = K2 * beta / (r * r);
+        for (int k = 0; k < 3; k++)
+            aj[k] = f * ((1.0 - (1.0 + sw) * vr / c) * xj[k] / r - (1.0 + sw) * vj[k] / c);
+    }
+}
+
+/* mfo_all  mercury6_2.for:4506-4592 (gravity, 1PN, oblateness and the
+ * non-gravitational model of orc_set_ngf).  Returns 1 when mfo_ngf fails. */
+static int mfo_all_impl(const double *jcen, int nbod, int nbig, const double *m,
+                        const double *x, const double *v, int pn, double *a)
 {
     mfo_grav(nbod, nbig, m, x, v, a, pn);
+    double *acor = NULL;
     if (jcen[0] != 0 || jcen[1] != 0 || jcen[2] != 0) {
-        double *acor = calloc((size_t)3 * nbod, sizeof(double));
+        acor = calloc((size_t)3 * nbod, sizeof(double));
         double acen[3];
         orc_mfo_obl(jcen, nbod, m, x, acor, acen);
         for (int j = 1; j < nbod; j++)
             for (int c = 0; c < 3; c++)
                 a[3 * j + c] = a[3 * j + c] + (acor[3 * j + c] - acen[c]);
-        free(acor);
     }
+    int bad = 0;
+    if (g_ngflag == 1 || g_ngflag == 3) {
+        if (!acor) acor = calloc((size_t)3 * nbod, sizeof(double));
+        bad = orc_mfo_ngf(nbod, x, v, acor, g_ngf);
+        for (int j = 1; j < nbod; j++)
+            for (int c = 0; c < 3; c++) a[3 * j + c] = a[3 * j + c] + acor[3 * j + c];
+    }
+    if (g_ngflag == 2 || g_ngflag == 3) {
+        if (!acor) acor = calloc((size_t)3 * nbod, sizeof(double));
+        orc_mfo_pr(nbod, m, x, v, acor, g_ngf);
+        for (int j = 1; j < nbod; j++)
+            for (int c = 0; c < 3; c++) a[3 * j + c] = a[3 * j + c] + acor[3 * j + c];
+    }
+    free(acor);
+    return bad;
+}
+
+void orc_mfo_all(const double *jcen, int nbod, int nbig, const double *m,
+                 const double *x, const double *v, int pn, double *a)
+{
+    (void)mfo_all_impl(jcen, nbod, nbig, m, x, v, pn, a);
 }
 
 /* mdt_ra15 constants (mercury6_2.for:4162-4199) */
@@ -1079,7 +1199,7 @@ long orc_mdt_ra15(double time, double *t, double *tdid, double tol, const double
             gk[5] = bk[6] * d[20] + bk[5];
             gk[6] = bk[6];
         }
-        const int vpred = pn != 0;
+        const int vpred = pn != 0 || g_ngflag != 0;   /* mercury6_2.for:4277 */
         for (int n = 1; n <= niter; n++) {
             for (int j = 2; j <= 8; j++) {
                 double hj = h[j - 1];

@@ -86,8 +86,15 @@ void orc_mco_mvs2h(double h, const double *jcen, int nbod, int nbig,
                    const double *m, const double *x, const double *v,
                    double *xh, double *vh);
 
-/* Heliocentric accelerations of mfo_all without non-gravitational or user
- * forces: mfo_grav (pn = 1 adds the central 1PN term) + mfo_obl. */
+/* The non-gravitational model (ngflag, ngf[5*nbod]) of the calls that follow;
+ * orc_set_ngf(0, NULL, 0) clears it.  ngf stays the caller's. */
+void orc_set_ngf(int ngflag, const double *ngf, int nbod);
+/* mfo_ngf (cometary A1..A3, Yarkovsky) and mfo_pr (radiation pressure, PR
+ * drag) for every body; orc_mfo_ngf returns 1 for an undefined direction. */
+int orc_mfo_ngf(int nbod, const double *x, const double *v, double *a, const double *ngf);
+void orc_mfo_pr(int nbod, const double *m, const double *x, const double *v, double *a, const double *ngf);
+/* Heliocentric accelerations of mfo_all without user forces: mfo_grav (pn = 1
+ * adds the central 1PN term) + mfo_obl + the orc_set_ngf model. */
 void orc_mfo_all(const double *jcen, int nbod, int nbig, const double *m,
                  const double *x, const double *v, int pn, double *a);
 

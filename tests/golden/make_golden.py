@@ -6,7 +6,8 @@ oracle/Makefile).  Run here, where /root/reference exists:
 
 (`radau` rewrites only the mfo_all / mdt_ra15 / mco_mvs2h fixtures, `hybrid`
 only the mdt_hy runs, `bs` only the BS runs, `mce_stat` only mce_stat's
-encounter and near-miss/collision lists.)
+encounter and near-miss/collision lists, `nongrav` only the non-gravitational
+forces and runs.)
 
 Each fixture holds inputs and the reference's outputs only (data, no code).
 """
@@ -120,6 +121,7 @@ def main():
     bs(R)
     hybrid(R)
     mce_stat(R)
+    nongrav(R)
     print("wrote", sorted(p.name for p in HERE.glob("*.npz")))
 
 
@@ -193,6 +195,57 @@ def mce_stat(R=None):
         print(f"mce_stat case {c}: {len(r['iclo'])} encounters, {len(r['ihit'])} near misses/collisions "
               f"({int(np.sum(np.asarray(r['chit']) == 1))} collisions)")
     np.savez_compressed(HERE / "mce_stat.npz", **out)
+
+
+def nongrav(R=None):
+    """mfo_ngf and mfo_pr on random bodies (every parameter, the 88 AU^2 and
+    1e-7 cutoffs, massive bodies skipped by mfo_pr), and whole runs with the
+    non-gravitational model: RA15 and BS with ngflag 3 and 1PN, BS2 and MVS
+    with the cometary terms (ngflag 1).  The MVS run comes last: mdt_mvs
+    keeps its cometary accelerations in a saved array."""
+    R = R or Reference()
+    rng = np.random.default_rng(20261205)
+    out = {}
+    k = 400
+    r = 10 ** rng.uniform(-0.5, 1.6, k)
+    d = rng.normal(size=(k, 3)); d /= np.linalg.norm(d, axis=1)[:, None]
+    x = np.concatenate([np.zeros((1, 3)), d * r[:, None]])
+    v = np.concatenate([np.zeros((1, 3)), rng.normal(size=(k, 3)) * 0.01])
+    v[5] = x[5] * 0.001                                   # radial velocity only (A3 normal undefined)
+    ngf = np.zeros((k + 1, 5))
+    for j in range(1, k + 1):
+        for q in range(5):
+            if rng.random() < 0.45:
+                ngf[j, q] = rng.choice([-1.0, 1.0]) * 10 ** rng.uniform(-10, -6.5)
+        if rng.random() < 0.4:
+            ngf[j, 3] = 10 ** rng.uniform(-2, -0.5)
+    ngf[5] = [1e-8, 0.0, 0.0, 0.0, 0.0]
+    m = np.zeros(k + 1); m[0] = MU
+    m[1:30] = 1e-9 * MU                                   # massive: mfo_pr skips them
+    out["ngf_x"] = x; out["ngf_v"] = v; out["ngf_p"] = ngf; out["ngf_m"] = m
+    out["ngf_a"] = R.mfo_ngf(x, v, ngf)
+    out["pr_a"] = R.mfo_pr(m, 30, x, v, ngf)
+    runs = [("ra15", 4, 3, 1, 60), ("bs", 2, 3, 1, 30), ("bs2", 3, 1, 0, 30), ("mvs", 1, 1, 0, 300)]
+    for name, algor, flag, pn, nsteps in runs:
+        s = systems.config_small(seed=30 + algor, nparticles=60)
+        kinds = ("a1", "a2", "a3", "yar", "beta") if flag == 3 else ("a1", "a2", "a3")
+        systems.add_nongrav(s, seed=40 + algor, fraction=0.6, kinds=kinds, big=(algor != 1))
+        assert s["ngflag"] == flag
+        for key in ("m", "x", "v", "ngf"):
+            out[f"{name}_{key}"] = s[key]
+        out[f"{name}_meta"] = np.array([s["nbig"], nsteps, s["h"], flag, pn], float)
+        if algor == 4:
+            xo, vo, hd, hn = R.ra15_run(s, nsteps, tol=1e-12, pn=pn)
+        elif algor in (2, 3):
+            xo, vo, hd, hn = R.bs_run(algor, s, nsteps, tol=1e-12, pn=pn)
+        else:
+            xo, vo, evi, evd = R.mdt_mvs_run(s, nsteps)
+            hd, hn = np.zeros(0), 0.0
+        out[f"{name}_xout"] = xo; out[f"{name}_vout"] = vo; out[f"{name}_hdid"] = np.asarray(hd)
+        out[f"{name}_hnext"] = np.array([hn])
+        print(f"nongrav run {name}: ngflag {flag}, {int(np.sum(np.any(s['ngf'] != 0, axis=1)))} bodies with "
+              f"parameters, {nsteps} steps")
+    np.savez_compressed(HERE / "nongrav.npz", **out)
 
 
 def radau(R=None):
@@ -277,6 +330,7 @@ def hybrid(R=None):
 if __name__ == "__main__":
     threading.stack_size(512 << 20)
     only = sys.argv[1] if len(sys.argv) > 1 else None
-    t = threading.Thread(target={"radau": radau, "bs": bs, "hybrid": hybrid, "mce_stat": mce_stat}.get(only, main))
+    t = threading.Thread(target={"radau": radau, "bs": bs, "hybrid": hybrid, "mce_stat": mce_stat,
+                                 "nongrav": nongrav}.get(only, main))
     t.start()
     t.join()

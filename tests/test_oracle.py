@@ -691,3 +691,122 @@ def test_fused_multiply_add_breaks_the_parity_bar(oracle):
     assert planets.max() > 1e-11
     assert (particles > 1e-11).sum() > particles.size // 10
 
+
+
+# ------------------------------------------------- non-gravitational forces
+C_AU_DAY = 299792458.0 * 86400.0 / 1.495978707e11   # speed of light in AU/day
+
+
+def nongrav_system(g, name):
+    nbig, nsteps, h, flag, pn = g[f"{name}_meta"]
+    n = len(g[f"{name}_m"])
+    s = dict(m=g[f"{name}_m"], x=g[f"{name}_x"], v=g[f"{name}_v"], ngf=g[f"{name}_ngf"], ngflag=int(flag),
+             nbig=int(nbig), h=h, jcen=np.zeros(3), rce=np.zeros(n), rphys=np.zeros(n), rcen=0.005)
+    return s, int(nsteps), int(pn)
+
+
+def test_golden_mfo_ngf_and_pr(oracle):
+    """mfo_ngf (mercury6_2.for:5106-5152) and mfo_pr (5307-5357) on 400 bodies:
+    every parameter, both cutoff tests, massive bodies skipped by mfo_pr; bit
+    for bit (the same libm pow as the reference build)."""
+    g = np.load(GOLD / "nongrav.npz")
+    a, bad = oracle.mfo_ngf(g["ngf_x"], g["ngf_v"], g["ngf_p"])
+    assert bad == 0
+    assert np.array_equal(a[1:], g["ngf_a"][1:])
+    ap = oracle.mfo_pr(g["ngf_m"], g["ngf_x"], g["ngf_v"], g["ngf_p"])
+    assert np.array_equal(ap[1:], g["pr_a"][1:])
+    assert np.all(ap[1:30] == 0.0) and np.any(ap[30:] != 0.0)
+
+
+@pytest.mark.parametrize("name", ["ra15", "bs", "bs2", "mvs"])
+def test_golden_nongrav_runs(oracle, name):
+    """Whole runs with the non-gravitational model: RA15 with ngflag 3 and 1PN
+    (velocity predictor on), BS with ngflag 3, BS2 and MVS with the cometary
+    and Yarkovsky terms (ngflag 1; MVS on particles only, its kick carrying
+    the saved accelerations); states and step sizes bit for bit."""
+    g = np.load(GOLD / "nongrav.npz")
+    s, nsteps, pn = nongrav_system(g, name)
+    assert np.any(s["ngf"] != 0)
+    if name == "ra15":
+        x, v, hd, hn, _ = oracle.ra15_run(s, nsteps, tol=1e-12, pn=pn)
+    elif name in ("bs", "bs2"):
+        x, v, hd, hn, _, _ = oracle.bs_run(2 if name == "bs" else 3, s, nsteps, tol=1e-12, pn=pn)
+    else:
+        x, v, _, _ = oracle.mvs_run(s, nsteps, screen=False)
+        hd, hn = g[f"{name}_hdid"], g[f"{name}_hnext"][0]
+    assert np.array_equal(hd, g[f"{name}_hdid"]) and hn == g[f"{name}_hnext"][0]
+    assert np.array_equal(x[1:], g[f"{name}_xout"][1:]) and np.array_equal(v[1:], g[f"{name}_vout"][1:])
+
+
+def test_reference_mfo_ngf_random(oracle, reference):
+    rng = np.random.default_rng(77)
+    n = 2000
+    x = rng.normal(size=(n, 3)) * 10 ** rng.uniform(-0.5, 1.5, (n, 1))
+    v = rng.normal(size=(n, 3)) * 0.01
+    p = np.where(rng.random((n, 5)) < 0.5, rng.choice([-1.0, 1.0], (n, 5)) * 10 ** rng.uniform(-11, -6, (n, 5)), 0)
+    m = np.where(rng.random(n) < 0.2, 1e-9, 0.0); m[0] = MU
+    a, bad = oracle.mfo_ngf(x, v, p)
+    assert bad == 0
+    assert np.array_equal(a[1:], reference.mfo_ngf(x, v, p)[1:])
+    assert np.array_equal(oracle.mfo_pr(m, x, v, p)[1:], reference.mfo_pr(m, 1, x, v, p)[1:])
+
+
+def test_reference_ra15_nongrav_run(oracle, reference):
+    s = systems.add_nongrav(systems.config_small(seed=12, nparticles=80), seed=13, big=True)
+    assert s["ngflag"] == 3
+    x, v, hd, hn, _ = oracle.ra15_run(s, 50, tol=1e-12, pn=1)
+    xr, vr, hr, hnr = run_big_stack(reference.ra15_run, s, 50, tol=1e-12, pn=1)
+    assert np.array_equal(hd, hr) and hn == hnr
+    assert np.array_equal(x, xr) and np.array_equal(v, vr)
+
+
+def test_mfo_ngf_geometry(oracle):
+    """What the model fixes, independently of the reference: Marsden et al.'s
+    g(r) (alpha 0.111262, r0 2.808 AU, m 2.15, n 5.093, k 4.6142) is 1 at
+    1 AU to 1e-3; A1 pushes along r, A2 along the in-plane transverse direction,
+    A3 along r x v; the cometary terms switch off beyond sqrt(88) AU unless some
+    |A| exceeds 1e-7; Yarkovsky is yar / r^2 along the transverse direction."""
+    x = np.array([[0, 0, 0], [1.0, 0, 0], [0, 2.0, 0], [12.0, 0, 0], [12.0, 0, 0], [0, 0, 3.0]])
+    v = np.array([[0, 0, 0], [0.003, 0.017, 0.001], [-0.012, 0.001, 0.002], [0, 0.005, 0], [0, 0.005, 0],
+                  [0.01, 0, 0.002]])
+    p = np.zeros((6, 5))
+    p[1, 0] = 2e-8
+    p[2, 1] = 3e-8
+    p[3, 2] = 4e-8                      # beyond the cutoff, small: no force
+    p[4, 2] = 4e-7                      # beyond the cutoff, large: force
+    p[5, 4] = 5e-11
+    a, bad = oracle.mfo_ngf(x, v, p)
+    assert bad == 0
+    assert np.allclose(a[1] / 2e-8, [1.0, 0, 0], atol=1.5e-3) and a[1, 1] == a[1, 2] == 0.0
+    t = v[2] - (x[2] @ v[2]) / (x[2] @ x[2]) * x[2]
+    assert abs(a[2] @ x[2]) < 1e-22 and abs(np.cross(a[2], t)).max() < 1e-22 and a[2] @ t > 0
+    assert np.all(a[3] == 0.0)
+    nrm = np.cross(x[4], v[4])
+    assert abs(np.cross(a[4], nrm)).max() < 1e-24 and a[4] @ nrm > 0
+    t5 = v[5] - (x[5] @ v[5]) / 9.0 * x[5]
+    assert np.isclose(np.linalg.norm(a[5]), 5e-11 / 9.0, rtol=1e-14) and a[5] @ t5 > 0
+    r = np.array([1.0, 2.0, 2.808, 5.0])
+    xs = np.zeros((5, 3)); xs[1:, 0] = r
+    vs = np.zeros((5, 3)); vs[1:, 1] = 0.01
+    ps = np.zeros((5, 5)); ps[1:, 0] = 1e-8
+    g = oracle.mfo_ngf(xs, vs, ps)[0][1:, 0] / 1e-8
+    q = r / 2.808
+    want = 0.111262 * q ** -2.15 * (1 + q ** 5.093) ** -4.6142
+    assert np.allclose(g, want, rtol=1e-13)
+
+
+def test_mfo_pr_closed_form(oracle):
+    """Radiation pressure at rest is beta times the Sun's pull, outward; on a
+    circular orbit the drag is -(1 + 0.3) beta k^2 v / (r^2 c) with c in AU/day
+    (Burns, Lamy and Soter 1979, with a 30% solar-wind share)."""
+    r = 2.0
+    beta = 0.05
+    x = np.array([[0, 0, 0], [r, 0, 0], [0, r, 0]])
+    vc = np.sqrt(MU / r)
+    v = np.array([[0, 0, 0], [0, 0, 0], [-vc, 0, 0]])
+    p = np.zeros((3, 5)); p[1:, 3] = beta
+    a = oracle.mfo_pr(np.array([MU, 0, 0]), x, v, p)
+    assert np.allclose(a[1], [MU * beta / r ** 2, 0, 0], rtol=1e-15)
+    want = MU * beta / r ** 2 * (x[2] / r - 1.3 * v[2] / C_AU_DAY)
+    assert np.allclose(a[2], want, rtol=1e-14)
+    assert np.all(oracle.mfo_pr(np.array([MU, 1e-9, 0]), x, v, p)[1] == 0.0)
