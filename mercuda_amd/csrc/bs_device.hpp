@@ -1,7 +1,8 @@
 // Bulirsch-Stoer integration of a whole system for gfx950, FP64: algorithm 2
 // (mdt_bs1, mercury6_2.for:3227-3426) and algorithm 3 (mdt_bs2,
 // mercury6_2.for:3428-3617), each with mfo_all as the force (mfo_grav with the
-// optional central 1PN term, plus oblateness; mercury6_2.for:4506-4707).
+// optional central 1PN term, plus oblateness and the non-gravitational model;
+// mercury6_2.for:4506-4707, 5106-5357).
 //
 // Massless particles do not act on the Big bodies, so one attempt runs as
 //   bsw_big        one block, a thread per (level, planet): every
@@ -213,9 +214,11 @@ __device__ __forceinline__ void bsw_scales(const double* x0, const double* v0, d
 // of the level forms the indirect term and the oblateness reaction.  When act,
 // the level's table T for this point is written too.  Every thread of the
 // block calls it (block barriers inside).
-template <bool PN>
+// NG: the non-gravitational terms added last (ngf: [5][cap] parameters).
+template <bool PN, bool NG>
 __device__ void bsw_planet_forces(const Sys& d, int np, const double* m, const double* X, const double* V, double* A,
-                                  double* aob, double* hdr, double light2, double* T, int* fault, int k, bool act) {
+                                  double* aob, double* hdr, double light2, double* T, int* fault, int k, bool act,
+                                  const double* ngf, long long cap) {
     if (act && k < np) {
         const double* xb = X + 3 * k;
         double a[3] = {0.0, 0.0, 0.0};
@@ -284,6 +287,11 @@ __device__ void bsw_planet_forces(const Sys& d, int np, const double* m, const d
         }
         if (d.obl)
             for (int c = 0; c < 3; c++) a[c] = a[c] + (aob[3 * k + c] - hdr[3 + c]);
+        if (NG) {
+            double pb[5];
+            for (int q = 0; q < 5; q++) pb[q] = ngf[q * cap + d.base + 1 + k];
+            add_nongrav(d.ngflag, pb, m[k + 1], xb[0], xb[1], xb[2], vb[0], vb[1], vb[2], a[0], a[1], a[2], fault);
+        }
         for (int c = 0; c < 3; c++) T[kHdr + kF * k + c] = xb[c];
         T[kHdr + kF * k + 3] = m[k + 1];
     }
@@ -295,7 +303,7 @@ __device__ void bsw_planet_forces(const Sys& d, int np, const double* m, const d
 // 1..HI first; the host runs the rest (LO > 1) only when none of those levels
 // converged.  A first pass (HI < L) keeps its levels' results in rsave; the
 // second reads them back, so that its extrapolation is that of all L levels.
-template <int ALG, bool PN, int LO, int HI>
+template <int ALG, bool PN, bool NG, int LO, int HI>
 __global__ void __launch_bounds__((HI - LO + 1) * kBsLanes) bsw_big(BswArgs A) {
     constexpr int L = BsShape<ALG>::L;
     constexpr int NL = HI - LO + 1;
@@ -334,7 +342,8 @@ __global__ void __launch_bounds__((HI - LO + 1) * kBsLanes) bsw_big(BswArgs A) {
         }
     __syncthreads();
     // the step start: one evaluation (the first level's lanes), shared by every level
-    bsw_planet_forces<PN>(d, np, m, X[0], V[0], F[0], OB[0], HD[0], A.light2, A.tab + d.tab, A.st.fault, k, n == LO);
+    bsw_planet_forces<PN, NG>(d, np, m, X[0], V[0], F[0], OB[0], HD[0], A.light2, A.tab + d.tab, A.st.fault, k, n == LO,
+                              A.st.ngf, cap);
     if (mem)
         for (int c = 0; c < 3; c++) a0[c] = F[0][3 * k + c];
     __syncthreads();
@@ -349,8 +358,9 @@ __global__ void __launch_bounds__((HI - LO + 1) * kBsLanes) bsw_big(BswArgs A) {
             for (int c = 0; c < 3; c++) { X[n - LO][3 * k + c] = xp[c]; V[n - LO][3 * k + c] = vp[c]; }
         }
         __syncthreads();
-        bsw_planet_forces<PN>(d, np, m, X[n - LO], V[n - LO], F[n - LO], OB[n - LO], HD[n - LO], A.light2,
-                              A.tab + d.tab + (long long)bsw_slot<ALG>(n, act ? r : 1) * d.ts, A.st.fault, k, act);
+        bsw_planet_forces<PN, NG>(d, np, m, X[n - LO], V[n - LO], F[n - LO], OB[n - LO], HD[n - LO], A.light2,
+                                  A.tab + d.tab + (long long)bsw_slot<ALG>(n, act ? r : 1) * d.ts, A.st.fault, k, act,
+                                  A.st.ngf, cap);
         if (act && mem) {
             for (int c = 0; c < 3; c++) a[c] = F[n - LO][3 * k + c];
             if (r < bsw_evals<ALG>(n)) bsw_advance<ALG>(Lv, r, a, x0, v0, a0);
@@ -410,7 +420,7 @@ template <int LO, int HI> __host__ __device__ constexpr int bsw_threads() {
     return (bsw_paired<LO, HI>() ? (HI - LO + 1) / 2 : HI - LO + 1) * kBswSlots;
 }
 
-template <int ALG, bool PN, int LO, int HI>
+template <int ALG, bool PN, bool NG, int LO, int HI>
 __global__ void __launch_bounds__((bsw_threads<LO, HI>())) bsw_particles(BswArgs A) {
     constexpr int L = BsShape<ALG>::L;
     constexpr int NS = kBswSlots;
@@ -445,8 +455,12 @@ __global__ void __launch_bounds__((bsw_threads<LO, HI>())) bsw_particles(BswArgs
     if (live) {
         double x0[3], v0[3], a0[3];
         for (int c = 0; c < 3; c++) { x0[c] = A.st.x[c * cap + g]; v0[c] = A.st.v[c * cap + g]; }
+        double pb[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        if (NG) load_ngf(A.st, g, pb);
         grav_small<PN, RangeOps>(d, A.tab + d.tab, np, A.light2, x0[0], x0[1], x0[2], v0[0], v0[1], v0[2], a0[0],
                                  a0[1], a0[2], A.st.fault);
+        if (NG)
+            add_nongrav(d.ngflag, pb, 0.0, x0[0], x0[1], x0[2], v0[0], v0[1], v0[2], a0[0], a0[1], a0[2], A.st.fault);
 #pragma unroll 1
         for (int k = 0; k < (PAIR ? 2 : 1); k++) {
             const int n = k == 0 ? LO + pair : HI - pair;
@@ -459,6 +473,9 @@ __global__ void __launch_bounds__((bsw_threads<LO, HI>())) bsw_particles(BswArgs
                 bsw_point<ALG>(Lv, r, v0, xp, vp);
                 grav_small<PN, RangeOps>(d, A.tab + d.tab + (long long)bsw_slot<ALG>(n, r) * d.ts, np, A.light2,
                                          xp[0], xp[1], xp[2], vp[0], vp[1], vp[2], a[0], a[1], a[2], A.st.fault);
+                if (NG)
+                    add_nongrav(d.ngflag, pb, 0.0, xp[0], xp[1], xp[2], vp[0], vp[1], vp[2], a[0], a[1], a[2],
+                                A.st.fault);
                 if (r < ne) bsw_advance<ALG>(Lv, r, a, x0, v0, a0);
             }
             double Rr[6];

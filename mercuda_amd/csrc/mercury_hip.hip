@@ -100,6 +100,17 @@ struct Context {
     int* dcnt = nullptr;             // control words (kCtl*)
     unsigned char* dslow = nullptr;  // slow-set flag per body
     int park_cap = 0;
+    // non-gravitational model (ngflag != 0): parameters [5][ngf_cap] (A1, A2, A3, beta, yar)
+    double* dngf = nullptr;
+    long long ngf_cap = 0;
+    // MVS with the cometary model: the particles with cometary or Yarkovsky
+    // parameters (global indices), permanent slow-set members, and their
+    // accelerations angf [3][angf_cap] carried between steps
+    std::vector<int> comets;
+    Park* dseed = nullptr;
+    int seed_cap = 0;
+    double* dangf = nullptr;
+    long long angf_cap = 0;
     int* hstatus = nullptr;          // pinned host words the call's last kernel writes (call_end)
     int* dstatus = nullptr;          // their device address
     unsigned upload_gen = 0;         // bumped by every upload (captured graphs depend on it)
@@ -262,6 +273,8 @@ void release_all() {
     dfree(c.dpark); dfree(c.dleave); dfree(c.dcnt); dfree(c.dslow); dfree(c.work); dfree(c.dreq); c.req_cap = 0;
     dfree(c.rb); dfree(c.re); dfree(c.ra1); dfree(c.bmax); c.ra_cap = 0;
     dfree(c.bscand); dfree(c.bsrsave); dfree(c.bserr); c.bs_cap = 0;
+    dfree(c.dngf); c.ngf_cap = 0; dfree(c.dangf); c.angf_cap = 0; dfree(c.dseed); c.seed_cap = 0;
+    c.comets.clear(); c.ngflag = 0;
     for (double** p : {&c.tape.tab, &c.tape.ck, &c.tape.rec, &c.tape.lam, &c.tape.part, &c.tape.src, &c.tape.grad,
                        &c.tape.rows, &c.tape.tan, &c.tape.fin, &c.tape.seed, &c.tape.psave})
         dfree(*p);
@@ -283,6 +296,8 @@ State device_state() {
     State s;
     s.x = ctx.x; s.v = ctx.v; s.a = ctx.a; s.m = ctx.m; s.rce = ctx.rce; s.rphys = ctx.rphys;
     s.ox = ctx.ox; s.ov = ctx.ov; s.v0k = ctx.v0k; s.v1 = ctx.v1; s.fault = ctx.fault; s.cap = ctx.cap;
+    s.ngf = (ctx.ngflag && !ctx.ensemble) ? ctx.dngf : nullptr;
+    s.angf = ctx.dangf;
     return s;
 }
 
@@ -315,11 +330,33 @@ void ensure_capacity(int n) {
     ctx.cap = cap;
 }
 
+__global__ void seed_slow(const Park* seeds, int n, unsigned char* slow) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) slow[seeds[i].j] = 5;
+}
+
 // Empties the slow set (a new state, or one put back): every particle starts
-// the next call in the fast kernel.
+// the next call in the fast kernel, except the comets of the cometary model,
+// which are permanent members: they enter the list the next call's
+// call_begin makes current (list 1 after the cleared control words' list 0)
+// as kept members.
 void clear_slow_set() {
     if (ctx.dslow) HIPCHECK(hipMemset(ctx.dslow, 0, (size_t)ctx.cap));
     if (ctx.dcnt) HIPCHECK(hipMemset(ctx.dcnt, 0, sizeof(int) * kCtlInts));
+    const int nc = (int)ctx.comets.size();
+    if (nc == 0 || ctx.ensemble || !ctx.dpark) return;
+    if (nc > ctx.park_cap) throw std::runtime_error("more comets than the slow-set capacity");
+    std::vector<Park> seeds(nc);
+    for (int i = 0; i < nc; i++) seeds[i] = Park{ctx.comets[i], 0, 0, -1};
+    if (ctx.seed_cap < nc) {
+        dalloc(ctx.dseed, (size_t)nc);
+        ctx.seed_cap = nc;
+    }
+    HIPCHECK(hipMemcpy(ctx.dseed, seeds.data(), sizeof(Park) * nc, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(ctx.dpark + ctx.park_cap, ctx.dseed, sizeof(Park) * nc, hipMemcpyDeviceToDevice));
+    HIPCHECK(hipMemcpy(ctx.dcnt + kCtlCount + 1, &nc, sizeof(int), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(seed_slow, dim3(std::min((nc + 255) / 256, 1024)), dim3(256), 0, 0, ctx.dseed, nc, ctx.dslow);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipDeviceSynchronize());
 }
 
 void ensure_table(long long doubles) {
@@ -419,6 +456,7 @@ void build_systems(int smax, bool use_call, double h_call, double rcen_call) {
         d.rcen = use_call ? rcen_call : hs.rcen;
         d.j2 = hs.j2; d.j4 = hs.j4; d.j6 = hs.j6;
         d.obl = (hs.j2 != 0 || hs.j4 != 0 || hs.j6 != 0);
+        d.ngflag = c.ensemble ? 0 : c.ngflag;
         ds[s] = d;
     }
     for (int s = 0; s < nsys; s++) {
@@ -459,7 +497,12 @@ void launch_fast(bool screen, bool save, int, dim3 grid, dim3 block, hipStream_t
 }
 void launch_general(bool screen, bool save, bool uniform, dim3 grid, dim3 block, hipStream_t st,
                     const ParticleArgs& args) {
-    if (uniform) {   // one system: wave-uniform table reads
+    if (uniform && !ctx.comets.empty()) {   // one system with the cometary model's members
+        if (screen && save) hipLaunchKernelGGL((mvs_particles_general<true, true, true, true>), grid, block, 0, st, args);
+        else if (screen) hipLaunchKernelGGL((mvs_particles_general<true, false, true, true>), grid, block, 0, st, args);
+        else if (save) hipLaunchKernelGGL((mvs_particles_general<false, true, true, true>), grid, block, 0, st, args);
+        else hipLaunchKernelGGL((mvs_particles_general<false, false, true, true>), grid, block, 0, st, args);
+    } else if (uniform) {   // one system: wave-uniform table reads
         if (screen && save) hipLaunchKernelGGL((mvs_particles_general<true, true, true>), grid, block, 0, st, args);
         else if (screen) hipLaunchKernelGGL((mvs_particles_general<true, false, true>), grid, block, 0, st, args);
         else if (save) hipLaunchKernelGGL((mvs_particles_general<false, true, true>), grid, block, 0, st, args);
@@ -757,13 +800,18 @@ double radau_pass(double t, int niter, bool reset) {
     RaArgs args{c.dsys, st, RaState{c.rb, c.re, c.ra1}, c.tab, c.bmax, K, t, niter, reset ? 1 : 0, c.pn,
                 light_speed2()};
     HIPCHECK(hipMemsetAsync(c.bmax, 0, sizeof(unsigned long long), c.stream));
-    if (c.pn) hipLaunchKernelGGL(ra_planets<true>, dim3(1), dim3(64), 0, c.stream, args);
-    else hipLaunchKernelGGL(ra_planets<false>, dim3(1), dim3(64), 0, c.stream, args);
+    const bool ng = c.ngflag != 0;
+    if (c.pn && ng) hipLaunchKernelGGL((ra_planets<true, true>), dim3(1), dim3(64), 0, c.stream, args);
+    else if (c.pn) hipLaunchKernelGGL((ra_planets<true, false>), dim3(1), dim3(64), 0, c.stream, args);
+    else if (ng) hipLaunchKernelGGL((ra_planets<false, true>), dim3(1), dim3(64), 0, c.stream, args);
+    else hipLaunchKernelGGL((ra_planets<false, false>), dim3(1), dim3(64), 0, c.stream, args);
     const int npart = c.n - c.nbig;
     if (npart > 0) {
         dim3 grid((npart + 127) / 128);
-        if (c.pn) hipLaunchKernelGGL(ra_particles<true>, grid, dim3(128), 0, c.stream, args);
-        else hipLaunchKernelGGL(ra_particles<false>, grid, dim3(128), 0, c.stream, args);
+        if (c.pn && ng) hipLaunchKernelGGL((ra_particles<true, true>), grid, dim3(128), 0, c.stream, args);
+        else if (c.pn) hipLaunchKernelGGL((ra_particles<true, false>), grid, dim3(128), 0, c.stream, args);
+        else if (ng) hipLaunchKernelGGL((ra_particles<false, true>), grid, dim3(128), 0, c.stream, args);
+        else hipLaunchKernelGGL((ra_particles<false, false>), grid, dim3(128), 0, c.stream, args);
     }
     HIPCHECK(hipGetLastError());
     unsigned long long bits = 0;
@@ -818,20 +866,26 @@ void radau_step(double time, double* h, double* hdid, double tol, int64_t* force
 
 int read_fault();
 void clear_fault();
+void check_ngf_fault(int f);
 
 // The kernels of one pass over the levels LO..HI.
 template <int ALG, int LO, int HI>
 void bs_pass(const BswArgs& A) {
     Context& c = ctx;
-    if (c.pn) hipLaunchKernelGGL((bsw_big<ALG, true, LO, HI>), dim3(1), dim3((HI - LO + 1) * kBsLanes), 0, c.stream, A);
-    else hipLaunchKernelGGL((bsw_big<ALG, false, LO, HI>), dim3(1), dim3((HI - LO + 1) * kBsLanes), 0, c.stream, A);
+    const bool ng = c.ngflag != 0;
+    const dim3 bb((HI - LO + 1) * kBsLanes);
+    if (c.pn && ng) hipLaunchKernelGGL((bsw_big<ALG, true, true, LO, HI>), dim3(1), bb, 0, c.stream, A);
+    else if (c.pn) hipLaunchKernelGGL((bsw_big<ALG, true, false, LO, HI>), dim3(1), bb, 0, c.stream, A);
+    else if (ng) hipLaunchKernelGGL((bsw_big<ALG, false, true, LO, HI>), dim3(1), bb, 0, c.stream, A);
+    else hipLaunchKernelGGL((bsw_big<ALG, false, false, LO, HI>), dim3(1), bb, 0, c.stream, A);
     const int npart = c.n - c.nbig;
     if (npart > 0) {
         dim3 grid((npart + kBswSlots - 1) / kBswSlots);
-        if (c.pn)
-            hipLaunchKernelGGL((bsw_particles<ALG, true, LO, HI>), grid, dim3(bsw_threads<LO, HI>()), 0, c.stream, A);
-        else
-            hipLaunchKernelGGL((bsw_particles<ALG, false, LO, HI>), grid, dim3(bsw_threads<LO, HI>()), 0, c.stream, A);
+        const dim3 pb(bsw_threads<LO, HI>());
+        if (c.pn && ng) hipLaunchKernelGGL((bsw_particles<ALG, true, true, LO, HI>), grid, pb, 0, c.stream, A);
+        else if (c.pn) hipLaunchKernelGGL((bsw_particles<ALG, true, false, LO, HI>), grid, pb, 0, c.stream, A);
+        else if (ng) hipLaunchKernelGGL((bsw_particles<ALG, false, true, LO, HI>), grid, pb, 0, c.stream, A);
+        else hipLaunchKernelGGL((bsw_particles<ALG, false, false, LO, HI>), grid, pb, 0, c.stream, A);
     }
     HIPCHECK(hipGetLastError());
 }
@@ -842,7 +896,9 @@ int bs_first_level(int lo, int hi, double tol2) {
     unsigned long long bits[12];
     HIPCHECK(hipMemcpyAsync(bits, c.bserr, sizeof(bits), hipMemcpyDeviceToHost, c.stream));
     HIPCHECK(hipStreamSynchronize(c.stream));
-    if (read_fault() & FAULT_FORCE) throw std::runtime_error("Nonfinite BS extrapolation");
+    const int f = read_fault();
+    check_ngf_fault(f);
+    if (f & FAULT_FORCE) throw std::runtime_error("Nonfinite BS extrapolation");
     for (int n = std::max(lo, 4); n <= hi; n++) {
         double err;
         std::memcpy(&err, &bits[n - 1], sizeof(err));
@@ -1597,8 +1653,16 @@ int read_fault() {
     return bad;
 }
 
+// mfo_ngf's fail: a comet with A2 or yar and no transverse velocity, or with
+// A3 and no orbit normal
+void check_ngf_fault(int f) {
+    if (f & FAULT_NGF)
+        throw std::runtime_error("non-gravitational force failed: undefined transverse direction or orbit normal");
+}
+
 void check_fault(const char* what) {
     int f = read_fault();
+    check_ngf_fault(f);
     if (f & FAULT_FORCE) throw std::runtime_error(std::string(what) + ": non-finite acceleration");
     if (f & FAULT_REQ_OVERFLOW)
         throw std::runtime_error("screening request buffer overflow: use fewer steps per launch");
@@ -1671,13 +1735,29 @@ int mercury_hip_limits(const double* rce) {
 int mercury_hip_upload(int n, int nbig, int pn, int ngflag, const double* m, const double* x, const double* v,
                        const double* ngf, const double* jcen, const double* rce, const double* rphys) {
     try {
-        (void)ngf;
         if (n < 1 || nbig < 1 || nbig > n) throw std::runtime_error("invalid body counts");
         if (nbig > kMaxBig) throw std::runtime_error("more Big bodies than the HIP path supports (64)");
         if (!(m[0] > 0)) throw std::runtime_error("central mass must be positive");
-        if (ngflag != 0) throw std::runtime_error("non-gravitational forces are not provided by the HIP path");
+        if (ngflag < 0 || ngflag > 3) throw std::runtime_error("invalid non-gravitational flag");
         if (pn != 0 && (mvs_family() || ctx.algorithm == 10))
             throw std::runtime_error("MVS and HYBRID do not take post-Newtonian forces");
+        // the symplectic maps apply only mfo_ngf (ngflag 1 or 3), and only to particles here
+        const bool cometary = (ngflag == 1 || ngflag == 3);
+        auto has_ngf = [&](int j) {
+            return ngf[5 * j] != 0.0 || ngf[5 * j + 1] != 0.0 || ngf[5 * j + 2] != 0.0 || ngf[5 * j + 4] != 0.0;
+        };
+        std::vector<int> comets;
+        if (cometary && (mvs_family() || ctx.algorithm == 10)) {
+            for (int j = 1; j < n; j++) {
+                if (!has_ngf(j)) continue;
+                if (ctx.algorithm == 10)
+                    throw std::runtime_error("HYBRID with non-gravitational forces is not provided by the HIP path");
+                if (j < nbig)
+                    throw std::runtime_error("MVS with non-gravitational forces on Big bodies is not provided by the "
+                                             "HIP path");
+                comets.push_back(j);
+            }
+        }
         if (ctx.enc.active) throw std::runtime_error("upload inside an encounter context");
         for (int j = nbig; j < n; j++)
             if (m[j] != 0) throw std::runtime_error("Small bodies with mass are not provided by the HIP path");
@@ -1691,6 +1771,27 @@ int mercury_hip_upload(int n, int nbig, int pn, int ngflag, const double* m, con
         ctx.ensemble = false;
         ctx.sys_valid = false;
         ctx.upload_gen++;
+        if (ngflag) {
+            // parameters component-major: [q][body], q = A1, A2, A3, beta, yar
+            if (ctx.ngf_cap != ctx.cap) {   // the kernels index it with the body capacity
+                dalloc(ctx.dngf, (size_t)5 * ctx.cap);
+                ctx.ngf_cap = ctx.cap;
+            }
+            std::vector<double> col(n);
+            for (int q = 0; q < 5; q++) {
+                for (int j = 0; j < n; j++) col[j] = ngf[5 * j + q];
+                HIPCHECK(hipMemcpy(ctx.dngf + (size_t)q * ctx.ngf_cap, col.data(), sizeof(double) * n,
+                                   hipMemcpyHostToDevice));
+            }
+        }
+        ctx.comets = comets;
+        if (!comets.empty()) {
+            if (ctx.angf_cap != ctx.cap) {
+                dalloc(ctx.dangf, (size_t)3 * ctx.cap);
+                ctx.angf_cap = ctx.cap;
+            }
+            HIPCHECK(hipMemset(ctx.dangf, 0, sizeof(double) * 3 * ctx.angf_cap));
+        }
         clear_slow_set();
         HIPCHECK(hipMemcpy(ctx.m, m, sizeof(double) * n, hipMemcpyHostToDevice));
         HIPCHECK(hipMemcpy(ctx.rce, rce, sizeof(double) * n, hipMemcpyHostToDevice));
@@ -1740,6 +1841,7 @@ int mercury_hip_step(double time, double* h, double* hdid, double tol, int64_t* 
         ctx.tape.valid = false;
         bool reset = !ctx.history;
         mvs_run(time, *h, 1, reset, false, true, 0.0);   // clears and reports the fault flag itself
+        check_ngf_fault(ctx.hstatus[0]);
         if (ctx.hstatus[0] & FAULT_FORCE) throw std::runtime_error("MVS force failed: non-finite acceleration");
         ctx.history = true;
         ctx.saved_valid = true;
@@ -1852,12 +1954,12 @@ int mercury_hip_force_noalloc(int n, int nbig, int pn, const double* m, const do
                     reinterpret_cast<double*>(scr + kNoAllocTab), nullptr, K, 0.0, 0, 0, pn ? 1 : 0,
                     light_speed2()};
         hipLaunchKernelGGL(noalloc_setup, dim3(1), dim3(64), 0, 0, d, scr, a, ld);
-        if (pn) hipLaunchKernelGGL(ra_planets<true>, dim3(1), dim3(64), 0, 0, args);
-        else hipLaunchKernelGGL(ra_planets<false>, dim3(1), dim3(64), 0, 0, args);
+        if (pn) hipLaunchKernelGGL((ra_planets<true, false>), dim3(1), dim3(64), 0, 0, args);
+        else hipLaunchKernelGGL((ra_planets<false, false>), dim3(1), dim3(64), 0, 0, args);
         if (n > nbig) {
             dim3 grid((n - nbig + 127) / 128);
-            if (pn) hipLaunchKernelGGL(ra_particles<true>, grid, dim3(128), 0, 0, args);
-            else hipLaunchKernelGGL(ra_particles<false>, grid, dim3(128), 0, 0, args);
+            if (pn) hipLaunchKernelGGL((ra_particles<true, false>), grid, dim3(128), 0, 0, args);
+            else hipLaunchKernelGGL((ra_particles<false, false>), grid, dim3(128), 0, 0, args);
         }
         HIPCHECK(hipGetLastError());
         int fault = 0;
@@ -1881,8 +1983,10 @@ int mercury_hip_export(double h, int physical, double* x, double* v) {
                              {h * rt10 * 3.0 / 10.0, h * rt10 / 5.0}, {-h * rt10 / 72.0, h * rt10 / 24.0}};
             hipLaunchKernelGGL(corrector_planets, dim3(1), dim3(64), 0, ctx.stream, ca);
             int npart = ctx.n - ctx.nbig;
-            if (npart > 0)
-                hipLaunchKernelGGL(corrector_particles, dim3((npart + 127) / 128), dim3(128), 0, ctx.stream, ca);
+            if (npart > 0 && ctx.comets.empty())
+                hipLaunchKernelGGL(corrector_particles<false>, dim3((npart + 127) / 128), dim3(128), 0, ctx.stream, ca);
+            else if (npart > 0)
+                hipLaunchKernelGGL(corrector_particles<true>, dim3((npart + 127) / 128), dim3(128), 0, ctx.stream, ca);
             HIPCHECK(hipGetLastError());
             HIPCHECK(hipStreamSynchronize(ctx.stream));
             download_vec3(ctx.cx, x);
@@ -1979,6 +2083,7 @@ int mercury_hip_advance(double time, double h, int nsteps, double rcen, int scre
             mvs_run(time, h, nsteps, reset, screen != 0, false, rcen);
             ctx.history = true;
             const int f = ctx.hstatus[0];
+            check_ngf_fault(f);
             if (f & FAULT_FORCE) throw std::runtime_error("MVS force failed: non-finite acceleration");
             if (f & FAULT_REQ_OVERFLOW)
                 throw std::runtime_error("screening request buffer overflow: use fewer steps per launch");
@@ -2030,6 +2135,7 @@ int mercury_hip_ensemble_upload(int nsys, const int* n, const int* nbig, const d
         ensure_stream();
         ensure_capacity((int)total);
         ctx.n = (int)total; ctx.nbig = 0; ctx.pn = 0; ctx.ngflag = 0;
+        ctx.comets.clear();
         for (int k = 0; k < 3; k++) ctx.jcen[k] = 0.0;
         ctx.systems = sys;
         ctx.ensemble = true;

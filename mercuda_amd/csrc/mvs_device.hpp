@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include "kepler.hpp"
+#include "nongrav.hpp"
 
 namespace mhip {
 
@@ -49,7 +50,7 @@ struct Sys {
     double rcen;       // central-body radius (impact check)
     double j2, j4, j6; // central oblateness terms J_n R^n
     int obl;           // jcen != 0
-    int pad0;
+    int ngflag;        // non-gravitational model: 1 cometary and Yarkovsky, 2 radiation, 3 both
 };
 
 // Body-state arrays: component-major SoA over all bodies of all systems.
@@ -67,7 +68,14 @@ struct State {
     double* v1;    // velocity after the drift
     int* fault;    // set on a non-finite acceleration
     long long cap;
+    const double* ngf = nullptr;   // [5][cap] A1, A2, A3, beta, yar (non-gravitational model only)
+    double* angf = nullptr;        // [3][cap] MVS: cometary accelerations carried between steps
 };
+
+// The five non-gravitational parameters of body g.
+__device__ __forceinline__ void load_ngf(const State& st, int g, double P[5]) {
+    for (int q = 0; q < 5; q++) P[q] = st.ngf[q * st.cap + g];
+}
 
 // Device event record: the reference's 128-byte MercuryEvent (mercury_cuda.h:7-11)
 // followed by the step within the launch and the system index.
@@ -113,7 +121,8 @@ struct ParkList {
     const int* snap;         // current-list entries when this launch's fast kernel finished
     int* leave;              // bodies leaving the slow set after this call
     int* nleave;
-    unsigned char* slow;     // per body: bit 0 member, bit 1 a kept member left the kepmd branch this call
+    unsigned char* slow;     // per body: bit 0 member, bit 1 a kept member left the kepmd branch this call,
+                             // bit 2 a permanent member (cometary parameters)
     int cap;
     int chunk;               // index of this launch within the call
     int per_wave;            // general kernel: entries per 64-lane wave (spreads the latency-bound
@@ -902,6 +911,10 @@ __global__ void __launch_bounds__(64) corrector_planets(CorrectorArgs args) {
     if (lane < 3) { args.xo[lane * cap + b0] = 0.0; args.vo[lane * cap + b0] = 0.0; }
 }
 
+// NG (cometary model): each kick adds mfo_ngf's angf at the kick point, as
+// hb * (angf + ausr + a) with no user force (mercury6_2.for:2171-2176,
+// 2200-2205).
+template <bool NG>
 __global__ void __launch_bounds__(128) corrector_particles(CorrectorArgs args) {
     const Sys d = args.sys[0];
     const int jb = d.nbig + blockIdx.x * blockDim.x + threadIdx.x;
@@ -911,16 +924,33 @@ __global__ void __launch_bounds__(128) corrector_particles(CorrectorArgs args) {
     const int np = d.nbig - 1;
     Orbit o{args.st.x[g], args.st.x[cap + g], args.st.x[2 * cap + g],
             args.st.v[g], args.st.v[cap + g], args.st.v[2 * cap + g]};
+    double P[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if (NG) load_ngf(args.st, g, P);
+    double n0 = 0.0, n1 = 0.0, n2 = 0.0;
     int fp = 0;
     for (int k = 0; k < 2; k++) {
         drift_one(d.mu, o, args.ha[k]);
         double ax, ay, az;
         particle_accel(d, args.tab + (long long)fp * d.ts, np, o.x, o.y, o.z, ax, ay, az);
-        o.vx = o.vx - args.hb[k] * ax; o.vy = o.vy - args.hb[k] * ay; o.vz = o.vz - args.hb[k] * az;
+        if (NG) {
+            ngf_body(P[0], P[1], P[2], P[4], o.x, o.y, o.z, o.vx, o.vy, o.vz, n0, n1, n2, args.st.fault);
+            o.vx = o.vx - args.hb[k] * ((n0 + 0.0) + ax);
+            o.vy = o.vy - args.hb[k] * ((n1 + 0.0) + ay);
+            o.vz = o.vz - args.hb[k] * ((n2 + 0.0) + az);
+        } else {
+            o.vx = o.vx - args.hb[k] * ax; o.vy = o.vy - args.hb[k] * ay; o.vz = o.vz - args.hb[k] * az;
+        }
         fp++;
         drift_one(d.mu, o, -2.0 * args.ha[k]);
         particle_accel(d, args.tab + (long long)fp * d.ts, np, o.x, o.y, o.z, ax, ay, az);
-        o.vx = o.vx + args.hb[k] * ax; o.vy = o.vy + args.hb[k] * ay; o.vz = o.vz + args.hb[k] * az;
+        if (NG) {
+            ngf_body(P[0], P[1], P[2], P[4], o.x, o.y, o.z, o.vx, o.vy, o.vz, n0, n1, n2, args.st.fault);
+            o.vx = o.vx + args.hb[k] * ((n0 + 0.0) + ax);
+            o.vy = o.vy + args.hb[k] * ((n1 + 0.0) + ay);
+            o.vz = o.vz + args.hb[k] * ((n2 + 0.0) + az);
+        } else {
+            o.vx = o.vx + args.hb[k] * ax; o.vy = o.vy + args.hb[k] * ay; o.vz = o.vz + args.hb[k] * az;
+        }
         fp++;
         drift_one(d.mu, o, args.ha[k]);
     }
@@ -1019,20 +1049,29 @@ __device__ __forceinline__ void finish_step_deferred(const Sys& d, const double*
     }
 }
 
-template <bool SCREEN>
+// NG: the kick adds the cometary acceleration ang (mdt_mvs's angf, evaluated
+// after the drift) as hby2 * (angf + ausr + a) with no user force.
+template <bool SCREEN, bool NG = false>
 __device__ __forceinline__ void finish_step(const Sys& d, const double* __restrict__ T, int np, int jb,
                                             double mj, double rce_j, double rphys_j, int s, int sid,
                                             const EventSink& es, int* fault, const double xs[6],
-                                            const Orbit& k0, Orbit& o, double& ax, double& ay, double& az) {
+                                            const Orbit& k0, Orbit& o, double& ax, double& ay, double& az,
+                                            const double* ang = nullptr) {
     if (SCREEN) {
         Ends e{k0.x, k0.y, k0.z, k0.vx, k0.vy, k0.vz, o.x, o.y, o.z, o.vx, o.vy, o.vz};
         screen_body(T, np, jb, d.nbig, mj, e, rce_j, rphys_j, d.h, T[6], s, sid, es);
     }
     particle_accel(d, T, np, o.x, o.y, o.z, ax, ay, az);
     if (!isfinite(ax) || !isfinite(ay) || !isfinite(az)) atomicOr(fault, FAULT_FORCE);
-    o.vx = o.vx + d.hby2 * ax;
-    o.vy = o.vy + d.hby2 * ay;
-    o.vz = o.vz + d.hby2 * az;
+    if (NG) {
+        o.vx = o.vx + d.hby2 * ((ang[0] + 0.0) + ax);
+        o.vy = o.vy + d.hby2 * ((ang[1] + 0.0) + ay);
+        o.vz = o.vz + d.hby2 * ((ang[2] + 0.0) + az);
+    } else {
+        o.vx = o.vx + d.hby2 * ax;
+        o.vy = o.vy + d.hby2 * ay;
+        o.vz = o.vz + d.hby2 * az;
+    }
     if (SCREEN)
         central_check(d.mu, mj, d.rcen, d.h, T[6], xs[0], xs[1], xs[2], xs[3], xs[4], xs[5], o.x, o.y, o.z,
                       o.vx, o.vy, o.vz, jb, s, sid, es);
@@ -1165,7 +1204,13 @@ __global__ void __launch_bounds__(256, MINW) mvs_particles_fast(ParticleArgs arg
 //
 // These few latency-bound waves share SIMDs with the fast kernel's; raised
 // issue priority keeps them from trailing the fast launches they overlap.
-template <bool SCREEN, bool SAVE, bool UNIFORM>
+//
+// NG (cometary model, ngflag 1 or 3): members carry mdt_mvs's angf
+// (mercury6_2.for:4054, 4059-4061, 4090-4095) in st.angf: mfo_ngf at a reset
+// and after every drift, added in both kicks.  Particles with cometary or
+// Yarkovsky parameters are permanent members (slow bit 2), so the fast kernel
+// never sees them.
+template <bool SCREEN, bool SAVE, bool UNIFORM, bool NG = false>
 __global__ void __launch_bounds__(64) mvs_particles_general(ParticleArgs args) {
     __builtin_amdgcn_s_setprio(3);
     const ParkList& P = args.park;
@@ -1200,15 +1245,22 @@ __global__ void __launch_bounds__(64) mvs_particles_general(ParticleArgs args) {
         Orbit o{0, 0, 0, 0, 0, 0};
         double ax = 0, ay = 0, az = 0;
         double mj = 0.0, rce_j = 0.0, rphys_j = 0.0;
+        double pg[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        double ang[3] = {0.0, 0.0, 0.0};
         if (act) {
             o = Orbit{args.st.x[g], args.st.x[cap + g], args.st.x[2 * cap + g],
                       args.st.v[g], args.st.v[cap + g], args.st.v[2 * cap + g]};
+            if (NG) load_ngf(args.st, g, pg);
             if (args.reset && pk.chunk < 0) {
                 // a kept member starts a call that recomputes accelerations
                 particle_accel(d, args.tab + d.start, np, o.x, o.y, o.z, ax, ay, az);
                 if (!isfinite(ax) || !isfinite(ay) || !isfinite(az)) atomicOr(args.st.fault, FAULT_FORCE);
+                if (NG)
+                    ngf_body(pg[0], pg[1], pg[2], pg[4], o.x, o.y, o.z, o.vx, o.vy, o.vz, ang[0], ang[1], ang[2],
+                             args.st.fault);
             } else {
                 ax = args.st.a[g]; ay = args.st.a[cap + g]; az = args.st.a[2 * cap + g];
+                if (NG) { ang[0] = args.st.angf[g]; ang[1] = args.st.angf[cap + g]; ang[2] = args.st.angf[2 * cap + g]; }
             }
             if (SCREEN || SAVE) { mj = args.st.m[g]; rce_j = args.st.rce[g]; rphys_j = args.st.rphys[g]; }
         }
@@ -1227,17 +1279,26 @@ __global__ void __launch_bounds__(64) mvs_particles_general(ParticleArgs args) {
                 if (s < from) continue;
                 const double* T = tab + (long long)s * d.ts;
                 double xs[6] = {o.x, o.y, o.z, o.vx, o.vy, o.vz};
-                o.vx = o.vx + d.hby2 * ax;
-                o.vy = o.vy + d.hby2 * ay;
-                o.vz = o.vz + d.hby2 * az;
+                if (NG) {
+                    o.vx = o.vx + d.hby2 * ((ang[0] + 0.0) + ax);
+                    o.vy = o.vy + d.hby2 * ((ang[1] + 0.0) + ay);
+                    o.vz = o.vz + d.hby2 * ((ang[2] + 0.0) + az);
+                } else {
+                    o.vx = o.vx + d.hby2 * ax;
+                    o.vy = o.vy + d.hby2 * ay;
+                    o.vz = o.vz + d.hby2 * az;
+                }
                 Orbit k0 = o;
                 const double rr = o.x * o.x + o.y * o.y + o.z * o.z;
                 bool kep = false;
                 drift_one(d.mu, o, d.h, &kep);
                 if (!(kep && in_range && rr >= 0x1p-200 && rr <= 0x1p200)) left_kepmd = true;
+                if (NG)
+                    ngf_body(pg[0], pg[1], pg[2], pg[4], o.x, o.y, o.z, o.vx, o.vy, o.vz, ang[0], ang[1], ang[2],
+                             args.st.fault);
                 if (SAVE) save_step(args.st, g, xs, k0, o);
-                finish_step<SCREEN>(d, T, np, jb, mj, rce_j, rphys_j, s + first, UNIFORM ? 0 : pk.sys,
-                                    args.es, args.st.fault, xs, k0, o, ax, ay, az);
+                finish_step<SCREEN, NG>(d, T, np, jb, mj, rce_j, rphys_j, s + first, UNIFORM ? 0 : pk.sys,
+                                        args.es, args.st.fault, xs, k0, o, ax, ay, az, ang);
             }
         };
         if (args.psteps > 0) run(args.ptab, a_from, args.psteps, args.step0 - args.psteps);
@@ -1246,20 +1307,23 @@ __global__ void __launch_bounds__(64) mvs_particles_general(ParticleArgs args) {
             args.st.x[g] = o.x; args.st.x[cap + g] = o.y; args.st.x[2 * cap + g] = o.z;
             args.st.v[g] = o.vx; args.st.v[cap + g] = o.vy; args.st.v[2 * cap + g] = o.vz;
             args.st.a[g] = ax; args.st.a[cap + g] = ay; args.st.a[2 * cap + g] = az;
+            if (NG) { args.st.angf[g] = ang[0]; args.st.angf[cap + g] = ang[1]; args.st.angf[2 * cap + g] = ang[2]; }
             if (args.work)
                 atomicAdd(args.work + 1, (unsigned long long)(args.psteps - a_from + args.steps - b_from));
-            // members parked in this call stay; kept members stay while they leave the kepmd branch
+            // members parked in this call stay; kept members stay while they leave the kepmd branch;
+            // permanent members always stay
             const bool kept = pk.chunk < 0;
+            const unsigned char bits = P.slow[g];
             if (P.last) {
-                if (!kept || left_kepmd || (P.slow[g] & 2)) {
+                if (!kept || left_kepmd || (bits & 6)) {
                     int slot = atomicAdd(P.count2 + (1 - par), 1);
                     if (slot < P.cap) next[slot] = Park{g, 0, pk.sys, -1};
-                    P.slow[g] = 1;
+                    P.slow[g] = (unsigned char)(1 | (bits & 4));
                 } else {
                     P.leave[atomicAdd(P.nleave, 1)] = g;
                 }
             } else if (kept && left_kepmd) {
-                P.slow[g] = 3;
+                P.slow[g] = (unsigned char)(bits | 2);
             }
         }
     }

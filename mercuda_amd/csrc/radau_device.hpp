@@ -1,8 +1,8 @@
 // RA15 (Everhart's RADAU) for massless particles on gfx950, FP64.
 //
 // One RA15 sequence (mdt_ra15, mercury6_2.for:4133-4500) with the force of
-// mfo_all (mfo_grav with the optional central 1PN term, plus oblateness;
-// mercury6_2.for:4506-4707, 5176-5225).  Massless particles do not act on the
+// mfo_all (mfo_grav with the optional central 1PN term, plus oblateness and
+// the non-gravitational model; mercury6_2.for:4506-4707, 5106-5357).  Massless particles do not act on the
 // Big bodies, so the planets' Gauss-Radau predictors at every (iteration,
 // substep) are independent of the particles:
 //   ra_planets   one wave: the planets' whole iteration; per evaluation point
@@ -248,10 +248,13 @@ __device__ __forceinline__ int ra_pair_index(int np, int i, int j) { return i * 
 // With pair terms (np <= kPairPlanets) the separations, s^-3 and 1/|x| come
 // from one parallel pass; each lane then sums its partners in the same order
 // (i < p subtracted, then j > p added) with the same operations.
-template <bool PN>
+//
+// NG: the Big bodies' non-gravitational terms (mfo_ngf, mfo_pr) are added
+// last, as mfo_all adds them; ngf is the [5][cap] parameter array.
+template <bool PN, bool NG = false>
 __device__ void ra_planet_forces(const Sys& d, int np, const double* m, const double* X, const double* V,
                                  double* A, double* hdr, double* aob, double light2, double* T, int* fault,
-                                 const RaPairs* P = nullptr) {
+                                 const RaPairs* P = nullptr, const double* ngf = nullptr, long long cap = 0) {
     const int lane = threadIdx.x;
     if (P) {
         const int npairs = np * (np - 1) / 2;
@@ -369,13 +372,20 @@ __device__ void ra_planet_forces(const Sys& d, int np, const double* m, const do
         }
         if (d.obl)
             for (int c = 0; c < 3; c++) a[c] = a[c] + (aob[3 * lane + c] - hdr[3 + c]);
+        if (NG) {
+            double pb[5];
+            for (int q = 0; q < 5; q++) pb[q] = ngf[q * cap + d.base + 1 + lane];
+            add_nongrav(d.ngflag, pb, m[lane + 1], xb[0], xb[1], xb[2], vb[0], vb[1], vb[2], a[0], a[1], a[2], fault);
+        }
         for (int c = 0; c < 3; c++) T[kHdr + kF * lane + c] = xb[c];
         T[kHdr + kF * lane + 3] = m[lane + 1];
     }
     __syncthreads();
 }
 
-template <bool PN>
+// NG: non-gravitational model on (d.ngflag != 0), which also turns the
+// velocity predictor on (mercury6_2.for:4276-4292).
+template <bool PN, bool NG>
 __global__ void __launch_bounds__(64) ra_planets(RaArgs args) {
     const Sys d = args.sys[0];
     const int np = d.nbig - 1;
@@ -408,7 +418,7 @@ __global__ void __launch_bounds__(64) ra_planets(RaArgs args) {
         }
     }
     __syncthreads();
-    ra_planet_forces<PN>(d, np, m, X, V, A, hdr, aob, args.light2, args.tab, args.st.fault, P);
+    ra_planet_forces<PN, NG>(d, np, m, X, V, A, hdr, aob, args.light2, args.tab, args.st.fault, P, args.st.ngf, cap);
     if (lane < np) {
         for (int c = 0; c < 3; c++) {
             A1[3 * lane + c] = A[3 * lane + c];
@@ -425,14 +435,14 @@ __global__ void __launch_bounds__(64) ra_planets(RaArgs args) {
                 ra_sx(args.t, hj, s);
                 for (int c = 0; c < 3; c++)
                     X[3 * lane + c] = ra_xpred(s, b[c], A1[3 * lane + c], V1[3 * lane + c], X1[3 * lane + c]);
-                if (PN) {
+                if (PN || NG) {
                     ra_sv(args.t, hj, s);
                     for (int c = 0; c < 3; c++) V[3 * lane + c] = ra_vpred(s, b[c], A1[3 * lane + c], V1[3 * lane + c]);
                 }
             }
             __syncthreads();
-            ra_planet_forces<PN>(d, np, m, X, V, A, hdr, aob, args.light2, args.tab + (long long)entry * d.ts,
-                                 args.st.fault, P);
+            ra_planet_forces<PN, NG>(d, np, m, X, V, A, hdr, aob, args.light2, args.tab + (long long)entry * d.ts,
+                                     args.st.fault, P, args.st.ngf, cap);
             if (lane < np)
                 for (int c = 0; c < 3; c++) ra_update(args.k, j, A[3 * lane + c], A1[3 * lane + c], g[c], b[c]);
         }
@@ -452,7 +462,9 @@ __global__ void __launch_bounds__(64) ra_planets(RaArgs args) {
 
 // Uncapped registers (151-153 VGPRs, 3 waves per SIMD): caps for 4 or 5 waves
 // spill b and g and measured 1.2x and 2.4x slower.
-template <bool PN>
+// NG: each particle's non-gravitational terms added after its gravity
+// (particles are massless: upload refuses Small bodies with mass).
+template <bool PN, bool NG>
 __global__ void __launch_bounds__(128) ra_particles(RaArgs args) {
     const Sys d = args.sys[0];
     const int jb = d.nbig + blockIdx.x * blockDim.x + threadIdx.x;
@@ -474,8 +486,13 @@ __global__ void __launch_bounds__(128) ra_particles(RaArgs args) {
                     if (args.reset) args.ra.e[(c * 7 + q) * cap + g_] = 0.0;
                 }
         }
+        double pb[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        if (NG) load_ngf(args.st, g_, pb);
         grav_small<PN, RangeOps>(d, args.tab, np, args.light2, x1[0], x1[1], x1[2], v1[0], v1[1], v1[2], a1[0],
                                  a1[1], a1[2], args.st.fault);
+        if (NG)
+            add_nongrav(d.ngflag, pb, 0.0, x1[0], x1[1], x1[2], v1[0], v1[1], v1[2], a1[0], a1[1], a1[2],
+                        args.st.fault);
         if (iter)
             for (int c = 0; c < 3; c++) ra_g_from_b(args.k, b[c], g[c]);
         int entry = 1;
@@ -486,13 +503,16 @@ __global__ void __launch_bounds__(128) ra_particles(RaArgs args) {
                 double s[9], x[3], v[3] = {0, 0, 0};
                 ra_sx(args.t, hj, s);
                 for (int c = 0; c < 3; c++) x[c] = ra_xpred(s, b[c], a1[c], v1[c], x1[c]);
-                if (PN) {
+                if (PN || NG) {
                     ra_sv(args.t, hj, s);
                     for (int c = 0; c < 3; c++) v[c] = ra_vpred(s, b[c], a1[c], v1[c]);
                 }
                 double a[3];
                 grav_small<PN, RangeOps>(d, args.tab + (long long)entry * d.ts, np, args.light2, x[0], x[1], x[2],
                                          v[0], v[1], v[2], a[0], a[1], a[2], args.st.fault);
+                if (NG)
+                    add_nongrav(d.ngflag, pb, 0.0, x[0], x[1], x[2], v[0], v[1], v[2], a[0], a[1], a[2],
+                                args.st.fault);
                 for (int c = 0; c < 3; c++) ra_update(args.k, j, a[c], a1[c], g[c], b[c]);
             }
         }

@@ -138,6 +138,11 @@ class MercuryHip:
         self.n, self.nbig = n, nbig
 
     def upload_system(self, sysd, **kw):
+        """Upload a system dict (mercuda_amd.systems), its non-gravitational
+        model (sysd["ngflag"], sysd["ngf"][n, 5]) included when present."""
+        if sysd.get("ngflag", 0):
+            kw.setdefault("ngflag", int(sysd["ngflag"]))
+            kw.setdefault("ngf", sysd["ngf"])
         self.upload(sysd["m"], sysd["x"], sysd["v"], sysd["nbig"], jcen=sysd.get("jcen", (0, 0, 0)),
                     rce=sysd["rce"], rphys=sysd["rphys"], **kw)
 

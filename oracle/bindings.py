@@ -336,8 +336,9 @@ class Oracle:
         jc = np.ascontiguousarray(sysd.get("jcen", (0.0, 0.0, 0.0)), np.float64)
         x = np.ascontiguousarray(x, np.float64); v = np.ascontiguousarray(v, np.float64)
         xh = np.zeros_like(x); vh = np.zeros_like(v)
-        self.L.orc_mco_mvs2h(sysd["h"], _d(jc), len(m), sysd["nbig"], _d(m), _d(x), _d(v),
-                             _d(xh), _d(vh))
+        with self.ngf_model(sysd):
+            self.L.orc_mco_mvs2h(sysd["h"], _d(jc), len(m), sysd["nbig"], _d(m), _d(x), _d(v),
+                                 _d(xh), _d(vh))
         return xh, vh
 
 
@@ -527,10 +528,11 @@ class Reference:
         jc = np.ascontiguousarray(sysd.get("jcen", (0.0, 0.0, 0.0)), np.float64)
         x = np.ascontiguousarray(x, np.float64); v = np.ascontiguousarray(v, np.float64)
         xh = np.zeros_like(x); vh = np.zeros_like(v)
-        ngf = np.zeros((n, 5)); opt = np.zeros(8, np.int32)
+        ngf = np.ascontiguousarray(sysd.get("ngf", np.zeros((n, 5))), np.float64)
+        opt = np.zeros(8, np.int32)
         self.L.mco_mvs2h_(self._r(0.0), _d(jc), C.byref(C.c_int(n)), C.byref(C.c_int(sysd["nbig"])),
                           self._r(sysd["h"]), _d(m), _d(x), _d(v), _d(xh), _d(vh), _d(ngf),
-                          C.byref(C.c_int(0)), _i(opt))
+                          C.byref(C.c_int(int(sysd.get("ngflag", 0)))), _i(opt))
         return xh, vh
 
     def mce_cent(self, time, h, rcen, m, x0, v0, x1, v1, i0=2):

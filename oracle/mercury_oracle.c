@@ -828,7 +828,8 @@ void orc_mdt_mvs(double time, double h0, double rcen, const double *jcen,
     free(xj); free(vj); free(x0); free(v0); free(gm); free(zero);
 }
 
-/* mco_mvs2h  mercury6_2.for:2107-2219 */
+/* mco_mvs2h  mercury6_2.for:2107-2219 (with mfo_ngf's angf in the kicks when
+ * the cometary model is on, 2171-2172 and 2200-2201) */
 void orc_mco_mvs2h(double h, const double *jcen, int nbod, int nbig,
                    const double *m, const double *x, const double *v,
                    double *xh, double *vh)
@@ -836,6 +837,7 @@ void orc_mco_mvs2h(double h, const double *jcen, int nbod, int nbig,
     size_t n3 = (size_t)3 * nbod;
     double *xj = calloc(n3, sizeof(double)), *vj = calloc(n3, sizeof(double));
     double *a = calloc(n3, sizeof(double)), *gm = calloc((size_t)nbod, sizeof(double));
+    double *ang = calloc(n3, sizeof(double));   /* angf: zero unless the cometary model is on */
     double rt10 = sqrt(10.0);
     double ha[2] = {h * rt10 * 3.0 / 10.0, h * rt10 / 5.0};
     double hb[2] = {-h * rt10 / 72.0, h * rt10 / 24.0};
@@ -865,18 +867,19 @@ void orc_mco_mvs2h(double h, const double *jcen, int nbod, int nbig,
             orc_mco_j2h(nbig, m, xj, vj, xh, vh);
             if (phase == 2) break;
             orc_mfo_mvs(jcen, nbod, nbig, m, xh, xj, a);
+            if (g_ngflag == 1 || g_ngflag == 3) orc_mfo_ngf(nbod, xh, vh, ang, g_ngf);
             /* vh - hb*(...) for the first kick, vh + hb*(...) for the second */
             for (int j = 1; j < nbod; j++)
                 for (int c = 0; c < 3; c++) {
                     if (phase == 0)
-                        vh[3 * j + c] = vh[3 * j + c] - hb[k] * (0.0 + 0.0 + a[3 * j + c]);
+                        vh[3 * j + c] = vh[3 * j + c] - hb[k] * (ang[3 * j + c] + 0.0 + a[3 * j + c]);
                     else
-                        vh[3 * j + c] = vh[3 * j + c] + hb[k] * (0.0 + 0.0 + a[3 * j + c]);
+                        vh[3 * j + c] = vh[3 * j + c] + hb[k] * (ang[3 * j + c] + 0.0 + a[3 * j + c]);
                 }
             (void)kick;
         }
     }
-    free(xj); free(vj); free(a); free(gm);
+    free(xj); free(vj); free(a); free(gm); free(ang);
 }
 
 /* ------------------------------------------------------------------------ */

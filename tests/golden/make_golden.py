@@ -241,6 +241,7 @@ def nongrav(R=None):
         else:
             xo, vo, evi, evd = R.mdt_mvs_run(s, nsteps)
             hd, hn = np.zeros(0), 0.0
+            out["mvs_xh"], out["mvs_vh"] = R.mvs2h(s, xo, vo)   # the corrector with angf in its kicks
         out[f"{name}_xout"] = xo; out[f"{name}_vout"] = vo; out[f"{name}_hdid"] = np.asarray(hd)
         out[f"{name}_hnext"] = np.array([hn])
         print(f"nongrav run {name}: ngflag {flag}, {int(np.sum(np.any(s['ngf'] != 0, axis=1)))} bodies with "

@@ -57,6 +57,10 @@ def write_run(path, sysd, *, algorithm="MVS", stop=None, nsteps=100, interval=No
             head = f"{name} m={mass:.17e}"
             if big:
                 head += f" r={rceh:.17e}"
+            if "ngf" in sysd:   # non-gravitational parameters as the driver reads them (a1 a2 a3 b yar)
+                for key, q in (("a1", 0), ("a2", 1), ("a3", 2), ("b", 3), ("yar", 4)):
+                    if sysd["ngf"][j, q] != 0.0:
+                        head += f" {key}={sysd['ngf'][j, q]:.17e}"
             lines.append(head)
             lines.append(" ".join(f"{q:.17e}" for q in list(sysd["x"][j]) + list(sysd["v"][j]) + [0.0] * 3))
         (path / fname).write_text("\n".join(lines) + "\n")

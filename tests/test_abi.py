@@ -77,6 +77,33 @@ def test_host_side_errors_without_device(lib, capfd):
     assert "scratch holds (64)" in err and "leading dimension" in err
 
 
+def test_nongrav_upload_refusals_without_device(lib, capfd):
+    """The non-gravitational model's limits, refused before any device call:
+    an invalid flag; HYBRID with cometary or Yarkovsky parameters; MVS with
+    them on a Big body (the symplectic maps take them on particles only)."""
+    import ctypes as C
+    n, nbig = 4, 2
+    m = np.array([2.959122082855911e-4, 1e-9, 0.0, 0.0])
+    x = np.array([[0, 0, 0], [1.0, 0, 0], [2.0, 0, 0], [3.0, 0, 0]])
+    v = np.array([[0, 0, 0], [0, 0.017, 0], [0, 0.012, 0], [0, 0.01, 0]])
+    z3 = np.zeros(3); zn = np.zeros(n)
+    dp = lambda a: np.ascontiguousarray(a, np.float64).ctypes.data_as(C.POINTER(C.c_double))
+    def upload(flag, ngf):
+        return lib.mercury_hip_upload(n, nbig, 0, flag, dp(m), dp(x), dp(v), dp(ngf), dp(z3), dp(zn), dp(zn))
+    on_particle = np.zeros((n, 5)); on_particle[2, 0] = 1e-8
+    on_big = np.zeros((n, 5)); on_big[1, 4] = 1e-11
+    assert lib.mercury_hip_configure(4) == 0
+    assert upload(4, on_particle) != 0
+    assert lib.mercury_hip_configure(10) == 0
+    assert upload(1, on_particle) != 0
+    assert lib.mercury_hip_configure(1) == 0
+    assert upload(3, on_big) != 0
+    err = capfd.readouterr().err
+    assert "invalid non-gravitational flag" in err
+    assert "HYBRID with non-gravitational forces" in err
+    assert "non-gravitational forces on Big bodies" in err
+
+
 def test_no_device_means_not_available(lib):
     """On a machine without a GPU the library says so instead of computing."""
     if Path("/dev/kfd").exists():

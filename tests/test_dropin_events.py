@@ -81,8 +81,13 @@ def test_dropin_central_impact_and_ejection(tmp_path, algorithm):
     infall["x"] = [0.006, 0.0, 0.0]; infall["v"] = [-0.1, 0.0, 0.0]
     escaping = R.body("ESCAPE")
     escaping["x"] = [99.9, 0.0, 0.0]; escaping["v"] = [10.0, 0.0, 0.0]
+    # the survivor carries cometary A2, Yarkovsky and radiation parameters
+    # where the driver allows them (tests/test_events.py:92-98)
+    params = {"a2": 2e-11, "yar": 3e-12, "b": 1e-4} if algorithm in ("BS", "RADAU") else {}
     out, ev = run_both(tmp_path, f"{algorithm} central impact and ejection", ["IMPACT", "ESCAPE"],
-                       algorithm=algorithm, small=[infall, escaping, R.body("KEEP")], stop=0.3, step=0.001,
-                       interval=0.05, periodic=10)
+                       algorithm=algorithm, small=[infall, escaping, R.body("KEEP", **params)], stop=0.3,
+                       step=0.001, interval=0.05, periodic=10)
     assert set(out["small.dmp"]) == {"KEEP"}
+    for key, value in params.items():
+        assert out["small.dmp"]["KEEP"]["params"][key] == value
     assert any("IMPACT" in ln for ln in ev) and any("ESCAPE" in ln for ln in ev)

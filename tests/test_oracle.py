@@ -722,8 +722,9 @@ def test_golden_mfo_ngf_and_pr(oracle):
 def test_golden_nongrav_runs(oracle, name):
     """Whole runs with the non-gravitational model: RA15 with ngflag 3 and 1PN
     (velocity predictor on), BS with ngflag 3, BS2 and MVS with the cometary
-    and Yarkovsky terms (ngflag 1; MVS on particles only, its kick carrying
-    the saved accelerations); states and step sizes bit for bit."""
+    terms (ngflag 1; MVS on particles only, its kick carrying the saved
+    accelerations, and the MVS corrector mco_mvs2h with them); states and
+    step sizes bit for bit."""
     g = np.load(GOLD / "nongrav.npz")
     s, nsteps, pn = nongrav_system(g, name)
     assert np.any(s["ngf"] != 0)
@@ -734,6 +735,10 @@ def test_golden_nongrav_runs(oracle, name):
     else:
         x, v, _, _ = oracle.mvs_run(s, nsteps, screen=False)
         hd, hn = g[f"{name}_hdid"], g[f"{name}_hnext"][0]
+        xh, vh = oracle.mvs2h(s, x, v)
+        assert np.array_equal(xh[1:], g["mvs_xh"][1:]) and np.array_equal(vh[1:], g["mvs_vh"][1:])
+        s0 = dict(s, ngflag=0)
+        assert not np.array_equal(oracle.mvs2h(s0, x, v)[1][1:], vh[1:])
     assert np.array_equal(hd, g[f"{name}_hdid"]) and hn == g[f"{name}_hnext"][0]
     assert np.array_equal(x[1:], g[f"{name}_xout"][1:]) and np.array_equal(v[1:], g[f"{name}_vout"][1:])
 
