@@ -274,6 +274,82 @@ def radau_rate(hip, nparticles=1_000_000, nseq=12):
             "ms_per_sequence": round(sec * 1e3 / nseq, 3), "mean_step_days": round(days / nseq, 4)}
 
 
+PUBLISHED_EXTRA_FORCES = {"cuda_rtx4070_s": 6.04, "mercuda_cpu_1thread_s": 33.4, "mercury6_matched_s": 90.9,
+                          "source": "reference docs/benchmarks.md:36-44 (whole program: startup, integration, "
+                                    "final output)"}
+
+
+def radau_extra_forces(hip, nparticles=100_000, days=365.0, tol=1e-11):
+    """The reference's published RADAU case with extra forces: solar 1PN,
+    radiation pressure and PR drag (beta 1e-4) and Yarkovsky (1e-12 AU/day^2)
+    on 8 planets + 100k particles for 365 days, tol 1e-11, first step 1 d
+    (systems.config_extra_forces).  Wall time of upload, the mercury_hip_step
+    loop to day 365 (the last size clipped to it) and the final download; the
+    reference's figure also counts its program start and file output."""
+    from mercuda_amd import systems
+    s = systems.config_extra_forces(nparticles=nparticles)
+    hip.configure(4)
+    hip.upload_system(dict(s, x=s["x"][:s["nbig"] + 1], v=s["v"][:s["nbig"] + 1], m=s["m"][:s["nbig"] + 1],
+                           rce=s["rce"][:s["nbig"] + 1], rphys=s["rphys"][:s["nbig"] + 1],
+                           ngf=s["ngf"][:s["nbig"] + 1]), pn=1)
+    hip.step(0.0, 1.0, tol)                      # warm: kernels loaded, arrays sized
+    tic = time.perf_counter()
+    hip.upload_system(s, pn=1)
+    h, t, nseq, forces = s["h"], 0.0, 0, 0
+    while t < days:
+        hn, hd, nf, _ = hip.step(t, min(h, days - t), tol)
+        t += hd
+        nseq += 1
+        forces += nf
+        h = hn
+    hip.download()
+    sec = time.perf_counter() - tic
+    return {"workload": f"RADAU tol {tol:g} with 1PN + radiation pressure/PR drag (beta 1e-4) + Yarkovsky "
+                        f"(1e-12 AU/day^2), 8 planets + {nparticles} particles, {days:g} d, first step 1 d",
+            "seconds": round(sec, 3), "sequences": nseq, "force_evaluations": forces,
+            "particle_steps_per_s": round(nparticles * nseq / sec, 1),
+            "reference_published": PUBLISHED_EXTRA_FORCES,
+            "speedup_vs_reference_cuda": round(PUBLISHED_EXTRA_FORCES["cuda_rtx4070_s"] / sec, 2)}
+
+
+def _cpu_ra15_days_worker(job):
+    sub, days, tol = job
+
+    def run():
+        from oracle.bindings import Reference
+        R = Reference(nmax=len(sub["m"]) + 16, cmax=len(sub["m"]) + 16)
+        R.ra15_run(sub, 1, pn=1, tol=tol)              # warm
+        t = time.perf_counter()
+        _, _, hd, _ = R.ra15_run(sub, 0, pn=1, tol=tol, until=days)
+        return time.perf_counter() - t, len(hd)
+    return _in_big_thread(run)
+
+
+def cpu_radau_extra_forces(nparticles=100_000, days=365.0, tol=1e-11, nproc=16):
+    """The reference mdt_ra15 with mfo_all (1PN, mfo_pr, mfo_ngf) on the same
+    case split into `nproc` disjoint particle shards, each integrated for the
+    full 365 days in its own process: the whole case's wall time on nproc
+    cores (each shard takes its own step sizes)."""
+    import multiprocessing as mp
+    import numpy as np
+    from mercuda_amd import systems
+    from oracle.bindings import REF_PATH
+    if not REF_PATH.exists():
+        return None
+    s = systems.config_extra_forces(nparticles=nparticles)
+    nb = s["nbig"]
+    shard = nparticles // nproc
+    jobs = []
+    for p in range(nproc):
+        idx = np.concatenate([np.arange(nb), nb + p * shard + np.arange(shard)])
+        sub = {k: s[k][idx] for k in ("m", "x", "v", "rce", "rphys", "ngf")}
+        sub.update(nbig=nb, h=s["h"], rcen=s["rcen"], jcen=np.zeros(3), ngflag=s["ngflag"])
+        jobs.append((sub, days, tol))
+    with mp.get_context("spawn").Pool(nproc) as pool:
+        res = pool.map(_cpu_ra15_days_worker, jobs)
+    return max(r[0] for r in res), shard, [r[1] for r in res]
+
+
 def bs_rate(hip, algor, nparticles=1_000_000, nsteps=8):
     """Algorithm 2 (mdt_bs1) or 3 (mdt_bs2) with mfo_all, tol 1e-12, on the
     MVS workload's 8 planets + 1M particles, first step 4 d: accepted steps x
@@ -400,8 +476,8 @@ def main():
     ap.add_argument("--cpu-procs", type=int, default=16)
     ap.add_argument("--no-extras", action="store_true", help="skip the ensemble/single ratio")
     ap.add_argument("--extras", default="all",
-                    help="comma list of extra workloads to run: ensemble, adjoint, radau, bs2, bs3, hybrid, "
-                         "hybrid_20k (default all)")
+                    help="comma list of extra workloads to run: ensemble, adjoint, radau, radau_extra_forces, "
+                         "bs2, bs3, hybrid, hybrid_20k (default all)")
     ap.add_argument("--autotune", action="store_true",
                     help="let the autotuner pick steps per launch and workgroup size instead of --spl/--block "
                          "(the defaults are its usual choice on MI355X)")
@@ -476,7 +552,7 @@ def main():
                        "planet_ms": round(tm["planet_ms"], 2), "wall_ms": round(sec * 1e3, 2)},
     }
     extras = {}
-    names = ("ensemble", "adjoint", "radau", "bs2", "bs3", "hybrid", "hybrid_20k")
+    names = ("ensemble", "adjoint", "radau", "radau_extra_forces", "bs2", "bs3", "hybrid", "hybrid_20k")
     want = set() if args.no_extras else (set(names) if args.extras == "all" else set(args.extras.split(",")))
     if want - set(names):
         raise SystemExit(f"unknown extras {sorted(want - set(names))}; choose from {names}")
@@ -488,6 +564,9 @@ def main():
     if "radau" in want:
         log("radau: 8 planets + 1M particles")
         extras["radau"] = radau_rate(hip)
+    if "radau_extra_forces" in want:
+        log("radau with 1PN + PR + Yarkovsky: 8 planets + 100k particles, 365 d")
+        extras["radau_extra_forces"] = radau_extra_forces(hip)
     for algor in (2, 3):
         if f"bs{algor}" in want:
             log(f"bs: algorithm {algor}, 8 planets + 1M particles")
@@ -519,6 +598,17 @@ def main():
                     "sample": f"{args.cpu_procs} processes x 62500 particles x 8 RA15 sequences of this system "
                               f"(tol 1e-12, 1PN; each shard takes its own step sizes), slowest process {r[1]:.2f} s",
                     "gpu_over_cpu": round(extras["radau"]["particle_steps_per_s"] / r[0], 1)}
+        if extras.get("radau_extra_forces"):
+            log(f"cpu baseline: reference mdt_ra15 with 1PN + PR + Yarkovsky, 365 d, {args.cpu_procs} processes")
+            r = cpu_radau_extra_forces(nproc=args.cpu_procs)
+            if r:
+                ex = extras["radau_extra_forces"]
+                ex["cpu_baseline"] = {
+                    "value": round(r[0], 3), "unit": "s", "cores": args.cpu_procs, "kind": "reference",
+                    "sample": f"the whole case: {args.cpu_procs} processes x {r[1]} particles, each shard "
+                              f"365 d with its own step sizes ({min(r[2])}-{max(r[2])} sequences); wall time of "
+                              f"the slowest",
+                    "cpu_over_gpu_time": round(r[0] / ex["seconds"], 1)}
         for algor in (2, 3):
             if extras.get(f"bs{algor}"):
                 log(f"cpu baseline: reference algorithm {algor} on {args.cpu_procs} processes")

@@ -217,3 +217,35 @@ def add_nongrav(sysd, seed, fraction=0.5, kinds=("a1", "a2", "a3", "yar", "beta"
     sysd["ngf"] = ngf
     sysd["ngflag"] = (1 if comet else 0) + (2 if rad else 0)
     return sysd
+
+
+def config_extra_forces(seed=1729, nparticles=100_000):
+    """The reference's published extra-force case (docs/benchmarks.md:36-44,
+    75-80): 8 planets + 1e5 massless particles at perihelion with random
+    azimuths, a U[3.2,3.8] AU, e U[0.01,0.05] (coplanar here), each with
+    beta = 1e-4 (radiation pressure and PR drag) and a Yarkovsky coefficient
+    of 1e-12 AU/day^2; first step 1 d.  The planet set is this module's
+    Hill-stable draw with no planet within 1 AU of the particle belt (the
+    reference's eight-planet system is synthetic and unpublished)."""
+    rng = np.random.default_rng(seed)
+    for _ in range(1000):
+        base = make_system(int(rng.integers(1 << 30)), 0, h=1.0)
+        ap, _ = osculating_a(base["m"][0] + base["m"][1:], base["x"][1:], base["v"][1:])
+        if np.all((ap < 2.2) | (ap > 4.8)):
+            break
+    k = nparticles
+    a = rng.uniform(3.2, 3.8, k)
+    e = rng.uniform(0.01, 0.05, k)
+    w = rng.uniform(0.0, 2 * math.pi, k)
+    zero = np.zeros(k)
+    xs, vs = elements_to_xv(np.full(k, base["m"][0]), a, e, zero, w, zero, zero)
+    nb = base["nbig"]
+    s = dict(m=np.concatenate([base["m"], np.zeros(k)]), x=np.concatenate([base["x"], xs]),
+             v=np.concatenate([base["v"], vs]), nbig=nb, h=1.0, rcen=base["rcen"], jcen=np.zeros(3), seed=seed)
+    s = finish_limits(s)
+    ngf = np.zeros((nb + k, 5))
+    ngf[nb:, 3] = 1e-4
+    ngf[nb:, 4] = 1e-12
+    s["ngf"] = ngf
+    s["ngflag"] = 3
+    return s

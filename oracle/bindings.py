@@ -407,8 +407,9 @@ class Reference:
                         C.byref(C.c_int(0)), _i(np.zeros(1, np.int32)), _i(np.zeros(1, np.int32)))
         return a
 
-    def ra15_run(self, sysd, nsteps, h0=None, tol=1e-12, pn=0, time0=0.0):
-        """nsteps calls of the reference mdt_ra15 with mfo_all as the force."""
+    def ra15_run(self, sysd, nsteps, h0=None, tol=1e-12, pn=0, time0=0.0, until=None):
+        """nsteps calls of the reference mdt_ra15 with mfo_all as the force (or,
+        with `until`, calls until that time, the last size clipped to it)."""
         m = np.ascontiguousarray(sysd["m"], np.float64)
         n = len(m)
         x = np.ascontiguousarray(sysd["x"], np.float64).copy()
@@ -423,7 +424,11 @@ class Reference:
         dtflag = C.c_int(0)
         hd = []
         time = time0
-        for _ in range(nsteps):
+        k = 0
+        while (k < nsteps) if until is None else (time < until):
+            k += 1
+            if until is not None and abs(t.value) > until - time:
+                t.value = until - time
             self.L.mdt_ra15_(self._r(time), C.byref(t), C.byref(tdid), self._r(tol), _d(jc), C.byref(C.c_int(n)),
                              C.byref(C.c_int(sysd["nbig"])), _d(m), _d(x), _d(v), _d(s), _d(rphys), _d(rc),
                              _d(ngf), _i(st), C.byref(dtflag), C.byref(ngflag), _i(opt),

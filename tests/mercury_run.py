@@ -67,8 +67,18 @@ def write_run(path, sysd, *, algorithm="MVS", stop=None, nsteps=100, interval=No
     return path
 
 
+def _big_stack():
+    """The driver keeps body-count-sized arrays on the stack: at 1e5 bodies
+    they exceed the usual 8 MiB limit."""
+    import resource
+    soft, hard = resource.getrlimit(resource.RLIMIT_STACK)
+    want = resource.RLIM_INFINITY if hard == resource.RLIM_INFINITY else hard
+    resource.setrlimit(resource.RLIMIT_STACK, (want, hard))
+
+
 def run(path, binary, timeout=600):
-    r = subprocess.run([str(binary)], cwd=path, capture_output=True, text=True, timeout=timeout)
+    r = subprocess.run([str(binary)], cwd=path, capture_output=True, text=True, timeout=timeout,
+                       preexec_fn=_big_stack)
     return r
 
 
