@@ -12,6 +12,10 @@
 //                     against the Big boxes (mce_snif);
 //   hy_big_finish / hy_particles_finish: second Sun-momentum drift, mfo_hy
 //                     accelerations, kick, physical velocities.
+// With the cometary model (ngflag 1 or 3) every body carries mdt_hy's angf
+// in st.angf: mfo_ngf at a reset (mercury6_2.for:3687) and after the second
+// Sun-momentum drift (3771), added in both kicks as hby2 * (angf + ausr + a)
+// with no user force; the encounter members' mfo_hkce has no such term.
 // Encounter pairs are appended by atomics and sorted into the Fortran loop
 // order on the host.  The encounter members' Keplerian part (mdt_hkce) is the
 // driver's, integrated through the compact context below (mdt_bs2 with
@@ -146,6 +150,7 @@ __global__ void __launch_bounds__(64) hy_big_begin(HyArgs args) {
     __shared__ double m[kMaxBig], rc[kMaxBig], X[3 * kMaxBig], V[3 * kMaxBig], A[3 * kMaxBig], aob[3 * kMaxBig];
     __shared__ double hdr[kHyHdr];
     __shared__ double X0[3 * kMaxBig], V0[3 * kMaxBig], X1[3 * kMaxBig], V1[3 * kMaxBig], BOX[4 * kMaxBig];
+    const bool ng = d.ngflag == 1 || d.ngflag == 3;
     for (int q = p; q < d.nbig; q += 64) m[q] = args.st.m[b0 + q];
     for (int q = p; q < np; q += 64) {
         rc[q] = args.rcrit[b0 + 1 + q];
@@ -157,8 +162,19 @@ __global__ void __launch_bounds__(64) hy_big_begin(HyArgs args) {
     }
     __syncthreads();
     double* T = args.tab;
-    if (p < np)
+    double ang[3] = {0.0, 0.0, 0.0};
+    if (p < np) {
         for (int c = 0; c < 3; c++) T[kHyHdr + HB_N * p + HB_XS + c] = X[3 * p + c];
+        if (ng && args.reset) {
+            double pb[5];
+            load_ngf(args.st, b0 + 1 + p, pb);
+            ngf_body(pb[0], pb[1], pb[2], pb[4], X[3 * p], X[3 * p + 1], X[3 * p + 2], V[3 * p], V[3 * p + 1],
+                     V[3 * p + 2], ang[0], ang[1], ang[2], args.fault);
+            for (int c = 0; c < 3; c++) args.st.angf[c * cap + b0 + 1 + p] = ang[c];
+        } else if (ng) {
+            for (int c = 0; c < 3; c++) ang[c] = args.st.angf[c * cap + b0 + 1 + p];
+        }
+    }
     if (args.reset) hy_big_accel(d, np, m, rc, X, A, aob, hdr + 6);
     else if (p == 0) for (int c = 0; c < 3; c++) hdr[6 + c] = 0.0;
     if (p == 0) {
@@ -170,7 +186,7 @@ __global__ void __launch_bounds__(64) hy_big_begin(HyArgs args) {
             args.st.ox[c * cap + b0 + 1 + p] = X[3 * p + c];
             args.st.ov[c * cap + b0 + 1 + p] = V[3 * p + c] + hdr[c];
             args.st.a[c * cap + b0 + 1 + p] = A[3 * p + c];
-            V[3 * p + c] = V[3 * p + c] + hby2 * (0.0 + 0.0 + A[3 * p + c]);
+            V[3 * p + c] = V[3 * p + c] + hby2 * (ang[c] + 0.0 + A[3 * p + c]);
         }
     __syncthreads();
     if (p == 0) momentum_shift(np, m, V, hby2 / m[0], hdr + 3);
@@ -236,6 +252,7 @@ __device__ inline void hy_particle_accel(const Sys& d, const double* __restrict_
     }
 }
 
+template <bool NG>
 __global__ void __launch_bounds__(128) hy_particles_begin(HyArgs args) {
     const Sys d = args.sys[0];
     const int jb = d.nbig + blockIdx.x * blockDim.x + threadIdx.x;
@@ -253,12 +270,22 @@ __global__ void __launch_bounds__(128) hy_particles_begin(HyArgs args) {
     const double rcj = args.rcrit[g];
     if (args.reset) hy_particle_accel<HB_N>(d, T, np, HB_XS, rcj, x, T + 6, a);
     else for (int c = 0; c < 3; c++) a[c] = args.st.a[c * cap + g];
+    double ang[3] = {0.0, 0.0, 0.0};
+    if (NG && args.reset) {
+        double pb[5];
+        load_ngf(args.st, g, pb);
+        ngf_body(pb[0], pb[1], pb[2], pb[4], x[0], x[1], x[2], v[0], v[1], v[2], ang[0], ang[1], ang[2],
+                 args.fault);
+        for (int c = 0; c < 3; c++) args.st.angf[c * cap + g] = ang[c];
+    } else if (NG) {
+        for (int c = 0; c < 3; c++) ang[c] = args.st.angf[c * cap + g];
+    }
     double x0[3], v0[3];
     for (int c = 0; c < 3; c++) {
         args.st.ox[c * cap + g] = x[c];
         args.st.ov[c * cap + g] = v[c] + T[c];
         if (args.reset) args.st.a[c * cap + g] = a[c];
-        v[c] = v[c] + hby2 * (0.0 + 0.0 + a[c]);
+        v[c] = v[c] + hby2 * (ang[c] + 0.0 + a[c]);
         x[c] = x[c] + T[3 + c];
         x0[c] = x[c];
         v0[c] = v[c];
@@ -337,8 +364,17 @@ __global__ void __launch_bounds__(64) hy_big_finish(HyArgs args) {
         for (int c = 0; c < 3; c++) X[3 * p + c] = X[3 * p + c] + hdr[c];
     __syncthreads();
     hy_big_accel(d, np, m, rc, X, A, aob, hdr + 6);
-    if (p < np)
-        for (int c = 0; c < 3; c++) V[3 * p + c] = V[3 * p + c] + hby2 * (0.0 + 0.0 + A[3 * p + c]);
+    if (p < np) {
+        double ang[3] = {0.0, 0.0, 0.0};
+        if (d.ngflag == 1 || d.ngflag == 3) {
+            double pb[5];
+            load_ngf(args.st, b0 + 1 + p, pb);
+            ngf_body(pb[0], pb[1], pb[2], pb[4], X[3 * p], X[3 * p + 1], X[3 * p + 2], V[3 * p], V[3 * p + 1],
+                     V[3 * p + 2], ang[0], ang[1], ang[2], args.fault);
+            for (int c = 0; c < 3; c++) args.st.angf[c * cap + b0 + 1 + p] = ang[c];
+        }
+        for (int c = 0; c < 3; c++) V[3 * p + c] = V[3 * p + c] + hby2 * (ang[c] + 0.0 + A[3 * p + c]);
+    }
     __syncthreads();
     if (p == 0) momentum_shift(np, m, V, 1.0 / m[0], hdr + 3);
     __syncthreads();
@@ -358,6 +394,7 @@ __global__ void __launch_bounds__(64) hy_big_finish(HyArgs args) {
     if (p < kHyHdr) T[p] = p < 9 ? hdr[p] : 0.0;
 }
 
+template <bool NG>
 __global__ void __launch_bounds__(128) hy_particles_finish(HyArgs args) {
     const Sys d = args.sys[0];
     const int jb = d.nbig + blockIdx.x * blockDim.x + threadIdx.x;
@@ -369,9 +406,17 @@ __global__ void __launch_bounds__(128) hy_particles_finish(HyArgs args) {
     double x[3], a[3];
     for (int c = 0; c < 3; c++) x[c] = args.st.x[c * cap + g] + T[c];
     hy_particle_accel<HF_N>(d, T, np, HF_X, args.rcrit[g], x, T + 6, a);
+    double ang[3] = {0.0, 0.0, 0.0};
+    if (NG) {
+        double pb[5];
+        load_ngf(args.st, g, pb);
+        ngf_body(pb[0], pb[1], pb[2], pb[4], x[0], x[1], x[2], args.st.v[g], args.st.v[cap + g],
+                 args.st.v[2 * cap + g], ang[0], ang[1], ang[2], args.fault);
+        for (int c = 0; c < 3; c++) args.st.angf[c * cap + g] = ang[c];
+    }
     bool bad = false;
     for (int c = 0; c < 3; c++) {
-        double v = args.st.v[c * cap + g] + d.hby2 * (0.0 + 0.0 + a[c]);
+        double v = args.st.v[c * cap + g] + d.hby2 * (ang[c] + 0.0 + a[c]);
         args.st.x[c * cap + g] = x[c];
         args.st.v[c * cap + g] = v;
         args.st.a[c * cap + g] = a[c];

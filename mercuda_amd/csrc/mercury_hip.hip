@@ -1309,13 +1309,18 @@ void hybrid_begin(double h, const double* crit, int flag, int cap, int* ce, int*
         // of x, v when the pair buffer has to grow and the step is retried
         hipLaunchKernelGGL(hy_big_begin, dim3(1), dim3(64), 0, c.stream, a);
         const int npart = c.n - c.nbig;
-        if (npart > 0)
-            hipLaunchKernelGGL(hy_particles_begin, dim3((npart + 127) / 128), dim3(128), 0, c.stream, a);
+        const bool ng = c.ngflag == 1 || c.ngflag == 3;
+        if (npart > 0 && ng)
+            hipLaunchKernelGGL(hy_particles_begin<true>, dim3((npart + 127) / 128), dim3(128), 0, c.stream, a);
+        else if (npart > 0)
+            hipLaunchKernelGGL(hy_particles_begin<false>, dim3((npart + 127) / 128), dim3(128), 0, c.stream, a);
         HIPCHECK(hipGetLastError());
         int cnt = 0;
         HIPCHECK(hipMemcpyAsync(&cnt, c.hnpairs, sizeof(int), hipMemcpyDeviceToHost, c.stream));
         HIPCHECK(hipStreamSynchronize(c.stream));
-        if (read_fault() & FAULT_FORCE) throw std::runtime_error("HYBRID force failed");
+        const int f = read_fault();
+        check_ngf_fault(f);
+        if (f & FAULT_FORCE) throw std::runtime_error("HYBRID force failed");
         if (cnt > c.hpair_cap) {
             // the scan overflowed after the state moved: the encounter list is
             // incomplete, and a retry would double-step; fail loudly
@@ -1391,7 +1396,10 @@ void hybrid_finish(double h, const double* m, const double* xx, const double* vv
     HyArgs a = hy_args(false);
     hipLaunchKernelGGL(hy_big_finish, dim3(1), dim3(64), 0, c.stream, a);
     const int npart = c.n - c.nbig;
-    if (npart > 0) hipLaunchKernelGGL(hy_particles_finish, dim3((npart + 127) / 128), dim3(128), 0, c.stream, a);
+    if (npart > 0 && (c.ngflag == 1 || c.ngflag == 3))
+        hipLaunchKernelGGL(hy_particles_finish<true>, dim3((npart + 127) / 128), dim3(128), 0, c.stream, a);
+    else if (npart > 0)
+        hipLaunchKernelGGL(hy_particles_finish<false>, dim3((npart + 127) / 128), dim3(128), 0, c.stream, a);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipStreamSynchronize(c.stream));
     check_fault("HYBRID force failed");
@@ -1741,23 +1749,23 @@ int mercury_hip_upload(int n, int nbig, int pn, int ngflag, const double* m, con
         if (ngflag < 0 || ngflag > 3) throw std::runtime_error("invalid non-gravitational flag");
         if (pn != 0 && (mvs_family() || ctx.algorithm == 10))
             throw std::runtime_error("MVS and HYBRID do not take post-Newtonian forces");
-        // the symplectic maps apply only mfo_ngf (ngflag 1 or 3), and only to particles here
+        // the symplectic maps apply only mfo_ngf (ngflag 1 or 3): HYBRID on every
+        // body, MVS on particles here
         const bool cometary = (ngflag == 1 || ngflag == 3);
         auto has_ngf = [&](int j) {
             return ngf[5 * j] != 0.0 || ngf[5 * j + 1] != 0.0 || ngf[5 * j + 2] != 0.0 || ngf[5 * j + 4] != 0.0;
         };
         std::vector<int> comets;
-        if (cometary && (mvs_family() || ctx.algorithm == 10)) {
+        if (cometary && mvs_family()) {
             for (int j = 1; j < n; j++) {
                 if (!has_ngf(j)) continue;
-                if (ctx.algorithm == 10)
-                    throw std::runtime_error("HYBRID with non-gravitational forces is not provided by the HIP path");
                 if (j < nbig)
                     throw std::runtime_error("MVS with non-gravitational forces on Big bodies is not provided by the "
                                              "HIP path");
                 comets.push_back(j);
             }
         }
+        const bool need_angf = !comets.empty() || (cometary && ctx.algorithm == 10);
         if (ctx.enc.active) throw std::runtime_error("upload inside an encounter context");
         for (int j = nbig; j < n; j++)
             if (m[j] != 0) throw std::runtime_error("Small bodies with mass are not provided by the HIP path");
@@ -1785,7 +1793,7 @@ int mercury_hip_upload(int n, int nbig, int pn, int ngflag, const double* m, con
             }
         }
         ctx.comets = comets;
-        if (!comets.empty()) {
+        if (need_angf) {
             if (ctx.angf_cap != ctx.cap) {
                 dalloc(ctx.dangf, (size_t)3 * ctx.cap);
                 ctx.angf_cap = ctx.cap;

@@ -290,8 +290,9 @@ class Oracle:
         jc = np.ascontiguousarray(sysd.get("jcen", (0.0, 0.0, 0.0)), np.float64)
         arrs = [np.ascontiguousarray(sysd[k], np.float64) for k in ("rcrit", "rce", "rphys")]
         evi = np.zeros((cap, 4), np.int32); evd = np.zeros((cap, 2)); nf = C.c_long(0)
-        k = self.L.orc_hy_run(time0, sysd["h"], tol, sysd["rcen"], _d(jc), len(m), sysd["nbig"], _d(m), _d(x),
-                              _d(v), *[_d(a) for a in arrs], nsteps, cap, _i(evi), _d(evd), C.byref(nf))
+        with self.ngf_model(sysd):
+            k = self.L.orc_hy_run(time0, sysd["h"], tol, sysd["rcen"], _d(jc), len(m), sysd["nbig"], _d(m), _d(x),
+                                  _d(v), *[_d(a) for a in arrs], nsteps, cap, _i(evi), _d(evd), C.byref(nf))
         if k < 0:
             raise RuntimeError("oracle HYBRID run failed (capacity or BS2 stall)")
         return x, v, evi[:k].copy(), evd[:k].copy(), nf.value
@@ -486,7 +487,9 @@ class Reference:
         h = sysd["h"]; rcen = sysd["rcen"]
         cm = self.cmax
         s = np.zeros((n, 3)); stat = np.zeros(n, np.int32)
-        ngf = np.zeros((n, 5)); opt = np.array([0, 0, 0, 2, 0, 1, 0, 0], np.int32)
+        ngf = np.ascontiguousarray(sysd.get("ngf", np.zeros((n, 5))), np.float64)
+        ngflag = int(sysd.get("ngflag", 0))
+        opt = np.array([0, 0, 0, 2, 0, 1, 0, 0], np.int32)
         ids = C.create_string_buffer(b" " * (25 * n))
         outfile = C.create_string_buffer(b"/dev/null".ljust(80) * 3)
         mem = C.create_string_buffer(b" " * (80 * self.NMESS))
@@ -512,7 +515,7 @@ class Reference:
                            C.byref(C.c_int(n)), C.byref(C.c_int(nbig)), _d(m), _d(x), _d(v),
                            _d(s), _d(rphys), _d(rcrit), _d(rce), _i(stat), ids,
                            _d(ngf), C.byref(C.c_int(10)), _i(opt), C.byref(dtflag),
-                           C.byref(C.c_int(0)), C.byref(C.c_int(-1)), C.byref(colflag),
+                           C.byref(C.c_int(ngflag)), C.byref(C.c_int(-1)), C.byref(colflag),
                            C.byref(nclo), _i(iclo), _i(jclo), _d(dclo), _d(tclo), _d(ixv), _d(jxv),
                            outfile, mem, _i(lmem),
                            C.c_size_t(25), C.c_size_t(80), C.c_size_t(80))

@@ -1865,7 +1865,10 @@ static int hkce(double time, double h0, double *hrec, double tol, double rcen, i
     return rej;
 }
 
-/* mdt_hy  mercury6_2.for:3618-3797 (no user or non-gravitational forces).
+/* mdt_hy  mercury6_2.for:3618-3797 (no user force; cometary forces from the
+ * orc_set_ngf model: angf by mfo_ngf at a reset and after the second
+ * Sun-momentum drift, 3687 and 3771, added in both kicks; the encounter
+ * members' BS2 force mfo_hkce has none).
  * st->a carries the interaction accelerations; st->dtflag, st->hrec as the
  * routine's saved state.  Returns the encounter pairs found (nce). */
 int orc_mdt_hy(double time, double h0, double tol, const double *jcen, double rcen, int nbod, int nbig,
@@ -1874,14 +1877,18 @@ int orc_mdt_hy(double time, double h0, double tol, const double *jcen, double rc
 {
     const double hby2 = h0 * .5;
     ev->nclo = 0;
+    double *angf = (g_angf && (g_ngflag == 1 || g_ngflag == 3)) ? g_angf : NULL;
+    double *zero = angf ? NULL : calloc((size_t)3 * nbod, sizeof(double));
+    double *ang = angf ? angf : zero;
     if (st->dtflag != 2) {
         if (st->dtflag == 0) st->hrec = h0;
         orc_mfo_hy(jcen, nbod, nbig, m, x, rcrit, st->a);
         st->nforce++;
         st->dtflag = 2;
+        if (angf) orc_mfo_ngf(nbod, x, v, angf, g_ngf);
     }
     for (int j = 1; j < nbod; j++)
-        for (int q = 0; q < 3; q++) v[3 * j + q] = v[3 * j + q] + hby2 * (0.0 + 0.0 + st->a[3 * j + q]);
+        for (int q = 0; q < 3; q++) v[3 * j + q] = v[3 * j + q] + hby2 * (ang[3 * j + q] + 0.0 + st->a[3 * j + q]);
     double mv[3] = {0, 0, 0};
     for (int j = 1; j < nbod; j++)
         for (int q = 0; q < 3; q++) mv[q] = mv[q] + m[j] * v[3 * j + q];
@@ -1900,14 +1907,14 @@ int orc_mdt_hy(double time, double h0, double tol, const double *jcen, double rc
     int *ce = calloc((size_t)nbod, sizeof(int));
     int cap = st->cap;
     int nce = orc_mce_snif(h0, nbod, nbig, x0, v0, x, v, rcrit, ce, cap, st->ice, st->jce);
-    if (nce < 0) { free(x0); free(v0); free(ce); return -1; }
+    if (nce < 0) { free(x0); free(v0); free(ce); free(zero); return -1; }
     if (nce > 0) {
         for (int j = 1; j < nbod; j++)
             if (ce[j] != 0)
                 for (int q = 0; q < 3; q++) { x[3 * j + q] = x0[3 * j + q]; v[3 * j + q] = v0[3 * j + q]; }
         if (hkce(time, h0, &st->hrec, tol, rcen, nbod, nbig, m, x, v, rcrit, rce, rphys, ce, nce, st->ice,
                  st->jce, ev, &st->nforce) < 0) {
-            free(x0); free(v0); free(ce);
+            free(x0); free(v0); free(ce); free(zero);
             return -1;
         }
     }
@@ -1919,9 +1926,10 @@ int orc_mdt_hy(double time, double h0, double tol, const double *jcen, double rc
         for (int q = 0; q < 3; q++) x[3 * j + q] = x[3 * j + q] + mv[q];
     orc_mfo_hy(jcen, nbod, nbig, m, x, rcrit, st->a);
     st->nforce++;
+    if (angf) orc_mfo_ngf(nbod, x, v, angf, g_ngf);
     for (int j = 1; j < nbod; j++)
-        for (int q = 0; q < 3; q++) v[3 * j + q] = v[3 * j + q] + hby2 * (0.0 + 0.0 + st->a[3 * j + q]);
-    free(x0); free(v0); free(ce);
+        for (int q = 0; q < 3; q++) v[3 * j + q] = v[3 * j + q] + hby2 * (ang[3 * j + q] + 0.0 + st->a[3 * j + q]);
+    free(x0); free(v0); free(ce); free(zero);
     return nce;
 }
 

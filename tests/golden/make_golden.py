@@ -225,6 +225,16 @@ def nongrav(R=None):
     out["ngf_x"] = x; out["ngf_v"] = v; out["ngf_p"] = ngf; out["ngf_m"] = m
     out["ngf_a"] = R.mfo_ngf(x, v, ngf)
     out["pr_a"] = R.mfo_pr(m, 30, x, v, ngf)
+    # HYBRID: cometary terms on particles and Big bodies, half the particles
+    # near planet orbits (encounter members go through mdt_hkce)
+    s = systems.make_system(3, 300, a_range=(0.5, 40.0), emax=0.3, imax_deg=20.0, h=2.0, near_fraction=0.5)
+    systems.add_nongrav(s, seed=51, fraction=0.6, kinds=("a1", "a2", "a3"), big=True)
+    for key in ("m", "x", "v", "ngf", "rce", "rphys", "rcrit"):
+        out[f"hy_{key}"] = s[key]
+    out["hy_meta"] = np.array([s["nbig"], 150, s["h"], s["ngflag"], s["rcen"]], float)
+    xo, vo, evi, evd = R.mdt_hy_run(s, 150)
+    out["hy_xout"] = xo; out["hy_vout"] = vo; out["hy_evi"] = evi; out["hy_evd"] = evd
+    print(f"nongrav run hybrid: ngflag {s['ngflag']}, 150 steps, {len(evi)} event rows")
     runs = [("ra15", 4, 3, 1, 60), ("bs", 2, 3, 1, 30), ("bs2", 3, 1, 0, 30), ("mvs", 1, 1, 0, 300)]
     for name, algor, flag, pn, nsteps in runs:
         s = systems.config_small(seed=30 + algor, nparticles=60)

@@ -79,8 +79,8 @@ def test_host_side_errors_without_device(lib, capfd):
 
 def test_nongrav_upload_refusals_without_device(lib, capfd):
     """The non-gravitational model's limits, refused before any device call:
-    an invalid flag; HYBRID with cometary or Yarkovsky parameters; MVS with
-    them on a Big body (the symplectic maps take them on particles only)."""
+    an invalid flag; MVS with cometary or Yarkovsky parameters on a Big body
+    (the MVS map takes them on particles only)."""
     import ctypes as C
     n, nbig = 4, 2
     m = np.array([2.959122082855911e-4, 1e-9, 0.0, 0.0])
@@ -94,13 +94,10 @@ def test_nongrav_upload_refusals_without_device(lib, capfd):
     on_big = np.zeros((n, 5)); on_big[1, 4] = 1e-11
     assert lib.mercury_hip_configure(4) == 0
     assert upload(4, on_particle) != 0
-    assert lib.mercury_hip_configure(10) == 0
-    assert upload(1, on_particle) != 0
     assert lib.mercury_hip_configure(1) == 0
     assert upload(3, on_big) != 0
     err = capfd.readouterr().err
     assert "invalid non-gravitational flag" in err
-    assert "HYBRID with non-gravitational forces" in err
     assert "non-gravitational forces on Big bodies" in err
 
 

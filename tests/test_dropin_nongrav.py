@@ -121,3 +121,22 @@ def test_dropin_published_extra_force_case(tmp_path):
     assert any("Execution backend: HIP" in ln for ln in R.info_lines(b))
     for f in ("big.dmp", "small.dmp", "ce.out", "xv.out"):
         assert (a / f).read_bytes() == (b / f).read_bytes(), f
+
+
+def test_dropin_hybrid_comets_match_reference(tmp_path):
+    """HYBRID with a1 and a3 on every small body (the reference's case,
+    tests/test_gpu_algorithms.py:92-93), over 600 particles half near planet
+    orbits, so that encounter members pass through mdt_hkce."""
+    s = systems.config_encounters(seed=3, nparticles=600)
+    n, nb = len(s["m"]), s["nbig"]
+    s["ngf"] = np.zeros((n, 5))
+    s["ngf"][nb:, 0] = 1e-9
+    s["ngf"][nb:, 2] = 1e-9
+    s["ngflag"] = 1
+    print("case: drop-in HYBRID with a1 = a3 = 1e-9 on 600 particles, 300 steps of 2 d", flush=True)
+    a, b = run_pair(tmp_path, s, 300, algorithm="HYBRID", interval=100.0)
+    worst = worst_dump_difference(a, b)
+    print(f"max relative dump difference {worst:.3e}", flush=True)
+    assert worst <= REL_TOL
+    assert (a / "ce.out").read_bytes() == (b / "ce.out").read_bytes()
+    assert (a / "xv.out").read_bytes() == (b / "xv.out").read_bytes()

@@ -247,3 +247,33 @@ def test_undefined_direction_fails(hip, algor, capfd):
         else:
             hip.advance(0.0, s["h"], 4, s["rcen"], screen=False)
     assert "non-gravitational force failed" in capfd.readouterr().err
+
+
+@pytest.mark.parametrize("loop", ["per-substep", "device"])
+def test_hybrid_comets_match_oracle(hip, oracle, loop):
+    """HYBRID with the cometary terms on Big bodies and particles, half the
+    particles near planet orbits (encounter members through mdt_hkce): angf at
+    a reset and after the second Sun-momentum drift, in both kicks.  The
+    driver's loop of one BS2 step per call, and the device encounter loop."""
+    from helpers import compare_rows, oracle_rows
+    from hybrid_driver import hy_run_device
+    from mercuda_amd import hybrid
+    s = systems.make_system(3, 300, a_range=(0.5, 40.0), emax=0.3, imax_deg=20.0, h=2.0, near_fraction=0.5)
+    systems.add_nongrav(s, seed=51, fraction=0.6, kinds=("a1", "a2", "a3"), big=True)
+    say(f"case: HYBRID cometary ({loop} encounter loop), 8 planets + 300 particles, 150 steps of 2 d")
+    if loop == "per-substep":
+        x, v, rows, most = hy_run_device(hip, oracle, s, 150)
+    else:
+        x, v, most = hybrid.hy_run(hip, s, 150)
+        rows = None
+    xo, vo, evi, evd, _ = oracle.hy_run(s, 150)
+    same = int(np.sum(np.all(x == xo, axis=1) & np.all(v == vo, axis=1)))
+    ex, _ = rel_err(x[1:], xo[1:]); ev, _ = rel_err(v[1:], vo[1:])
+    say(f"encounter pairs in one step, at most {most}; max rel err x {ex:.3e} v {ev:.3e}; "
+        f"bit-identical {same}/{len(x)}")
+    assert most > 0
+    assert ex <= REL_TOL and ev <= REL_TOL
+    if rows is not None:
+        compare_rows(rows, oracle_rows(evi, evd))
+    x0, _, _, _, _ = oracle.hy_run(dict(s, ngflag=0), 150)
+    assert rel_err(x0[1:], xo[1:])[0] > 1e-9

@@ -743,6 +743,22 @@ def test_golden_nongrav_runs(oracle, name):
     assert np.array_equal(x[1:], g[f"{name}_xout"][1:]) and np.array_equal(v[1:], g[f"{name}_vout"][1:])
 
 
+def test_golden_nongrav_hybrid(oracle):
+    """mdt_hy with the cometary terms on Big bodies and particles (angf at a
+    reset and after the second Sun-momentum drift, both kicks), encounters
+    through mdt_hkce: states and event rows bit for bit."""
+    g = np.load(GOLD / "nongrav.npz")
+    nbig, nsteps, h, flag, rcen = g["hy_meta"]
+    s = dict(m=g["hy_m"], x=g["hy_x"], v=g["hy_v"], ngf=g["hy_ngf"], ngflag=int(flag), nbig=int(nbig), h=h,
+             rcen=rcen, jcen=np.zeros(3), rce=g["hy_rce"], rphys=g["hy_rphys"], rcrit=g["hy_rcrit"])
+    x, v, evi, evd, _ = oracle.hy_run(s, int(nsteps))
+    assert len(g["hy_evi"]) > 0
+    assert np.array_equal(evi, g["hy_evi"]) and np.array_equal(evd, g["hy_evd"])
+    assert np.array_equal(x[1:], g["hy_xout"][1:]) and np.array_equal(v[1:], g["hy_vout"][1:])
+    x0, _, _, _, _ = oracle.hy_run(dict(s, ngflag=0), int(nsteps))
+    assert not np.array_equal(x0[1:], x[1:])
+
+
 def test_reference_mfo_ngf_random(oracle, reference):
     rng = np.random.default_rng(77)
     n = 2000
