@@ -120,7 +120,7 @@ struct Context {
     hipStream_t sp = nullptr, sg = nullptr;   // planet and general-kernel streams
     hipStream_t sm = nullptr;        // the MVS pipeline's particle stream (the main stream)
     hipEvent_t ev_plan[kRing + 1] = {}, ev_fast[kRing + 1] = {}, ev_done[kRing + 1] = {};
-    hipEvent_t ev_gate = nullptr;    // the first fast launch of a call waits for it
+    hipEvent_t ev_gate = nullptr;    // each fast launch waits for it (recorded after a marker)
     // host
     std::vector<double> stage;
     std::vector<DevEvent> hev;
@@ -516,12 +516,23 @@ void launch_general(bool screen, bool save, bool uniform, dim3 grid, dim3 block,
 
 // Launch lengths for nsteps with at most smax per launch.  Launch boundaries
 // do not change results.
+// Launch lengths of one call: a short first launch, so that the particles
+// start after a few planet steps while the planet wave tabulates the next
+// launch's steps, then launches growing by a factor (4, 16, 32, ... with the
+// defaults).  The planet wave takes ~9 us per step and the particle launch
+// ~30 us per step at 1e6 particles, so a launch of k steps covers the next
+// one's ~3k planet steps.  MERCURY_HIP_PLAN_HEAD and MERCURY_HIP_PLAN_GROW
+// override the two numbers (head 0: every launch smax steps).
 std::vector<int> chunk_plan(int nsteps, int smax) {
+    static const int head = std::getenv("MERCURY_HIP_PLAN_HEAD") ? std::atoi(std::getenv("MERCURY_HIP_PLAN_HEAD")) : 4;
+    static const int grow = std::getenv("MERCURY_HIP_PLAN_GROW") ? std::atoi(std::getenv("MERCURY_HIP_PLAN_GROW")) : 4;
     std::vector<int> plan;
+    int c = head > 0 ? std::min(head, smax) : smax;
     for (int rest = nsteps; rest > 0;) {
-        const int c = std::min(smax, rest);
-        plan.push_back(c);
-        rest -= c;
+        const int k = std::min(c, rest);
+        plan.push_back(k);
+        rest -= k;
+        c = std::min(smax, c * std::max(grow, 1));
     }
     return plan;
 }
@@ -629,16 +640,15 @@ void mvs_enqueue(double time, double h, int nsteps, bool reset, bool screen, boo
                 HIPCHECK(hipStreamWaitEvent(c.sg, c.ev_fast[(k - 1) % kRing], 0));
             }
             HIPCHECK(hipStreamWaitEvent(c.sg, c.ev_plan[slot], 0));
-            // The call's first fast launch starts behind a marker on the general
-            // stream: the slow set's waves, dispatched right after it, then take
-            // their CU slots before the fast kernel's fill every SIMD's registers.
-            hipEvent_t gate = c.ev_plan[slot];
-            if (k == 0) {
-                hipLaunchKernelGGL(marker, dim3(1), dim3(64), 0, c.sg);
-                HIPCHECK(hipGetLastError());
-                HIPCHECK(hipEventRecord(c.ev_gate, c.sg));
-                gate = c.ev_gate;
-            }
+            // Each fast launch starts behind a marker on the general stream: the
+            // slow set's waves, dispatched right after it, then take their CU
+            // slots before the fast kernel's fill every SIMD's registers (a slow-
+            // set launch queued behind a running fast launch would otherwise wait
+            // for it to drain).
+            hipLaunchKernelGGL(marker, dim3(1), dim3(64), 0, c.sg);
+            HIPCHECK(hipGetLastError());
+            HIPCHECK(hipEventRecord(c.ev_gate, c.sg));
+            const hipEvent_t gate = c.ev_gate;
             general(ga);
             if (k > 0) {
                 // launch k-1's screening details; its table slot is free after them

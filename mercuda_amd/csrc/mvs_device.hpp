@@ -503,6 +503,74 @@ __device__ inline void j2h_serial(int np_rt, const double* jhx, const double* XJ
     }
 }
 
+// The same transforms with lane p < np computing body p+1 only: its prefix
+// sums run over the bodies before it in body order, with the serial
+// version's operations, so every lane's result has the serial bits.  The
+// loads of a lane are independent of the other lanes' stores.
+template <int NPL = -1>
+__device__ inline void h2j_lanes(int np_rt, const double* m, const double* jdx, const double* X, const double* V,
+                                 double* XJ, double* VJ) {
+    const int np = NPL >= 0 ? NPL : np_rt;
+    const int p = threadIdx.x;
+    if (p >= np) return;
+    if (p == 0) {
+        for (int k = 0; k < 3; k++) { XJ[k] = X[k]; VJ[k] = V[k]; }
+        return;
+    }
+    double mx = m[1] * X[0], my = m[1] * X[1], mz = m[1] * X[2];
+    double mu = m[1] * V[0], mv = m[1] * V[1], mw = m[1] * V[2];
+#pragma unroll
+    for (int q = 1; q < (NPL >= 0 ? NPL : kMaxBig); q++) {
+        if (q >= p) break;
+        double mb = m[q + 1];
+        mx = mx + mb * X[3 * q + 0];
+        my = my + mb * X[3 * q + 1];
+        mz = mz + mb * X[3 * q + 2];
+        mu = mu + mb * V[3 * q + 0];
+        mv = mv + mb * V[3 * q + 1];
+        mw = mw + mb * V[3 * q + 2];
+    }
+    double temp = jdx[p];
+    XJ[3 * p + 0] = X[3 * p + 0] - temp * mx;
+    XJ[3 * p + 1] = X[3 * p + 1] - temp * my;
+    XJ[3 * p + 2] = X[3 * p + 2] - temp * mz;
+    VJ[3 * p + 0] = V[3 * p + 0] - temp * mu;
+    VJ[3 * p + 1] = V[3 * p + 1] - temp * mv;
+    VJ[3 * p + 2] = V[3 * p + 2] - temp * mw;
+}
+
+template <int NPL = -1>
+__device__ inline void j2h_lanes(int np_rt, const double* jhx, const double* XJ, const double* VJ, double* X,
+                                 double* V) {
+    const int np = NPL >= 0 ? NPL : np_rt;
+    const int p = threadIdx.x;
+    if (p >= np) return;
+    if (p == 0) {
+        for (int k = 0; k < 3; k++) { X[k] = XJ[k]; V[k] = VJ[k]; }
+        return;
+    }
+    double temp = jhx[0];
+    double mx = temp * XJ[0], my = temp * XJ[1], mz = temp * XJ[2];
+    double mu = temp * VJ[0], mv = temp * VJ[1], mw = temp * VJ[2];
+#pragma unroll
+    for (int q = 1; q < (NPL >= 0 ? NPL : kMaxBig); q++) {
+        if (q >= p) break;
+        temp = jhx[q];
+        mx = mx + temp * XJ[3 * q + 0];
+        my = my + temp * XJ[3 * q + 1];
+        mz = mz + temp * XJ[3 * q + 2];
+        mu = mu + temp * VJ[3 * q + 0];
+        mv = mv + temp * VJ[3 * q + 1];
+        mw = mw + temp * VJ[3 * q + 2];
+    }
+    X[3 * p + 0] = XJ[3 * p + 0] + mx;
+    X[3 * p + 1] = XJ[3 * p + 1] + my;
+    X[3 * p + 2] = XJ[3 * p + 2] + mz;
+    V[3 * p + 0] = VJ[3 * p + 0] + mu;
+    V[3 * p + 1] = VJ[3 * p + 1] + mv;
+    V[3 * p + 2] = VJ[3 * p + 2] + mw;
+}
+
 // Big-body part of mfo_mvs (mercury6_2.for:4945-5025, 5062-5070) for the
 // planets in LDS.  Lane p computes body p+1; serial recurrences run in lane 0.
 // Writes A (Big accelerations) and the table header terms a0tp and acen.
@@ -585,19 +653,16 @@ __device__ inline void big_forces(const Sys& d, int np_rt, const double* m, cons
                             aob[3 * p + 2]);
     }
     __syncthreads();
-    if (lane == 0) {
+    // the serial sums of mfo_mvs, each lane for its own body: a0 over every
+    // planet, a2 over the planets up to this one, both in body order, then
+    // the body's acceleration with the Fortran's expression; lane 0 also
+    // writes the particles' a0tp and the oblateness reaction
+    if (lane < (np > 0 ? np : 1)) {
         double a0[3] = {0.0, 0.0, 0.0};
-        double a2[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-        for (int q = 1; q < np; q++)
+        for (int q = 1; q < (NPL >= 0 ? NPL : kMaxBig); q++) {
+            if (q >= np) break;
             for (int c = 0; c < 3; c++) a0[c] = a0[c] - t0[3 * q + c];
-        double a0tp[3] = {0.0, 0.0, 0.0};
-        if (np >= 1) {
-            double r2 = X[0] * X[0] + X[1] * X[1] + X[2] * X[2];
-            double r = 1.0 / sqrt(r2);
-            double r3 = r * r * r;
-            double fac0 = m[1] * r3;
-            for (int c = 0; c < 3; c++) a0tp[c] = a0[c] - fac0 * X[c];
         }
         double acen[3] = {0.0, 0.0, 0.0};
         if (d.obl) {
@@ -606,16 +671,31 @@ __device__ inline void big_forces(const Sys& d, int np_rt, const double* m, cons
                 for (int c = 0; c < 3; c++) acen[c] = acen[c] - tmp4 * aob[3 * q + c];
             }
         }
+        if (lane < np) {
+            double a2[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-        for (int q = 0; q < np; q++) {
+            for (int q = 1; q < (NPL >= 0 ? NPL : kMaxBig); q++) {
+                if (q > lane) break;
+                for (int c = 0; c < 3; c++) a2[c] = a2[c] + t2[3 * q + c];
+            }
+            const int q = lane;
             for (int c = 0; c < 3; c++) {
-                if (q >= 1) a2[c] = a2[c] + t2[3 * q + c];
                 double a = a0[c] + a1[3 * q + c] + (q >= 1 ? a2[c] : 0.0) + A[3 * q + c];
                 if (d.obl) a = a + (aob[3 * q + c] - acen[c]);
                 A[3 * q + c] = a;
             }
         }
-        for (int c = 0; c < 3; c++) { hdr[c] = a0tp[c]; hdr[3 + c] = acen[c]; }
+        if (lane == 0) {
+            double a0tp[3] = {0.0, 0.0, 0.0};
+            if (np >= 1) {
+                double r2 = X[0] * X[0] + X[1] * X[1] + X[2] * X[2];
+                double r = 1.0 / sqrt(r2);
+                double r3 = r * r * r;
+                double fac0 = m[1] * r3;
+                for (int c = 0; c < 3; c++) a0tp[c] = a0[c] - fac0 * X[c];
+            }
+            for (int c = 0; c < 3; c++) { hdr[c] = a0tp[c]; hdr[3 + c] = acen[c]; }
+        }
     }
     __syncthreads();
 }
@@ -623,10 +703,11 @@ __device__ inline void big_forces(const Sys& d, int np_rt, const double* m, cons
 // Encounter screen of Big body jb against the Big bodies before it (the
 // planet kernel's part of mce_stat): screen_body with the planet loop
 // unrolled when the planet count is a compile-time constant.
+// box: the planets' padded x-y boxes of this step (LDS copy of the table's).
 template <int NPL>
-__device__ inline void screen_planet(const double* __restrict__ T, int np_rt, int jb, double mj, const Ends& e,
-                                     double rce_j, double rphys_j, double h, double time_end, int step, int sys,
-                                     const EventSink& es) {
+__device__ inline void screen_planet(const double* __restrict__ T, const double* box, int np_rt, int jb, double mj,
+                                     const Ends& e, double rce_j, double rphys_j, double h, double time_end, int step,
+                                     int sys, const EventSink& es) {
     if (NPL < 0) {
         screen_body(T, np_rt, jb, np_rt + 1, mj, e, rce_j, rphys_j, h, time_end, step, sys, es);
         return;
@@ -641,10 +722,10 @@ __device__ inline void screen_planet(const double* __restrict__ T, int np_rt, in
     const double* Sc = F + kF * NPL;
 #pragma unroll
     for (int p = 0; p < NPL; p++) {
-        const double* B = Sc + kScr * p;
-        if (p < jb - 1 && B[SX1] >= xmin && xmax >= B[SX0] && B[SY1] >= ymin && ymax >= B[SY0])
-            screen_pair(F + kF * p, B, p + 1, jb, true, F[kF * p + 3], mj, e, rce_j, rphys_j, h, time_end, step,
-                        sys, es);
+        const double* b = box + 4 * p;
+        if (p < jb - 1 && b[1] >= xmin && xmax >= b[0] && b[3] >= ymin && ymax >= b[2])
+            screen_pair(F + kF * p, Sc + kScr * p, p + 1, jb, true, F[kF * p + 3], mj, e, rce_j, rphys_j, h,
+                        time_end, step, sys, es);
     }
 }
 
@@ -655,6 +736,7 @@ struct PlanetLds {
     double X[3 * kMaxBig], V[3 * kMaxBig], A[3 * kMaxBig], XJ[3 * kMaxBig], VJ[3 * kMaxBig];
     double X0[3 * kMaxBig], V0[3 * kMaxBig], t0[3 * kMaxBig], t2[3 * kMaxBig];
     double a1[3 * kMaxBig], aob[3 * kMaxBig], hdr[8];
+    double box[4 * kMaxBig];   // this step's padded x-y boxes (screening)
 };
 
 template <int NPL>
@@ -733,7 +815,7 @@ __device__ __forceinline__ void planets_run(const PlanetArgs& args, PlanetLds& L
             }
         }
         __syncthreads();
-        if (lane == 0) h2j_serial<NPL>(np, m, jdx, X, V, XJ, VJ);
+        h2j_lanes<NPL>(np, m, jdx, X, V, XJ, VJ);
         __syncthreads();
         if (p < np) {
             Orbit o{XJ[3 * p], XJ[3 * p + 1], XJ[3 * p + 2], VJ[3 * p], VJ[3 * p + 1], VJ[3 * p + 2]};
@@ -742,17 +824,21 @@ __device__ __forceinline__ void planets_run(const PlanetArgs& args, PlanetLds& L
             VJ[3 * p] = o.vx; VJ[3 * p + 1] = o.vy; VJ[3 * p + 2] = o.vz;
         }
         __syncthreads();
-        if (lane == 0) j2h_serial<NPL>(np, jhx, XJ, VJ, X, V);
+        j2h_lanes<NPL>(np, jhx, XJ, VJ, X, V);
         __syncthreads();
         // screening data of this step (velocities after the drift, before the kick)
+        double vd[3] = {0.0, 0.0, 0.0};
         if (p < np) {
             double* B = T + kHdr + kF * np + kScr * p;
             double lo, hi;
             box_axis(h, X0[3 * p], V0[3 * p], X[3 * p], V[3 * p], lo, hi);
             double pad = rce_p * 1.2;
             B[SX0] = lo - pad; B[SX1] = hi + pad;
+            L.box[4 * p] = lo - pad; L.box[4 * p + 1] = hi + pad;
             box_axis(h, X0[3 * p + 1], V0[3 * p + 1], X[3 * p + 1], V[3 * p + 1], lo, hi);
             B[SY0] = lo - pad; B[SY1] = hi + pad;
+            L.box[4 * p + 2] = lo - pad; L.box[4 * p + 3] = hi + pad;
+            for (int c = 0; c < 3; c++) vd[c] = V[3 * p + c];
             for (int c = 0; c < 3; c++) {
                 B[SP + c] = X0[3 * p + c];
                 B[SV0 + c] = V0[3 * p + c];
@@ -781,9 +867,9 @@ __device__ __forceinline__ void planets_run(const PlanetArgs& args, PlanetLds& L
             __threadfence_block();
             int jb = p + 1;
             Ends e{X0[3 * p], X0[3 * p + 1], X0[3 * p + 2], V0[3 * p], V0[3 * p + 1], V0[3 * p + 2],
-                   X[3 * p], X[3 * p + 1], X[3 * p + 2], T[kHdr + kF * np + kScr * p + SV1],
-                   T[kHdr + kF * np + kScr * p + SV1 + 1], T[kHdr + kF * np + kScr * p + SV1 + 2]};
-            screen_planet<NPL>(T, np, jb, m[jb], e, rce_p, rphys_p, h, time_end, s + args.step0, sid, args.es);
+                   X[3 * p], X[3 * p + 1], X[3 * p + 2], vd[0], vd[1], vd[2]};
+            screen_planet<NPL>(T, L.box, np, jb, m[jb], e, rce_p, rphys_p, h, time_end, s + args.step0, sid,
+                               args.es);
             central_check(m[0], m[jb], d.rcen, h, time_end, xs[0], xs[1], xs[2], vs[0], vs[1], vs[2],
                           X[3 * p], X[3 * p + 1], X[3 * p + 2], V[3 * p], V[3 * p + 1], V[3 * p + 2], jb,
                           s + args.step0, sid, args.es);
