@@ -119,9 +119,11 @@ int mercury_hip_advance_events(const MercuryHipEvent **events, int *count);
 /* Launch shape: steps fused per particle launch (>= 1), particle workgroup
  * size (64..256, multiple of 64).  0 keeps the current value. */
 int mercury_hip_set_tuning(int steps_per_launch, int block_size);
-/* Kernel timing with HIP events on the launch stream: enable = 1 clears and
- * starts recording; the read returns the summed milliseconds and launch
- * counts of the fast particle kernel and of the planet kernel. */
+/* Kernel timing with HIP events on the launch streams: enable = 1 clears and
+ * starts recording every kernel kind, 2 the fast particle kernel only (no
+ * events on the planet and slow-set streams), 0 stops; the read returns the
+ * summed milliseconds and launch counts of the fast particle kernel, the
+ * planet kernel and the slow-set kernel. */
 int mercury_hip_timing(int enable);
 int mercury_hip_timing_read(double *particle_ms, int *particle_launches,
                             double *planet_ms, int *planet_launches,

@@ -50,6 +50,7 @@ void dfree(T*& p) {
 
 struct Timing {
     bool on = false;
+    bool fast_only = false;   // time the fast particle kernel's launches only
     std::vector<hipEvent_t> pool;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> marks;   // kind, (start, stop)
     size_t used = 0;
@@ -420,13 +421,13 @@ void download_vec3(const double* src, double* dst) {
         for (int k = 0; k < 3; k++) dst[3 * j + k] = ctx.stage[(size_t)k * n + j];
 }
 
-void timed_launch_begin(hipStream_t st, hipEvent_t& start) {
-    if (!ctx.timing.on) return;
+void timed_launch_begin(hipStream_t st, hipEvent_t& start, int kind = 0) {
+    if (!ctx.timing.on || (ctx.timing.fast_only && kind != 0)) return;
     start = ctx.timing.get();
     HIPCHECK(hipEventRecord(start, st));
 }
 void timed_launch_end(int kind, hipStream_t st, hipEvent_t start) {
-    if (!ctx.timing.on) return;
+    if (!ctx.timing.on || (ctx.timing.fast_only && kind != 0)) return;
     hipEvent_t stop = ctx.timing.get();
     HIPCHECK(hipEventRecord(stop, st));
     ctx.timing.marks.push_back({kind, {start, stop}});
@@ -597,7 +598,7 @@ void mvs_enqueue(double time, double h, int nsteps, bool reset, bool screen, boo
     };
     auto general = [&](const ParticleArgs& a) {
         hipEvent_t g0 = nullptr;
-        timed_launch_begin(c.sg, g0);
+        timed_launch_begin(c.sg, g0, K_GENERAL);
         launch_general(screen, save, single, dim3(gblocks), dim3(64), c.sg, a);
         timed_launch_end(K_GENERAL, c.sg, g0);
     };
@@ -624,7 +625,7 @@ void mvs_enqueue(double time, double h, int nsteps, bool reset, bool screen, boo
         const bool set_time = !device_time && ((single && !c.ensemble) || k == 0);
         PlanetArgs pa{c.dsys, st, tab, c.tcur, es, t, set_time ? 1 : 0, steps, rs0 ? 1 : 0, screen ? 1 : 0,
                       save ? 1 : 0, reqcount, 1, done};
-        timed_launch_begin(c.sp, t0);
+        timed_launch_begin(c.sp, t0, K_PLANET);
         hipLaunchKernelGGL(mvs_planets, dim3(nsys), dim3(64), 0, c.sp, pa);
         HIPCHECK(hipGetLastError());
         timed_launch_end(K_PLANET, c.sp, t0);
@@ -2208,6 +2209,7 @@ int mercury_hip_timing(int enable) {
     try {
         ctx.timing.clear();
         ctx.timing.on = enable != 0;
+        ctx.timing.fast_only = enable == 2;
         ctx.work_total = 0;
         if (ctx.timing.on) {
             if (!ctx.work) dalloc(ctx.work, 2);

@@ -198,8 +198,10 @@ class MercuryHip:
     def set_tuning(self, steps_per_launch=0, block_size=0):
         self._ok(self.L.mercury_hip_set_tuning(steps_per_launch, block_size), "set_tuning")
 
-    def timing(self, enable=True):
-        self._ok(self.L.mercury_hip_timing(int(enable)), "timing")
+    def timing(self, enable=True, fast_only=False):
+        """Kernel timing on (every kernel kind, or the fast particle kernel
+        only) or off."""
+        self._ok(self.L.mercury_hip_timing(0 if not enable else (2 if fast_only else 1)), "timing")
 
     def timing_read(self):
         pm = C.c_double(); pl = C.c_int(); qm = C.c_double(); ql = C.c_int(); gm = C.c_double(); gl = C.c_int()
