@@ -145,6 +145,10 @@ struct Context {
     Timing timing;
     // RA15 state (algorithm 4): b, e [21][cap], a1 [3][cap], max |b7|
     double *rb = nullptr, *re = nullptr, *ra1 = nullptr;
+    // the last sequence's finish, not yet applied (the next sequence's kernels
+    // apply it, or radau_flush when the state is read first)
+    bool ra_pend = false, ra_preset = false;
+    double ra_ptdid = 0.0, ra_pq = 0.0;
     // whole-system Bulirsch-Stoer (algorithms 2, 3): accepted states per level, level errors
     double* bscand = nullptr;
     double* bsrsave = nullptr;
@@ -272,7 +276,7 @@ void release_all() {
         dfree(*p);
     dfree(c.fault); dfree(c.dsys); dfree(c.tcur); c.sys_valid = false; c.sys_cap = 0; c.systems.clear(); dfree(c.dtiles); dfree(c.dev); dfree(c.devcount);
     dfree(c.dpark); dfree(c.dleave); dfree(c.dcnt); dfree(c.dslow); dfree(c.work); dfree(c.dreq); c.req_cap = 0;
-    dfree(c.rb); dfree(c.re); dfree(c.ra1); dfree(c.bmax); c.ra_cap = 0;
+    dfree(c.rb); dfree(c.re); dfree(c.ra1); dfree(c.bmax); c.ra_cap = 0; c.ra_pend = false;
     dfree(c.bscand); dfree(c.bsrsave); dfree(c.bserr); c.bs_cap = 0;
     dfree(c.dngf); c.ngf_cap = 0; dfree(c.dangf); c.angf_cap = 0; dfree(c.dseed); c.seed_cap = 0;
     c.comets.clear(); c.ngflag = 0;
@@ -810,6 +814,16 @@ double radau_pass(double t, int niter, bool reset) {
     static const RaConst K = ra_constants();
     RaArgs args{c.dsys, st, RaState{c.rb, c.re, c.ra1}, c.tab, c.bmax, K, t, niter, reset ? 1 : 0, c.pn,
                 light_speed2()};
+    if (c.ra_pend) {
+        // the previous sequence's finish rides on this pass; a redo of this
+        // sequence then starts from the state it wrote back
+        if (niter == 0) throw std::runtime_error("internal: RA15 finish pending at a force-only pass");
+        args.pend = 1;
+        args.ptdid = c.ra_ptdid;
+        args.pq = c.ra_pq;
+        args.preset = c.ra_preset ? 1 : 0;
+        c.ra_pend = false;
+    }
     HIPCHECK(hipMemsetAsync(c.bmax, 0, sizeof(unsigned long long), c.stream));
     const bool ng = c.ngflag != 0;
     if (c.pn && ng) hipLaunchKernelGGL((ra_planets<true, true>), dim3(1), dim3(64), 0, c.stream, args);
@@ -863,14 +877,28 @@ void radau_step(double time, double* h, double* hdid, double tol, int64_t* force
         if (std::fabs(t / tdid) > 1.4) t = tdid * 1.4;
         break;
     }
-    static const RaConst K = ra_constants();
-    RaFinishArgs fa{c.dsys, device_state(), RaState{c.rb, c.re, c.ra1}, K, tdid, t / tdid, reset ? 1 : 0};
-    if (c.n > 1) hipLaunchKernelGGL(ra_finish, dim3((c.n - 1 + 127) / 128), dim3(128), 0, c.stream, fa);
-    HIPCHECK(hipGetLastError());
-    HIPCHECK(hipStreamSynchronize(c.stream));
+    // the finish (x, v at the end; e, b predicted for the next sequence) waits
+    // for the next sequence's pass, or for radau_flush if the state is read first
+    c.ra_pend = true;
+    c.ra_ptdid = tdid;
+    c.ra_pq = t / tdid;
+    c.ra_preset = reset;
     *hdid = tdid;
     *h = t;
     c.history = true;
+}
+
+// Applies a pending RA15 finish (ra_finish over every body j >= 1: the
+// step-start state kept in ox, ov; x, v at the end; e, b predicted).
+void radau_flush() {
+    Context& c = ctx;
+    if (!c.ra_pend) return;
+    c.ra_pend = false;
+    static const RaConst K = ra_constants();
+    RaFinishArgs fa{c.dsys, device_state(), RaState{c.rb, c.re, c.ra1}, K, c.ra_ptdid, c.ra_pq, c.ra_preset ? 1 : 0};
+    if (c.n > 1) hipLaunchKernelGGL(ra_finish, dim3((c.n - 1 + 127) / 128), dim3(128), 0, c.stream, fa);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(c.stream));
 }
 
 // ---- Bulirsch-Stoer over the whole system (algorithms 2 and 3) ------------
@@ -1790,6 +1818,7 @@ int mercury_hip_upload(int n, int nbig, int pn, int ngflag, const double* m, con
         ctx.ensemble = false;
         ctx.sys_valid = false;
         ctx.upload_gen++;
+        ctx.ra_pend = false;   // a new state: the last sequence's finish is moot
         if (ngflag) {
             // parameters component-major: [q][body], q = A1, A2, A3, beta, yar
             if (ctx.ngf_cap != ctx.cap) {   // the kernels index it with the body capacity
@@ -1878,6 +1907,7 @@ int mercury_hip_download(double* x, double* v, int previous) {
             return 0;
         }
         require_state();
+        radau_flush();
         if (previous && !ctx.saved_valid) throw std::runtime_error("no previous state");
         download_vec3(previous ? ctx.ox : ctx.x, x);
         download_vec3(previous ? ctx.ov : ctx.v, v);
@@ -1888,6 +1918,7 @@ int mercury_hip_download(double* x, double* v, int previous) {
 int mercury_hip_events(double time, double h, double rcen, const MercuryEvent** out, int* count) {
     try {
         require_state();
+        radau_flush();
         if (!ctx.saved_valid) throw std::runtime_error("no step to screen");
         // single-system integrators that keep the step-start states in ox, ov
         const bool ra = ctx.algorithm == 4 || ctx.algorithm == 2 || ctx.algorithm == 3;
@@ -1928,6 +1959,7 @@ int mercury_hip_events(double time, double h, double rcen, const MercuryEvent** 
 int mercury_hip_force(double* a) {
     try {
         require_state();
+        radau_flush();
         if (ctx.systems.size() != 1) throw std::runtime_error("force needs one system");
         ensure_stream();
         clear_fault();
@@ -1991,6 +2023,7 @@ int mercury_hip_force_noalloc(int n, int nbig, int pn, const double* m, const do
 int mercury_hip_export(double h, int physical, double* x, double* v) {
     try {
         require_state();
+        radau_flush();
         if (physical && ctx.algorithm == 1) {
             // mco_mvs2h on scratch (mercury6_2.for:2107-2219); the state is untouched
             if (ctx.systems.size() != 1) throw std::runtime_error("corrector needs one system");

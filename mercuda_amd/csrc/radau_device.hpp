@@ -47,7 +47,44 @@ struct RaArgs {
     int reset;                // zero b (and e) first
     int pn;                   // central 1PN term, velocity predictors
     double light2;            // (speed of light in AU/day)^2
+    // The previous sequence's finish (mercury6_2.for:4426-4489), deferred to
+    // this launch when nothing read the state in between (pend): the start
+    // state advanced over ptdid with its final b and start acceleration, and
+    // the predicted e, b of this sequence with pq = this size / ptdid (no b - e
+    // correction after a reset sequence, preset; none at all when this one
+    // resets).  The start state is written back; b and e as usual.
+    double ptdid = 0.0, pq = 0.0;
+    int pend = 0, preset = 0;
 };
+
+// The finish of one component of one body (ra_finish's arithmetic): x1, v1
+// advanced over tdid from the final b and the start acceleration a1; then,
+// unless the next sequence resets, e and b predicted for it in place.
+__device__ __forceinline__ void ra_finish_comp(const RaConst& k, double tdid, double q, bool preset, bool reset,
+                                               double* b, double* e, double a1, double& x1, double& v1) {
+    const double temp = tdid * tdid;
+    const double* xc = k.xc;
+    const double* vc = k.vc;
+    x1 = (xc[7] * b[6] + xc[6] * b[5] + xc[5] * b[4] + xc[4] * b[3] + xc[3] * b[2] + xc[2] * b[1] + xc[1] * b[0]
+          + xc[0] * a1) * temp + v1 * tdid + x1;
+    v1 = (vc[6] * b[6] + vc[5] * b[5] + vc[4] * b[4] + vc[3] * b[3] + vc[2] * b[2] + vc[1] * b[1] + vc[0] * b[0]
+          + a1) * tdid + v1;
+    if (reset) {
+        for (int i = 0; i < 7; i++) { b[i] = 0.0; e[i] = 0.0; }
+        return;
+    }
+    const double q2 = q * q, q3 = q * q2, q4 = q2 * q2, q5 = q2 * q3, q6 = q3 * q3, q7 = q3 * q4;
+    double s[7];
+    for (int i = 0; i < 7; i++) s[i] = preset ? 0.0 : b[i] - e[i];
+    e[0] = q * (b[6] * 7.0 + b[5] * 6.0 + b[4] * 5.0 + b[3] * 4.0 + b[2] * 3.0 + b[1] * 2.0 + b[0]);
+    e[1] = q2 * (b[6] * 21.0 + b[5] * 15.0 + b[4] * 10.0 + b[3] * 6.0 + b[2] * 3.0 + b[1]);
+    e[2] = q3 * (b[6] * 35.0 + b[5] * 20.0 + b[4] * 10.0 + b[3] * 4.0 + b[2]);
+    e[3] = q4 * (b[6] * 35.0 + b[5] * 15.0 + b[4] * 5.0 + b[3]);
+    e[4] = q5 * (b[6] * 21.0 + b[5] * 6.0 + b[4]);
+    e[5] = q6 * (b[6] * 7.0 + b[5]);
+    e[6] = q7 * b[6];
+    for (int i = 0; i < 7; i++) b[i] = e[i] + s[i];
+}
 
 // Heliocentric acceleration of a massless body (mfo_grav for j > nbig, plus
 // oblateness): direct terms from the Big bodies in order, then the indirect
@@ -406,15 +443,28 @@ __global__ void __launch_bounds__(64) ra_planets(RaArgs args) {
     const bool iter = args.niter > 0;
     if (lane < np) {
         for (int c = 0; c < 3; c++) {
-            X1[3 * lane + c] = args.st.x[c * cap + gb];
-            V1[3 * lane + c] = args.st.v[c * cap + gb];
-            X[3 * lane + c] = X1[3 * lane + c];
-            V[3 * lane + c] = V1[3 * lane + c];
-            if (iter)
+            double x1 = args.st.x[c * cap + gb], v1 = args.st.v[c * cap + gb];
+            if (args.pend) {
+                double e[7];
+                for (int q = 0; q < 7; q++) {
+                    b[c][q] = args.ra.b[(c * 7 + q) * cap + gb];
+                    e[q] = args.ra.e[(c * 7 + q) * cap + gb];
+                }
+                ra_finish_comp(args.k, args.ptdid, args.pq, args.preset, args.reset, b[c], e,
+                               args.ra.a1[c * cap + gb], x1, v1);
+                for (int q = 0; q < 7; q++) args.ra.e[(c * 7 + q) * cap + gb] = e[q];
+                args.st.x[c * cap + gb] = x1;
+                args.st.v[c * cap + gb] = v1;
+            } else if (iter) {
                 for (int q = 0; q < 7; q++) {
                     b[c][q] = args.reset ? 0.0 : args.ra.b[(c * 7 + q) * cap + gb];
                     if (args.reset) args.ra.e[(c * 7 + q) * cap + gb] = 0.0;
                 }
+            }
+            X1[3 * lane + c] = x1;
+            V1[3 * lane + c] = v1;
+            X[3 * lane + c] = x1;
+            V[3 * lane + c] = v1;
         }
     }
     __syncthreads();
@@ -480,11 +530,23 @@ __global__ void __launch_bounds__(128) ra_particles(RaArgs args) {
         for (int c = 0; c < 3; c++) {
             x1[c] = args.st.x[c * cap + g_];
             v1[c] = args.st.v[c * cap + g_];
-            if (iter)
+            if (args.pend) {
+                double e[7];
+                for (int q = 0; q < 7; q++) {
+                    b[c][q] = args.ra.b[(c * 7 + q) * cap + g_];
+                    e[q] = args.ra.e[(c * 7 + q) * cap + g_];
+                }
+                ra_finish_comp(args.k, args.ptdid, args.pq, args.preset, args.reset, b[c], e,
+                               args.ra.a1[c * cap + g_], x1[c], v1[c]);
+                for (int q = 0; q < 7; q++) args.ra.e[(c * 7 + q) * cap + g_] = e[q];
+                args.st.x[c * cap + g_] = x1[c];
+                args.st.v[c * cap + g_] = v1[c];
+            } else if (iter) {
                 for (int q = 0; q < 7; q++) {
                     b[c][q] = args.reset ? 0.0 : args.ra.b[(c * 7 + q) * cap + g_];
                     if (args.reset) args.ra.e[(c * 7 + q) * cap + g_] = 0.0;
                 }
+            }
         }
         double pb[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
         if (NG) load_ngf(args.st, g_, pb);

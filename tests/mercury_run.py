@@ -111,14 +111,15 @@ def body(name="PARTICLE", mass=0.0, a=1.0, e=0.1, phase=0.0, **params):
 
 def write_case(path, small=None, big=None, *, algorithm="BS", stop=100.0, step=1.0, interval=7.3, tol=1e-12,
                backend="cpu", collisions=False, pn=False, rcen=0.005, rmax=100.0, periodic=100,
-               dump_every=5000, changeover=3.0, jcen=(0.0, 0.0, 0.0)):
+               dump_every=5000, changeover=3.0, jcen=(0.0, 0.0, 0.0), start=0.0, epoch=0.0):
     """A run directory from named bodies (body()), with the driver's parameters
-    as the reference's own test cases set them (tests/cases.py:20-48)."""
+    as the reference's own test cases set them (tests/cases.py:20-48); start
+    and epoch are the integration start time and the Big bodies' epoch."""
     path = Path(path)
     path.mkdir(parents=True, exist_ok=True)
     for name in ("message.in", "files.in"):
         shutil.copyfile(REF_DIR / name, path / name)
-    values = [algorithm, 0.0, stop, interval, step, tol, "no", "yes" if collisions else "no", "no", "days", "no",
+    values = [algorithm, start, stop, interval, step, tol, "no", "yes" if collisions else "no", "no", "days", "no",
               "high", "unused", "yes" if pn else "no", "no", rmax, rcen, 1.0, *jcen, "unused", "unused",
               changeover, dump_every, periodic]
     fmt = lambda v: f"{v:.17e}" if isinstance(v, float) else str(v)
@@ -129,7 +130,7 @@ def write_case(path, small=None, big=None, *, algorithm="BS", stop=100.0, step=1
                                                              False)):
         lines = [")O+_06 Initial data", " style = Cartesian"]
         if is_big:
-            lines.append(f" epoch = {0.0:.17e}")
+            lines.append(f" epoch = {epoch:.17e}")
         for o in objs:
             params = {"m": o["mass"], **o.get("params", {})}
             lines.append(o["name"] + " " + " ".join(f"{k}={v:.17e}" for k, v in params.items()))

@@ -91,3 +91,31 @@ def test_dropin_central_impact_and_ejection(tmp_path, algorithm):
     for key, value in params.items():
         assert out["small.dmp"]["KEEP"]["params"][key] == value
     assert any("IMPACT" in ln for ln in ev) and any("ESCAPE" in ln for ln in ev)
+
+
+def test_dropin_hybrid_encounter_at_large_epoch(tmp_path):
+    """Two Big bodies inside each other's changeover radius at an epoch of
+    1e15 d, where one ulp of the epoch (0.125 d) exceeds the encounter set's
+    BS2 substeps: both backends must measure the substeps' progress on the
+    encounter's local clock (mdt_hkce, mercury6_2.for:3883-3905).  The states
+    after 8 days agree with the same run at epoch 0, on the CPU and on the
+    device (the reference's case, tests/test_gpu_algorithms.py:49-65)."""
+    big = [R.body("A", mass=1e-4, a=1.0), R.body("B", mass=1e-4, a=1.005, phase=0.005)]
+    states = []
+    for k, (epoch, backend, binary) in enumerate([(0.0, "cpu", R.CPU_BIN), (1e15, "cpu", R.CPU_BIN),
+                                                  (1e15, "hip", R.HIP_BIN)]):
+        print(f"case: HYBRID encounter pair, epoch {epoch:g}, backend {backend}", flush=True)
+        p = R.write_case(tmp_path / str(k), small=[], big=big, algorithm="HYBRID", backend=backend,
+                         epoch=epoch, start=epoch, stop=epoch + 8.0, interval=8.0, step=0.5)
+        r = R.run(p, binary)
+        assert r.returncode == 0, r.stdout + r.stderr
+        if backend == "hip":
+            assert any("Execution backend: HIP" in ln for ln in R.info_lines(p)), R.info_lines(p)
+        states.append(R.read_dump_full(p, "big.dmp"))
+    for st in states[1:]:
+        assert st.keys() == states[0].keys()
+        for name in st:
+            for q in ("x", "v"):
+                d = np.max(np.abs(st[name][q] - states[0][name][q]))
+                print(f"{name} {q}: max |difference| from the epoch-0 run {d:.3e}", flush=True)
+                assert d < 1e-10
