@@ -33,6 +33,14 @@ constexpr int kScr = 15;               // screen block per planet
 enum { SX0 = 0, SX1, SY0, SY1, SP = 4, SV0 = 7, SV1 = 10, SRCE = 13, SRPH = 14 };
 
 __host__ __device__ inline int table_stride(int np) { return (kHdr + (kF + kScr) * np + 7) & ~7; }
+
+// A step table seen through the constant address space.  Uniform reads from
+// it compile to scalar loads into SGPRs, also in loops that store to global
+// memory (request and event records), where the compiler could not otherwise
+// show the table unclobbered and would read it with vector loads into VGPRs.
+// Only for tables that no thread writes while the kernel runs.
+typedef const __attribute__((address_space(4))) double ConstTab;
+__device__ __forceinline__ ConstTab* const_tab(const double* p) { return (ConstTab*)p; }
 __host__ __device__ inline int start_stride(int np) { return (kHdr + kF * np + 7) & ~7; }
 
 // Per-system launch parameters (device array, read with scalar loads).
@@ -204,11 +212,12 @@ __device__ __forceinline__ void obl_body(double mu, double j2, double j4, double
 // Ops = RangeOps (fast kernel): 1/sqrt(s2) is exact for s2 >= 2^-767; below
 // that s^-3 overflows, so both this and the IEEE form end in the non-finite
 // force fault.
-template <int NP = -1, class Ops = IeeeOps>
-__device__ __forceinline__ void particle_accel(const Sys& d, const double* __restrict__ T, int np,
+// TP: const double* or ConstTab* (scalar loads of the table).
+template <int NP = -1, class Ops = IeeeOps, class TP>
+__device__ __forceinline__ void particle_accel(const Sys& d, TP T, int np,
                                                double x, double y, double z,
                                                double& ax, double& ay, double& az) {
-    const double* F = T + kHdr;
+    const TP F = T + kHdr;
     const int npl = NP >= 0 ? NP : np;
     double a3x = 0.0, a3y = 0.0, a3z = 0.0;
 #pragma unroll
@@ -1098,8 +1107,11 @@ __device__ __forceinline__ void save_step(const State& st, int g, const double x
 // central gate needs only rr0 = |x|^2 and rv0 = v.x of the step start.
 constexpr int kEnds = 9;       // doubles per lane in the fast kernel's LDS
 
+// The box test of every planet is evaluated without branches into a bit mask
+// (the four comparisons of mce_stat, mercury6_2.for:1810-1811, all made); the
+// rare lanes with a hit then push one request per set bit in planet order.
 template <bool SCREEN, int NP>
-__device__ __forceinline__ void finish_step_deferred(const Sys& d, const double* __restrict__ T, int np, int g,
+__device__ __forceinline__ void finish_step_deferred(const Sys& d, ConstTab* T, int np, int g,
                                                      int jb, double pad, int s, int sid, const ReqSink& rs,
                                                      int* fault, double rr0, double rv0, const double* E, int ls,
                                                      Orbit& o, double& ax, double& ay, double& az) {
@@ -1110,14 +1122,20 @@ __device__ __forceinline__ void finish_step_deferred(const Sys& d, const double*
         xmin = xmin - pad; xmax = xmax + pad;
         ymin = ymin - pad; ymax = ymax + pad;
         const int npl = NP >= 0 ? NP : np;
-        const double* Sc = T + kHdr + kF * npl;
+        ConstTab* Sc = T + kHdr + kF * npl;
         // particles (jb >= nbig) test every planet; this kernel never sees Big bodies
+        unsigned long long hits = 0;
 #pragma unroll
         for (int p = 0; p < npl; p++) {
-            const double* B = Sc + kScr * p;
-            if (B[SX1] >= xmin && xmax >= B[SX0] && B[SY1] >= ymin && ymax >= B[SY0])
-                push_req(rs, 0, g, jb, s, p, sid, E[0], E[ls], E[2 * ls], E[3 * ls], E[4 * ls], E[5 * ls],
-                         o.x, o.y, o.z, o.vx, o.vy, o.vz);
+            ConstTab* B = Sc + kScr * p;
+            const bool hit = (B[SX1] >= xmin) & (xmax >= B[SX0]) & (B[SY1] >= ymin) & (ymax >= B[SY0]);
+            hits |= (unsigned long long)hit << p;
+        }
+        while (hits) {
+            const int p = __ffsll((long long)hits) - 1;
+            hits &= hits - 1;
+            push_req(rs, 0, g, jb, s, p, sid, E[0], E[ls], E[2 * ls], E[3 * ls], E[4 * ls], E[5 * ls],
+                     o.x, o.y, o.z, o.vx, o.vy, o.vz);
         }
     }
     particle_accel<NP, RangeOps>(d, T, np, o.x, o.y, o.z, ax, ay, az);
@@ -1184,7 +1202,7 @@ __device__ __forceinline__ void fast_tile(const ParticleArgs& args, double* lds_
         o = Orbit{args.st.x[g], args.st.x[cap + g], args.st.x[2 * cap + g],
                   args.st.v[g], args.st.v[cap + g], args.st.v[2 * cap + g]};
         if (args.reset) {
-            particle_accel<NP, RangeOps>(d, args.tab + d.start, np, o.x, o.y, o.z, ax, ay, az);
+            particle_accel<NP, RangeOps>(d, const_tab(args.tab + d.start), np, o.x, o.y, o.z, ax, ay, az);
             if (!isfinite(ax) || !isfinite(ay) || !isfinite(az)) atomicOr(args.st.fault, FAULT_FORCE);
         } else {
             ax = args.st.a[g]; ay = args.st.a[cap + g]; az = args.st.a[2 * cap + g];
@@ -1194,14 +1212,14 @@ __device__ __forceinline__ void fast_tile(const ParticleArgs& args, double* lds_
     const double pad = rce_j * 1.2;                 // mce_stat's box padding
     const int ls = blockDim.x;
     double* E = lds_ends + threadIdx.x;
-    const double* tab = args.tab + d.tab;
+    ConstTab* tab = const_tab(args.tab + d.tab);
     const int nsteps = live ? args.steps : 0;
     // drift_fast's exact range needs mu and |h| in [2^-100, 2^100]; a system
     // outside it parks every particle at its first step
     const bool in_range = d.mu >= 0x1p-100 && d.mu <= 0x1p100 && fabs(d.h) >= 0x1p-100 && fabs(d.h) <= 0x1p100;
     int s = 0;
     for (; s < nsteps; s++) {
-        const double* T = tab + (long long)s * d.ts;
+        ConstTab* T = tab + (long long)s * d.ts;
         double xs[6] = {o.x, o.y, o.z, o.vx, o.vy, o.vz};
         Orbit w = o;
         w.vx = w.vx + d.hby2 * ax;
