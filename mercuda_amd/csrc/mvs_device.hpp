@@ -342,7 +342,9 @@ __device__ __noinline__ void screen_pair(const double* __restrict__ X1, const do
 
 // Encounter screen of body jb against the Big bodies 1..min(jb,nbig)-1 of one
 // step's table (mce_stat loop order is restored by the host's sort).
-__device__ inline void screen_body(const double* __restrict__ T, int np, int jb, int nbig, double mj,
+// TP: const double* or ConstTab* (a table every lane of the wave shares).
+template <class TP>
+__device__ inline void screen_body(TP T, int np, int jb, int nbig, double mj,
                                    const Ends& e, double rce_j, double rphys_j, double h, double time_end,
                                    int step, int sys, const EventSink& es) {
     double xmin, xmax, ymin, ymax;
@@ -351,14 +353,14 @@ __device__ inline void screen_body(const double* __restrict__ T, int np, int jb,
     double temp = rce_j * 1.2;
     xmin = xmin - temp; xmax = xmax + temp;
     ymin = ymin - temp; ymax = ymax + temp;
-    const double* F = T + kHdr;
-    const double* Sc = F + kF * np;
+    const TP F = T + kHdr;
+    const TP Sc = F + kF * np;
     int lim = jb < nbig ? jb - 1 : np;
     for (int p = 0; p < lim; p++) {
-        const double* B = Sc + kScr * p;
-        if (B[SX1] >= xmin && xmax >= B[SX0] && B[SY1] >= ymin && ymax >= B[SY0]) {
-            screen_pair(F + kF * p, B, p + 1, jb, jb < nbig, F[kF * p + 3], mj, e, rce_j, rphys_j, h,
-                        time_end, step, sys, es);
+        const TP B = Sc + kScr * p;
+        if ((B[SX1] >= xmin) & (xmax >= B[SX0]) & (B[SY1] >= ymin) & (ymax >= B[SY0])) {
+            screen_pair((const double*)(F + kF * p), (const double*)B, p + 1, jb, jb < nbig, F[kF * p + 3], mj, e,
+                        rce_j, rphys_j, h, time_end, step, sys, es);
         }
     }
 }
@@ -1155,8 +1157,8 @@ __device__ __forceinline__ void finish_step_deferred(const Sys& d, ConstTab* T, 
 
 // NG: the kick adds the cometary acceleration ang (mdt_mvs's angf, evaluated
 // after the drift) as hby2 * (angf + ausr + a) with no user force.
-template <bool SCREEN, bool NG = false>
-__device__ __forceinline__ void finish_step(const Sys& d, const double* __restrict__ T, int np, int jb,
+template <bool SCREEN, bool NG = false, class TP>
+__device__ __forceinline__ void finish_step(const Sys& d, TP T, int np, int jb,
                                             double mj, double rce_j, double rphys_j, int s, int sid,
                                             const EventSink& es, int* fault, const double xs[6],
                                             const Orbit& k0, Orbit& o, double& ax, double& ay, double& az,
@@ -1401,8 +1403,13 @@ __global__ void __launch_bounds__(64) mvs_particles_general(ParticleArgs args) {
                     ngf_body(pg[0], pg[1], pg[2], pg[4], o.x, o.y, o.z, o.vx, o.vy, o.vz, ang[0], ang[1], ang[2],
                              args.st.fault);
                 if (SAVE) save_step(args.st, g, xs, k0, o);
-                finish_step<SCREEN, NG>(d, T, np, jb, mj, rce_j, rphys_j, s + first, UNIFORM ? 0 : pk.sys,
-                                        args.es, args.st.fault, xs, k0, o, ax, ay, az, ang);
+                // one system: the table is the same for every lane, read with scalar loads
+                if constexpr (UNIFORM)
+                    finish_step<SCREEN, NG>(d, const_tab(T), np, jb, mj, rce_j, rphys_j, s + first, 0, args.es,
+                                            args.st.fault, xs, k0, o, ax, ay, az, ang);
+                else
+                    finish_step<SCREEN, NG>(d, T, np, jb, mj, rce_j, rphys_j, s + first, pk.sys, args.es,
+                                            args.st.fault, xs, k0, o, ax, ay, az, ang);
             }
         };
         if (args.psteps > 0) run(args.ptab, a_from, args.psteps, args.step0 - args.psteps);

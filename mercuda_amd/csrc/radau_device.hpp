@@ -92,11 +92,12 @@ __device__ __forceinline__ void ra_finish_comp(const RaConst& k, double tdid, do
 // kernel's RangeOps give the IEEE bits for every squared distance sq_ok
 // admits and raise the force fault for the rest, where s^-3 (or s^2) has
 // overflowed in the IEEE form as well.
-template <bool PN, class Ops = IeeeOps>
-__device__ __forceinline__ void grav_small(const Sys& d, const double* __restrict__ T, int np, double light2,
+// TP: const double* or ConstTab* (scalar loads of a table the wave shares).
+template <bool PN, class Ops = IeeeOps, class TP>
+__device__ __forceinline__ void grav_small(const Sys& d, TP T, int np, double light2,
                                            double x, double y, double z, double vx, double vy, double vz,
                                            double& ax, double& ay, double& az, int* fault) {
-    const double* F = T + kHdr;
+    const TP F = T + kHdr;
     ax = 0.0; ay = 0.0; az = 0.0;
     for (int p = 0; p < np; p++) {
         double dx = x - F[kF * p], dy = y - F[kF * p + 1], dz = z - F[kF * p + 2];
@@ -550,8 +551,8 @@ __global__ void __launch_bounds__(128) ra_particles(RaArgs args) {
         }
         double pb[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
         if (NG) load_ngf(args.st, g_, pb);
-        grav_small<PN, RangeOps>(d, args.tab, np, args.light2, x1[0], x1[1], x1[2], v1[0], v1[1], v1[2], a1[0],
-                                 a1[1], a1[2], args.st.fault);
+        grav_small<PN, RangeOps>(d, const_tab(args.tab), np, args.light2, x1[0], x1[1], x1[2], v1[0], v1[1],
+                                 v1[2], a1[0], a1[1], a1[2], args.st.fault);
         if (NG)
             add_nongrav(d.ngflag, pb, 0.0, x1[0], x1[1], x1[2], v1[0], v1[1], v1[2], a1[0], a1[1], a1[2],
                         args.st.fault);
@@ -570,8 +571,8 @@ __global__ void __launch_bounds__(128) ra_particles(RaArgs args) {
                     for (int c = 0; c < 3; c++) v[c] = ra_vpred(s, b[c], a1[c], v1[c]);
                 }
                 double a[3];
-                grav_small<PN, RangeOps>(d, args.tab + (long long)entry * d.ts, np, args.light2, x[0], x[1], x[2],
-                                         v[0], v[1], v[2], a[0], a[1], a[2], args.st.fault);
+                grav_small<PN, RangeOps>(d, const_tab(args.tab + (long long)entry * d.ts), np, args.light2, x[0],
+                                         x[1], x[2], v[0], v[1], v[2], a[0], a[1], a[2], args.st.fault);
                 if (NG)
                     add_nongrav(d.ngflag, pb, 0.0, x[0], x[1], x[2], v[0], v[1], v[2], a[0], a[1], a[2],
                                 args.st.fault);
