@@ -216,32 +216,36 @@ struct Orbit { double x, y, z, vx, vy, vz; };
 // 0.6, |dm| <= 0.4.  The one unbounded numerator, u in es = u / (en a^2),
 // keeps the IEEE divide.  A state outside the gate returns false, so the
 // general kernel's IEEE chain takes it.
-__device__ __forceinline__ bool drift_fast(double mu, Orbit& o, double dt) {
+// rr: x.x of the state, x*x + y*y + z*z in that order (the caller's copy).
+// The quotients by r0 (2mu/r0, a/r0) and by en (dm/en, (s - x)/en) share
+// their divisor's reciprocal refinement (rn_ops.hpp).
+__device__ __forceinline__ bool drift_fast(double mu, Orbit& o, double dt, double rr) {
     using Ops = RangeOps;
     constexpr double rTwoPi = 1.0 / kTwoPi;
-    double rr = o.x * o.x + o.y * o.y + o.z * o.z;
     if (!(rr >= 0x1p-200 && rr <= 0x1p200)) return false;
     double r0 = Ops::sqrt(rr);
+    const double rr0 = rcp_refine(r0);
     double v0s = o.vx * o.vx + o.vy * o.vy + o.vz * o.vz;
     double u = o.x * o.vx + o.y * o.vy + o.z * o.vz;
-    double alpha = Ops::div(2.0 * mu, r0) - v0s;
+    double alpha = div_rn_r(2.0 * mu, r0, rr0) - v0s;
     if (!(alpha > 0.0)) return false;
     double a = Ops::div(mu, alpha);
     double asq = a * a;
     double en = Ops::sqrt(Ops::div(mu, a * asq));
+    const double ren = rcp_refine(en);
     double ec = 1.0 - Ops::div(r0, a);
     double es = u / (en * asq);
     double esq = ec * ec + es * es;
     double dm = dt * en - (double)((int)Ops::div_c(dt * en, kTwoPi, rTwoPi)) * kTwoPi;
-    double dtr = Ops::div(dm, en);
+    double dtr = div_rn_r(dm, en, ren);
     if ((dm * dm > 0.16) || (esq > 0.36) || !(esq * dm * dm < 0.0016)) return false;
     double xkep, s, c;
     kepmd<Ops>(dm, es, ec, xkep, s, c);
     double fchk = (xkep - ec * s + es * (1.0 - c) - dm);
     if (fchk * fchk > kDanbyB) return false;
     double fp = 1.0 - ec * c + es * s;
-    double f = Ops::div(a, r0) * (c - 1.0) + 1.0;
-    double g = dtr + Ops::div(s - xkep, en);
+    double f = div_rn_r(a, r0, rr0) * (c - 1.0) + 1.0;
+    double g = dtr + div_rn_r(s - xkep, en, ren);
     double fdot = -Ops::div(a, r0 * fp) * en * s;
     double gdot = Ops::div(c - 1., fp) + 1.;
     double nx = o.x * f + o.vx * g, ny = o.y * f + o.vy * g, nz = o.z * f + o.vz * g;

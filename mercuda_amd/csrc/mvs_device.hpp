@@ -212,27 +212,26 @@ __device__ __forceinline__ void obl_body(double mu, double j2, double j4, double
 // Ops = RangeOps (fast kernel): 1/sqrt(s2) is exact for s2 >= 2^-767; below
 // that s^-3 overflows, so both this and the IEEE form end in the non-finite
 // force fault.
-// TP: const double* or ConstTab* (scalar loads of the table).
-template <int NP = -1, class Ops = IeeeOps, class TP>
-__device__ __forceinline__ void particle_accel(const Sys& d, TP T, int np,
-                                               double x, double y, double z,
-                                               double& ax, double& ay, double& az) {
-    const TP F = T + kHdr;
-    const int npl = NP >= 0 ? NP : np;
-    double a3x = 0.0, a3y = 0.0, a3z = 0.0;
-#pragma unroll
-    for (int p = 0; p < npl; p++) {
-        double dx = x - F[kF * p + 0];
-        double dy = y - F[kF * p + 1];
-        double dz = z - F[kF * p + 2];
-        double s2 = dx * dx + dy * dy + dz * dz;
-        double s_1 = Ops::rcp(Ops::sqrt(s2));
-        double s_3 = s_1 * s_1 * s_1;
-        double faci = F[kF * p + 3] * s_3;
-        a3x = a3x - faci * dx;
-        a3y = a3y - faci * dy;
-        a3z = a3z - faci * dz;
-    }
+// The two parts of particle_accel: one planet's term of the sum (Fp: the
+// planet's force block) and the header terms added after the sum.
+template <class Ops, class TP>
+__device__ __forceinline__ void planet_term(TP Fp, double x, double y, double z, double& a3x, double& a3y,
+                                            double& a3z) {
+    double dx = x - Fp[0];
+    double dy = y - Fp[1];
+    double dz = z - Fp[2];
+    double s2 = dx * dx + dy * dy + dz * dz;
+    double s_1 = Ops::rcp(Ops::sqrt(s2));
+    double s_3 = s_1 * s_1 * s_1;
+    double faci = Fp[3] * s_3;
+    a3x = a3x - faci * dx;
+    a3y = a3y - faci * dy;
+    a3z = a3z - faci * dz;
+}
+
+template <class Ops, class TP>
+__device__ __forceinline__ void accel_finish(const Sys& d, TP T, double x, double y, double z, double a3x,
+                                             double a3y, double a3z, double& ax, double& ay, double& az) {
     ax = T[0] + a3x;
     ay = T[1] + a3y;
     az = T[2] + a3z;
@@ -243,6 +242,19 @@ __device__ __forceinline__ void particle_accel(const Sys& d, TP T, int np,
         ay = ay + (oy - T[4]);
         az = az + (oz - T[5]);
     }
+}
+
+// TP: const double* or ConstTab* (scalar loads of the table).
+template <int NP = -1, class Ops = IeeeOps, class TP>
+__device__ __forceinline__ void particle_accel(const Sys& d, TP T, int np,
+                                               double x, double y, double z,
+                                               double& ax, double& ay, double& az) {
+    const TP F = T + kHdr;
+    const int npl = NP >= 0 ? NP : np;
+    double a3x = 0.0, a3y = 0.0, a3z = 0.0;
+#pragma unroll
+    for (int p = 0; p < npl; p++) planet_term<Ops>(F + kF * p, x, y, z, a3x, a3y, a3z);
+    accel_finish<Ops>(d, T, x, y, z, a3x, a3y, a3z, ax, ay, az);
 }
 
 // mce_min (mercury6_2.for:1572-1624)
@@ -1106,53 +1118,71 @@ __device__ __forceinline__ void save_step(const State& st, int g, const double x
 // The states a request records stay out of registers: E (the lane's slots in
 // LDS, stride `ls`) holds the state after the first kick (E[0..5]) and the
 // step-start velocities (E[6..8]; the step-start position is E[0..2]); the
-// central gate needs only rr0 = |x|^2 and rv0 = v.x of the step start.
+// central gate needs only rr = |x|^2 and rv = v.x of the step start.
 constexpr int kEnds = 9;       // doubles per lane in the fast kernel's LDS
 
-// The box test of every planet is evaluated without branches into a bit mask
-// (the four comparisons of mce_stat, mercury6_2.for:1810-1811, all made); the
-// rare lanes with a hit then push one request per set bit in planet order.
+// The box test of every planet is evaluated without branches (the four
+// comparisons of mce_stat, mercury6_2.for:1810-1811, all made) into one flag;
+// the rare lanes with a hit then repeat the tests and push one request per
+// overlapping planet in planet order.
 template <bool SCREEN, int NP>
 __device__ __forceinline__ void finish_step_deferred(const Sys& d, ConstTab* T, int np, int g,
                                                      int jb, double pad, int s, int sid, const ReqSink& rs,
-                                                     int* fault, double rr0, double rv0, const double* E, int ls,
+                                                     int* fault, double& rr, double& rv, const double* E, int ls,
                                                      Orbit& o, double& ax, double& ay, double& az) {
+    double xmin = 0.0, xmax = 0.0, ymin = 0.0, ymax = 0.0;
     if (SCREEN) {
-        double xmin, xmax, ymin, ymax;
         box_axis(d.h, E[0], E[3 * ls], o.x, o.vx, xmin, xmax);
         box_axis(d.h, E[ls], E[4 * ls], o.y, o.vy, ymin, ymax);
         xmin = xmin - pad; xmax = xmax + pad;
         ymin = ymin - pad; ymax = ymax + pad;
-        const int npl = NP >= 0 ? NP : np;
-        ConstTab* Sc = T + kHdr + kF * npl;
-        // particles (jb >= nbig) test every planet; this kernel never sees Big bodies
-        unsigned long long hits = 0;
+    }
+    // per planet: the box test and the force term (each planet's table block
+    // is live for one iteration); the acceleration sum stays in planet order
+    const int npl = NP >= 0 ? NP : np;
+    ConstTab* F = T + kHdr;
+    ConstTab* Sc = F + kF * npl;
+    bool any = false;
+    double a3x = 0.0, a3y = 0.0, a3z = 0.0;
 #pragma unroll
-        for (int p = 0; p < npl; p++) {
+    for (int p = 0; p < npl; p++) {
+        if (SCREEN) {
+            // particles (jb >= nbig) test every planet; this kernel never sees Big bodies
             ConstTab* B = Sc + kScr * p;
-            const bool hit = (B[SX1] >= xmin) & (xmax >= B[SX0]) & (B[SY1] >= ymin) & (ymax >= B[SY0]);
-            hits |= (unsigned long long)hit << p;
+            any |= (B[SX1] >= xmin) & (xmax >= B[SX0]) & (B[SY1] >= ymin) & (ymax >= B[SY0]);
         }
-        while (hits) {
-            const int p = __ffsll((long long)hits) - 1;
-            hits &= hits - 1;
-            push_req(rs, 0, g, jb, s, p, sid, E[0], E[ls], E[2 * ls], E[3 * ls], E[4 * ls], E[5 * ls],
-                     o.x, o.y, o.z, o.vx, o.vy, o.vz);
+        planet_term<RangeOps>(F + kF * p, o.x, o.y, o.z, a3x, a3y, a3z);
+        // the scheduler keeps each planet's scalar loads in its own iteration
+        // (hoisting every planet's 16 SGPRs of table data spills SGPRs)
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    accel_finish<RangeOps>(d, T, o.x, o.y, o.z, a3x, a3y, a3z, ax, ay, az);
+    if (SCREEN) {
+        if (any) {
+#pragma unroll 1
+            for (int p = 0; p < npl; p++) {
+                ConstTab* B = Sc + kScr * p;
+                if ((B[SX1] >= xmin) & (xmax >= B[SX0]) & (B[SY1] >= ymin) & (ymax >= B[SY0]))
+                    push_req(rs, 0, g, jb, s, p, sid, E[0], E[ls], E[2 * ls], E[3 * ls], E[4 * ls], E[5 * ls],
+                             o.x, o.y, o.z, o.vx, o.vy, o.vz);
+            }
         }
     }
-    particle_accel<NP, RangeOps>(d, T, np, o.x, o.y, o.z, ax, ay, az);
     if (!isfinite(ax) || !isfinite(ay) || !isfinite(az)) atomicOr(fault, FAULT_FORCE);
     o.vx = o.vx + d.hby2 * ax;
     o.vy = o.vy + d.hby2 * ay;
     o.vz = o.vz + d.hby2 * az;
+    // x.x and v.x at the step's end: the next step's start values
+    const double rr1 = o.x * o.x + o.y * o.y + o.z * o.z;
     if (SCREEN) {
         double rcen2 = d.rcen * d.rcen;
-        double rr1 = o.x * o.x + o.y * o.y + o.z * o.z;
         double rv1 = o.vx * o.x + o.vy * o.y + o.vz * o.z;
-        if ((rv0 * d.h <= 0.0 && rv1 * d.h >= 0.0) || fmin(rr0, rr1) <= rcen2)
+        if ((rv * d.h <= 0.0 && rv1 * d.h >= 0.0) || fmin(rr, rr1) <= rcen2)
             push_req(rs, 1, g, jb, s, 0, sid, E[0], E[ls], E[2 * ls], E[6 * ls], E[7 * ls], E[8 * ls],
                      o.x, o.y, o.z, o.vx, o.vy, o.vz);
+        rv = rv1;
     }
+    rr = rr1;
 }
 
 // NG: the kick adds the cometary acceleration ang (mdt_mvs's angf, evaluated
@@ -1219,28 +1249,35 @@ __device__ __forceinline__ void fast_tile(const ParticleArgs& args, double* lds_
     // drift_fast's exact range needs mu and |h| in [2^-100, 2^100]; a system
     // outside it parks every particle at its first step
     const bool in_range = d.mu >= 0x1p-100 && d.mu <= 0x1p100 && fabs(d.h) >= 0x1p-100 && fabs(d.h) <= 0x1p100;
+    // x.x and v.x of the state at each step's start (the central gate's r0^2
+    // and v.x, the drift's r0^2), carried from the previous step's end
+    double rr = o.x * o.x + o.y * o.y + o.z * o.z;
+    double rv = SCREEN ? o.vx * o.x + o.vy * o.y + o.vz * o.z : 0.0;
     int s = 0;
     for (; s < nsteps; s++) {
         ConstTab* T = tab + (long long)s * d.ts;
         double xs[6] = {o.x, o.y, o.z, o.vx, o.vy, o.vz};
+        // With screening, the step-start state also waits in LDS (E[0..2] and
+        // E[6..8]), so the kick and drift update o in place and a parking lane
+        // reloads it from there: no register copy of the state is kept across
+        // the drift's exits.  Without screening, w is the working copy.
         Orbit w = o;
         w.vx = w.vx + d.hby2 * ax;
         w.vy = w.vy + d.hby2 * ay;
         w.vz = w.vz + d.hby2 * az;
         Orbit k0 = w;
-        double rr0 = 0.0, rv0 = 0.0;
         if (SCREEN) {
-            rr0 = o.x * o.x + o.y * o.y + o.z * o.z;
-            rv0 = o.vx * o.x + o.vy * o.y + o.vz * o.z;
             E[0] = w.x; E[ls] = w.y; E[2 * ls] = w.z;
             E[3 * ls] = w.vx; E[4 * ls] = w.vy; E[5 * ls] = w.vz;
             E[6 * ls] = o.vx; E[7 * ls] = o.vy; E[8 * ls] = o.vz;
+            o = w;
         }
-        bool ok = in_range && drift_fast(d.mu, w, d.h);
+        bool ok = in_range && drift_fast(d.mu, SCREEN ? o : w, d.h, rr);
         // wave-ballot compaction: lanes whose drift leaves the kepmd branch
         // park here and finish in the general kernel
         unsigned long long mask = __ballot(!ok);
         if (!ok) {
+            if (SCREEN) o = Orbit{E[0], E[ls], E[2 * ls], E[6 * ls], E[7 * ls], E[8 * ls]};
             int leader = __ffsll((long long)mask) - 1;
             int lane = __lane_id();
             const int par = *args.park.par;
@@ -1252,9 +1289,9 @@ __device__ __forceinline__ void fast_tile(const ParticleArgs& args, double* lds_
             args.park.slow[g] = 1;
             break;
         }
-        o = w;
+        if (!SCREEN) o = w;
         if (SAVE) save_step(args.st, g, xs, k0, o);
-        finish_step_deferred<SCREEN, NP>(d, T, np, g, jb, pad, s, sid, args.req, args.st.fault, rr0, rv0, E, ls,
+        finish_step_deferred<SCREEN, NP>(d, T, np, g, jb, pad, s, sid, args.req, args.st.fault, rr, rv, E, ls,
                                          o, ax, ay, az);
     }
     if (!live) return;

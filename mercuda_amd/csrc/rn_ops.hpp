@@ -40,6 +40,23 @@ __device__ __forceinline__ double div_rn(double a, double b) {
     return __builtin_fma(t, r, q);
 }
 
+// div_rn split at its divisor: rcp_refine(b) is the Newton-refined reciprocal
+// that div_rn computes from b alone, div_rn_r(a, b, r) the rest, so several
+// quotients by one divisor share the refinement with div_rn's bits.
+__device__ __forceinline__ double rcp_refine(double b) {
+    double r = __builtin_amdgcn_rcp(b);
+    double e = __builtin_fma(-b, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-b, r, 1.0);
+    return __builtin_fma(r, e, r);
+}
+
+__device__ __forceinline__ double div_rn_r(double a, double b, double r) {
+    double q = a * r;
+    double t = __builtin_fma(-b, q, a);
+    return __builtin_fma(t, r, q);
+}
+
 __device__ __forceinline__ double rcp_rn(double b) {
     double r = __builtin_amdgcn_rcp(b);
     double e = __builtin_fma(-b, r, 1.0);
