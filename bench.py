@@ -163,12 +163,13 @@ def cpu_hybrid(nparticles=20_000, nsteps=1):
     return nparticles * nsteps / sec, sec
 
 
-def ensemble_ratio(hip, steps, spl, block):
+def ensemble_ratio(hip, steps, spl, block, replays=7):
     """Batched ensemble (512 systems, 2..12 planets, log-U[100, 5e4] particles,
     dt 4 d) against one 8-planet system with the same total particle count,
     both through mercury_hip_advance with screening on.  Each side is
-    autotuned (steps per launch, workgroup size) on its own state and runs
-    as a replayed hipGraph; the timed call is a replay."""
+    autotuned (steps per launch, workgroup size, particles per lane) on its
+    own state and runs as a replayed hipGraph; each side's rate is the median
+    of `replays` timed replays of `steps` steps (min and max reported)."""
     from mercuda_amd import systems
     log("ensemble: generating 512 systems")
     ens = systems.ensemble(seed=4)
@@ -178,9 +179,12 @@ def ensemble_ratio(hip, steps, spl, block):
         advance(0.0, 16)                          # first call (acceleration reset)
         advance(16 * 4.0, steps)                  # captures the steady shape
         hip.graph_stats()
-        tic = time.perf_counter()
-        advance((16 + steps) * 4.0, steps)        # replay
-        return time.perf_counter() - tic
+        secs = []
+        for r in range(replays):
+            tic = time.perf_counter()
+            advance((16 + (r + 1) * steps) * 4.0, steps)        # replay
+            secs.append(time.perf_counter() - tic)
+        return sorted(secs)
 
     hip.configure(1)
     hip.ensemble_upload(ens)
@@ -198,14 +202,21 @@ def ensemble_ratio(hip, steps, spl, block):
     hip.set_graph(True)
     t_one = timed(lambda t, n: hip.advance(t, one["h"], n, one["rcen"], screen=True))
     hip.set_graph(False)
-    r_ens = ntot * steps / t_ens
-    r_one = ntot * steps / t_one
-    return {"systems": len(ens), "particles": ntot, "steps": steps,
+    med = replays // 2
+    r_ens = ntot * steps / t_ens[med]
+    r_one = ntot * steps / t_one[med]
+    # spread of the ratio: slowest ensemble replay over fastest single one, and the reverse
+    lo = t_one[0] / t_ens[-1]
+    hi = t_one[-1] / t_ens[0]
+    return {"systems": len(ens), "particles": ntot, "steps_per_replay": steps, "replays": replays,
             "ensemble_particle_steps_per_s": round(r_ens, 1),
             "single_system_particle_steps_per_s": round(r_one, 1),
             "ratio": round(r_ens / r_one, 3),
-            "ensemble_tuning": {"steps_per_launch": tune_e["steps_per_launch"], "block": tune_e["block"]},
-            "single_tuning": {"steps_per_launch": tune_1["steps_per_launch"], "block": tune_1["block"]},
+            "ratio_range": [round(lo, 3), round(hi, 3)],
+            "ensemble_replay_ms": [round(1e3 * t, 3) for t in t_ens],
+            "single_replay_ms": [round(1e3 * t, 3) for t in t_one],
+            "ensemble_tuning": {k: tune_e[k] for k in ("steps_per_launch", "block", "particles_per_lane")},
+            "single_tuning": {k: tune_1[k] for k in ("steps_per_launch", "block", "particles_per_lane")},
             "graph": hip.graph_stats()}
 
 
@@ -471,6 +482,7 @@ def main():
     ap.add_argument("--particles", type=int, default=1_000_000)
     ap.add_argument("--spl", type=int, default=32, help="steps fused per particle launch")
     ap.add_argument("--block", type=int, default=128)
+    ap.add_argument("--ppl", type=int, default=1, help="particles per lane of the fast particle kernel")
     ap.add_argument("--no-screen", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-procs", type=int, default=16)
@@ -503,14 +515,14 @@ def main():
     sysd = systems.config_large(seed=2, nparticles=args.particles)
     hip.configure(1)
     hip.upload_system(sysd)
-    hip.set_tuning(steps_per_launch=args.spl, block_size=args.block)
+    hip.set_tuning(steps_per_launch=args.spl, block_size=args.block, particles_per_lane=args.ppl)
     screen = not args.no_screen
     h = sysd["h"]
-    spl, block = args.spl, args.block
+    spl, block, ppl = args.spl, args.block, args.ppl
     if args.autotune:
-        log("autotuning steps per launch and workgroup size (128-step probes, state restored)")
+        log("autotuning steps per launch, workgroup size and particles per lane (128-step probes, state restored)")
         tune = hip.autotune(h=h, probe_steps=128)
-        spl, block = tune["steps_per_launch"], tune["block"]
+        spl, block, ppl = tune["steps_per_launch"], tune["block"], tune["particles_per_lane"]
         log(f"autotuner: {tune}")
     log(f"warmup {args.warmup} steps")
     hip.advance(0.0, h, args.warmup, sysd["rcen"], screen=screen)
@@ -566,7 +578,7 @@ def main():
     if want - set(names):
         raise SystemExit(f"unknown extras {sorted(want - set(names))}; choose from {names}")
     if "ensemble" in want:
-        extras["ensemble"] = ensemble_ratio(hip, steps=64, spl=spl, block=block)
+        extras["ensemble"] = ensemble_ratio(hip, steps=128, spl=spl, block=block)
     if "adjoint" in want:
         log("adjoint: orbit-fitting workload")
         extras["adjoint"] = adjoint_cost(hip)
@@ -655,7 +667,7 @@ def main():
         "config": {"workload": f"MVS, 8 planets + {npart} massless particles, dt 4 d, "
                                f"encounter screening {'on' if screen else 'off'}",
                    "particles": npart, "planets": 8, "dt_days": h,
-                   "steps_per_launch": spl, "block": block,
+                   "steps_per_launch": spl, "block": block, "particles_per_lane": ppl,
                    "tuning": "autotuned" if args.autotune else "fixed", "parallelism": "1 GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,

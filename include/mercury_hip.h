@@ -117,8 +117,10 @@ int mercury_hip_ensemble_download(double *x, double *v);
  * (system, step, min index, max index). */
 int mercury_hip_advance_events(const MercuryHipEvent **events, int *count);
 /* Launch shape: steps fused per particle launch (>= 1), particle workgroup
- * size (64..256, multiple of 64).  0 keeps the current value. */
-int mercury_hip_set_tuning(int steps_per_launch, int block_size);
+ * size (64..256, multiple of 64), particles per lane of the fast particle
+ * kernel (1..8; each lane takes that many particles of its tile one after the
+ * other).  0 keeps the current value. */
+int mercury_hip_set_tuning(int steps_per_launch, int block_size, int particles_per_lane);
 /* Kernel timing with HIP events on the launch streams: enable = 1 clears and
  * starts recording every kernel kind, 2 the fast particle kernel only (no
  * events on the planet and slow-set streams), 0 stops; the read returns the
@@ -161,12 +163,13 @@ int mercury_hip_set_graph(int enable);
 /* Graphs captured and replays made since the library was loaded. */
 int mercury_hip_graph_stats(int *captures, int *replays);
 /* Autotuner (no counterpart in the reference): times probe_steps MVS steps of
- * the uploaded system or ensemble for every steps-per-launch in {8,16,32,64}
- * and workgroup size in {64,128,256} (h: the step of a single system; ensemble
- * members use their own), keeps the fastest (mercury_hip_set_tuning) and
- * leaves the state as it was.  Returns the choice and its particle-steps/s. */
+ * the uploaded system or ensemble for every steps-per-launch in {8,16,32,64},
+ * workgroup size in {64,128,256} and particles per lane in {1,2,4} (h: the
+ * step of a single system; ensemble members use their own), keeps the fastest
+ * (mercury_hip_set_tuning) and leaves the state as it was.  Returns the choice
+ * and its particle-steps/s. */
 int mercury_hip_autotune(double h, int probe_steps, int *steps_per_launch, int *block_size,
-                         double *particle_steps_per_s);
+                         int *particles_per_lane, double *particle_steps_per_s);
 
 /* Allocation-free force (no counterpart in the reference): for callers that
  * cannot allocate device memory -- code under HIP graph capture, tight serving

@@ -138,6 +138,7 @@ struct Context {
     hipStream_t stream = nullptr;
     int steps_per_launch = 32;
     int block = 128;
+    int ppl = 1;          // fast kernel: particles per lane
     int gen_per_wave = 4;
     int64_t parked = 0;
     unsigned long long* work = nullptr;   // device particle-step counters (timing mode)
@@ -218,7 +219,7 @@ __global__ void noalloc_setup(Sys d, unsigned char* scr, double* a, long long ld
 
 // What a captured advance depends on: replay only when all of it is unchanged.
 struct GraphKey {
-    int nsteps, screen, reset, spl, block, gpw, ntiles, nsys, ev_cap, req_cap, park_cap, n, npart;
+    int nsteps, screen, reset, spl, block, ppl, gpw, ntiles, nsys, ev_cap, req_cap, park_cap, n, npart;
     unsigned gen;
     long long cap, slot;
     const void* ptr[11];
@@ -386,15 +387,16 @@ void ensure_systems(int nsys) {
     ctx.sys_valid = false;
 }
 
-// Workgroup tiles over the particles of every system; a tile never spans two
-// systems, so each workgroup reads one system's descriptor and table.
+// Workgroup tiles over the particles of every system, block * ppl particles
+// each; a tile never spans two systems, so each workgroup reads one system's
+// descriptor and table.
 void build_tiles() {
     int b = ctx.block;
     std::vector<int2> t;
     int npart = 0;
     for (size_t s = 0; s < ctx.systems.size(); s++) {
         const auto& hs = ctx.systems[s];
-        for (int j = hs.base + hs.nbig; j < hs.base + hs.n; j += b) t.push_back(make_int2((int)s, j));
+        for (int j = hs.base + hs.nbig; j < hs.base + hs.n; j += b * ctx.ppl) t.push_back(make_int2((int)s, j));
         npart += hs.n - hs.nbig;
     }
     dalloc(ctx.dtiles, std::max<size_t>(t.size(), 1));
@@ -625,6 +627,7 @@ void mvs_enqueue(double time, double h, int nsteps, bool reset, bool screen, boo
         if (k >= kRing) HIPCHECK(hipStreamWaitEvent(c.sp, c.ev_done[slot], 0));
         ParticleArgs args{c.dsys, c.dtiles, st, tab, es, park_list(k, false), req_sink(k),
                           c.timing.on ? c.work : nullptr, steps, rs0 ? 1 : 0, screen ? 1 : 0, save ? 1 : 0, done};
+        args.ppl = c.ppl;
         // one system follows the caller's clock; ensemble members carry theirs on the device
         const bool set_time = !device_time && ((single && !c.ensemble) || k == 0);
         PlanetArgs pa{c.dsys, st, tab, c.tcur, es, t, set_time ? 1 : 0, steps, rs0 ? 1 : 0, screen ? 1 : 0,
@@ -718,6 +721,7 @@ void mvs_run(double time, double h, int nsteps, bool reset, bool screen, bool sa
     GraphKey k;
     std::memset(&k, 0, sizeof(k));
     k.nsteps = nsteps; k.screen = screen; k.reset = reset; k.spl = c.steps_per_launch; k.block = c.block;
+    k.ppl = c.ppl;
     k.gpw = c.gen_per_wave; k.ntiles = c.ntiles; k.nsys = nsys; k.ev_cap = c.ev_cap; k.req_cap = c.req_cap;
     k.park_cap = c.park_cap; k.cap = c.cap; k.slot = c.slot_doubles;
     k.n = c.n; k.npart = c.npart; k.gen = c.upload_gen;
@@ -2225,15 +2229,16 @@ int mercury_hip_advance_events(const MercuryHipEvent** events, int* count) {
     return 0;
 }
 
-int mercury_hip_set_tuning(int steps_per_launch, int block_size) {
+int mercury_hip_set_tuning(int steps_per_launch, int block_size, int particles_per_lane) {
     try {
-        if (steps_per_launch < 0 || block_size < 0) throw std::runtime_error("invalid tuning");
+        if (steps_per_launch < 0 || block_size < 0 || particles_per_lane < 0) throw std::runtime_error("invalid tuning");
+        if (block_size && (block_size % 64 || block_size > 256))
+            throw std::runtime_error("block size must be 64..256, multiple of 64");
+        if (particles_per_lane > kMaxPpl) throw std::runtime_error("particles per lane must be 1..8");
         if (steps_per_launch) ctx.steps_per_launch = steps_per_launch;
-        if (block_size) {
-            if (block_size % 64 || block_size > 256) throw std::runtime_error("block size must be 64..256, multiple of 64");
-            ctx.block = block_size;
-            if (ctx.have_state) build_tiles();
-        }
+        if (block_size) ctx.block = block_size;
+        if (particles_per_lane) ctx.ppl = particles_per_lane;
+        if ((block_size || particles_per_lane) && ctx.have_state) build_tiles();
         return 0;
     } catch (const std::exception& e) { return fail(e); }
 }
@@ -2314,7 +2319,7 @@ int mercury_hip_graph_stats(int* captures, int* replays) {
     return 0;
 }
 
-int mercury_hip_autotune(double h, int probe_steps, int* spl_out, int* block_out, double* rate_out) {
+int mercury_hip_autotune(double h, int probe_steps, int* spl_out, int* block_out, int* ppl_out, double* rate_out) {
     Context& c = ctx;
     double* save = nullptr;
     double* tsave = nullptr;
@@ -2335,7 +2340,7 @@ int mercury_hip_autotune(double h, int probe_steps, int* spl_out, int* block_out
             HIPCHECK(hipMemcpy(save + (size_t)q * 3 * c.cap, arr[q], sizeof(double) * 3 * c.cap, hipMemcpyDeviceToDevice));
         HIPCHECK(hipMemcpy(tsave, c.tcur, sizeof(double) * nsys, hipMemcpyDeviceToDevice));
         const bool hist = c.history;
-        const int spl0 = c.steps_per_launch, block0 = c.block;
+        const int spl0 = c.steps_per_launch, block0 = c.block, ppl0 = c.ppl;
         auto restore = [&]() {
             for (int q = 0; q < 3; q++)
                 HIPCHECK(hipMemcpy(arr[q], save + (size_t)q * 3 * c.cap, sizeof(double) * 3 * c.cap,
@@ -2346,13 +2351,16 @@ int mercury_hip_autotune(double h, int probe_steps, int* spl_out, int* block_out
         };
         const int spls[4] = {8, 16, 32, 64};
         const int blocks[3] = {64, 128, 256};
+        const int ppls[3] = {1, 2, 4};
         double best = -1.0;
-        int bspl = spl0, bblock = block0;
+        int bspl = spl0, bblock = block0, bppl = ppl0;
         const double rc = single ? c.systems[0].rcen : 0.0;
-        for (int bi = 0; bi < 3; bi++)
+        for (int pi = 0; pi < 3; pi++)
+          for (int bi = 0; bi < 3; bi++)
             for (int si = 0; si < 4; si++) {
                 c.steps_per_launch = spls[si];
                 c.block = blocks[bi];
+                c.ppl = ppls[pi];
                 build_tiles();
                 double sec = 0.0;
                 for (int rep = 0; rep < 2; rep++) {   // the first run warms up (and captures a graph)
@@ -2366,10 +2374,11 @@ int mercury_hip_autotune(double h, int probe_steps, int* spl_out, int* block_out
                 }
                 check_fault("MVS force failed");
                 double rate = (double)c.npart * probe_steps / sec;
-                if (rate > best) { best = rate; bspl = spls[si]; bblock = blocks[bi]; }
+                if (rate > best) { best = rate; bspl = spls[si]; bblock = blocks[bi]; bppl = ppls[pi]; }
             }
         c.steps_per_launch = bspl;
         c.block = bblock;
+        c.ppl = bppl;
         build_tiles();
         restore();
         HIPCHECK(hipDeviceSynchronize());
@@ -2377,6 +2386,7 @@ int mercury_hip_autotune(double h, int probe_steps, int* spl_out, int* block_out
         dfree(tsave);
         *spl_out = bspl;
         *block_out = bblock;
+        if (ppl_out) *ppl_out = bppl;
         *rate_out = best;
         return 0;
     } catch (const std::exception& e) {

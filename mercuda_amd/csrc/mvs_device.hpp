@@ -26,6 +26,7 @@
 namespace mhip {
 
 constexpr int kMaxBig = 64;            // Big bodies per system, central body included
+constexpr int kMaxPpl = 8;             // particles per lane of the fast kernel
 constexpr int kHdr = 8;                // table header: a0tp[3], acen[3], time_end, pad
 constexpr int kF = 4;                  // force block per planet: X1[3], m
 constexpr int kScr = 15;               // screen block per planet
@@ -1090,6 +1091,9 @@ struct ParticleArgs {
     // 0 for the call's first launch)
     const double* ptab = nullptr;
     int psteps = 0;
+    // fast kernel: particles per lane; lane t of a tile takes its particles
+    // t, t + blockDim.x, ... (ppl of them) one after the other
+    int ppl = 1;
 };
 
 // Adds each lane's completed particle-steps into *counter, one atomic per wave.
@@ -1213,15 +1217,12 @@ __device__ __forceinline__ void finish_step(const Sys& d, TP T, int np, int jb,
                       o.vx, o.vy, o.vz, jb, s, sid, es);
 }
 
-// One workgroup's launch: S steps of its tile.  NP >= 0: the planet count as a
+// One particle's launch: S steps of particle g.  NP >= 0: the planet count as a
 // compile-time constant (the force and box loops unroll and the table reads
 // batch into wide scalar loads); NP < 0: read at run time.
 template <bool SCREEN, bool SAVE, int NP>
-__device__ __forceinline__ void fast_tile(const ParticleArgs& args, double* lds_ends) {
-    const int2 tile = args.tiles[blockIdx.x];
-    const int sid = tile.x;
-    const Sys d = args.sys[sid];
-    const int g = tile.y + threadIdx.x;            // global body index
+__device__ __forceinline__ void fast_particle(const ParticleArgs& args, double* lds_ends, int sid, const Sys& d,
+                                              int g) {
     const int jb = g - d.base;                     // local body index
     // lanes past the system's end, and slow-set members, idle to the end
     const bool live = jb < d.n && !args.park.slow[g];
@@ -1299,6 +1300,18 @@ __device__ __forceinline__ void fast_tile(const ParticleArgs& args, double* lds_
     args.st.x[g] = o.x; args.st.x[cap + g] = o.y; args.st.x[2 * cap + g] = o.z;
     args.st.v[g] = o.vx; args.st.v[cap + g] = o.vy; args.st.v[2 * cap + g] = o.vz;
     args.st.a[g] = ax; args.st.a[cap + g] = ay; args.st.a[2 * cap + g] = az;
+}
+
+// One workgroup's tile: ppl particles per lane, one after the other (g is the
+// global body index; lanes past the system's end idle).
+template <bool SCREEN, bool SAVE, int NP>
+__device__ __forceinline__ void fast_tile(const ParticleArgs& args, double* lds_ends) {
+    const int2 tile = args.tiles[blockIdx.x];
+    const int sid = tile.x;
+    const Sys d = args.sys[sid];
+#pragma unroll 1
+    for (int k = 0; k < args.ppl; k++)
+        fast_particle<SCREEN, SAVE, NP>(args, lds_ends, sid, d, tile.y + threadIdx.x + k * (int)blockDim.x);
 }
 
 // The fast particle kernel.  A tile never spans two systems, so each

@@ -60,7 +60,7 @@ SIGNATURES = {
     "mercury_hip_free": (None, []),
     "mercury_hip_advance": (C.c_int, [C.c_double, C.c_double, C.c_int, C.c_double, C.c_int, _i64p]),
     "mercury_hip_advance_events": (C.c_int, [C.POINTER(C.POINTER(MercuryHipEvent)), _ip]),
-    "mercury_hip_set_tuning": (C.c_int, [C.c_int, C.c_int]),
+    "mercury_hip_set_tuning": (C.c_int, [C.c_int, C.c_int, C.c_int]),
     "mercury_hip_timing": (C.c_int, [C.c_int]),
     "mercury_hip_timing_read": (C.c_int, [_dp, _ip, _dp, _ip, _dp, _ip]),
     "mercury_hip_parked": (C.c_int64, []),
@@ -71,7 +71,7 @@ SIGNATURES = {
     "mercury_hip_ensemble_download": (C.c_int, [_dp, _dp]),
     "mercury_hip_set_graph": (C.c_int, [C.c_int]),
     "mercury_hip_graph_stats": (C.c_int, [_ip, _ip]),
-    "mercury_hip_autotune": (C.c_int, [C.c_double, C.c_int, _ip, _ip, _dp]),
+    "mercury_hip_autotune": (C.c_int, [C.c_double, C.c_int, _ip, _ip, _ip, _dp]),
     "mercury_hip_adjoint_forward": (C.c_int, [C.c_int, C.c_double, C.c_int, _dp, _dp]),
     "mercury_hip_adjoint_backward": (C.c_int, [_dp, _dp, _dp, _dp, _dp]),
     "mercury_hip_tangent": (C.c_int, [_dp, _dp, _dp, _dp, _dp]),
@@ -195,8 +195,8 @@ class MercuryHip:
         self._ok(self.L.mercury_hip_advance_events(C.byref(ptr), C.byref(cnt)), "advance_events")
         return [(ptr[k].system, ptr[k].step) + event_tuple(ptr[k].e) for k in range(cnt.value)]
 
-    def set_tuning(self, steps_per_launch=0, block_size=0):
-        self._ok(self.L.mercury_hip_set_tuning(steps_per_launch, block_size), "set_tuning")
+    def set_tuning(self, steps_per_launch=0, block_size=0, particles_per_lane=0):
+        self._ok(self.L.mercury_hip_set_tuning(steps_per_launch, block_size, particles_per_lane), "set_tuning")
 
     def timing(self, enable=True, fast_only=False):
         """Kernel timing on (every kernel kind, or the fast particle kernel
@@ -263,10 +263,11 @@ class MercuryHip:
         return {"captures": a.value, "replays": b.value}
 
     def autotune(self, h=0.0, probe_steps=32):
-        spl = C.c_int(0); blk = C.c_int(0); rate = C.c_double(0.0)
-        self._ok(self.L.mercury_hip_autotune(float(h), int(probe_steps), C.byref(spl), C.byref(blk), C.byref(rate)),
-                 "autotune")
-        return {"steps_per_launch": spl.value, "block": blk.value, "particle_steps_per_s": rate.value}
+        spl = C.c_int(0); blk = C.c_int(0); ppl = C.c_int(0); rate = C.c_double(0.0)
+        self._ok(self.L.mercury_hip_autotune(float(h), int(probe_steps), C.byref(spl), C.byref(blk), C.byref(ppl),
+                                             C.byref(rate)), "autotune")
+        return {"steps_per_launch": spl.value, "block": blk.value, "particles_per_lane": ppl.value,
+                "particle_steps_per_s": rate.value}
 
     # ---- HYBRID (algorithm 10) and the compact encounter context ----
     def hybrid_begin(self, h, rcrit, flag, cap, x, v):

@@ -64,8 +64,10 @@ def test_host_side_errors_without_device(lib, capfd):
     x = np.zeros(3)
     assert lib.mercury_hip_download(x.ctypes.data_as(C.POINTER(C.c_double)),
                                     x.ctypes.data_as(C.POINTER(C.c_double)), 0) != 0
-    assert lib.mercury_hip_set_tuning(0, 512) != 0
-    assert lib.mercury_hip_set_tuning(0, 96) != 0
+    assert lib.mercury_hip_set_tuning(0, 512, 0) != 0
+    assert lib.mercury_hip_set_tuning(0, 96, 0) != 0
+    assert lib.mercury_hip_set_tuning(0, 0, 9) != 0      # particles per lane 1..8
+    assert lib.mercury_hip_set_tuning(0, 0, -1) != 0
     jc = np.zeros(3)
     jp = jc.ctypes.data_as(C.POINTER(C.c_double))
     dummy = C.c_void_p(1)

@@ -81,3 +81,44 @@ def test_ensemble_of_one_system(hip, oracle):
     ex, _ = rel_err(x[1:], xo[1:])
     ev_, _ = rel_err(v[1:], vo[1:])
     assert ex <= REL_TOL and ev_ <= REL_TOL, (ex, ev_)
+
+
+def test_particles_per_lane_bitwise(hip):
+    """Lanes that take 2 or 4 particles of their tile one after the other give
+    the states and event lists of one particle per lane, bit for bit: on one
+    system whose particle count fills no whole tile, and on an ensemble."""
+    one = systems.config_small(seed=21, nparticles=1000)
+    one["rce"][1:one["nbig"]] *= 4.0
+    ens = systems.ensemble(seed=22, nsystems=7, nmin=50, nmax=1500)
+    nsteps = 48
+
+    def run_one(ppl):
+        hip.configure(1)
+        hip.upload_system(one)
+        hip.set_tuning(steps_per_launch=16, block_size=128, particles_per_lane=ppl)
+        hip.advance(0.0, one["h"], nsteps, one["rcen"], screen=True)
+        x, v = hip.download()
+        return x, v, sorted(hip.advance_events())
+
+    def run_ens(ppl):
+        hip.configure(1)
+        hip.ensemble_upload(ens)
+        hip.set_tuning(steps_per_launch=16, block_size=64, particles_per_lane=ppl)
+        hip.ensemble_advance(0.0, nsteps, screen=True)
+        return hip.ensemble_download(), sorted(hip.advance_events())
+
+    try:
+        base_one = run_one(1)
+        base_ens = run_ens(1)
+        for ppl in (2, 4):
+            say(f"case: particles per lane {ppl}, one system of 1000 particles and an ensemble of {len(ens)}")
+            x, v, ev = run_one(ppl)
+            assert np.array_equal(x, base_one[0]) and np.array_equal(v, base_one[1]), ppl
+            assert ev == base_one[2], ppl
+            states, evs = run_ens(ppl)
+            for k in range(len(ens)):
+                assert np.array_equal(states[k][0], base_ens[0][k][0]), (ppl, k)
+                assert np.array_equal(states[k][1], base_ens[0][k][1]), (ppl, k)
+            assert evs == base_ens[1], ppl
+    finally:
+        hip.set_tuning(steps_per_launch=32, block_size=128, particles_per_lane=1)

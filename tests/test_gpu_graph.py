@@ -89,13 +89,14 @@ def test_autotune_leaves_state_and_results(hip):
     choice = hip.autotune(probe_steps=16)
     say(f"autotune choice {choice}")
     assert choice["steps_per_launch"] in (8, 16, 32, 64) and choice["block"] in (64, 128, 256)
+    assert choice["particles_per_lane"] in (1, 2, 4)
     after = hip.download()
     assert np.array_equal(ref[0], after[0]) and np.array_equal(ref[1], after[1])
     hip.ensemble_advance(80.0, 50, screen=True)
     tuned = hip.download()
     hip.configure(1)
     hip.ensemble_upload(ens)
-    hip.set_tuning(steps_per_launch=16, block_size=128)
+    hip.set_tuning(steps_per_launch=16, block_size=128, particles_per_lane=1)
     hip.ensemble_advance(0.0, 20, screen=True)
     hip.ensemble_advance(80.0, 50, screen=True)
     plain = hip.download()
