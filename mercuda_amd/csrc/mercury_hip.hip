@@ -557,8 +557,10 @@ std::vector<int> chunk_plan(int nsteps, int smax) {
 // A last general launch finishes the last fast kernel's parks.  The
 // latency-bound chains (planets, slow set) thus run beside the fast kernel
 // instead of before or after it; the planet stream runs up to kRing launches
-// ahead.  Last, the streams join the main stream and call_end writes the
-// host's status words.  Enqueue only (no host synchronisation): usable under
+// ahead, and its launches are enqueued two launches ahead of the others (a
+// call's first kernels would otherwise wait on the host enqueueing them).
+// Last, the streams join the main stream and call_end writes the host's
+// status words.  Enqueue only (no host synchronisation): usable under
 // stream capture.  device_time: every planet launch reads the systems' clocks
 // from the device (set beforehand) instead of taking the call's time as an
 // argument; this is the captured form, whose planet stream first joins the
@@ -579,8 +581,13 @@ void mvs_enqueue(double time, double h, int nsteps, bool reset, bool screen, boo
     int* const pers = c.dcnt + kCtlPers;
     CallBeginArgs cb{c.dslow, c.dleave, count2, par, pers, c.dcnt + kCtlLeave, c.dcnt + kCtlSnap, 2 * kRing,
                      c.devcount, c.fault};
-    hipLaunchKernelGGL(call_begin, dim3(1), dim3(256), 0, c.sp, cb);
-    HIPCHECK(hipGetLastError());
+    // one system: the first planet launch does call_begin's work itself (its
+    // one workgroup before the planets); an ensemble's planet workgroups run
+    // concurrently, so call_begin precedes them as a launch of its own
+    if (!single) {
+        hipLaunchKernelGGL(call_begin, dim3(1), dim3(256), 0, c.sp, cb);
+        HIPCHECK(hipGetLastError());
+    }
     const int npart = c.npart;
     // The slow-set and screening kernels grid-stride over counts only the device
     // knows (a few hundred slow-set members and tens of thousands of requests per
@@ -614,6 +621,34 @@ void mvs_enqueue(double time, double h, int nsteps, bool reset, bool screen, boo
         hipLaunchKernelGGL(screen_requests, dim3(rblocks), dim3(256), 0, c.sg, ra);
         HIPCHECK(hipGetLastError());
     };
+    // steps done before each launch
+    std::vector<int> done_at(nk + 1, 0);
+    for (int k = 0; k < nk; k++) done_at[k + 1] = done_at[k] + plan[k];
+    // Planet launch j on the planet stream.  They are enqueued up to
+    // kPlanAhead launches ahead of the particle launches, so the planet wave
+    // never waits on the host enqueueing the launches between them; a ring
+    // slot's reuse waits on an event recorded kRing - kPlanAhead iterations
+    // earlier.
+    constexpr int kPlanAhead = 2;
+    static_assert(kPlanAhead + 1 <= kRing, "a slot's release is recorded before its reuse is enqueued");
+    auto enqueue_planets = [&](int j) {
+        const int slot = j % kRing;
+        if (j >= kRing) HIPCHECK(hipStreamWaitEvent(c.sp, c.ev_done[slot], 0));
+        const bool rs0 = reset && j == 0;
+        // one system follows the caller's clock; ensemble members carry theirs on the device
+        const bool set_time = !device_time && ((single && !c.ensemble) || j == 0);
+        PlanetArgs pa{c.dsys, st, slot_tab(j), c.tcur, es, t, set_time ? 1 : 0, plan[j], rs0 ? 1 : 0,
+                      screen ? 1 : 0, save ? 1 : 0, c.dcnt + kCtlReq + slot, 1, done_at[j]};
+        if (single && j == 0) { pa.begin = 1; pa.cb = cb; }
+        hipEvent_t t0 = nullptr;
+        timed_launch_begin(c.sp, t0, K_PLANET);
+        hipLaunchKernelGGL(mvs_planets, dim3(nsys), dim3(64), 0, c.sp, pa);
+        HIPCHECK(hipGetLastError());
+        timed_launch_end(K_PLANET, c.sp, t0);
+        for (int q = 0; q < plan[j]; q++) t = t + h;
+        HIPCHECK(hipEventRecord(c.ev_plan[slot], c.sp));
+    };
+    for (int j = 0; j < std::min(nk, kPlanAhead); j++) enqueue_planets(j);
     int prev_done = 0;
     for (int k = 0; k < nk; k++) {
         const int steps = plan[k];
@@ -622,22 +657,10 @@ void mvs_enqueue(double time, double h, int nsteps, bool reset, bool screen, boo
         const bool rs0 = reset && k == 0;
         const bool last = k + 1 == nk;
         int* snap = c.dcnt + kCtlSnap + slot;
-        int* reqcount = c.dcnt + kCtlReq + slot;
         hipEvent_t t0 = nullptr;
-        if (k >= kRing) HIPCHECK(hipStreamWaitEvent(c.sp, c.ev_done[slot], 0));
         ParticleArgs args{c.dsys, c.dtiles, st, tab, es, park_list(k, false), req_sink(k),
                           c.timing.on ? c.work : nullptr, steps, rs0 ? 1 : 0, screen ? 1 : 0, save ? 1 : 0, done};
         args.ppl = c.ppl;
-        // one system follows the caller's clock; ensemble members carry theirs on the device
-        const bool set_time = !device_time && ((single && !c.ensemble) || k == 0);
-        PlanetArgs pa{c.dsys, st, tab, c.tcur, es, t, set_time ? 1 : 0, steps, rs0 ? 1 : 0, screen ? 1 : 0,
-                      save ? 1 : 0, reqcount, 1, done};
-        timed_launch_begin(c.sp, t0, K_PLANET);
-        hipLaunchKernelGGL(mvs_planets, dim3(nsys), dim3(64), 0, c.sp, pa);
-        HIPCHECK(hipGetLastError());
-        timed_launch_end(K_PLANET, c.sp, t0);
-        for (int q = 0; q < steps; q++) t = t + h;
-        HIPCHECK(hipEventRecord(c.ev_plan[slot], c.sp));
         if (parts) {
             // the slow set over this launch's steps, beside its fast kernel
             ParticleArgs ga = args;
@@ -676,6 +699,7 @@ void mvs_enqueue(double time, double h, int nsteps, bool reset, bool screen, boo
         }
         prev_done = done;
         done += steps;
+        if (k + kPlanAhead < nk) enqueue_planets(k + kPlanAhead);
     }
     if (parts && nk > 0) {
         // the last fast kernel's parks finish their block; then its screening details

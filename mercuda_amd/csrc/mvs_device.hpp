@@ -448,6 +448,39 @@ __device__ __forceinline__ void central_check(double mcen, double mj, double rce
 
 // ---------------------------------------------------------------------------
 // Planet prologue: one 64-lane workgroup per system, lane p owns Big body p+1.
+// Start of a call (one workgroup): members that left the slow set lose their
+// flag, the per-launch snapshot and request counts clear, the slow-set lists
+// rotate, the event count and fault flags clear.
+struct CallBeginArgs {
+    unsigned char* slow;
+    const int* leave;
+    int* count2;       // entries of list 0 and list 1
+    int* par;          // current list
+    int* pers;         // kept members that start the call
+    int* nleave;
+    int* zero;         // per-launch snapshots and request counts
+    int nzero;
+    int* devcount;
+    int* fault;
+};
+
+__device__ __forceinline__ void call_begin_body(const CallBeginArgs& a) {
+    const int nl = *a.nleave;
+    for (int i = threadIdx.x; i < nl; i += blockDim.x) a.slow[a.leave[i]] = 0;
+    for (int i = threadIdx.x; i < a.nzero; i += blockDim.x) a.zero[i] = 0;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int par = *a.par ^ 1;
+        *a.par = par;
+        *a.pers = a.count2[par];
+        a.count2[1 - par] = 0;
+        *a.nleave = 0;
+        *a.devcount = 0;
+        *a.fault = 0;
+    }
+    __syncthreads();
+}
+
 struct PlanetArgs {
     const Sys* sys;
     State st;
@@ -463,6 +496,10 @@ struct PlanetArgs {
     int* zero;        // counters cleared by the launch (park and request counts), may be null
     int nzero;
     int step0;        // steps of the call done before this launch (event step numbers)
+    // a call's first launch for one system starts with call_begin's work
+    // (its single workgroup does it before anything else)
+    int begin = 0;
+    CallBeginArgs cb{};
 };
 
 // NPL: the planet count as a compile-time constant (loops unroll, so the
@@ -545,14 +582,20 @@ __device__ inline void h2j_lanes(int np_rt, const double* m, const double* jdx, 
     double mu = m[1] * V[0], mv = m[1] * V[1], mw = m[1] * V[2];
 #pragma unroll
     for (int q = 1; q < (NPL >= 0 ? NPL : kMaxBig); q++) {
-        if (q >= p) break;
-        double mb = m[q + 1];
-        mx = mx + mb * X[3 * q + 0];
-        my = my + mb * X[3 * q + 1];
-        mz = mz + mb * X[3 * q + 2];
-        mu = mu + mb * V[3 * q + 0];
-        mv = mv + mb * V[3 * q + 1];
-        mw = mw + mb * V[3 * q + 2];
+        if (NPL < 0 && q >= p) break;
+        // with a compile-time count every body's terms load unconditionally
+        // (in bounds) and only the sum is predicated, so the loads batch
+        const double mb = m[q + 1];
+        const double x0 = X[3 * q + 0], x1 = X[3 * q + 1], x2 = X[3 * q + 2];
+        const double v0 = V[3 * q + 0], v1 = V[3 * q + 1], v2 = V[3 * q + 2];
+        if (q < p) {
+            mx = mx + mb * x0;
+            my = my + mb * x1;
+            mz = mz + mb * x2;
+            mu = mu + mb * v0;
+            mv = mv + mb * v1;
+            mw = mw + mb * v2;
+        }
     }
     double temp = jdx[p];
     XJ[3 * p + 0] = X[3 * p + 0] - temp * mx;
@@ -578,14 +621,18 @@ __device__ inline void j2h_lanes(int np_rt, const double* jhx, const double* XJ,
     double mu = temp * VJ[0], mv = temp * VJ[1], mw = temp * VJ[2];
 #pragma unroll
     for (int q = 1; q < (NPL >= 0 ? NPL : kMaxBig); q++) {
-        if (q >= p) break;
-        temp = jhx[q];
-        mx = mx + temp * XJ[3 * q + 0];
-        my = my + temp * XJ[3 * q + 1];
-        mz = mz + temp * XJ[3 * q + 2];
-        mu = mu + temp * VJ[3 * q + 0];
-        mv = mv + temp * VJ[3 * q + 1];
-        mw = mw + temp * VJ[3 * q + 2];
+        if (NPL < 0 && q >= p) break;
+        const double tq = jhx[q];
+        const double x0 = XJ[3 * q + 0], x1 = XJ[3 * q + 1], x2 = XJ[3 * q + 2];
+        const double v0 = VJ[3 * q + 0], v1 = VJ[3 * q + 1], v2 = VJ[3 * q + 2];
+        if (q < p) {
+            mx = mx + tq * x0;
+            my = my + tq * x1;
+            mz = mz + tq * x2;
+            mu = mu + tq * v0;
+            mv = mv + tq * v1;
+            mw = mw + tq * v2;
+        }
     }
     X[3 * p + 0] = XJ[3 * p + 0] + mx;
     X[3 * p + 1] = XJ[3 * p + 1] + my;
@@ -598,13 +645,39 @@ __device__ inline void j2h_lanes(int np_rt, const double* jhx, const double* XJ,
 // Big-body part of mfo_mvs (mercury6_2.for:4945-5025, 5062-5070) for the
 // planets in LDS.  Lane p computes body p+1; serial recurrences run in lane 0.
 // Writes A (Big accelerations) and the table header terms a0tp and acen.
+// NPL >= 0: the Big-Big terms are formed lane-parallel, one unordered pair
+// (i < j) per lane: s^-3 once from dx = X_j - X_i (the difference the Fortran's
+// pair loop forms for either body), both bodies' products into pt (LDS,
+// [body][partner][3]); each body then sums its partners' products in the
+// Fortran's order.  NPL < 0: each lane forms its own body's terms.
 template <int NPL = -1>
 __device__ inline void big_forces(const Sys& d, int np_rt, const double* m, const double* minside,
                                   const double* X, const double* XJ, double* A, double* t0, double* t2,
-                                  double* a1, double* aob, double* hdr) {
+                                  double* a1, double* aob, double* hdr, double* pt = nullptr) {
     const int np = NPL >= 0 ? NPL : np_rt;
     int lane = threadIdx.x;
     int p = lane;
+    if (NPL >= 2) {
+        constexpr int NPAIR = NPL * (NPL - 1) / 2;
+        for (int w = lane; w < NPAIR; w += 64) {
+            int i = 0, rem = w;
+            while (rem >= NPL - 1 - i) { rem -= NPL - 1 - i; i++; }
+            const int j = i + 1 + rem;
+            double dx = X[3 * j] - X[3 * i];
+            double dy = X[3 * j + 1] - X[3 * i + 1];
+            double dz = X[3 * j + 2] - X[3 * i + 2];
+            double s2 = dx * dx + dy * dy + dz * dz;
+            double s_1 = 1.0 / sqrt(s2);
+            double s_3 = s_1 * s_1 * s_1;
+            double fi = m[i + 1] * s_3;        // body j's term from partner i (before it)
+            double fj = m[j + 1] * s_3;        // body i's term from partner j (after it)
+            double* Tj = pt + 3 * (j * NPL + i);
+            double* Ti = pt + 3 * (i * NPL + j);
+            Tj[0] = fi * dx; Tj[1] = fi * dy; Tj[2] = fi * dz;
+            Ti[0] = fj * dx; Ti[1] = fj * dy; Ti[2] = fj * dz;
+        }
+        __syncthreads();
+    }
     if (p < np) {
         // a1, and the terms summed serially into a0 and a2, for Fortran k = p+2 >= 3
         if (p >= 1) {
@@ -636,24 +709,17 @@ __device__ inline void big_forces(const Sys& d, int np_rt, const double* m, cons
         // partners before it (subtracted), then those after it (added)
         double a3[3] = {0.0, 0.0, 0.0};
         const double* xb = X + 3 * p;
-        if (NPL >= 0) {
-            // every partner's term is independent of the others until the sum;
-            // the body's own slot is computed and discarded
+        if (NPL >= 2) {
+            // the partners before this body subtract, those after it add
 #pragma unroll
             for (int q = 0; q < NPL; q++) {
-                const bool before = q < p;
-                double dx = before ? xb[0] - X[3 * q] : X[3 * q] - xb[0];
-                double dy = before ? xb[1] - X[3 * q + 1] : X[3 * q + 1] - xb[1];
-                double dz = before ? xb[2] - X[3 * q + 2] : X[3 * q + 2] - xb[2];
-                double s2 = dx * dx + dy * dy + dz * dz;
-                double s_1 = 1.0 / sqrt(s2);
-                double s_3 = s_1 * s_1 * s_1;
-                double fac = m[q + 1] * s_3;
-                if (q != p) {
-                    if (before) { a3[0] = a3[0] - fac * dx; a3[1] = a3[1] - fac * dy; a3[2] = a3[2] - fac * dz; }
-                    else { a3[0] = a3[0] + fac * dx; a3[1] = a3[1] + fac * dy; a3[2] = a3[2] + fac * dz; }
-                }
+                const double* Tq = pt + 3 * (p * NPL + q);
+                const double tx = Tq[0], ty = Tq[1], tz = Tq[2];
+                if (q < p) { a3[0] = a3[0] - tx; a3[1] = a3[1] - ty; a3[2] = a3[2] - tz; }
+                else if (q > p) { a3[0] = a3[0] + tx; a3[1] = a3[1] + ty; a3[2] = a3[2] + tz; }
             }
+        } else if (NPL >= 0) {
+            // one planet: no Big-Big pair
         } else {
             for (int i = 0; i < p; i++) {
                 double dx = xb[0] - X[3 * i], dy = xb[1] - X[3 * i + 1], dz = xb[2] - X[3 * i + 2];
@@ -699,8 +765,9 @@ __device__ inline void big_forces(const Sys& d, int np_rt, const double* m, cons
             double a2[3] = {0.0, 0.0, 0.0};
 #pragma unroll
             for (int q = 1; q < (NPL >= 0 ? NPL : kMaxBig); q++) {
-                if (q > lane) break;
-                for (int c = 0; c < 3; c++) a2[c] = a2[c] + t2[3 * q + c];
+                if (NPL < 0 && q > lane) break;
+                const double u0 = t2[3 * q], u1 = t2[3 * q + 1], u2 = t2[3 * q + 2];
+                if (q <= lane) { a2[0] = a2[0] + u0; a2[1] = a2[1] + u1; a2[2] = a2[2] + u2; }
             }
             const int q = lane;
             for (int c = 0; c < 3; c++) {
@@ -761,6 +828,7 @@ struct PlanetLds {
     double X0[3 * kMaxBig], V0[3 * kMaxBig], t0[3 * kMaxBig], t2[3 * kMaxBig];
     double a1[3 * kMaxBig], aob[3 * kMaxBig], hdr[8];
     double box[4 * kMaxBig];   // this step's padded x-y boxes (screening)
+    double pt[3 * 12 * 12];    // Big-Big pair products (big_forces, up to 12 planets unrolled)
 };
 
 template <int NPL>
@@ -807,7 +875,7 @@ __device__ __forceinline__ void planets_run(const PlanetArgs& args, PlanetLds& L
     if (args.reset) {
         if (lane == 0) h2j_serial<NPL>(np, m, jdx, X, V, XJ, VJ);
         __syncthreads();
-        big_forces<NPL>(d, np, m, minside, X, XJ, A, t0, t2, a1, aob, hdr);
+        big_forces<NPL>(d, np, m, minside, X, XJ, A, t0, t2, a1, aob, hdr, L.pt);
         if (lane < np && !(isfinite(A[3 * lane]) && isfinite(A[3 * lane + 1]) && isfinite(A[3 * lane + 2])))
             atomicOr(args.st.fault, FAULT_FORCE);
         double* St = args.tab + d.start;
@@ -843,7 +911,11 @@ __device__ __forceinline__ void planets_run(const PlanetArgs& args, PlanetLds& L
         __syncthreads();
         if (p < np) {
             Orbit o{XJ[3 * p], XJ[3 * p + 1], XJ[3 * p + 2], VJ[3 * p], VJ[3 * p + 1], VJ[3 * p + 2]};
-            drift_one(gm[p], o, h);
+            // the kepmd branch through drift_fast (same bits, shorter divides
+            // and roots) where its gate admits the drift, else the whole chain
+            const double mu = gm[p];
+            const bool gate = mu >= 0x1p-100 && mu <= 0x1p100 && fabs(h) >= 0x1p-100 && fabs(h) <= 0x1p100;
+            if (!(gate && drift_fast(mu, o, h, o.x * o.x + o.y * o.y + o.z * o.z))) drift_one(mu, o, h);
             XJ[3 * p] = o.x; XJ[3 * p + 1] = o.y; XJ[3 * p + 2] = o.z;
             VJ[3 * p] = o.vx; VJ[3 * p + 1] = o.vy; VJ[3 * p + 2] = o.vz;
         }
@@ -878,7 +950,7 @@ __device__ __forceinline__ void planets_run(const PlanetArgs& args, PlanetLds& L
                     args.st.v1[c * cap + b0 + 1 + p] = V[3 * p + c];
                 }
         }
-        big_forces<NPL>(d, np, m, minside, X, XJ, A, t0, t2, a1, aob, hdr);
+        big_forces<NPL>(d, np, m, minside, X, XJ, A, t0, t2, a1, aob, hdr, L.pt);
         if (lane < np && !(isfinite(A[3 * lane]) && isfinite(A[3 * lane + 1]) && isfinite(A[3 * lane + 2])))
             atomicOr(args.st.fault, FAULT_FORCE);
         if (lane < 8) T[lane] = (lane < 6) ? hdr[lane] : (lane == 6 ? time_end : 0.0);
@@ -915,6 +987,7 @@ __device__ __forceinline__ void planets_run(const PlanetArgs& args, PlanetLds& L
 __global__ void __launch_bounds__(64) mvs_planets(PlanetArgs args) {
     __builtin_amdgcn_s_setprio(3);
     __shared__ PlanetLds L;
+    if (args.begin) call_begin_body(args.cb);
     switch (args.sys[blockIdx.x].nbig - 1) {
         case 0: planets_run<0>(args, L); break;
         case 1: planets_run<1>(args, L); break;
@@ -1587,34 +1660,7 @@ __global__ void snap_count(const int* count2, const int* par, int* dst) {
 // Start of a call, before its first planet launch: bodies that left the slow
 // set in the previous call rejoin the fast kernel, the list of kept members
 // becomes current, and the call's counters, event count and fault flag clear.
-struct CallBeginArgs {
-    unsigned char* slow;
-    const int* leave;
-    int* count2;       // entries of list 0 and list 1
-    int* par;          // current list
-    int* pers;         // kept members that start the call
-    int* nleave;
-    int* zero;         // per-launch snapshots and request counts
-    int nzero;
-    int* devcount;
-    int* fault;
-};
-
-__global__ void __launch_bounds__(256) call_begin(CallBeginArgs a) {
-    const int nl = *a.nleave;
-    for (int i = threadIdx.x; i < nl; i += blockDim.x) a.slow[a.leave[i]] = 0;
-    for (int i = threadIdx.x; i < a.nzero; i += blockDim.x) a.zero[i] = 0;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const int par = *a.par ^ 1;
-        *a.par = par;
-        *a.pers = a.count2[par];
-        a.count2[1 - par] = 0;
-        *a.nleave = 0;
-        *a.devcount = 0;
-        *a.fault = 0;
-    }
-}
+__global__ void __launch_bounds__(256) call_begin(CallBeginArgs a) { call_begin_body(a); }
 
 // End of a call, after every stream's work: the counts the host reads, written
 // into pinned host memory so that no copy follows the call.
