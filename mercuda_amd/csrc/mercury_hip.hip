@@ -553,8 +553,10 @@ std::vector<int> chunk_plan(int nsteps, int smax) {
 //                    first finishes launch k-1's parks): beside launch k's fast
 //                    kernel; then launch k-1's deferred screening details
 //   main stream:     the fast particle kernel once the launch's planet tables
-//                    exist, then the slow-set count snapshot
-// A last general launch finishes the last fast kernel's parks.  The
+//                    exist, then the slow-set count snapshot (not after the
+//                    call's last one: the finishing launch reads the count)
+// A last general launch finishes the last fast kernel's parks, beside that
+// kernel's screening details on the main stream.  The
 // latency-bound chains (planets, slow set) thus run beside the fast kernel
 // instead of before or after it; the planet stream runs up to kRing launches
 // ahead, and its launches are enqueued two launches ahead of the others (a
@@ -615,10 +617,10 @@ void mvs_enqueue(double time, double h, int nsteps, bool reset, bool screen, boo
         launch_general(screen, save, single, dim3(gblocks), dim3(64), c.sg, a);
         timed_launch_end(K_GENERAL, c.sg, g0);
     };
-    auto screen_details = [&](int k, int step0) {
+    auto screen_details = [&](int k, int step0, hipStream_t stream) {
         if (!screen) return;
         RequestArgs ra{c.dsys, st, slot_tab(k), es, req_sink(k), step0};
-        hipLaunchKernelGGL(screen_requests, dim3(rblocks), dim3(256), 0, c.sg, ra);
+        hipLaunchKernelGGL(screen_requests, dim3(rblocks), dim3(256), 0, stream, ra);
         HIPCHECK(hipGetLastError());
     };
     // steps done before each launch
@@ -683,7 +685,7 @@ void mvs_enqueue(double time, double h, int nsteps, bool reset, bool screen, boo
             general(ga);
             if (k > 0) {
                 // launch k-1's screening details; its table slot is free after them
-                screen_details(k - 1, prev_done);
+                screen_details(k - 1, prev_done, c.sg);
                 HIPCHECK(hipEventRecord(c.ev_done[(k - 1) % kRing], c.sg));
             }
             HIPCHECK(hipStreamWaitEvent(c.sm, gate, 0));
@@ -691,7 +693,10 @@ void mvs_enqueue(double time, double h, int nsteps, bool reset, bool screen, boo
             launch_fast(screen, save, single ? c.systems[0].nbig - 1 : -1, dim3(c.ntiles), dim3(c.block), c.sm,
                         args);
             timed_launch_end(K_PARTICLE, c.sm, t0);
-            hipLaunchKernelGGL(snap_count, dim3(1), dim3(64), 0, c.sm, count2, par, snap);
+            // the next launch's slow set reads the count of this launch's
+            // parks from a snapshot (the fast kernel after it appends more);
+            // the call's finishing launch reads it directly
+            if (!last) hipLaunchKernelGGL(snap_count, dim3(1), dim3(64), 0, c.sm, count2, par, snap);
             HIPCHECK(hipEventRecord(c.ev_fast[slot], c.sm));
         } else {
             HIPCHECK(hipStreamWaitEvent(c.sg, c.ev_plan[slot], 0));
@@ -702,15 +707,18 @@ void mvs_enqueue(double time, double h, int nsteps, bool reset, bool screen, boo
         if (k + kPlanAhead < nk) enqueue_planets(k + kPlanAhead);
     }
     if (parts && nk > 0) {
-        // the last fast kernel's parks finish their block; then its screening details
+        // the last fast kernel's parks finish their block (general stream, the
+        // live count); beside them, its screening details (main stream, behind
+        // the fast kernel that wrote the requests)
         ParticleArgs fa{c.dsys, c.dtiles, st, slot_tab(nk), es, park_list(nk, true), req_sink(nk - 1),
                         c.timing.on ? c.work : nullptr, 0, 0, screen ? 1 : 0, save ? 1 : 0, done};
+        fa.park.live = 1;
         fa.ptab = slot_tab(nk - 1);
         fa.psteps = plan[nk - 1];
         HIPCHECK(hipStreamWaitEvent(c.sg, c.ev_fast[(nk - 1) % kRing], 0));
         general(fa);
-        screen_details(nk - 1, prev_done);
-        HIPCHECK(hipEventRecord(c.ev_done[(nk - 1) % kRing], c.sg));
+        screen_details(nk - 1, prev_done, c.sm);
+        HIPCHECK(hipEventRecord(c.ev_done[(nk - 1) % kRing], c.sm));
     }
     // the planet and general streams join the main stream; then the status words
     HIPCHECK(hipEventRecord(c.ev_plan[kRing], c.sp));

@@ -137,6 +137,8 @@ struct ParkList {
     int per_wave;            // general kernel: entries per 64-lane wave (spreads the latency-bound
                              // slow set over more SIMDs)
     int last;                // the call's last launch: members stay or leave
+    int live = 0;            // no fast kernel runs beside this launch: the current list's count is read
+                             // directly (in place of snap)
 };
 
 // Screening work the fast particle kernel defers: a box overlap with planet p
@@ -448,9 +450,10 @@ __device__ __forceinline__ void central_check(double mcen, double mj, double rce
 
 // ---------------------------------------------------------------------------
 // Planet prologue: one 64-lane workgroup per system, lane p owns Big body p+1.
-// Start of a call (one workgroup): members that left the slow set lose their
-// flag, the per-launch snapshot and request counts clear, the slow-set lists
-// rotate, the event count and fault flags clear.
+// Start of a call, before its first planet launch (one workgroup): bodies
+// that left the slow set in the previous call rejoin the fast kernel, the
+// per-launch snapshots and request counts clear, the list of kept members
+// becomes current, and the event count and fault flags clear.
 struct CallBeginArgs {
     unsigned char* slow;
     const int* leave;
@@ -1446,7 +1449,7 @@ __global__ void __launch_bounds__(64) mvs_particles_general(ParticleArgs args) {
     const int par = *P.par;
     const Park* list = P.buf + (long long)par * P.cap;
     Park* next = P.buf + (long long)(1 - par) * P.cap;
-    const int count = min(P.chunk == 0 ? *P.pers : *P.snap, P.cap);
+    const int count = min(P.chunk == 0 ? *P.pers : (P.live ? P.count2[par] : *P.snap), P.cap);
     const long long cap = args.st.cap;
     const int per_wave = P.per_wave;
     const int lane = threadIdx.x;          // one 64-lane wave per workgroup
@@ -1657,9 +1660,8 @@ __global__ void snap_count(const int* count2, const int* par, int* dst) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *dst = count2[*par];
 }
 
-// Start of a call, before its first planet launch: bodies that left the slow
-// set in the previous call rejoin the fast kernel, the list of kept members
-// becomes current, and the call's counters, event count and fault flag clear.
+// call_begin_body as a launch of its own (ensembles: their planet launches
+// run several workgroups at once).
 __global__ void __launch_bounds__(256) call_begin(CallBeginArgs a) { call_begin_body(a); }
 
 // End of a call, after every stream's work: the counts the host reads, written
