@@ -37,10 +37,10 @@ FLOP_PER_PSTEP = 351.6
 BYTES_PER_PARTICLE_LAUNCH = 152.0
 # HBM bytes per particle per launch of the fast kernel as rocprofv3 measured
 # them on this workload (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, the factors
-# a 256 MiB device copy reads on the same counters; 1M particles, the 69
-# launches of the default timed run): profiles/mvs_particles_fast_1M_hbm_traffic.md.
-# It includes about one write-back per launch of the kernel's register spill.
-TRAFFIC_PER_PARTICLE_LAUNCH = 193.0
+# a 256 MiB device copy reads on the same counters; 1M particles, the 63
+# 32-step launches of the default timed run): profiles/mvs_particles_fast_1M_hbm_traffic.md.
+# Beyond the state it includes the screening requests the kernel appends.
+TRAFFIC_PER_PARTICLE_LAUNCH = 177.5
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 
