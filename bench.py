@@ -41,6 +41,18 @@ BYTES_PER_PARTICLE_LAUNCH = 152.0
 # 32-step launches of the default timed run): profiles/mvs_particles_fast_1M_hbm_traffic.md.
 # Beyond the state it includes the screening requests the kernel appends.
 TRAFFIC_PER_PARTICLE_LAUNCH = 177.5
+# Algorithmic FP64 operations per particle per steady RA15 sequence (2
+# iterations, 15 force evaluations; tol 1e-12, central 1PN, 8 planets), from
+# the counted restatement of mdt_ra15 checked bit for bit against the oracle
+# (`python -m oracle.opcount`; oracle.opcount.ra15_measure).
+FLOP_PER_RA15_PSEQ = 5550.0
+# Algorithmic HBM bytes per particle per sequence of the RA15 particle kernel:
+# read x, v, b, e, a1 (51 doubles), write them back (51 doubles).
+BYTES_PER_RA15_PSEQ = 816.0
+# HBM bytes per particle per sequence of the RA15 particle kernel as rocprofv3
+# measured them (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, 1M particles, the
+# steady sequences): profiles/ra15_1M_kernel_trace.md.
+TRAFFIC_PER_RA15_PSEQ = 882.3
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 
@@ -272,6 +284,7 @@ def radau_rate(hip, nparticles=1_000_000, nseq=12):
         t += hd
     forces = 0
     days = 0.0
+    hip.timing(True, fast_only=True)      # HIP events around the particle kernel's launches
     tic = time.perf_counter()
     for _ in range(nseq):
         h, hd, nf, _ = hip.step(t, h, 1e-12)
@@ -279,10 +292,24 @@ def radau_rate(hip, nparticles=1_000_000, nseq=12):
         days += hd
         forces += nf
     sec = time.perf_counter() - tic
+    tm = hip.timing_read()
+    hip.timing(False)
+    launches = max(tm["particle_launches"], 1)
+    avg_s = tm["particle_ms"] / launches / 1e3
+    # one particle-kernel launch per steady sequence (a redo adds one)
+    flop = FLOP_PER_RA15_PSEQ * nparticles * nseq / launches
+    roofline = {"bound": "fp64_valu", "kernel": "ra_particles", "achieved": round(flop / avg_s / 1e12, 3),
+                "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(flop / avg_s / 1e12 / PEAK_FP64_TFLOPS, 4),
+                "traffic": None if TRAFFIC_PER_RA15_PSEQ is None else round(TRAFFIC_PER_RA15_PSEQ * nparticles),
+                "avg_launch_ms": round(avg_s * 1e3, 4), "launches": tm["particle_launches"],
+                "flop_per_particle_sequence": FLOP_PER_RA15_PSEQ,
+                "hbm_bytes_algorithmic_per_launch": round(BYTES_PER_RA15_PSEQ * nparticles)}
     return {"workload": f"RADAU tol 1e-12 with 1PN, 8 planets + {nparticles} particles",
             "sequences": nseq, "particle_steps_per_s": round(nparticles * nseq / sec, 1),
             "particle_force_evaluations_per_s": round(nparticles * forces / sec, 1),
-            "ms_per_sequence": round(sec * 1e3 / nseq, 3), "mean_step_days": round(days / nseq, 4)}
+            "ms_per_sequence": round(sec * 1e3 / nseq, 3), "mean_step_days": round(days / nseq, 4),
+            "roofline": roofline}
 
 
 PUBLISHED_EXTRA_FORCES = {"cuda_rtx4070_s": 6.04, "mercuda_cpu_1thread_s": 33.4, "mercury6_matched_s": 90.9,

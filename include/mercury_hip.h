@@ -125,7 +125,8 @@ int mercury_hip_set_tuning(int steps_per_launch, int block_size, int particles_p
  * starts recording every kernel kind, 2 the fast particle kernel only (no
  * events on the planet and slow-set streams), 0 stops; the read returns the
  * summed milliseconds and launch counts of the fast particle kernel, the
- * planet kernel and the slow-set kernel. */
+ * planet kernel and the slow-set kernel.  RADAU's particle kernel counts as
+ * the particle kernel. */
 int mercury_hip_timing(int enable);
 int mercury_hip_timing_read(double *particle_ms, int *particle_launches,
                             double *planet_ms, int *planet_launches,

@@ -869,10 +869,13 @@ double radau_pass(double t, int niter, bool reset) {
     const int npart = c.n - c.nbig;
     if (npart > 0) {
         dim3 grid((npart + 127) / 128);
+        hipEvent_t t0 = nullptr;      // kernel timing: the particle kernel counts as "particle"
+        timed_launch_begin(c.stream, t0);
         if (c.pn && ng) hipLaunchKernelGGL((ra_particles<true, true>), grid, dim3(128), 0, c.stream, args);
         else if (c.pn) hipLaunchKernelGGL((ra_particles<true, false>), grid, dim3(128), 0, c.stream, args);
         else if (ng) hipLaunchKernelGGL((ra_particles<false, true>), grid, dim3(128), 0, c.stream, args);
         else hipLaunchKernelGGL((ra_particles<false, false>), grid, dim3(128), 0, c.stream, args);
+        timed_launch_end(K_PARTICLE, c.stream, t0);
     }
     HIPCHECK(hipGetLastError());
     unsigned long long bits = 0;

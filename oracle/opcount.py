@@ -1,5 +1,5 @@
-"""Algorithmic FP64 operation count of one MVS particle-step -- TEST/MEASUREMENT
-INFRASTRUCTURE ONLY.
+"""Algorithmic FP64 operation counts of one MVS particle-step and of one RA15
+particle-sequence -- TEST/MEASUREMENT INFRASTRUCTURE ONLY.
 
 The particle side of mdt_mvs (mercury6_2.for:4057-4097: kick, drift_one,
 mce_stat's particle box and pair screen, mfo_mvs small-body force, kick) and
@@ -416,6 +416,296 @@ def verify(sysd, nsample=300, seed=1):
     return True
 
 
+# ---------------------------------------------------------------------------
+# RA15: one sequence of mdt_ra15 (mercury6_2.for:4133-4505) with mfo_all's
+# gravity and central 1PN (mfo_grav, mercury6_2.for:4593-4707), no oblateness
+# or non-gravitational terms.  The whole system is restated; the particles'
+# rows are F values, so the count covers the operations on each particle's
+# own data flow: its predictors, its force (the planets' direct terms, the
+# Sun's term with 1PN, plus the indirect sum the planets' table supplies), its
+# corrector and its end-of-sequence update and b/e prediction.  The planets'
+# work (plain floats) is per system, not per particle, and is not counted.
+# A massless particle's zero-mass terms on the planets and in the indirect sum
+# change no bit of the reference's result (x - 0 and x + 0 for nonzero x), so
+# they are left out, as the device kernels leave them out.
+
+RA_H = (0.0, .0562625605369221, .1802406917368924, .3526247171131696,
+        .5471536263305554, .7342101772154105, .8853209468390958, .9775206135612875)   # :4162-4165
+RA_XC = (.5, .1666666666666667, .08333333333333333, .05, .03333333333333333,
+         .02380952380952381, .01785714285714286, .01388888888888889)                   # :4166-4168
+RA_VC = (.5, .3333333333333333, .25, .2, .1666666666666667, .1428571428571429, .125)   # :4169-4170
+
+
+def ra_constants():
+    """r, c, d of mdt_ra15 (mercury6_2.for:4176-4199), 0-based."""
+    h = RA_H
+    r = [1.0 / (h[j] - h[k]) for j in range(1, 8) for k in range(j)]
+    c = [0.0] * 21; d = [0.0] * 21
+    c[0] = -h[1]; d[0] = h[1]
+    n = 1
+    for j in range(3, 8):
+        n += 1
+        c[n - 1] = -h[j - 1] * c[n - j + 1]
+        d[n - 1] = h[1] * d[n - j + 1]
+        for k in range(3, j):
+            n += 1
+            c[n - 1] = c[n - j] - h[j - 1] * c[n - j + 1]
+            d[n - 1] = d[n - j] + h[k - 1] * d[n - j + 1]
+        n += 1
+        c[n - 1] = c[n - j] - h[j - 1]
+        d[n - 1] = d[n - j] + h[j - 1]
+    return r, c, d
+
+
+_CL = 299792458.0 * 86400.0 / (1.495978707e13 * .01)
+LIGHT2 = _CL * _CL
+
+
+def grav_pn(m, nbig, x, v):
+    """mfo_grav with 1PN for every body j >= 1: x, v lists of [3] rows (floats
+    for the Big bodies, F for the particles)."""
+    nbod = len(x)
+    a = [[0.0, 0.0, 0.0] for _ in range(nbod)]
+    ri = [None] * nbod; r3 = [None] * nbod
+    for i in range(1, nbod):
+        s2 = x[i][0] * x[i][0] + x[i][1] * x[i][1] + x[i][2] * x[i][2]
+        s_1 = 1.0 / (sqrt(s2) if isinstance(s2, F) else math.sqrt(s2))
+        ri[i] = s_1
+        r3[i] = s_1 * s_1 * s_1
+    sx = sy = sz = 0.0
+    for i in range(1, nbig):                  # massless bodies add zero terms
+        tmp1 = m[i] * r3[i]
+        sx = sx - tmp1 * x[i][0]; sy = sy - tmp1 * x[i][1]; sz = sz - tmp1 * x[i][2]
+    for i in range(1, nbig):
+        for j in range(i + 1, nbod):
+            dx = x[j][0] - x[i][0]; dy = x[j][1] - x[i][1]; dz = x[j][2] - x[i][2]
+            s2 = dx * dx + dy * dy + dz * dz
+            s_1 = 1.0 / (sqrt(s2) if isinstance(s2, F) else math.sqrt(s2))
+            s_3 = s_1 * s_1 * s_1
+            tmp1 = s_3 * m[i]
+            a[j][0] = a[j][0] - tmp1 * dx; a[j][1] = a[j][1] - tmp1 * dy; a[j][2] = a[j][2] - tmp1 * dz
+            if j < nbig:
+                tmp2 = s_3 * m[j]
+                a[i][0] = a[i][0] + tmp2 * dx; a[i][1] = a[i][1] + tmp2 * dy; a[i][2] = a[i][2] + tmp2 * dz
+    s = (sx, sy, sz)
+    for i in range(1, nbod):
+        xi, vi = x[i], v[i]
+        tmp1 = m[0] * r3[i]
+        rv = xi[0] * vi[0] + xi[1] * vi[1] + xi[2] * vi[2]
+        v2 = vi[0] * vi[0] + vi[1] * vi[1] + vi[2] * vi[2]
+        f = tmp1 / LIGHT2
+        radial = 4.0 * m[0] * ri[i] - v2
+        for c in range(3):
+            a[i][c] = a[i][c] + s[c] - tmp1 * xi[c] + f * (radial * xi[c] + 4.0 * rv * vi[c])
+    return a
+
+
+class Ra15State:
+    """mdt_ra15's saved b, e and dtflag for the bodies j >= 1 (rows [j][c][7])."""
+
+    def __init__(self, nbod):
+        self.dtflag = 0
+        self.b = [[[0.0] * 7 for _ in range(3)] for _ in range(nbod)]
+        self.e = [[[0.0] * 7 for _ in range(3)] for _ in range(nbod)]
+
+
+def ra15_sequence(m, nbig, x1, v1, t, tol, st):
+    """One mdt_ra15 sequence with 1PN from trial size t (x1, v1 updated in
+    place); returns (tdid, tnext, niter)."""
+    h = RA_H
+    r, c, d = ra_constants()
+    nbod = len(x1)
+    if st.dtflag == 0:
+        st.dtflag = 1
+    b, e = st.b, st.e
+    while True:
+        if st.dtflag == 1:
+            niter = 6
+            for j in range(1, nbod):
+                for cc in range(3):
+                    b[j][cc] = [0.0] * 7; e[j][cc] = [0.0] * 7
+        else:
+            niter = 2
+        a1 = grav_pn(m, nbig, x1, v1)
+        g = [[None] * 3 for _ in range(nbod)]
+        for j in range(1, nbod):
+            for cc in range(3):
+                bk = b[j][cc]
+                g[j][cc] = [bk[6] * d[15] + bk[5] * d[10] + bk[4] * d[6] + bk[3] * d[3] + bk[2] * d[1] + bk[1] * d[0] + bk[0],
+                            bk[6] * d[16] + bk[5] * d[11] + bk[4] * d[7] + bk[3] * d[4] + bk[2] * d[2] + bk[1],
+                            bk[6] * d[17] + bk[5] * d[12] + bk[4] * d[8] + bk[3] * d[5] + bk[2],
+                            bk[6] * d[18] + bk[5] * d[13] + bk[4] * d[9] + bk[3],
+                            bk[6] * d[19] + bk[5] * d[14] + bk[4],
+                            bk[6] * d[20] + bk[5],
+                            bk[6]]
+        for _n in range(niter):
+            for j in range(2, 9):
+                hj = h[j - 1]
+                sv = [0.0] * 9
+                sv[0] = t * hj
+                sv[1] = sv[0] * sv[0] * .5
+                sv[2] = sv[1] * hj * .3333333333333333
+                sv[3] = sv[2] * hj * .5
+                sv[4] = sv[3] * hj * .6
+                sv[5] = sv[4] * hj * .6666666666666667
+                sv[6] = sv[5] * hj * .7142857142857143
+                sv[7] = sv[6] * hj * .75
+                sv[8] = sv[7] * hj * .7777777777777778
+                x = [[0.0] * 3 for _ in range(nbod)]
+                v = [[0.0] * 3 for _ in range(nbod)]
+                for q in range(1, nbod):
+                    for cc in range(3):
+                        bk = b[q][cc]
+                        x[q][cc] = (sv[8] * bk[6] + sv[7] * bk[5] + sv[6] * bk[4] + sv[5] * bk[3] + sv[4] * bk[2]
+                                    + sv[3] * bk[1] + sv[2] * bk[0] + sv[1] * a1[q][cc] + sv[0] * v1[q][cc] + x1[q][cc])
+                sw = [0.0] * 8
+                sw[0] = t * hj
+                sw[1] = sw[0] * hj * .5
+                sw[2] = sw[1] * hj * .6666666666666667
+                sw[3] = sw[2] * hj * .75
+                sw[4] = sw[3] * hj * .8
+                sw[5] = sw[4] * hj * .8333333333333333
+                sw[6] = sw[5] * hj * .8571428571428571
+                sw[7] = sw[6] * hj * .875
+                for q in range(1, nbod):
+                    for cc in range(3):
+                        bk = b[q][cc]
+                        v[q][cc] = (sw[7] * bk[6] + sw[6] * bk[5] + sw[5] * bk[4] + sw[4] * bk[3] + sw[3] * bk[2]
+                                    + sw[2] * bk[1] + sw[1] * bk[0] + sw[0] * a1[q][cc] + v1[q][cc])
+                a = grav_pn(m, nbig, x, v)
+                for q in range(1, nbod):
+                    for cc in range(3):
+                        bk, gk = b[q][cc], g[q][cc]
+                        ak, a1k = a[q][cc], a1[q][cc]
+                        if j == 2:
+                            temp = gk[0]
+                            gk[0] = (ak - a1k) * r[0]
+                            bk[0] = bk[0] + gk[0] - temp
+                            continue
+                        gkk = ak - a1k
+                        if j == 3:
+                            temp = gk[1]; gk[1] = (gkk * r[1] - gk[0]) * r[2]; temp = gk[1] - temp
+                            bk[0] = bk[0] + temp * c[0]; bk[1] = bk[1] + temp
+                        elif j == 4:
+                            temp = gk[2]; gk[2] = ((gkk * r[3] - gk[0]) * r[4] - gk[1]) * r[5]; temp = gk[2] - temp
+                            bk[0] = bk[0] + temp * c[1]; bk[1] = bk[1] + temp * c[2]; bk[2] = bk[2] + temp
+                        elif j == 5:
+                            temp = gk[3]
+                            gk[3] = (((gkk * r[6] - gk[0]) * r[7] - gk[1]) * r[8] - gk[2]) * r[9]
+                            temp = gk[3] - temp
+                            bk[0] = bk[0] + temp * c[3]; bk[1] = bk[1] + temp * c[4]; bk[2] = bk[2] + temp * c[5]
+                            bk[3] = bk[3] + temp
+                        elif j == 6:
+                            temp = gk[4]
+                            gk[4] = ((((gkk * r[10] - gk[0]) * r[11] - gk[1]) * r[12] - gk[2]) * r[13] - gk[3]) * r[14]
+                            temp = gk[4] - temp
+                            bk[0] = bk[0] + temp * c[6]; bk[1] = bk[1] + temp * c[7]; bk[2] = bk[2] + temp * c[8]
+                            bk[3] = bk[3] + temp * c[9]; bk[4] = bk[4] + temp
+                        elif j == 7:
+                            temp = gk[5]
+                            gk[5] = (((((gkk * r[15] - gk[0]) * r[16] - gk[1]) * r[17] - gk[2]) * r[18] - gk[3]) * r[19]
+                                     - gk[4]) * r[20]
+                            temp = gk[5] - temp
+                            bk[0] = bk[0] + temp * c[10]; bk[1] = bk[1] + temp * c[11]; bk[2] = bk[2] + temp * c[12]
+                            bk[3] = bk[3] + temp * c[13]; bk[4] = bk[4] + temp * c[14]; bk[5] = bk[5] + temp
+                        else:
+                            temp = gk[6]
+                            gk[6] = ((((((gkk * r[21] - gk[0]) * r[22] - gk[1]) * r[23] - gk[2]) * r[24] - gk[3]) * r[25]
+                                      - gk[4]) * r[26] - gk[5]) * r[27]
+                            temp = gk[6] - temp
+                            bk[0] = bk[0] + temp * c[15]; bk[1] = bk[1] + temp * c[16]; bk[2] = bk[2] + temp * c[17]
+                            bk[3] = bk[3] + temp * c[18]; bk[4] = bk[4] + temp * c[19]; bk[5] = bk[5] + temp * c[20]
+                            bk[6] = bk[6] + temp
+        # the step-size law (per system, floats)
+        temp = 0.0
+        for q in range(1, nbod):
+            for cc in range(3):
+                temp = max(temp, abs(F._v(b[q][cc][6])))
+        temp = temp / (72.0 * math.pow(abs(t), 7.0))
+        tdid = t
+        if temp == 0.0:
+            t = tdid * 1.4
+        else:
+            t = math.copysign(math.pow(tol / temp, 1.0 / 9.0), tdid)
+        if st.dtflag == 1 and abs(t / tdid) < 1.0:
+            t = t * .8
+            continue
+        if abs(t / tdid) > 1.4:
+            t = tdid * 1.4
+        break
+    temp = tdid * tdid
+    xc, vc = RA_XC, RA_VC
+    for q in range(1, nbod):
+        for cc in range(3):
+            bk = b[q][cc]
+            x1[q][cc] = (xc[7] * bk[6] + xc[6] * bk[5] + xc[5] * bk[4] + xc[4] * bk[3] + xc[3] * bk[2]
+                         + xc[2] * bk[1] + xc[1] * bk[0] + xc[0] * a1[q][cc]) * temp + v1[q][cc] * tdid + x1[q][cc]
+            v1[q][cc] = (vc[6] * bk[6] + vc[5] * bk[5] + vc[4] * bk[4] + vc[3] * bk[3] + vc[2] * bk[2]
+                         + vc[1] * bk[1] + vc[0] * bk[0] + a1[q][cc]) * tdid + v1[q][cc]
+    qq = t / tdid
+    q2 = qq * qq; q3 = qq * q2; q4 = q2 * q2; q5 = q2 * q3; q6 = q3 * q3; q7 = q3 * q4
+    for q in range(1, nbod):
+        for cc in range(3):
+            bk, ek = b[q][cc], e[q][cc]
+            sd = [0.0] * 7 if st.dtflag == 1 else [bk[i] - ek[i] for i in range(7)]
+            ek[0] = qq * (bk[6] * 7.0 + bk[5] * 6.0 + bk[4] * 5.0 + bk[3] * 4.0 + bk[2] * 3.0 + bk[1] * 2.0 + bk[0])
+            ek[1] = q2 * (bk[6] * 21.0 + bk[5] * 15.0 + bk[4] * 10.0 + bk[3] * 6.0 + bk[2] * 3.0 + bk[1])
+            ek[2] = q3 * (bk[6] * 35.0 + bk[5] * 20.0 + bk[4] * 10.0 + bk[3] * 4.0 + bk[2])
+            ek[3] = q4 * (bk[6] * 35.0 + bk[5] * 15.0 + bk[4] * 5.0 + bk[3])
+            ek[4] = q5 * (bk[6] * 21.0 + bk[5] * 6.0 + bk[4])
+            ek[5] = q6 * (bk[6] * 7.0 + bk[5])
+            ek[6] = q7 * bk[6]
+            for i in range(7):
+                bk[i] = ek[i] + sd[i]
+    st.dtflag = 2
+    return tdid, t, niter
+
+
+def ra15_rows(sysd, nparticles):
+    """The system's Big bodies (floats) and its first particles (F rows)."""
+    nb = sysd["nbig"]
+    sel = list(range(nb)) + list(range(nb, nb + nparticles))
+    m = [float(sysd["m"][i]) for i in sel]
+    x = [[float(q) for q in sysd["x"][i]] if i < nb else [F(q) for q in sysd["x"][i]] for i in sel]
+    v = [[float(q) for q in sysd["v"][i]] if i < nb else [F(q) for q in sysd["v"][i]] for i in sel]
+    return m, nb, x, v, sel
+
+
+def ra15_measure(sysd, nparticles=8, nseq=4, tol=1e-12):
+    """Counted operations per particle for each of nseq sequences from the
+    system's trial step (the first is mdt_ra15's reset sequence)."""
+    m, nb, x, v, _ = ra15_rows(sysd, nparticles)
+    st = Ra15State(len(m))
+    t = float(sysd["h"])
+    out = []
+    for _ in range(nseq):
+        OPS[0] = 0
+        tdid, t, niter = ra15_sequence(m, nb, x, v, t, tol, st)
+        out.append((niter, OPS[0] / nparticles, tdid))
+    return out
+
+
+def ra15_verify(sysd, nparticles=8, nseq=3, tol=1e-12):
+    """The counted RA15 restatement reproduces the oracle's RA15 with 1PN
+    (orc_ra15_run) bit for bit over nseq sequences: states and sizes."""
+    sys.path.insert(0, str(Path(__file__).resolve().parents[1]))
+    from oracle.bindings import Oracle
+    m, nb, x, v, sel = ra15_rows(sysd, nparticles)
+    sub = {k: np.asarray(sysd[k])[sel].copy() for k in ("m", "x", "v")}
+    sub.update(nbig=nb, h=float(sysd["h"]))
+    xo, vo, hdid, hn, _ = Oracle().ra15_run(sub, nseq, tol=tol, pn=1)
+    st = Ra15State(len(m))
+    t = float(sysd["h"])
+    sizes = []
+    for _ in range(nseq):
+        tdid, t, _ = ra15_sequence(m, nb, x, v, t, tol, st)
+        sizes.append(tdid)
+    got_x = np.array([[F._v(q) for q in row] for row in x])
+    got_v = np.array([[F._v(q) for q in row] for row in v])
+    return (np.array_equal(got_x[1:], xo[1:]) and np.array_equal(got_v[1:], vo[1:])
+            and np.array_equal(np.array(sizes), hdid) and t == hn)
+
+
 if __name__ == "__main__":
     sys.path.insert(0, str(Path(__file__).resolve().parents[1]))
     from mercuda_amd import systems
@@ -424,3 +714,6 @@ if __name__ == "__main__":
     r = measure(s)
     for k, (avg, n) in r.items():
         print(f"{k:6s} {avg:8.1f} ops/particle-step over {n} particle-steps")
+    print("counted RA15 restatement matches the oracle RA15 with 1PN:", ra15_verify(s))
+    for niter, ops, tdid in ra15_measure(s):
+        print(f"RA15 sequence, {niter} iterations, {tdid:.4f} d: {ops:.1f} ops per particle")

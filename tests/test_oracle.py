@@ -831,3 +831,18 @@ def test_mfo_pr_closed_form(oracle):
     want = MU * beta / r ** 2 * (x[2] / r - 1.3 * v[2] / C_AU_DAY)
     assert np.allclose(a[2], want, rtol=1e-14)
     assert np.all(oracle.mfo_pr(np.array([MU, 1e-9, 0]), x, v, p)[1] == 0.0)
+
+
+def test_operation_counts_follow_the_oracle():
+    """The op-counted restatements behind bench.py's FLOP constants reproduce
+    the oracle bit for bit (the MVS drift on sampled particles, RA15 with 1PN
+    over three sequences of a whole system), and their counts are the frozen
+    constants: 5,550 FP64 operations per particle per steady RA15 sequence."""
+    from oracle import opcount
+    import bench
+    s = systems.config_large(seed=2, nparticles=2000)
+    assert opcount.verify(s, nsample=40)
+    assert opcount.ra15_verify(s, nparticles=4, nseq=3)
+    seqs = opcount.ra15_measure(s, nparticles=4, nseq=3)
+    assert [n for n, _, _ in seqs] == [6, 2, 2]
+    assert all(ops == bench.FLOP_PER_RA15_PSEQ for n, ops, _ in seqs[1:])
