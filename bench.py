@@ -452,13 +452,18 @@ def cpu_bs(sysd, algor, nproc=16, shard=62500, nsteps=8):
     return nproc * shard * nsteps / max(times), max(times)
 
 
-def hybrid_rate(hip, nparticles=200_000, nsteps=3):
+def hybrid_rate(hip, nparticles=200_000, nsteps=3, driver_path=False):
     """HYBRID on 8 planets + 200k particles, half within 0.1 AU of a planet's
     orbit (changeover 3 Hill radii, dt 2 d): every step's device work --
     regular map, changeover scan, encounter members' BS2 substeps through the
     compact context -- driven by mercuda_amd.hybrid as the driver drives it.
-    The driver's own host bookkeeping (encounter listing, output) is not in
-    the timed loop."""
+    driver_path False: the encounter loop's step control on the device
+    (mercury_hip_encounter_advance, one queued launch per BS2 attempt).
+    driver_path True: the path the unchanged Fortran driver takes, one
+    mercury_hip_step per BS2 substep (the step control on the host) and the
+    members' states downloaded after each substep for the driver's encounter
+    listing.  The listing itself (the driver's own mce_stat on the host) and
+    its output are not in the timed loop."""
     from mercuda_amd import hybrid, systems
     s = systems.config_encounters(seed=3, nparticles=nparticles)
     hip.configure(10)
@@ -467,12 +472,16 @@ def hybrid_rate(hip, nparticles=200_000, nsteps=3):
     hip.configure(10)
     hip.upload_system(s)
     hybrid.mdt_hkce.substeps = 0
+    # the driver's path: each substep's states come back to the host
+    pull = (lambda *a: None) if driver_path else None
     tic = time.perf_counter()
-    _, _, most = hybrid.hy_run(hip, s, nsteps, upload=False,
+    _, _, most = hybrid.hy_run(hip, s, nsteps, upload=False, on_substep=pull,
                                progress=lambda k, npairs: log(f"hybrid step {k}: {npairs} encounter pairs, "
                                                               f"{hybrid.mdt_hkce.substeps} BS2 substeps so far"))
     sec = time.perf_counter() - tic
     return {"workload": f"HYBRID, 8 planets + {nparticles} particles (half near planet orbits), dt 2 d",
+            "encounter_loop": "per-substep mercury_hip_step + state download (the driver's path)" if driver_path
+            else "device loop (mercury_hip_encounter_advance)",
             "steps": nsteps, "particle_steps_per_s": round(nparticles * nsteps / sec, 1),
             "ms_per_step": round(sec * 1e3 / nsteps, 3), "most_encounter_pairs_in_a_step": most,
             "bs2_substeps": hybrid.mdt_hkce.substeps}
@@ -516,7 +525,7 @@ def main():
     ap.add_argument("--no-extras", action="store_true", help="skip the ensemble/single ratio")
     ap.add_argument("--extras", default="all",
                     help="comma list of extra workloads to run: ensemble, adjoint, radau, radau_extra_forces, "
-                         "bs2, bs3, hybrid, hybrid_20k (default all)")
+                         "bs2, bs3, hybrid, hybrid_driver_path, hybrid_20k, hybrid_20k_driver_path (default all)")
     ap.add_argument("--autotune", action="store_true",
                     help="let the autotuner pick steps per launch and workgroup size instead of --spl/--block "
                          "(the defaults are its usual choice on MI355X)")
@@ -600,7 +609,8 @@ def main():
                                        "planet_ms": round(tm2["planet_ms"], 2)}},
     }
     extras = {}
-    names = ("ensemble", "adjoint", "radau", "radau_extra_forces", "bs2", "bs3", "hybrid", "hybrid_20k")
+    names = ("ensemble", "adjoint", "radau", "radau_extra_forces", "bs2", "bs3", "hybrid", "hybrid_driver_path",
+             "hybrid_20k", "hybrid_20k_driver_path")
     want = set() if args.no_extras else (set(names) if args.extras == "all" else set(args.extras.split(",")))
     if want - set(names):
         raise SystemExit(f"unknown extras {sorted(want - set(names))}; choose from {names}")
@@ -622,9 +632,15 @@ def main():
     if "hybrid" in want:
         log("hybrid: 8 planets + 200k particles")
         extras["hybrid"] = hybrid_rate(hip)
+    if "hybrid_driver_path" in want:
+        log("hybrid, the driver's per-substep path: 8 planets + 200k particles")
+        extras["hybrid_driver_path"] = hybrid_rate(hip, nsteps=1, driver_path=True)
     if "hybrid_20k" in want:
         log("hybrid: 8 planets + 20k particles")
         extras["hybrid_20k"] = hybrid_rate(hip, nparticles=20_000, nsteps=5)
+    if "hybrid_20k_driver_path" in want:
+        log("hybrid, the driver's per-substep path: 8 planets + 20k particles")
+        extras["hybrid_20k_driver_path"] = hybrid_rate(hip, nparticles=20_000, nsteps=5, driver_path=True)
     cpu = None
     if not args.no_cpu:
         log(f"cpu baseline: reference mdt_mvs on {args.cpu_procs} processes")

@@ -336,6 +336,17 @@ __global__ void hy_scatter(State st, const int* ids, int count, const double* pa
     }
 }
 
+// Encounter members' x, v ([3][ncap] each) in the driver's layout, x(3,n)
+// then v(3,n), for one contiguous copy to the host.
+__global__ void enc_pack_aos(const double* x, const double* v, int n, int ncap, double* out) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    for (int c = 0; c < 3; c++) {
+        out[3 * j + c] = x[(long long)c * ncap + j];
+        out[(long long)3 * n + 3 * j + c] = v[(long long)c * ncap + j];
+    }
+}
+
 // Finish table: header mvs (solar shift), mve (physical shift of the end
 // state), acen; per Big: x after the shift (HF_X), rcrit, mass.
 enum { HF_X = 0, HF_N = 21 };

@@ -169,6 +169,7 @@ struct Context {
         bool active = false;
         int n = 0, nbig = 0, ncap = 0;
         double *m = nullptr, *rcrit = nullptr, *x = nullptr, *v = nullptr, *tp = nullptr, *cand = nullptr;
+        double* aos = nullptr;         // [6][ncap]: the members' states in the driver's layout (download)
         int *bbp = nullptr, *poff = nullptr, *pidx = nullptr;
         int4* pinfo = nullptr;         // packed partners per particle (pack_partners)
         double4* pcon = nullptr;
@@ -176,6 +177,12 @@ struct Context {
         unsigned long long* err = nullptr;
         BsCtl* ctl = nullptr;          // device step control (encounter_advance)
         std::vector<double> stage;
+        // pinned host words of a per-substep attempt (12 level errors, the
+        // fault flags) and the pinned staging of a state download
+        unsigned long long* hbits = nullptr;
+        double* hstage = nullptr;
+        size_t hstage_cap = 0;
+        bool pending = false;          // an accepted substep's kernel may still run on the stream
     } enc;
     // differentiation tape (mercury_hip_adjoint_forward)
     struct Tape {
@@ -272,6 +279,11 @@ void sync_all() {
 void release_all() {
     Context& c = ctx;
     if (c.sg) (void)hipStreamSynchronize(c.sg);   // a tape's tangent pass may still run
+    if (c.stream) (void)hipStreamSynchronize(c.stream);   // an encounter substep's accept may still run
+    c.enc.pending = false;
+    if (c.enc.hbits) (void)hipHostFree(c.enc.hbits);
+    if (c.enc.hstage) (void)hipHostFree(c.enc.hstage);
+    c.enc.hbits = nullptr; c.enc.hstage = nullptr; c.enc.hstage_cap = 0;
     graphs.drop();
     for (double** p : {&c.x, &c.v, &c.a, &c.m, &c.rce, &c.rphys, &c.ox, &c.ov, &c.v0k, &c.v1, &c.tab, &c.cx, &c.cv})
         dfree(*p);
@@ -287,7 +299,7 @@ void release_all() {
     dfree(c.tape.dsys);
     c.tape.valid = false;
     for (double** p : {&c.rcrit, &c.hx0, &c.hv0, &c.hytab, &c.hpack, &c.enc.m, &c.enc.rcrit, &c.enc.x, &c.enc.v,
-                       &c.enc.tp, &c.enc.cand})
+                       &c.enc.tp, &c.enc.cand, &c.enc.aos})
         dfree(*p);
     dfree(c.hpairs); dfree(c.hnpairs); dfree(c.hids); dfree(c.enc.bbp); dfree(c.enc.poff); dfree(c.enc.pidx);
     dfree(c.enc.err); dfree(c.enc.ctl); dfree(c.enc.pinfo); dfree(c.enc.pcon);
@@ -1487,6 +1499,8 @@ void hybrid_finish(double h, const double* m, const double* xx, const double* vv
 
 // ---- compact encounter context (BS2) -------------------------------------
 
+void enc_settle();
+
 // Each particle's partners, their max(rcrit) and masses, packed for the BS2
 // kernels; again whenever the masses change.
 void encounter_pack() {
@@ -1502,6 +1516,7 @@ void encounter_pack() {
 
 void encounter_enter(int n, int nbig, const double* m, const double* x, const double* v, const double* crit,
                      int np_, const int* pi, const int* pj) {
+    enc_settle();
     Context& c = ctx;
     Context::Enc& e = c.enc;
     if (c.algorithm != 10) throw std::runtime_error("encounter context needs algorithm 10");
@@ -1516,6 +1531,7 @@ void encounter_enter(int n, int nbig, const double* m, const double* x, const do
         for (double** p : {&e.m, &e.rcrit}) dalloc(*p, (size_t)cap);
         for (double** p : {&e.x, &e.v}) dalloc(*p, (size_t)3 * cap);
         dalloc(e.cand, (size_t)108 * cap);   // two rotating sets of 9 levels x 6 components
+        dalloc(e.aos, (size_t)6 * cap);
         dalloc(e.poff, (size_t)cap + 1);
         dalloc(e.pinfo, (size_t)cap);
         dalloc(e.pcon, (size_t)cap);
@@ -1569,6 +1585,7 @@ void encounter_enter(int n, int nbig, const double* m, const double* x, const do
 }
 
 void encounter_states(const double* x, const double* v) {
+    enc_settle();
     Context::Enc& e = ctx.enc;
     e.stage.resize((size_t)3 * e.n);
     for (const double* src : {x, v}) {
@@ -1581,18 +1598,37 @@ void encounter_states(const double* x, const double* v) {
     }
 }
 
-void encounter_download(double* x, double* v) {
-    Context::Enc& e = ctx.enc;
-    e.stage.resize((size_t)3 * e.n);
-    for (int which = 0; which < 2; which++) {
-        double* src = which == 0 ? e.x : e.v;
-        double* dst = which == 0 ? x : v;
-        for (int k = 0; k < 3; k++)
-            HIPCHECK(hipMemcpy(e.stage.data() + (size_t)k * e.n, src + (size_t)k * e.ncap, sizeof(double) * e.n,
-                               hipMemcpyDeviceToHost));
-        for (int j = 0; j < e.n; j++)
-            for (int k = 0; k < 3; k++) dst[3 * j + k] = e.stage[(size_t)k * e.n + j];
+// The stream's pending work on the encounter buffers (a substep's accept,
+// launched without a wait) ends before the host or a null-stream copy touches
+// them.
+void enc_settle() {
+    if (ctx.enc.pending) {
+        HIPCHECK(hipStreamSynchronize(ctx.stream));
+        ctx.enc.pending = false;
     }
+}
+
+// Members' states in the driver's layout: packed on the device behind the
+// last substep's kernels, one copy on the stream into pinned staging, and one
+// wait.
+void encounter_download(double* x, double* v) {
+    Context& c = ctx;
+    Context::Enc& e = c.enc;
+    const size_t need = (size_t)6 * e.n;
+    if (e.hstage_cap < need) {
+        if (e.hstage) HIPCHECK(hipHostFree(e.hstage));
+        e.hstage = nullptr;
+        HIPCHECK(hipHostMalloc(reinterpret_cast<void**>(&e.hstage), sizeof(double) * std::max<size_t>(need, 6),
+                               hipHostMallocDefault));
+        e.hstage_cap = need;
+    }
+    hipLaunchKernelGGL(enc_pack_aos, dim3((e.n + 255) / 256), dim3(256), 0, c.stream, e.x, e.v, e.n, e.ncap, e.aos);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpyAsync(e.hstage, e.aos, sizeof(double) * need, hipMemcpyDeviceToHost, c.stream));
+    HIPCHECK(hipStreamSynchronize(c.stream));
+    e.pending = false;
+    std::memcpy(x, e.hstage, sizeof(double) * 3 * e.n);
+    std::memcpy(v, e.hstage + (size_t)3 * e.n, sizeof(double) * 3 * e.n);
 }
 
 // mdt_bs2 (mercury6_2.for:3428-3617) over the encounter set: every attempt
@@ -1609,10 +1645,15 @@ void bs2_step(double time, double* h, double* hdid, double tol, int64_t* forces,
     A.tol2 = tol * tol;
     A.tp = e.tp; A.cand = e.cand; A.err = e.err; A.fault = c.fault;
     *forces = 1;           // the start accelerations, once per call
+    if (!e.hbits)
+        HIPCHECK(hipHostMalloc(reinterpret_cast<void**>(&e.hbits), 16 * sizeof(unsigned long long),
+                               hipHostMallocDefault));
     for (;;) {
         if (time + *h == time) throw std::runtime_error("BS cannot advance time at the requested tolerance");
         A.h0 = *h;
-        clear_fault();
+        // one wait per attempt: the fault flags clear and the level errors and
+        // flags come back on the stream, behind the previous substep's accept
+        HIPCHECK(hipMemsetAsync(c.fault, 0, sizeof(int), c.stream));
         hipLaunchKernelGGL(zero_bytes, dim3(1), dim3(128), 0, c.stream, reinterpret_cast<unsigned char*>(e.err),
                            (long long)(12 * sizeof(unsigned long long)));
         hipLaunchKernelGGL(bs2_big, dim3(1), dim3(kLevels * kEncBig), 0, c.stream, A);
@@ -1620,10 +1661,14 @@ void bs2_step(double time, double* h, double* hdid, double tol, int64_t* forces,
             hipLaunchKernelGGL(bs2_particles, dim3((npart + kBsPairSlots - 1) / kBsPairSlots), dim3(kBsPairThreads),
                                0, c.stream, A);
         HIPCHECK(hipGetLastError());
-        unsigned long long bits[12];
-        HIPCHECK(hipMemcpyAsync(bits, e.err, sizeof(bits), hipMemcpyDeviceToHost, c.stream));
+        unsigned long long* bits = e.hbits;
+        HIPCHECK(hipMemcpyAsync(bits, e.err, 12 * sizeof(unsigned long long), hipMemcpyDeviceToHost, c.stream));
+        HIPCHECK(hipMemcpyAsync(bits + 12, c.fault, sizeof(int), hipMemcpyDeviceToHost, c.stream));
         HIPCHECK(hipStreamSynchronize(c.stream));
-        if (read_fault() & FAULT_FORCE) throw std::runtime_error("Nonfinite BS extrapolation");
+        e.pending = false;
+        int fault = 0;
+        std::memcpy(&fault, bits + 12, sizeof(int));
+        if (fault & FAULT_FORCE) throw std::runtime_error("Nonfinite BS extrapolation");
         int level = 0;
         for (int n = 4; n <= 12 && !level; n++) {
             double err;
@@ -1632,10 +1677,11 @@ void bs2_step(double time, double* h, double* hdid, double tol, int64_t* forces,
         }
         if (level) {
             for (int n = 1; n <= level; n++) *forces += n;
-            if (e.n > 1)
+            if (e.n > 1) {
                 hipLaunchKernelGGL(bs2_accept, dim3((e.n - 1 + 127) / 128), dim3(128), 0, c.stream, A, level);
+                e.pending = true;   // settled by the next reader of the encounter buffers
+            }
             HIPCHECK(hipGetLastError());
-            HIPCHECK(hipStreamSynchronize(c.stream));
             *hdid = *h;
             if (level >= 8) *h = *h * SHRINK;
             if (level < 7) *h = *h * GROW;
@@ -1659,6 +1705,7 @@ void bs2_step(double time, double* h, double* hdid, double tol, int64_t* forces,
 // encounter listing the driver makes after each substep is not made.
 void encounter_advance(double h0, double* hrec, double tol, int64_t* substeps, int64_t* forces,
                        int64_t* rejected) {
+    enc_settle();
     Context& c = ctx;
     Context::Enc& e = c.enc;
     if (!e.active) throw std::runtime_error("no encounter context");
@@ -2134,6 +2181,7 @@ int mercury_hip_encounter_update(const double* m, const double* x, const double*
     try {
         Context::Enc& e = ctx.enc;
         if (!e.active) throw std::runtime_error("no encounter context");
+        enc_settle();
         for (int j = e.nbig; j < e.n; j++)
             if (m[j] != 0) throw std::runtime_error("Small bodies with mass are not provided by the HIP path");
         HIPCHECK(hipMemcpy(e.m, m, sizeof(double) * e.n, hipMemcpyHostToDevice));
@@ -2143,7 +2191,11 @@ int mercury_hip_encounter_update(const double* m, const double* x, const double*
     } catch (const std::exception& e) { return fail(e); }
 }
 
-void mercury_hip_encounter_exit(void) { ctx.enc.active = false; }
+void mercury_hip_encounter_exit(void) {
+    if (ctx.enc.pending) (void)hipStreamSynchronize(ctx.stream);   // no error crosses this void entry point
+    ctx.enc.pending = false;
+    ctx.enc.active = false;
+}
 
 int mercury_hip_encounter_advance(double h0, double* hrec, double tol, int64_t* substeps, int64_t* forces,
                                   int64_t* rejected) {
