@@ -475,20 +475,39 @@ struct BsArgs {
 };
 
 // mfo_hkce pair factor (1 - K) / s^3 inside rc, else 0 (mercury6_2.for, mfo_hkce).
-__device__ __forceinline__ bool hkce_factor(double s2, double rc, double& tmp2) {
-    double rc2 = rc * rc;
-    if (!(s2 < rc2)) return false;
-    double s_1 = 1.0 / sqrt(s2);
+template <class Ops>
+__device__ __forceinline__ void hkce_factor_ops(double s2, double rc, double rc2, double& tmp2) {
+    double s_1 = Ops::rcp(Ops::sqrt(s2));
     double s_3 = s_1 * s_1 * s_1;
     if (s2 <= 0.01 * rc2) {
         tmp2 = s_3;
     } else {
-        double s = 1.0 / s_1;
-        double q = (s - 0.1 * rc) / (0.9 * rc);
+        double s = Ops::rcp(s_1);
+        double q = Ops::div(s - 0.1 * rc, 0.9 * rc);
         double q2 = q * q, q3 = q * q2, q4 = q2 * q2, q5 = q2 * q3;
         tmp2 = (1.0 - 10.0 * q3 + 15.0 * q4 - 6.0 * q5) * s_3;
     }
+}
+
+// With rc in [2^-200, 2^200] and s^2 inside RangeOps' root range (sq_ok), the
+// short exact sequences of rn_ops.hpp give the IEEE bits: s^-1 and s lie in
+// [2^-500, 2^384]; in the outer branch s - 0.1 rc is zero or at least a few
+// ulp of 0.1 rc (>= 2^-256) and at most rc, and 0.9 rc is in range, so the
+// quotient's operands and result lie where div_rn is exact.  Outside the gate
+// the IEEE forms run.
+__device__ __forceinline__ bool hkce_factor(double s2, double rc, double& tmp2) {
+    double rc2 = rc * rc;
+    if (!(s2 < rc2)) return false;
+    if (RangeOps::sq_ok(s2) && rc >= 0x1p-200 && rc <= 0x1p200) hkce_factor_ops<RangeOps>(s2, rc, rc2, tmp2);
+    else hkce_factor_ops<IeeeOps>(s2, rc, rc2, tmp2);
     return true;
+}
+
+// 1 / sqrt(s2) with the IEEE bits: the short exact sequences where RangeOps
+// admits s2, the IEEE forms elsewhere.
+__device__ __forceinline__ double rsqrt_exact(double s2) {
+    if (RangeOps::sq_ok(s2)) return RangeOps::rcp(RangeOps::sqrt(s2));
+    return 1.0 / sqrt(s2);
 }
 
 __device__ __forceinline__ void merge_err(unsigned long long* slot, double e) {
@@ -536,7 +555,7 @@ __device__ inline void big_accel(const BigShared& S, int nbbp, int me, const dou
     }
     const double* x = P + 3 * (me - 1);
     double s2 = x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
-    double s_1 = 1.0 / sqrt(s2);
+    double s_1 = rsqrt_exact(s2);
     double tmp2 = S.m[0] * s_1 * s_1 * s_1;
     for (int c = 0; c < 3; c++) acc[c] = acc[c] - tmp2 * x[c];
 }
@@ -793,7 +812,7 @@ __device__ __forceinline__ void pforce_reg(const RegPartners& R, const double* x
     if (R.n > 0) pair_term(xe, X0, R.rc0, R.m0, acc);
     if (R.n > 1) pair_term(xe, X1, R.rc1, R.m1, acc);
     double s2 = xe[0] * xe[0] + xe[1] * xe[1] + xe[2] * xe[2];
-    double s_1 = 1.0 / sqrt(s2);
+    double s_1 = rsqrt_exact(s2);
     double tmp2 = mc * s_1 * s_1 * s_1;
     for (int c = 0; c < 3; c++) out[c] = acc[c] - tmp2 * xe[c];
 }
@@ -854,7 +873,7 @@ __device__ inline void bs2_particles_body(const BsArgs& A, const double* xs, con
             acc[0] = acc[0] - faci * dx; acc[1] = acc[1] - faci * dy; acc[2] = acc[2] - faci * dz;
         }
         double s2 = xe[0] * xe[0] + xe[1] * xe[1] + xe[2] * xe[2];
-        double s_1 = 1.0 / sqrt(s2);
+        double s_1 = rsqrt_exact(s2);
         double tmp2 = A.m[0] * s_1 * s_1 * s_1;
         for (int c = 0; c < 3; c++) a[c] = acc[c] - tmp2 * xe[c];
     };
@@ -1178,7 +1197,7 @@ __global__ void __launch_bounds__(kBsPairThreads) bs2_loop_paired(BsArgs A, BsLo
             acc[0] = acc[0] - faci * dx; acc[1] = acc[1] - faci * dy; acc[2] = acc[2] - faci * dz;
         }
         double s2 = xe[0] * xe[0] + xe[1] * xe[1] + xe[2] * xe[2];
-        double s_1 = 1.0 / sqrt(s2);
+        double s_1 = rsqrt_exact(s2);
         double tmp2 = A.m[0] * s_1 * s_1 * s_1;
         for (int qc = 0; qc < 3; qc++) acc_out[qc] = acc[qc] - tmp2 * xe[qc];
     };
@@ -1339,7 +1358,7 @@ __global__ void __launch_bounds__(kLevels * kBsPart) bs2_loop_fused(BsArgs A, Bs
             acc[0] = acc[0] - faci * dx; acc[1] = acc[1] - faci * dy; acc[2] = acc[2] - faci * dz;
         }
         double s2 = xe[0] * xe[0] + xe[1] * xe[1] + xe[2] * xe[2];
-        double s_1 = 1.0 / sqrt(s2);
+        double s_1 = rsqrt_exact(s2);
         double tmp2 = A.m[0] * s_1 * s_1 * s_1;
         for (int qc = 0; qc < 3; qc++) acc_out[qc] = acc[qc] - tmp2 * xe[qc];
     };
