@@ -198,9 +198,10 @@ int mercury_hip_force_noalloc(int n, int nbig, int pn, const double *m, const do
  * v[3n] (the same numbers mercury_hip_advance produces). */
 int mercury_hip_adjoint_forward(int nsteps, double h, int segment, double *x, double *v);
 /* Reverse pass over the tape: given cotangents gx[3n], gv[3n] of the final
- * states (rows of Big bodies must be zero), the gradient with respect to the
- * initial states (gx0[3n], gv0[3n]) and the masses (gm[n]; Big bodies only,
- * zero elsewhere). */
+ * states (particles and Big bodies; the central body's heliocentric state is
+ * identically zero, so its rows contribute nothing), the gradient with
+ * respect to the initial states (gx0[3n], gv0[3n]) and the masses (gm[n]; Big
+ * bodies only, zero elsewhere). */
 int mercury_hip_adjoint_backward(const double *gx, const double *gv, double *gx0, double *gv0,
                                  double *gm);
 /* Tangent-linear pass over the tape: the derivative of the final states

@@ -13,8 +13,9 @@ on the CPU (the library's boundary takes host arrays).
     loss = ((xT[nbig:] - target) ** 2).sum()
     loss.backward()        # x0.grad, v0.grad, m.grad
 
-Only particle final states may carry gradient (rows of Big bodies in the
-output gradients must be zero); masses receive gradient for Big bodies.
+Every final state may carry gradient, the Big bodies' as well as the
+particles' (the central body's heliocentric state is identically zero);
+masses receive gradient for Big bodies.
 """
 from __future__ import annotations
 

@@ -193,6 +193,7 @@ struct Context {
         double *tab = nullptr, *ck = nullptr, *rec = nullptr, *lam = nullptr, *part = nullptr, *src = nullptr;
         double *grad = nullptr, *rows = nullptr, *tan = nullptr, *fin = nullptr, *seed = nullptr;
         double *psave = nullptr;
+        double *pfin = nullptr;           // final planet tangents per direction [7np][6np]
         std::vector<double> xfin, vfin;   // final states (host, Fortran layout)
     } tape;
 };
@@ -294,7 +295,7 @@ void release_all() {
     dfree(c.dngf); c.ngf_cap = 0; dfree(c.dangf); c.angf_cap = 0; dfree(c.dseed); c.seed_cap = 0;
     c.comets.clear(); c.ngflag = 0;
     for (double** p : {&c.tape.tab, &c.tape.ck, &c.tape.rec, &c.tape.lam, &c.tape.part, &c.tape.src, &c.tape.grad,
-                       &c.tape.rows, &c.tape.tan, &c.tape.fin, &c.tape.seed, &c.tape.psave})
+                       &c.tape.rows, &c.tape.tan, &c.tape.fin, &c.tape.seed, &c.tape.psave, &c.tape.pfin})
         dfree(*p);
     dfree(c.tape.dsys);
     c.tape.valid = false;
@@ -1131,7 +1132,9 @@ void adjoint_forward(int nsteps, double h, int seg, double* xo, double* vo) {
     // the planets' tangent rows of every unit direction, on their own stream
     if (np > 0) {
         HIPCHECK(hipStreamWaitEvent(c.sg, c.ev_plan[0], 0));
-        TlmArgs ta{t.dsys, c.m, t.psave, t.psave + 3 * np, nsteps, nullptr, t.rows, nullptr};
+        dfree(t.pfin);
+        dalloc(t.pfin, (size_t)42 * np * np);
+        TlmArgs ta{t.dsys, c.m, t.psave, t.psave + 3 * np, nsteps, nullptr, t.rows, t.pfin};
         hipLaunchKernelGGL(tlm_planets, dim3(7 * np), dim3(64), 0, c.sg, ta);
         HIPCHECK(hipGetLastError());
         HIPCHECK(hipEventRecord(c.ev_done[0], c.sg));
@@ -1196,10 +1199,6 @@ void adjoint_backward(const double* gx, const double* gv, double* gx0, double* g
     Context::Tape& t = c.tape;
     require_tape();
     const int np = t.np, npart = t.npart, R = row_len(np), T = t.steps;
-    for (int j = 0; j < c.nbig; j++)
-        for (int k = 0; k < 3; k++)
-            if (gx[3 * j + k] != 0 || gv[3 * j + k] != 0)
-                throw std::runtime_error("cotangents of Big-body final states are not provided");
     dfree(t.src); dfree(t.grad);
     dalloc(t.src, (size_t)(T + 1) * R);
     dalloc(t.grad, (size_t)std::max(7 * np, 1));
@@ -1237,15 +1236,28 @@ void adjoint_backward(const double* gx, const double* gv, double* gx0, double* g
         HIPCHECK(hipMemsetAsync(t.src, 0, sizeof(double) * (T + 1) * R, c.stream));
     }
     std::vector<double> grad((size_t)std::max(7 * np, 1), 0.0);
+    std::vector<double> pfin((size_t)42 * np * np);
     if (np > 0) {
         HIPCHECK(hipStreamWaitEvent(c.stream, c.ev_done[0], 0));
         hipLaunchKernelGGL(adj_contract, dim3(7 * np), dim3(256), 0, c.stream, t.src, t.rows, t.grad, T + 1, R);
         HIPCHECK(hipGetLastError());
         HIPCHECK(hipMemcpyAsync(grad.data(), t.grad, sizeof(double) * 7 * np, hipMemcpyDeviceToHost, c.stream));
+        HIPCHECK(hipMemcpyAsync(pfin.data(), t.pfin, sizeof(double) * 42 * np * np, hipMemcpyDeviceToHost,
+                                c.stream));
     }
     if (npart > 0)
         HIPCHECK(hipMemcpyAsync(lam0.data(), t.lam, sizeof(double) * 6 * npart, hipMemcpyDeviceToHost, c.stream));
     HIPCHECK(hipStreamSynchronize(c.stream));
+    // the planets' own final states: each direction's final tangent against
+    // their cotangents (the central body's heliocentric state is identically 0)
+    for (int dir = 0; dir < 7 * np; dir++) {
+        const double* f = pfin.data() + (size_t)dir * 6 * np;
+        double acc = 0.0;
+        for (int p = 0; p < np; p++)
+            for (int k = 0; k < 3; k++)
+                acc += gx[3 * (p + 1) + k] * f[3 * p + k] + gv[3 * (p + 1) + k] * f[3 * np + 3 * p + k];
+        grad[dir] += acc;
+    }
     std::fill(gx0, gx0 + 3 * c.n, 0.0);
     std::fill(gv0, gv0 + 3 * c.n, 0.0);
     std::fill(gm, gm + c.n, 0.0);

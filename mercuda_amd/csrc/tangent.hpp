@@ -220,7 +220,7 @@ struct TlmArgs {
     int steps;
     const double* seed; // one direction [3np dX, 3np dV, np dm]; null: unit direction blockIdx.x
     double* rows;       // per-table rows [ndir][(steps + 1)][R], start table first
-    double* fin;        // final planet tangents [3np dX, 3np dV] of direction 0 (may be null)
+    double* fin;        // final planet tangents [ndir][3np dX, 3np dV] (may be null)
 };
 
 // Big-body accelerations of mfo_mvs (the planet kernel's big_forces, jcen = 0)
@@ -419,12 +419,14 @@ __global__ void __launch_bounds__(64) tlm_planets(TlmArgs args) {
             for (int c = 0; c < 3; c++) V[3 * p + c] = V[3 * p + c] + hby2 * A[3 * p + c];
         __syncthreads();
     }
-    if (args.fin && dir == 0)
+    if (args.fin) {
+        double* fin = args.fin + (long long)dir * 6 * np;
         for (int p = lane; p < np; p += 64)
             for (int c = 0; c < 3; c++) {
-                args.fin[3 * p + c] = X[3 * p + c].d;
-                args.fin[3 * np + 3 * p + c] = V[3 * p + c].d;
+                fin[3 * p + c] = X[3 * p + c].d;
+                fin[3 * np + 3 * p + c] = V[3 * p + c].d;
             }
+    }
 }
 
 // ---------------------------------------------------------------- particles

@@ -104,6 +104,99 @@ def test_gradient_matches_central_differences(hip, oracle):
         assert rel <= tol
 
 
+def test_planet_final_state_loss_matches_central_differences(hip, oracle):
+    """Cotangents on the Big bodies' final states (positions and velocities of
+    three planets) against central differences (relative step 1e-7) of the
+    oracle's CPU integrator: planet initial states and masses; particle
+    initial states get no gradient from it."""
+    nsteps = 300
+    s = fit_system(45, 16)
+    nb = s["nbig"]
+    rng = np.random.default_rng(6)
+    sel = [2, 5, 8]
+    wx = rng.normal(size=(len(sel), 3))
+    wv = rng.normal(size=(len(sel), 3)) * 100.0
+    say(f"case: loss on planet final states vs central differences, 8 planets + 16 particles, {nsteps} steps")
+    hip.configure(1)
+    hip.upload_system(s)
+    xT, vT = hip.adjoint_forward(nsteps, s["h"])
+    gx = np.zeros_like(xT); gv = np.zeros_like(vT)
+    gx[sel] = 2.0 * wx * xT[sel]
+    gv[sel] = wv
+    gx0, gv0, gm = hip.adjoint_backward(gx, gv)
+
+    def loss_cpu(sd):
+        x, v, _, _ = oracle.mvs_run(sd, nsteps, screen=False)
+        return float(np.sum(wx * x[sel] ** 2) + np.sum(wv * v[sel]))
+
+    def central(kind, j, c, rel_step):
+        base = s[kind][j] if c is None else s[kind][j, c]
+        step = rel_step * (abs(base) if kind == "m" else np.linalg.norm(s[kind][j]))
+        vals = []
+        for sign in (1, -1):
+            sd = {k: (np.array(v, copy=True) if isinstance(v, np.ndarray) else v) for k, v in s.items()}
+            if c is None:
+                sd[kind][j] += sign * step
+            else:
+                sd[kind][j, c] += sign * step
+            vals.append(loss_cpu(sd))
+        return (vals[0] - vals[1]) / (2 * step)
+
+    # The loss's own planets' states: central differences at relative step
+    # 1e-7.  Other planets' states reach the loss only through the planets'
+    # mutual pull and move it ~1e3-1e6 times less: there the difference is
+    # Richardson-extrapolated from relative steps 5e-5 and 1e-4.  The masses
+    # (1e-10 to 1e-8 here) move the loss so little that below a relative step
+    # of ~1e-4 its rounding scatters the differences by ~1e-3; they take a
+    # plain central difference at 3e-4, whose truncation is ~1e-6.
+    picks = [("x", 2, 0, "direct", 1e-5), ("x", 5, 1, "direct", 1e-5), ("x", 8, 2, "direct", 1e-5),
+             ("v", 2, 2, "direct", 1e-5), ("v", 8, 0, "direct", 1e-5), ("x", 3, 1, "richardson", 1e-3),
+             ("v", 1, 1, "richardson", 1e-3), ("m", 2, None, "wide", 1e-3), ("m", 5, None, "wide", 1e-3),
+             ("m", 7, None, "wide", 1e-3)]
+    for kind, j, c, mode, tol in picks:
+        if mode == "direct":
+            fd = central(kind, j, c, 1e-7)
+        elif mode == "wide":
+            fd = central(kind, j, c, 3e-4)
+        else:
+            fd = (4.0 * central(kind, j, c, 5e-5) - central(kind, j, c, 1e-4)) / 3.0
+        ad = {"x": gx0, "v": gv0}[kind][j, c] if c is not None else gm[j]
+        rel = abs(ad - fd) / abs(fd)
+        say(f"  d L / d {kind}[{j}{'' if c is None else ',' + str(c)}]: adjoint {ad:+.10e} fd {fd:+.10e} rel {rel:.2e}")
+        assert rel <= tol
+    assert not np.any(gx0[nb:]) and not np.any(gv0[nb:])
+
+
+def test_adjoint_is_transpose_of_tangent_with_planet_cotangents(hip):
+    """<grad L, d> = dL along d from the tangent-linear pass when the
+    cotangent covers Big bodies and particles together."""
+    nsteps = 600
+    s = fit_system(46, 300)
+    nb = s["nbig"]
+    n = len(s["m"])
+    rng = np.random.default_rng(12)
+    say(f"case: dot-product test with Big-body cotangents, 8 planets + 300 particles, {nsteps} steps")
+    hip.configure(1)
+    hip.upload_system(s)
+    hip.adjoint_forward(nsteps, s["h"])
+    gx = rng.normal(size=(n, 3)); gv = rng.normal(size=(n, 3)) * 50.0
+    gx[0] = 0.0; gv[0] = 0.0
+    gx0, gv0, gm = hip.adjoint_backward(gx, gv)
+    for trial in range(3):
+        dx0 = rng.normal(size=(n, 3)) * 1e-3
+        dv0 = rng.normal(size=(n, 3)) * 1e-5
+        dm = np.zeros(n)
+        dm[1:nb] = rng.normal(size=nb - 1) * s["m"][1:nb] * 1e-2
+        dx0[0] = 0.0; dv0[0] = 0.0
+        if trial == 1:      # planets only
+            dx0[nb:] = 0.0; dv0[nb:] = 0.0
+        dx, dv = hip.tangent(dx0, dv0, dm)
+        lin = float(np.sum(gx * dx) + np.sum(gv * dv))
+        adj = float(np.sum(gx0 * dx0) + np.sum(gv0 * dv0) + np.sum(gm * dm))
+        say(f"  trial {trial}: tangent {lin:+.12e} adjoint {adj:+.12e}")
+        assert abs(lin - adj) <= 1e-9 * max(abs(lin), abs(adj))
+
+
 def test_tangent_matches_differences_of_final_states(hip, oracle):
     """The tangent-linear pass against central differences (relative step
     1e-7 along the direction) of every body's final state from the oracle's
@@ -207,10 +300,11 @@ def test_torch_autograd_function(hip):
     v0 = torch.tensor(s["v"], requires_grad=True)
     m = torch.tensor(s["m"], requires_grad=True)
     xT, vT = MVSFlow.apply(x0, v0, m, nb, 120, s["h"])
-    loss = (xT[nb:] ** 2).sum() + 3.0 * (vT[nb:, 2] ** 2).sum()
+    loss = (xT[nb:] ** 2).sum() + 3.0 * (vT[nb:, 2] ** 2).sum() + 0.5 * (xT[1:nb] ** 2).sum()
     loss.backward()
     lib = library()
     gx = np.zeros((len(s["m"]), 3)); gx[nb:] = 2 * xT.detach().numpy()[nb:]
+    gx[1:nb] = xT.detach().numpy()[1:nb]
     gv = np.zeros((len(s["m"]), 3)); gv[nb:, 2] = 6 * vT.detach().numpy()[nb:, 2]
     xT2, _ = lib.adjoint_forward(120, s["h"])
     assert np.array_equal(xT2, xT.detach().numpy())
