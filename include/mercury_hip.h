@@ -38,9 +38,8 @@ typedef struct MercuryEvent {
 /* mercury_cuda.h:13  1 if a HIP device is present, else 0. */
 int mercury_hip_available(void);
 /* mercury_cuda.h:14  select the algorithm: 1 MVS, 2 BS, 3 BS2, 4 RADAU,
- * 9 TEST (MVS, no corrector), 10 HYBRID; any other returns 1.  BS and BS2
- * take at most 16 Big bodies besides the central body (a step with more
- * fails). */
+ * 9 TEST (MVS, no corrector), 10 HYBRID; any other returns 1.  Every
+ * algorithm takes up to 63 Big bodies besides the central body. */
 int mercury_hip_configure(int algorithm);
 /* mercury_cuda.h:15  flag != 2 discards the MVS acceleration / RADAU history. */
 void mercury_hip_reset(int flag);

@@ -25,7 +25,8 @@
 
 namespace mhip {
 
-constexpr int kBsLanes = 16;      // planets one level of bsw_big can hold
+constexpr int kBsLanes = 16;      // planets one level of bsw_big holds (up to 16 planets)
+constexpr int kBsWide = 64;       // the same for more planets (up to kMaxBig - 1)
 constexpr int kBsMaxLevels = 12;
 
 template <int ALG> struct BsShape;
@@ -209,7 +210,7 @@ __device__ __forceinline__ void bsw_scales(const double* x0, const double* v0, d
 }
 
 // Big-body accelerations of mfo_all (mfo_grav mercury6_2.for:4616-4690, 1PN,
-// oblateness) for one level's planets at X, V ([3][kBsLanes] each), the
+// oblateness) for one level's planets at X, V ([3][lanes] each), the
 // arithmetic of ra_planet_forces: thread (level, k) computes planet k+1; lane 0
 // of the level forms the indirect term and the oblateness reaction.  When act,
 // the level's table T for this point is written too.  Every thread of the
@@ -298,16 +299,17 @@ __device__ void bsw_planet_forces(const Sys& d, int np, const double* m, const d
     __syncthreads();
 }
 
-// One attempt over the levels LO..HI, planets: one block of (HI-LO+1) x
-// kBsLanes threads, thread (level n, planet k+1).  An attempt runs levels
+// One attempt over the levels LO..HI, planets: one block of (HI-LO+1) x NB
+// threads, thread (level n, planet k+1); NB = kBsLanes lanes per level for up
+// to 16 planets, kBsWide for more.  An attempt runs levels
 // 1..HI first; the host runs the rest (LO > 1) only when none of those levels
 // converged.  A first pass (HI < L) keeps its levels' results in rsave; the
 // second reads them back, so that its extrapolation is that of all L levels.
-template <int ALG, bool PN, bool NG, int LO, int HI>
-__global__ void __launch_bounds__((HI - LO + 1) * kBsLanes) bsw_big(BswArgs A) {
+template <int ALG, bool PN, bool NG, int LO, int HI, int NB>
+__global__ void __launch_bounds__((HI - LO + 1) * NB) bsw_big(BswArgs A) {
     constexpr int L = BsShape<ALG>::L;
     constexpr int NL = HI - LO + 1;
-    constexpr int NT = NL * kBsLanes;
+    constexpr int NT = NL * NB;
     constexpr int R = bsw_evals<ALG>(HI);
     constexpr int ELO = LO > 4 ? LO : 4;
     static_assert(LO == 1 || HI == L, "a first pass or the completing pass");
@@ -315,22 +317,22 @@ __global__ void __launch_bounds__((HI - LO + 1) * kBsLanes) bsw_big(BswArgs A) {
     const Sys d = A.sys[0];
     const int np = d.nbig - 1;
     const int t = threadIdx.x;
-    const int n = LO + t / kBsLanes;
-    const int k = t % kBsLanes;
+    const int n = LO + t / NB;
+    const int k = t % NB;
     const bool mem = k < np;
     const long long cap = A.st.cap;
-    __shared__ double m[kBsLanes + 1];
-    __shared__ double X[NL][3 * kBsLanes], V[NL][3 * kBsLanes], F[NL][3 * kBsLanes], OB[NL][3 * kBsLanes],
+    __shared__ double m[NB + 1];
+    __shared__ double X[NL][3 * NB], V[NL][3 * NB], F[NL][3 * NB], OB[NL][3 * NB],
         HD[NL][8];
-    __shared__ double D[6 * HI * kBsLanes];
-    __shared__ double E[kBsLanes][6][L - 3];
+    __shared__ double D[6 * HI * NB];
+    __shared__ double E[NB][6][L - 3];
     __shared__ double W[bsw_pairs<ALG>()][2];
     if (t <= np) m[t] = A.st.m[d.base + t];
     bsw_weights<ALG>(A.h0, W, t, NT);
     if constexpr (LO > 1) {
-        for (int w = t; w < (LO - 1) * 6 * kBsLanes; w += NT) {
-            const int row = w / (6 * kBsLanes), q = w / kBsLanes % 6, km = w % kBsLanes;
-            if (km < np) D[(6 * row + q) * kBsLanes + km] = A.rsave[(long long)(6 * row + q) * cap + d.base + 1 + km];
+        for (int w = t; w < (LO - 1) * 6 * NB; w += NT) {
+            const int row = w / (6 * NB), q = w / NB % 6, km = w % NB;
+            if (km < np) D[(6 * row + q) * NB + km] = A.rsave[(long long)(6 * row + q) * cap + d.base + 1 + km];
         }
     }
     double x0[3] = {0, 0, 0}, v0[3] = {0, 0, 0}, a0[3] = {0, 0, 0};
@@ -369,25 +371,25 @@ __global__ void __launch_bounds__((HI - LO + 1) * kBsLanes) bsw_big(BswArgs A) {
     if (mem) {
         double Rr[6];
         bsw_result<ALG>(Lv, a, v0, Rr);
-        for (int q = 0; q < 6; q++) D[(6 * (n - 1) + q) * kBsLanes + k] = Rr[q];
+        for (int q = 0; q < 6; q++) D[(6 * (n - 1) + q) * NB + k] = Rr[q];
         if constexpr (HI < L)
             for (int q = 0; q < 6; q++) A.rsave[(long long)(6 * (n - 1) + q) * cap + d.base + 1 + k] = Rr[q];
     }
     // extrapolation: item (planet, component), in passes separated by barriers
-    for (int p = 0; p < (6 * kBsLanes + NT - 1) / NT; p++) {
+    for (int p = 0; p < (6 * NB + NT - 1) / NT; p++) {
         __syncthreads();
         const int w = p * NT + t;
         const int km = w / 6, q = w % 6;
-        if (w < 6 * kBsLanes && km < np) {
+        if (w < 6 * NB && km < np) {
             double e[L - 3];
-            bsw_extrapolate<HI>(W, D + q * kBsLanes + km, kBsLanes, e, A.cand + (long long)q * cap + d.base + 1 + km,
+            bsw_extrapolate<HI>(W, D + q * NB + km, NB, e, A.cand + (long long)q * cap + d.base + 1 + km,
                                 cap, LO);
             for (int l = ELO - 4; l <= HI - 4; l++) E[km][q][l] = e[l];
         }
     }
     __syncthreads();
     constexpr int NE = HI - ELO + 1;
-    for (int w = t; w < NE * kBsLanes; w += NT) {
+    for (int w = t; w < NE * NB; w += NT) {
         const int km = w / NE, l = ELO - 4 + w % NE;
         if (km < np) {
             double xm[3], vm[3], xs, vs, d1[6];

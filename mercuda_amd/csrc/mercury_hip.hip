@@ -964,11 +964,20 @@ template <int ALG, int LO, int HI>
 void bs_pass(const BswArgs& A) {
     Context& c = ctx;
     const bool ng = c.ngflag != 0;
-    const dim3 bb((HI - LO + 1) * kBsLanes);
-    if (c.pn && ng) hipLaunchKernelGGL((bsw_big<ALG, true, true, LO, HI>), dim3(1), bb, 0, c.stream, A);
-    else if (c.pn) hipLaunchKernelGGL((bsw_big<ALG, true, false, LO, HI>), dim3(1), bb, 0, c.stream, A);
-    else if (ng) hipLaunchKernelGGL((bsw_big<ALG, false, true, LO, HI>), dim3(1), bb, 0, c.stream, A);
-    else hipLaunchKernelGGL((bsw_big<ALG, false, false, LO, HI>), dim3(1), bb, 0, c.stream, A);
+    // the planets: kBsLanes lanes per level for up to 16 planets, kBsWide beyond
+    if (c.nbig - 1 <= kBsLanes) {
+        const dim3 bb((HI - LO + 1) * kBsLanes);
+        if (c.pn && ng) hipLaunchKernelGGL((bsw_big<ALG, true, true, LO, HI, kBsLanes>), dim3(1), bb, 0, c.stream, A);
+        else if (c.pn) hipLaunchKernelGGL((bsw_big<ALG, true, false, LO, HI, kBsLanes>), dim3(1), bb, 0, c.stream, A);
+        else if (ng) hipLaunchKernelGGL((bsw_big<ALG, false, true, LO, HI, kBsLanes>), dim3(1), bb, 0, c.stream, A);
+        else hipLaunchKernelGGL((bsw_big<ALG, false, false, LO, HI, kBsLanes>), dim3(1), bb, 0, c.stream, A);
+    } else {
+        const dim3 bb((HI - LO + 1) * kBsWide);
+        if (c.pn && ng) hipLaunchKernelGGL((bsw_big<ALG, true, true, LO, HI, kBsWide>), dim3(1), bb, 0, c.stream, A);
+        else if (c.pn) hipLaunchKernelGGL((bsw_big<ALG, true, false, LO, HI, kBsWide>), dim3(1), bb, 0, c.stream, A);
+        else if (ng) hipLaunchKernelGGL((bsw_big<ALG, false, true, LO, HI, kBsWide>), dim3(1), bb, 0, c.stream, A);
+        else hipLaunchKernelGGL((bsw_big<ALG, false, false, LO, HI, kBsWide>), dim3(1), bb, 0, c.stream, A);
+    }
     const int npart = c.n - c.nbig;
     if (npart > 0) {
         dim3 grid((npart + kBswSlots - 1) / kBswSlots);
@@ -1030,7 +1039,7 @@ void bs_step(int alg, double time, double* h, double* hdid, double tol, int64_t*
     Context& c = ctx;
     const double SHRINK = .55, GROW = 1.3;
     if (c.systems.size() != 1) throw std::runtime_error("BS needs one system");
-    if (c.nbig - 1 > kBsLanes) throw std::runtime_error("more than 16 Big bodies besides the central body");
+    if (c.nbig - 1 > kBsWide) throw std::runtime_error("more than 64 Big bodies besides the central body");
     const int P = alg == 2 ? BsShape<2>::P : BsShape<3>::P;
     build_systems(1 + P, true, *h, 0.0);
     if (c.bs_cap != c.cap) {

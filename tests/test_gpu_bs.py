@@ -99,12 +99,23 @@ def test_bs_planet_counts(hip, oracle):
             check_bitwise(hip, oracle, algor, s, 10, f"BS{algor} {npl} planets")
 
 
-def test_bs_too_many_planets_fails_loudly(hip):
-    s = wide_system(17, 50, 40)
+@pytest.mark.parametrize("algor", [2, 3])
+@pytest.mark.parametrize("npl", [17, 40, 63])
+def test_bs_more_than_16_planets(hip, oracle, algor, npl):
+    """More planets than one 16-lane level holds, up to the 63 a system may
+    carry: the levels run 64 lanes wide, bitwise the oracle's states and sizes."""
+    s = wide_system(npl, 200, 40 + npl)
+    say(f"case: algorithm {algor}, {npl} planets + 200 particles, 1PN, 12 steps")
+    check_bitwise(hip, oracle, algor, s, 12, f"BS{algor} {npl} planets", pn=1, h0=2.0)
+
+
+def test_bs_too_many_big_bodies_fails_loudly(hip):
+    """64 planets besides the central body are more than the library holds:
+    the upload refuses them."""
+    s = wide_system(64, 50, 40)
     hip.configure(3)
-    hip.upload_system(s)
     with pytest.raises(Exception):
-        hip.step(0.0, 4.0, 1e-12)
+        hip.upload_system(s)
 
 
 def test_bs_events_match_mce_stat(hip, oracle):
@@ -135,3 +146,4 @@ def test_bs_events_match_mce_stat(hip, oracle):
             total += len(want)
     say(f"events compared: {total}")
     assert total > 10
+
