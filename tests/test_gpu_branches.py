@@ -11,7 +11,11 @@ configurations, against the oracle:
 - RA15 on 100,000 particles of the MVS/RADAU workload's generator (tol 1e-12,
   1PN): the step-size law takes max |b7| over every body, so the sizes must
   be the oracle's bit for bit;
-- the 512-system ensemble, sampled systems against the oracle run alone."""
+- the 512-system ensemble, sampled systems against the oracle run alone;
+- HYBRID on 50,000 particles of the HYBRID workload's generator (a quarter of
+  its 200,000: the oracle's single-threaded BS2 over the 200,000-particle
+  system's ~25,000-pair encounter set takes more than 15 minutes a step),
+  with the encounter loop's step control on the device."""
 import numpy as np
 import pytest
 
@@ -210,3 +214,15 @@ def test_ensemble_512_systems_sampled(hip, oracle):
         ex, _ = rel_err(states[k][0][1:], xo[1:])
         ev, _ = rel_err(states[k][1][1:], vo[1:])
         assert ex <= REL_TOL and ev <= REL_TOL, (k, ex, ev)
+
+
+def test_hybrid_50k_particles(hip, oracle):
+    from mercuda_amd import hybrid
+    s = systems.config_encounters(seed=3, nparticles=50_000)
+    say("case: HYBRID, 8 planets + 50000 particles (half near planet orbits), 2 steps of 2 d, device loop")
+    hybrid.mdt_hkce.substeps = 0
+    x, v, most = hybrid.hy_run(hip, s, 2)
+    say(f"encounter pairs in one step, at most {most}; BS2 substeps {hybrid.mdt_hkce.substeps}")
+    xo, vo, _, _, _ = oracle.hy_run(s, 2)
+    assert most > 1000
+    check(x, v, xo, vo, "HYBRID 50k")
