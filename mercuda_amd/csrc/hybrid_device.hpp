@@ -336,8 +336,23 @@ __global__ void hy_scatter(State st, const int* ids, int count, const double* pa
     }
 }
 
+// A per-substep BS2 attempt's bookkeeping: the fault flags and the 12 level
+// errors cleared before it, and reported after it into the host's pinned
+// words (out[0..11] the errors' bits, out[12] the fault flags).
+__global__ void bs_attempt_clear(int* fault, unsigned long long* err) {
+    const int t = threadIdx.x;
+    if (t < 12) err[t] = 0ull;
+    if (t == 12) *fault = 0;
+}
+
+__global__ void bs_attempt_report(const unsigned long long* err, const int* fault, unsigned long long* out) {
+    const int t = threadIdx.x;
+    if (t < 12) out[t] = err[t];
+    if (t == 12) out[12] = (unsigned long long)(unsigned)*fault;
+}
+
 // Encounter members' x, v ([3][ncap] each) in the driver's layout, x(3,n)
-// then v(3,n), for one contiguous copy to the host.
+// then v(3,n); out is the device address of the host's pinned staging.
 __global__ void enc_pack_aos(const double* x, const double* v, int n, int ncap, double* out) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;

@@ -169,7 +169,6 @@ struct Context {
         bool active = false;
         int n = 0, nbig = 0, ncap = 0;
         double *m = nullptr, *rcrit = nullptr, *x = nullptr, *v = nullptr, *tp = nullptr, *cand = nullptr;
-        double* aos = nullptr;         // [6][ncap]: the members' states in the driver's layout (download)
         int *bbp = nullptr, *poff = nullptr, *pidx = nullptr;
         int4* pinfo = nullptr;         // packed partners per particle (pack_partners)
         double4* pcon = nullptr;
@@ -178,9 +177,12 @@ struct Context {
         BsCtl* ctl = nullptr;          // device step control (encounter_advance)
         std::vector<double> stage;
         // pinned host words of a per-substep attempt (12 level errors, the
-        // fault flags) and the pinned staging of a state download
+        // fault flags) and the pinned staging of a state download, with the
+        // device addresses the kernels write them through
         unsigned long long* hbits = nullptr;
+        unsigned long long* dbits = nullptr;
         double* hstage = nullptr;
+        double* dstage = nullptr;
         size_t hstage_cap = 0;
         bool pending = false;          // an accepted substep's kernel may still run on the stream
     } enc;
@@ -285,6 +287,7 @@ void release_all() {
     if (c.enc.hbits) (void)hipHostFree(c.enc.hbits);
     if (c.enc.hstage) (void)hipHostFree(c.enc.hstage);
     c.enc.hbits = nullptr; c.enc.hstage = nullptr; c.enc.hstage_cap = 0;
+    c.enc.dbits = nullptr; c.enc.dstage = nullptr;
     graphs.drop();
     for (double** p : {&c.x, &c.v, &c.a, &c.m, &c.rce, &c.rphys, &c.ox, &c.ov, &c.v0k, &c.v1, &c.tab, &c.cx, &c.cv})
         dfree(*p);
@@ -300,7 +303,7 @@ void release_all() {
     dfree(c.tape.dsys);
     c.tape.valid = false;
     for (double** p : {&c.rcrit, &c.hx0, &c.hv0, &c.hytab, &c.hpack, &c.enc.m, &c.enc.rcrit, &c.enc.x, &c.enc.v,
-                       &c.enc.tp, &c.enc.cand, &c.enc.aos})
+                       &c.enc.tp, &c.enc.cand})
         dfree(*p);
     dfree(c.hpairs); dfree(c.hnpairs); dfree(c.hids); dfree(c.enc.bbp); dfree(c.enc.poff); dfree(c.enc.pidx);
     dfree(c.enc.err); dfree(c.enc.ctl); dfree(c.enc.pinfo); dfree(c.enc.pcon);
@@ -1552,7 +1555,6 @@ void encounter_enter(int n, int nbig, const double* m, const double* x, const do
         for (double** p : {&e.m, &e.rcrit}) dalloc(*p, (size_t)cap);
         for (double** p : {&e.x, &e.v}) dalloc(*p, (size_t)3 * cap);
         dalloc(e.cand, (size_t)108 * cap);   // two rotating sets of 9 levels x 6 components
-        dalloc(e.aos, (size_t)6 * cap);
         dalloc(e.poff, (size_t)cap + 1);
         dalloc(e.pinfo, (size_t)cap);
         dalloc(e.pcon, (size_t)cap);
@@ -1641,11 +1643,13 @@ void encounter_download(double* x, double* v) {
         e.hstage = nullptr;
         HIPCHECK(hipHostMalloc(reinterpret_cast<void**>(&e.hstage), sizeof(double) * std::max<size_t>(need, 6),
                                hipHostMallocDefault));
+        HIPCHECK(hipHostGetDevicePointer(reinterpret_cast<void**>(&e.dstage), e.hstage, 0));
         e.hstage_cap = need;
     }
-    hipLaunchKernelGGL(enc_pack_aos, dim3((e.n + 255) / 256), dim3(256), 0, c.stream, e.x, e.v, e.n, e.ncap, e.aos);
+    // the pack writes the driver's layout straight into the pinned staging
+    hipLaunchKernelGGL(enc_pack_aos, dim3((e.n + 255) / 256), dim3(256), 0, c.stream, e.x, e.v, e.n, e.ncap,
+                       e.dstage);
     HIPCHECK(hipGetLastError());
-    HIPCHECK(hipMemcpyAsync(e.hstage, e.aos, sizeof(double) * need, hipMemcpyDeviceToHost, c.stream));
     HIPCHECK(hipStreamSynchronize(c.stream));
     e.pending = false;
     std::memcpy(x, e.hstage, sizeof(double) * 3 * e.n);
@@ -1666,25 +1670,25 @@ void bs2_step(double time, double* h, double* hdid, double tol, int64_t* forces,
     A.tol2 = tol * tol;
     A.tp = e.tp; A.cand = e.cand; A.err = e.err; A.fault = c.fault;
     *forces = 1;           // the start accelerations, once per call
-    if (!e.hbits)
+    if (!e.hbits) {
         HIPCHECK(hipHostMalloc(reinterpret_cast<void**>(&e.hbits), 16 * sizeof(unsigned long long),
                                hipHostMallocDefault));
+        HIPCHECK(hipHostGetDevicePointer(reinterpret_cast<void**>(&e.dbits), e.hbits, 0));
+    }
     for (;;) {
         if (time + *h == time) throw std::runtime_error("BS cannot advance time at the requested tolerance");
         A.h0 = *h;
-        // one wait per attempt: the fault flags clear and the level errors and
-        // flags come back on the stream, behind the previous substep's accept
-        HIPCHECK(hipMemsetAsync(c.fault, 0, sizeof(int), c.stream));
-        hipLaunchKernelGGL(zero_bytes, dim3(1), dim3(128), 0, c.stream, reinterpret_cast<unsigned char*>(e.err),
-                           (long long)(12 * sizeof(unsigned long long)));
+        // one wait per attempt: one kernel clears the fault flags and level
+        // errors, one writes them into the pinned words, behind the previous
+        // substep's accept on the stream
+        hipLaunchKernelGGL(bs_attempt_clear, dim3(1), dim3(64), 0, c.stream, c.fault, e.err);
         hipLaunchKernelGGL(bs2_big, dim3(1), dim3(kLevels * kEncBig), 0, c.stream, A);
         if (npart > 0)
             hipLaunchKernelGGL(bs2_particles, dim3((npart + kBsPairSlots - 1) / kBsPairSlots), dim3(kBsPairThreads),
                                0, c.stream, A);
+        hipLaunchKernelGGL(bs_attempt_report, dim3(1), dim3(64), 0, c.stream, e.err, c.fault, e.dbits);
         HIPCHECK(hipGetLastError());
         unsigned long long* bits = e.hbits;
-        HIPCHECK(hipMemcpyAsync(bits, e.err, 12 * sizeof(unsigned long long), hipMemcpyDeviceToHost, c.stream));
-        HIPCHECK(hipMemcpyAsync(bits + 12, c.fault, sizeof(int), hipMemcpyDeviceToHost, c.stream));
         HIPCHECK(hipStreamSynchronize(c.stream));
         e.pending = false;
         int fault = 0;
