@@ -1202,9 +1202,15 @@ __device__ __forceinline__ void save_step(const State& st, int g, const double x
 constexpr int kEnds = 9;       // doubles per lane in the fast kernel's LDS
 
 // The box test of every planet is evaluated without branches (the four
-// comparisons of mce_stat, mercury6_2.for:1810-1811, all made) into one flag;
-// the rare lanes with a hit then repeat the tests and push one request per
-// overlapping planet in planet order.
+// comparisons of mce_stat, mercury6_2.for:1810-1811, all made) into the
+// wave's lane mask `hits`: each comparison's lane mask comes straight from its
+// v_cmp, the masks are ANDed and ORed on the scalar unit, and an empty asm
+// statement at each planet's end makes the OR happen there, so the planet's
+// box bounds die in its iteration (left to the compiler, the comparisons sink
+// to the mask's use after the loop and every planet's bounds stay live in
+// SGPRs, which then spill into VGPR lanes).  The rare lanes with a hit then
+// repeat the tests and push one request per overlapping planet in planet
+// order.
 template <bool SCREEN, int NP>
 __device__ __forceinline__ void finish_step_deferred(const Sys& d, ConstTab* T, int np, int g,
                                                      int jb, double pad, int s, int sid, const ReqSink& rs,
@@ -1222,23 +1228,30 @@ __device__ __forceinline__ void finish_step_deferred(const Sys& d, ConstTab* T, 
     const int npl = NP >= 0 ? NP : np;
     ConstTab* F = T + kHdr;
     ConstTab* Sc = F + kF * npl;
-    bool any = false;
+    unsigned long long hits = 0ull;
     double a3x = 0.0, a3y = 0.0, a3z = 0.0;
 #pragma unroll
     for (int p = 0; p < npl; p++) {
+        unsigned long long hp = 0ull;
         if (SCREEN) {
             // particles (jb >= nbig) test every planet; this kernel never sees Big bodies
+            constexpr int OGE = 3;   // ordered >=: false on a NaN, as the C comparison
             ConstTab* B = Sc + kScr * p;
-            any |= (B[SX1] >= xmin) & (xmax >= B[SX0]) & (B[SY1] >= ymin) & (ymax >= B[SY0]);
+            hp = __builtin_amdgcn_fcmp(B[SX1], xmin, OGE) & __builtin_amdgcn_fcmp(xmax, B[SX0], OGE) &
+                 __builtin_amdgcn_fcmp(B[SY1], ymin, OGE) & __builtin_amdgcn_fcmp(ymax, B[SY0], OGE);
         }
         planet_term<RangeOps>(F + kF * p, o.x, o.y, o.z, a3x, a3y, a3z);
+        if (SCREEN) {
+            hits |= hp;
+            asm volatile("" : "+s"(hits));
+        }
         // the scheduler keeps each planet's scalar loads in its own iteration
         // (hoisting every planet's 16 SGPRs of table data spills SGPRs)
         __builtin_amdgcn_sched_barrier(0);
     }
     accel_finish<RangeOps>(d, T, o.x, o.y, o.z, a3x, a3y, a3z, ax, ay, az);
     if (SCREEN) {
-        if (any) {
+        if ((hits >> __lane_id()) & 1ull) {
 #pragma unroll 1
             for (int p = 0; p < npl; p++) {
                 ConstTab* B = Sc + kScr * p;
