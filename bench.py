@@ -40,7 +40,7 @@ BYTES_PER_PARTICLE_LAUNCH = 152.0
 # a 256 MiB device copy reads on the same counters; 1M particles, the 63
 # 32-step launches of the default timed run): profiles/mvs_particles_fast_1M_hbm_traffic.md.
 # Beyond the state it includes the screening requests the kernel appends.
-TRAFFIC_PER_PARTICLE_LAUNCH = 177.5
+TRAFFIC_PER_PARTICLE_LAUNCH = 177.6
 # Algorithmic FP64 operations per particle per steady RA15 sequence (2
 # iterations, 15 force evaluations; tol 1e-12, central 1PN, 8 planets), from
 # the counted restatement of mdt_ra15 checked bit for bit against the oracle
