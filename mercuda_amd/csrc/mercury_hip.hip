@@ -806,37 +806,7 @@ void mvs_run(double time, double h, int nsteps, bool reset, bool screen, bool sa
 // mdt_ra15's constants (mercury6_2.for:4162-4199): Gauss-Radau spacings h,
 // the position/velocity finishing weights xc, vc, and the r, c, d tables
 // built by the Fortran's recurrences in its order.
-RaConst ra_constants() {
-    static const double h[8] = {0.0, .0562625605369221, .1802406917368924, .3526247171131696,
-                                .5471536263305554, .7342101772154105, .8853209468390958, .9775206135612875};
-    static const double xc[8] = {.5, .1666666666666667, .08333333333333333, .05, .03333333333333333,
-                                 .02380952380952381, .01785714285714286, .01388888888888889};
-    static const double vc[7] = {.5, .3333333333333333, .25, .2, .1666666666666667, .1428571428571429, .125};
-    RaConst k{};
-    for (int i = 0; i < 8; i++) { k.h[i] = h[i]; k.xc[i] = xc[i]; }
-    for (int i = 0; i < 7; i++) k.vc[i] = vc[i];
-    int n = 0;
-    for (int j = 1; j < 8; j++)
-        for (int q = 0; q < j; q++) k.r[n++] = 1.0 / (h[j] - h[q]);
-    // 1-based Fortran indices: c(n), d(n) stored at [n - 1]
-    k.c[0] = -h[1];
-    k.d[0] = h[1];
-    n = 1;
-    for (int j = 3; j <= 7; j++) {
-        n++;
-        k.c[n - 1] = -h[j - 1] * k.c[n - j + 1];
-        k.d[n - 1] = h[1] * k.d[n - j + 1];
-        for (int q = 3; q <= j - 1; q++) {
-            n++;
-            k.c[n - 1] = k.c[n - j] - h[j - 1] * k.c[n - j + 1];
-            k.d[n - 1] = k.d[n - j] + h[q - 1] * k.d[n - j + 1];
-        }
-        n++;
-        k.c[n - 1] = k.c[n - j] - h[j - 1];
-        k.d[n - 1] = k.d[n - j] + h[j - 1];
-    }
-    return k;
-}
+RaConst ra_constants() { return make_ra_constants(); }
 
 // (speed of light in AU/day)^2 of mfo_grav's 1PN term: C = 299792458 m/s,
 // AU = 1.495978707e11 m (mercury.inc constants, as the Fortran evaluates them)

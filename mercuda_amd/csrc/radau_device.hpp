@@ -27,6 +27,43 @@ struct RaConst {
     double h[8], xc[8], vc[7], c[21], d[21], r[28];
 };
 
+// The Gauss-Radau spacings h and the series constants xc, vc, c, d, r of
+// mdt_ra15's first call (mercury6_2.for:4160-4199), evaluated at compile time
+// with the same double operations in the same order.  The kernels read them
+// as literals: passed in the kernel arguments, the 93 doubles stayed in SGPRs
+// that spilled into VGPR lanes, and every use became a v_readlane.
+constexpr RaConst make_ra_constants() {
+    constexpr double h[8] = {0.0, .0562625605369221, .1802406917368924, .3526247171131696,
+                             .5471536263305554, .7342101772154105, .8853209468390958, .9775206135612875};
+    constexpr double xc[8] = {.5, .1666666666666667, .08333333333333333, .05, .03333333333333333,
+                              .02380952380952381, .01785714285714286, .01388888888888889};
+    constexpr double vc[7] = {.5, .3333333333333333, .25, .2, .1666666666666667, .1428571428571429, .125};
+    RaConst k{};
+    for (int i = 0; i < 8; i++) { k.h[i] = h[i]; k.xc[i] = xc[i]; }
+    for (int i = 0; i < 7; i++) k.vc[i] = vc[i];
+    int n = 0;
+    for (int j = 1; j < 8; j++)
+        for (int q = 0; q < j; q++) k.r[n++] = 1.0 / (h[j] - h[q]);
+    // 1-based Fortran indices: c(n), d(n) stored at [n - 1]
+    k.c[0] = -h[1];
+    k.d[0] = h[1];
+    n = 1;
+    for (int j = 3; j <= 7; j++) {
+        n++;
+        k.c[n - 1] = -h[j - 1] * k.c[n - j + 1];
+        k.d[n - 1] = h[1] * k.d[n - j + 1];
+        for (int q = 3; q <= j - 1; q++) {
+            n++;
+            k.c[n - 1] = k.c[n - j] - h[j - 1] * k.c[n - j + 1];
+            k.d[n - 1] = k.d[n - j] + h[q - 1] * k.d[n - j + 1];
+        }
+        n++;
+        k.c[n - 1] = k.c[n - j] - h[j - 1];
+        k.d[n - 1] = k.d[n - j] + h[j - 1];
+    }
+    return k;
+}
+
 // RA15 state, component-major: b[(3*i + comp) ... ] laid out as [21][cap]
 // with index (comp * 7 + q) * cap + body, likewise e; a1 [3][cap].
 struct RaState {
@@ -60,11 +97,12 @@ struct RaArgs {
 // The finish of one component of one body (ra_finish's arithmetic): x1, v1
 // advanced over tdid from the final b and the start acceleration a1; then,
 // unless the next sequence resets, e and b predicted for it in place.
-__device__ __forceinline__ void ra_finish_comp(const RaConst& k, double tdid, double q, bool preset, bool reset,
+__device__ __forceinline__ void ra_finish_comp(double tdid, double q, bool preset, bool reset,
                                                double* b, double* e, double a1, double& x1, double& v1) {
+    constexpr RaConst K = make_ra_constants();
     const double temp = tdid * tdid;
-    const double* xc = k.xc;
-    const double* vc = k.vc;
+    const double* xc = K.xc;
+    const double* vc = K.vc;
     x1 = (xc[7] * b[6] + xc[6] * b[5] + xc[5] * b[4] + xc[4] * b[3] + xc[3] * b[2] + xc[2] * b[1] + xc[1] * b[0]
           + xc[0] * a1) * temp + v1 * tdid + x1;
     v1 = (vc[6] * b[6] + vc[5] * b[5] + vc[4] * b[4] + vc[3] * b[3] + vc[2] * b[2] + vc[1] * b[1] + vc[0] * b[0]
@@ -138,8 +176,9 @@ __device__ __forceinline__ void grav_small(const Sys& d, TP T, int np, double li
 }
 
 // g from the b predicted at the last call (Everhart eq. 7; mercury6_2.for:4228-4240)
-__device__ __forceinline__ void ra_g_from_b(const RaConst& k, const double* b, double* g) {
-    const double* d = k.d;
+__device__ __forceinline__ void ra_g_from_b(const double* b, double* g) {
+    constexpr RaConst K = make_ra_constants();
+    const double* d = K.d;
     g[0] = b[6] * d[15] + b[5] * d[10] + b[4] * d[6] + b[3] * d[3] + b[2] * d[1] + b[1] * d[0] + b[0];
     g[1] = b[6] * d[16] + b[5] * d[11] + b[4] * d[7] + b[3] * d[4] + b[2] * d[2] + b[1];
     g[2] = b[6] * d[17] + b[5] * d[12] + b[4] * d[8] + b[3] * d[5] + b[2];
@@ -182,9 +221,10 @@ __device__ __forceinline__ double ra_vpred(const double* s, const double* b, dou
 
 // g and b update after the force at substep j (Everhart eqs. 4, 5;
 // mercury6_2.for:4295-4391).
-__device__ __forceinline__ void ra_update(const RaConst& k, int j, double a, double a1, double* g, double* b) {
-    const double* r = k.r;
-    const double* c = k.c;
+__device__ __forceinline__ void ra_update(int j, double a, double a1, double* g, double* b) {
+    constexpr RaConst K = make_ra_constants();
+    const double* r = K.r;
+    const double* c = K.c;
     double temp, gk = a - a1;
     switch (j) {
     case 2:
@@ -451,7 +491,7 @@ __global__ void __launch_bounds__(64) ra_planets(RaArgs args) {
                     b[c][q] = args.ra.b[(c * 7 + q) * cap + gb];
                     e[q] = args.ra.e[(c * 7 + q) * cap + gb];
                 }
-                ra_finish_comp(args.k, args.ptdid, args.pq, args.preset, args.reset, b[c], e,
+                ra_finish_comp(args.ptdid, args.pq, args.preset, args.reset, b[c], e,
                                args.ra.a1[c * cap + gb], x1, v1);
                 for (int q = 0; q < 7; q++) args.ra.e[(c * 7 + q) * cap + gb] = e[q];
                 args.st.x[c * cap + gb] = x1;
@@ -473,7 +513,7 @@ __global__ void __launch_bounds__(64) ra_planets(RaArgs args) {
     if (lane < np) {
         for (int c = 0; c < 3; c++) {
             A1[3 * lane + c] = A[3 * lane + c];
-            if (iter) ra_g_from_b(args.k, b[c], g[c]);
+            if (iter) ra_g_from_b(b[c], g[c]);
         }
     }
     int entry = 1;
@@ -495,7 +535,7 @@ __global__ void __launch_bounds__(64) ra_planets(RaArgs args) {
             ra_planet_forces<PN, NG>(d, np, m, X, V, A, hdr, aob, args.light2, args.tab + (long long)entry * d.ts,
                                      args.st.fault, P, args.st.ngf, cap);
             if (lane < np)
-                for (int c = 0; c < 3; c++) ra_update(args.k, j, A[3 * lane + c], A1[3 * lane + c], g[c], b[c]);
+                for (int c = 0; c < 3; c++) ra_update(j, A[3 * lane + c], A1[3 * lane + c], g[c], b[c]);
         }
     }
     double bm = 0.0;
@@ -537,7 +577,7 @@ __global__ void __launch_bounds__(128) ra_particles(RaArgs args) {
                     b[c][q] = args.ra.b[(c * 7 + q) * cap + g_];
                     e[q] = args.ra.e[(c * 7 + q) * cap + g_];
                 }
-                ra_finish_comp(args.k, args.ptdid, args.pq, args.preset, args.reset, b[c], e,
+                ra_finish_comp(args.ptdid, args.pq, args.preset, args.reset, b[c], e,
                                args.ra.a1[c * cap + g_], x1[c], v1[c]);
                 for (int q = 0; q < 7; q++) args.ra.e[(c * 7 + q) * cap + g_] = e[q];
                 args.st.x[c * cap + g_] = x1[c];
@@ -557,7 +597,7 @@ __global__ void __launch_bounds__(128) ra_particles(RaArgs args) {
             add_nongrav(d.ngflag, pb, 0.0, x1[0], x1[1], x1[2], v1[0], v1[1], v1[2], a1[0], a1[1], a1[2],
                         args.st.fault);
         if (iter)
-            for (int c = 0; c < 3; c++) ra_g_from_b(args.k, b[c], g[c]);
+            for (int c = 0; c < 3; c++) ra_g_from_b(b[c], g[c]);
         int entry = 1;
         for (int n = 1; n <= args.niter; n++) {
 #pragma unroll 1
@@ -576,7 +616,7 @@ __global__ void __launch_bounds__(128) ra_particles(RaArgs args) {
                 if (NG)
                     add_nongrav(d.ngflag, pb, 0.0, x[0], x[1], x[2], v[0], v[1], v[2], a[0], a[1], a[2],
                                 args.st.fault);
-                for (int c = 0; c < 3; c++) ra_update(args.k, j, a[c], a1[c], g[c], b[c]);
+                for (int c = 0; c < 3; c++) ra_update(j, a[c], a1[c], g[c], b[c]);
             }
         }
         for (int c = 0; c < 3; c++) {
@@ -609,8 +649,9 @@ __global__ void __launch_bounds__(128) ra_finish(RaFinishArgs args) {
     const int g = d.base + jb;
     const double tdid = args.tdid, temp = tdid * tdid;
     const double q = args.q, q2 = q * q, q3 = q * q2, q4 = q2 * q2, q5 = q2 * q3, q6 = q3 * q3, q7 = q3 * q4;
-    const double* xc = args.k.xc;
-    const double* vc = args.k.vc;
+    constexpr RaConst K = make_ra_constants();
+    const double* xc = K.xc;
+    const double* vc = K.vc;
     for (int c = 0; c < 3; c++) {
         double b[7], e[7];
         for (int i = 0; i < 7; i++) {
